@@ -1,0 +1,197 @@
+# SPDX-License-Identifier: Apache-2.0
+"""Contact solves per second of the world step on one MI355X.
+
+Workload: 10,000 mixed bodies (spheres r 0.2-0.6, capsules r 0.2-0.4 / half
+height 0.3-0.8, boxes half 0.2-0.7, 12-vertex hulls) dropped from 5-20 m
+into an open 40 x 40 x 20 m bin of static boxes; friction 0.5, dt 1/60,
+4 substeps, FP32 with f64 body centres.  A step is one world step; its
+contact solves are the constraint-graph contacts the solver took times the
+substeps (solver.c:810-843).  The timed region is whole steps: pairs,
+narrow phase, serial bookkeeping, solver, finalize.
+
+Beside it: the roofline of the dominant kernel family (the constraint
+passes, measured with HIP events on the world's stream), and the
+reference's own CPU step (oracle/_ref, its job system on every host core)
+on a bounded sample of the same scene.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from nepheline_amd import physics as P  # noqa: E402
+from nepheline_amd import scenes as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+DT = 1.0 / 60.0
+SUBSTEPS = 4
+# Algorithmic bytes per contact-substep of the three constraint passes
+# (warm start, solve, relax), FP32 structure-of-arrays:
+# B = 3 (C + P (K + 2 M)) + E (3 * 2 V + 2 X), C = 68 contact constants,
+# K = 116 point constants, M = 16 point accumulators (read and written),
+# V = 24 (v, w) read and written per pass, X = 28 (dp, dq) read by solve and
+# relax; P points, E dynamic endpoints of the contact.
+C_BYTES, K_BYTES, M_BYTES, V_BYTES, X_BYTES = 68, 116, 16, 24, 28
+
+
+def contact_bytes(points, endpoints):
+    return 3 * (C_BYTES + points * (K_BYTES + 2 * M_BYTES)) + endpoints * (
+        3 * 2 * V_BYTES + 2 * X_BYTES)
+
+
+def graph_mix(world):
+    """Mean algorithmic bytes per graph contact-substep, from the contacts the
+    graph holds now (touching ones with a colour)."""
+    tot = 0.0
+    n = 0
+    for c in world.contacts():
+        if c.color == P.NULL32 or c.point_count == 0:
+            continue
+        # Dynamic endpoints: the ground and the walls are bodies 0..4.
+        e = (1 if c.body_a >= 5 else 0) + (1 if c.body_b >= 5 else 0)
+        tot += contact_bytes(c.point_count, e)
+        n += 1
+    return (tot / n if n else 0.0), n
+
+
+def build(api, bodies, seed, bank):
+    return S.bin_drop(api, bodies, seed, bank, half=20.0, height=20.0, y_lo=5.0)
+
+
+def run_gpu(args, bank):
+    a = P.product_api()
+    a.lib.nph_amd_solved_contacts.restype = C.c_uint32
+    a.lib.nph_amd_solved_contacts.argtypes = [C.c_void_p]
+    a.lib.nph_amd_device_times.restype = None
+    a.lib.nph_amd_device_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    w = build(a, args.bodies, args.seed, bank)
+    for _ in range(args.warmup):
+        w.step(DT)
+    times = (C.c_double * 3)()
+    solved = 0
+    passes_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        w.step(DT)
+        solved += a.lib.nph_amd_solved_contacts(w.ptr)
+        a.lib.nph_amd_device_times(w.ptr, times)
+        passes_ms += times[2]
+        if args.progress and (i + 1) % 100 == 0:
+            print(f"# gpu step {i + 1}/{args.steps}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    mean_bytes, graph_n = graph_mix(w)
+    return {
+        "wall_s": wall,
+        "solved": solved,
+        "passes_ms": passes_ms,
+        "mean_bytes": mean_bytes,
+        "graph_contacts_end": graph_n,
+        "hash": "%016x" % w.hash(),
+    }
+
+
+def run_cpu(args, bank):
+    path = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
+    if not os.path.exists(path):
+        return None
+    a = P.api(path, "ref_shim_allocator")
+    a.lib.ref_shim_graph_contacts.restype = C.c_uint32
+    a.lib.ref_shim_graph_contacts.argtypes = [C.c_void_p]
+    a.lib.ref_shim_jobs_create.restype = C.c_void_p
+    a.lib.ref_shim_jobs_create.argtypes = [C.c_uint32]
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = min(cores, 16)
+    jobs = a.lib.ref_shim_jobs_create(max(cores - 1, 1)) if cores > 1 else None
+    w = build(a, args.bodies, args.seed, bank)
+    for _ in range(args.warmup):
+        w.step(DT, jobs)
+    solved = 0
+    steps = 0
+    t0 = time.perf_counter()
+    # A bounded sample: the same steps as the GPU's timed region, stopped
+    # once the CPU has worked `cpu_seconds`.
+    while steps < args.steps:
+        w.step(DT, jobs)
+        solved += a.lib.ref_shim_graph_contacts(w.ptr)
+        steps += 1
+        if time.perf_counter() - t0 > args.cpu_seconds:
+            break
+    wall = time.perf_counter() - t0
+    return {"wall_s": wall, "solved": solved, "steps": steps, "cores": cores,
+            "hash": "%016x" % w.hash()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--bodies", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--progress", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the world step has no multi-GPU path "
+              "(worlds would shard by cost across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    bank = S.load_hull_bank(os.path.join(ROOT, "tests", "golden", "hulls_12v.bin"))
+    g = run_gpu(args, bank)
+    value = g["solved"] * SUBSTEPS / g["wall_s"]
+    # Roofline of the constraint passes: their algorithmic bytes over their
+    # event-timed device time.
+    pass_bytes = g["solved"] * SUBSTEPS * g["mean_bytes"]
+    achieved = pass_bytes / (g["passes_ms"] * 1e-3) / 1e9 if g["passes_ms"] > 0 else 0.0
+    cpu = None if args.no_cpu else run_cpu(args, bank)
+    line = {
+        "metric": "contact solves/sec (contacts x substeps/s) plain vs learned warm start, "
+                  "equal residual",
+        "value": value,
+        "unit": "contact-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "bin drop: %d mixed bodies (spheres, capsules, boxes, 12-vertex "
+                               "hulls) into a 40x40x20 m bin, dt 1/60, 4 substeps, plain "
+                               "solver" % args.bodies,
+                   "bodies": args.bodies, "substeps": SUBSTEPS, "seed": args.seed,
+                   "sleep": False, "continuous": False},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "constraint passes (k_color_pass + k_overflow_pass + "
+                               "integration), event-timed",
+                     "bytes_per_contact_substep": g["mean_bytes"]},
+        "cpu_baseline": None if cpu is None else {
+            "value": cpu["solved"] * SUBSTEPS / cpu["wall_s"],
+            "unit": "contact-substeps/s",
+            "cores": cpu["cores"],
+            "kind": "reference",
+            "sample": "the reference's step (oracle/_ref, job system with %d threads) on the "
+                      "same scene, steps %d-%d of the timed window (%.1f s)" % (
+                          cpu["cores"], args.warmup + 1, args.warmup + cpu["steps"], cpu["wall_s"]),
+        },
+        "gpu_hash": g["hash"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

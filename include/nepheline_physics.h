@@ -1,0 +1,493 @@
+// SPDX-License-Identifier: Apache-2.0
+// nepheline-mi355x: the C ABI of the rigid-body world step, drop-in for the
+// reference engine's physics API (engine/physics/include/nepheline/physics/).
+//
+// Every declaration below replaces the reference function or type cited
+// next to it, with the same name, argument meaning, struct layout and error
+// behaviour, so a caller written against the reference links against
+// libnepheline_amd.so unchanged.  The arithmetic of a step runs on an
+// MI355X (gfx950) as HIP kernels; the serial bookkeeping around it (handles,
+// contact records, the constraint graph, the broadphase tree) runs on the
+// host in C17, as in the reference.
+//
+// Additions of this library, with no counterpart in the reference's public
+// headers, are marked "EXTENSION".
+#ifndef NEPHELINE_PHYSICS_H
+#define NEPHELINE_PHYSICS_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// ---------------------------------------------------------------------------
+// Base types (engine/core/include/nepheline/core/types.h; math/vec.h:21-67,
+// math/quat.h:20-24, math/mat.h:24-33, math/geometry.h)
+// ---------------------------------------------------------------------------
+#ifndef NPH_BASE_TYPES_DEFINED
+#define NPH_BASE_TYPES_DEFINED
+typedef uint8_t u8;
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int32_t i32;
+typedef float f32;
+typedef double f64;
+typedef int32_t b32;
+#endif
+
+typedef struct NphVec3 {
+    f32 x, y, z;
+} NphVec3; // 12 B, unpadded (vec.h:63)
+typedef struct NphDVec3 {
+    f64 x, y, z;
+} NphDVec3; // 24 B (vec.h:67)
+typedef struct NphQuat {
+    f32 x, y, z, w;
+} NphQuat; // 16 B (quat.h:24)
+typedef struct NphMat3 {
+    NphVec3 c[3]; // columns (mat.h:24-33)
+} NphMat3;
+typedef struct NphAABB {
+    NphVec3 min, max;
+} NphAABB; // 24 B
+typedef struct NphPlane {
+    NphVec3 normal;
+    f32 d; // n.p + d = 0
+} NphPlane;
+
+// engine/core/include/nepheline/core/result.h:15-37 (same values).
+typedef enum NphResult {
+    NPH_RESULT_OK = 0,
+    NPH_RESULT_OUT_OF_MEMORY,
+    NPH_RESULT_INVALID_ARGUMENT,
+    NPH_RESULT_NOT_FOUND,
+    NPH_RESULT_ALREADY_EXISTS,
+    NPH_RESULT_FULL,
+    NPH_RESULT_EMPTY,
+    NPH_RESULT_OVERFLOW,
+    NPH_RESULT_UNSUPPORTED,
+    NPH_RESULT_CORRUPT,
+    NPH_RESULT_VERSION_MISMATCH,
+    NPH_RESULT_TYPE_MISMATCH,
+    NPH_RESULT_PARSE_ERROR,
+    NPH_RESULT_END_OF_STREAM,
+    NPH_RESULT_IO_ERROR,
+    NPH_RESULT_ACCESS_DENIED,
+    NPH_RESULT_BUSY,
+    NPH_RESULT_DEVICE_LOST,
+    NPH_RESULT_INVALID_STATE,
+    NPH_RESULT_UNKNOWN,
+    NPH_RESULT_COUNT
+} NphResult;
+
+// engine/core/include/nepheline/core/alloc.h:55-67: the allocator vtable the
+// world allocates its host memory through.
+typedef struct NphCallsite {
+    const char *file;
+    u32 line;
+} NphCallsite;
+typedef struct NphAllocator NphAllocator;
+typedef struct NphAllocatorVtable {
+    void *(*alloc)(NphAllocator *self, u64 size, u32 align, NphCallsite site);
+    void *(*realloc)(NphAllocator *self, void *ptr, u64 old_size, u64 new_size, u32 align,
+                     NphCallsite site);
+    void (*free)(NphAllocator *self, void *ptr, u64 size, u32 align, NphCallsite site);
+} NphAllocatorVtable;
+struct NphAllocator {
+    const NphAllocatorVtable *vtable;
+    const char *name;
+};
+
+// engine/jobs/include/nepheline/jobs/jobs.h: opaque; the step accepts it
+// for signature compatibility and ignores it (the device does the parallel
+// work), as the reference's results do not depend on it (world.h:45-46).
+typedef struct NphJobSystem NphJobSystem;
+
+// EXTENSION: a malloc-backed allocator for callers without an engine core
+// (the reference's nph_allocator_system, core/alloc.h:143, lives in a module
+// this library does not replace).
+NphAllocator *nph_amd_allocator_system(void);
+
+// ---------------------------------------------------------------------------
+// Tolerances and poses (physics/types.h:38-70)
+// ---------------------------------------------------------------------------
+#define NPH_PHYSICS_LINEAR_SLOP          0.005F
+#define NPH_PHYSICS_MAX_COORDINATE       1.0e5F
+#define NPH_PHYSICS_SPECULATIVE_DISTANCE (4.0F * NPH_PHYSICS_LINEAR_SLOP)
+#define NPH_PHYSICS_AABB_MARGIN          0.1F
+#define NPH_PHYSICS_MAX_OBJECTS          (1U << 20U)
+#define NPH_PHYSICS_MANIFOLD_MAX_POINTS  4U
+
+typedef struct NphPhysicsPose {
+    NphVec3 position;
+    NphQuat rotation;
+} NphPhysicsPose; // 28 B (types.h:70)
+
+typedef struct NphPhysicsMassData {
+    f32 mass;
+    NphVec3 center;
+    NphMat3 inertia;
+} NphPhysicsMassData; // types.h
+
+// ---------------------------------------------------------------------------
+// Hull blocks (physics/hull.h:49-78): caller-owned, immutable, pointer-free;
+// the same bytes the reference's nph_physics_hull_build writes.
+// ---------------------------------------------------------------------------
+typedef struct NphPhysicsHullEdge {
+    u16 next;
+    u16 twin;
+    u16 origin;
+    u16 face;
+} NphPhysicsHullEdge; // 8 B (hull.h:56)
+
+typedef struct NphPhysicsHull {
+    u32 byte_size;
+    u32 vertex_count;
+    u32 edge_count;
+    u32 face_count;
+    u32 vertices_offset;
+    u32 planes_offset;
+    u32 faces_offset;
+    u32 edges_offset;
+    NphAABB bounds;
+    NphVec3 centroid;
+    f32 volume;
+    NphMat3 inertia;
+    f32 inner_radius;
+    f32 outer_radius;
+    f32 tolerance;
+    u32 reserved[2];
+} NphPhysicsHull; // 128 B (hull.h:78)
+
+typedef struct NphPhysicsMesh NphPhysicsMesh;               // mesh.h (not supported: rejected)
+typedef struct NphPhysicsHeightfield NphPhysicsHeightfield; // heightfield.h (not supported)
+
+// ---------------------------------------------------------------------------
+// Geometry (physics/geometry.h:37-78)
+// ---------------------------------------------------------------------------
+typedef enum NphPhysicsShapeType {
+    NPH_PHYSICS_SHAPE_SPHERE = 0,
+    NPH_PHYSICS_SHAPE_CAPSULE = 1,
+    NPH_PHYSICS_SHAPE_BOX = 2,
+    NPH_PHYSICS_SHAPE_HULL = 3,
+    NPH_PHYSICS_SHAPE_MESH = 4,
+    NPH_PHYSICS_SHAPE_HEIGHTFIELD = 5,
+    NPH_PHYSICS_SHAPE_TYPE_COUNT = 6,
+} NphPhysicsShapeType;
+
+typedef struct NphPhysicsSphere {
+    f32 radius;
+} NphPhysicsSphere;
+typedef struct NphPhysicsCapsule {
+    f32 half_height;
+    f32 radius;
+} NphPhysicsCapsule;
+typedef struct NphPhysicsBox {
+    NphVec3 half_extents;
+} NphPhysicsBox;
+
+typedef struct NphPhysicsGeometry {
+    NphPhysicsShapeType type;
+    union {
+        NphPhysicsSphere sphere;
+        NphPhysicsCapsule capsule;
+        NphPhysicsBox box;
+        const NphPhysicsHull *hull;
+        const NphPhysicsMesh *mesh;
+        const NphPhysicsHeightfield *heightfield;
+    };
+} NphPhysicsGeometry;
+
+// geometry.c:57-140 (mass closed forms), :178-229 (boxes).
+NphPhysicsMassData nph_physics_geometry_mass(const NphPhysicsGeometry *g, f32 density);
+NphAABB nph_physics_geometry_aabb(const NphPhysicsGeometry *g, NphPhysicsPose pose);
+bool nph_physics_geometry_is_valid(const NphPhysicsGeometry *g);
+
+// ---------------------------------------------------------------------------
+// World (physics/world.h:57-326)
+// ---------------------------------------------------------------------------
+typedef struct NphPhysicsWorld NphPhysicsWorld;
+typedef struct NphPhysicsBody {
+    u32 value;
+} NphPhysicsBody;
+typedef struct NphPhysicsShape {
+    u32 value;
+} NphPhysicsShape;
+
+typedef enum NphPhysicsSolver {
+    NPH_PHYSICS_SOLVER_SOFT_STEP = 0,
+    NPH_PHYSICS_SOLVER_TGS = 1,  // rejected: NPH_RESULT_UNSUPPORTED
+    NPH_PHYSICS_SOLVER_XPBD = 2, // rejected: NPH_RESULT_UNSUPPORTED
+    NPH_PHYSICS_SOLVER_COUNT = 3,
+} NphPhysicsSolver;
+
+typedef struct NphPhysicsWorldDesc {
+    NphAllocator *alloc;
+    NphVec3 gravity;
+    NphPhysicsSolver solver;
+    u32 substeps;
+    f32 contact_hertz;
+    f32 contact_damping_ratio;
+    f32 contact_push_max_velocity;
+    f32 restitution_threshold;
+    f32 max_linear_speed;
+    f32 joint_hertz;
+    f32 joint_damping_ratio;
+    f32 time_to_sleep;
+    b32 enable_sleep;
+    b32 enable_continuous;
+    u32 body_capacity;
+    u32 shape_capacity;
+    u32 contact_capacity;
+} NphPhysicsWorldDesc; // world.h:104-143
+
+typedef struct NphPhysicsContactBeginEvent {
+    NphPhysicsShape shape_a;
+    NphPhysicsShape shape_b;
+} NphPhysicsContactBeginEvent;
+typedef struct NphPhysicsContactEndEvent {
+    NphPhysicsShape shape_a;
+    NphPhysicsShape shape_b;
+} NphPhysicsContactEndEvent;
+typedef struct NphPhysicsContactHitEvent {
+    NphPhysicsShape shape_a;
+    NphPhysicsShape shape_b;
+    NphDVec3 point;
+    NphVec3 normal;
+    f32 approach_speed;
+} NphPhysicsContactHitEvent;
+typedef struct NphPhysicsBodyMoveEvent {
+    NphPhysicsBody body;
+    u32 fell_asleep;
+    u64 user_data;
+    NphDVec3 position;
+    NphQuat rotation;
+    NphVec3 linear_velocity;
+    NphVec3 angular_velocity;
+} NphPhysicsBodyMoveEvent;
+typedef struct NphPhysicsSensorEvent {
+    NphPhysicsShape sensor;
+    NphPhysicsShape visitor;
+    u64 sensor_user_data;
+    u64 visitor_user_data;
+} NphPhysicsSensorEvent;
+typedef struct NphPhysicsEvents {
+    const NphPhysicsContactBeginEvent *begin;
+    u32 begin_count;
+    const NphPhysicsContactEndEvent *end;
+    u32 end_count;
+    const NphPhysicsContactHitEvent *hit;
+    u32 hit_count;
+    const NphPhysicsBodyMoveEvent *move;
+    u32 move_count;
+    const NphPhysicsSensorEvent *sensor_begin;
+    u32 sensor_begin_count;
+    const NphPhysicsSensorEvent *sensor_end;
+    u32 sensor_end_count;
+} NphPhysicsEvents; // world.h:175-240
+
+typedef struct NphPhysicsStats {
+    u32 bodies;
+    u32 awake_bodies;
+    u32 shapes;
+    u32 contacts;
+    u32 touching_contacts;
+    u32 manifold_points;
+    u32 joints;
+    u32 islands;
+    u32 awake_islands;
+    u32 colors_used;
+    u32 overflow_constraints;
+    u32 moved_proxies;
+    u32 new_pairs;
+    u32 swept_bodies;
+    u32 impact_bodies;
+    u32 sensors;
+    u32 sensor_visitors;
+    u32 workers;
+    u64 steps;
+    f64 step_ms;
+    f64 pairs_ms;
+    f64 collide_ms;
+    f64 solve_ms;
+    f64 finalize_ms;
+    f64 sleep_ms;
+    f64 sensors_ms;
+} NphPhysicsStats; // world.h:250-280
+
+NphPhysicsWorldDesc nph_physics_world_desc_default(void);                   // world.c:85-105
+NphResult nph_physics_world_create(const NphPhysicsWorldDesc *desc,
+                                   NphPhysicsWorld **out);                  // world.c:112-215
+void nph_physics_world_destroy(NphPhysicsWorld *world);                     // world.c:217
+NphResult nph_physics_world_step(NphPhysicsWorld *world, f32 dt,
+                                 NphJobSystem *jobs);                       // world.c:307-383
+void nph_physics_world_set_gravity(NphPhysicsWorld *world, NphVec3 gravity); // world.h:170
+NphVec3 nph_physics_world_gravity(const NphPhysicsWorld *world);             // world.h:171
+NphPhysicsEvents nph_physics_world_events(const NphPhysicsWorld *world);     // world.h:240
+void nph_physics_world_set_hit_threshold(NphPhysicsWorld *world, f32 speed); // world.h:246
+NphPhysicsStats nph_physics_world_stats(const NphPhysicsWorld *world);       // world.h:280
+u64 nph_physics_world_hash(const NphPhysicsWorld *world);                    // world.c:436-462
+bool nph_physics_world_validate(const NphPhysicsWorld *world);               // world.c:482-695
+
+// ---------------------------------------------------------------------------
+// Bodies (physics/body.h:45-164)
+// ---------------------------------------------------------------------------
+typedef enum NphPhysicsBodyType {
+    NPH_PHYSICS_BODY_STATIC = 0,
+    NPH_PHYSICS_BODY_KINEMATIC = 1,
+    NPH_PHYSICS_BODY_DYNAMIC = 2,
+} NphPhysicsBodyType;
+
+enum {
+    NPH_PHYSICS_LOCK_LINEAR_X = 1U << 0U,
+    NPH_PHYSICS_LOCK_LINEAR_Y = 1U << 1U,
+    NPH_PHYSICS_LOCK_LINEAR_Z = 1U << 2U,
+    NPH_PHYSICS_LOCK_ANGULAR_X = 1U << 3U,
+    NPH_PHYSICS_LOCK_ANGULAR_Y = 1U << 4U,
+    NPH_PHYSICS_LOCK_ANGULAR_Z = 1U << 5U,
+};
+
+typedef struct NphPhysicsBodyDesc {
+    NphPhysicsBodyType type;
+    NphDVec3 position;
+    NphQuat rotation;
+    NphVec3 linear_velocity;
+    NphVec3 angular_velocity;
+    f32 linear_damping;
+    f32 angular_damping;
+    f32 gravity_scale;
+    f32 sleep_threshold;
+    u32 locks;
+    b32 enable_sleep;
+    b32 awake;
+    b32 gyroscopic;
+    b32 bullet;
+    u64 user_data;
+} NphPhysicsBodyDesc; // body.h:45-72
+
+NphPhysicsBodyDesc nph_physics_body_desc_default(void);                          // body.c:12-24
+NphResult nph_physics_body_create(NphPhysicsWorld *world, const NphPhysicsBodyDesc *desc,
+                                  NphPhysicsBody *out);                          // body.c:56-130
+bool nph_physics_body_is_valid(const NphPhysicsWorld *world, NphPhysicsBody body);
+NphPhysicsBodyType nph_physics_body_type(const NphPhysicsWorld *world, NphPhysicsBody body);
+NphDVec3 nph_physics_body_position(const NphPhysicsWorld *world, NphPhysicsBody body);
+NphQuat nph_physics_body_rotation(const NphPhysicsWorld *world, NphPhysicsBody body);
+NphDVec3 nph_physics_body_center_of_mass(const NphPhysicsWorld *world, NphPhysicsBody body);
+NphVec3 nph_physics_body_linear_velocity(const NphPhysicsWorld *world, NphPhysicsBody body);
+NphVec3 nph_physics_body_angular_velocity(const NphPhysicsWorld *world, NphPhysicsBody body);
+void nph_physics_body_set_linear_velocity(NphPhysicsWorld *world, NphPhysicsBody body,
+                                          NphVec3 velocity);
+void nph_physics_body_set_angular_velocity(NphPhysicsWorld *world, NphPhysicsBody body,
+                                           NphVec3 velocity);
+NphPhysicsMassData nph_physics_body_mass_data(const NphPhysicsWorld *world, NphPhysicsBody body);
+u64 nph_physics_body_user_data(const NphPhysicsWorld *world, NphPhysicsBody body);
+bool nph_physics_body_is_awake(const NphPhysicsWorld *world, NphPhysicsBody body);
+
+// ---------------------------------------------------------------------------
+// Shapes (physics/shape.h:36-143)
+// ---------------------------------------------------------------------------
+typedef struct NphPhysicsFilter {
+    u64 category;
+    u64 mask;
+    i32 group;
+} NphPhysicsFilter;
+
+typedef enum NphPhysicsCombine {
+    NPH_PHYSICS_COMBINE_DEFAULT = 0,
+    NPH_PHYSICS_COMBINE_AVERAGE = 1,
+    NPH_PHYSICS_COMBINE_MIN = 2,
+    NPH_PHYSICS_COMBINE_MULTIPLY = 3,
+    NPH_PHYSICS_COMBINE_MAX = 4,
+    NPH_PHYSICS_COMBINE_COUNT = 5,
+} NphPhysicsCombine;
+
+typedef struct NphPhysicsShapeDesc {
+    NphPhysicsGeometry geometry;
+    NphPhysicsPose pose;
+    f32 density;
+    f32 friction;
+    f32 restitution;
+    NphPhysicsCombine friction_combine;
+    NphPhysicsCombine restitution_combine;
+    NphPhysicsFilter filter;
+    b32 contact_events;
+    b32 hit_events;
+    b32 sensor;
+    b32 sensor_events;
+    b32 ghost;
+    u64 user_data;
+} NphPhysicsShapeDesc; // shape.h:59-81
+
+NphPhysicsFilter nph_physics_filter_default(void);                                   // shape.c:12
+bool nph_physics_filter_should_collide(NphPhysicsFilter a, NphPhysicsFilter b);      // shape.c:21-27
+NphPhysicsShapeDesc nph_physics_shape_desc_default(NphPhysicsGeometry geometry);     // shape.h:83-87
+NphResult nph_physics_shape_create(NphPhysicsWorld *world, NphPhysicsBody body,
+                                   const NphPhysicsShapeDesc *desc,
+                                   NphPhysicsShape *out);                            // shape.c:179-263
+bool nph_physics_shape_is_valid(const NphPhysicsWorld *world, NphPhysicsShape shape);
+NphPhysicsBody nph_physics_shape_body(const NphPhysicsWorld *world, NphPhysicsShape shape);
+NphAABB nph_physics_shape_aabb(const NphPhysicsWorld *world, NphPhysicsShape shape);
+
+// ---------------------------------------------------------------------------
+// EXTENSION: contact readback.  The reference exposes contacts only through
+// events, stats and debug drawing; its tests read the contact records
+// white-box (test_world.c:123-124).  This copies every live contact record
+// (the pool's order; child records of mesh pairs included) as the record
+// below, whose point layout is PhysContactPoint's (world_internal.h:196-205).
+// Returns the number of live records; writes at most `capacity`.
+// ---------------------------------------------------------------------------
+typedef struct NphPhysicsContactPointInfo {
+    NphVec3 anchor_a;
+    NphVec3 anchor_b;
+    f32 separation;
+    u32 id;
+    f32 normal_impulse;
+    f32 max_normal_impulse;
+    NphVec3 tangent_impulse;
+    f32 relative_velocity;
+} NphPhysicsContactPointInfo; // 56 B
+
+typedef struct NphPhysicsContactInfo {
+    u32 record;  // index in the contact pool
+    u32 shape_a; // slots (handle & 0xFFFFF)
+    u32 shape_b;
+    u32 body_a;
+    u32 body_b;
+    u32 flags;      // bit 0: touching
+    u32 color;      // constraint-graph colour; 24 = overflow; 0xFFFFFFFF: not in the graph
+    u32 color_slot; // position in the colour's list
+    u32 parent;
+    u32 child;
+    f32 friction;
+    f32 restitution;
+    NphVec3 normal;
+    u32 point_count;
+    NphPhysicsContactPointInfo points[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+} NphPhysicsContactInfo;
+
+u32 nph_physics_world_contacts(const NphPhysicsWorld *world, NphPhysicsContactInfo *out,
+                               u32 capacity);
+
+// EXTENSION: the name of the compute backend the world steps on ("hip:gfx950"
+// for this library).
+const char *nph_amd_backend_name(const NphPhysicsWorld *world);
+
+// EXTENSION: the contacts the last step's solver took: every colour's and the
+// overflow's (the lanes of solver.c:810-843); times substeps, the step's
+// contact solves.
+u32 nph_amd_solved_contacts(const NphPhysicsWorld *world);
+
+// EXTENSION: device time of the last step's stages, in milliseconds, from HIP
+// events on the world's stream: [0] narrow phase, [1] solver and finalize,
+// [2] the constraint passes alone (warm start, solve, relax, restitution).
+void nph_amd_device_times(const NphPhysicsWorld *world, f64 out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif // NEPHELINE_PHYSICS_H

@@ -1,0 +1,613 @@
+// SPDX-License-Identifier: Apache-2.0
+// Contacts on the host: the pool and its free list, the pair map, the
+// broadphase's new pairs, the serial update of what began and stopped
+// touching, and the constraint graph's colours.  Restated from
+// engine/physics/src/contact.c (:12-263, :591-854) and island.c (:34-135);
+// the per-contact narrow phase runs on the device (nph_dev_collide).
+#include "world_impl.h"
+
+// ---------------------------------------------------------------------------
+// The pair map
+// ---------------------------------------------------------------------------
+#define MAP_EMPTY 0ULL
+#define MAP_GONE (~0ULL)
+
+static u64 pair_key(u32 a, u32 b)
+{
+    // contact.c:12-15; +1 keeps key 0 free for empty buckets.
+    return (a < b ? ((u64)a << 32U) | b : ((u64)b << 32U) | a) + 1ULL;
+}
+
+static u64 mix64(u64 x)
+{
+    x ^= x >> 33U;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33U;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33U;
+    return x;
+}
+
+static void *halloc(NphAllocator *a, u64 size)
+{
+    NphCallsite cs = {__FILE__, (u32)__LINE__};
+    return a->vtable->alloc(a, size, 16U, cs);
+}
+
+static void hfree(NphAllocator *a, void *p, u64 size)
+{
+    NphCallsite cs = {__FILE__, (u32)__LINE__};
+    if (p != NULL) {
+        a->vtable->free(a, p, size, 16U, cs);
+    }
+}
+
+NphResult pairmap_init(NphAllocator *a, NphPairMap *m)
+{
+    memset(m, 0, sizeof *m);
+    m->cap = 1024U;
+    m->keys = (u64 *)halloc(a, (u64)m->cap * 8U);
+    m->values = (u32 *)halloc(a, (u64)m->cap * 4U);
+    if (m->keys == NULL || m->values == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    memset(m->keys, 0, (size_t)m->cap * 8U);
+    return NPH_RESULT_OK;
+}
+
+void pairmap_free(NphAllocator *a, NphPairMap *m)
+{
+    hfree(a, m->keys, (u64)m->cap * 8U);
+    hfree(a, m->values, (u64)m->cap * 4U);
+    memset(m, 0, sizeof *m);
+}
+
+static u32 map_find(const NphPairMap *m, u64 key)
+{
+    u32 i = (u32)mix64(key) & (m->cap - 1U);
+    for (;;) {
+        const u64 k = m->keys[i];
+        if (k == key) {
+            return i;
+        }
+        if (k == MAP_EMPTY) {
+            return NPH_NULL32;
+        }
+        i = (i + 1U) & (m->cap - 1U);
+    }
+}
+
+static NphResult map_insert(NphAllocator *a, NphPairMap *m, u64 key, u32 value);
+
+static NphResult map_grow(NphAllocator *a, NphPairMap *m)
+{
+    NphPairMap n;
+    memset(&n, 0, sizeof n);
+    n.cap = m->count * 4U > m->cap ? m->cap * 2U : m->cap;
+    n.keys = (u64 *)halloc(a, (u64)n.cap * 8U);
+    n.values = (u32 *)halloc(a, (u64)n.cap * 4U);
+    if (n.keys == NULL || n.values == NULL) {
+        hfree(a, n.keys, (u64)n.cap * 8U);
+        hfree(a, n.values, (u64)n.cap * 4U);
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    memset(n.keys, 0, (size_t)n.cap * 8U);
+    for (u32 i = 0U; i < m->cap; ++i) {
+        if (m->keys[i] != MAP_EMPTY && m->keys[i] != MAP_GONE) {
+            map_insert(a, &n, m->keys[i], m->values[i]);
+        }
+    }
+    pairmap_free(a, m);
+    *m = n;
+    return NPH_RESULT_OK;
+}
+
+static NphResult map_insert(NphAllocator *a, NphPairMap *m, u64 key, u32 value)
+{
+    if ((m->used + 1U) * 2U > m->cap) {
+        NphResult r = map_grow(a, m);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    u32 i = (u32)mix64(key) & (m->cap - 1U);
+    while (m->keys[i] != MAP_EMPTY && m->keys[i] != MAP_GONE) {
+        i = (i + 1U) & (m->cap - 1U);
+    }
+    if (m->keys[i] == MAP_EMPTY) {
+        m->used += 1U;
+    }
+    m->keys[i] = key;
+    m->values[i] = value;
+    m->count += 1U;
+    return NPH_RESULT_OK;
+}
+
+static void map_remove(NphPairMap *m, u64 key)
+{
+    const u32 i = map_find(m, key);
+    if (i != NPH_NULL32) {
+        m->keys[i] = MAP_GONE;
+        m->count -= 1U;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lifetime (contact.c:27-237)
+// ---------------------------------------------------------------------------
+static HContact *contact_at(NphPhysicsWorld *w, u32 i) { return &VB(w->contacts, HContact)[i]; }
+static HBody *body_at(NphPhysicsWorld *w, u32 i) { return &VB(w->bodies, HBody)[i]; }
+static HShape *shape_at(NphPhysicsWorld *w, u32 i) { return &VB(w->shapes, HShape)[i]; }
+static int body_awake(const HBody *b) { return (b->flags & HB_AWAKE) != 0U; }
+
+static void list_insert(NphPhysicsWorld *w, u32 contact, u32 side, u32 body)
+{
+    HBody *b = body_at(w, body);
+    HContact *c = contact_at(w, contact);
+    const u32 key = (contact << 1U) | side;
+    c->prev[side] = NPH_NULL32;
+    c->next[side] = b->contact_head;
+    if (b->contact_head != NPH_NULL32) {
+        contact_at(w, b->contact_head >> 1U)->prev[b->contact_head & 1U] = key;
+    }
+    b->contact_head = key;
+    b->contact_count += 1U;
+}
+
+static void list_remove(NphPhysicsWorld *w, u32 contact, u32 side, u32 body)
+{
+    HBody *b = body_at(w, body);
+    HContact *c = contact_at(w, contact);
+    const u32 prev = c->prev[side];
+    const u32 next = c->next[side];
+    if (prev != NPH_NULL32) {
+        contact_at(w, prev >> 1U)->next[prev & 1U] = next;
+    } else {
+        b->contact_head = next;
+    }
+    if (next != NPH_NULL32) {
+        contact_at(w, next >> 1U)->prev[next & 1U] = prev;
+    }
+    b->contact_count -= 1U;
+}
+
+static NphResult awake_list_add(NphPhysicsWorld *w, u32 contact)
+{
+    HContact *c = contact_at(w, contact);
+    if (c->awake_slot != NPH_NULL32) {
+        return NPH_RESULT_OK;
+    }
+    NphResult r = vbuf_push(w->alloc, &w->awake_contacts, 4U, &contact);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    c->awake_slot = w->awake_contacts.len - 1U;
+    return NPH_RESULT_OK;
+}
+
+static void awake_list_remove(NphPhysicsWorld *w, u32 contact)
+{
+    HContact *c = contact_at(w, contact);
+    const u32 slot = c->awake_slot;
+    if (slot == NPH_NULL32) {
+        return;
+    }
+    u32 *list = VB(w->awake_contacts, u32);
+    const u32 last = list[w->awake_contacts.len - 1U];
+    list[slot] = last;
+    contact_at(w, last)->awake_slot = slot;
+    w->awake_contacts.len -= 1U;
+    c->awake_slot = NPH_NULL32;
+}
+
+// ---------------------------------------------------------------------------
+// The constraint graph (island.c:34-104)
+// ---------------------------------------------------------------------------
+static int bit_test(const NphVecBuf *bits, u32 i)
+{
+    return (VB(*bits, u64)[i >> 6U] >> (i & 63U)) & 1U;
+}
+
+static void bit_set(NphVecBuf *bits, u32 i, int on)
+{
+    u64 *wds = VB(*bits, u64);
+    if (on) {
+        wds[i >> 6U] |= 1ULL << (i & 63U);
+    } else {
+        wds[i >> 6U] &= ~(1ULL << (i & 63U));
+    }
+}
+
+static u32 graph_pick_color(NphPhysicsWorld *w, u32 body_a, u32 body_b)
+{
+    const int dyn_a = body_at(w, body_a)->type == NPH_PHYSICS_BODY_DYNAMIC;
+    const int dyn_b = body_at(w, body_b)->type == NPH_PHYSICS_BODY_DYNAMIC;
+    if (dyn_a && dyn_b) {
+        for (u32 k = 0U; k < NPH_GRAPH_DYNAMIC_COLORS; ++k) {
+            if (!bit_test(&w->color_bits[k], body_a) && !bit_test(&w->color_bits[k], body_b)) {
+                return k;
+            }
+        }
+        return NPH_GRAPH_OVERFLOW;
+    }
+    const u32 body = dyn_a ? body_a : body_b;
+    for (u32 k = 1U; k < NPH_GRAPH_COLORS; ++k) {
+        if (!bit_test(&w->color_bits[k], body)) {
+            return k;
+        }
+    }
+    return NPH_GRAPH_OVERFLOW;
+}
+
+static void graph_set_bodies(NphPhysicsWorld *w, u32 color, u32 body_a, u32 body_b, int set)
+{
+    if (color == NPH_GRAPH_OVERFLOW) {
+        return;
+    }
+    const u32 bodies[2] = {body_a, body_b};
+    for (u32 i = 0U; i < 2U; ++i) {
+        if (body_at(w, bodies[i])->type == NPH_PHYSICS_BODY_DYNAMIC) {
+            bit_set(&w->color_bits[color], bodies[i], set);
+        }
+    }
+}
+
+static NphResult graph_add(NphPhysicsWorld *w, u32 contact)
+{
+    HContact *c = contact_at(w, contact);
+    const u32 color = graph_pick_color(w, c->body_a, c->body_b);
+    NphVecBuf *list = &w->color_contacts[color];
+    NphResult r = vbuf_push(w->alloc, list, 4U, &contact);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    graph_set_bodies(w, color, c->body_a, c->body_b, 1);
+    c->color = color;
+    c->color_slot = list->len - 1U;
+    return NPH_RESULT_OK;
+}
+
+static void graph_remove(NphPhysicsWorld *w, u32 contact)
+{
+    HContact *c = contact_at(w, contact);
+    const u32 color = c->color;
+    NphVecBuf *list = &w->color_contacts[color];
+    u32 *d = VB(*list, u32);
+    const u32 slot = c->color_slot;
+    const u32 last = d[list->len - 1U];
+    d[slot] = last;
+    contact_at(w, last)->color_slot = slot;
+    list->len -= 1U;
+    graph_set_bodies(w, color, c->body_a, c->body_b, 0);
+    c->color = NPH_NULL32;
+    c->color_slot = NPH_NULL32;
+}
+
+// contact.c:239-263 (no mesh children)
+static NphResult contact_sync(NphPhysicsWorld *w, u32 contact)
+{
+    const HContact *c = contact_at(w, contact);
+    const HBody *a = body_at(w, c->body_a);
+    const HBody *b = body_at(w, c->body_b);
+    const int awake = body_awake(a) || body_awake(b);
+    NphResult r = NPH_RESULT_OK;
+    if (awake) {
+        r = awake_list_add(w, contact);
+    } else {
+        awake_list_remove(w, contact);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    const int solvable = awake && (body_awake(a) || a->type != NPH_PHYSICS_BODY_DYNAMIC) &&
+                         (body_awake(b) || b->type != NPH_PHYSICS_BODY_DYNAMIC);
+    const int in_graph = solvable && (c->flags & HC_TOUCHING) != 0U;
+    if (in_graph && c->color == NPH_NULL32) {
+        return graph_add(w, contact);
+    }
+    if (!in_graph && c->color != NPH_NULL32) {
+        graph_remove(w, contact);
+    }
+    return NPH_RESULT_OK;
+}
+
+static NphResult contact_alloc(NphPhysicsWorld *w, u32 *out)
+{
+    if (w->free_contacts.len > 0U) {
+        *out = VB(w->free_contacts, u32)[w->free_contacts.len - 1U];
+        w->free_contacts.len -= 1U;
+        return NPH_RESULT_OK;
+    }
+    NphResult r = vbuf_reserve(w->alloc, &w->contacts, sizeof(HContact), w->contacts.len + 1U);
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->free_contacts, 4U, w->contacts.len + 1U);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = nph_dev_reserve(w->dev, 0U, 0U, w->contacts.len + 1U);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    w->contacts.len += 1U;
+    *out = w->contacts.len - 1U;
+    return NPH_RESULT_OK;
+}
+
+// contact.c:129-179 for a convex pair (one record).
+static NphResult contact_create(NphPhysicsWorld *w, u32 shape_a, u32 shape_b)
+{
+    u32 index = 0U;
+    NphResult r = contact_alloc(w, &index);
+    if (r == NPH_RESULT_OK) {
+        r = map_insert(w->alloc, &w->pairs, pair_key(shape_a, shape_b), index);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->awake_contacts, 4U, w->contacts.len);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    const HShape *sa = shape_at(w, shape_a);
+    const HShape *sb = shape_at(w, shape_b);
+    HContact *c = contact_at(w, index);
+    memset(c, 0, sizeof *c);
+    c->shape_a = shape_a;
+    c->shape_b = shape_b;
+    c->body_a = sa->body;
+    c->body_b = sb->body;
+    c->color = NPH_NULL32;
+    c->color_slot = NPH_NULL32;
+    c->awake_slot = NPH_NULL32;
+    if (((sa->flags | sb->flags) & HS_CONTACT_EVENTS) != 0U) {
+        c->flags |= HC_EVENTS;
+    }
+    if (((sa->flags | sb->flags) & HS_HIT_EVENTS) != 0U) {
+        c->flags |= HC_HIT_EVENTS;
+    }
+    const NphContactInit init = {index, shape_a, shape_b, c->body_a, c->body_b};
+    r = vbuf_push(w->alloc, &w->inits, sizeof init, &init);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    list_insert(w, index, 0U, c->body_a);
+    list_insert(w, index, 1U, c->body_b);
+    if (body_awake(body_at(w, c->body_a)) || body_awake(body_at(w, c->body_b))) {
+        r = awake_list_add(w, index);
+    }
+    return r;
+}
+
+// contact.c:181-237
+static void contact_destroy(NphPhysicsWorld *w, u32 contact, int events)
+{
+    HContact *c = contact_at(w, contact);
+    if ((c->flags & HC_TOUCHING) != 0U && events && (c->flags & HC_EVENTS) != 0U) {
+        NphPhysicsContactEndEvent e;
+        e.shape_a.value = shape_at(w, c->shape_a)->handle;
+        e.shape_b.value = shape_at(w, c->shape_b)->handle;
+        (void)vbuf_push(w->alloc, &w->end_events, sizeof e, &e);
+    }
+    if (c->color != NPH_NULL32) {
+        graph_remove(w, contact);
+    }
+    awake_list_remove(w, contact);
+    list_remove(w, contact, 0U, c->body_a);
+    list_remove(w, contact, 1U, c->body_b);
+    map_remove(&w->pairs, pair_key(c->shape_a, c->shape_b));
+    c->shape_a = NPH_NULL32;
+    c->shape_b = NPH_NULL32;
+    c->flags = 0U;
+    (void)vbuf_push(w->alloc, &w->free_contacts, 4U, &contact); // reserved when made
+}
+
+// contact.c:591-655 (no islands: the world runs without sleep)
+static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
+{
+    HContact *c = contact_at(w, contact);
+    if ((change & NPH_CHANGE_DISJOINT) != 0U) {
+        contact_destroy(w, contact, 1);
+        return NPH_RESULT_OK;
+    }
+    if ((change & NPH_CHANGE_STARTED) != 0U) {
+        c->flags |= HC_TOUCHING;
+        if ((c->flags & HC_EVENTS) != 0U) {
+            NphPhysicsContactBeginEvent e;
+            e.shape_a.value = shape_at(w, c->shape_a)->handle;
+            e.shape_b.value = shape_at(w, c->shape_b)->handle;
+            NphResult r = vbuf_push(w->alloc, &w->begin_events, sizeof e, &e);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+        }
+        return contact_sync(w, contact);
+    }
+    if ((change & NPH_CHANGE_STOPPED) != 0U) {
+        c->flags &= ~(u32)HC_TOUCHING;
+        if ((c->flags & HC_EVENTS) != 0U) {
+            NphPhysicsContactEndEvent e;
+            e.shape_a.value = shape_at(w, c->shape_a)->handle;
+            e.shape_b.value = shape_at(w, c->shape_b)->handle;
+            NphResult r = vbuf_push(w->alloc, &w->end_events, sizeof e, &e);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+        }
+        return contact_sync(w, contact);
+    }
+    return NPH_RESULT_OK;
+}
+
+NphResult contacts_collide(NphPhysicsWorld *w)
+{
+    NphResult r = nph_dev_init_contacts(w->dev, VB(w->inits, NphContactInit), w->inits.len);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    w->inits.len = 0U;
+    const u32 n = w->awake_contacts.len;
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    if ((r = vbuf_reserve(w->alloc, &w->changed, 4U, n)) != NPH_RESULT_OK ||
+        (r = vbuf_reserve(w->alloc, &w->codes, 4U, n)) != NPH_RESULT_OK) {
+        return r;
+    }
+    u32 m = 0U;
+    r = nph_dev_collide(w->dev, VB(w->awake_contacts, u32), n, w->step_dt, VB(w->changed, u32),
+                        VB(w->codes, u32), &m);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    // The changes in the awake list's order (contact.c:677-687).
+    for (u32 i = 0U; i < m; ++i) {
+        r = apply_change(w, VB(w->changed, u32)[i], VB(w->codes, u32)[i]);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    return NPH_RESULT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// New pairs (contact.c:691-854)
+// ---------------------------------------------------------------------------
+static int filter_should_collide(NphPhysicsFilter a, NphPhysicsFilter b)
+{
+    if (a.group == b.group && a.group != 0) {
+        return a.group > 0;
+    }
+    return (a.category & b.mask) != 0U && (b.category & a.mask) != 0U;
+}
+
+typedef struct PairQuery {
+    NphPhysicsWorld *w;
+    u32 shape;
+    u32 key;
+    int fill;
+    u32 count;
+} PairQuery;
+
+static int pair_callback(u32 proxy, u64 user_data, void *context)
+{
+    PairQuery *q = (PairQuery *)context;
+    NphPhysicsWorld *w = q->w;
+    const u32 other = (u32)user_data;
+    if (other == q->shape) {
+        return 1;
+    }
+    const HShape *s = shape_at(w, q->shape);
+    const HShape *o = shape_at(w, other);
+    const u32 other_key = (proxy << 2U) | o->tree;
+    if ((o->flags & HS_IN_MOVE_BUFFER) != 0U && other_key < q->key) {
+        return 1;
+    }
+    if (s->body == o->body) {
+        return 1;
+    }
+    if (((s->flags | o->flags) & (HS_SENSOR | HS_GHOST)) != 0U) {
+        return 1;
+    }
+    const HBody *bs = body_at(w, s->body);
+    const HBody *bo = body_at(w, o->body);
+    if (bs->type != NPH_PHYSICS_BODY_DYNAMIC && bo->type != NPH_PHYSICS_BODY_DYNAMIC) {
+        return 1;
+    }
+    if (!filter_should_collide(s->filter, o->filter)) {
+        return 1;
+    }
+    if (map_find(&w->pairs, pair_key(q->shape, other)) != NPH_NULL32) {
+        return 1;
+    }
+    const u32 pair[2] = {q->shape, other};
+    if (vbuf_push(w->alloc, &w->pair_buffer, 8U, pair) != NPH_RESULT_OK) {
+        return 0;
+    }
+    q->count += 1U;
+    return 1;
+}
+
+NphResult contacts_update_pairs(NphPhysicsWorld *w)
+{
+    for (u32 t = NPH_TREE_KINEMATIC; t <= NPH_TREE_DYNAMIC; ++t) {
+        NphResult r = tree_rebuild(&w->trees[t], 0);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    const u32 n = w->move_buffer.len;
+    w->stats.moved_proxies = n;
+    w->stats.new_pairs = 0U;
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    // Each moved proxy's pairs in the trees' order, the proxies in the move
+    // buffer's: the order the reference's count-then-fill leaves them in.
+    w->pair_buffer.len = 0U;
+    for (u32 i = 0U; i < n; ++i) {
+        const u32 key = VB(w->move_buffer, u32)[i];
+        const u32 tree = key & 3U;
+        const u32 proxy = key >> 2U;
+        PairQuery q;
+        q.w = w;
+        q.shape = (u32)tree_user_data(&w->trees[tree], proxy);
+        q.key = key;
+        q.fill = 1;
+        q.count = 0U;
+        const HShape *s = shape_at(w, q.shape);
+        const NphAABB box = s->fat_aabb;
+        const u64 mask = s->filter.group > 0 ? ~0ULL : s->filter.mask;
+        if (tree == NPH_TREE_DYNAMIC) {
+            tree_query(&w->trees[NPH_TREE_STATIC], box, mask, pair_callback, &q);
+            tree_query(&w->trees[NPH_TREE_KINEMATIC], box, mask, pair_callback, &q);
+        }
+        tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, pair_callback, &q);
+    }
+    for (u32 i = 0U; i < n; ++i) {
+        const u32 key = VB(w->move_buffer, u32)[i];
+        const u32 s = (u32)tree_user_data(&w->trees[key & 3U], key >> 2U);
+        shape_at(w, s)->flags &= ~(u32)HS_IN_MOVE_BUFFER;
+    }
+    w->move_buffer.len = 0U;
+    const u32 total = w->pair_buffer.len;
+    for (u32 i = 0U; i < total; ++i) {
+        const u32 *p = &VB(w->pair_buffer, u32)[2U * i];
+        if (map_find(&w->pairs, pair_key(p[0], p[1])) == NPH_NULL32) {
+            NphResult r = contact_create(w, p[0], p[1]);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+            w->stats.new_pairs += 1U;
+        }
+    }
+    return NPH_RESULT_OK;
+}
+
+NphResult move_buffer_push(NphPhysicsWorld *w, u32 shape)
+{
+    HShape *s = shape_at(w, shape);
+    if ((s->flags & HS_IN_MOVE_BUFFER) != 0U) {
+        return NPH_RESULT_OK;
+    }
+    const u32 key = (s->proxy << 2U) | s->tree;
+    NphResult r = vbuf_push(w->alloc, &w->move_buffer, 4U, &key);
+    if (r == NPH_RESULT_OK) {
+        s->flags |= HS_IN_MOVE_BUFFER;
+    }
+    return r;
+}
+
+// solver.c:868-885: grown proxies into the trees and the move buffer, in the
+// awake list's order (the device reports them in that order).
+NphResult contacts_post_solve(NphPhysicsWorld *w, const NphGrown *grown, u32 count)
+{
+    for (u32 i = 0U; i < count; ++i) {
+        HShape *s = shape_at(w, grown[i].shape);
+        s->fat_aabb = grown[i].fat_aabb;
+        tree_enlarge_proxy(&w->trees[s->tree], s->proxy, s->fat_aabb);
+        NphResult r = move_buffer_push(w, grown[i].shape);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    return NPH_RESULT_OK;
+}

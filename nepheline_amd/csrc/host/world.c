@@ -1,0 +1,1198 @@
+// SPDX-License-Identifier: Apache-2.0
+// The world, its bodies and shapes, and the step's order, on the host.
+//
+//   world create / step / stats / hash   world.c:85-462
+//   slots and handles                    world.c:18-49, world_internal.h:20-32
+//   bodies and their mass                body.c:12-130, :392-448
+//   shapes, world boxes, proxies         shape.c:12-263
+//   the solver's host part               solver.c:961-1017 (softness, groups' lists)
+//
+// The step runs the reference's phases in its order: new pairs (host tree),
+// the narrow phase (device), the changes it found (host), the solver and
+// finalize (device), the proxies that grew (host).  The world keeps sleep
+// and continuous collision off (the reference's throughput setting);
+// asking for either, or for the TGS or XPBD solver, is NPH_RESULT_UNSUPPORTED.
+#define _POSIX_C_SOURCE 200809L
+#include "world_impl.h"
+
+#include <stdio.h>
+#include <time.h>
+
+// ---------------------------------------------------------------------------
+// Allocation
+// ---------------------------------------------------------------------------
+static void *sys_alloc(NphAllocator *self, u64 size, u32 align, NphCallsite site)
+{
+    (void)self;
+    (void)site;
+    void *p = NULL;
+    if (posix_memalign(&p, align < 16U ? 16U : align, size == 0U ? 16U : size) != 0) {
+        return NULL;
+    }
+    return p;
+}
+
+static void *sys_realloc(NphAllocator *self, void *ptr, u64 old_size, u64 new_size, u32 align,
+                         NphCallsite site)
+{
+    void *p = sys_alloc(self, new_size, align, site);
+    if (p != NULL && ptr != NULL) {
+        memcpy(p, ptr, old_size < new_size ? old_size : new_size);
+        free(ptr);
+    }
+    return p;
+}
+
+static void sys_free(NphAllocator *self, void *ptr, u64 size, u32 align, NphCallsite site)
+{
+    (void)self;
+    (void)size;
+    (void)align;
+    (void)site;
+    free(ptr);
+}
+
+static const NphAllocatorVtable g_sys_vtable = {sys_alloc, sys_realloc, sys_free};
+static NphAllocator g_sys_alloc = {&g_sys_vtable, "nph_amd.system"};
+
+NphAllocator *nph_amd_allocator_system(void) { return &g_sys_alloc; }
+
+NphResult vbuf_reserve(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap)
+{
+    if (cap <= v->cap) {
+        return NPH_RESULT_OK;
+    }
+    u32 c = v->cap < 16U ? 16U : v->cap;
+    while (c < cap) {
+        c *= 2U;
+    }
+    NphCallsite cs = {__FILE__, (u32)__LINE__};
+    void *p = v->data == NULL ? a->vtable->alloc(a, (u64)c * elem, 16U, cs)
+                              : a->vtable->realloc(a, v->data, (u64)v->cap * elem, (u64)c * elem,
+                                                   16U, cs);
+    if (p == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    v->data = p;
+    v->cap = c;
+    return NPH_RESULT_OK;
+}
+
+NphResult vbuf_push(NphAllocator *a, NphVecBuf *v, u32 elem, const void *item)
+{
+    NphResult r = vbuf_reserve(a, v, elem, v->len + 1U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    memcpy((u8 *)v->data + (u64)v->len * elem, item, elem);
+    v->len += 1U;
+    return NPH_RESULT_OK;
+}
+
+void vbuf_free(NphAllocator *a, NphVecBuf *v, u32 elem)
+{
+    if (v->data != NULL) {
+        NphCallsite cs = {__FILE__, (u32)__LINE__};
+        a->vtable->free(a, v->data, (u64)v->cap * elem, 16U, cs);
+    }
+    memset(v, 0, sizeof *v);
+}
+
+// ---------------------------------------------------------------------------
+// Slots and handles (world.c:18-80)
+// ---------------------------------------------------------------------------
+#define SLOT_BITS 20U
+#define SLOT_MASK ((1U << SLOT_BITS) - 1U)
+#define GEN_MAX 4095U
+
+u32 phys_handle(u32 slot, u32 generation) { return (generation << SLOT_BITS) | slot; }
+u32 phys_slot(u32 handle) { return handle & SLOT_MASK; }
+
+NphResult slots_alloc(NphAllocator *a, NphSlots *s, u32 *slot, u32 *generation)
+{
+    if (s->free.len > 0U) {
+        const u32 k = VB(s->free, u32)[s->free.len - 1U];
+        s->free.len -= 1U;
+        *slot = k;
+        *generation = VB(s->generation, u32)[k];
+        return NPH_RESULT_OK;
+    }
+    if (s->generation.len >= NPH_PHYSICS_MAX_OBJECTS) {
+        return NPH_RESULT_FULL;
+    }
+    const u32 one = 1U;
+    NphResult r = vbuf_push(a, &s->generation, 4U, &one);
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(a, &s->free, 4U, s->generation.len);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    *slot = s->generation.len - 1U;
+    *generation = 1U;
+    return NPH_RESULT_OK;
+}
+
+void slots_release(NphAllocator *a, NphSlots *s, u32 slot)
+{
+    u32 *g = &VB(s->generation, u32)[slot];
+    *g = *g >= GEN_MAX ? 1U : *g + 1U;
+    (void)vbuf_push(a, &s->free, 4U, &slot);
+}
+
+HBody *world_body(const NphPhysicsWorld *w, NphPhysicsBody h)
+{
+    const u32 slot = phys_slot(h.value);
+    if (h.value == 0U || slot >= w->bodies.len || VB(w->bodies, HBody)[slot].handle != h.value) {
+        return NULL;
+    }
+    return &VB(w->bodies, HBody)[slot];
+}
+
+HShape *world_shape(const NphPhysicsWorld *w, NphPhysicsShape h)
+{
+    const u32 slot = phys_slot(h.value);
+    if (h.value == 0U || slot >= w->shapes.len || VB(w->shapes, HShape)[slot].handle != h.value) {
+        return NULL;
+    }
+    return &VB(w->shapes, HShape)[slot];
+}
+
+static NphBodyRec *rec_at(NphPhysicsWorld *w, u32 slot) { return &VB(w->body_recs, NphBodyRec)[slot]; }
+
+NphResult world_sync_bodies(NphPhysicsWorld *w)
+{
+    if (!w->bodies_stale) {
+        return NPH_RESULT_OK;
+    }
+    NphResult r = nph_dev_read_bodies(w->dev, VB(w->body_recs, NphBodyRec), w->bodies.len);
+    if (r == NPH_RESULT_OK) {
+        w->bodies_stale = 0;
+    }
+    return r;
+}
+
+NphResult world_upload_body(NphPhysicsWorld *w, u32 slot)
+{
+    return nph_dev_write_body(w->dev, slot, rec_at(w, slot));
+}
+
+NphResult world_wake_body(NphPhysicsWorld *w, u32 slot)
+{
+    HBody *b = &VB(w->bodies, HBody)[slot];
+    if (b->type == NPH_PHYSICS_BODY_STATIC || (b->flags & HB_AWAKE) != 0U) {
+        return NPH_RESULT_OK;
+    }
+    NphResult r = vbuf_push(w->alloc, &w->awake_bodies, 4U, &slot);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    b->awake_slot = w->awake_bodies.len - 1U;
+    b->flags |= HB_AWAKE;
+    rec_at(w, slot)->sleep_time = 0.0F;
+    return NPH_RESULT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// World lifecycle (world.c:85-215)
+// ---------------------------------------------------------------------------
+NphPhysicsWorldDesc nph_physics_world_desc_default(void)
+{
+    NphPhysicsWorldDesc d;
+    memset(&d, 0, sizeof d);
+    d.gravity = v3(0.0F, -9.81F, 0.0F);
+    d.substeps = 4U;
+    d.contact_hertz = 0.0F;
+    d.contact_damping_ratio = 10.0F;
+    d.joint_hertz = 0.0F;
+    d.joint_damping_ratio = 2.0F;
+    d.contact_push_max_velocity = 3.0F;
+    d.restitution_threshold = 1.0F;
+    d.max_linear_speed = 400.0F;
+    d.time_to_sleep = 0.5F;
+    d.enable_sleep = 1;
+    d.enable_continuous = 1;
+    d.body_capacity = 64U;
+    d.shape_capacity = 64U;
+    d.contact_capacity = 256U;
+    return d;
+}
+
+static int finite_nonneg(f32 v) { return nph_isfinitef(v) && v >= 0.0F; }
+
+void nph_physics_world_destroy(NphPhysicsWorld *w);
+
+NphResult nph_physics_world_create(const NphPhysicsWorldDesc *desc, NphPhysicsWorld **out)
+{
+    if (desc == NULL || out == NULL || desc->alloc == NULL || desc->substeps == 0U ||
+        (u32)desc->solver >= (u32)NPH_PHYSICS_SOLVER_COUNT || desc->substeps > 64U ||
+        !nph_isfinitef(desc->gravity.x) || !nph_isfinitef(desc->gravity.y) ||
+        !nph_isfinitef(desc->gravity.z) || !finite_nonneg(desc->contact_hertz) ||
+        !finite_nonneg(desc->contact_damping_ratio) || !finite_nonneg(desc->joint_hertz) ||
+        !finite_nonneg(desc->joint_damping_ratio) ||
+        !finite_nonneg(desc->contact_push_max_velocity) ||
+        !finite_nonneg(desc->restitution_threshold) || !(desc->max_linear_speed > 0.0F) ||
+        !nph_isfinitef(desc->max_linear_speed) || !finite_nonneg(desc->time_to_sleep)) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    if (desc->solver != NPH_PHYSICS_SOLVER_SOFT_STEP || desc->enable_sleep != 0 ||
+        desc->enable_continuous != 0) {
+        return NPH_RESULT_UNSUPPORTED;
+    }
+    NphAllocator *a = desc->alloc;
+    NphCallsite cs = {__FILE__, (u32)__LINE__};
+    NphPhysicsWorld *w = (NphPhysicsWorld *)a->vtable->alloc(a, sizeof *w, 16U, cs);
+    if (w == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    memset(w, 0, sizeof *w);
+    w->alloc = a;
+    w->desc = *desc;
+    w->hit_threshold = 1.0F;
+    NphResult r = nph_dev_create(&w->dev);
+    for (u32 t = 0U; t < 3U && r == NPH_RESULT_OK; ++t) {
+        r = tree_init(&w->trees[t], a, 16U);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = pairmap_init(a, &w->pairs);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = nph_dev_reserve(w->dev, desc->body_capacity, desc->shape_capacity,
+                            desc->contact_capacity);
+    }
+    if (r != NPH_RESULT_OK) {
+        nph_physics_world_destroy(w);
+        return r;
+    }
+    *out = w;
+    return NPH_RESULT_OK;
+}
+
+void nph_physics_world_destroy(NphPhysicsWorld *w)
+{
+    if (w == NULL) {
+        return;
+    }
+    NphAllocator *a = w->alloc;
+    nph_dev_destroy(w->dev);
+    vbuf_free(a, &w->body_slots.generation, 4U);
+    vbuf_free(a, &w->body_slots.free, 4U);
+    vbuf_free(a, &w->bodies, sizeof(HBody));
+    vbuf_free(a, &w->body_recs, sizeof(NphBodyRec));
+    vbuf_free(a, &w->shape_slots.generation, 4U);
+    vbuf_free(a, &w->shape_slots.free, 4U);
+    vbuf_free(a, &w->shapes, sizeof(HShape));
+    vbuf_free(a, &w->contacts, sizeof(HContact));
+    vbuf_free(a, &w->free_contacts, 4U);
+    vbuf_free(a, &w->awake_bodies, 4U);
+    vbuf_free(a, &w->awake_contacts, 4U);
+    vbuf_free(a, &w->inits, sizeof(NphContactInit));
+    vbuf_free(a, &w->move_buffer, 4U);
+    vbuf_free(a, &w->pair_buffer, 8U);
+    for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
+        vbuf_free(a, &w->color_bits[c], 8U);
+    }
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        vbuf_free(a, &w->color_contacts[c], 4U);
+    }
+    vbuf_free(a, &w->graph, 4U);
+    vbuf_free(a, &w->changed, 4U);
+    vbuf_free(a, &w->codes, 4U);
+    vbuf_free(a, &w->begin_events, sizeof(NphPhysicsContactBeginEvent));
+    vbuf_free(a, &w->end_events, sizeof(NphPhysicsContactEndEvent));
+    vbuf_free(a, &w->move_events, sizeof(NphPhysicsBodyMoveEvent));
+    vbuf_free(a, &w->contact_view, sizeof(NphContactRec));
+    for (u32 t = 0U; t < 3U; ++t) {
+        tree_free(&w->trees[t]);
+    }
+    pairmap_free(a, &w->pairs);
+    NphCallsite cs = {__FILE__, (u32)__LINE__};
+    a->vtable->free(a, w, sizeof *w, 16U, cs);
+}
+
+void nph_physics_world_set_gravity(NphPhysicsWorld *w, NphVec3 gravity) { w->desc.gravity = gravity; }
+NphVec3 nph_physics_world_gravity(const NphPhysicsWorld *w) { return w->desc.gravity; }
+void nph_physics_world_set_hit_threshold(NphPhysicsWorld *w, f32 speed) { w->hit_threshold = speed; }
+const char *nph_amd_backend_name(const NphPhysicsWorld *w)
+{
+    (void)w;
+    return nph_dev_name();
+}
+
+u32 nph_amd_solved_contacts(const NphPhysicsWorld *w) { return w->solved_contacts; }
+
+void nph_amd_device_times(const NphPhysicsWorld *w, f64 out[3])
+{
+    out[0] = nph_dev_last_collide_ms(w->dev);
+    out[1] = nph_dev_last_solve_ms(w->dev);
+    out[2] = nph_dev_last_passes_ms(w->dev);
+}
+
+// ---------------------------------------------------------------------------
+// Bodies (body.c:12-130, :392-448)
+// ---------------------------------------------------------------------------
+NphPhysicsBodyDesc nph_physics_body_desc_default(void)
+{
+    NphPhysicsBodyDesc d;
+    memset(&d, 0, sizeof d);
+    d.type = NPH_PHYSICS_BODY_DYNAMIC;
+    d.rotation = q_identity();
+    d.gravity_scale = 1.0F;
+    d.sleep_threshold = 0.05F;
+    d.enable_sleep = 1;
+    d.awake = 1;
+    d.gyroscopic = 1;
+    return d;
+}
+
+static int quat_is_normalized(NphQuat q, f32 tol)
+{
+    return nph_absf(q_len_sq(q) - 1.0F) <= 2.0F * tol;
+}
+
+static int body_desc_valid(const NphPhysicsBodyDesc *d)
+{
+    const NphDVec3 p = d->position;
+    const NphQuat q = d->rotation;
+    return (u32)d->type <= (u32)NPH_PHYSICS_BODY_DYNAMIC && __builtin_isfinite(p.x) &&
+           __builtin_isfinite(p.y) && __builtin_isfinite(p.z) && nph_isfinitef(q.x) &&
+           nph_isfinitef(q.y) && nph_isfinitef(q.z) && nph_isfinitef(q.w) &&
+           quat_is_normalized(q, 1e-3F) && nph_isfinitef(d->linear_velocity.x) &&
+           nph_isfinitef(d->linear_velocity.y) && nph_isfinitef(d->linear_velocity.z) &&
+           nph_isfinitef(d->angular_velocity.x) && nph_isfinitef(d->angular_velocity.y) &&
+           nph_isfinitef(d->angular_velocity.z) && finite_nonneg(d->linear_damping) &&
+           finite_nonneg(d->angular_damping) && nph_isfinitef(d->gravity_scale) &&
+           finite_nonneg(d->sleep_threshold) && (d->locks & ~0x3FU) == 0U;
+}
+
+static NphResult grow_color_bits(NphPhysicsWorld *w, u32 slots)
+{
+    const u32 words = (slots + 63U) / 64U;
+    for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
+        NphVecBuf *b = &w->color_bits[c];
+        if (b->len < words) {
+            NphResult r = vbuf_reserve(w->alloc, b, 8U, words);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+            memset(VB(*b, u64) + b->len, 0, (size_t)(words - b->len) * 8U);
+            b->len = words;
+        }
+    }
+    return NPH_RESULT_OK;
+}
+
+static HShape *shape_slot(NphPhysicsWorld *w, u32 s) { return &VB(w->shapes, HShape)[s]; }
+
+static NphGeom host_geom(const HShape *s)
+{
+    return geom_from(&s->geometry, s->geometry.type == NPH_PHYSICS_SHAPE_HULL ? s->geometry.hull : 0);
+}
+
+// body.c:392-448
+static void update_mass(NphPhysicsWorld *w, u32 slot)
+{
+    HBody *hb = &VB(w->bodies, HBody)[slot];
+    NphBodyRec *b = rec_at(w, slot);
+    NphPhysicsMassData total;
+    memset(&total, 0, sizeof total);
+    f32 extent = 0.0F;
+    for (u32 s = hb->shape_head; s != NPH_NULL32; s = shape_slot(w, s)->next) {
+        const HShape *shape = shape_slot(w, s);
+        if (shape->density <= 0.0F || (shape->flags & (HS_SENSOR | HS_GHOST)) != 0U) {
+            continue;
+        }
+        const NphGeom g = host_geom(shape);
+        total = mass_add(total, mass_transform(geom_mass(&g, shape->density), shape->pose));
+    }
+    b->mass = total.mass;
+    b->local_center = total.center;
+    b->inertia = total.inertia;
+    f32 inner = NPH_F32_MAX;
+    for (u32 s = hb->shape_head; s != NPH_NULL32; s = shape_slot(w, s)->next) {
+        const HShape *shape = shape_slot(w, s);
+        const NphGeom g = host_geom(shape);
+        const f32 d = v3_dist(shape->pose.position, b->local_center) + geom_outer_radius(&g);
+        extent = nph_maxf(extent, d);
+        inner = nph_minf(inner, geom_inner_radius(&g));
+    }
+    b->max_extent = extent;
+    b->min_extent = inner;
+    if (hb->type == NPH_PHYSICS_BODY_DYNAMIC) {
+        if (b->mass > 0.0F) {
+            b->inv_mass = 1.0F / b->mass;
+        } else {
+            b->mass = 1.0F;
+            b->inv_mass = 1.0F;
+            b->inertia = m3_diag(v3_splat(0.4F));
+            b->local_center = v3_zero();
+        }
+        if (!m3_inverse(b->inertia, &b->inv_inertia)) {
+            const f32 k = 0.4F * b->mass * nph_maxf(extent * extent, 1e-4F);
+            b->inertia = m3_diag(v3_splat(k));
+            b->inv_inertia = m3_diag(v3_splat(1.0F / k));
+        }
+    } else {
+        b->inv_mass = 0.0F;
+        b->inv_inertia = m3_zero();
+    }
+    b->center = dv3_add_v3(b->origin, q_rotate(b->rotation, b->local_center));
+}
+
+NphResult nph_physics_body_create(NphPhysicsWorld *w, const NphPhysicsBodyDesc *desc,
+                                  NphPhysicsBody *out)
+{
+    if (w == NULL || out == NULL || desc == NULL || !body_desc_valid(desc)) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    u32 slot = 0U;
+    u32 generation = 0U;
+    NphResult r = slots_alloc(w->alloc, &w->body_slots, &slot, &generation);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    if (slot >= w->bodies.len) {
+        r = vbuf_reserve(w->alloc, &w->bodies, sizeof(HBody), slot + 1U);
+        if (r == NPH_RESULT_OK) {
+            r = vbuf_reserve(w->alloc, &w->body_recs, sizeof(NphBodyRec), slot + 1U);
+        }
+        if (r == NPH_RESULT_OK) {
+            r = grow_color_bits(w, slot + 1U);
+        }
+        if (r == NPH_RESULT_OK) {
+            r = nph_dev_reserve(w->dev, slot + 1U, 0U, 0U);
+        }
+        if (r != NPH_RESULT_OK) {
+            slots_release(w->alloc, &w->body_slots, slot);
+            return r;
+        }
+        w->bodies.len = slot + 1U;
+        w->body_recs.len = slot + 1U;
+    }
+    HBody *hb = &VB(w->bodies, HBody)[slot];
+    NphBodyRec *b = rec_at(w, slot);
+    memset(hb, 0, sizeof *hb);
+    memset(b, 0, sizeof *b);
+    hb->type = (u32)desc->type;
+    b->type = (u32)desc->type;
+    b->locks = desc->locks;
+    b->origin = desc->position;
+    b->rotation = q_normalize(desc->rotation);
+    b->v = desc->type == NPH_PHYSICS_BODY_STATIC ? v3_zero() : desc->linear_velocity;
+    b->w = desc->type == NPH_PHYSICS_BODY_STATIC ? v3_zero() : desc->angular_velocity;
+    b->linear_damping = desc->linear_damping;
+    b->angular_damping = desc->angular_damping;
+    b->gravity_scale = desc->gravity_scale;
+    b->sleep_threshold = desc->sleep_threshold;
+    hb->flags = (desc->enable_sleep != 0 ? HB_ENABLE_SLEEP : 0U) |
+                (desc->gyroscopic != 0 ? HB_GYROSCOPIC : 0U) |
+                (desc->bullet != 0 && desc->type == NPH_PHYSICS_BODY_DYNAMIC ? HB_BULLET : 0U);
+    b->flags = hb->flags & (HB_ENABLE_SLEEP | HB_GYROSCOPIC | HB_BULLET);
+    hb->shape_head = NPH_NULL32;
+    b->shape_head = NPH_NULL32;
+    hb->contact_head = NPH_NULL32;
+    hb->awake_slot = NPH_NULL32;
+    hb->user_data = desc->user_data;
+    hb->handle = phys_handle(slot, generation);
+    update_mass(w, slot);
+    if (desc->type != NPH_PHYSICS_BODY_STATIC && desc->awake != 0) {
+        r = world_wake_body(w, slot);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = world_upload_body(w, slot);
+    }
+    if (r != NPH_RESULT_OK) {
+        hb->handle = 0U;
+        slots_release(w->alloc, &w->body_slots, slot);
+        return r;
+    }
+    out->value = hb->handle;
+    return NPH_RESULT_OK;
+}
+
+bool nph_physics_body_is_valid(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    return world_body(w, body) != NULL;
+}
+
+NphPhysicsBodyType nph_physics_body_type(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const HBody *b = world_body(w, body);
+    return b != NULL ? (NphPhysicsBodyType)b->type : NPH_PHYSICS_BODY_STATIC;
+}
+
+static const NphBodyRec *body_rec(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    if (world_body(w, body) == NULL) {
+        return NULL;
+    }
+    NphPhysicsWorld *mw = (NphPhysicsWorld *)w;
+    if (world_sync_bodies(mw) != NPH_RESULT_OK) {
+        return NULL;
+    }
+    return rec_at(mw, phys_slot(body.value));
+}
+
+NphDVec3 nph_physics_body_position(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const NphBodyRec *b = body_rec(w, body);
+    return b != NULL ? b->origin : dv3(0.0, 0.0, 0.0);
+}
+
+NphQuat nph_physics_body_rotation(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const NphBodyRec *b = body_rec(w, body);
+    return b != NULL ? b->rotation : q_identity();
+}
+
+NphDVec3 nph_physics_body_center_of_mass(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const NphBodyRec *b = body_rec(w, body);
+    return b != NULL ? b->center : dv3(0.0, 0.0, 0.0);
+}
+
+NphVec3 nph_physics_body_linear_velocity(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const NphBodyRec *b = body_rec(w, body);
+    return b != NULL ? b->v : v3_zero();
+}
+
+NphVec3 nph_physics_body_angular_velocity(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const NphBodyRec *b = body_rec(w, body);
+    return b != NULL ? b->w : v3_zero();
+}
+
+static void set_velocity(NphPhysicsWorld *w, NphPhysicsBody body, NphVec3 v, int angular)
+{
+    HBody *hb = world_body(w, body);
+    if (hb == NULL || hb->type == NPH_PHYSICS_BODY_STATIC || world_sync_bodies(w) != NPH_RESULT_OK) {
+        return;
+    }
+    const u32 slot = phys_slot(body.value);
+    if (angular) {
+        rec_at(w, slot)->w = v;
+    } else {
+        rec_at(w, slot)->v = v;
+    }
+    if (v3_dot(v, v) > 0.0F) {
+        (void)world_wake_body(w, slot);
+    }
+    (void)world_upload_body(w, slot);
+}
+
+void nph_physics_body_set_linear_velocity(NphPhysicsWorld *w, NphPhysicsBody body, NphVec3 v)
+{
+    set_velocity(w, body, v, 0);
+}
+
+void nph_physics_body_set_angular_velocity(NphPhysicsWorld *w, NphPhysicsBody body, NphVec3 v)
+{
+    set_velocity(w, body, v, 1);
+}
+
+NphPhysicsMassData nph_physics_body_mass_data(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    NphPhysicsMassData m;
+    memset(&m, 0, sizeof m);
+    const NphBodyRec *b = body_rec(w, body);
+    if (b != NULL) {
+        m.mass = b->mass;
+        m.center = b->local_center;
+        m.inertia = b->inertia;
+    }
+    return m;
+}
+
+u64 nph_physics_body_user_data(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const HBody *b = world_body(w, body);
+    return b != NULL ? b->user_data : 0U;
+}
+
+bool nph_physics_body_is_awake(const NphPhysicsWorld *w, NphPhysicsBody body)
+{
+    const HBody *b = world_body(w, body);
+    return b != NULL && (b->flags & HB_AWAKE) != 0U;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry (geometry.c) through the public calls
+// ---------------------------------------------------------------------------
+NphPhysicsMassData nph_physics_geometry_mass(const NphPhysicsGeometry *g, f32 density)
+{
+    const NphGeom gg = geom_from(g, g->type == NPH_PHYSICS_SHAPE_HULL ? g->hull : 0);
+    return geom_mass(&gg, density);
+}
+
+NphAABB nph_physics_geometry_aabb(const NphPhysicsGeometry *g, NphPhysicsPose pose)
+{
+    const NphGeom gg = geom_from(g, g->type == NPH_PHYSICS_SHAPE_HULL ? g->hull : 0);
+    return geom_aabb(&gg, pose);
+}
+
+static int size_ok(f32 v) { return v > 0.0F && v <= NPH_PHYSICS_MAX_COORDINATE; }
+
+bool nph_physics_geometry_is_valid(const NphPhysicsGeometry *g)
+{
+    if (g == NULL) {
+        return false;
+    }
+    switch (g->type) {
+    case NPH_PHYSICS_SHAPE_SPHERE:
+        return size_ok(g->sphere.radius);
+    case NPH_PHYSICS_SHAPE_CAPSULE:
+        return size_ok(g->capsule.radius) && g->capsule.half_height >= 0.0F &&
+               g->capsule.half_height <= NPH_PHYSICS_MAX_COORDINATE;
+    case NPH_PHYSICS_SHAPE_BOX:
+        return size_ok(g->box.half_extents.x) && size_ok(g->box.half_extents.y) &&
+               size_ok(g->box.half_extents.z);
+    case NPH_PHYSICS_SHAPE_HULL:
+        return g->hull != NULL && g->hull->vertex_count >= 4U;
+    default:
+        return false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shapes (shape.c:12-263)
+// ---------------------------------------------------------------------------
+NphPhysicsFilter nph_physics_filter_default(void)
+{
+    NphPhysicsFilter f;
+    f.category = 1U;
+    f.mask = ~0ULL;
+    f.group = 0;
+    return f;
+}
+
+bool nph_physics_filter_should_collide(NphPhysicsFilter a, NphPhysicsFilter b)
+{
+    if (a.group == b.group && a.group != 0) {
+        return a.group > 0;
+    }
+    return (a.category & b.mask) != 0U && (b.category & a.mask) != 0U;
+}
+
+NphPhysicsShapeDesc nph_physics_shape_desc_default(NphPhysicsGeometry geometry)
+{
+    NphPhysicsShapeDesc d;
+    memset(&d, 0, sizeof d);
+    d.geometry = geometry;
+    d.pose = pose_identity();
+    d.density = 1000.0F;
+    d.friction = 0.6F;
+    d.filter = nph_physics_filter_default();
+    d.contact_events = 1;
+    d.sensor_events = 1;
+    return d;
+}
+
+static NphShapeRec shape_record(NphPhysicsWorld *w, u32 s, const NphPhysicsHull *device_hull)
+{
+    const HShape *h = shape_slot(w, s);
+    NphShapeRec r;
+    memset(&r, 0, sizeof r);
+    r.geom = geom_from(&h->geometry, device_hull);
+    r.pose = h->pose;
+    r.aabb = h->aabb;
+    r.fat_aabb = h->fat_aabb;
+    r.friction = h->friction;
+    r.restitution = h->restitution;
+    r.body = h->body;
+    r.next = h->next;
+    r.combine = h->combine;
+    return r;
+}
+
+NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
+                                   const NphPhysicsShapeDesc *desc, NphPhysicsShape *out)
+{
+    if (w == NULL || out == NULL || desc == NULL || !nph_physics_geometry_is_valid(&desc->geometry) ||
+        !nph_isfinitef(desc->pose.position.x) || !nph_isfinitef(desc->pose.position.y) ||
+        !nph_isfinitef(desc->pose.position.z) || !quat_is_normalized(desc->pose.rotation, 1e-3F) ||
+        !(desc->density >= 0.0F) || !nph_isfinitef(desc->density) || !(desc->friction >= 0.0F) ||
+        !nph_isfinitef(desc->friction) || !(desc->restitution >= 0.0F) ||
+        !(desc->restitution <= 1.0F) || (u32)desc->friction_combine >= NPH_PHYSICS_COMBINE_COUNT ||
+        (u32)desc->restitution_combine >= NPH_PHYSICS_COMBINE_COUNT) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    HBody *hb = world_body(w, body);
+    if (hb == NULL) {
+        return NPH_RESULT_NOT_FOUND;
+    }
+    if (desc->sensor != 0 || desc->ghost != 0) {
+        return NPH_RESULT_UNSUPPORTED; // sensors and ghosts: not on this path
+    }
+    if (desc->geometry.type == NPH_PHYSICS_SHAPE_HULL &&
+        desc->geometry.hull->vertex_count > NPH_HULL_MAX_VERTICES) {
+        return NPH_RESULT_UNSUPPORTED;
+    }
+    NphResult r = world_sync_bodies(w);
+    const NphPhysicsHull *device_hull = NULL;
+    if (r == NPH_RESULT_OK && desc->geometry.type == NPH_PHYSICS_SHAPE_HULL) {
+        r = nph_dev_hull(w->dev, desc->geometry.hull, &device_hull);
+    }
+    const u32 body_slot = phys_slot(body.value);
+    u32 slot = 0U;
+    u32 generation = 0U;
+    if (r == NPH_RESULT_OK) {
+        r = slots_alloc(w->alloc, &w->shape_slots, &slot, &generation);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    if (slot >= w->shapes.len) {
+        r = vbuf_reserve(w->alloc, &w->shapes, sizeof(HShape), slot + 1U);
+        if (r == NPH_RESULT_OK) {
+            r = nph_dev_reserve(w->dev, 0U, slot + 1U, 0U);
+        }
+        if (r != NPH_RESULT_OK) {
+            slots_release(w->alloc, &w->shape_slots, slot);
+            return r;
+        }
+        w->shapes.len = slot + 1U;
+    }
+    hb = &VB(w->bodies, HBody)[body_slot];
+    HShape *s = shape_slot(w, slot);
+    memset(s, 0, sizeof *s);
+    s->body = body_slot;
+    s->geometry = desc->geometry;
+    s->pose = pose_make(desc->pose.position, q_normalize(desc->pose.rotation));
+    s->density = desc->density;
+    s->friction = desc->friction;
+    s->restitution = desc->restitution;
+    s->combine = (u32)desc->friction_combine | ((u32)desc->restitution_combine << 8U);
+    s->filter = desc->filter;
+    s->flags = (desc->contact_events != 0 ? HS_CONTACT_EVENTS : 0U) |
+               (desc->hit_events != 0 ? HS_HIT_EVENTS : 0U);
+    s->user_data = desc->user_data;
+    s->next = hb->shape_head;
+    hb->shape_head = slot;
+    hb->shape_count += 1U;
+    NphBodyRec *b = rec_at(w, body_slot);
+    b->shape_head = slot;
+    update_mass(w, body_slot);
+    // The proxy (shape.c:128-146): static shapes only a speculative margin.
+    s->tree = hb->type == NPH_PHYSICS_BODY_STATIC      ? NPH_TREE_STATIC
+              : hb->type == NPH_PHYSICS_BODY_KINEMATIC ? NPH_TREE_KINEMATIC
+                                                        : NPH_TREE_DYNAMIC;
+    {
+        NphShapeRec tmp = shape_record(w, slot, device_hull);
+        tmp.geom = host_geom(s); // the host's own hull address for the box
+        s->aabb = shape_world_aabb(b, &tmp);
+    }
+    const f32 margin =
+        s->tree == NPH_TREE_STATIC ? NPH_PHYSICS_SPECULATIVE_DISTANCE : NPH_PHYSICS_AABB_MARGIN;
+    s->fat_aabb = aabb_grow(s->aabb, margin);
+    r = tree_create_proxy(&w->trees[s->tree], s->fat_aabb, s->filter.category, slot, &s->proxy);
+    if (r == NPH_RESULT_OK) {
+        r = move_buffer_push(w, slot);
+    }
+    if (r == NPH_RESULT_OK) {
+        const NphShapeRec rec = shape_record(w, slot, device_hull);
+        r = nph_dev_write_shape(w->dev, slot, &rec);
+    }
+    if (r == NPH_RESULT_OK && hb->type != NPH_PHYSICS_BODY_STATIC) {
+        r = world_wake_body(w, body_slot);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = world_upload_body(w, body_slot);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    s->handle = phys_handle(slot, generation);
+    out->value = s->handle;
+    return NPH_RESULT_OK;
+}
+
+bool nph_physics_shape_is_valid(const NphPhysicsWorld *w, NphPhysicsShape shape)
+{
+    return world_shape(w, shape) != NULL;
+}
+
+NphPhysicsBody nph_physics_shape_body(const NphPhysicsWorld *w, NphPhysicsShape shape)
+{
+    NphPhysicsBody b = {0U};
+    const HShape *s = world_shape(w, shape);
+    if (s != NULL) {
+        b.value = VB(w->bodies, HBody)[s->body].handle;
+    }
+    return b;
+}
+
+NphAABB nph_physics_shape_aabb(const NphPhysicsWorld *w, NphPhysicsShape shape)
+{
+    const HShape *s = world_shape(w, shape);
+    if (s == NULL) {
+        return aabb_make(v3_zero(), v3_zero());
+    }
+    NphShapeRec r;
+    if (nph_dev_read_shapes(w->dev, NULL, 0U) == NPH_RESULT_OK) {
+        // Tight boxes of moving bodies live on the device; read this one.
+        NphShapeRec *all = (NphShapeRec *)malloc((size_t)w->shapes.len * sizeof(NphShapeRec));
+        if (all != NULL && nph_dev_read_shapes(w->dev, all, w->shapes.len) == NPH_RESULT_OK) {
+            r = all[phys_slot(shape.value)];
+            free(all);
+            return r.aabb;
+        }
+        free(all);
+    }
+    return s->aabb;
+}
+
+// ---------------------------------------------------------------------------
+// The step (world.c:307-383, solver.c:961-1105)
+// ---------------------------------------------------------------------------
+static f64 now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (f64)ts.tv_sec * 1e3 + (f64)ts.tv_nsec * 1e-6;
+}
+
+static NphResult solve(NphPhysicsWorld *w, f32 dt)
+{
+    const u32 n = w->awake_bodies.len;
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    NphSolveInput in;
+    memset(&in, 0, sizeof in);
+    in.awake_bodies = VB(w->awake_bodies, u32);
+    in.awake_count = n;
+    in.body_slots = w->bodies.len;
+    u32 graph = 0U;
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        graph += w->color_contacts[c].len;
+    }
+    NphResult r = vbuf_reserve(w->alloc, &w->graph, 4U, graph > 0U ? graph : 1U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    u32 at = 0U;
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        in.color_count[c] = w->color_contacts[c].len;
+        memcpy(VB(w->graph, u32) + at, w->color_contacts[c].data, (size_t)in.color_count[c] * 4U);
+        at += in.color_count[c];
+    }
+    in.graph_contacts = VB(w->graph, u32);
+    w->solved_contacts = graph;
+    NphStepParams *sp = &in.params;
+    sp->gravity = w->desc.gravity;
+    sp->dt = dt;
+    sp->h = dt / (f32)w->desc.substeps;
+    sp->inv_h = 1.0F / sp->h;
+    sp->push_max = w->desc.contact_push_max_velocity;
+    sp->restitution_threshold = w->desc.restitution_threshold;
+    sp->max_linear_speed = w->desc.max_linear_speed;
+    sp->substeps = w->desc.substeps;
+    const f32 contact_hertz =
+        w->desc.contact_hertz > 0.0F ? w->desc.contact_hertz : 0.125F * sp->inv_h;
+    const f32 hertz = nph_minf(contact_hertz, 0.25F * sp->inv_h);
+    make_soft(hertz, w->desc.contact_damping_ratio, sp->h, &sp->contact_bias, &sp->contact_mass,
+              &sp->contact_impulse);
+    make_soft(2.0F * hertz, w->desc.contact_damping_ratio, sp->h, &sp->static_bias,
+              &sp->static_mass, &sp->static_impulse);
+    const NphGrown *grown = NULL;
+    u32 grown_count = 0U;
+    r = nph_dev_solve(w->dev, &in, &grown, &grown_count);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    w->bodies_stale = 1;
+    return contacts_post_solve(w, grown, grown_count);
+}
+
+NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
+{
+    (void)jobs; // the device does the parallel work; results never depend on it
+    if (!(dt > 0.0F) || !nph_isfinitef(dt)) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    const f64 t0 = now_ms();
+    NphResult r = NPH_RESULT_OK;
+    w->stepping = 1;
+    w->step_dt = dt;
+    w->begin_events.len = 0U;
+    w->end_events.len = 0U;
+    w->move_events.len = 0U;
+    w->move_events_ready = 0;
+    f64 t = now_ms();
+    if (r == NPH_RESULT_OK) {
+        r = contacts_update_pairs(w);
+        w->stats.pairs_ms = now_ms() - t;
+    }
+    if (r == NPH_RESULT_OK) {
+        t = now_ms();
+        r = contacts_collide(w);
+        w->stats.collide_ms = now_ms() - t;
+    }
+    if (r == NPH_RESULT_OK) {
+        t = now_ms();
+        r = solve(w, dt);
+        w->stats.solve_ms = now_ms() - t;
+    }
+    w->stepping = 0;
+    w->stats.steps += 1U;
+    w->stats.step_ms = now_ms() - t0;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Readback: events, stats, hash, contacts
+// ---------------------------------------------------------------------------
+NphPhysicsEvents nph_physics_world_events(const NphPhysicsWorld *cw)
+{
+    NphPhysicsWorld *w = (NphPhysicsWorld *)cw;
+    NphPhysicsEvents e;
+    memset(&e, 0, sizeof e);
+    if (!w->move_events_ready && w->stats.steps > 0U && world_sync_bodies(w) == NPH_RESULT_OK &&
+        vbuf_reserve(w->alloc, &w->move_events, sizeof(NphPhysicsBodyMoveEvent),
+                     w->awake_bodies.len) == NPH_RESULT_OK) {
+        // solver.c:886-896: one per awake body, in the awake list's order.
+        for (u32 i = 0U; i < w->awake_bodies.len; ++i) {
+            const u32 slot = VB(w->awake_bodies, u32)[i];
+            const HBody *hb = &VB(w->bodies, HBody)[slot];
+            const NphBodyRec *b = rec_at(w, slot);
+            NphPhysicsBodyMoveEvent *m = &VB(w->move_events, NphPhysicsBodyMoveEvent)[i];
+            m->body.value = hb->handle;
+            m->fell_asleep = 0U;
+            m->user_data = hb->user_data;
+            m->position = b->origin;
+            m->rotation = b->rotation;
+            m->linear_velocity = b->v;
+            m->angular_velocity = b->w;
+        }
+        w->move_events.len = w->awake_bodies.len;
+        w->move_events_ready = 1;
+    }
+    e.begin = VB(w->begin_events, NphPhysicsContactBeginEvent);
+    e.begin_count = w->begin_events.len;
+    e.end = VB(w->end_events, NphPhysicsContactEndEvent);
+    e.end_count = w->end_events.len;
+    e.move = VB(w->move_events, NphPhysicsBodyMoveEvent);
+    e.move_count = w->move_events.len;
+    return e;
+}
+
+static NphResult read_contacts(NphPhysicsWorld *w)
+{
+    NphResult r = vbuf_reserve(w->alloc, &w->contact_view, sizeof(NphContactRec),
+                               w->contacts.len > 0U ? w->contacts.len : 1U);
+    if (r == NPH_RESULT_OK) {
+        r = nph_dev_read_contacts(w->dev, VB(w->contact_view, NphContactRec), w->contacts.len);
+    }
+    return r;
+}
+
+NphPhysicsStats nph_physics_world_stats(const NphPhysicsWorld *cw)
+{
+    NphPhysicsWorld *w = (NphPhysicsWorld *)cw;
+    NphPhysicsStats s = w->stats;
+    s.bodies = w->body_slots.generation.len - w->body_slots.free.len;
+    s.awake_bodies = w->awake_bodies.len;
+    s.shapes = w->shape_slots.generation.len - w->shape_slots.free.len;
+    s.contacts = w->contacts.len - w->free_contacts.len;
+    u32 touching = 0U;
+    u32 points = 0U;
+    if (read_contacts(w) == NPH_RESULT_OK) {
+        for (u32 i = 0U; i < w->awake_contacts.len; ++i) {
+            const u32 k = VB(w->awake_contacts, u32)[i];
+            if ((VB(w->contacts, HContact)[k].flags & HC_TOUCHING) != 0U) {
+                touching += 1U;
+                points += VB(w->contact_view, NphContactRec)[k].point_count;
+            }
+        }
+    }
+    s.touching_contacts = touching;
+    s.manifold_points = points;
+    u32 colors = 0U;
+    for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
+        colors += w->color_contacts[c].len > 0U ? 1U : 0U;
+    }
+    s.colors_used = colors;
+    s.overflow_constraints = w->color_contacts[NPH_GRAPH_OVERFLOW].len;
+    s.workers = 1U;
+    return s;
+}
+
+// nph_hash64 (engine/core/src/hash.c:73-120, the wyhash construction): the
+// state fingerprint's hash, restated for the records below.
+static void mul128(u64 *a, u64 *b)
+{
+    const unsigned __int128 r = (unsigned __int128)*a * (unsigned __int128)*b;
+    *a = (u64)r;
+    *b = (u64)(r >> 64U);
+}
+
+static u64 hmix(u64 a, u64 b)
+{
+    mul128(&a, &b);
+    return a ^ b;
+}
+
+static u64 rd8(const u8 *p)
+{
+    u64 v;
+    memcpy(&v, p, 8U);
+    return v;
+}
+
+static u64 rd4(const u8 *p)
+{
+    u32 v;
+    memcpy(&v, p, 4U);
+    return v;
+}
+
+static u64 nph_hash64(const void *data, u64 len, u64 seed)
+{
+    const u64 s0 = 0x9E3779B97F4A7C15ULL;
+    const u64 s1 = 0x6A09E667F3BCC909ULL;
+    const u64 s2 = 0xBB67AE8584CAA73BULL;
+    const u64 s3 = 0x3C6EF372FE94F82BULL;
+    const u8 *p = (const u8 *)data;
+    seed ^= hmix(seed ^ s0, s1);
+    u64 a;
+    u64 b;
+    if (len <= 16U) {
+        if (len >= 4U) {
+            a = (rd4(p) << 32U) | rd4(p + ((len >> 3U) << 2U));
+            b = (rd4(p + len - 4U) << 32U) | rd4(p + len - 4U - ((len >> 3U) << 2U));
+        } else if (len > 0U) {
+            a = ((u64)p[0] << 16U) | ((u64)p[len >> 1U] << 8U) | (u64)p[len - 1U];
+            b = 0U;
+        } else {
+            a = 0U;
+            b = 0U;
+        }
+    } else {
+        u64 i = len;
+        if (i > 48U) {
+            u64 see1 = seed;
+            u64 see2 = seed;
+            do {
+                seed = hmix(rd8(p) ^ s1, rd8(p + 8) ^ seed);
+                see1 = hmix(rd8(p + 16) ^ s2, rd8(p + 24) ^ see1);
+                see2 = hmix(rd8(p + 32) ^ s3, rd8(p + 40) ^ see2);
+                p += 48;
+                i -= 48U;
+            } while (i > 48U);
+            seed ^= see1 ^ see2;
+        }
+        while (i > 16U) {
+            seed = hmix(rd8(p) ^ s1, rd8(p + 8) ^ seed);
+            i -= 16U;
+            p += 16;
+        }
+        a = rd8(p + i - 16U);
+        b = rd8(p + i - 8U);
+    }
+    a ^= s1;
+    b ^= seed;
+    mul128(&a, &b);
+    return hmix(a ^ s0 ^ len, b ^ s1);
+}
+
+// world.c:436-462
+u64 nph_physics_world_hash(const NphPhysicsWorld *cw)
+{
+    NphPhysicsWorld *w = (NphPhysicsWorld *)cw;
+    if (world_sync_bodies(w) != NPH_RESULT_OK) {
+        return 0U;
+    }
+    u64 h = 0x6E70687973696373ULL;
+    for (u32 i = 0U; i < w->bodies.len; ++i) {
+        const HBody *hb = &VB(w->bodies, HBody)[i];
+        if (hb->handle == 0U) {
+            continue;
+        }
+        const NphBodyRec *b = rec_at(w, i);
+        struct {
+            u32 handle;
+            u32 flags;
+            NphDVec3 origin;
+            NphQuat rotation;
+            NphVec3 v;
+            NphVec3 w;
+        } rec;
+        memset(&rec, 0, sizeof rec);
+        rec.handle = hb->handle;
+        rec.flags = hb->flags & HB_AWAKE;
+        rec.origin = b->origin;
+        rec.rotation = b->rotation;
+        rec.v = b->v;
+        rec.w = b->w;
+        h = nph_hash64(&rec, sizeof rec, h);
+    }
+    return h;
+}
+
+bool nph_physics_world_validate(const NphPhysicsWorld *w)
+{
+    // The bookkeeping's invariants (world.c:482-695, the parts kept here):
+    // awake slots point back, colour slots point back, pairs are in the map.
+    for (u32 i = 0U; i < w->awake_contacts.len; ++i) {
+        const u32 k = VB(w->awake_contacts, u32)[i];
+        if (k >= w->contacts.len || VB(w->contacts, HContact)[k].awake_slot != i) {
+            return false;
+        }
+    }
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        for (u32 i = 0U; i < w->color_contacts[c].len; ++i) {
+            const u32 k = VB(w->color_contacts[c], u32)[i];
+            const HContact *hc = &VB(w->contacts, HContact)[k];
+            if (hc->color != c || hc->color_slot != i || (hc->flags & HC_TOUCHING) == 0U) {
+                return false;
+            }
+        }
+    }
+    for (u32 i = 0U; i < w->awake_bodies.len; ++i) {
+        const u32 s = VB(w->awake_bodies, u32)[i];
+        if (VB(w->bodies, HBody)[s].awake_slot != i) {
+            return false;
+        }
+    }
+    return true;
+}
+
+u32 nph_physics_world_contacts(const NphPhysicsWorld *cw, NphPhysicsContactInfo *out, u32 capacity)
+{
+    NphPhysicsWorld *w = (NphPhysicsWorld *)cw;
+    if (read_contacts(w) != NPH_RESULT_OK) {
+        return 0U;
+    }
+    u32 n = 0U;
+    for (u32 i = 0U; i < w->contacts.len; ++i) {
+        const HContact *hc = &VB(w->contacts, HContact)[i];
+        if (hc->shape_a == NPH_NULL32) {
+            continue;
+        }
+        if (n < capacity && out != NULL) {
+            const NphContactRec *c = &VB(w->contact_view, NphContactRec)[i];
+            NphPhysicsContactInfo *o = &out[n];
+            memset(o, 0, sizeof *o);
+            o->record = i;
+            o->shape_a = hc->shape_a;
+            o->shape_b = hc->shape_b;
+            o->body_a = hc->body_a;
+            o->body_b = hc->body_b;
+            o->flags = hc->flags;
+            o->color = hc->color;
+            o->color_slot = hc->color_slot;
+            o->parent = NPH_NULL32;
+            o->child = NPH_NULL32;
+            o->friction = c->friction;
+            o->restitution = c->restitution;
+            o->normal = c->normal;
+            o->point_count = c->point_count;
+            for (u32 k = 0U; k < c->point_count; ++k) {
+                memcpy(&o->points[k], &c->points[k], sizeof(NphPhysicsContactPointInfo));
+            }
+        }
+        n += 1U;
+    }
+    return n;
+}

@@ -1,0 +1,216 @@
+// SPDX-License-Identifier: Apache-2.0
+// The host side of the world: the serial bookkeeping the reference keeps on
+// the CPU (engine/physics/src/world_internal.h), restated around a device
+// that holds every numeric record.
+#ifndef NPH_WORLD_IMPL_H
+#define NPH_WORLD_IMPL_H
+
+#include "../nph_device.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+// ---------------------------------------------------------------------------
+// Growable arrays over the world's allocator
+// ---------------------------------------------------------------------------
+typedef struct NphVecBuf {
+    void *data;
+    u32 len;
+    u32 cap;
+} NphVecBuf;
+
+NphResult vbuf_reserve(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap);
+NphResult vbuf_push(NphAllocator *a, NphVecBuf *v, u32 elem, const void *item);
+void vbuf_free(NphAllocator *a, NphVecBuf *v, u32 elem);
+#define VB(v, T) ((T *)(v).data)
+
+// ---------------------------------------------------------------------------
+// The dynamic AABB tree (aabb_tree.c): proxy ids, insertion, growth, the
+// partial SAH rebuild and the query order fix which pairs a step creates
+// and in what order.
+// ---------------------------------------------------------------------------
+#define NPH_TREE_NULL NPH_NULL32
+typedef struct NphTreeNode {
+    NphAABB aabb;
+    u64 category_bits;
+    u32 child1; // a leaf's user data, low word
+    u32 child2; // a leaf's user data, high word
+    u32 parent; // the next free node in the free list
+    u16 height;
+    u16 flags;
+} NphTreeNode; // 48 B (aabb_tree.h:52-67)
+
+typedef struct NphTree {
+    NphAllocator *alloc;
+    NphTreeNode *nodes;
+    u32 root;
+    u32 node_count;
+    u32 node_capacity;
+    u32 free_list;
+    u32 proxy_count;
+    u32 rebuild_capacity;
+    u32 *leaf_indices;
+    NphAABB *leaf_boxes;
+    NphVec3 *leaf_centers;
+    u32 *bins_scratch;
+} NphTree;
+
+typedef int (*NphTreeQueryFn)(u32 proxy, u64 user_data, void *context);
+
+NphResult tree_init(NphTree *t, NphAllocator *alloc, u32 initial_capacity);
+void tree_free(NphTree *t);
+NphResult tree_create_proxy(NphTree *t, NphAABB aabb, u64 category, u64 user_data, u32 *out);
+void tree_enlarge_proxy(NphTree *t, u32 proxy, NphAABB aabb);
+void tree_query(const NphTree *t, NphAABB aabb, u64 mask, NphTreeQueryFn fn, void *ctx);
+NphResult tree_rebuild(NphTree *t, int full);
+u64 tree_user_data(const NphTree *t, u32 proxy);
+
+// ---------------------------------------------------------------------------
+// Records
+// ---------------------------------------------------------------------------
+enum {
+    HB_AWAKE = 1U << 0U,
+    HB_ENABLE_SLEEP = 1U << 1U,
+    HB_GYROSCOPIC = 1U << 2U,
+    HB_BULLET = 1U << 5U,
+};
+
+typedef struct HBody {
+    u32 handle; // 0: free slot
+    u32 type;
+    u32 flags;
+    u32 shape_head;
+    u32 shape_count;
+    u32 contact_head; // contact << 1 | side
+    u32 contact_count;
+    u32 awake_slot;
+    u64 user_data;
+} HBody;
+
+enum {
+    HS_CONTACT_EVENTS = 1U << 0U,
+    HS_HIT_EVENTS = 1U << 1U,
+    HS_IN_MOVE_BUFFER = 1U << 3U,
+    HS_SENSOR = 1U << 4U,
+    HS_GHOST = 1U << 5U,
+};
+
+typedef struct HShape {
+    u32 handle;
+    u32 body;
+    u32 next;
+    u32 flags;
+    NphPhysicsGeometry geometry; // as given (host hull address)
+    NphPhysicsPose pose;
+    f32 density;
+    f32 friction;
+    f32 restitution;
+    u32 combine;
+    NphPhysicsFilter filter;
+    u32 tree;
+    u32 proxy;
+    NphAABB aabb;
+    NphAABB fat_aabb;
+    u64 user_data;
+} HShape;
+
+enum {
+    HC_TOUCHING = 1U << 0U,
+    HC_EVENTS = 1U << 1U,
+    HC_HIT_EVENTS = 1U << 2U,
+};
+
+typedef struct HContact {
+    u32 shape_a; // NPH_NULL32: free record
+    u32 shape_b;
+    u32 body_a;
+    u32 body_b;
+    u32 flags;
+    u32 prev[2];
+    u32 next[2];
+    u32 color;
+    u32 color_slot;
+    u32 awake_slot;
+} HContact;
+
+typedef struct NphSlots {
+    NphVecBuf generation; // u32
+    NphVecBuf free;       // u32
+} NphSlots;
+
+// (shape a << 32 | shape b) -> contact; open addressing, linear probing.
+typedef struct NphPairMap {
+    u64 *keys; // 0: empty; ~0: removed
+    u32 *values;
+    u32 cap; // power of two
+    u32 count;
+    u32 used; // count + removed
+} NphPairMap;
+
+#define NPH_TREE_STATIC 0U
+#define NPH_TREE_KINEMATIC 1U
+#define NPH_TREE_DYNAMIC 2U
+
+struct NphPhysicsWorld {
+    NphAllocator *alloc;
+    NphPhysicsWorldDesc desc;
+    NphDevice *dev;
+    f32 hit_threshold;
+    int stepping;
+
+    NphSlots body_slots;
+    NphVecBuf bodies;    // HBody
+    NphVecBuf body_recs; // NphBodyRec, the host copy (fresh unless bodies_stale)
+    int bodies_stale;
+    NphSlots shape_slots;
+    NphVecBuf shapes;     // HShape
+    NphVecBuf contacts;   // HContact
+    NphVecBuf free_contacts; // u32
+    NphVecBuf awake_bodies;   // u32 body slots
+    NphVecBuf awake_contacts; // u32
+    NphPairMap pairs;
+    NphVecBuf inits; // NphContactInit of records made since the last collide
+
+    NphTree trees[3];
+    NphVecBuf move_buffer; // u32 proxy keys
+    NphVecBuf pair_buffer; // u32 pairs (a, b)
+
+    NphVecBuf color_bits[NPH_GRAPH_COLORS]; // u64 words over body slots
+    NphVecBuf color_contacts[NPH_GRAPH_COLORS + 1U]; // u32; [24] = overflow
+    NphVecBuf graph; // u32, the colour lists concatenated for the device
+
+    NphVecBuf changed; // u32
+    NphVecBuf codes;   // u32
+
+    NphVecBuf begin_events; // NphPhysicsContactBeginEvent
+    NphVecBuf end_events;
+    NphVecBuf move_events; // NphPhysicsBodyMoveEvent, built on request
+    int move_events_ready;
+    NphVecBuf contact_view; // NphContactRec, readback scratch
+
+    NphPhysicsStats stats;
+    f32 step_dt;
+    u32 solved_contacts; // the last step's graph contacts
+};
+
+u32 phys_handle(u32 slot, u32 generation);
+u32 phys_slot(u32 handle);
+NphResult slots_alloc(NphAllocator *a, NphSlots *s, u32 *slot, u32 *generation);
+void slots_release(NphAllocator *a, NphSlots *s, u32 slot);
+
+HBody *world_body(const NphPhysicsWorld *w, NphPhysicsBody h);
+HShape *world_shape(const NphPhysicsWorld *w, NphPhysicsShape h);
+NphResult world_sync_bodies(NphPhysicsWorld *w); // device -> host copy when stale
+NphResult world_upload_body(NphPhysicsWorld *w, u32 slot);
+NphResult world_wake_body(NphPhysicsWorld *w, u32 slot);
+
+// contact.c restated
+NphResult contacts_update_pairs(NphPhysicsWorld *w);
+NphResult contacts_collide(NphPhysicsWorld *w);
+NphResult contacts_post_solve(NphPhysicsWorld *w, const NphGrown *grown, u32 count);
+NphResult move_buffer_push(NphPhysicsWorld *w, u32 shape);
+
+NphResult pairmap_init(NphAllocator *a, NphPairMap *m);
+void pairmap_free(NphAllocator *a, NphPairMap *m);
+
+#endif // NPH_WORLD_IMPL_H

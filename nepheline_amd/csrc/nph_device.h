@@ -1,0 +1,86 @@
+// SPDX-License-Identifier: Apache-2.0
+// The boundary between the host bookkeeping (C17) and the device that runs
+// the step's data-parallel stages.  One implementation ships: HIP on
+// gfx950 (device_hip.hip).  All records live in device memory; the host
+// sends what the serial bookkeeping decided (new contacts, the constraint
+// graph's colour lists) and reads back what it needs to decide the next
+// serial step (changed contacts, grown proxies).
+#ifndef NPH_DEVICE_H
+#define NPH_DEVICE_H
+
+#include "nph_step.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct NphDevice NphDevice;
+
+#define NPH_GRAPH_COLORS 24U // world_internal.h:359
+#define NPH_GRAPH_DYNAMIC_COLORS 20U
+#define NPH_GRAPH_OVERFLOW NPH_GRAPH_COLORS
+
+// A contact record the host made this step (contact.c:111-127 on the device).
+typedef struct NphContactInit {
+    u32 record;
+    u32 shape_a;
+    u32 shape_b;
+    u32 body_a;
+    u32 body_b;
+} NphContactInit;
+
+// A shape whose fat box grew in finalize, in the awake list's order and each
+// body's shape-list order (solver.c:868-885).
+typedef struct NphGrown {
+    u32 shape;
+    u32 pad;
+    NphAABB fat_aabb;
+} NphGrown;
+
+typedef struct NphSolveInput {
+    const u32 *awake_bodies; // body slots, solver order
+    u32 awake_count;
+    u32 body_slots; // slots in use (the solver map's size)
+    // The graph's contacts: colours 0..23, then the overflow, each list in
+    // its own order (the reference's colour arrays).
+    const u32 *graph_contacts;
+    u32 color_count[NPH_GRAPH_COLORS + 1U];
+    NphStepParams params;
+} NphSolveInput;
+
+NphResult nph_dev_create(NphDevice **out);
+void nph_dev_destroy(NphDevice *d);
+const char *nph_dev_name(void);
+// Grows the record arrays (contents kept) to at least these many slots.
+NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts);
+// A device copy of a hull block, made once per block and kept for the
+// device's life; *out is its device address.
+NphResult nph_dev_hull(NphDevice *d, const NphPhysicsHull *hull, const NphPhysicsHull **out);
+NphResult nph_dev_write_body(NphDevice *d, u32 slot, const NphBodyRec *b);
+NphResult nph_dev_write_shape(NphDevice *d, u32 slot, const NphShapeRec *s);
+NphResult nph_dev_read_bodies(NphDevice *d, NphBodyRec *out, u32 count);
+NphResult nph_dev_read_shapes(NphDevice *d, NphShapeRec *out, u32 count);
+NphResult nph_dev_read_contacts(NphDevice *d, NphContactRec *out, u32 count);
+NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 count);
+// The narrow phase over `awake` (contact records, the awake list's order):
+// manifolds updated in place; the records with a change, in list order, and
+// their change codes, into changed[] and codes[] (room for `count`).
+NphResult nph_dev_collide(NphDevice *d, const u32 *awake, u32 count, f32 step_dt, u32 *changed,
+                          u32 *codes, u32 *changed_count);
+// The solver stages and finalize; the grown proxies into *grown (the
+// device's buffer, valid until the next call), *grown_count of them.
+NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **grown,
+                        u32 *grown_count);
+// Seconds of device time the last solve took (event-timed), and the last
+// collide.
+f64 nph_dev_last_solve_ms(const NphDevice *d);
+f64 nph_dev_last_collide_ms(const NphDevice *d);
+// The constraint passes alone (warm start, solve, relax over every substep,
+// then restitution): the stream's events around them.
+f64 nph_dev_last_passes_ms(const NphDevice *d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif // NPH_DEVICE_H

@@ -1,0 +1,348 @@
+# SPDX-License-Identifier: Apache-2.0
+"""Python mirror of the reference's physics C API (engine/physics/include/
+nepheline/physics/{world,body,shape,geometry,types}.h) over ctypes.
+
+One binding class drives any library that exports that API: this package's
+own libnepheline_amd.so (the HIP world step) or, in tests, the reference's
+own code built under oracle/_ref.  Struct layouts follow the reference's
+headers field for field (world.h:104-143, body.h:45-72, shape.h:59-81,
+geometry.h:68-78, types.h:65-70); the readback record is the extension
+declared in include/nepheline_physics.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PRODUCT_LIB = os.path.join(HERE, "libnepheline_amd.so")
+
+# NphResult (engine/core/include/nepheline/core/result.h:15-37)
+OK, OUT_OF_MEMORY, INVALID_ARGUMENT, NOT_FOUND = 0, 1, 2, 3
+UNSUPPORTED = 8
+
+SHAPE_SPHERE, SHAPE_CAPSULE, SHAPE_BOX, SHAPE_HULL = 0, 1, 2, 3
+BODY_STATIC, BODY_KINEMATIC, BODY_DYNAMIC = 0, 1, 2
+NULL32 = 0xFFFFFFFF
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class DVec3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+class Quat(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class Mat3(C.Structure):
+    _fields_ = [("c", Vec3 * 3)]
+
+
+class AABB(C.Structure):
+    _fields_ = [("min", Vec3), ("max", Vec3)]
+
+
+class Pose(C.Structure):
+    _fields_ = [("position", Vec3), ("rotation", Quat)]
+
+
+class MassData(C.Structure):
+    _fields_ = [("mass", C.c_float), ("center", Vec3), ("inertia", Mat3)]
+
+
+class _GeomUnion(C.Union):
+    _fields_ = [("radius", C.c_float), ("capsule", C.c_float * 2), ("half_extents", Vec3),
+                ("ptr", C.c_void_p)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("type", C.c_int), ("u", _GeomUnion)]
+
+
+class WorldDesc(C.Structure):
+    _fields_ = [("alloc", C.c_void_p), ("gravity", Vec3), ("solver", C.c_int),
+                ("substeps", C.c_uint32), ("contact_hertz", C.c_float),
+                ("contact_damping_ratio", C.c_float), ("contact_push_max_velocity", C.c_float),
+                ("restitution_threshold", C.c_float), ("max_linear_speed", C.c_float),
+                ("joint_hertz", C.c_float), ("joint_damping_ratio", C.c_float),
+                ("time_to_sleep", C.c_float), ("enable_sleep", C.c_int32),
+                ("enable_continuous", C.c_int32), ("body_capacity", C.c_uint32),
+                ("shape_capacity", C.c_uint32), ("contact_capacity", C.c_uint32)]
+
+
+class BodyDesc(C.Structure):
+    _fields_ = [("type", C.c_int), ("position", DVec3), ("rotation", Quat),
+                ("linear_velocity", Vec3), ("angular_velocity", Vec3),
+                ("linear_damping", C.c_float), ("angular_damping", C.c_float),
+                ("gravity_scale", C.c_float), ("sleep_threshold", C.c_float),
+                ("locks", C.c_uint32), ("enable_sleep", C.c_int32), ("awake", C.c_int32),
+                ("gyroscopic", C.c_int32), ("bullet", C.c_int32), ("user_data", C.c_uint64)]
+
+
+class Filter(C.Structure):
+    _fields_ = [("category", C.c_uint64), ("mask", C.c_uint64), ("group", C.c_int32)]
+
+
+class ShapeDesc(C.Structure):
+    _fields_ = [("geometry", Geometry), ("pose", Pose), ("density", C.c_float),
+                ("friction", C.c_float), ("restitution", C.c_float),
+                ("friction_combine", C.c_int), ("restitution_combine", C.c_int),
+                ("filter", Filter), ("contact_events", C.c_int32), ("hit_events", C.c_int32),
+                ("sensor", C.c_int32), ("sensor_events", C.c_int32), ("ghost", C.c_int32),
+                ("user_data", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "bodies", "awake_bodies", "shapes", "contacts", "touching_contacts", "manifold_points",
+        "joints", "islands", "awake_islands", "colors_used", "overflow_constraints",
+        "moved_proxies", "new_pairs", "swept_bodies", "impact_bodies", "sensors",
+        "sensor_visitors", "workers")] + [("steps", C.c_uint64)] + [(n, C.c_double) for n in (
+        "step_ms", "pairs_ms", "collide_ms", "solve_ms", "finalize_ms", "sleep_ms",
+        "sensors_ms")]
+
+
+class ContactPointInfo(C.Structure):
+    _fields_ = [("anchor_a", Vec3), ("anchor_b", Vec3), ("separation", C.c_float),
+                ("id", C.c_uint32), ("normal_impulse", C.c_float),
+                ("max_normal_impulse", C.c_float), ("tangent_impulse", Vec3),
+                ("relative_velocity", C.c_float)]
+
+
+class ContactInfo(C.Structure):
+    _fields_ = [("record", C.c_uint32), ("shape_a", C.c_uint32), ("shape_b", C.c_uint32),
+                ("body_a", C.c_uint32), ("body_b", C.c_uint32), ("flags", C.c_uint32),
+                ("color", C.c_uint32), ("color_slot", C.c_uint32), ("parent", C.c_uint32),
+                ("child", C.c_uint32), ("friction", C.c_float), ("restitution", C.c_float),
+                ("normal", Vec3), ("point_count", C.c_uint32),
+                ("points", ContactPointInfo * 4)]
+
+
+assert C.sizeof(Pose) == 28 and C.sizeof(Geometry) == 24
+assert C.sizeof(ContactPointInfo) == 56 and C.sizeof(ContactInfo) == 288
+
+_U32 = C.c_uint32
+
+
+def _bind(lib, name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+class Api:
+    """The bound functions of one library exporting the reference's API."""
+
+    def __init__(self, path: str, allocator_symbol: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"physics library not built: {path}")
+        # RTLD_LOCAL: the product and the reference export the same names.
+        self.lib = C.CDLL(path, mode=os.RTLD_NOW | os.RTLD_LOCAL)
+        L = self.lib
+        self.path = path
+        self.allocator = _bind(L, allocator_symbol, C.c_void_p, [])
+        self.world_desc_default = _bind(L, "nph_physics_world_desc_default", WorldDesc, [])
+        self.world_create = _bind(L, "nph_physics_world_create", C.c_int,
+                                  [C.POINTER(WorldDesc), C.POINTER(C.c_void_p)])
+        self.world_destroy = _bind(L, "nph_physics_world_destroy", None, [C.c_void_p])
+        self.world_step = _bind(L, "nph_physics_world_step", C.c_int,
+                                [C.c_void_p, C.c_float, C.c_void_p])
+        self.world_hash = _bind(L, "nph_physics_world_hash", C.c_uint64, [C.c_void_p])
+        self.world_stats = _bind(L, "nph_physics_world_stats", Stats, [C.c_void_p])
+        self.world_validate = _bind(L, "nph_physics_world_validate", C.c_bool, [C.c_void_p])
+        self.world_contacts = _bind(L, "nph_physics_world_contacts", _U32,
+                                    [C.c_void_p, C.POINTER(ContactInfo), _U32])
+        self.body_desc_default = _bind(L, "nph_physics_body_desc_default", BodyDesc, [])
+        self.body_create = _bind(L, "nph_physics_body_create", C.c_int,
+                                 [C.c_void_p, C.POINTER(BodyDesc), C.POINTER(_U32)])
+        self.body_position = _bind(L, "nph_physics_body_position", DVec3, [C.c_void_p, _U32])
+        self.body_rotation = _bind(L, "nph_physics_body_rotation", Quat, [C.c_void_p, _U32])
+        self.body_center = _bind(L, "nph_physics_body_center_of_mass", DVec3,
+                                 [C.c_void_p, _U32])
+        self.body_linear_velocity = _bind(L, "nph_physics_body_linear_velocity", Vec3,
+                                          [C.c_void_p, _U32])
+        self.body_angular_velocity = _bind(L, "nph_physics_body_angular_velocity", Vec3,
+                                           [C.c_void_p, _U32])
+        self.body_mass_data = _bind(L, "nph_physics_body_mass_data", MassData,
+                                    [C.c_void_p, _U32])
+        self.shape_desc_default = _bind(L, "nph_physics_shape_desc_default", ShapeDesc,
+                                        [Geometry])
+        self.shape_create = _bind(L, "nph_physics_shape_create", C.c_int,
+                                  [C.c_void_p, _U32, C.POINTER(ShapeDesc), C.POINTER(_U32)])
+        self.shape_aabb = _bind(L, "nph_physics_shape_aabb", AABB, [C.c_void_p, _U32])
+        self.geometry_mass = _bind(L, "nph_physics_geometry_mass", MassData,
+                                   [C.POINTER(Geometry), C.c_float])
+
+
+_APIS: dict = {}
+
+
+def api(path: str = PRODUCT_LIB, allocator_symbol: str = "nph_amd_allocator_system") -> Api:
+    key = (path, allocator_symbol)
+    if key not in _APIS:
+        _APIS[key] = Api(path, allocator_symbol)
+    return _APIS[key]
+
+
+def product_api() -> Api:
+    """The HIP world step.  Raises if the extension is not built: there is no
+    CPU fallback."""
+    return api(PRODUCT_LIB, "nph_amd_allocator_system")
+
+
+def geometry_sphere(radius):
+    g = Geometry()
+    g.type = SHAPE_SPHERE
+    g.u.radius = radius
+    return g
+
+
+def geometry_capsule(half_height, radius):
+    g = Geometry()
+    g.type = SHAPE_CAPSULE
+    g.u.capsule[0] = half_height
+    g.u.capsule[1] = radius
+    return g
+
+
+def geometry_box(hx, hy, hz):
+    g = Geometry()
+    g.type = SHAPE_BOX
+    g.u.half_extents = Vec3(hx, hy, hz)
+    return g
+
+
+def geometry_hull(block_ptr):
+    g = Geometry()
+    g.type = SHAPE_HULL
+    g.u.ptr = block_ptr
+    return g
+
+
+class World:
+    """A world of either library, with the calls the tests and the bench use."""
+
+    def __init__(self, a: Api, desc_overrides: dict | None = None):
+        self.api = a
+        d = a.world_desc_default()
+        d.alloc = a.allocator()
+        for k, v in (desc_overrides or {}).items():
+            if k == "gravity":
+                d.gravity = Vec3(*v)
+            else:
+                setattr(d, k, v)
+        self.desc = d
+        w = C.c_void_p()
+        r = a.world_create(C.byref(d), C.byref(w))
+        if r != OK:
+            raise RuntimeError(f"nph_physics_world_create failed: {r}")
+        self.ptr = w
+        self.bodies: list[int] = []
+        self.shapes: list[int] = []
+        self._keep = []  # hull blocks and other memory the world points at
+
+    def close(self):
+        if self.ptr:
+            self.api.world_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_body(self, body_type, position, rotation=(0.0, 0.0, 0.0, 1.0), velocity=None,
+                 angular_velocity=None, **fields):
+        a = self.api
+        bd = a.body_desc_default()
+        bd.type = body_type
+        bd.position = DVec3(*position)
+        bd.rotation = Quat(*rotation)
+        if velocity is not None:
+            bd.linear_velocity = Vec3(*velocity)
+        if angular_velocity is not None:
+            bd.angular_velocity = Vec3(*angular_velocity)
+        for k, v in fields.items():
+            setattr(bd, k, v)
+        h = _U32()
+        r = a.body_create(self.ptr, C.byref(bd), C.byref(h))
+        if r != OK:
+            raise RuntimeError(f"nph_physics_body_create failed: {r}")
+        self.bodies.append(h.value)
+        return h.value
+
+    def add_shape(self, body, geometry, friction=0.6, restitution=0.0, density=None,
+                  pose=None, **fields):
+        a = self.api
+        sd = a.shape_desc_default(geometry)
+        sd.friction = friction
+        sd.restitution = restitution
+        if density is not None:
+            sd.density = density
+        if pose is not None:
+            sd.pose = Pose(Vec3(*pose[0]), Quat(*pose[1]))
+        for k, v in fields.items():
+            setattr(sd, k, v)
+        h = _U32()
+        r = a.shape_create(self.ptr, body, C.byref(sd), C.byref(h))
+        if r != OK:
+            raise RuntimeError(f"nph_physics_shape_create failed: {r}")
+        self.shapes.append(h.value)
+        return h.value
+
+    def step(self, dt, jobs=None):
+        r = self.api.world_step(self.ptr, dt, jobs)
+        if r != OK:
+            raise RuntimeError(f"nph_physics_world_step failed: {r}")
+
+    def hash(self) -> int:
+        return int(self.api.world_hash(self.ptr))
+
+    def stats(self) -> Stats:
+        return self.api.world_stats(self.ptr)
+
+    def contacts(self):
+        n = self.api.world_contacts(self.ptr, None, 0)
+        buf = (ContactInfo * max(n, 1))()
+        m = self.api.world_contacts(self.ptr, buf, n)
+        assert m == n
+        return buf[:n]
+
+    def body_state(self) -> np.ndarray:
+        """Per body (creation order): origin xyz (f64), rotation xyzw, v, w."""
+        out = np.zeros((len(self.bodies), 13), dtype=np.float64)
+        a = self.api
+        for i, h in enumerate(self.bodies):
+            p = a.body_position(self.ptr, h)
+            q = a.body_rotation(self.ptr, h)
+            v = a.body_linear_velocity(self.ptr, h)
+            w = a.body_angular_velocity(self.ptr, h)
+            out[i] = (p.x, p.y, p.z, q.x, q.y, q.z, q.w, v.x, v.y, v.z, w.x, w.y, w.z)
+        return out
+
+
+def contacts_as_arrays(contacts):
+    """Canonical views of a contact readback: the pair list (min, max shape
+    slot) and the contact-id list (slot_a, slot_b, chain index, point id) of
+    touching manifolds, both sorted, plus a dict of per-contact values keyed
+    by (shape_a, shape_b)."""
+    pairs = sorted((min(c.shape_a, c.shape_b), max(c.shape_a, c.shape_b))
+                   for c in contacts if c.parent == NULL32)
+    ids = []
+    by_pair = {}
+    for c in contacts:
+        if c.point_count > 0:
+            for k in range(c.point_count):
+                ids.append((c.shape_a, c.shape_b, 0 if c.parent == NULL32 else 1,
+                            c.points[k].id))
+        by_pair[(c.shape_a, c.shape_b)] = c
+    ids.sort()
+    return pairs, ids, by_pair

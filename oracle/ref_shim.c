@@ -1,0 +1,151 @@
+// SPDX-License-Identifier: Apache-2.0
+// Test-infrastructure shim compiled into oracle/_ref/libnph_ref.so next to
+// the reference's own physics sources (oracle/Makefile).  It gives the
+// reference world the readback that include/nepheline_physics.h declares as
+// an extension (nph_physics_world_contacts), read white-box from the
+// reference's records the way its own tests do (test_world.c:123-124), and
+// a few helpers a ctypes caller needs (the system allocator, a job system,
+// the solver's forced thread count, world_internal.h:653).
+//
+// Only tests, smoke() and bench.py's CPU-baseline leg load this library.
+
+#include <nepheline/core/alloc.h>
+#include <nepheline/jobs/jobs.h>
+
+#include "world_internal.h"
+
+#include <string.h>
+
+// Same layout as NphPhysicsContactInfo in include/nepheline_physics.h.
+typedef struct ShimPoint {
+    NphVec3 anchor_a;
+    NphVec3 anchor_b;
+    f32 separation;
+    u32 id;
+    f32 normal_impulse;
+    f32 max_normal_impulse;
+    NphVec3 tangent_impulse;
+    f32 relative_velocity;
+} ShimPoint;
+
+typedef struct ShimContact {
+    u32 record;
+    u32 shape_a;
+    u32 shape_b;
+    u32 body_a;
+    u32 body_b;
+    u32 flags;
+    u32 color;
+    u32 color_slot;
+    u32 parent;
+    u32 child;
+    f32 friction;
+    f32 restitution;
+    NphVec3 normal;
+    u32 point_count;
+    ShimPoint points[4];
+} ShimContact;
+
+_Static_assert(sizeof(ShimPoint) == 56, "PhysContactPoint layout");
+_Static_assert(sizeof(ShimContact) == 64 + 4 * 56, "contact info layout");
+
+u32 nph_physics_world_contacts(const NphPhysicsWorld *w, ShimContact *out, u32 capacity)
+{
+    u32 n = 0U;
+    for (u32 i = 0U; i < w->contacts.len; ++i) {
+        const PhysContact *c = &w->contacts.data[i];
+        if (c->shape_a == PHYS_NULL) {
+            continue;
+        }
+        if (n < capacity) {
+            ShimContact *o = &out[n];
+            memset(o, 0, sizeof *o);
+            o->record = i;
+            o->shape_a = c->shape_a;
+            o->shape_b = c->shape_b;
+            o->body_a = c->body_a;
+            o->body_b = c->body_b;
+            o->flags = c->flags;
+            o->color = c->color;
+            o->color_slot = c->color_slot;
+            o->parent = c->parent;
+            o->child = c->child;
+            o->friction = c->friction;
+            o->restitution = c->restitution;
+            o->normal = c->normal;
+            o->point_count = c->point_count;
+            for (u32 k = 0U; k < c->point_count && k < 4U; ++k) {
+                const PhysContactPoint *p = &c->points[k];
+                o->points[k].anchor_a = p->anchor_a;
+                o->points[k].anchor_b = p->anchor_b;
+                o->points[k].separation = p->separation;
+                o->points[k].id = p->id;
+                o->points[k].normal_impulse = p->normal_impulse;
+                o->points[k].max_normal_impulse = p->max_normal_impulse;
+                o->points[k].tangent_impulse = p->tangent_impulse;
+                o->points[k].relative_velocity = p->relative_velocity;
+            }
+        }
+        n += 1U;
+    }
+    return n;
+}
+
+NphAllocator *ref_shim_allocator(void)
+{
+    return nph_allocator_system();
+}
+
+NphJobSystem *ref_shim_jobs_create(u32 workers)
+{
+    NphJobSystemDesc d;
+    memset(&d, 0, sizeof d);
+    d.alloc = nph_allocator_system();
+    d.worker_count = workers;
+    NphJobSystem *js = NULL;
+    if (nph_job_system_create(&d, &js) != NPH_RESULT_OK) {
+        return NULL;
+    }
+    return js;
+}
+
+void ref_shim_jobs_destroy(NphJobSystem *js)
+{
+    nph_job_system_destroy(js);
+}
+
+void ref_shim_force_participants(NphPhysicsWorld *w, u32 n)
+{
+    w->forced_participants = n;
+}
+
+// The awake contact list, in its order (the order the narrow phase's changes
+// are applied in, contact.c:677-687).
+u32 ref_shim_awake_contacts(const NphPhysicsWorld *w, u32 *out, u32 capacity)
+{
+    for (u32 i = 0U; i < w->awake_contacts.len && i < capacity; ++i) {
+        out[i] = w->awake_contacts.data[i];
+    }
+    return w->awake_contacts.len;
+}
+
+// A colour's contact list in order (c == 24: the overflow).
+u32 ref_shim_color_contacts(const NphPhysicsWorld *w, u32 c, u32 *out, u32 capacity)
+{
+    const NphArrayU32 *list = c == PHYS_GRAPH_OVERFLOW ? &w->overflow : &w->colors[c].contacts;
+    for (u32 i = 0U; i < list->len && i < capacity; ++i) {
+        out[i] = list->data[i];
+    }
+    return list->len;
+}
+
+// Contacts the solver took in the last step: every colour's and the
+// overflow's (the lanes build_groups fills, solver.c:810-843).
+u32 ref_shim_graph_contacts(const NphPhysicsWorld *w)
+{
+    u32 n = w->overflow.len;
+    for (u32 c = 0U; c < PHYS_GRAPH_COLORS; ++c) {
+        n += w->colors[c].contacts.len;
+    }
+    return n;
+}

@@ -1,0 +1,110 @@
+# SPDX-License-Identifier: Apache-2.0
+"""Writes the golden fixtures of tests/golden/ from the reference's own
+physics code, compiled from its sources by oracle/Makefile into
+oracle/_ref/libnph_ref.so (run in a container that has /root/reference).
+
+  hulls_12v.bin        256 hull blocks built by the reference's Quickhull
+                       (nph_physics_hull_build, hull.c:1141-1197) from seeded
+                       12-point clouds; the bank format of scenes.load_hull_bank
+  pyramid_hashes.json  nph_physics_world_hash (world.c:436-462) after each of
+                       300 steps of the 20-row pyramid, dt 1/60, 4 substeps
+  bin_hashes.json      the same for small bin drops (every shape type)
+
+usage: python tests/golden/make_fixtures.py
+"""
+import ctypes as C
+import json
+import math
+import os
+import struct
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from nepheline_amd import physics as P  # noqa: E402
+from nepheline_amd import scenes as S  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
+
+
+class HullDesc(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("point_count", C.c_uint32),
+                ("max_vertices", C.c_uint32), ("tolerance", C.c_float)]
+
+
+def hull_points(r: S.Pcg32, count: int):
+    """`count` points in a ball of radius U[0.3, 0.6]: a uniform direction
+    (two draws, z = 1 - 2u, phi = 2 pi v) times radius * sqrt(0.5 + 0.5 w)."""
+    radius = float(r.next_range(0.3, 0.6))
+    pts = []
+    for _ in range(count):
+        u = float(r.next_f32())
+        v = float(r.next_f32())
+        z = 1.0 - 2.0 * u
+        s = math.sqrt(max(0.0, 1.0 - z * z))
+        phi = 2.0 * math.pi * v
+        d = radius * math.sqrt(float(r.next_f32()) * 0.5 + 0.5)
+        pts.append((s * math.cos(phi) * d, s * math.sin(phi) * d, z * d))
+    return np.array(pts, dtype=np.float32)
+
+
+def make_hull_bank(api, n=256, seed=7, verts=12):
+    lib = api.lib
+    build = lib.nph_physics_hull_build
+    build.restype = C.c_int
+    build.argtypes = [C.POINTER(HullDesc), C.c_void_p, C.POINTER(C.c_void_p)]
+    r = S.Pcg32(seed)
+    blocks = []
+    while len(blocks) < n:
+        pts = hull_points(r, verts)
+        d = HullDesc(pts.ctypes.data, verts, 0, 0.0)
+        out = C.c_void_p()
+        if build(C.byref(d), api.allocator(), C.byref(out)) != 0:
+            continue
+        size = C.cast(out, C.POINTER(C.c_uint32))[0]
+        blocks.append(C.string_at(out, size))
+    head = 4 + 4 * n
+    head = (head + 15) & ~15
+    offsets = []
+    body = b""
+    for b in blocks:
+        offsets.append(head + len(body))
+        body += b + b"\0" * ((16 - len(b) % 16) % 16)
+    data = struct.pack("<I", n) + struct.pack("<%dI" % n, *offsets)
+    data += b"\0" * (head - len(data)) + body
+    with open(os.path.join(HERE, "hulls_12v.bin"), "wb") as f:
+        f.write(data)
+    return len(data)
+
+
+def hashes(world, steps, dt=1.0 / 60.0):
+    out = []
+    for _ in range(steps):
+        world.step(dt)
+        out.append("%016x" % world.hash())
+    return out
+
+
+def main():
+    api = P.api(REF, "ref_shim_allocator")
+    size = make_hull_bank(api)
+    print("hulls_12v.bin", size, "bytes")
+    w = S.pyramid(api)
+    with open(os.path.join(HERE, "pyramid_hashes.json"), "w") as f:
+        json.dump({"scene": "pyramid rows=20 dt=1/60 substeps=4", "hashes": hashes(w, 300)}, f)
+    bank = S.load_hull_bank(os.path.join(HERE, "hulls_12v.bin"))
+    runs = {}
+    for bodies, seed, steps in ((64, 11, 240), (400, 12, 200)):
+        w = S.bin_drop(api, bodies, seed, bank, half=4.0, height=10.0, y_lo=1.0)
+        runs["bin_%d_%d" % (bodies, seed)] = hashes(w, steps)
+    with open(os.path.join(HERE, "bin_hashes.json"), "w") as f:
+        json.dump({"scene": "bin_drop half=4 height=10 y_lo=1 dt=1/60 substeps=4",
+                   "runs": runs}, f)
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()
