@@ -23,6 +23,8 @@
 
 #define NPH_BLOCK 256U
 #define NPH_SCAN_BLOCK 1024U
+// The narrow phase keeps its clip buffers in private memory: one wave a block.
+#define NPH_COLLIDE_BLOCK 64U
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
@@ -641,8 +643,9 @@ NphResult nph_dev_collide(NphDevice *d, const u32 *awake, u32 count, f32 step_dt
     }
     HIP_TRY(hipMemcpyAsync(d->list, awake, (u64)count * 4U, hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipEventRecord(d->ev[0], d->stream));
-    hipLaunchKernelGGL(k_collide, dim3(blocks_for(count)), dim3(64), 0, d->stream, d->contacts,
-                       d->shapes, d->bodies, d->list, d->codes, count, step_dt);
+    hipLaunchKernelGGL(k_collide, dim3((count + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
+                       dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes, d->bodies,
+                       d->list, d->codes, count, step_dt);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_compact_changes, dim3(1), dim3(NPH_SCAN_BLOCK), 0, d->stream, d->list,
                        d->codes, count, d->compact);
