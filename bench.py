@@ -77,12 +77,16 @@ def run_gpu(args, bank):
     times = (C.c_double * 3)()
     solved = 0
     passes_ms = 0.0
+    phases = np.zeros(6)
     t0 = time.perf_counter()
     for i in range(args.steps):
         w.step(DT)
         solved += a.lib.nph_amd_solved_contacts(w.ptr)
         a.lib.nph_amd_device_times(w.ptr, times)
         passes_ms += times[2]
+        st = a.world_stats(w.ptr) if args.phases else None
+        if st is not None:
+            phases += (st.pairs_ms, st.collide_ms, st.solve_ms, times[0], times[1], times[2])
         if args.progress and (i + 1) % 100 == 0:
             print(f"# gpu step {i + 1}/{args.steps}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
@@ -94,6 +98,9 @@ def run_gpu(args, bank):
         "mean_bytes": mean_bytes,
         "graph_contacts_end": graph_n,
         "hash": "%016x" % w.hash(),
+        "phases_ms": {k: round(v / max(args.steps, 1), 4) for k, v in zip(
+            ("host_pairs", "host_collide", "host_solve", "dev_collide", "dev_solve", "dev_passes"),
+            phases)},
     }
 
 
@@ -142,6 +149,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--progress", action="store_true")
+    ap.add_argument("--phases", action="store_true",
+                    help="also report host and device time per step phase (reads stats "
+                         "each step, which adds a contact readback to the timed region)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the world step has no multi-GPU path "
@@ -189,6 +199,7 @@ def main():
                           cpu["cores"], args.warmup + 1, args.warmup + cpu["steps"], cpu["wall_s"]),
         },
         "gpu_hash": g["hash"],
+        "phases_ms_per_step": g["phases_ms"] if args.phases else None,
     }
     print(json.dumps(line), flush=True)
 

@@ -483,7 +483,7 @@ typedef struct PairQuery {
     NphPhysicsWorld *w;
     u32 shape;
     u32 key;
-    int fill;
+    u32 *out; // NULL: the counting pass
     u32 count;
 } PairQuery;
 
@@ -518,12 +518,34 @@ static int pair_callback(u32 proxy, u64 user_data, void *context)
     if (map_find(&w->pairs, pair_key(q->shape, other)) != NPH_NULL32) {
         return 1;
     }
-    const u32 pair[2] = {q->shape, other};
-    if (vbuf_push(w->alloc, &w->pair_buffer, 8U, pair) != NPH_RESULT_OK) {
-        return 0;
+    if (q->out != NULL) {
+        q->out[2U * q->count] = q->shape;
+        q->out[2U * q->count + 1U] = other;
     }
     q->count += 1U;
     return 1;
+}
+
+// contact.c:748-771: one moved proxy's new pairs, into `out` (NULL: counted).
+static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out)
+{
+    const u32 tree = key & 3U;
+    const u32 proxy = key >> 2U;
+    PairQuery q;
+    q.w = w;
+    q.shape = (u32)tree_user_data(&w->trees[tree], proxy);
+    q.key = key;
+    q.out = out;
+    q.count = 0U;
+    const HShape *s = shape_at(w, q.shape);
+    const NphAABB box = s->fat_aabb;
+    const u64 mask = s->filter.group > 0 ? ~0ULL : s->filter.mask;
+    if (tree == NPH_TREE_DYNAMIC) {
+        tree_query(&w->trees[NPH_TREE_STATIC], box, mask, pair_callback, &q);
+        tree_query(&w->trees[NPH_TREE_KINEMATIC], box, mask, pair_callback, &q);
+    }
+    tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, pair_callback, &q);
+    return q.count;
 }
 
 NphResult contacts_update_pairs(NphPhysicsWorld *w)
@@ -540,27 +562,39 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
     if (n == 0U) {
         return NPH_RESULT_OK;
     }
-    // Each moved proxy's pairs in the trees' order, the proxies in the move
-    // buffer's: the order the reference's count-then-fill leaves them in.
-    w->pair_buffer.len = 0U;
+    // Count, place by prefix sums, fill (contact.c:820-843): each moved proxy
+    // writes its own range, so the pairs come out in the move buffer's order
+    // and each proxy's in its trees' order, however many threads query.
+    NphResult r = vbuf_reserve(w->alloc, &w->pair_counts, 4U, n + 1U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    u32 *counts = VB(w->pair_counts, u32);
+    const u32 *moved = VB(w->move_buffer, u32);
+#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U)
     for (u32 i = 0U; i < n; ++i) {
-        const u32 key = VB(w->move_buffer, u32)[i];
-        const u32 tree = key & 3U;
-        const u32 proxy = key >> 2U;
-        PairQuery q;
-        q.w = w;
-        q.shape = (u32)tree_user_data(&w->trees[tree], proxy);
-        q.key = key;
-        q.fill = 1;
-        q.count = 0U;
-        const HShape *s = shape_at(w, q.shape);
-        const NphAABB box = s->fat_aabb;
-        const u64 mask = s->filter.group > 0 ? ~0ULL : s->filter.mask;
-        if (tree == NPH_TREE_DYNAMIC) {
-            tree_query(&w->trees[NPH_TREE_STATIC], box, mask, pair_callback, &q);
-            tree_query(&w->trees[NPH_TREE_KINEMATIC], box, mask, pair_callback, &q);
+        counts[i] = query_moved(w, moved[i], NULL);
+    }
+    u32 total = 0U;
+    for (u32 i = 0U; i < n; ++i) {
+        const u32 c = counts[i];
+        counts[i] = total;
+        total += c;
+    }
+    counts[n] = total;
+    r = vbuf_reserve(w->alloc, &w->pair_buffer, 8U, total > 0U ? total : 1U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    w->pair_buffer.len = total;
+    u32 *pairs = VB(w->pair_buffer, u32);
+    if (total > 0U) {
+#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U)
+        for (u32 i = 0U; i < n; ++i) {
+            if (counts[i + 1U] > counts[i]) {
+                (void)query_moved(w, moved[i], &pairs[2U * counts[i]]);
+            }
         }
-        tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, pair_callback, &q);
     }
     for (u32 i = 0U; i < n; ++i) {
         const u32 key = VB(w->move_buffer, u32)[i];
@@ -568,7 +602,6 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
         shape_at(w, s)->flags &= ~(u32)HS_IN_MOVE_BUFFER;
     }
     w->move_buffer.len = 0U;
-    const u32 total = w->pair_buffer.len;
     for (u32 i = 0U; i < total; ++i) {
         const u32 *p = &VB(w->pair_buffer, u32)[2U * i];
         if (map_find(&w->pairs, pair_key(p[0], p[1])) == NPH_NULL32) {

@@ -173,6 +173,7 @@ struct NphPhysicsWorld {
 
     NphTree trees[3];
     NphVecBuf move_buffer; // u32 proxy keys
+    NphVecBuf pair_counts; // u32 per moved proxy, then their prefix sums
     NphVecBuf pair_buffer; // u32 pairs (a, b)
 
     NphVecBuf color_bits[NPH_GRAPH_COLORS]; // u64 words over body slots

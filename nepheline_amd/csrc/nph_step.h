@@ -852,13 +852,84 @@ NPH_HD NphVec3 lane_axis(const NphStepData *d, u32 f, u32 L)
     return v3(LF(d, f, L), LF(d, f + 1U, L), LF(d, f + 2U, L));
 }
 
+// A lane's point rows held in registers for a pass: every load issued before
+// any arithmetic, every store after it (the rows are private to the lane,
+// so nothing in between can change them).
+typedef struct NphPointRegs {
+    NphVec3 ra;
+    NphVec3 rb;
+    f32 sep;
+    f32 nmass;
+    f32 nimp;
+    f32 nmax;
+    f32 relv;
+    f32 tm0;
+    f32 tm1;
+    f32 tm2;
+    f32 ti1;
+    f32 ti2;
+    NphVec3 aan;
+    NphVec3 abn;
+    NphVec3 aat1;
+    NphVec3 abt1;
+    NphVec3 aat2;
+    NphVec3 abt2;
+} NphPointRegs;
+
+// Which fields a pass reads.
+enum {
+    NPH_LOAD_WARM = 1U,  // impulses and angular terms
+    NPH_LOAD_SOLVE = 2U, // everything but the approach speed
+    NPH_LOAD_REST = 4U,  // anchors, mass, normal impulses, approach speed, normal terms
+};
+
+NPH_HD void point_load(const NphStepData *d, u64 R, u32 what, NphPointRegs *p)
+{
+    if ((what & (NPH_LOAD_SOLVE | NPH_LOAD_REST)) != 0U) {
+        p->ra = PV3(d, NPH_P_RAX, R);
+        p->rb = PV3(d, NPH_P_RBX, R);
+        p->nmass = PF(d, NPH_P_NMASS, R);
+        p->nmax = PF(d, NPH_P_NMAX, R);
+    }
+    if ((what & NPH_LOAD_SOLVE) != 0U) {
+        p->sep = PF(d, NPH_P_SEP, R);
+        p->tm0 = PF(d, NPH_P_TM0, R);
+        p->tm1 = PF(d, NPH_P_TM1, R);
+        p->tm2 = PF(d, NPH_P_TM2, R);
+    }
+    if ((what & NPH_LOAD_REST) != 0U) {
+        p->relv = PF(d, NPH_P_RELV, R);
+    }
+    p->nimp = PF(d, NPH_P_NIMP, R);
+    p->aan = PV3(d, NPH_P_AAN, R);
+    p->abn = PV3(d, NPH_P_ABN, R);
+    if ((what & (NPH_LOAD_WARM | NPH_LOAD_SOLVE)) != 0U) {
+        p->ti1 = PF(d, NPH_P_TI1, R);
+        p->ti2 = PF(d, NPH_P_TI2, R);
+        p->aat1 = PV3(d, NPH_P_AAT1, R);
+        p->abt1 = PV3(d, NPH_P_ABT1, R);
+        p->aat2 = PV3(d, NPH_P_AAT2, R);
+        p->abt2 = PV3(d, NPH_P_ABT2, R);
+    }
+}
+
 // solver_impl.h:181-258 (the soft step's warm start: nothing scales it).
 NPH_HD void lane_warm_start(const NphStepData *d, u32 g, u32 l)
 {
     const NphGroup *grp = &d->groups[g];
     const u32 L = g * NPH_LANES + l;
+    const u32 P = grp->point_count;
     const u32 ia = d->lane_body_a[L];
     const u32 ib = d->lane_body_b[L];
+    NphPointRegs pt[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            point_load(d, (u64)(grp->first_point + k) * NPH_LANES + l, NPH_LOAD_WARM, &pt[k]);
+        }
+    }
     NphLaneBody a = lane_gather(&d->sbodies[ia], 0);
     NphLaneBody b = lane_gather(&d->sbodies[ib], 0);
     const f32 ma = LF(d, NPH_L_IMA, L);
@@ -866,14 +937,15 @@ NPH_HD void lane_warm_start(const NphStepData *d, u32 g, u32 l)
     const NphVec3 n = lane_axis(d, NPH_L_NX, L);
     const NphVec3 t1 = lane_axis(d, NPH_L_T1X, L);
     const NphVec3 t2 = lane_axis(d, NPH_L_T2X, L);
-    for (u32 k = 0U; k < grp->point_count; ++k) {
-        const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
-        lane_apply(&a, &b, ma, mb, PF(d, NPH_P_NIMP, R), n, PV3(d, NPH_P_AAN, R),
-                   PV3(d, NPH_P_ABN, R));
-        lane_apply(&a, &b, ma, mb, PF(d, NPH_P_TI1, R), t1, PV3(d, NPH_P_AAT1, R),
-                   PV3(d, NPH_P_ABT1, R));
-        lane_apply(&a, &b, ma, mb, PF(d, NPH_P_TI2, R), t2, PV3(d, NPH_P_AAT2, R),
-                   PV3(d, NPH_P_ABT2, R));
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            lane_apply(&a, &b, ma, mb, pt[k].nimp, n, pt[k].aan, pt[k].abn);
+            lane_apply(&a, &b, ma, mb, pt[k].ti1, t1, pt[k].aat1, pt[k].abt1);
+            lane_apply(&a, &b, ma, mb, pt[k].ti2, t2, pt[k].aat2, pt[k].abt2);
+        }
     }
     const u32 flags = d->lane_flags[L];
     if ((flags & NPH_LANE_WRITE_A) != 0U) {
@@ -932,89 +1004,123 @@ NPH_HD void lane_solve(const NphStepData *d, u32 g, u32 l, f32 inv_h, f32 push_m
 {
     const NphGroup *grp = &d->groups[g];
     const u32 L = g * NPH_LANES + l;
+    const u32 P = grp->point_count;
     const u32 ia = d->lane_body_a[L];
     const u32 ib = d->lane_body_b[L];
+    NphPointRegs pt[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            point_load(d, (u64)(grp->first_point + k) * NPH_LANES + l, NPH_LOAD_SOLVE, &pt[k]);
+        }
+    }
     NphLaneBody a = lane_gather(&d->sbodies[ia], 1);
     NphLaneBody b = lane_gather(&d->sbodies[ib], 1);
     const f32 ma = LF(d, NPH_L_IMA, L);
     const f32 mb = LF(d, NPH_L_IMB, L);
     const NphVec3 n = lane_axis(d, NPH_L_NX, L);
+    const NphVec3 t1 = lane_axis(d, NPH_L_T1X, L);
+    const NphVec3 t2 = lane_axis(d, NPH_L_T2X, L);
+    const f32 friction = LF(d, NPH_L_FRICTION, L);
+    const u32 lane_flags = d->lane_flags[L];
     const f32 min_bias = -push_max;
     const f32 soft_bias = use_bias ? LF(d, NPH_L_BIAS, L) : 0.0F;
     const f32 soft_mass = use_bias ? LF(d, NPH_L_MASS_SCALE, L) : 1.0F;
     const f32 soft_impulse = use_bias ? LF(d, NPH_L_IMPULSE_SCALE, L) : 0.0F;
-    const u32 P = grp->point_count;
-    for (u32 k = 0U; k < P; ++k) {
-        const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
-        const NphVec3 ra = PV3(d, NPH_P_RAX, R);
-        const NphVec3 rb = PV3(d, NPH_P_RBX, R);
-        f32 bias;
-        f32 mscale;
-        f32 iscale;
-        normal_row(lane_separation(&a, &b, ra, rb, n, PF(d, NPH_P_SEP, R)), inv_h, slop, soft_bias,
-                   min_bias, soft_mass, soft_impulse, &bias, &mscale, &iscale);
-        const f32 vn = v3_dot(lane_relative_velocity(&a, &b, ra, rb), n);
-        const f32 row = PF(d, NPH_P_NMASS, R) * mscale;
-        const f32 accumulated = PF(d, NPH_P_NIMP, R);
-        const f32 impulse = -(row * (vn + bias)) - iscale * accumulated;
-        const f32 total = nph_maxf(accumulated + impulse, 0.0F);
-        const f32 delta = total - accumulated;
-        PF(d, NPH_P_NIMP, R) = total;
-        PF(d, NPH_P_NMAX, R) = nph_maxf(PF(d, NPH_P_NMAX, R), total);
-        lane_apply(&a, &b, ma, mb, delta, n, PV3(d, NPH_P_AAN, R), PV3(d, NPH_P_ABN, R));
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            NphPointRegs *p = &pt[k];
+            f32 bias;
+            f32 mscale;
+            f32 iscale;
+            normal_row(lane_separation(&a, &b, p->ra, p->rb, n, p->sep), inv_h, slop, soft_bias,
+                       min_bias, soft_mass, soft_impulse, &bias, &mscale, &iscale);
+            const f32 vn = v3_dot(lane_relative_velocity(&a, &b, p->ra, p->rb), n);
+            const f32 row = p->nmass * mscale;
+            const f32 accumulated = p->nimp;
+            const f32 impulse = -(row * (vn + bias)) - iscale * accumulated;
+            const f32 total = nph_maxf(accumulated + impulse, 0.0F);
+            const f32 delta = total - accumulated;
+            p->nimp = total;
+            p->nmax = nph_maxf(p->nmax, total);
+            lane_apply(&a, &b, ma, mb, delta, n, p->aan, p->abn);
+        }
     }
     if (P > 1U && grp->impact != 0U) {
-        const int impact = (d->lane_flags[L] & NPH_LANE_IMPACT) != 0U;
+        const int impact = (lane_flags & NPH_LANE_IMPACT) != 0U;
         f32 row_bias[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
         f32 row_mass[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
         f32 row_soft[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
-        for (u32 k = 0U; k < P; ++k) {
-            const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
-            f32 mscale;
-            normal_row(lane_separation(&a, &b, PV3(d, NPH_P_RAX, R), PV3(d, NPH_P_RBX, R), n,
-                                       PF(d, NPH_P_SEP, R)),
-                       inv_h, slop, soft_bias, min_bias, soft_mass, soft_impulse, &row_bias[k],
-                       &mscale, &row_soft[k]);
-            row_mass[k] = PF(d, NPH_P_NMASS, R) * mscale;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+        for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+            if (k < P) {
+                f32 mscale;
+                normal_row(lane_separation(&a, &b, pt[k].ra, pt[k].rb, n, pt[k].sep), inv_h, slop,
+                           soft_bias, min_bias, soft_mass, soft_impulse, &row_bias[k], &mscale,
+                           &row_soft[k]);
+                row_mass[k] = pt[k].nmass * mscale;
+            }
         }
         for (u32 sweep = 0U; sweep < NPH_IMPACT_SWEEPS; ++sweep) {
-            for (u32 k = 0U; k < P; ++k) {
-                const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
-                const NphVec3 ra = PV3(d, NPH_P_RAX, R);
-                const NphVec3 rb = PV3(d, NPH_P_RBX, R);
-                const f32 vn = v3_dot(lane_relative_velocity(&a, &b, ra, rb), n);
-                const f32 accumulated = PF(d, NPH_P_NIMP, R);
-                const f32 impulse = -(row_mass[k] * (vn + row_bias[k])) - row_soft[k] * accumulated;
-                const f32 total = impact ? nph_maxf(accumulated + impulse, 0.0F) : accumulated;
-                const f32 delta = total - accumulated;
-                PF(d, NPH_P_NIMP, R) = total;
-                PF(d, NPH_P_NMAX, R) = nph_maxf(PF(d, NPH_P_NMAX, R), total);
-                lane_apply(&a, &b, ma, mb, delta, n, PV3(d, NPH_P_AAN, R), PV3(d, NPH_P_ABN, R));
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+            for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+                if (k < P) {
+                    NphPointRegs *p = &pt[k];
+                    const f32 vn = v3_dot(lane_relative_velocity(&a, &b, p->ra, p->rb), n);
+                    const f32 accumulated = p->nimp;
+                    const f32 impulse =
+                        -(row_mass[k] * (vn + row_bias[k])) - row_soft[k] * accumulated;
+                    const f32 total = impact ? nph_maxf(accumulated + impulse, 0.0F) : accumulated;
+                    const f32 delta = total - accumulated;
+                    p->nimp = total;
+                    p->nmax = nph_maxf(p->nmax, total);
+                    lane_apply(&a, &b, ma, mb, delta, n, p->aan, p->abn);
+                }
             }
         }
     }
-    const NphVec3 t1 = lane_axis(d, NPH_L_T1X, L);
-    const NphVec3 t2 = lane_axis(d, NPH_L_T2X, L);
-    const f32 friction = LF(d, NPH_L_FRICTION, L);
-    for (u32 k = 0U; k < P; ++k) {
-        const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
-        const NphVec3 dv = lane_relative_velocity(&a, &b, PV3(d, NPH_P_RAX, R), PV3(d, NPH_P_RBX, R));
-        const f32 vt1 = v3_dot(dv, t1);
-        const f32 vt2 = v3_dot(dv, t2);
-        const f32 m00 = PF(d, NPH_P_TM0, R);
-        const f32 m01 = PF(d, NPH_P_TM1, R);
-        const f32 m11 = PF(d, NPH_P_TM2, R);
-        const f32 old1 = PF(d, NPH_P_TI1, R);
-        const f32 old2 = PF(d, NPH_P_TI2, R);
-        f32 new1 = old1 - (m00 * vt1 + m01 * vt2);
-        f32 new2 = old2 - (m01 * vt1 + m11 * vt2);
-        friction_circle(vt1, vt2, friction * PF(d, NPH_P_NIMP, R), &new1, &new2);
-        PF(d, NPH_P_TI1, R) = new1;
-        PF(d, NPH_P_TI2, R) = new2;
-        lane_apply(&a, &b, ma, mb, new1 - old1, t1, PV3(d, NPH_P_AAT1, R), PV3(d, NPH_P_ABT1, R));
-        lane_apply(&a, &b, ma, mb, new2 - old2, t2, PV3(d, NPH_P_AAT2, R), PV3(d, NPH_P_ABT2, R));
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            NphPointRegs *p = &pt[k];
+            const NphVec3 dv = lane_relative_velocity(&a, &b, p->ra, p->rb);
+            const f32 vt1 = v3_dot(dv, t1);
+            const f32 vt2 = v3_dot(dv, t2);
+            const f32 old1 = p->ti1;
+            const f32 old2 = p->ti2;
+            f32 new1 = old1 - (p->tm0 * vt1 + p->tm1 * vt2);
+            f32 new2 = old2 - (p->tm1 * vt1 + p->tm2 * vt2);
+            friction_circle(vt1, vt2, friction * p->nimp, &new1, &new2);
+            p->ti1 = new1;
+            p->ti2 = new2;
+            lane_apply(&a, &b, ma, mb, new1 - old1, t1, p->aat1, p->abt1);
+            lane_apply(&a, &b, ma, mb, new2 - old2, t2, p->aat2, p->abt2);
+        }
     }
-    const u32 flags = d->lane_flags[L];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
+            PF(d, NPH_P_NIMP, R) = pt[k].nimp;
+            PF(d, NPH_P_NMAX, R) = pt[k].nmax;
+            PF(d, NPH_P_TI1, R) = pt[k].ti1;
+            PF(d, NPH_P_TI2, R) = pt[k].ti2;
+        }
+    }
+    const u32 flags = lane_flags;
     if ((flags & NPH_LANE_WRITE_A) != 0U) {
         lane_scatter(&d->sbodies[ia], &a);
     }
@@ -1050,35 +1156,54 @@ NPH_HD void lane_restitution(const NphStepData *d, u32 g, u32 l)
         return;
     }
     const u32 L = g * NPH_LANES + l;
+    const u32 P = grp->point_count;
     const u32 ia = d->lane_body_a[L];
     const u32 ib = d->lane_body_b[L];
+    NphPointRegs pt[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            point_load(d, (u64)(grp->first_point + k) * NPH_LANES + l, NPH_LOAD_REST, &pt[k]);
+        }
+    }
     NphLaneBody a = lane_gather(&d->sbodies[ia], 0);
     NphLaneBody b = lane_gather(&d->sbodies[ib], 0);
     const f32 ma = LF(d, NPH_L_IMA, L);
     const f32 mb = LF(d, NPH_L_IMB, L);
     const NphVec3 n = lane_axis(d, NPH_L_NX, L);
     const f32 e = LF(d, NPH_L_RESTITUTION, L);
-    const u32 P = grp->point_count;
     const u32 sweeps = P > 1U ? 1U + NPH_IMPACT_SWEEPS : 1U;
     for (u32 sweep = 0U; sweep < sweeps; ++sweep) {
-        for (u32 k = 0U; k < P; ++k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+        for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
             const u32 row_mask = (loaded_all >> (k * NPH_LANES)) & 0xFFU;
-            if (row_mask == 0U) {
+            if (k >= P || row_mask == 0U) {
                 continue;
             }
+            NphPointRegs *p = &pt[k];
             const int loaded = (row_mask >> l) & 1U;
-            const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
-            const NphVec3 ra = PV3(d, NPH_P_RAX, R);
-            const NphVec3 rb = PV3(d, NPH_P_RBX, R);
-            const f32 vn = v3_dot(lane_relative_velocity(&a, &b, ra, rb), n);
-            const f32 rel = PF(d, NPH_P_RELV, R);
-            const f32 accumulated = PF(d, NPH_P_NIMP, R);
-            const f32 impulse = -(PF(d, NPH_P_NMASS, R) * (vn + e * rel));
+            const f32 vn = v3_dot(lane_relative_velocity(&a, &b, p->ra, p->rb), n);
+            const f32 accumulated = p->nimp;
+            const f32 impulse = -(p->nmass * (vn + e * p->relv));
             const f32 total = nph_maxf(accumulated + impulse, 0.0F);
             const f32 delta = loaded ? total - accumulated : 0.0F;
-            PF(d, NPH_P_NIMP, R) = accumulated + delta;
-            PF(d, NPH_P_NMAX, R) = nph_maxf(PF(d, NPH_P_NMAX, R), accumulated + delta);
-            lane_apply(&a, &b, ma, mb, delta, n, PV3(d, NPH_P_AAN, R), PV3(d, NPH_P_ABN, R));
+            p->nimp = accumulated + delta;
+            p->nmax = nph_maxf(p->nmax, accumulated + delta);
+            lane_apply(&a, &b, ma, mb, delta, n, p->aan, p->abn);
+        }
+    }
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (u32 k = 0U; k < NPH_PHYSICS_MANIFOLD_MAX_POINTS; ++k) {
+        if (k < P) {
+            const u64 R = (u64)(grp->first_point + k) * NPH_LANES + l;
+            PF(d, NPH_P_NIMP, R) = pt[k].nimp;
+            PF(d, NPH_P_NMAX, R) = pt[k].nmax;
         }
     }
     const u32 flags = d->lane_flags[L];
