@@ -51,12 +51,16 @@ class Pcg32:
         return np.float32(np.float32(lo) + np.float32(np.float32(hi) - np.float32(lo)) * t)
 
     def next_bounded(self, bound: int) -> int:
-        """Unbiased [0, bound) by rejection (Lemire's threshold)."""
-        threshold = (-bound & 0xFFFFFFFF) % bound
-        while True:
-            r = self.next_u32()
-            if r >= threshold:
-                return r % bound
+        """Unbiased [0, bound): Lemire's multiply-shift with rejection, as
+        nph_pcg32_next_bounded (engine/math/src/random.c:20-38)."""
+        m = self.next_u32() * bound
+        low = m & 0xFFFFFFFF
+        if low < bound:
+            t = (-bound & 0xFFFFFFFF) % bound
+            while low < t:
+                m = self.next_u32() * bound
+                low = m & 0xFFFFFFFF
+        return m >> 32
 
 
 def rotation(r: Pcg32):

@@ -149,3 +149,51 @@ u32 ref_shim_graph_contacts(const NphPhysicsWorld *w)
     }
     return n;
 }
+
+// The reference's eight-lane contact kernels (solver_impl.h, the table of
+// solver_sse4.c / solver_avx2.c) on caller-owned lanes, for pinning the
+// oracle's restatement of them bit for bit.
+void ref_shim_prepare(const PhysSolverBody *bodies, PhysContactLanes *g, PhysPointLanes *points,
+                      const PhysPrepareLanes *in)
+{
+    phys_solver_kernels_best()->prepare(bodies, g, points, in);
+}
+
+void ref_shim_warm_start(PhysSolverBody *bodies, PhysContactLanes *g, PhysPointLanes *points)
+{
+    phys_solver_kernels_best()->warm_start(bodies, g, points);
+}
+
+void ref_shim_solve(PhysSolverBody *bodies, PhysContactLanes *g, PhysPointLanes *points, f32 inv_h,
+                    f32 push_max, f32 slop, u32 use_bias)
+{
+    phys_solver_kernels_best()->solve(bodies, g, points, inv_h, push_max, slop, use_bias != 0U);
+}
+
+void ref_shim_restitution(PhysSolverBody *bodies, PhysContactLanes *g, PhysPointLanes *points,
+                          f32 threshold)
+{
+    phys_solver_kernels_best()->restitution(bodies, g, points, threshold);
+}
+
+void ref_shim_make_soft(f32 hertz, f32 zeta, f32 h, f32 out[3])
+{
+    const PhysSoftness s = phys_make_soft(hertz, zeta, h);
+    out[0] = s.bias_rate;
+    out[1] = s.mass_scale;
+    out[2] = s.impulse_scale;
+}
+
+u32 ref_shim_sizes(u32 which)
+{
+    switch (which) {
+    case 0:
+        return (u32)sizeof(PhysSolverBody);
+    case 1:
+        return (u32)sizeof(PhysContactLanes);
+    case 2:
+        return (u32)sizeof(PhysPointLanes);
+    default:
+        return (u32)sizeof(PhysPrepareLanes);
+    }
+}
