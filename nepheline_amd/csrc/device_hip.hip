@@ -341,7 +341,7 @@ NphResult nph_dev_read_contacts(NphDevice *d, NphContactRec *out, u32 count)
 // Kernels
 // ---------------------------------------------------------------------------
 
-__global__ void k_init_contacts(NphContactRec *contacts, const NphContactInit *list, u32 n)
+__global__ void __launch_bounds__(NPH_BLOCK) k_init_contacts(NphContactRec *contacts, const NphContactInit *list, u32 n)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) {
@@ -357,7 +357,7 @@ __global__ void k_init_contacts(NphContactRec *contacts, const NphContactInit *l
     c->normal = v3(0.0F, 1.0F, 0.0F);
 }
 
-__global__ void k_collide(NphContactRec *contacts, const NphShapeRec *shapes,
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide(NphContactRec *contacts, const NphShapeRec *shapes,
                           const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -390,7 +390,7 @@ __device__ static u32 block_exclusive_scan(u32 v, u32 *total)
 
 // Order-keeping compaction of the changed records: out[0] = count, then the
 // records, then their codes.
-__global__ void k_compact_changes(const u32 *list, const u32 *codes, u32 n, u32 *out)
+__global__ void __launch_bounds__(NPH_SCAN_BLOCK) k_compact_changes(const u32 *list, const u32 *codes, u32 n, u32 *out)
 {
     u32 base = 0U;
     for (u32 start = 0U; start < n; start += NPH_SCAN_BLOCK) {
@@ -409,7 +409,7 @@ __global__ void k_compact_changes(const u32 *list, const u32 *codes, u32 n, u32 
     }
 }
 
-__global__ void k_solver_map_fill(u32 *map, u32 slots, u32 dummy)
+__global__ void __launch_bounds__(NPH_BLOCK) k_solver_map_fill(u32 *map, u32 slots, u32 dummy)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < slots) {
@@ -417,7 +417,7 @@ __global__ void k_solver_map_fill(u32 *map, u32 slots, u32 dummy)
     }
 }
 
-__global__ void k_solver_map_set(u32 *map, const u32 *awake, u32 n)
+__global__ void __launch_bounds__(NPH_BLOCK) k_solver_map_set(u32 *map, const u32 *awake, u32 n)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -425,7 +425,7 @@ __global__ void k_solver_map_set(u32 *map, const u32 *awake, u32 n)
     }
 }
 
-__global__ void k_prepare_bodies(NphStepData d, NphStepParams sp)
+__global__ void __launch_bounds__(NPH_BLOCK) k_prepare_bodies(NphStepData d, NphStepParams sp)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d.body_count) {
@@ -436,7 +436,7 @@ __global__ void k_prepare_bodies(NphStepData d, NphStepParams sp)
     }
 }
 
-__global__ void k_group_points(NphStepData d, u32 group_count)
+__global__ void __launch_bounds__(NPH_BLOCK) k_group_points(NphStepData d, u32 group_count)
 {
     const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= group_count) {
@@ -450,7 +450,7 @@ __global__ void k_group_points(NphStepData d, u32 group_count)
     grp->point_count = p;
 }
 
-__global__ void k_group_scan(NphStepData d, u32 group_count)
+__global__ void __launch_bounds__(NPH_SCAN_BLOCK) k_group_scan(NphStepData d, u32 group_count)
 {
     u32 base = 0U;
     for (u32 start = 0U; start < group_count; start += NPH_SCAN_BLOCK) {
@@ -465,7 +465,7 @@ __global__ void k_group_scan(NphStepData d, u32 group_count)
     }
 }
 
-__global__ void k_lane_prepare(NphStepData d, NphStepParams sp, u32 group_count)
+__global__ void __launch_bounds__(NPH_BLOCK) k_lane_prepare(NphStepData d, NphStepParams sp, u32 group_count)
 {
     const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
     if (L < group_count * NPH_LANES) {
@@ -473,7 +473,7 @@ __global__ void k_lane_prepare(NphStepData d, NphStepParams sp, u32 group_count)
     }
 }
 
-__global__ void k_group_impact(NphStepData d, u32 group_count)
+__global__ void __launch_bounds__(NPH_BLOCK) k_group_impact(NphStepData d, u32 group_count)
 {
     const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < group_count) {
@@ -481,7 +481,7 @@ __global__ void k_group_impact(NphStepData d, u32 group_count)
     }
 }
 
-__global__ void k_integrate_velocities(NphStepData d, NphStepParams sp)
+__global__ void __launch_bounds__(NPH_BLOCK) k_integrate_velocities(NphStepData d, NphStepParams sp)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d.body_count) {
@@ -489,7 +489,7 @@ __global__ void k_integrate_velocities(NphStepData d, NphStepParams sp)
     }
 }
 
-__global__ void k_integrate_positions(NphStepData d, NphStepParams sp)
+__global__ void __launch_bounds__(NPH_BLOCK) k_integrate_positions(NphStepData d, NphStepParams sp)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d.body_count) {
@@ -519,7 +519,7 @@ __device__ static void run_lane(const NphStepData *d, const NphStepParams *sp, u
 }
 
 // One colour's groups [first, first + count): a thread per lane.
-__global__ void k_color_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count)
+__global__ void __launch_bounds__(NPH_BLOCK) k_color_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count)
 {
     const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < count * NPH_LANES) {
@@ -528,7 +528,7 @@ __global__ void k_color_pass(NphStepData d, NphStepParams sp, u32 pass, u32 firs
 }
 
 // The overflow: its groups (one contact each) in order, on one thread.
-__global__ void k_overflow_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count)
+__global__ void __launch_bounds__(64) k_overflow_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count)
 {
     if (blockIdx.x == 0U && threadIdx.x == 0U) {
         for (u32 g = first; g < first + count; ++g) {
@@ -537,7 +537,7 @@ __global__ void k_overflow_pass(NphStepData d, NphStepParams sp, u32 pass, u32 f
     }
 }
 
-__global__ void k_restitution_masks(NphStepData d, f32 threshold, u32 group_count)
+__global__ void __launch_bounds__(NPH_BLOCK) k_restitution_masks(NphStepData d, f32 threshold, u32 group_count)
 {
     const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < group_count) {
@@ -545,7 +545,7 @@ __global__ void k_restitution_masks(NphStepData d, f32 threshold, u32 group_coun
     }
 }
 
-__global__ void k_store(NphStepData d, u32 group_count)
+__global__ void __launch_bounds__(NPH_BLOCK) k_store(NphStepData d, u32 group_count)
 {
     const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
     if (L < group_count * NPH_LANES) {
@@ -553,7 +553,7 @@ __global__ void k_store(NphStepData d, u32 group_count)
     }
 }
 
-__global__ void k_finalize(NphStepData d, NphStepParams sp)
+__global__ void __launch_bounds__(NPH_BLOCK) k_finalize(NphStepData d, NphStepParams sp)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d.body_count) {
@@ -563,7 +563,7 @@ __global__ void k_finalize(NphStepData d, NphStepParams sp)
 
 // The grown shapes in the awake list's order, each body's shapes in its
 // list's order; their flags cleared.  out[0].shape = count.
-__global__ void k_collect_grown(NphStepData d, NphGrown *out)
+__global__ void __launch_bounds__(NPH_SCAN_BLOCK) k_collect_grown(NphStepData d, NphGrown *out)
 {
     u32 base = 0U;
     for (u32 start = 0U; start < d.body_count; start += NPH_SCAN_BLOCK) {
