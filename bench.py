@@ -139,6 +139,18 @@ def run_cpu(args, bank):
             "hash": "%016x" % w.hash()}
 
 
+def copy_calibration():
+    import torch
+    n = 256 << 20
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,10 +164,17 @@ def main():
     ap.add_argument("--phases", action="store_true",
                     help="also report host and device time per step phase (reads stats "
                          "each step, which adds a contact readback to the timed region)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device three times (the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the world step has no multi-GPU path "
-              "(worlds would shard by cost across GPUs, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; the world step has no multi-GPU path "
+              "(batched worlds would shard by cost across GPUs, one large world by slabs)",
+              file=sys.stderr)
         sys.exit(2)
     bank = S.load_hull_bank(os.path.join(ROOT, "tests", "golden", "hulls_12v.bin"))
     g = run_gpu(args, bank)

@@ -41,6 +41,20 @@ typedef struct HullEntry {
     const NphPhysicsHull *device;
 } HullEntry;
 
+// Record writes from the host (body and shape creation, velocity changes)
+// wait here and reach HBM together, in one copy and one scatter, before the
+// next device read or stage.  A slot written twice keeps its last record.
+typedef struct Staged {
+    u32 rec_words;  // record size in 4-byte words
+    u32 *recs;      // count * rec_words
+    u32 *slots;     // count
+    u32 count, cap;
+    u32 *where;     // per slot: staged index + 1, or 0
+    u32 cap_where;
+    u32 *dev;       // device copy: slots then records
+    u64 dev_words;
+} Staged;
+
 struct NphDevice {
     hipStream_t stream;
     hipEvent_t ev[4];
@@ -60,7 +74,6 @@ struct NphDevice {
     u32 *list;  // awake contacts / graph contacts (cap_list)
     u32 cap_list;
     u32 *codes; // change codes, per list entry
-    u32 *compact; // changed records then codes, 2 * cap_list + 1
     NphContactInit *inits;
     u32 cap_inits;
     NphGroup *groups;
@@ -70,13 +83,24 @@ struct NphDevice {
     f32 *points;
     u32 cap_point_rows;
     NphGrown *grown; // cap_shapes + 1 (count first)
+    u32 *grown_blocks; // per NPH_BLOCK awake bodies: grown shapes (cap_bodies / NPH_BLOCK + 1)
     // Host staging (pinned).
     u32 *h_u32;
     u32 cap_h_u32;
+    u32 *h_list;  // the awake list being collided
+    u64 cap_h_list;
+    u32 *h_codes; // change codes read back
+    u64 cap_h_codes;
+    NphContactInit *h_inits;
+    u64 cap_h_inits;
+    u32 collide_first; // entries [0, collide_first) queued by collide_begin
+    hipEvent_t cev[4];
     NphGrown *h_grown;
     u32 cap_h_grown;
     HullEntry *hulls;
     u32 hull_count, hull_cap;
+    Staged staged_bodies;
+    Staged staged_shapes;
 };
 
 const char *nph_dev_name(void) { return "hip:gfx950"; }
@@ -126,6 +150,14 @@ NphResult nph_dev_create(NphDevice **out)
             return NPH_RESULT_DEVICE_LOST;
         }
     }
+    for (u32 i = 0U; i < 4U; ++i) {
+        if (hipEventCreate(&d->cev[i]) != hipSuccess) {
+            free(d);
+            return NPH_RESULT_DEVICE_LOST;
+        }
+    }
+    d->staged_bodies.rec_words = (u32)(sizeof(NphBodyRec) / 4U);
+    d->staged_shapes.rec_words = (u32)(sizeof(NphShapeRec) / 4U);
     *out = d;
     return NPH_RESULT_OK;
 }
@@ -137,9 +169,9 @@ void nph_dev_destroy(NphDevice *d)
     }
     (void)hipStreamSynchronize(d->stream);
     void *bufs[] = {d->bodies, d->shapes, d->contacts, d->solver_of_body, d->awake_bodies,
-                    d->sbodies, d->params, d->list, d->codes, d->compact, d->inits, d->groups,
+                    d->sbodies, d->params, d->list, d->codes, d->inits, d->groups,
                     d->lane_contact, d->lane_body_a, d->lane_body_b, d->lane_flags, d->lanes,
-                    d->points, d->grown};
+                    d->points, d->grown, d->grown_blocks};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
             (void)hipFree(bufs[i]);
@@ -149,14 +181,30 @@ void nph_dev_destroy(NphDevice *d)
         (void)hipFree((void *)d->hulls[i].device);
     }
     free(d->hulls);
+    Staged *st[2] = {&d->staged_bodies, &d->staged_shapes};
+    for (u32 i = 0U; i < 2U; ++i) {
+        free(st[i]->recs);
+        free(st[i]->slots);
+        free(st[i]->where);
+        if (st[i]->dev != NULL) {
+            (void)hipFree(st[i]->dev);
+        }
+    }
     if (d->h_u32 != NULL) {
         (void)hipHostFree(d->h_u32);
+    }
+    void *pinned[] = {d->h_list, d->h_codes, d->h_inits};
+    for (u32 i = 0U; i < 3U; ++i) {
+        if (pinned[i] != NULL) {
+            (void)hipHostFree(pinned[i]);
+        }
     }
     if (d->h_grown != NULL) {
         (void)hipHostFree(d->h_grown);
     }
     for (u32 i = 0U; i < 4U; ++i) {
         (void)hipEventDestroy(d->ev[i]);
+        (void)hipEventDestroy(d->cev[i]);
     }
     (void)hipStreamDestroy(d->stream);
     free(d);
@@ -180,7 +228,9 @@ NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts)
             (r = grow_buffer((void **)&d->sbodies, 0, (u64)(c + 1U) * sizeof(NphSolverBody),
                              d->stream, 0)) != NPH_RESULT_OK ||
             (r = grow_buffer((void **)&d->params, 0, (u64)c * sizeof(NphSolverParams), d->stream,
-                             0)) != NPH_RESULT_OK) {
+                             0)) != NPH_RESULT_OK ||
+            (r = grow_buffer((void **)&d->grown_blocks, 0, (u64)(c / NPH_BLOCK + 1U) * 4U,
+                             d->stream, 0)) != NPH_RESULT_OK) {
             return r;
         }
         d->cap_bodies = c;
@@ -216,9 +266,7 @@ static NphResult reserve_list(NphDevice *d, u32 n)
     const u32 c = grow_cap(d->cap_list, n);
     NphResult r;
     if ((r = grow_buffer((void **)&d->list, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->codes, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->compact, 0, (u64)(2U * c + 1U) * 4U, d->stream, 0)) !=
-            NPH_RESULT_OK) {
+        (r = grow_buffer((void **)&d->codes, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK) {
         return r;
     }
     d->cap_list = c;
@@ -293,22 +341,126 @@ NphResult nph_dev_hull(NphDevice *d, const NphPhysicsHull *hull, const NphPhysic
     return NPH_RESULT_OK;
 }
 
+static_assert(sizeof(NphBodyRec) % 4U == 0U && sizeof(NphShapeRec) % 4U == 0U,
+              "records are copied as 4-byte words");
+
+static NphResult stage_put(Staged *st, u32 slot, const void *rec)
+{
+    if (slot >= st->cap_where) {
+        const u32 c = grow_cap(st->cap_where, slot + 1U);
+        u32 *n = (u32 *)realloc(st->where, (size_t)c * 4U);
+        if (n == NULL) {
+            return NPH_RESULT_OUT_OF_MEMORY;
+        }
+        memset(n + st->cap_where, 0, (size_t)(c - st->cap_where) * 4U);
+        st->where = n;
+        st->cap_where = c;
+    }
+    u32 i = st->where[slot];
+    if (i == 0U) {
+        if (st->count == st->cap) {
+            const u32 c = grow_cap(st->cap, st->count + 1U);
+            u32 *r = (u32 *)realloc(st->recs, (size_t)c * st->rec_words * 4U);
+            if (r == NULL) {
+                return NPH_RESULT_OUT_OF_MEMORY;
+            }
+            st->recs = r;
+            u32 *q = (u32 *)realloc(st->slots, (size_t)c * 4U);
+            if (q == NULL) {
+                return NPH_RESULT_OUT_OF_MEMORY;
+            }
+            st->slots = q;
+            st->cap = c;
+        }
+        i = ++st->count;
+        st->where[slot] = i;
+        st->slots[i - 1U] = slot;
+    }
+    memcpy(st->recs + (u64)(i - 1U) * st->rec_words, rec, (size_t)st->rec_words * 4U);
+    return NPH_RESULT_OK;
+}
+
+// Record i of the staged list to its slot, one word a thread.
+__global__ void __launch_bounds__(NPH_BLOCK) k_scatter_records(u32 *dst, const u32 *staged,
+                                                               u32 count, u32 words)
+{
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (u64)count * words) {
+        const u32 i = (u32)(t / words);
+        const u32 w = (u32)(t % words);
+        dst[(u64)staged[i] * words + w] = staged[count + (u64)i * words + w];
+    }
+}
+
+static NphResult stage_flush(NphDevice *d, Staged *st, void *dst)
+{
+    if (st->count == 0U) {
+        return NPH_RESULT_OK;
+    }
+    const u64 words = (u64)st->count * (1U + st->rec_words);
+    if (words > st->dev_words) {
+        u64 c = st->dev_words < 4096U ? 4096U : st->dev_words;
+        while (c < words) {
+            c *= 2U;
+        }
+        NphResult r = grow_buffer((void **)&st->dev, 0, c * 4U, d->stream, 0);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        st->dev_words = c;
+    }
+    HIP_TRY(hipMemcpyAsync(st->dev, st->slots, (u64)st->count * 4U, hipMemcpyHostToDevice,
+                           d->stream));
+    HIP_TRY(hipMemcpyAsync(st->dev + st->count, st->recs, (u64)st->count * st->rec_words * 4U,
+                           hipMemcpyHostToDevice, d->stream));
+    const u64 threads = (u64)st->count * st->rec_words;
+    hipLaunchKernelGGL(k_scatter_records, dim3((u32)((threads + NPH_BLOCK - 1U) / NPH_BLOCK)),
+                       dim3(NPH_BLOCK), 0, d->stream, (u32 *)dst, st->dev, st->count,
+                       st->rec_words);
+    HIP_TRY(hipGetLastError());
+    // The host lists are reused at once.
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (u32 i = 0U; i < st->count; ++i) {
+        st->where[st->slots[i]] = 0U;
+    }
+    st->count = 0U;
+    return NPH_RESULT_OK;
+}
+
+// Every device read and stage starts here.
+static NphResult flush_staged(NphDevice *d)
+{
+    NphResult r = stage_flush(d, &d->staged_bodies, d->bodies);
+    if (r == NPH_RESULT_OK) {
+        r = stage_flush(d, &d->staged_shapes, d->shapes);
+    }
+    return r;
+}
+
 NphResult nph_dev_write_body(NphDevice *d, u32 slot, const NphBodyRec *b)
 {
-    HIP_TRY(hipMemcpyAsync(&d->bodies[slot], b, sizeof *b, hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
-    return NPH_RESULT_OK;
+    if (slot >= d->cap_bodies) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    return stage_put(&d->staged_bodies, slot, b);
 }
 
 NphResult nph_dev_write_shape(NphDevice *d, u32 slot, const NphShapeRec *s)
 {
-    HIP_TRY(hipMemcpyAsync(&d->shapes[slot], s, sizeof *s, hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
-    return NPH_RESULT_OK;
+    if (slot >= d->cap_shapes) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    return stage_put(&d->staged_shapes, slot, s);
 }
 
 NphResult nph_dev_read_bodies(NphDevice *d, NphBodyRec *out, u32 count)
 {
+    {
+        const NphResult r = flush_staged(d);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
     if (count > 0U) {
         HIP_TRY(hipMemcpyAsync(out, d->bodies, (u64)count * sizeof *out, hipMemcpyDeviceToHost,
                                d->stream));
@@ -319,6 +471,12 @@ NphResult nph_dev_read_bodies(NphDevice *d, NphBodyRec *out, u32 count)
 
 NphResult nph_dev_read_shapes(NphDevice *d, NphShapeRec *out, u32 count)
 {
+    {
+        const NphResult r = flush_staged(d);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
     if (count > 0U) {
         HIP_TRY(hipMemcpyAsync(out, d->shapes, (u64)count * sizeof *out, hipMemcpyDeviceToHost,
                                d->stream));
@@ -329,6 +487,12 @@ NphResult nph_dev_read_shapes(NphDevice *d, NphShapeRec *out, u32 count)
 
 NphResult nph_dev_read_contacts(NphDevice *d, NphContactRec *out, u32 count)
 {
+    {
+        const NphResult r = flush_staged(d);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
     if (count > 0U) {
         HIP_TRY(hipMemcpyAsync(out, d->contacts, (u64)count * sizeof *out, hipMemcpyDeviceToHost,
                                d->stream));
@@ -386,27 +550,6 @@ __device__ static u32 block_exclusive_scan(u32 v, u32 *total)
     *total = s[NPH_SCAN_BLOCK - 1U];
     __syncthreads();
     return incl - v;
-}
-
-// Order-keeping compaction of the changed records: out[0] = count, then the
-// records, then their codes.
-__global__ void __launch_bounds__(NPH_SCAN_BLOCK) k_compact_changes(const u32 *list, const u32 *codes, u32 n, u32 *out)
-{
-    u32 base = 0U;
-    for (u32 start = 0U; start < n; start += NPH_SCAN_BLOCK) {
-        const u32 i = start + threadIdx.x;
-        const u32 code = i < n ? codes[i] : 0U;
-        u32 total = 0U;
-        const u32 pos = block_exclusive_scan(code != 0U ? 1U : 0U, &total);
-        if (code != 0U) {
-            out[1U + base + pos] = list[i];
-            out[1U + n + base + pos] = code;
-        }
-        base += total;
-    }
-    if (threadIdx.x == 0U) {
-        out[0] = base;
-    }
 }
 
 __global__ void __launch_bounds__(NPH_BLOCK) k_solver_map_fill(u32 *map, u32 slots, u32 dummy)
@@ -562,50 +705,125 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_finalize(NphStepData d, NphStepPa
 }
 
 // The grown shapes in the awake list's order, each body's shapes in its
-// list's order; their flags cleared.  out[0].shape = count.
-__global__ void __launch_bounds__(NPH_SCAN_BLOCK) k_collect_grown(NphStepData d, NphGrown *out)
+// list's order; their flags cleared.  Two launches: a count per block of
+// NPH_BLOCK awake bodies, then each block places its bodies' shapes after the
+// counts of the blocks before it.  out[0].shape = count.
+__device__ static u32 grown_of_body(const NphStepData *d, u32 i)
 {
-    u32 base = 0U;
-    for (u32 start = 0U; start < d.body_count; start += NPH_SCAN_BLOCK) {
-        const u32 i = start + threadIdx.x;
-        u32 mine = 0U;
-        NphBodyRec *b = 0;
-        if (i < d.body_count) {
-            b = &d.bodies[d.awake_bodies[i]];
-            if (b->enlarged != 0U) {
-                for (u32 s = b->shape_head; s != NPH_NULL32; s = d.shapes[s].next) {
-                    mine += (d.shapes[s].flags & NPH_SHAPE_ENLARGED) != 0U ? 1U : 0U;
-                }
-            }
+    u32 mine = 0U;
+    const NphBodyRec *b = &d->bodies[d->awake_bodies[i]];
+    if (b->enlarged != 0U) {
+        for (u32 s = b->shape_head; s != NPH_NULL32; s = d->shapes[s].next) {
+            mine += (d->shapes[s].flags & NPH_SHAPE_ENLARGED) != 0U ? 1U : 0U;
         }
-        u32 total = 0U;
-        const u32 pos = block_exclusive_scan(mine, &total);
-        if (mine != 0U) {
-            u32 k = 0U;
-            for (u32 s = b->shape_head; s != NPH_NULL32; s = d.shapes[s].next) {
-                NphShapeRec *sh = &d.shapes[s];
-                if ((sh->flags & NPH_SHAPE_ENLARGED) != 0U) {
-                    NphGrown *g = &out[1U + base + pos + k];
-                    g->shape = s;
-                    g->pad = 0U;
-                    g->fat_aabb = sh->fat_aabb;
-                    sh->flags &= ~NPH_SHAPE_ENLARGED;
-                    k += 1U;
-                }
-            }
-            b->enlarged = 0U;
-        }
-        base += total;
     }
+    return mine;
+}
+
+__device__ static u32 block_sum(u32 v)
+{
+    __shared__ u32 part[NPH_BLOCK / 64U];
+    const u32 lane = threadIdx.x & 63U;
+    const u32 wave = threadIdx.x >> 6U;
+    for (u32 off = 32U; off > 0U; off >>= 1U) {
+        v += __shfl_down(v, off, 64);
+    }
+    if (lane == 0U) {
+        part[wave] = v;
+    }
+    __syncthreads();
+    u32 total = 0U;
+    for (u32 w = 0U; w < NPH_BLOCK / 64U; ++w) {
+        total += part[w];
+    }
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(NPH_BLOCK) k_grown_count(NphStepData d, u32 *block_counts)
+{
+    const u32 i = blockIdx.x * NPH_BLOCK + threadIdx.x;
+    const u32 total = block_sum(i < d.body_count ? grown_of_body(&d, i) : 0U);
     if (threadIdx.x == 0U) {
-        out[0].shape = base;
+        block_counts[blockIdx.x] = total;
+    }
+}
+
+__global__ void __launch_bounds__(NPH_BLOCK) k_grown_scatter(NphStepData d,
+                                                             const u32 *block_counts, NphGrown *out)
+{
+    __shared__ u32 scan[NPH_BLOCK];
+    u32 before = 0U;
+    for (u32 j = threadIdx.x; j < blockIdx.x; j += NPH_BLOCK) {
+        before += block_counts[j];
+    }
+    const u32 base = block_sum(before);
+    const u32 i = blockIdx.x * NPH_BLOCK + threadIdx.x;
+    const u32 mine = i < d.body_count ? grown_of_body(&d, i) : 0U;
+    scan[threadIdx.x] = mine;
+    __syncthreads();
+    for (u32 off = 1U; off < NPH_BLOCK; off <<= 1U) {
+        const u32 x = threadIdx.x >= off ? scan[threadIdx.x - off] : 0U;
+        __syncthreads();
+        scan[threadIdx.x] += x;
+        __syncthreads();
+    }
+    const u32 pos = base + scan[threadIdx.x] - mine;
+    if (mine != 0U) {
+        NphBodyRec *b = &d.bodies[d.awake_bodies[i]];
+        u32 k = 0U;
+        for (u32 s = b->shape_head; s != NPH_NULL32; s = d.shapes[s].next) {
+            NphShapeRec *sh = &d.shapes[s];
+            if ((sh->flags & NPH_SHAPE_ENLARGED) != 0U) {
+                NphGrown *g = &out[1U + pos + k];
+                g->shape = s;
+                g->pad = 0U;
+                g->fat_aabb = sh->fat_aabb;
+                sh->flags &= ~NPH_SHAPE_ENLARGED;
+                k += 1U;
+            }
+        }
+        b->enlarged = 0U;
+    }
+    if (blockIdx.x == gridDim.x - 1U && threadIdx.x == NPH_BLOCK - 1U) {
+        out[0].shape = base + scan[NPH_BLOCK - 1U];
     }
 }
 
 static u32 blocks_for(u32 n) { return (n + NPH_BLOCK - 1U) / NPH_BLOCK; }
 
+// A pinned host buffer of at least `bytes`; the stream is drained before an
+// old one is freed (a queued copy may still read it).
+static NphResult reserve_pinned(NphDevice *d, void **p, u64 *cap, u64 bytes)
+{
+    if (bytes <= *cap) {
+        return NPH_RESULT_OK;
+    }
+    u64 c = *cap < 4096U ? 4096U : *cap;
+    while (c < bytes) {
+        c *= 2U;
+    }
+    if (*p != NULL) {
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        HIP_TRY(hipHostFree(*p));
+        *p = NULL;
+        *cap = 0U;
+    }
+    HIP_TRY(hipHostMalloc(p, c, hipHostMallocDefault));
+    *cap = c;
+    return NPH_RESULT_OK;
+}
+
+// Queued, not waited for: the records are staged in pinned memory, so the
+// caller's list may change at once.
 NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 count)
 {
+    {
+        const NphResult fr = flush_staged(d);
+        if (fr != NPH_RESULT_OK) {
+            return fr;
+        }
+    }
     if (count == 0U) {
         return NPH_RESULT_OK;
     }
@@ -617,54 +835,108 @@ NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 co
         }
         d->cap_inits = c;
     }
-    HIP_TRY(hipMemcpyAsync(d->inits, list, (u64)count * sizeof *list, hipMemcpyHostToDevice,
+    // The previous step's copy out of the staging buffer is done: every step
+    // ends with the stream drained.
+    NphResult r = reserve_pinned(d, (void **)&d->h_inits, &d->cap_h_inits,
+                                 (u64)count * sizeof *list);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    memcpy(d->h_inits, list, (size_t)count * sizeof *list);
+    HIP_TRY(hipMemcpyAsync(d->inits, d->h_inits, (u64)count * sizeof *list, hipMemcpyHostToDevice,
                            d->stream));
     hipLaunchKernelGGL(k_init_contacts, dim3(blocks_for(count)), dim3(NPH_BLOCK), 0, d->stream,
                        d->contacts, d->inits, count);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(d->stream));
     return NPH_RESULT_OK;
 }
 
-NphResult nph_dev_collide(NphDevice *d, const u32 *awake, u32 count, f32 step_dt, u32 *changed,
-                          u32 *codes, u32 *changed_count)
+// The narrow phase over list entries [first, count): the entries staged in
+// pinned memory, copied up, one thread each.
+static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 count, f32 step_dt,
+                               hipEvent_t e0, hipEvent_t e1)
 {
-    *changed_count = 0U;
+    if (count > d->cap_list) {
+        // Keep what an earlier range wrote (stream-ordered copies).
+        const u32 c = grow_cap(d->cap_list, count);
+        NphResult r;
+        if ((r = grow_buffer((void **)&d->list, (u64)d->cap_list * 4U, (u64)c * 4U, d->stream,
+                             1)) != NPH_RESULT_OK ||
+            (r = grow_buffer((void **)&d->codes, (u64)d->cap_list * 4U, (u64)c * 4U, d->stream,
+                             1)) != NPH_RESULT_OK) {
+            return r;
+        }
+        d->cap_list = c;
+    }
+    // Entries [0, first) went up with an earlier range: growing may drop them.
+    {
+        NphResult r = reserve_pinned(d, (void **)&d->h_list, &d->cap_h_list, (u64)count * 4U);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    const u32 n = count - first;
+    memcpy(d->h_list + first, awake + first, (size_t)n * 4U);
+    HIP_TRY(hipMemcpyAsync(d->list + first, d->h_list + first, (u64)n * 4U, hipMemcpyHostToDevice,
+                           d->stream));
+    HIP_TRY(hipEventRecord(e0, d->stream));
+    hipLaunchKernelGGL(k_collide, dim3((n + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
+                       dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes, d->bodies,
+                       d->list + first, d->codes + first, n, step_dt);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, d->stream));
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_dev_collide_begin(NphDevice *d, const u32 *awake, u32 count, f32 step_dt)
+{
+    {
+        const NphResult fr = flush_staged(d);
+        if (fr != NPH_RESULT_OK) {
+            return fr;
+        }
+    }
     d->collide_ms = 0.0;
+    d->collide_first = count;
     if (count == 0U) {
         return NPH_RESULT_OK;
     }
-    NphResult r = reserve_list(d, count);
-    if (r == NPH_RESULT_OK) {
-        r = reserve_host_u32(d, 2U * count + 1U);
+    return collide_range(d, awake, 0U, count, step_dt, d->cev[0], d->cev[1]);
+}
+
+NphResult nph_dev_collide_end(NphDevice *d, const u32 *awake, u32 count, f32 step_dt,
+                              const u32 **codes)
+{
+    *codes = NULL;
+    const u32 first = d->collide_first;
+    if (count < first) {
+        return NPH_RESULT_INVALID_ARGUMENT;
     }
-    if (r != NPH_RESULT_OK) {
-        return r;
+    if (count > first) {
+        NphResult r = collide_range(d, awake, first, count, step_dt, d->cev[2], d->cev[3]);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
     }
-    HIP_TRY(hipMemcpyAsync(d->list, awake, (u64)count * 4U, hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipEventRecord(d->ev[0], d->stream));
-    hipLaunchKernelGGL(k_collide, dim3((count + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
-                       dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes, d->bodies,
-                       d->list, d->codes, count, step_dt);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_compact_changes, dim3(1), dim3(NPH_SCAN_BLOCK), 0, d->stream, d->list,
-                       d->codes, count, d->compact);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(d->ev[1], d->stream));
-    HIP_TRY(hipMemcpyAsync(d->h_u32, d->compact, 4U, hipMemcpyDeviceToHost, d->stream));
+    if (count > 0U) {
+        NphResult r = reserve_pinned(d, (void **)&d->h_codes, &d->cap_h_codes, (u64)count * 4U);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        HIP_TRY(hipMemcpyAsync(d->h_codes, d->codes, (u64)count * 4U, hipMemcpyDeviceToHost,
+                               d->stream));
+    }
     HIP_TRY(hipStreamSynchronize(d->stream));
-    const u32 n = d->h_u32[0];
-    if (n > 0U) {
-        HIP_TRY(hipMemcpyAsync(changed, d->compact + 1U, (u64)n * 4U, hipMemcpyDeviceToHost,
-                               d->stream));
-        HIP_TRY(hipMemcpyAsync(codes, d->compact + 1U + count, (u64)n * 4U, hipMemcpyDeviceToHost,
-                               d->stream));
-        HIP_TRY(hipStreamSynchronize(d->stream));
-    }
+    f64 total = 0.0;
     float ms = 0.0F;
-    (void)hipEventElapsedTime(&ms, d->ev[0], d->ev[1]);
-    d->collide_ms = ms;
-    *changed_count = n;
+    if (first > 0U && hipEventElapsedTime(&ms, d->cev[0], d->cev[1]) == hipSuccess) {
+        total += ms;
+    }
+    if (count > first && hipEventElapsedTime(&ms, d->cev[2], d->cev[3]) == hipSuccess) {
+        total += ms;
+    }
+    d->collide_ms = total;
+    *codes = d->h_codes;
     return NPH_RESULT_OK;
 }
 
@@ -673,6 +945,12 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
 {
     *grown = NULL;
     *grown_count = 0U;
+    {
+        const NphResult fr = flush_staged(d);
+        if (fr != NPH_RESULT_OK) {
+            return fr;
+        }
+    }
     d->solve_ms = 0.0;
     const u32 n = in->awake_count;
     if (n == 0U) {
@@ -811,7 +1089,10 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
     }
 #undef PASS
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd, sp);
-    hipLaunchKernelGGL(k_collect_grown, dim3(1), dim3(NPH_SCAN_BLOCK), 0, s, sd, d->grown);
+    hipLaunchKernelGGL(k_grown_count, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
+                       d->grown_blocks);
+    hipLaunchKernelGGL(k_grown_scatter, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
+                       d->grown_blocks, d->grown);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(d->ev[1], s));
     // The count, then the records.

@@ -437,7 +437,23 @@ static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
     return NPH_RESULT_OK;
 }
 
-NphResult contacts_collide(NphPhysicsWorld *w)
+// The narrow phase of the contacts that exist when the step starts, queued
+// on the device before the host looks for new pairs: a contact's manifold
+// depends only on its own pair's state, and new contacts join the end of the
+// awake list, so the two parts together are the reference's one pass over
+// the list (contact.c:657-690).
+NphResult contacts_collide_begin(NphPhysicsWorld *w)
+{
+    NphResult r = nph_dev_init_contacts(w->dev, VB(w->inits, NphContactInit), w->inits.len);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    w->inits.len = 0U;
+    return nph_dev_collide_begin(w->dev, VB(w->awake_contacts, u32), w->awake_contacts.len,
+                                 w->step_dt);
+}
+
+NphResult contacts_collide_end(NphPhysicsWorld *w)
 {
     NphResult r = nph_dev_init_contacts(w->dev, VB(w->inits, NphContactInit), w->inits.len);
     if (r != NPH_RESULT_OK) {
@@ -445,22 +461,30 @@ NphResult contacts_collide(NphPhysicsWorld *w)
     }
     w->inits.len = 0U;
     const u32 n = w->awake_contacts.len;
-    if (n == 0U) {
-        return NPH_RESULT_OK;
+    const u32 *codes = NULL;
+    r = nph_dev_collide_end(w->dev, VB(w->awake_contacts, u32), n, w->step_dt, &codes);
+    if (r != NPH_RESULT_OK || n == 0U) {
+        return r;
     }
     if ((r = vbuf_reserve(w->alloc, &w->changed, 4U, n)) != NPH_RESULT_OK ||
         (r = vbuf_reserve(w->alloc, &w->codes, 4U, n)) != NPH_RESULT_OK) {
         return r;
     }
+    // The changed records in list order, taken before any change edits the
+    // list (contact.c:677-687).
+    const u32 *list = VB(w->awake_contacts, u32);
+    u32 *changed = VB(w->changed, u32);
+    u32 *changed_codes = VB(w->codes, u32);
     u32 m = 0U;
-    r = nph_dev_collide(w->dev, VB(w->awake_contacts, u32), n, w->step_dt, VB(w->changed, u32),
-                        VB(w->codes, u32), &m);
-    if (r != NPH_RESULT_OK) {
-        return r;
+    for (u32 i = 0U; i < n; ++i) {
+        if (codes[i] != 0U) {
+            changed[m] = list[i];
+            changed_codes[m] = codes[i];
+            m += 1U;
+        }
     }
-    // The changes in the awake list's order (contact.c:677-687).
     for (u32 i = 0U; i < m; ++i) {
-        r = apply_change(w, VB(w->changed, u32)[i], VB(w->codes, u32)[i]);
+        r = apply_change(w, changed[i], changed_codes[i]);
         if (r != NPH_RESULT_OK) {
             return r;
         }

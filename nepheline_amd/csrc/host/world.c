@@ -920,15 +920,21 @@ NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
     w->end_events.len = 0U;
     w->move_events.len = 0U;
     w->move_events_ready = 0;
+    // The existing contacts' narrow phase runs on the device while the host
+    // finds new pairs; the new contacts' follows (world.c:330-345 order of
+    // results).
     f64 t = now_ms();
+    r = contacts_collide_begin(w);
+    f64 collide_ms = now_ms() - t;
     if (r == NPH_RESULT_OK) {
+        t = now_ms();
         r = contacts_update_pairs(w);
         w->stats.pairs_ms = now_ms() - t;
     }
     if (r == NPH_RESULT_OK) {
         t = now_ms();
-        r = contacts_collide(w);
-        w->stats.collide_ms = now_ms() - t;
+        r = contacts_collide_end(w);
+        w->stats.collide_ms = collide_ms + now_ms() - t;
     }
     if (r == NPH_RESULT_OK) {
         t = now_ms();

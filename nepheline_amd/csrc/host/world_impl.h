@@ -207,7 +207,8 @@ NphResult world_wake_body(NphPhysicsWorld *w, u32 slot);
 
 // contact.c restated
 NphResult contacts_update_pairs(NphPhysicsWorld *w);
-NphResult contacts_collide(NphPhysicsWorld *w);
+NphResult contacts_collide_begin(NphPhysicsWorld *w);
+NphResult contacts_collide_end(NphPhysicsWorld *w);
 NphResult contacts_post_solve(NphPhysicsWorld *w, const NphGrown *grown, u32 count);
 NphResult move_buffer_push(NphPhysicsWorld *w, u32 shape);
 

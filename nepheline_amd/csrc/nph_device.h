@@ -61,12 +61,17 @@ NphResult nph_dev_write_shape(NphDevice *d, u32 slot, const NphShapeRec *s);
 NphResult nph_dev_read_bodies(NphDevice *d, NphBodyRec *out, u32 count);
 NphResult nph_dev_read_shapes(NphDevice *d, NphShapeRec *out, u32 count);
 NphResult nph_dev_read_contacts(NphDevice *d, NphContactRec *out, u32 count);
+// Queued behind earlier work; the list may be reused on return.
 NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 count);
-// The narrow phase over `awake` (contact records, the awake list's order):
-// manifolds updated in place; the records with a change, in list order, and
-// their change codes, into changed[] and codes[] (room for `count`).
-NphResult nph_dev_collide(NphDevice *d, const u32 *awake, u32 count, f32 step_dt, u32 *changed,
-                          u32 *codes, u32 *changed_count);
+// The narrow phase in two parts, so the host can look for new pairs while
+// the device works: collide_begin queues list entries [0, count) (the awake
+// contacts when the step starts) and returns at once; collide_end queues
+// entries [first, count) where first is begin's count (the contacts made
+// since, after init_contacts), waits for both, and points *codes at the change
+// code of every entry (0: none), valid until the next call.
+NphResult nph_dev_collide_begin(NphDevice *d, const u32 *awake, u32 count, f32 step_dt);
+NphResult nph_dev_collide_end(NphDevice *d, const u32 *awake, u32 count, f32 step_dt,
+                              const u32 **codes);
 // The solver stages and finalize; the grown proxies into *grown (the
 // device's buffer, valid until the next call), *grown_count of them.
 NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **grown,
