@@ -9,10 +9,11 @@ contact solves are the constraint-graph contacts the solver took times the
 substeps (solver.c:810-843).  The timed region is whole steps: pairs,
 narrow phase, serial bookkeeping, solver, finalize.
 
-Beside it: the roofline of the dominant kernel family (the constraint
-passes, measured with HIP events on the world's stream), and the
-reference's own CPU step (oracle/_ref, its job system on every host core)
-on a bounded sample of the same scene.
+Beside it: the roofline of the dominant kernel, the colour pass (one launch
+per colour per constraint pass; HIP events around each launch, in a window of
+steps right after the timed region so the headline stays uninstrumented),
+and the reference's own CPU step (oracle/_ref, its job system on up to 16
+host threads) on a bounded sample of the same scene.
 """
 import argparse
 import ctypes as C
@@ -39,6 +40,9 @@ SUBSTEPS = 4
 # V = 24 (v, w) read and written per pass, X = 28 (dp, dq) read by solve and
 # relax; P points, E dynamic endpoints of the contact.
 C_BYTES, K_BYTES, M_BYTES, V_BYTES, X_BYTES = 68, 116, 16, 24, 28
+# HBM bytes per contact-substep of the colour passes, from rocprofv3 counters
+# (FETCH_SIZE and WRITE_SIZE, scaled by a device copy of known size).
+TRAFFIC_FILE = "k_color_pass_bin10k_traffic.json"
 
 
 def contact_bytes(points, endpoints):
@@ -91,12 +95,30 @@ def run_gpu(args, bank):
             print(f"# gpu step {i + 1}/{args.steps}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     mean_bytes, graph_n = graph_mix(w)
+    # The roofline window: the next steps of the same run with every
+    # colour-pass launch bracketed by HIP events (kept out of the timed
+    # region above, whose throughput is the headline).
+    a.lib.nph_amd_time_kernels.argtypes = [C.c_void_p, C.c_bool]
+    a.lib.nph_amd_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    a.lib.nph_amd_time_kernels(w.ptr, True)
+    k_solved = 0
+    for _ in range(args.roofline_steps):
+        w.step(DT)
+        k_solved += a.lib.nph_amd_solved_contacts(w.ptr)
+    ktimes = (C.c_double * 2)()
+    a.lib.nph_amd_kernel_times(w.ptr, ktimes)
+    a.lib.nph_amd_time_kernels(w.ptr, False)
+    k_bytes, _ = graph_mix(w)
     return {
         "wall_s": wall,
         "solved": solved,
         "passes_ms": passes_ms,
         "mean_bytes": mean_bytes,
         "graph_contacts_end": graph_n,
+        "kernel_ms": ktimes[0],
+        "kernel_launches": int(ktimes[1]),
+        "kernel_solved": k_solved,
+        "kernel_bytes": k_bytes,
         "hash": "%016x" % w.hash(),
         "phases_ms": {k: round(v / max(args.steps, 1), 4) for k, v in zip(
             ("host_pairs", "host_collide", "host_solve", "dev_collide", "dev_solve", "dev_passes"),
@@ -160,6 +182,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-steps", type=int, default=30,
+                    help="steps after the timed region with per-launch kernel timing")
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--phases", action="store_true",
                     help="also report host and device time per step phase (reads stats "
@@ -179,10 +203,18 @@ def main():
     bank = S.load_hull_bank(os.path.join(ROOT, "tests", "golden", "hulls_12v.bin"))
     g = run_gpu(args, bank)
     value = g["solved"] * SUBSTEPS / g["wall_s"]
-    # Roofline of the constraint passes: their algorithmic bytes over their
-    # event-timed device time.
-    pass_bytes = g["solved"] * SUBSTEPS * g["mean_bytes"]
-    achieved = pass_bytes / (g["passes_ms"] * 1e-3) / 1e9 if g["passes_ms"] > 0 else 0.0
+    # Roofline of the dominant kernel, the colour pass: algorithmic bytes of
+    # the launches in the roofline window over their event-timed durations.
+    launches = max(g["kernel_launches"], 1)
+    k_cs = g["kernel_solved"] * SUBSTEPS
+    bytes_per_launch = k_cs * g["kernel_bytes"] / launches
+    avg_launch_s = g["kernel_ms"] * 1e-3 / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        traffic = t["hbm_bytes_per_contact_substep"] * k_cs / launches
     cpu = None if args.no_cpu else run_cpu(args, bank)
     line = {
         "metric": "contact solves/sec (contacts x substeps/s) plain vs learned warm start, "
@@ -204,10 +236,14 @@ def main():
                    "bodies": args.bodies, "substeps": SUBSTEPS, "seed": args.seed,
                    "sleep": False, "continuous": False},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "constraint passes (k_color_pass + k_overflow_pass + "
-                               "integration), event-timed",
-                     "bytes_per_contact_substep": g["mean_bytes"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_color_pass (one colour of one constraint pass), HIP events "
+                               "around each launch over %d steps after the timed region"
+                               % args.roofline_steps,
+                     "launches": g["kernel_launches"],
+                     "avg_launch_us": avg_launch_s * 1e6,
+                     "bytes_per_launch": bytes_per_launch,
+                     "bytes_per_contact_substep": g["kernel_bytes"]},
         "cpu_baseline": None if cpu is None else {
             "value": cpu["solved"] * SUBSTEPS / cpu["wall_s"],
             "unit": "contact-substeps/s",
