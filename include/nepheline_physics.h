@@ -433,6 +433,158 @@ NphPhysicsBody nph_physics_shape_body(const NphPhysicsWorld *world, NphPhysicsSh
 NphAABB nph_physics_shape_aabb(const NphPhysicsWorld *world, NphPhysicsShape shape);
 
 // ---------------------------------------------------------------------------
+// Joints (joint.h:51-272).  Ball, hinge and fixed joints are solved; the
+// slider, distance and 6-DOF types are declared for binding compatibility and
+// rejected by nph_physics_joint_create with NPH_RESULT_UNSUPPORTED.
+// ---------------------------------------------------------------------------
+typedef struct NphPhysicsJoint {
+    u32 value;
+} NphPhysicsJoint; // joint.h:51-53
+
+typedef enum NphPhysicsJointType {
+    NPH_PHYSICS_JOINT_BALL = 0,
+    NPH_PHYSICS_JOINT_HINGE = 1,
+    NPH_PHYSICS_JOINT_FIXED = 2,
+    NPH_PHYSICS_JOINT_SLIDER = 3,
+    NPH_PHYSICS_JOINT_DISTANCE = 4,
+    NPH_PHYSICS_JOINT_D6 = 5,
+    NPH_PHYSICS_JOINT_TYPE_COUNT = 6,
+} NphPhysicsJointType; // joint.h:62-70
+
+typedef struct NphPhysicsBallJoint {
+    b32 enable_swing_limit;
+    f32 swing_y;
+    f32 swing_z;
+    b32 enable_twist_limit;
+    f32 twist_lower;
+    f32 twist_upper;
+    f32 friction_torque;
+} NphPhysicsBallJoint; // joint.h:73-86
+
+typedef struct NphPhysicsHingeJoint {
+    b32 enable_limit;
+    f32 lower;
+    f32 upper;
+    b32 enable_motor;
+    f32 motor_speed;
+    f32 max_motor_torque;
+    b32 enable_spring;
+    f32 spring_hertz;
+    f32 spring_damping_ratio;
+    f32 spring_target;
+} NphPhysicsHingeJoint; // joint.h:90-104
+
+typedef struct NphPhysicsSliderJoint {
+    b32 enable_limit;
+    f32 lower;
+    f32 upper;
+    b32 enable_motor;
+    f32 motor_speed;
+    f32 max_motor_force;
+    b32 enable_spring;
+    f32 spring_hertz;
+    f32 spring_damping_ratio;
+    f32 spring_target;
+} NphPhysicsSliderJoint; // joint.h:108-122
+
+typedef struct NphPhysicsDistanceJoint {
+    f32 length;
+    b32 enable_limit;
+    f32 min_length;
+    f32 max_length;
+    b32 enable_spring;
+    f32 spring_hertz;
+    f32 spring_damping_ratio;
+    b32 enable_motor;
+    f32 motor_speed;
+    f32 max_motor_force;
+} NphPhysicsDistanceJoint; // joint.h:128-141
+
+typedef enum NphPhysicsAxisMotion {
+    NPH_PHYSICS_AXIS_LOCKED = 0,
+    NPH_PHYSICS_AXIS_LIMITED = 1,
+    NPH_PHYSICS_AXIS_FREE = 2,
+} NphPhysicsAxisMotion;
+
+typedef struct NphPhysicsD6Axis {
+    NphPhysicsAxisMotion motion;
+    f32 lower;
+    f32 upper;
+    b32 enable_motor;
+    f32 motor_speed;
+    f32 max_motor_force;
+    b32 enable_spring;
+    f32 spring_hertz;
+    f32 spring_damping_ratio;
+    f32 spring_target;
+} NphPhysicsD6Axis; // joint.h:152-163
+
+enum {
+    NPH_PHYSICS_D6_LINEAR_X = 0,
+    NPH_PHYSICS_D6_LINEAR_Y = 1,
+    NPH_PHYSICS_D6_LINEAR_Z = 2,
+    NPH_PHYSICS_D6_TWIST = 3,
+    NPH_PHYSICS_D6_SWING_Y = 4,
+    NPH_PHYSICS_D6_SWING_Z = 5,
+    NPH_PHYSICS_D6_AXES = 6,
+};
+
+typedef struct NphPhysicsD6Joint {
+    NphPhysicsD6Axis axes[NPH_PHYSICS_D6_AXES];
+} NphPhysicsD6Joint;
+
+typedef struct NphPhysicsJointDesc {
+    NphPhysicsJointType type;
+    NphPhysicsBody body_a;
+    NphPhysicsBody body_b;
+    NphPhysicsPose frame_a; // in body A's frame (its origin)
+    NphPhysicsPose frame_b;
+    b32 collide_connected;
+    u64 user_data;
+    union {
+        NphPhysicsBallJoint ball;
+        NphPhysicsHingeJoint hinge;
+        NphPhysicsSliderJoint slider;
+        NphPhysicsDistanceJoint distance;
+        NphPhysicsD6Joint d6;
+    };
+} NphPhysicsJointDesc; // joint.h:180-195
+
+NphPhysicsJointDesc nph_physics_joint_desc_default(NphPhysicsJointType type, NphPhysicsBody a,
+                                                   NphPhysicsBody b);              // joint.c:37-81
+void nph_physics_joint_frames_at(const NphPhysicsWorld *world, NphPhysicsBody a, NphPhysicsBody b,
+                                 NphDVec3 anchor, NphVec3 axis, NphPhysicsPose *frame_a,
+                                 NphPhysicsPose *frame_b);                         // joint.c:83-107
+NphResult nph_physics_joint_create(NphPhysicsWorld *world, const NphPhysicsJointDesc *desc,
+                                   NphPhysicsJoint *out);                          // joint.c:357-443
+void nph_physics_joint_destroy(NphPhysicsWorld *world, NphPhysicsJoint joint);     // joint.c:466-473
+bool nph_physics_joint_is_valid(const NphPhysicsWorld *world, NphPhysicsJoint joint);
+NphPhysicsJointDesc nph_physics_joint_desc(const NphPhysicsWorld *world,
+                                           NphPhysicsJoint joint);                 // joint.c:475-509
+void nph_physics_joint_set_ball(NphPhysicsWorld *world, NphPhysicsJoint joint,
+                                const NphPhysicsBallJoint *ball);                  // joint.c:517-531
+void nph_physics_joint_set_hinge(NphPhysicsWorld *world, NphPhysicsJoint joint,
+                                 const NphPhysicsHingeJoint *hinge);               // joint.c:533-547
+void nph_physics_joint_set_slider(NphPhysicsWorld *world, NphPhysicsJoint joint,
+                                  const NphPhysicsSliderJoint *slider);
+void nph_physics_joint_set_distance(NphPhysicsWorld *world, NphPhysicsJoint joint,
+                                    const NphPhysicsDistanceJoint *distance);
+void nph_physics_joint_set_d6(NphPhysicsWorld *world, NphPhysicsJoint joint,
+                              const NphPhysicsD6Joint *d6);
+f32 nph_physics_joint_hinge_angle(const NphPhysicsWorld *world, NphPhysicsJoint joint); // :627-631
+f32 nph_physics_joint_hinge_speed(const NphPhysicsWorld *world, NphPhysicsJoint joint); // :633-644
+f32 nph_physics_joint_slider_position(const NphPhysicsWorld *world, NphPhysicsJoint joint);
+f32 nph_physics_joint_slider_speed(const NphPhysicsWorld *world, NphPhysicsJoint joint);
+f32 nph_physics_joint_length(const NphPhysicsWorld *world, NphPhysicsJoint joint);     // :621-625
+void nph_physics_joint_reaction(const NphPhysicsWorld *world, NphPhysicsJoint joint,
+                                NphVec3 *force, NphVec3 *torque);                  // :646-665
+u64 nph_physics_joint_user_data(const NphPhysicsWorld *world, NphPhysicsJoint joint);
+void nph_physics_joint_set_user_data(NphPhysicsWorld *world, NphPhysicsJoint joint,
+                                     u64 user_data);
+u32 nph_physics_body_joints(const NphPhysicsWorld *world, NphPhysicsBody body,
+                            NphPhysicsJoint *out, u32 capacity);                   // joint.c:675-693
+
+// ---------------------------------------------------------------------------
 // EXTENSION: contact readback.  The reference exposes contacts only through
 // events, stats and debug drawing; its tests read the contact records
 // white-box (test_world.c:123-124).  This copies every live contact record
@@ -485,6 +637,14 @@ u32 nph_amd_solved_contacts(const NphPhysicsWorld *world);
 // events on the world's stream: [0] narrow phase, [1] solver and finalize,
 // [2] the constraint passes alone (warm start, solve, relax, restitution).
 void nph_amd_device_times(const NphPhysicsWorld *world, f64 out[3]);
+
+// EXTENSION: per-launch timing of the constraint-pass kernel (one launch per
+// colour per pass), for measurement. Enabling (or disabling) clears the
+// totals; while enabled each launch is bracketed by two HIP events on the
+// world's stream. out[0] = device milliseconds summed over the timed
+// launches, out[1] = how many.
+void nph_amd_time_kernels(NphPhysicsWorld *world, bool enable);
+void nph_amd_kernel_times(const NphPhysicsWorld *world, f64 out[2]);
 
 #ifdef __cplusplus
 }

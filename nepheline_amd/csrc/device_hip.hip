@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nph_device.h"
+#include "nph_joint.h"
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -94,6 +95,12 @@ struct NphDevice {
     NphContactInit *h_inits;
     u64 cap_h_inits;
     u32 collide_first; // entries [0, collide_first) queued by collide_begin
+    // Per-launch timing of the colour passes (off unless asked for).
+    int time_kernels;
+    hipEvent_t *kev;
+    u32 kev_pairs;
+    f64 kernel_ms;
+    u64 kernel_launches;
     hipEvent_t cev[4];
     NphGrown *h_grown;
     u32 cap_h_grown;
@@ -101,6 +108,14 @@ struct NphDevice {
     u32 hull_count, hull_cap;
     Staged staged_bodies;
     Staged staged_shapes;
+    // Joints by slot: host-written configuration, device-written state.
+    NphJointConf *joint_conf;
+    NphJointState *joint_state;
+    u32 cap_joints;
+    Staged staged_jconf;
+    Staged staged_jstate;
+    u32 *step_joints; // the graph's joints for a step
+    u32 cap_step_joints;
 };
 
 const char *nph_dev_name(void) { return "hip:gfx950"; }
@@ -158,6 +173,8 @@ NphResult nph_dev_create(NphDevice **out)
     }
     d->staged_bodies.rec_words = (u32)(sizeof(NphBodyRec) / 4U);
     d->staged_shapes.rec_words = (u32)(sizeof(NphShapeRec) / 4U);
+    d->staged_jconf.rec_words = (u32)(sizeof(NphJointConf) / 4U);
+    d->staged_jstate.rec_words = (u32)(sizeof(NphJointState) / 4U);
     *out = d;
     return NPH_RESULT_OK;
 }
@@ -171,7 +188,7 @@ void nph_dev_destroy(NphDevice *d)
     void *bufs[] = {d->bodies, d->shapes, d->contacts, d->solver_of_body, d->awake_bodies,
                     d->sbodies, d->params, d->list, d->codes, d->inits, d->groups,
                     d->lane_contact, d->lane_body_a, d->lane_body_b, d->lane_flags, d->lanes,
-                    d->points, d->grown, d->grown_blocks};
+                    d->points, d->grown, d->grown_blocks, d->joint_conf, d->joint_state, d->step_joints};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
             (void)hipFree(bufs[i]);
@@ -181,8 +198,8 @@ void nph_dev_destroy(NphDevice *d)
         (void)hipFree((void *)d->hulls[i].device);
     }
     free(d->hulls);
-    Staged *st[2] = {&d->staged_bodies, &d->staged_shapes};
-    for (u32 i = 0U; i < 2U; ++i) {
+    Staged *st[4] = {&d->staged_bodies, &d->staged_shapes, &d->staged_jconf, &d->staged_jstate};
+    for (u32 i = 0U; i < 4U; ++i) {
         free(st[i]->recs);
         free(st[i]->slots);
         free(st[i]->where);
@@ -206,6 +223,10 @@ void nph_dev_destroy(NphDevice *d)
         (void)hipEventDestroy(d->ev[i]);
         (void)hipEventDestroy(d->cev[i]);
     }
+    for (u32 i = 0U; i < 2U * d->kev_pairs; ++i) {
+        (void)hipEventDestroy(d->kev[i]);
+    }
+    free(d->kev);
     (void)hipStreamDestroy(d->stream);
     free(d);
 }
@@ -213,6 +234,19 @@ void nph_dev_destroy(NphDevice *d)
 f64 nph_dev_last_solve_ms(const NphDevice *d) { return d->solve_ms; }
 f64 nph_dev_last_collide_ms(const NphDevice *d) { return d->collide_ms; }
 f64 nph_dev_last_passes_ms(const NphDevice *d) { return d->passes_ms; }
+
+void nph_dev_time_kernels(NphDevice *d, int enable)
+{
+    d->time_kernels = enable;
+    d->kernel_ms = 0.0;
+    d->kernel_launches = 0U;
+}
+
+void nph_dev_kernel_times(const NphDevice *d, f64 out[2])
+{
+    out[0] = d->kernel_ms;
+    out[1] = (f64)d->kernel_launches;
+}
 
 NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts)
 {
@@ -341,7 +375,8 @@ NphResult nph_dev_hull(NphDevice *d, const NphPhysicsHull *hull, const NphPhysic
     return NPH_RESULT_OK;
 }
 
-static_assert(sizeof(NphBodyRec) % 4U == 0U && sizeof(NphShapeRec) % 4U == 0U,
+static_assert(sizeof(NphBodyRec) % 4U == 0U && sizeof(NphShapeRec) % 4U == 0U &&
+                  sizeof(NphJointConf) % 4U == 0U && sizeof(NphJointState) % 4U == 0U,
               "records are copied as 4-byte words");
 
 static NphResult stage_put(Staged *st, u32 slot, const void *rec)
@@ -434,7 +469,62 @@ static NphResult flush_staged(NphDevice *d)
     if (r == NPH_RESULT_OK) {
         r = stage_flush(d, &d->staged_shapes, d->shapes);
     }
+    if (r == NPH_RESULT_OK) {
+        r = stage_flush(d, &d->staged_jconf, d->joint_conf);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = stage_flush(d, &d->staged_jstate, d->joint_state);
+    }
     return r;
+}
+
+NphResult nph_dev_reserve_joints(NphDevice *d, u32 joints)
+{
+    if (joints <= d->cap_joints) {
+        return NPH_RESULT_OK;
+    }
+    const u32 c = grow_cap(d->cap_joints, joints);
+    NphResult r;
+    if ((r = grow_buffer((void **)&d->joint_conf, (u64)d->cap_joints * sizeof(NphJointConf),
+                         (u64)c * sizeof(NphJointConf), d->stream, 1)) != NPH_RESULT_OK ||
+        (r = grow_buffer((void **)&d->joint_state, (u64)d->cap_joints * sizeof(NphJointState),
+                         (u64)c * sizeof(NphJointState), d->stream, 1)) != NPH_RESULT_OK) {
+        return r;
+    }
+    d->cap_joints = c;
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_dev_write_joint(NphDevice *d, u32 slot, const NphJointConf *conf, int reset)
+{
+    if (slot >= d->cap_joints) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    NphResult r = NPH_RESULT_OK;
+    if (conf != NULL) {
+        r = stage_put(&d->staged_jconf, slot, conf);
+    }
+    if (r == NPH_RESULT_OK && reset) {
+        NphJointState zero;
+        memset(&zero, 0, sizeof zero);
+        r = stage_put(&d->staged_jstate, slot, &zero);
+    }
+    return r;
+}
+
+NphResult nph_dev_read_joint(NphDevice *d, u32 slot, NphJointState *out)
+{
+    if (slot >= d->cap_joints) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    NphResult r = flush_staged(d);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    HIP_TRY(hipMemcpyAsync(out, &d->joint_state[slot], sizeof *out, hipMemcpyDeviceToHost,
+                           d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return NPH_RESULT_OK;
 }
 
 NphResult nph_dev_write_body(NphDevice *d, u32 slot, const NphBodyRec *b)
@@ -661,19 +751,74 @@ __device__ static void run_lane(const NphStepData *d, const NphStepParams *sp, u
     }
 }
 
-// One colour's groups [first, first + count): a thread per lane.
-__global__ void __launch_bounds__(NPH_BLOCK) k_color_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count)
+// A joint's pass: warm start, or the soft (solve) or rigid (relax) rows.
+// The joint's state is held in registers for the pass.
+__device__ static void run_joint(const NphStepData *d, const NphStepParams *sp, u32 pass, u32 slot)
 {
-    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < count * NPH_LANES) {
-        run_lane(&d, &sp, pass, first + t / NPH_LANES, t % NPH_LANES);
+    const NphJointConf cf = d->joint_conf[slot];
+    NphJointState st = d->joint_state[slot];
+    if (pass == PASS_WARM) {
+        joint_warm_start(d->sbodies, &cf, &st);
+    } else {
+        NphJointStep step;
+        step.h = sp->h;
+        step.inv_h = sp->inv_h;
+        step.soft.bias = sp->joint_bias;
+        step.soft.mass = sp->joint_mass;
+        step.soft.impulse = sp->joint_impulse;
+        joint_solve(d->sbodies, &step, &cf, &st, pass == PASS_SOLVE);
+    }
+    NphJointState *out = &d->joint_state[slot];
+    out->linear_impulse = st.linear_impulse;
+    out->angular_impulse = st.angular_impulse;
+    out->friction_impulse = st.friction_impulse;
+    out->lower_impulse = st.lower_impulse;
+    out->upper_impulse = st.upper_impulse;
+    out->swing_impulse = st.swing_impulse;
+    out->motor_impulse = st.motor_impulse;
+    out->spring_impulse = st.spring_impulse;
+}
+
+// The step's joints prepared (solver.c:1036, joint_solver.c:726-824).
+__global__ void __launch_bounds__(NPH_BLOCK) k_joint_prepare(NphStepData d, NphStepParams sp)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.joint_count) {
+        const u32 slot = d.step_joints[i];
+        const NphJointConf *cf = &d.joint_conf[slot];
+        joint_prepare(cf, &d.joint_state[slot], &d.bodies[cf->body_a], &d.bodies[cf->body_b],
+                      d.solver_of_body[cf->body_a], d.solver_of_body[cf->body_b], d.params, sp.h);
     }
 }
 
-// The overflow: its groups (one contact each) in order, on one thread.
-__global__ void __launch_bounds__(64) k_overflow_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count)
+// One colour's joints [jfirst, jfirst + jcount) and groups [first, first +
+// count): the joints take the first threads (rounded up to whole waves),
+// the groups' lanes the rest. A colour's constraints share no dynamic body,
+// so their order inside the launch is free.
+__global__ void __launch_bounds__(NPH_BLOCK) k_color_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count, u32 jfirst, u32 jcount)
+{
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 jpad = (jcount + 63U) & ~63U;
+    if (t < jpad) {
+        if (t < jcount) {
+            run_joint(&d, &sp, pass, d.step_joints[jfirst + t]);
+        }
+        return;
+    }
+    const u32 u = t - jpad;
+    if (u < count * NPH_LANES) {
+        run_lane(&d, &sp, pass, first + u / NPH_LANES, u % NPH_LANES);
+    }
+}
+
+// The overflow: its joints, then its groups (one contact each), in order, on
+// one thread (solver.c:771-779).
+__global__ void __launch_bounds__(64) k_overflow_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count, u32 jfirst, u32 jcount)
 {
     if (blockIdx.x == 0U && threadIdx.x == 0U) {
+        for (u32 j = jfirst; j < jfirst + jcount; ++j) {
+            run_joint(&d, &sp, pass, d.step_joints[j]);
+        }
         for (u32 g = first; g < first + count; ++g) {
             run_lane(&d, &sp, pass, g, 0U);
         }
@@ -969,6 +1114,22 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
         groups += count[c];
         graph += in->color_count[c];
     }
+    u32 jfirst[NPH_GRAPH_COLORS + 1U];
+    u32 jcount[NPH_GRAPH_COLORS + 1U];
+    u32 joints = 0U;
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        jfirst[c] = joints;
+        jcount[c] = in->joint_color_count[c];
+        joints += jcount[c];
+    }
+    if (joints > d->cap_step_joints) {
+        const u32 c = grow_cap(d->cap_step_joints, joints);
+        NphResult gr = grow_buffer((void **)&d->step_joints, 0, (u64)c * 4U, d->stream, 0);
+        if (gr != NPH_RESULT_OK) {
+            return gr;
+        }
+        d->cap_step_joints = c;
+    }
     NphResult r = reserve_groups(d, groups > 0U ? groups : 1U);
     if (r == NPH_RESULT_OK) {
         r = reserve_host_u32(d, groups * (NPH_LANES + 1U) + 1U);
@@ -1012,6 +1173,10 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
     }
     HIP_TRY(hipMemcpyAsync(d->awake_bodies, in->awake_bodies, (u64)n * 4U, hipMemcpyHostToDevice,
                            d->stream));
+    if (joints > 0U) {
+        HIP_TRY(hipMemcpyAsync(d->step_joints, in->graph_joints, (u64)joints * 4U,
+                               hipMemcpyHostToDevice, d->stream));
+    }
     HIP_TRY(hipStreamSynchronize(d->stream)); // the host buffers are reused below
     free(gtab);
 
@@ -1034,6 +1199,10 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
     sd.lane_stride = d->cap_groups * NPH_LANES;
     sd.point_stride = d->cap_point_rows * NPH_LANES;
     sd.body_count = n;
+    sd.joint_conf = d->joint_conf;
+    sd.joint_state = d->joint_state;
+    sd.step_joints = d->step_joints;
+    sd.joint_count = joints;
     const NphStepParams sp = in->params;
     hipStream_t s = d->stream;
 
@@ -1043,6 +1212,10 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
     hipLaunchKernelGGL(k_solver_map_set, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s,
                        d->solver_of_body, d->awake_bodies, n);
     hipLaunchKernelGGL(k_prepare_bodies, dim3(blocks_for(n + 1U)), dim3(NPH_BLOCK), 0, s, sd, sp);
+    if (joints > 0U) {
+        hipLaunchKernelGGL(k_joint_prepare, dim3(blocks_for(joints)), dim3(NPH_BLOCK), 0, s, sd,
+                           sp);
+    }
     if (groups > 0U) {
         hipLaunchKernelGGL(k_group_points, dim3(blocks_for(groups)), dim3(NPH_BLOCK), 0, s, sd,
                            groups);
@@ -1054,19 +1227,49 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
     }
     HIP_TRY(hipGetLastError());
     // One constraint pass: the overflow first, then colours 0..23 (solver.c:774-789).
+    u32 timed = 0U; // colour-pass launches bracketed by events this step
 #define PASS(kind)                                                                                 \
     do {                                                                                           \
-        if (count[NPH_GRAPH_OVERFLOW] > 0U) {                                                      \
+        const int joints_ = (kind) != PASS_RESTITUTION;                                            \
+        const u32 ojc_ = joints_ ? jcount[NPH_GRAPH_OVERFLOW] : 0U;                                \
+        if (count[NPH_GRAPH_OVERFLOW] > 0U || ojc_ > 0U) {                                         \
             hipLaunchKernelGGL(k_overflow_pass, dim3(1), dim3(64), 0, s, sd, sp, (u32)(kind),      \
-                               first[NPH_GRAPH_OVERFLOW], count[NPH_GRAPH_OVERFLOW]);               \
+                               first[NPH_GRAPH_OVERFLOW], count[NPH_GRAPH_OVERFLOW],                \
+                               jfirst[NPH_GRAPH_OVERFLOW], ojc_);                                   \
         }                                                                                          \
         for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {                                              \
-            if (count[c] > 0U) {                                                                   \
-                hipLaunchKernelGGL(k_color_pass, dim3(blocks_for(count[c] * NPH_LANES)),           \
-                                   dim3(NPH_BLOCK), 0, s, sd, sp, (u32)(kind), first[c], count[c]); \
+            const u32 jc_ = joints_ ? jcount[c] : 0U;                                              \
+            if (count[c] > 0U || jc_ > 0U) {                                                       \
+                const int ev_ = d->time_kernels && timed < d->kev_pairs;                           \
+                if (ev_) {                                                                         \
+                    HIP_TRY(hipEventRecord(d->kev[2U * timed], s));                                \
+                }                                                                                  \
+                hipLaunchKernelGGL(k_color_pass,                                                   \
+                                   dim3(blocks_for(((jc_ + 63U) & ~63U) + count[c] * NPH_LANES)),  \
+                                   dim3(NPH_BLOCK), 0, s, sd, sp, (u32)(kind), first[c], count[c],  \
+                                   jfirst[c], jc_);                                                \
+                if (ev_) {                                                                         \
+                    HIP_TRY(hipEventRecord(d->kev[2U * timed + 1U], s));                           \
+                    timed += 1U;                                                                   \
+                }                                                                                  \
             }                                                                                      \
         }                                                                                          \
     } while (0)
+    if (d->time_kernels) {
+        // Enough event pairs for every colour of every pass of this step.
+        const u32 need = (3U * sp.substeps + 1U) * NPH_GRAPH_COLORS;
+        while (d->kev_pairs < need) {
+            hipEvent_t *n = (hipEvent_t *)realloc(d->kev, (size_t)(2U * need) * sizeof *n);
+            if (n == NULL) {
+                return NPH_RESULT_OUT_OF_MEMORY;
+            }
+            d->kev = n;
+            for (u32 i = 2U * d->kev_pairs; i < 2U * need; ++i) {
+                HIP_TRY(hipEventCreate(&d->kev[i]));
+            }
+            d->kev_pairs = need;
+        }
+    }
     HIP_TRY(hipEventRecord(d->ev[2], s));
     for (u32 step = 0U; step < sp.substeps; ++step) {
         hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
@@ -1114,6 +1317,12 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
         HIP_TRY(hipMemcpyAsync(d->h_grown, d->grown, (u64)(m + 1U) * sizeof(NphGrown),
                                hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+    }
+    for (u32 i = 0U; i < timed; ++i) {
+        float kms = 0.0F;
+        HIP_TRY(hipEventElapsedTime(&kms, d->kev[2U * i], d->kev[2U * i + 1U]));
+        d->kernel_ms += kms;
+        d->kernel_launches += 1U;
     }
     float ms = 0.0F;
     (void)hipEventElapsedTime(&ms, d->ev[0], d->ev[1]);

@@ -218,7 +218,7 @@ static void bit_set(NphVecBuf *bits, u32 i, int on)
     }
 }
 
-static u32 graph_pick_color(NphPhysicsWorld *w, u32 body_a, u32 body_b)
+u32 graph_pick_color(NphPhysicsWorld *w, u32 body_a, u32 body_b)
 {
     const int dyn_a = body_at(w, body_a)->type == NPH_PHYSICS_BODY_DYNAMIC;
     const int dyn_b = body_at(w, body_b)->type == NPH_PHYSICS_BODY_DYNAMIC;
@@ -239,7 +239,7 @@ static u32 graph_pick_color(NphPhysicsWorld *w, u32 body_a, u32 body_b)
     return NPH_GRAPH_OVERFLOW;
 }
 
-static void graph_set_bodies(NphPhysicsWorld *w, u32 color, u32 body_a, u32 body_b, int set)
+void graph_set_bodies(NphPhysicsWorld *w, u32 color, u32 body_a, u32 body_b, int set)
 {
     if (color == NPH_GRAPH_OVERFLOW) {
         return;
@@ -400,6 +400,13 @@ static void contact_destroy(NphPhysicsWorld *w, u32 contact, int events)
     (void)vbuf_push(w->alloc, &w->free_contacts, 4U, &contact); // reserved when made
 }
 
+void contacts_destroy(NphPhysicsWorld *w, u32 contact, int events)
+{
+    contact_destroy(w, contact, events);
+}
+
+HContact *contacts_at(NphPhysicsWorld *w, u32 contact) { return contact_at(w, contact); }
+
 // contact.c:591-655 (no islands: the world runs without sleep)
 static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
 {
@@ -537,6 +544,10 @@ static int pair_callback(u32 proxy, u64 user_data, void *context)
         return 1;
     }
     if (!filter_should_collide(s->filter, o->filter)) {
+        return 1;
+    }
+    if (bs->joint_count != 0U && bo->joint_count != 0U &&
+        !joints_allow_contact(w, s->body, o->body)) {
         return 1;
     }
     if (map_find(&w->pairs, pair_key(q->shape, other)) != NPH_NULL32) {

@@ -297,6 +297,7 @@ void nph_physics_world_destroy(NphPhysicsWorld *w)
         vbuf_free(a, &w->color_contacts[c], 4U);
     }
     vbuf_free(a, &w->graph, 4U);
+    joints_free(w);
     vbuf_free(a, &w->changed, 4U);
     vbuf_free(a, &w->codes, 4U);
     vbuf_free(a, &w->begin_events, sizeof(NphPhysicsContactBeginEvent));
@@ -328,6 +329,10 @@ void nph_amd_device_times(const NphPhysicsWorld *w, f64 out[3])
     out[1] = nph_dev_last_solve_ms(w->dev);
     out[2] = nph_dev_last_passes_ms(w->dev);
 }
+
+void nph_amd_time_kernels(NphPhysicsWorld *w, bool enable) { nph_dev_time_kernels(w->dev, enable); }
+
+void nph_amd_kernel_times(const NphPhysicsWorld *w, f64 out[2]) { nph_dev_kernel_times(w->dev, out); }
 
 // ---------------------------------------------------------------------------
 // Bodies (body.c:12-130, :392-448)
@@ -492,6 +497,7 @@ NphResult nph_physics_body_create(NphPhysicsWorld *w, const NphPhysicsBodyDesc *
     hb->shape_head = NPH_NULL32;
     b->shape_head = NPH_NULL32;
     hb->contact_head = NPH_NULL32;
+    hb->joint_head = NPH_NULL32;
     hb->awake_slot = NPH_NULL32;
     hb->user_data = desc->user_data;
     hb->handle = phys_handle(slot, generation);
@@ -880,6 +886,23 @@ static NphResult solve(NphPhysicsWorld *w, f32 dt)
     }
     in.graph_contacts = VB(w->graph, u32);
     w->solved_contacts = graph;
+    // The joints, colour by colour, the overflow last (solver.c:791-806).
+    u32 joints = 0U;
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        joints += w->color_joints[c].len;
+    }
+    r = vbuf_reserve(w->alloc, &w->step_joints, 4U, joints > 0U ? joints : 1U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    at = 0U;
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        in.joint_color_count[c] = w->color_joints[c].len;
+        memcpy(VB(w->step_joints, u32) + at, w->color_joints[c].data,
+               (size_t)in.joint_color_count[c] * 4U);
+        at += in.joint_color_count[c];
+    }
+    in.graph_joints = VB(w->step_joints, u32);
     NphStepParams *sp = &in.params;
     sp->gravity = w->desc.gravity;
     sp->dt = dt;
@@ -896,6 +919,11 @@ static NphResult solve(NphPhysicsWorld *w, f32 dt)
               &sp->contact_impulse);
     make_soft(2.0F * hertz, w->desc.contact_damping_ratio, sp->h, &sp->static_bias,
               &sp->static_mass, &sp->static_impulse);
+    // Joints at most a quarter of the substep rate stiff (solver.c:998-1005).
+    const f32 joint_hertz = w->desc.joint_hertz > 0.0F ? w->desc.joint_hertz : 0.25F * sp->inv_h;
+    make_soft(nph_minf(joint_hertz, 0.25F * sp->inv_h), w->desc.joint_damping_ratio, sp->h,
+              &sp->joint_bias, &sp->joint_mass, &sp->joint_impulse);
+    w->last_substep = sp->h;
     const NphGrown *grown = NULL;
     u32 grown_count = 0U;
     r = nph_dev_solve(w->dev, &in, &grown, &grown_count);
@@ -1015,6 +1043,7 @@ NphPhysicsStats nph_physics_world_stats(const NphPhysicsWorld *cw)
     }
     s.touching_contacts = touching;
     s.manifold_points = points;
+    s.joints = w->joint_count;
     u32 colors = 0U;
     for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
         colors += w->color_contacts[c].len > 0U ? 1U : 0U;

@@ -6,6 +6,7 @@
 #define NPH_WORLD_IMPL_H
 
 #include "../nph_device.h"
+#include "../nph_joint.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -83,6 +84,8 @@ typedef struct HBody {
     u32 shape_count;
     u32 contact_head; // contact << 1 | side
     u32 contact_count;
+    u32 joint_head; // joint << 1 | side
+    u32 joint_count;
     u32 awake_slot;
     u64 user_data;
 } HBody;
@@ -133,6 +136,23 @@ typedef struct HContact {
     u32 awake_slot;
 } HContact;
 
+// joint.c / world_internal.h:280-330, less the step's numbers (the device's).
+typedef struct HJoint {
+    u32 handle; // 0: free slot
+    u32 type;
+    u32 body_a; // slots
+    u32 body_b;
+    u32 flags; // NPH_JOINT_COLLIDE_CONNECTED
+    u32 prev[2];
+    u32 next[2];
+    u32 color;
+    u32 color_slot;
+    NphPhysicsPose frame_a;
+    NphPhysicsPose frame_b;
+    NphJointSettings s;
+    u64 user_data;
+} HJoint;
+
 typedef struct NphSlots {
     NphVecBuf generation; // u32
     NphVecBuf free;       // u32
@@ -180,6 +200,12 @@ struct NphPhysicsWorld {
     NphVecBuf color_contacts[NPH_GRAPH_COLORS + 1U]; // u32; [24] = overflow
     NphVecBuf graph; // u32, the colour lists concatenated for the device
 
+    NphSlots joint_slots;
+    NphVecBuf joints;                              // HJoint
+    NphVecBuf color_joints[NPH_GRAPH_COLORS + 1U]; // u32 joint slots; [24] = overflow
+    NphVecBuf step_joints;                         // u32, the colours' joints concatenated
+    u32 joint_count;                               // live joints
+
     NphVecBuf changed; // u32
     NphVecBuf codes;   // u32
 
@@ -191,6 +217,7 @@ struct NphPhysicsWorld {
 
     NphPhysicsStats stats;
     f32 step_dt;
+    f32 last_substep; // the last step's substep length (the joint reaction's)
     u32 solved_contacts; // the last step's graph contacts
 };
 
@@ -211,6 +238,14 @@ NphResult contacts_collide_begin(NphPhysicsWorld *w);
 NphResult contacts_collide_end(NphPhysicsWorld *w);
 NphResult contacts_post_solve(NphPhysicsWorld *w, const NphGrown *grown, u32 count);
 NphResult move_buffer_push(NphPhysicsWorld *w, u32 shape);
+u32 graph_pick_color(NphPhysicsWorld *w, u32 body_a, u32 body_b);
+void graph_set_bodies(NphPhysicsWorld *w, u32 color, u32 body_a, u32 body_b, int set);
+void contacts_destroy(NphPhysicsWorld *w, u32 contact, int events);
+HContact *contacts_at(NphPhysicsWorld *w, u32 contact);
+
+// joint.c restated
+int joints_allow_contact(const NphPhysicsWorld *w, u32 body_a, u32 body_b);
+void joints_free(NphPhysicsWorld *w);
 
 NphResult pairmap_init(NphAllocator *a, NphPairMap *m);
 void pairmap_free(NphAllocator *a, NphPairMap *m);

@@ -45,6 +45,9 @@ typedef struct NphSolveInput {
     // its own order (the reference's colour arrays).
     const u32 *graph_contacts;
     u32 color_count[NPH_GRAPH_COLORS + 1U];
+    // The graph's joints (slots) in the same colour order.
+    const u32 *graph_joints;
+    u32 joint_color_count[NPH_GRAPH_COLORS + 1U];
     NphStepParams params;
 } NphSolveInput;
 
@@ -57,6 +60,11 @@ NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts);
 // device's life; *out is its device address.
 NphResult nph_dev_hull(NphDevice *d, const NphPhysicsHull *hull, const NphPhysicsHull **out);
 NphResult nph_dev_write_body(NphDevice *d, u32 slot, const NphBodyRec *b);
+// Joint records by slot: the configuration (when conf is not NULL) and, with
+// reset, accumulated impulses from zero; staged as the body writes are.
+NphResult nph_dev_reserve_joints(NphDevice *d, u32 joints);
+NphResult nph_dev_write_joint(NphDevice *d, u32 slot, const NphJointConf *conf, int reset);
+NphResult nph_dev_read_joint(NphDevice *d, u32 slot, NphJointState *out);
 NphResult nph_dev_write_shape(NphDevice *d, u32 slot, const NphShapeRec *s);
 NphResult nph_dev_read_bodies(NphDevice *d, NphBodyRec *out, u32 count);
 NphResult nph_dev_read_shapes(NphDevice *d, NphShapeRec *out, u32 count);
@@ -83,6 +91,11 @@ f64 nph_dev_last_collide_ms(const NphDevice *d);
 // The constraint passes alone (warm start, solve, relax over every substep,
 // then restitution): the stream's events around them.
 f64 nph_dev_last_passes_ms(const NphDevice *d);
+// Per-launch event timing of the colour-pass kernel, off by default (it adds
+// two event records a launch): enabling clears the totals; out[0] = device
+// milliseconds summed over the timed launches, out[1] = their count.
+void nph_dev_time_kernels(NphDevice *d, int enable);
+void nph_dev_kernel_times(const NphDevice *d, f64 out[2]);
 
 #ifdef __cplusplus
 }

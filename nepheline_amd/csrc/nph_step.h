@@ -193,6 +193,9 @@ enum {
 
 enum { NPH_LANE_WRITE_A = 1U, NPH_LANE_WRITE_B = 2U, NPH_LANE_IMPACT = 4U };
 
+typedef struct NphJointConf NphJointConf;   // nph_joint.h
+typedef struct NphJointState NphJointState;
+
 // Everything a step's kernels touch, by device address.
 typedef struct NphStepData {
     NphBodyRec *bodies;       // by body slot
@@ -212,6 +215,10 @@ typedef struct NphStepData {
     u32 lane_stride;          // capacity in lanes
     u32 point_stride;         // capacity in point lanes (rows * 8)
     u32 body_count;           // awake bodies (the dummy's index)
+    const NphJointConf *joint_conf; // by joint slot
+    NphJointState *joint_state;     // by joint slot
+    const u32 *step_joints;         // the graph's joints, colour by colour
+    u32 joint_count;
 } NphStepData;
 
 // The world's numbers for one step (solver.c:961-1017).
@@ -225,6 +232,7 @@ typedef struct NphStepParams {
     f32 max_linear_speed;
     f32 contact_bias, contact_mass, contact_impulse; // phys_make_soft(contact)
     f32 static_bias, static_mass, static_impulse;    // phys_make_soft(2x contact)
+    f32 joint_bias, joint_mass, joint_impulse;       // phys_make_soft(joint)
     u32 substeps;
 } NphStepParams;
 

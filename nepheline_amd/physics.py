@@ -124,6 +124,62 @@ class ContactInfo(C.Structure):
                 ("points", ContactPointInfo * 4)]
 
 
+class BallJoint(C.Structure):
+    _fields_ = [("enable_swing_limit", C.c_int32), ("swing_y", C.c_float), ("swing_z", C.c_float),
+                ("enable_twist_limit", C.c_int32), ("twist_lower", C.c_float),
+                ("twist_upper", C.c_float), ("friction_torque", C.c_float)]
+
+
+class HingeJoint(C.Structure):
+    _fields_ = [("enable_limit", C.c_int32), ("lower", C.c_float), ("upper", C.c_float),
+                ("enable_motor", C.c_int32), ("motor_speed", C.c_float),
+                ("max_motor_torque", C.c_float), ("enable_spring", C.c_int32),
+                ("spring_hertz", C.c_float), ("spring_damping_ratio", C.c_float),
+                ("spring_target", C.c_float)]
+
+
+class SliderJoint(C.Structure):
+    _fields_ = [("enable_limit", C.c_int32), ("lower", C.c_float), ("upper", C.c_float),
+                ("enable_motor", C.c_int32), ("motor_speed", C.c_float),
+                ("max_motor_force", C.c_float), ("enable_spring", C.c_int32),
+                ("spring_hertz", C.c_float), ("spring_damping_ratio", C.c_float),
+                ("spring_target", C.c_float)]
+
+
+class DistanceJoint(C.Structure):
+    _fields_ = [("length", C.c_float), ("enable_limit", C.c_int32), ("min_length", C.c_float),
+                ("max_length", C.c_float), ("enable_spring", C.c_int32),
+                ("spring_hertz", C.c_float), ("spring_damping_ratio", C.c_float),
+                ("enable_motor", C.c_int32), ("motor_speed", C.c_float),
+                ("max_motor_force", C.c_float)]
+
+
+class D6Axis(C.Structure):
+    _fields_ = [("motion", C.c_int), ("lower", C.c_float), ("upper", C.c_float),
+                ("enable_motor", C.c_int32), ("motor_speed", C.c_float),
+                ("max_motor_force", C.c_float), ("enable_spring", C.c_int32),
+                ("spring_hertz", C.c_float), ("spring_damping_ratio", C.c_float),
+                ("spring_target", C.c_float)]
+
+
+class D6Joint(C.Structure):
+    _fields_ = [("axes", D6Axis * 6)]
+
+
+class _JointUnion(C.Union):
+    _fields_ = [("ball", BallJoint), ("hinge", HingeJoint), ("slider", SliderJoint),
+                ("distance", DistanceJoint), ("d6", D6Joint)]
+
+
+class JointDesc(C.Structure):
+    _anonymous_ = ("u",)
+    _fields_ = [("type", C.c_int), ("body_a", C.c_uint32), ("body_b", C.c_uint32),
+                ("frame_a", Pose), ("frame_b", Pose), ("collide_connected", C.c_int32),
+                ("user_data", C.c_uint64), ("u", _JointUnion)]
+
+
+JOINT_BALL, JOINT_HINGE, JOINT_FIXED, JOINT_SLIDER, JOINT_DISTANCE, JOINT_D6 = range(6)
+
 assert C.sizeof(Pose) == 28 and C.sizeof(Geometry) == 24
 assert C.sizeof(ContactPointInfo) == 56 and C.sizeof(ContactInfo) == 288
 
@@ -172,6 +228,10 @@ class Api:
                                            [C.c_void_p, _U32])
         self.body_mass_data = _bind(L, "nph_physics_body_mass_data", MassData,
                                     [C.c_void_p, _U32])
+        self.body_set_linear_velocity = _bind(L, "nph_physics_body_set_linear_velocity", None,
+                                              [C.c_void_p, _U32, Vec3])
+        self.body_set_angular_velocity = _bind(L, "nph_physics_body_set_angular_velocity", None,
+                                               [C.c_void_p, _U32, Vec3])
         self.shape_desc_default = _bind(L, "nph_physics_shape_desc_default", ShapeDesc,
                                         [Geometry])
         self.shape_create = _bind(L, "nph_physics_shape_create", C.c_int,
@@ -179,6 +239,19 @@ class Api:
         self.shape_aabb = _bind(L, "nph_physics_shape_aabb", AABB, [C.c_void_p, _U32])
         self.geometry_mass = _bind(L, "nph_physics_geometry_mass", MassData,
                                    [C.POINTER(Geometry), C.c_float])
+        self.joint_desc_default = _bind(L, "nph_physics_joint_desc_default", JointDesc,
+                                        [C.c_int, _U32, _U32])
+        self.joint_frames_at = _bind(L, "nph_physics_joint_frames_at", None,
+                                     [C.c_void_p, _U32, _U32, DVec3, Vec3, C.POINTER(Pose),
+                                      C.POINTER(Pose)])
+        self.joint_create = _bind(L, "nph_physics_joint_create", C.c_int,
+                                  [C.c_void_p, C.POINTER(JointDesc), C.POINTER(_U32)])
+        self.joint_destroy = _bind(L, "nph_physics_joint_destroy", None, [C.c_void_p, _U32])
+        self.joint_hinge_angle = _bind(L, "nph_physics_joint_hinge_angle", C.c_float,
+                                       [C.c_void_p, _U32])
+        self.joint_length = _bind(L, "nph_physics_joint_length", C.c_float, [C.c_void_p, _U32])
+        self.joint_reaction = _bind(L, "nph_physics_joint_reaction", None,
+                                    [C.c_void_p, _U32, C.POINTER(Vec3), C.POINTER(Vec3)])
 
 
 _APIS: dict = {}
@@ -296,6 +369,27 @@ class World:
         if r != OK:
             raise RuntimeError(f"nph_physics_shape_create failed: {r}")
         self.shapes.append(h.value)
+        return h.value
+
+    def add_joint(self, joint_type, body_a, body_b, anchor, axis, **settings):
+        """A joint whose frames meet at the world point `anchor` with X along
+        `axis` (nph_physics_joint_frames_at), then `settings` applied: a
+        dict per union member, e.g. ball={"enable_swing_limit": 1, ...}."""
+        a = self.api
+        d = a.joint_desc_default(joint_type, body_a, body_b)
+        a.joint_frames_at(self.ptr, body_a, body_b, DVec3(*anchor), Vec3(*axis),
+                          C.byref(d.frame_a), C.byref(d.frame_b))
+        for member, values in settings.items():
+            if isinstance(values, dict):
+                target = getattr(d, member)
+                for k, v in values.items():
+                    setattr(target, k, v)
+            else:
+                setattr(d, member, values)
+        h = _U32()
+        r = a.joint_create(self.ptr, C.byref(d), C.byref(h))
+        if r != OK:
+            raise RuntimeError(f"nph_physics_joint_create failed: {r}")
         return h.value
 
     def step(self, dt, jobs=None):

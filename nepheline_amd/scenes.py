@@ -6,6 +6,9 @@ same calls drive this package's library and, in tests, the reference's.
   on a static ground box, as engine/physics/tests/test_world.c:365-379.
 * bin: mixed spheres, capsules, boxes and 12-vertex hulls dropped into an
   open static bin (the 10k-body workload and smaller versions of it).
+* ragdolls: the reference's eleven-body ragdoll (ball joints with cones and
+  twists, hinges at elbows and knees, test_world_support.h:281-379) and its
+  criss-cross pile (test_ragdoll.c:269-303).
 
 Random draws come from PCG32 (engine/math/include/nepheline/math/random.h:
 38-74, O'Neill's pcg32 XSH-RR with the reference's seeding), one draw per
@@ -153,4 +156,177 @@ def bin_drop(api, bodies: int, seed: int, hulls: HullBank | None, half: float = 
         b = w.add_body(P.BODY_DYNAMIC, (x, y, z), q)
         w.add_shape(b, g, friction=0.5)
     w._keep.append(hulls)
+    return w
+
+
+# ---------------------------------------------------------------------------
+# Ragdolls.  Every float operation below is the reference's, in single
+# precision where it uses f32 (numpy float32 rounds each operation), f64
+# where it uses f64, and sinf/cosf from the C library.
+# ---------------------------------------------------------------------------
+_F = np.float32
+_libm = C.CDLL("libm.so.6")
+_libm.sinf.restype = C.c_float
+_libm.sinf.argtypes = [C.c_float]
+_libm.cosf.restype = C.c_float
+_libm.cosf.argtypes = [C.c_float]
+
+K_PI = _F(3.14159265)
+DEG = _F(K_PI / _F(180.0))
+
+
+def quat_axis_angle(axis, angle):
+    """nph_quat_from_axis_angle (quat.h:121-127)."""
+    half = _F(_F(0.5) * _F(angle))
+    sn = _F(_libm.sinf(float(half)))
+    cs = _F(_libm.cosf(float(half)))
+    return (_F(_F(axis[0]) * sn), _F(_F(axis[1]) * sn), _F(_F(axis[2]) * sn), cs)
+
+
+def quat_mul(a, b):
+    """Hamilton product (quat.h:70-76), left to right in f32."""
+    ax, ay, az, aw = (_F(v) for v in a)
+    bx, by, bz, bw = (_F(v) for v in b)
+    return (_F(_F(_F(aw * bx) + _F(ax * bw)) + _F(ay * bz)) - _F(az * by),
+            _F(_F(_F(aw * by) - _F(ax * bz)) + _F(ay * bw)) + _F(az * bx),
+            _F(_F(_F(aw * bz) + _F(ax * by)) - _F(ay * bx)) + _F(az * bw),
+            _F(_F(_F(aw * bw) - _F(ax * bx)) - _F(ay * by)) - _F(az * bz))
+
+
+def _cross(a, b, T):
+    return (T(T(a[1] * b[2]) - T(a[2] * b[1])), T(T(a[2] * b[0]) - T(a[0] * b[2])),
+            T(T(a[0] * b[1]) - T(a[1] * b[0])))
+
+
+def quat_rotate(q, v):
+    """v + 2w (u x v) + 2 u x (u x v) in f32 (quat.h:157-164)."""
+    u = (_F(q[0]), _F(q[1]), _F(q[2]))
+    v = tuple(_F(x) for x in v)
+    c = _cross(u, v, _F)
+    t = tuple(_F(x * _F(2.0)) for x in c)
+    c2 = _cross(u, t, _F)
+    return tuple(_F(_F(v[i] + _F(t[i] * _F(q[3]))) + c2[i]) for i in range(3))
+
+
+def ragdoll_turn(q, v):
+    """`v` turned by `q` in f64 (test_world_support.h:293-299)."""
+    u = (float(q[0]), float(q[1]), float(q[2]))
+    c = _cross(u, v, float)
+    t = tuple(x * 2.0 for x in c)
+    c2 = _cross(u, t, float)
+    return tuple((v[i] + t[i] * float(q[3])) + c2[i] for i in range(3))
+
+
+_RAGDOLL_PARTS = (  # x, y, geometry, sideways
+    (0.0, 1.0, ("box", (0.15, 0.1, 0.1)), False),
+    (0.0, 1.32, ("box", (0.16, 0.17, 0.1)), False),
+    (0.0, 1.62, ("sphere", 0.1), False),
+    (-0.33, 1.4, ("capsule", (0.11, 0.05)), True),
+    (0.33, 1.4, ("capsule", (0.11, 0.05)), True),
+    (-0.62, 1.4, ("capsule", (0.11, 0.045)), True),
+    (0.62, 1.4, ("capsule", (0.11, 0.045)), True),
+    (-0.1, 0.68, ("capsule", (0.14, 0.07)), False),
+    (0.1, 0.68, ("capsule", (0.14, 0.07)), False),
+    (-0.1, 0.25, ("capsule", (0.14, 0.06)), False),
+    (0.1, 0.25, ("capsule", (0.14, 0.06)), False),
+)
+
+_RAGDOLL_JOINTS = (  # a, b, x, y, axis, hinge, p0, p1, p2
+    (0, 1, 0.0, 1.12, (0.0, 1.0, 0.0), False, 30.0, 20.0, 20.0),
+    (1, 2, 0.0, 1.5, (0.0, 1.0, 0.0), False, 40.0, 40.0, 45.0),
+    (1, 3, -0.17, 1.42, (-1.0, 0.0, 0.0), False, 80.0, 60.0, 45.0),
+    (1, 4, 0.17, 1.42, (1.0, 0.0, 0.0), False, 80.0, 60.0, 45.0),
+    (3, 5, -0.48, 1.4, (0.0, 1.0, 0.0), True, 0.0, 140.0, 0.0),
+    (4, 6, 0.48, 1.4, (0.0, -1.0, 0.0), True, 0.0, 140.0, 0.0),
+    (0, 7, -0.1, 0.9, (0.0, -1.0, 0.0), False, 60.0, 30.0, 30.0),
+    (0, 8, 0.1, 0.9, (0.0, -1.0, 0.0), False, 60.0, 30.0, 30.0),
+    (7, 9, -0.1, 0.46, (1.0, 0.0, 0.0), True, 0.0, 140.0, 0.0),
+    (8, 10, 0.1, 0.46, (1.0, 0.0, 0.0), True, 0.0, 140.0, 0.0),
+)
+
+
+def _geometry(g):
+    kind, v = g
+    if kind == "box":
+        return P.geometry_box(*v)
+    if kind == "sphere":
+        return P.geometry_sphere(v)
+    return P.geometry_capsule(*v)
+
+
+def ragdoll(w: P.World, base, turn=(0.0, 0.0, 0.0, 1.0)):
+    """An eleven-body ragdoll standing in a T-pose at `base`, turned by `turn`
+    (build_ragdoll, test_world_support.h:301-379). Returns (bodies, joints)."""
+    side = quat_axis_angle((0.0, 0.0, 1.0), _F(_F(90.0) * DEG))
+    turn = tuple(_F(x) for x in turn)
+    bodies = []
+    for x, y, g, sideways in _RAGDOLL_PARTS:
+        t = ragdoll_turn(turn, (x, y, 0.0))
+        pos = (base[0] + t[0], base[1] + t[1], base[2] + t[2])
+        rot = quat_mul(turn, side if sideways else (0.0, 0.0, 0.0, 1.0))
+        b = w.add_body(P.BODY_DYNAMIC, pos, tuple(float(c) for c in rot))
+        w.add_shape(b, _geometry(g), friction=0.6, restitution=0.0)
+        bodies.append(b)
+    joints = []
+    for a, b, x, y, axis, hinge, p0, p1, p2 in _RAGDOLL_JOINTS:
+        t = ragdoll_turn(turn, (x, y, 0.0))
+        at = (base[0] + t[0], base[1] + t[1], base[2] + t[2])
+        ax = tuple(float(c) for c in quat_rotate(turn, axis))
+        if hinge:
+            j = w.add_joint(P.JOINT_HINGE, bodies[a], bodies[b], at, ax, hinge={
+                "enable_limit": 1, "lower": float(_F(_F(p0) * DEG)),
+                "upper": float(_F(_F(p1) * DEG)), "enable_motor": 1, "max_motor_torque": 2.0})
+        else:
+            j = w.add_joint(P.JOINT_BALL, bodies[a], bodies[b], at, ax, ball={
+                "enable_swing_limit": 1, "swing_y": float(_F(_F(p0) * DEG)),
+                "swing_z": float(_F(_F(p1) * DEG)), "enable_twist_limit": 1,
+                "twist_lower": float(_F(_F(-p2) * DEG)), "twist_upper": float(_F(_F(p2) * DEG)),
+                "friction_torque": 2.0})
+        joints.append(j)
+    return bodies, joints
+
+
+def throw_ragdoll(w: P.World, bodies, v, omega):
+    """Moving as one rigid body: `v` at the pelvis, turning at `omega`
+    (test_ragdoll.c:269-281)."""
+    a = w.api
+    about = a.body_center(w.ptr, bodies[0])
+    om = tuple(_F(c) for c in omega)
+    for b in bodies:
+        c = a.body_center(w.ptr, b)
+        arm = (_F(c.x - about.x), _F(c.y - about.y), _F(c.z - about.z))
+        cr = _cross(om, arm, _F)
+        lin = tuple(float(_F(_F(v[i]) + cr[i])) for i in range(3))
+        a.body_set_linear_velocity(w.ptr, b, P.Vec3(*lin))
+        a.body_set_angular_velocity(w.ptr, b, P.Vec3(*(float(c) for c in om)))
+
+
+def ground(w: P.World, top=0.0):
+    """A static slab whose top face is y = top (test_world_support.h:63-68)."""
+    g = w.add_body(P.BODY_STATIC, (0.0, top - 1.0, 0.0))
+    w.add_shape(g, P.geometry_box(50.0, 1.0, 50.0), friction=0.6)
+    return g
+
+
+def ragdoll_pile(api, count: int = 4, seed: int = 2026, desc: dict | None = None) -> P.World:
+    """`count` ragdolls lying criss-cross one above another, each a quarter
+    turn from the last and thrown at random (build_pile, test_ragdoll.c:
+    289-303), on a ground slab."""
+    d = dict(DESC)
+    d.update(desc or {})
+    w = P.World(api, d)
+    ground(w, 0.0)
+    r = Pcg32(seed)
+    lying = quat_axis_angle((1.0, 0.0, 0.0), _F(_F(-90.0) * DEG))
+    w.ragdolls = []
+    for k in range(count):
+        turn = quat_mul(quat_axis_angle((0.0, 1.0, 0.0), _F(_F(_F(k) * _F(90.0)) * DEG)), lying)
+        pelvis = (0.0, 0.15 + 0.3 * float(k), 0.0)
+        off = ragdoll_turn(turn, (0.0, 1.0, 0.0))
+        base = (pelvis[0] - off[0], pelvis[1] - off[1], pelvis[2] - off[2])
+        bodies, joints = ragdoll(w, base, turn)
+        v = tuple(float(r.next_range(-0.5, 0.5)) for _ in range(3))
+        om = tuple(float(r.next_range(-1.0, 1.0)) for _ in range(3))
+        throw_ragdoll(w, bodies, v, om)
+        w.ragdolls.append((bodies, joints))
     return w

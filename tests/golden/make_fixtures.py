@@ -103,6 +103,17 @@ def main():
     with open(os.path.join(HERE, "bin_hashes.json"), "w") as f:
         json.dump({"scene": "bin_drop half=4 height=10 y_lo=1 dt=1/60 substeps=4",
                    "runs": runs}, f)
+    rag = {}
+    w = S.ragdoll_pile(api, 4, 2026)
+    rag["pile_4"] = hashes(w, 240)
+    w = P.World(api, S.DESC)
+    S.ground(w, 0.0)
+    S.ragdoll(w, (0.0, 0.95, 0.0))
+    rag["standing_1"] = hashes(w, 180)
+    with open(os.path.join(HERE, "ragdoll_hashes.json"), "w") as f:
+        json.dump({"scene": "ragdoll pile (test_ragdoll.c:289-303) seed 2026 and one standing "
+                            "ragdoll at (0, 0.95, 0), on a ground slab, dt=1/60 substeps=4",
+                   "runs": rag}, f)
     print("fixtures written")
 
 

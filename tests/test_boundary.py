@@ -44,7 +44,9 @@ LAYOUT_TYPES = {
     "NphPhysicsWorldDesc": P.WorldDesc, "NphPhysicsBodyDesc": P.BodyDesc,
     "NphPhysicsFilter": P.Filter, "NphPhysicsShapeDesc": P.ShapeDesc, "NphPhysicsStats": P.Stats,
     "NphPhysicsHull": None, "NphPhysicsEvents": None, "NphPhysicsContactHitEvent": None,
-    "NphPhysicsBodyMoveEvent": None,
+    "NphPhysicsBodyMoveEvent": None, "NphPhysicsJointDesc": P.JointDesc,
+    "NphPhysicsBallJoint": P.BallJoint, "NphPhysicsHingeJoint": P.HingeJoint,
+    "NphPhysicsD6Joint": P.D6Joint,
 }
 
 
@@ -93,7 +95,8 @@ def test_header_layouts_are_the_references(tmp_path):
         pytest.skip("reference sources not present")
     mine = _layouts(tmp_path, [os.path.join(ROOT, "include")], "<nepheline_physics.h>")
     hdr = "<nepheline/physics/world.h>\n#include <nepheline/physics/body.h>\n" \
-          "#include <nepheline/physics/shape.h>\n#include <nepheline/physics/hull.h>"
+          "#include <nepheline/physics/shape.h>\n#include <nepheline/physics/hull.h>\n" \
+          "#include <nepheline/physics/joint.h>"
     ref = _layouts(tmp_path, REF_INCLUDE, hdr)
     assert mine == ref
 
