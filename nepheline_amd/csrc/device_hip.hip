@@ -116,6 +116,17 @@ struct NphDevice {
     Staged staged_jstate;
     u32 *step_joints; // the graph's joints for a step
     u32 cap_step_joints;
+    // Between nph_dev_solve and nph_dev_solve_finish.
+    NphStepData pending_sd;
+    u32 pending_timed;
+    // Continuous collision: fast bodies (count first) and their packed rows.
+    u32 *fast_list;
+    NphFastBody *fast_pack;
+    u32 cap_fast_pack;
+    u32 *h_fast_count;
+    u64 cap_h_fast_count;
+    NphFastBody *h_fast_pack;
+    u64 cap_h_fast_pack;
 };
 
 const char *nph_dev_name(void) { return "hip:gfx950"; }
@@ -188,7 +199,8 @@ void nph_dev_destroy(NphDevice *d)
     void *bufs[] = {d->bodies, d->shapes, d->contacts, d->solver_of_body, d->awake_bodies,
                     d->sbodies, d->params, d->list, d->codes, d->inits, d->groups,
                     d->lane_contact, d->lane_body_a, d->lane_body_b, d->lane_flags, d->lanes,
-                    d->points, d->grown, d->grown_blocks, d->joint_conf, d->joint_state, d->step_joints};
+                    d->points, d->grown, d->grown_blocks, d->joint_conf, d->joint_state, d->step_joints,
+                    d->fast_list, d->fast_pack};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
             (void)hipFree(bufs[i]);
@@ -210,8 +222,8 @@ void nph_dev_destroy(NphDevice *d)
     if (d->h_u32 != NULL) {
         (void)hipHostFree(d->h_u32);
     }
-    void *pinned[] = {d->h_list, d->h_codes, d->h_inits};
-    for (u32 i = 0U; i < 3U; ++i) {
+    void *pinned[] = {d->h_list, d->h_codes, d->h_inits, d->h_fast_count, d->h_fast_pack};
+    for (u32 i = 0U; i < 5U; ++i) {
         if (pinned[i] != NULL) {
             (void)hipHostFree(pinned[i]);
         }
@@ -264,7 +276,9 @@ NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts)
             (r = grow_buffer((void **)&d->params, 0, (u64)c * sizeof(NphSolverParams), d->stream,
                              0)) != NPH_RESULT_OK ||
             (r = grow_buffer((void **)&d->grown_blocks, 0, (u64)(c / NPH_BLOCK + 1U) * 4U,
-                             d->stream, 0)) != NPH_RESULT_OK) {
+                             d->stream, 0)) != NPH_RESULT_OK ||
+            (r = grow_buffer((void **)&d->fast_list, 0, (u64)(c + 1U) * 4U, d->stream, 0)) !=
+                NPH_RESULT_OK) {
             return r;
         }
         d->cap_bodies = c;
@@ -935,6 +949,66 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_grown_scatter(NphStepData d,
     }
 }
 
+// The fast bodies (continuous collision) in the awake list's order: counts
+// per block, then placement; out[0] = count, then awake indices.
+__global__ void __launch_bounds__(NPH_BLOCK) k_fast_count(NphStepData d, u32 *block_counts)
+{
+    const u32 i = blockIdx.x * NPH_BLOCK + threadIdx.x;
+    const u32 fast = i < d.body_count &&
+                     (d.bodies[d.awake_bodies[i]].flags & NPH_BODY_FAST) != 0U ? 1U : 0U;
+    const u32 total = block_sum(fast);
+    if (threadIdx.x == 0U) {
+        block_counts[blockIdx.x] = total;
+    }
+}
+
+__global__ void __launch_bounds__(NPH_BLOCK) k_fast_scatter(NphStepData d,
+                                                            const u32 *block_counts, u32 *out)
+{
+    __shared__ u32 scan[NPH_BLOCK];
+    u32 before = 0U;
+    for (u32 j = threadIdx.x; j < blockIdx.x; j += NPH_BLOCK) {
+        before += block_counts[j];
+    }
+    const u32 base = block_sum(before);
+    const u32 i = blockIdx.x * NPH_BLOCK + threadIdx.x;
+    const u32 mine = i < d.body_count &&
+                     (d.bodies[d.awake_bodies[i]].flags & NPH_BODY_FAST) != 0U ? 1U : 0U;
+    scan[threadIdx.x] = mine;
+    __syncthreads();
+    for (u32 off = 1U; off < NPH_BLOCK; off <<= 1U) {
+        const u32 x = threadIdx.x >= off ? scan[threadIdx.x - off] : 0U;
+        __syncthreads();
+        scan[threadIdx.x] += x;
+        __syncthreads();
+    }
+    if (mine != 0U) {
+        out[1U + base + scan[threadIdx.x] - 1U] = i;
+    }
+    if (blockIdx.x == gridDim.x - 1U && threadIdx.x == NPH_BLOCK - 1U) {
+        out[0] = base + scan[NPH_BLOCK - 1U];
+    }
+}
+
+// Each fast body's record and solver rows, for the host; its flag cleared
+// (the sweep is done once).
+__global__ void __launch_bounds__(NPH_BLOCK) k_fast_pack(NphStepData d, const u32 *list, u32 count,
+                                                         NphFastBody *out)
+{
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) {
+        const u32 i = list[1U + k];
+        const u32 slot = d.awake_bodies[i];
+        NphBodyRec *b = &d.bodies[slot];
+        b->flags &= ~(u32)NPH_BODY_FAST;
+        out[k].body = *b;
+        out[k].sb = d.sbodies[i];
+        out[k].params = d.params[i];
+        out[k].index = i;
+        out[k].slot = slot;
+    }
+}
+
 static u32 blocks_for(u32 n) { return (n + NPH_BLOCK - 1U) / NPH_BLOCK; }
 
 // A pinned host buffer of at least `bytes`; the stream is drained before an
@@ -1085,11 +1159,12 @@ NphResult nph_dev_collide_end(NphDevice *d, const u32 *awake, u32 count, f32 ste
     return NPH_RESULT_OK;
 }
 
-NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **grown,
-                        u32 *grown_count)
+NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody **fast,
+                        u32 *fast_count)
 {
-    *grown = NULL;
-    *grown_count = 0U;
+    *fast = NULL;
+    *fast_count = 0U;
+    d->pending_sd.body_count = 0U;
     {
         const NphResult fr = flush_staged(d);
         if (fr != NPH_RESULT_OK) {
@@ -1292,6 +1367,69 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
     }
 #undef PASS
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd, sp);
+    HIP_TRY(hipGetLastError());
+    d->pending_sd = sd;
+    d->pending_timed = timed;
+    if (sp.continuous == 0U) {
+        return NPH_RESULT_OK;
+    }
+    // The fast bodies in the awake list's order, packed with their solver
+    // rows for the host's sweeps.
+    hipLaunchKernelGGL(k_fast_count, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
+                       d->grown_blocks);
+    hipLaunchKernelGGL(k_fast_scatter, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
+                       d->grown_blocks, d->fast_list);
+    HIP_TRY(hipGetLastError());
+    NphResult pr = reserve_pinned(d, (void **)&d->h_fast_count, &d->cap_h_fast_count, 64U);
+    if (pr != NPH_RESULT_OK) {
+        return pr;
+    }
+    HIP_TRY(hipMemcpyAsync(d->h_fast_count, d->fast_list, 4U, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const u32 f = d->h_fast_count[0];
+    if (f == 0U) {
+        return NPH_RESULT_OK;
+    }
+    if (f > d->cap_fast_pack) {
+        const u32 c = grow_cap(d->cap_fast_pack, f);
+        pr = grow_buffer((void **)&d->fast_pack, 0, (u64)c * sizeof(NphFastBody), s, 0);
+        if (pr != NPH_RESULT_OK) {
+            return pr;
+        }
+        d->cap_fast_pack = c;
+    }
+    pr = reserve_pinned(d, (void **)&d->h_fast_pack, &d->cap_h_fast_pack,
+                        (u64)f * sizeof(NphFastBody));
+    if (pr != NPH_RESULT_OK) {
+        return pr;
+    }
+    hipLaunchKernelGGL(k_fast_pack, dim3(blocks_for(f)), dim3(NPH_BLOCK), 0, s, sd, d->fast_list,
+                       f, d->fast_pack);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d->h_fast_pack, d->fast_pack, (u64)f * sizeof(NphFastBody),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *fast = d->h_fast_pack;
+    *fast_count = f;
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_dev_solve_finish(NphDevice *d, const NphGrown **grown, u32 *grown_count)
+{
+    *grown = NULL;
+    *grown_count = 0U;
+    {
+        const NphResult fr = flush_staged(d); // the host's sweeps
+        if (fr != NPH_RESULT_OK) {
+            return fr;
+        }
+    }
+    const NphStepData sd = d->pending_sd;
+    const u32 n = sd.body_count;
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    hipStream_t s = d->stream;
     hipLaunchKernelGGL(k_grown_count, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
                        d->grown_blocks);
     hipLaunchKernelGGL(k_grown_scatter, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
@@ -1318,12 +1456,13 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **
                                hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
     }
-    for (u32 i = 0U; i < timed; ++i) {
+    for (u32 i = 0U; i < d->pending_timed; ++i) {
         float kms = 0.0F;
         HIP_TRY(hipEventElapsedTime(&kms, d->kev[2U * i], d->kev[2U * i + 1U]));
         d->kernel_ms += kms;
         d->kernel_launches += 1U;
     }
+    d->pending_timed = 0U;
     float ms = 0.0F;
     (void)hipEventElapsedTime(&ms, d->ev[0], d->ev[1]);
     d->solve_ms = ms;

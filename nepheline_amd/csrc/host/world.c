@@ -235,8 +235,7 @@ NphResult nph_physics_world_create(const NphPhysicsWorldDesc *desc, NphPhysicsWo
         !nph_isfinitef(desc->max_linear_speed) || !finite_nonneg(desc->time_to_sleep)) {
         return NPH_RESULT_INVALID_ARGUMENT;
     }
-    if (desc->solver != NPH_PHYSICS_SOLVER_SOFT_STEP || desc->enable_sleep != 0 ||
-        desc->enable_continuous != 0) {
+    if (desc->solver != NPH_PHYSICS_SOLVER_SOFT_STEP || desc->enable_sleep != 0) {
         return NPH_RESULT_UNSUPPORTED;
     }
     NphAllocator *a = desc->alloc;
@@ -712,6 +711,22 @@ static NphShapeRec shape_record(NphPhysicsWorld *w, u32 s, const NphPhysicsHull 
     return r;
 }
 
+NphResult world_shape_record(NphPhysicsWorld *w, u32 s, NphShapeRec *out)
+{
+    const HShape *h = shape_slot(w, s);
+    const NphPhysicsHull *device_hull = NULL;
+    if (h->geometry.type == NPH_PHYSICS_SHAPE_HULL) {
+        NphResult r = nph_dev_hull(w->dev, h->geometry.hull, &device_hull);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    *out = shape_record(w, s, device_hull);
+    return NPH_RESULT_OK;
+}
+
+NphGeom world_host_geom(const HShape *s) { return host_geom(s); }
+
 NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
                                    const NphPhysicsShapeDesc *desc, NphPhysicsShape *out)
 {
@@ -924,13 +939,29 @@ static NphResult solve(NphPhysicsWorld *w, f32 dt)
     make_soft(nph_minf(joint_hertz, 0.25F * sp->inv_h), w->desc.joint_damping_ratio, sp->h,
               &sp->joint_bias, &sp->joint_mass, &sp->joint_impulse);
     w->last_substep = sp->h;
-    const NphGrown *grown = NULL;
-    u32 grown_count = 0U;
-    r = nph_dev_solve(w->dev, &in, &grown, &grown_count);
+    sp->continuous = w->desc.enable_continuous != 0 ? 1U : 0U;
+    w->stats.swept_bodies = 0U;
+    w->stats.impact_bodies = 0U;
+    const NphFastBody *fast = NULL;
+    u32 fast_count = 0U;
+    r = nph_dev_solve(w->dev, &in, &fast, &fast_count);
     if (r != NPH_RESULT_OK) {
         return r;
     }
     w->bodies_stale = 1;
+    if (fast_count > 0U) {
+        r = continuous_sweep(w, fast, fast_count);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    const NphGrown *grown = NULL;
+    u32 grown_count = 0U;
+    r = nph_dev_solve_finish(w->dev, &grown, &grown_count);
+    w->bodies_stale = 1; // the sweeps may have read the records mid-step
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
     return contacts_post_solve(w, grown, grown_count);
 }
 

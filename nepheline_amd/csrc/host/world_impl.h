@@ -243,6 +243,13 @@ void graph_set_bodies(NphPhysicsWorld *w, u32 color, u32 body_a, u32 body_b, int
 void contacts_destroy(NphPhysicsWorld *w, u32 contact, int events);
 HContact *contacts_at(NphPhysicsWorld *w, u32 contact);
 
+NphResult world_shape_record(NphPhysicsWorld *w, u32 s, NphShapeRec *out); // as the device holds it
+NphGeom world_host_geom(const HShape *s); // the geometry with the host's hull address
+
+// continuous.c restated: the fast bodies' sweeps, between finalize and the
+// grown proxies.
+NphResult continuous_sweep(NphPhysicsWorld *w, const NphFastBody *fast, u32 count);
+
 // joint.c restated
 int joints_allow_contact(const NphPhysicsWorld *w, u32 body_a, u32 body_b);
 void joints_free(NphPhysicsWorld *w);

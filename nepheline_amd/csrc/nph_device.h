@@ -80,10 +80,25 @@ NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 co
 NphResult nph_dev_collide_begin(NphDevice *d, const u32 *awake, u32 count, f32 step_dt);
 NphResult nph_dev_collide_end(NphDevice *d, const u32 *awake, u32 count, f32 step_dt,
                               const u32 **codes);
-// The solver stages and finalize; the grown proxies into *grown (the
-// device's buffer, valid until the next call), *grown_count of them.
-NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphGrown **grown,
-                        u32 *grown_count);
+// A fast body (continuous collision) after finalize: its record (pose at
+// the step's end, boxes not yet refreshed unless a bullet), its solver rows.
+typedef struct NphFastBody {
+    NphBodyRec body;
+    NphSolverBody sb;
+    NphSolverParams params;
+    u32 index; // awake index
+    u32 slot;  // body slot
+    u32 pad[2];
+} NphFastBody;
+
+// The solver stages and finalize.  With params.continuous, the fast bodies
+// come back through *fast (valid until the next call) for the host to sweep
+// and rewrite (nph_dev_write_body/shape) before nph_dev_solve_finish.
+NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody **fast,
+                        u32 *fast_count);
+// The grown proxies into *grown (the device's buffer, valid until the next
+// call), *grown_count of them, after the host's writes reach the device.
+NphResult nph_dev_solve_finish(NphDevice *d, const NphGrown **grown, u32 *grown_count);
 // Seconds of device time the last solve took (event-timed), and the last
 // collide.
 f64 nph_dev_last_solve_ms(const NphDevice *d);

@@ -32,6 +32,7 @@ enum {
     NPH_BODY_GYROSCOPIC = 1U << 2U, // world_internal.h:58-68
     NPH_BODY_BULLET = 1U << 5U,
     NPH_BODY_ENABLE_SLEEP = 1U << 1U,
+    NPH_BODY_FAST = 1U << 6U, // moved far enough this step to be swept (continuous collision)
 };
 
 // A body slot's numeric state (PhysBody's numbers, world_internal.h:71-107).
@@ -116,7 +117,7 @@ typedef struct NphSolverBody {
     f32 dq[4];
 } NphSolverBody;
 
-enum { NPH_PARAM_DYNAMIC = 1U, NPH_PARAM_GYROSCOPIC = 2U };
+enum { NPH_PARAM_DYNAMIC = 1U, NPH_PARAM_GYROSCOPIC = 2U, NPH_PARAM_BULLET = 4U };
 
 typedef struct NphSolverParams {
     NphVec3 linear_accel;
@@ -234,6 +235,7 @@ typedef struct NphStepParams {
     f32 static_bias, static_mass, static_impulse;    // phys_make_soft(2x contact)
     f32 joint_bias, joint_mass, joint_impulse;       // phys_make_soft(joint)
     u32 substeps;
+    u32 continuous; // enable_continuous: finalize marks fast bodies
 } NphStepParams;
 
 // solver.c:48-65
@@ -336,6 +338,9 @@ NPH_HD void prepare_body(const NphStepParams *sp, const NphBodyRec *b, u32 slot,
         p->flags = NPH_PARAM_DYNAMIC;
         if ((b->flags & NPH_BODY_GYROSCOPIC) != 0U && !inertia_is_isotropic(b->inertia)) {
             p->flags |= NPH_PARAM_GYROSCOPIC;
+        }
+        if ((b->flags & NPH_BODY_BULLET) != 0U) {
+            p->flags |= NPH_PARAM_BULLET;
         }
         p->linear_accel =
             v3_add(v3_scale(sp->gravity, b->gravity_scale), v3_scale(b->force, b->inv_mass));
@@ -458,12 +463,15 @@ NPH_HD NphAABB shape_world_aabb(const NphBodyRec *b, const NphShapeRec *s)
     return nph_world_box(b->origin, geom_aabb(&s->geom, pose));
 }
 
-// solver.c:304-369 without sleep's island logic and continuous collision
-// (the world rejects both): the pose, velocities, forces cleared, the sleep
-// timer, and the shapes' boxes.  Returns nonzero when a shape grew.
+NPH_HD u32 body_refresh_boxes(NphBodyRec *b, NphShapeRec *shapes);
+
+// solver.c:304-369 without sleep's island logic (the world rejects sleep):
+// the pose, velocities, forces cleared, the sleep timer, and the shapes'
+// boxes.  Returns nonzero when a shape grew.
 NPH_HD u32 finalize_body(const NphStepParams *sp, const NphSolverParams *p,
                          const NphSolverBody *sb, NphBodyRec *b, NphShapeRec *shapes)
 {
+    b->enlarged = 0U;
     b->v = sb_v(sb);
     b->w = sb_w(sb);
     b->rotation = q_normalize(q_mul(sb_dq(sb), p->q0));
@@ -477,6 +485,34 @@ NPH_HD u32 finalize_body(const NphStepParams *sp, const NphSolverParams *p,
     } else {
         b->sleep_time += sp->dt;
     }
+    // Moved farther than half its thinnest shape's inner radius, or than
+    // speculative contacts reach (solver.c:326-347): swept on the host
+    // against the static shapes before its boxes are refreshed, unless a
+    // bullet (refreshed now, swept after every body is final).
+    if (sp->continuous != 0U && (p->flags & NPH_PARAM_DYNAMIC) != 0U) {
+        const f32 limit =
+            nph_minf(0.5F * b->min_extent, NPH_PHYSICS_AABB_MARGIN - NPH_PHYSICS_LINEAR_SLOP);
+        const f32 half = 0.25F * limit * limit;
+        const f32 arm = 3.14159265F * b->max_extent;
+        const NphQuat dq = sb_dq(sb);
+        const f32 turned = v3_len_sq(q_xyz(dq)) * arm * arm;
+        const NphVec3 dp = sb_dp(sb);
+        if ((v3_len_sq(dp) > half || turned > half) &&
+            v3_len(dp) + 2.0F * nph_atan2f(v3_len(q_xyz(dq)), nph_absf(dq.w)) * b->max_extent >
+                limit) {
+            b->flags |= NPH_BODY_FAST;
+            if ((p->flags & NPH_PARAM_BULLET) == 0U) {
+                return 0U;
+            }
+        }
+    }
+    return body_refresh_boxes(b, shapes);
+}
+
+// solver.c:351-369: the shapes' boxes; a shape that left its fat box gets a
+// new one and is flagged for the tree.
+NPH_HD u32 body_refresh_boxes(NphBodyRec *b, NphShapeRec *shapes)
+{
     u32 grew = 0U;
     for (u32 s = b->shape_head; s != NPH_NULL32; s = shapes[s].next) {
         NphShapeRec *shape = &shapes[s];
@@ -492,7 +528,7 @@ NPH_HD u32 finalize_body(const NphStepParams *sp, const NphSolverParams *p,
             grew = 1U;
         }
     }
-    b->enlarged = grew;
+    b->enlarged = b->enlarged != 0U || grew != 0U ? 1U : 0U;
     return grew;
 }
 

@@ -330,3 +330,34 @@ def ragdoll_pile(api, count: int = 4, seed: int = 2026, desc: dict | None = None
         throw_ragdoll(w, bodies, v, om)
         w.ragdolls.append((bodies, joints))
     return w
+
+
+def projectiles(api, count: int = 40, seed: int = 5, bullets: bool = True,
+                desc: dict | None = None) -> P.World:
+    """Continuous collision: spheres, capsules and boxes shot at 60-150 m/s
+    along +Z, spinning, at a 4 cm static wall 5 m away; every fifth a bullet
+    (swept against bodies too); a stack of six dynamic boxes between them."""
+    d = {"enable_sleep": 0, "enable_continuous": 1}
+    d.update(desc or {})
+    w = P.World(api, d)
+    ground(w, 0.0)
+    wall = w.add_body(P.BODY_STATIC, (0.0, 2.0, 5.0))
+    w.add_shape(wall, P.geometry_box(3.0, 2.0, 0.02))
+    r = Pcg32(seed)
+    for i in range(count):
+        x = float(r.next_range(-2.0, 2.0))
+        y = float(r.next_range(0.5, 3.5))
+        q = rotation(r)
+        vz = float(r.next_range(60.0, 150.0))
+        wx = float(r.next_range(-20.0, 20.0))
+        b = w.add_body(P.BODY_DYNAMIC, (x, y, 0.0), q, velocity=(0.0, 0.0, vz),
+                       angular_velocity=(wx, 0.0, 0.0),
+                       bullet=1 if (bullets and i % 5 == 0) else 0)
+        k = i % 3
+        g = (P.geometry_sphere(0.1) if k == 0 else
+             P.geometry_capsule(0.15, 0.05) if k == 1 else P.geometry_box(0.1, 0.05, 0.2))
+        w.add_shape(b, g)
+    for j in range(6):
+        b = w.add_body(P.BODY_DYNAMIC, (0.0, 0.25 + 0.5 * j, 2.5))
+        w.add_shape(b, P.geometry_box(0.25, 0.25, 0.25))
+    return w

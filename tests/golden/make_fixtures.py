@@ -114,6 +114,14 @@ def main():
         json.dump({"scene": "ragdoll pile (test_ragdoll.c:289-303) seed 2026 and one standing "
                             "ragdoll at (0, 0.95, 0), on a ground slab, dt=1/60 substeps=4",
                    "runs": rag}, f)
+    ccd = {"projectiles_40": hashes(S.projectiles(api, 40, 5), 120)}
+    w = S.bin_drop(api, 300, 21, bank, half=4.0, height=30.0, y_lo=1.0,
+                   desc={"enable_continuous": 1})
+    ccd["bin_300_21_h30"] = hashes(w, 200)
+    with open(os.path.join(HERE, "continuous_hashes.json"), "w") as f:
+        json.dump({"scene": "continuous collision on: projectiles (40 shots at a thin wall, "
+                            "every fifth a bullet) and a 300-body bin drop from 30 m",
+                   "runs": ccd}, f)
     print("fixtures written")
 
 

@@ -135,6 +135,9 @@ def test_unsupported_settings_are_rejected_like_the_reference_rejects_bad_ones()
     d.enable_sleep = 0
     d.enable_continuous = 0
     assert a.world_create(C.byref(d), C.byref(w)) == P.UNSUPPORTED
+    d.solver = 0
+    d.enable_sleep = 1
+    assert a.world_create(C.byref(d), C.byref(w)) == P.UNSUPPORTED
 
 
 def test_desc_defaults_match_the_reference(ref_api):
