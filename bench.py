@@ -12,8 +12,11 @@ narrow phase, serial bookkeeping, solver, finalize.
 Beside it: the roofline of the dominant kernel, the colour pass (one launch
 per colour per constraint pass; HIP events around each launch, in a window of
 steps right after the timed region so the headline stays uninstrumented),
-and the reference's own CPU step (oracle/_ref, its job system on up to 16
-host threads) on a bounded sample of the same scene.
+the reference's own CPU step (oracle/_ref, its job system on up to 16 host
+threads) on a bounded sample of the same scene, and the learned warm start:
+a network trained on the device from plain bin drops of other seeds, the
+same scene and steps timed with it, the post-solve residual and penetration
+of both starts, and the network's forward kernel against the MFMA peak.
 """
 import argparse
 import ctypes as C
@@ -31,6 +34,9 @@ from nepheline_amd import physics as P  # noqa: E402
 from nepheline_amd import scenes as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
+# The network's forward flops per contact point: 2 (14*64 + 64*64 + 64*3).
+MLP_FLOPS_PER_ROW = 2 * (14 * 64 + 64 * 64 + 64 * 3)
 DT = 1.0 / 60.0
 SUBSTEPS = 4
 # Algorithmic bytes per contact-substep of the three constraint passes
@@ -126,6 +132,104 @@ def run_gpu(args, bank):
     }
 
 
+def residual_run(a, bank, args, ws=None, params=None):
+    """Mean and largest post-solve residual (m/s) and the largest penetration
+    over the headline's steps, reading the world's residual statistics after
+    every step (outside any timed region)."""
+    from nepheline_amd import learned as L
+    w = build(a, args.bodies, args.seed, bank)
+    if ws is None:
+        L.measure_residual(w, True)
+    else:
+        L.set_warm_start(w, ws, params)
+    total, count, worst, pen, learned, gated = 0.0, 0, 0.0, 0.0, 0, 0
+    for i in range(args.warmup + args.steps):
+        w.step(DT)
+        if i < args.warmup:
+            continue
+        s = L.warm_stats(w)
+        total += s.residual_sum_plain + s.residual_sum_learned
+        count += s.contacts_plain + s.contacts_learned
+        worst = max(worst, s.residual_max_plain, s.residual_max_learned)
+        pen = max(pen, s.penetration_max)
+        learned += s.contacts_learned
+        gated += s.gated
+    w.close()
+    return {"residual_mean": total / max(count, 1), "residual_max": worst,
+            "penetration_max": pen, "contacts_from_network": learned, "gated": gated}
+
+
+def run_learned(args, bank):
+    """The learned warm start: trained on the device from plain bin drops of
+    other seeds, then the headline's scene and steps timed with it, and the
+    residuals of both starts."""
+    from nepheline_amd import learned as L
+    a = P.product_api()
+    t0 = time.perf_counter()
+    tr = L.Trainer(a, capacity=len(args.train_seeds) * args.train_log, seed=1)
+    logged = L.log_bin_drops(a, tr, bank, args.train_seeds, args.bodies, args.train_steps,
+                             min_flag=1.0, log_capacity=args.train_log)
+    tr.prepare()
+    loss0 = tr.loss()
+    for e in range(args.train_epochs):
+        tr.epoch(8192, 2e-3 if e < args.train_epochs - 2 else 6e-4)
+    loss1 = tr.loss()
+    rows = tr.rows
+    ws, params = tr.warm_start(mode=L.NEW_POINTS, gate=True)
+    tr.close()
+    train_s = time.perf_counter() - t0
+    # Throughput with the network, same scene and steps as the headline.
+    a.lib.nph_amd_solved_contacts.restype = C.c_uint32
+    a.lib.nph_amd_solved_contacts.argtypes = [C.c_void_p]
+    w = build(a, args.bodies, args.seed, bank)
+    L.set_warm_start(w, ws, params)
+    for _ in range(args.warmup):
+        w.step(DT)
+    solved = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        w.step(DT)
+        solved += a.lib.nph_amd_solved_contacts(w.ptr)
+    wall = time.perf_counter() - t0
+    w.close()
+    plain = residual_run(a, bank, args)
+    learned = residual_run(a, bank, args, ws, params)
+    # The network's forward kernel alone on 4M rows (MFMA-bound).
+    n = 1 << 22
+    x = L.DeviceArray(a, shape=(n, L.INPUTS), dtype=np.uint16)
+    x.upload(np.zeros((n, L.INPUTS), np.uint16))
+    net = L.Network(a, params)
+    y = L.DeviceArray(a, shape=(n, L.OUTPUTS), dtype=np.float32)
+    f = net.f
+    reps = 20
+    for _ in range(3):
+        f.mlp_forward(x.ptr, n, net.params.ptr, y.ptr, None)
+    y.numpy()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f.mlp_forward(x.ptr, n, net.params.ptr, y.ptr, None)
+    y.numpy()
+    fwd_s = (time.perf_counter() - t0) / reps
+    flops = MLP_FLOPS_PER_ROW * n
+    return {
+        "value": solved * SUBSTEPS / wall,
+        "unit": "contact-substeps/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "mode": "points that carry no impulse start from the network; residual gate on",
+        "residual_plain": plain,
+        "residual_learned": learned,
+        "training": {"seeds": list(args.train_seeds), "steps_per_seed": args.train_steps,
+                     "rows_logged": logged, "rows_trained": rows,
+                     "epochs": args.train_epochs, "loss_before": loss0, "loss_after": loss1,
+                     "seconds": train_s},
+        "forward_roofline": {"bound": "mfma", "achieved": flops / fwd_s / 1e12,
+                             "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": flops / fwd_s / 1e12 / BF16_PEAK_TFLOPS,
+                             "rows": n, "flop_per_row": MLP_FLOPS_PER_ROW,
+                             "us_per_launch": fwd_s * 1e6},
+    }
+
+
 def run_cpu(args, bank):
     path = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
     if not os.path.exists(path):
@@ -185,6 +289,13 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=30,
                     help="steps after the timed region with per-launch kernel timing")
     ap.add_argument("--progress", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned warm start's training and runs")
+    ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101, 102, 103])
+    ap.add_argument("--train-steps", type=int, default=120)
+    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-log", type=int, default=8_000_000,
+                    help="contact-log rows kept per training world")
     ap.add_argument("--phases", action="store_true",
                     help="also report host and device time per step phase (reads stats "
                          "each step, which adds a contact readback to the timed region)")
@@ -215,6 +326,7 @@ def main():
     if os.path.exists(tpath):
         t = json.load(open(tpath))
         traffic = t["hbm_bytes_per_contact_substep"] * k_cs / launches
+    learned = None if args.no_learned else run_learned(args, bank)
     cpu = None if args.no_cpu else run_cpu(args, bank)
     line = {
         "metric": "contact solves/sec (contacts x substeps/s) plain vs learned warm start, "
@@ -253,6 +365,7 @@ def main():
                       "same scene, steps %d-%d of the timed window (%.1f s)" % (
                           cpu["cores"], args.warmup + 1, args.warmup + cpu["steps"], cpu["wall_s"]),
         },
+        "learned_warm_start": learned,
         "gpu_hash": g["hash"],
         "phases_ms_per_step": g["phases_ms"] if args.phases else None,
     }

@@ -646,6 +646,123 @@ void nph_amd_device_times(const NphPhysicsWorld *world, f64 out[3]);
 void nph_amd_time_kernels(NphPhysicsWorld *world, bool enable);
 void nph_amd_kernel_times(const NphPhysicsWorld *world, f64 out[2]);
 
+// ---------------------------------------------------------------------------
+// EXTENSION: the learned warm start's network (no counterpart in the
+// reference).  A 16 -> 64 -> 64 -> 16 MLP with GELU: 14 contact features in
+// (zero-padded to 16), the normal and two friction impulses out (columns
+// 0..2 of 4), BF16 weights and activations on the matrix cores with FP32
+// accumulation.  All pointers are device addresses; `stream` is a
+// hipStream_t (NULL: the null stream).  Parameters are one FP32 array of
+// NPH_AMD_MLP_PARAMS: W1 [16][64], b1 [64], W2 [64][64], b2 [64],
+// W3 [64][16], b3 [16], row-major, input index first.
+// ---------------------------------------------------------------------------
+#define NPH_AMD_MLP_FEATURES 14U
+#define NPH_AMD_MLP_INPUTS 16U
+#define NPH_AMD_MLP_OUTPUTS 3U
+#define NPH_AMD_MLP_PARAMS 6288U
+
+// y[r][0..3] = the network at x[r][0..15] (BF16), for r < rows.
+NphResult nph_amd_mlp_forward(const void *x_bf16, u32 rows, const f32 *params, f32 *y,
+                              void *stream);
+// Adds the gradient of scale * sum (y - target)^2 over rows and outputs
+// 0..2 (targets as y's layout, 4 floats a row) into grad[NPH_AMD_MLP_PARAMS],
+// and scale * sum (y - target)^2 into *loss.
+NphResult nph_amd_mlp_grad(const void *x_bf16, const f32 *target, u32 rows, const f32 *params,
+                           f32 scale, f32 *grad, f32 *loss, void *stream);
+// One Adam step (beta 0.9, 0.999, eps 1e-8) at `step` (from 1).
+NphResult nph_amd_mlp_adam(f32 *params, const f32 *grad, f32 *m, f32 *v, u32 step, f32 lr,
+                           void *stream);
+
+// Device memory for the network's callers (no counterpart in the
+// reference): allocation (NULL on failure), release, and a synchronous copy
+// between any two host or device addresses.
+void *nph_amd_device_alloc(u64 bytes);
+void nph_amd_device_free(void *p);
+NphResult nph_amd_device_copy(void *dst, const void *src, u64 bytes);
+
+// ---------------------------------------------------------------------------
+// EXTENSION: the learned warm start of a world (no counterpart in the
+// reference, whose warm start is the impulse carried by point id or place,
+// contact.c:296-374).  With a network set, the step computes every contact
+// point's 14 features after the constraint rows are prepared (approach speed
+// along the normal and both tangents, separation, normal, the two
+// inverse masses, friction, normal mass, the three tangent mass terms, in
+// that order), runs the network, and starts the chosen points from its
+// impulses: impulse = output * output_scale * normal mass, the normal one
+// clamped at zero and the friction ones to the friction circle.  The plain
+// solver then runs every substep as before.  With the gate on, a contact
+// whose post-solve residual exceeds the largest residual among the contacts
+// that started plainly that step is started plainly (carried impulses) for
+// the rest of its life.  (With NPH_AMD_WARM_ALL_POINTS, contact records whose
+// index is a multiple of 16 always start plainly: that control sample gives
+// the plain residual.)  The residual of a point is the relax pass's
+// condition at the step's end: with s the separation and
+// e = vn + (s > 0 ? s / h : 0), |e| if its normal impulse is positive, else
+// max(0, -e); a contact's is its points' largest.
+// ---------------------------------------------------------------------------
+#define NPH_AMD_WARM_NEW_POINTS 0U // points that carried no impulse
+#define NPH_AMD_WARM_ALL_POINTS 1U // every point
+
+typedef struct NphAmdWarmStart {
+    const f32 *params;        // host, NPH_AMD_MLP_PARAMS (copied)
+    f32 feature_shift[16];    // network input = (feature - shift) * scale
+    f32 feature_scale[16];
+    f32 output_scale[4];      // [0..2]: normal and two friction outputs
+    u32 mode;                 // NPH_AMD_WARM_*
+    bool gate;                // the residual gate
+} NphAmdWarmStart;
+
+// Sets (a copy of) the network, or with NULL returns to the plain warm start.
+NphResult nph_amd_world_set_warm_start(NphPhysicsWorld *world, const NphAmdWarmStart *ws);
+
+// Residual statistics of the last step (computed while a network is set or
+// measuring is enabled), over contacts (lanes) that started plainly and
+// from the network.
+typedef struct NphAmdWarmStats {
+    f32 residual_max_plain;
+    f32 residual_max_learned;
+    f32 residual_sum_plain;
+    f32 residual_sum_learned;
+    f32 penetration_max;
+    u32 contacts_plain;
+    u32 contacts_learned;
+    u32 gated; // contacts the gate handed to the plain warm start this step
+} NphAmdWarmStats;
+void nph_amd_world_measure_residual(NphPhysicsWorld *world, bool enable);
+NphResult nph_amd_world_warm_stats(const NphPhysicsWorld *world, NphAmdWarmStats *out);
+
+// The contact log, the network's training data: while enabled (capacity > 0
+// rows; 0 disables and frees it) every step appends, for each contact point,
+// its 16 features as above (slot 14 the normal impulse it carried, slot 15
+// one when it carried no impulse) and its 4 targets (the normal and two friction
+// impulses the step converged to, after restitution, then the same flag).
+// Rows past the capacity are dropped.  The arrays are device memory owned by
+// the world, valid until the log is disabled or the world destroyed.
+NphResult nph_amd_world_log_contacts(NphPhysicsWorld *world, u32 capacity);
+NphResult nph_amd_world_contact_log(const NphPhysicsWorld *world, u32 *rows,
+                                    const f32 **features, const f32 **targets);
+
+// Training on the device with the network's kernels: rows appended from
+// device arrays (a contact log), a normalisation fixed from them (inputs to
+// zero mean and unit deviation; targets = impulses / normal mass, scaled to
+// unit deviation), then epochs of minibatch Adam in a fresh pseudo-random
+// order each.
+typedef struct NphAmdTrainer NphAmdTrainer;
+NphResult nph_amd_trainer_create(u32 capacity, u64 seed, NphAmdTrainer **out);
+void nph_amd_trainer_destroy(NphAmdTrainer *t);
+// Only points whose flag (slot 15) is at least `min_flag` are kept.
+NphResult nph_amd_trainer_add(NphAmdTrainer *t, const f32 *features, const f32 *targets,
+                              u32 rows, f32 min_flag);
+u32 nph_amd_trainer_rows(const NphAmdTrainer *t);
+NphResult nph_amd_trainer_prepare(NphAmdTrainer *t);
+// One epoch; *mean_loss = its batches' mean loss (normalised units).
+NphResult nph_amd_trainer_epoch(NphAmdTrainer *t, u32 batch, f32 lr, f32 *mean_loss);
+// The loss over every row.
+NphResult nph_amd_trainer_loss(NphAmdTrainer *t, f32 *loss);
+// The trained network: its parameters into params (host,
+// NPH_AMD_MLP_PARAMS), the rest into *out (out->params = params).
+NphResult nph_amd_trainer_warm_start(const NphAmdTrainer *t, f32 *params, NphAmdWarmStart *out);
+
 #ifdef __cplusplus
 }
 #endif

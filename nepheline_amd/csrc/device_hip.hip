@@ -17,6 +17,7 @@
 
 #include "nph_device.h"
 #include "nph_joint.h"
+#include "nph_learned.h"
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -127,6 +128,25 @@ struct NphDevice {
     u64 cap_h_fast_count;
     NphFastBody *h_fast_pack;
     u64 cap_h_fast_pack;
+    // The learned warm start (off unless a network is set), residual
+    // measurement and the contact log.
+    int learn_on;
+    int measure;
+    u32 learn_mode;
+    int gate_on;
+    f32 shift[16], scale[16], out_scale[4];
+    f32 *mlp_params;     // NPH_AMD_MLP_PARAMS
+    u16 *lx;             // lanes * 4 network rows of 16 BF16
+    f32 *lraw;           // lanes * 4 rows of 16 raw features
+    f32 *ly;             // lanes * 4 rows of 4 outputs
+    f32 *lres;           // per lane residual
+    u32 cap_learn_lanes;
+    u32 *gate;           // per contact record: 1 = plain warm start for good
+    u32 *wstats;         // NPH_WSTATS words
+    u32 *h_wstats;
+    f32 *log_x, *log_t;  // log rows: 16 features, 4 targets
+    u32 *log_count;
+    u32 log_cap;
 };
 
 const char *nph_dev_name(void) { return "hip:gfx950"; }
@@ -200,7 +220,8 @@ void nph_dev_destroy(NphDevice *d)
                     d->sbodies, d->params, d->list, d->codes, d->inits, d->groups,
                     d->lane_contact, d->lane_body_a, d->lane_body_b, d->lane_flags, d->lanes,
                     d->points, d->grown, d->grown_blocks, d->joint_conf, d->joint_state, d->step_joints,
-                    d->fast_list, d->fast_pack};
+                    d->fast_list, d->fast_pack, d->gate, d->mlp_params, d->lx, d->lraw, d->ly,
+                    d->lres, d->wstats, d->log_x, d->log_t, d->log_count};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
             (void)hipFree(bufs[i]);
@@ -222,8 +243,9 @@ void nph_dev_destroy(NphDevice *d)
     if (d->h_u32 != NULL) {
         (void)hipHostFree(d->h_u32);
     }
-    void *pinned[] = {d->h_list, d->h_codes, d->h_inits, d->h_fast_count, d->h_fast_pack};
-    for (u32 i = 0U; i < 5U; ++i) {
+    void *pinned[] = {d->h_list, d->h_codes, d->h_inits, d->h_fast_count, d->h_fast_pack,
+                      d->h_wstats};
+    for (u32 i = 0U; i < 6U; ++i) {
         if (pinned[i] != NULL) {
             (void)hipHostFree(pinned[i]);
         }
@@ -298,6 +320,9 @@ NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts)
         const u32 c = grow_cap(d->cap_contacts, contacts);
         NphResult r = grow_buffer((void **)&d->contacts, (u64)d->cap_contacts * sizeof(NphContactRec),
                                   (u64)c * sizeof(NphContactRec), d->stream, 1);
+        if (r == NPH_RESULT_OK) {
+            r = grow_buffer((void **)&d->gate, (u64)d->cap_contacts * 4U, (u64)c * 4U, d->stream, 1);
+        }
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -609,7 +634,7 @@ NphResult nph_dev_read_contacts(NphDevice *d, NphContactRec *out, u32 count)
 // Kernels
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(NPH_BLOCK) k_init_contacts(NphContactRec *contacts, const NphContactInit *list, u32 n)
+__global__ void __launch_bounds__(NPH_BLOCK) k_init_contacts(NphContactRec *contacts, u32 *gate, const NphContactInit *list, u32 n)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) {
@@ -617,6 +642,7 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_init_contacts(NphContactRec *cont
     }
     const NphContactInit in = list[i];
     NphContactRec *c = &contacts[in.record];
+    gate[in.record] = 0U;
     memset(c, 0, sizeof *c);
     c->shape_a = in.shape_a;
     c->shape_b = in.shape_b;
@@ -1009,7 +1035,289 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_fast_pack(NphStepData d, const u3
     }
 }
 
+// ---------------------------------------------------------------------------
+// The learned warm start
+// ---------------------------------------------------------------------------
+
+// Statistics words: maxima as the bits of non-negative floats (ordered as
+// u32), sums as floats, counts; the gate's threshold persists across steps.
+enum {
+    WS_MAX_PLAIN = 0,
+    WS_MAX_LEARNED,
+    WS_COUNT_PLAIN,
+    WS_COUNT_LEARNED,
+    WS_GATED,
+    WS_MAX_PEN,
+    WS_SUM_PLAIN,
+    WS_SUM_LEARNED,
+    WS_CLEARED, // the words above are cleared each step
+    WS_THRESHOLD = WS_CLEARED,
+    NPH_WSTATS = 16
+};
+
+typedef struct NphLearnArgs {
+    u16 *x;          // network rows (NULL: no inference)
+    f32 *raw;        // raw feature rows (NULL: no log)
+    const f32 *y;    // network outputs
+    f32 *res;        // per lane residual
+    u32 *gate;       // per contact record
+    u32 *stats;
+    f32 *log_x;
+    f32 *log_t;
+    u32 *log_count;
+    u32 log_cap;
+    u32 mode;
+    u32 gate_on;
+    u32 group_count;
+    f32 inv_h;
+    f32 shift[16];
+    f32 scale[16];
+    f32 out_scale[4];
+} NphLearnArgs;
+
+__device__ static inline u16 bf16_bits(f32 f)
+{
+    u32 u = __float_as_uint(f); // round to nearest even (finite values)
+    u += 0x7FFFU + ((u >> 16U) & 1U);
+    return (u16)(u >> 16U);
+}
+
+__device__ static inline int lane_active(const NphStepData *d, const NphLearnArgs *a, u32 L,
+                                         const NphContactRec **c)
+{
+    const u32 g = L / NPH_LANES;
+    if (g >= a->group_count || L % NPH_LANES >= d->groups[g].lane_count) {
+        return 0;
+    }
+    *c = &d->contacts[d->lane_contact[L]];
+    return 1;
+}
+
+// Each lane's four network rows (zero where there is no point) and, when
+// logging, its points' raw features and carried-nothing flags.
+__global__ void __launch_bounds__(NPH_BLOCK) k_learned_features(NphStepData d, NphLearnArgs a)
+{
+    const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= a.group_count * NPH_LANES) {
+        return;
+    }
+    const NphContactRec *c = NULL;
+    const u32 points = lane_active(&d, &a, L, &c) ? c->point_count : 0U;
+    const u32 g = L / NPH_LANES;
+    const u32 l = L % NPH_LANES;
+    for (u32 k = 0U; k < NPH_LEARNED_ROWS_PER_LANE; ++k) {
+        const u64 row = (u64)L * NPH_LEARNED_ROWS_PER_LANE + k;
+        f32 f[NPH_F_COUNT];
+        f32 fresh = 0.0F;
+        f32 carried = 0.0F;
+        if (k < points) {
+            learned_features(&d, g, l, k, f);
+            const u64 R = (u64)(d.groups[g].first_point + k) * NPH_LANES + l;
+            carried = PF(&d, NPH_P_NIMP, R);
+            fresh = (carried == 0.0F && PF(&d, NPH_P_TI1, R) == 0.0F &&
+                     PF(&d, NPH_P_TI2, R) == 0.0F)
+                        ? 1.0F
+                        : 0.0F;
+        } else {
+            for (u32 i = 0U; i < NPH_F_COUNT; ++i) {
+                f[i] = 0.0F;
+            }
+        }
+        if (a.x != NULL) {
+            u16 *xr = a.x + row * NPH_LEARNED_IN;
+            for (u32 i = 0U; i < NPH_LEARNED_IN; ++i) {
+                xr[i] = (i < NPH_F_COUNT && k < points)
+                            ? bf16_bits((f[i] - a.shift[i]) * a.scale[i])
+                            : (u16)0U;
+            }
+        }
+        if (a.raw != NULL) {
+            f32 *fr = a.raw + row * NPH_LEARNED_IN;
+            for (u32 i = 0U; i < NPH_F_COUNT; ++i) {
+                fr[i] = f[i];
+            }
+            fr[14] = carried;
+            fr[15] = fresh;
+        }
+    }
+}
+
+// The network's impulses into the chosen points of every lane the gate
+// leaves to it; such lanes are flagged.
+__global__ void __launch_bounds__(NPH_BLOCK) k_learned_apply(NphStepData d, NphLearnArgs a)
+{
+    const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
+    const NphContactRec *c = NULL;
+    if (L >= a.group_count * NPH_LANES || !lane_active(&d, &a, L, &c)) {
+        return;
+    }
+    const u32 record = d.lane_contact[L];
+    if (a.gate_on != 0U && a.gate[record] != 0U) {
+        return;
+    }
+    // With every point taken by the network, one contact record in
+    // NPH_LEARNED_CONTROL stays plain: the sample the gate measures the
+    // plain solver's residual on.
+    if (a.mode == NPH_AMD_WARM_ALL_POINTS && record % NPH_LEARNED_CONTROL == 0U) {
+        return;
+    }
+    const u32 g = L / NPH_LANES;
+    const u32 l = L % NPH_LANES;
+    const f32 friction = LF(&d, NPH_L_FRICTION, L);
+    u32 any = 0U;
+    for (u32 k = 0U; k < c->point_count; ++k) {
+        const u64 R = (u64)(d.groups[g].first_point + k) * NPH_LANES + l;
+        const int fresh = PF(&d, NPH_P_NIMP, R) == 0.0F && PF(&d, NPH_P_TI1, R) == 0.0F &&
+                          PF(&d, NPH_P_TI2, R) == 0.0F;
+        if (a.mode == NPH_AMD_WARM_ALL_POINTS || fresh) {
+            const f32 *y = a.y + ((u64)L * NPH_LEARNED_ROWS_PER_LANE + k) * NPH_LEARNED_OUT;
+            f32 n, t1, t2;
+            learned_impulses(y, a.out_scale, PF(&d, NPH_P_NMASS, R), friction, &n, &t1, &t2);
+            PF(&d, NPH_P_NIMP, R) = n;
+            PF(&d, NPH_P_TI1, R) = t1;
+            PF(&d, NPH_P_TI2, R) = t2;
+            any = 1U;
+        }
+    }
+    if (any != 0U) {
+        d.lane_flags[L] |= NPH_LANE_LEARNED;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_learned_clear(u32 *stats)
+{
+    if (threadIdx.x < WS_CLEARED) {
+        stats[threadIdx.x] = 0U;
+    }
+}
+
+// Every lane's residual and penetration after the step, into the
+// statistics by how the lane started: reduced across the wave first, so
+// each wave adds to the shared words once.
+__device__ static inline f32 wave_max(f32 v)
+{
+    for (u32 o = 32U; o > 0U; o >>= 1U) {
+        v = nph_maxf(v, __shfl_xor(v, (int)o, 64));
+    }
+    return v;
+}
+
+__device__ static inline f32 wave_sum(f32 v)
+{
+    for (u32 o = 32U; o > 0U; o >>= 1U) {
+        v += __shfl_xor(v, (int)o, 64);
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(NPH_BLOCK) k_learned_residual(NphStepData d, NphLearnArgs a)
+{
+    const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
+    const NphContactRec *c = NULL;
+    f32 res = 0.0F;
+    f32 pen = 0.0F;
+    int active = 0;
+    int learned = 0;
+    if (L < a.group_count * NPH_LANES && lane_active(&d, &a, L, &c)) {
+        learned_residual(&d, a.inv_h, L / NPH_LANES, L % NPH_LANES, c, &res, &pen);
+        a.res[L] = res;
+        active = 1;
+        learned = (d.lane_flags[L] & NPH_LANE_LEARNED) != 0U;
+    }
+    const f32 rp = active && !learned ? res : 0.0F;
+    const f32 rl = active && learned ? res : 0.0F;
+    const f32 max_p = wave_max(rp);
+    const f32 max_l = wave_max(rl);
+    const f32 sum_p = wave_sum(rp);
+    const f32 sum_l = wave_sum(rl);
+    const f32 max_pen = wave_max(active ? nph_maxf(pen, 0.0F) : 0.0F);
+    const u64 bp = __ballot(active && !learned);
+    const u64 bl = __ballot(active && learned);
+    if ((threadIdx.x & 63U) == 0U && (bp | bl) != 0U) {
+        atomicMax(&a.stats[WS_MAX_PLAIN], __float_as_uint(max_p));
+        atomicMax(&a.stats[WS_MAX_LEARNED], __float_as_uint(max_l));
+        atomicAdd(&a.stats[WS_COUNT_PLAIN], (u32)__popcll(bp));
+        atomicAdd(&a.stats[WS_COUNT_LEARNED], (u32)__popcll(bl));
+        atomicAdd((f32 *)&a.stats[WS_SUM_PLAIN], sum_p);
+        atomicAdd((f32 *)&a.stats[WS_SUM_LEARNED], sum_l);
+        atomicMax(&a.stats[WS_MAX_PEN], __float_as_uint(max_pen));
+    }
+}
+
+// The gate's threshold: the step's largest plain residual, or the last
+// one while no contact started plainly.
+__global__ void __launch_bounds__(64) k_learned_threshold(u32 *stats)
+{
+    if (threadIdx.x == 0U && stats[WS_COUNT_PLAIN] > 0U) {
+        stats[WS_THRESHOLD] = stats[WS_MAX_PLAIN];
+    }
+}
+
+__global__ void __launch_bounds__(NPH_BLOCK) k_learned_gate(NphStepData d, NphLearnArgs a)
+{
+    const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
+    const NphContactRec *c = NULL;
+    if (L >= a.group_count * NPH_LANES || !lane_active(&d, &a, L, &c) ||
+        (d.lane_flags[L] & NPH_LANE_LEARNED) == 0U) {
+        return;
+    }
+    if (a.res[L] > __uint_as_float(a.stats[WS_THRESHOLD])) {
+        a.gate[d.lane_contact[L]] = 1U;
+        atomicAdd(&a.stats[WS_GATED], 1U);
+    }
+}
+
+// Appends every point's raw features and converged impulses to the log
+// (one counter update a wave).
+__global__ void __launch_bounds__(NPH_BLOCK) k_learned_log(NphStepData d, NphLearnArgs a)
+{
+    const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
+    const NphContactRec *c = NULL;
+    const u32 points =
+        (L < a.group_count * NPH_LANES && lane_active(&d, &a, L, &c)) ? c->point_count : 0U;
+    // Inclusive scan of the points over the wave.
+    const u32 lane = threadIdx.x & 63U;
+    u32 incl = points;
+    for (u32 o = 1U; o < 64U; o <<= 1U) {
+        const u32 v = (u32)__shfl_up((int)incl, o, 64);
+        if (lane >= o) {
+            incl += v;
+        }
+    }
+    const u32 total = (u32)__shfl((int)incl, 63, 64);
+    u32 base = 0U;
+    if (lane == 63U && total > 0U) {
+        base = atomicAdd(a.log_count, total);
+    }
+    base = (u32)__shfl((int)base, 63, 64);
+    if (points == 0U) {
+        return;
+    }
+    const u32 g = L / NPH_LANES;
+    const u32 l = L % NPH_LANES;
+    for (u32 k = 0U; k < points; ++k) {
+        const u32 at = base + incl - points + k;
+        if (at >= a.log_cap) {
+            return;
+        }
+        const u64 R = (u64)(d.groups[g].first_point + k) * NPH_LANES + l;
+        const f32 *fr = a.raw + ((u64)L * NPH_LEARNED_ROWS_PER_LANE + k) * NPH_LEARNED_IN;
+        f32 *xo = a.log_x + (u64)at * NPH_LEARNED_IN;
+        for (u32 i = 0U; i < NPH_LEARNED_IN; ++i) {
+            xo[i] = fr[i];
+        }
+        f32 *to = a.log_t + (u64)at * NPH_LEARNED_OUT;
+        to[0] = PF(&d, NPH_P_NIMP, R);
+        to[1] = PF(&d, NPH_P_TI1, R);
+        to[2] = PF(&d, NPH_P_TI2, R);
+        to[3] = fr[15];
+    }
+}
+
 static u32 blocks_for(u32 n) { return (n + NPH_BLOCK - 1U) / NPH_BLOCK; }
+
+static NphResult learned_before(NphDevice *d, const NphStepData *sd, u32 groups, f32 inv_h);
+static NphResult learned_after(NphDevice *d, const NphStepData *sd, u32 groups, f32 inv_h);
 
 // A pinned host buffer of at least `bytes`; the stream is drained before an
 // old one is freed (a queued copy may still read it).
@@ -1065,7 +1373,7 @@ NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 co
     HIP_TRY(hipMemcpyAsync(d->inits, d->h_inits, (u64)count * sizeof *list, hipMemcpyHostToDevice,
                            d->stream));
     hipLaunchKernelGGL(k_init_contacts, dim3(blocks_for(count)), dim3(NPH_BLOCK), 0, d->stream,
-                       d->contacts, d->inits, count);
+                       d->contacts, d->gate, d->inits, count);
     HIP_TRY(hipGetLastError());
     return NPH_RESULT_OK;
 }
@@ -1301,6 +1609,10 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody
                            groups);
     }
     HIP_TRY(hipGetLastError());
+    r = learned_before(d, &sd, groups, sp.inv_h);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
     // One constraint pass: the overflow first, then colours 0..23 (solver.c:774-789).
     u32 timed = 0U; // colour-pass launches bracketed by events this step
 #define PASS(kind)                                                                                 \
@@ -1366,6 +1678,10 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody
                            groups);
     }
 #undef PASS
+    r = learned_after(d, &sd, groups, sp.inv_h);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd, sp);
     HIP_TRY(hipGetLastError());
     d->pending_sd = sd;
@@ -1471,5 +1787,223 @@ NphResult nph_dev_solve_finish(NphDevice *d, const NphGrown **grown, u32 *grown_
     d->passes_ms = ms;
     *grown = d->h_grown + 1;
     *grown_count = m;
+    return NPH_RESULT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The learned warm start: configuration, statistics, the contact log
+// ---------------------------------------------------------------------------
+
+static NphResult reserve_stats(NphDevice *d)
+{
+    if (d->wstats != NULL) {
+        return NPH_RESULT_OK;
+    }
+    HIP_TRY(hipMalloc((void **)&d->wstats, NPH_WSTATS * 4U));
+    HIP_TRY(hipMemsetAsync(d->wstats, 0, NPH_WSTATS * 4U, d->stream));
+    const u32 inf = 0x7F800000U; // no threshold until some contact starts plainly
+    HIP_TRY(hipMemcpyAsync(d->wstats + WS_THRESHOLD, &inf, 4U, hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipHostMalloc((void **)&d->h_wstats, NPH_WSTATS * 4U, hipHostMallocDefault));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_dev_set_warm_start(NphDevice *d, const NphAmdWarmStart *ws)
+{
+    if (ws == NULL) {
+        d->learn_on = 0;
+        return NPH_RESULT_OK;
+    }
+    if (ws->params == NULL || ws->mode > NPH_AMD_WARM_ALL_POINTS) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    if (d->mlp_params == NULL) {
+        HIP_TRY(hipMalloc((void **)&d->mlp_params, NPH_AMD_MLP_PARAMS * 4U));
+    }
+    NphResult r = reserve_stats(d);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    HIP_TRY(hipMemcpyAsync(d->mlp_params, ws->params, NPH_AMD_MLP_PARAMS * 4U,
+                           hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    memcpy(d->shift, ws->feature_shift, sizeof d->shift);
+    memcpy(d->scale, ws->feature_scale, sizeof d->scale);
+    memcpy(d->out_scale, ws->output_scale, sizeof d->out_scale);
+    d->learn_mode = ws->mode;
+    d->gate_on = ws->gate ? 1 : 0;
+    d->learn_on = 1;
+    return NPH_RESULT_OK;
+}
+
+void nph_dev_measure_residual(NphDevice *d, int enable) { d->measure = enable; }
+
+NphResult nph_dev_warm_stats(NphDevice *d, NphAmdWarmStats *out)
+{
+    memset(out, 0, sizeof *out);
+    if (d->wstats == NULL) {
+        return NPH_RESULT_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(d->h_wstats, d->wstats, NPH_WSTATS * 4U, hipMemcpyDeviceToHost,
+                           d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    const u32 *w = d->h_wstats;
+    f32 f;
+    memcpy(&f, &w[WS_MAX_PLAIN], 4U);
+    out->residual_max_plain = f;
+    memcpy(&f, &w[WS_MAX_LEARNED], 4U);
+    out->residual_max_learned = f;
+    memcpy(&f, &w[WS_SUM_PLAIN], 4U);
+    out->residual_sum_plain = f;
+    memcpy(&f, &w[WS_SUM_LEARNED], 4U);
+    out->residual_sum_learned = f;
+    memcpy(&f, &w[WS_MAX_PEN], 4U);
+    out->penetration_max = f;
+    out->contacts_plain = w[WS_COUNT_PLAIN];
+    out->contacts_learned = w[WS_COUNT_LEARNED];
+    out->gated = w[WS_GATED];
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_dev_log_contacts(NphDevice *d, u32 capacity)
+{
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    void *old[3] = {d->log_x, d->log_t, d->log_count};
+    for (u32 i = 0U; i < 3U; ++i) {
+        if (old[i] != NULL) {
+            HIP_TRY(hipFree(old[i]));
+        }
+    }
+    d->log_x = NULL;
+    d->log_t = NULL;
+    d->log_count = NULL;
+    d->log_cap = 0U;
+    if (capacity == 0U) {
+        return NPH_RESULT_OK;
+    }
+    HIP_TRY(hipMalloc((void **)&d->log_x, (u64)capacity * NPH_LEARNED_IN * 4U));
+    HIP_TRY(hipMalloc((void **)&d->log_t, (u64)capacity * NPH_LEARNED_OUT * 4U));
+    HIP_TRY(hipMalloc((void **)&d->log_count, 4U));
+    HIP_TRY(hipMemsetAsync(d->log_count, 0, 4U, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    d->log_cap = capacity;
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_dev_contact_log(NphDevice *d, u32 *rows, const f32 **features, const f32 **targets)
+{
+    *rows = 0U;
+    *features = d->log_x;
+    *targets = d->log_t;
+    if (d->log_cap == 0U) {
+        return NPH_RESULT_OK;
+    }
+    u32 n = 0U;
+    HIP_TRY(hipMemcpyAsync(&n, d->log_count, 4U, hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    *rows = n < d->log_cap ? n : d->log_cap;
+    return NPH_RESULT_OK;
+}
+
+// The per-lane buffers of the learned kernels for `lanes` lanes.
+static NphResult reserve_learn(NphDevice *d, u32 lanes)
+{
+    NphResult r = reserve_stats(d);
+    if (r != NPH_RESULT_OK || lanes <= d->cap_learn_lanes) {
+        return r;
+    }
+    const u32 c = grow_cap(d->cap_learn_lanes, lanes);
+    const u64 rows = (u64)c * NPH_LEARNED_ROWS_PER_LANE;
+    if ((r = grow_buffer((void **)&d->lx, 0, rows * NPH_LEARNED_IN * 2U, d->stream, 0)) !=
+            NPH_RESULT_OK ||
+        (r = grow_buffer((void **)&d->lraw, 0, rows * NPH_LEARNED_IN * 4U, d->stream, 0)) !=
+            NPH_RESULT_OK ||
+        (r = grow_buffer((void **)&d->ly, 0, rows * NPH_LEARNED_OUT * 4U, d->stream, 0)) !=
+            NPH_RESULT_OK ||
+        (r = grow_buffer((void **)&d->lres, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK) {
+        return r;
+    }
+    d->cap_learn_lanes = c;
+    return NPH_RESULT_OK;
+}
+
+static NphLearnArgs learn_args(const NphDevice *d, u32 groups, f32 inv_h)
+{
+    NphLearnArgs a;
+    memset(&a, 0, sizeof a);
+    a.x = d->learn_on ? d->lx : NULL;
+    a.raw = d->log_cap > 0U ? d->lraw : NULL;
+    a.y = d->ly;
+    a.res = d->lres;
+    a.gate = d->gate;
+    a.stats = d->wstats;
+    a.log_x = d->log_x;
+    a.log_t = d->log_t;
+    a.log_count = d->log_count;
+    a.log_cap = d->log_cap;
+    a.mode = d->learn_mode;
+    a.gate_on = d->gate_on ? 1U : 0U;
+    a.group_count = groups;
+    a.inv_h = inv_h;
+    memcpy(a.shift, d->shift, sizeof a.shift);
+    memcpy(a.scale, d->scale, sizeof a.scale);
+    memcpy(a.out_scale, d->out_scale, sizeof a.out_scale);
+    return a;
+}
+
+// After the lanes are prepared: features, the network and its impulses.
+static NphResult learned_before(NphDevice *d, const NphStepData *sd, u32 groups, f32 inv_h)
+{
+    if (groups == 0U || (!d->learn_on && d->log_cap == 0U && !d->measure)) {
+        return NPH_RESULT_OK;
+    }
+    const u32 lanes = groups * NPH_LANES;
+    NphResult r = reserve_learn(d, lanes);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    const NphLearnArgs a = learn_args(d, groups, inv_h);
+    hipStream_t s = d->stream;
+    if (a.x != NULL || a.raw != NULL) {
+        hipLaunchKernelGGL(k_learned_features, dim3(blocks_for(lanes)), dim3(NPH_BLOCK), 0, s, *sd,
+                           a);
+        HIP_TRY(hipGetLastError());
+    }
+    if (d->learn_on) {
+        r = nph_amd_mlp_forward(d->lx, lanes * NPH_LEARNED_ROWS_PER_LANE, d->mlp_params, d->ly,
+                                (void *)s);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        hipLaunchKernelGGL(k_learned_apply, dim3(blocks_for(lanes)), dim3(NPH_BLOCK), 0, s, *sd,
+                           a);
+        HIP_TRY(hipGetLastError());
+    }
+    return NPH_RESULT_OK;
+}
+
+// After the store: residuals, the gate, the log.
+static NphResult learned_after(NphDevice *d, const NphStepData *sd, u32 groups, f32 inv_h)
+{
+    if (groups == 0U || (!d->learn_on && d->log_cap == 0U && !d->measure)) {
+        return NPH_RESULT_OK;
+    }
+    const u32 lanes = groups * NPH_LANES;
+    const NphLearnArgs a = learn_args(d, groups, inv_h);
+    hipStream_t s = d->stream;
+    if (d->learn_on || d->measure) {
+        hipLaunchKernelGGL(k_learned_clear, dim3(1), dim3(64), 0, s, d->wstats);
+        hipLaunchKernelGGL(k_learned_residual, dim3(blocks_for(lanes)), dim3(NPH_BLOCK), 0, s, *sd,
+                           a);
+        if (d->learn_on && d->gate_on) {
+            hipLaunchKernelGGL(k_learned_threshold, dim3(1), dim3(64), 0, s, d->wstats);
+            hipLaunchKernelGGL(k_learned_gate, dim3(blocks_for(lanes)), dim3(NPH_BLOCK), 0, s, *sd,
+                               a);
+        }
+    }
+    if (d->log_cap > 0U) {
+        hipLaunchKernelGGL(k_learned_log, dim3(blocks_for(lanes)), dim3(NPH_BLOCK), 0, s, *sd, a);
+    }
+    HIP_TRY(hipGetLastError());
     return NPH_RESULT_OK;
 }

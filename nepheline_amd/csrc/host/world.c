@@ -333,6 +333,44 @@ void nph_amd_time_kernels(NphPhysicsWorld *w, bool enable) { nph_dev_time_kernel
 
 void nph_amd_kernel_times(const NphPhysicsWorld *w, f64 out[2]) { nph_dev_kernel_times(w->dev, out); }
 
+NphResult nph_amd_world_set_warm_start(NphPhysicsWorld *w, const NphAmdWarmStart *ws)
+{
+    if (w == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    return nph_dev_set_warm_start(w->dev, ws);
+}
+
+void nph_amd_world_measure_residual(NphPhysicsWorld *w, bool enable)
+{
+    nph_dev_measure_residual(w->dev, enable ? 1 : 0);
+}
+
+NphResult nph_amd_world_warm_stats(const NphPhysicsWorld *w, NphAmdWarmStats *out)
+{
+    if (w == NULL || out == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    return nph_dev_warm_stats(w->dev, out);
+}
+
+NphResult nph_amd_world_log_contacts(NphPhysicsWorld *w, u32 capacity)
+{
+    if (w == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    return nph_dev_log_contacts(w->dev, capacity);
+}
+
+NphResult nph_amd_world_contact_log(const NphPhysicsWorld *w, u32 *rows, const f32 **features,
+                                    const f32 **targets)
+{
+    if (w == NULL || rows == NULL || features == NULL || targets == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    return nph_dev_contact_log(w->dev, rows, features, targets);
+}
+
 // ---------------------------------------------------------------------------
 // Bodies (body.c:12-130, :392-448)
 // ---------------------------------------------------------------------------

@@ -112,6 +112,15 @@ f64 nph_dev_last_passes_ms(const NphDevice *d);
 void nph_dev_time_kernels(NphDevice *d, int enable);
 void nph_dev_kernel_times(const NphDevice *d, f64 out[2]);
 
+// The learned warm start: the network (NULL: off), residual measurement,
+// its statistics of the last solve, and the contact log.
+NphResult nph_dev_set_warm_start(NphDevice *d, const NphAmdWarmStart *ws);
+void nph_dev_measure_residual(NphDevice *d, int enable);
+NphResult nph_dev_warm_stats(NphDevice *d, NphAmdWarmStats *out);
+NphResult nph_dev_log_contacts(NphDevice *d, u32 capacity);
+NphResult nph_dev_contact_log(NphDevice *d, u32 *rows, const f32 **features,
+                              const f32 **targets);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,7 @@ def rotation(r: Pcg32):
 
 
 DESC = {"enable_sleep": 0, "enable_continuous": 0}
+DT = 1.0 / 60.0
 
 
 def pyramid(api, rows: int = 20, desc: dict | None = None) -> P.World:
