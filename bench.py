@@ -166,7 +166,7 @@ def run_learned(args, bank):
     from nepheline_amd import learned as L
     a = P.product_api()
     t0 = time.perf_counter()
-    tr = L.Trainer(a, capacity=len(args.train_seeds) * args.train_log, seed=1)
+    tr = L.Trainer(a, capacity=args.train_capacity, seed=1)
     logged = L.log_bin_drops(a, tr, bank, args.train_seeds, args.bodies, args.train_steps,
                              min_flag=1.0, log_capacity=args.train_log)
     tr.prepare()
@@ -291,11 +291,13 @@ def main():
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned warm start's training and runs")
-    ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101, 102, 103])
-    ap.add_argument("--train-steps", type=int, default=120)
+    ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101])
+    ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-log", type=int, default=8_000_000,
-                    help="contact-log rows kept per training world")
+                    help="contact-log rows a training world holds between flushes")
+    ap.add_argument("--train-capacity", type=int, default=40_000_000,
+                    help="rows the trainer holds")
     ap.add_argument("--phases", action="store_true",
                     help="also report host and device time per step phase (reads stats "
                          "each step, which adds a contact readback to the timed region)")

@@ -9,9 +9,10 @@
 //
 // The step runs the reference's phases in its order: new pairs (host tree),
 // the narrow phase (device), the changes it found (host), the solver and
-// finalize (device), the proxies that grew (host).  The world keeps sleep
-// and continuous collision off (the reference's throughput setting);
-// asking for either, or for the TGS or XPBD solver, is NPH_RESULT_UNSUPPORTED.
+// finalize (device), continuous sweeps of the fast bodies (host), the
+// proxies that grew (host).  The world keeps sleep off (the reference's
+// throughput setting); asking for it, or for the TGS or XPBD solver, is
+// NPH_RESULT_UNSUPPORTED.
 #define _POSIX_C_SOURCE 200809L
 #include "world_impl.h"
 

@@ -268,15 +268,18 @@ def download_log(world: P.World):
 
 
 def log_bin_drops(a: P.Api, trainer: Trainer, bank, seeds, bodies, steps, min_flag=0.0,
-                  log_capacity=4_000_000, progress=None):
-    """Runs plain bin drops, appending each one's contact log to the trainer."""
+                  log_capacity=4_000_000, flush_every=50, progress=None):
+    """Runs plain bin drops, appending their contact logs to the trainer
+    every `flush_every` steps (the log restarts empty after each)."""
     total = 0
     for seed in seeds:
         w = S.bin_drop(a, bodies, seed, bank)
         log_contacts(w, log_capacity)
-        for _ in range(steps):
+        for i in range(steps):
             w.step(S.DT)
-        total += trainer.add_log(w, min_flag)
+            if (i + 1) % flush_every == 0 or i + 1 == steps:
+                total += trainer.add_log(w, min_flag)
+                log_contacts(w, log_capacity)
         w.close()
         if progress:
             progress(f"logged seed {seed}: {trainer.rows} rows kept of {total}")
@@ -284,11 +287,13 @@ def log_bin_drops(a: P.Api, trainer: Trainer, bank, seeds, bodies, steps, min_fl
 
 
 def train(a: P.Api, bank, seeds=(100, 101), bodies=2000, steps=120, epochs=6, batch=4096,
-          lr=2e-3, min_flag=1.0, seed=0, progress=None):
+          lr=2e-3, min_flag=1.0, seed=0, capacity=None, log_capacity=4_000_000,
+          progress=None):
     """Logs plain bin drops and trains a network on their converged impulses.
     Returns (WarmStart, params, history of epoch losses)."""
-    tr = Trainer(a, capacity=len(seeds) * 4_000_000, seed=seed)
-    log_bin_drops(a, tr, bank, seeds, bodies, steps, min_flag=min_flag, progress=progress)
+    tr = Trainer(a, capacity=capacity or len(seeds) * 4_000_000, seed=seed)
+    log_bin_drops(a, tr, bank, seeds, bodies, steps, min_flag=min_flag,
+                  log_capacity=log_capacity, progress=progress)
     tr.prepare()
     hist = [tr.loss()]
     for e in range(epochs):
