@@ -163,7 +163,46 @@ typedef struct NphPhysicsHull {
     u32 reserved[2];
 } NphPhysicsHull; // 128 B (hull.h:78)
 
-typedef struct NphPhysicsMesh NphPhysicsMesh;               // mesh.h (not supported: rejected)
+// ---------------------------------------------------------------------------
+// Mesh blocks (physics/mesh.h:44-150): caller-owned, immutable, pointer-free;
+// the same bytes the reference's nph_physics_mesh_build writes (a four-wide
+// tree over triangles in leaf order, active edges, input indices).
+// ---------------------------------------------------------------------------
+#define NPH_PHYSICS_MESH_MAX_TRIANGLES  (1U << 28U)
+#define NPH_PHYSICS_MESH_LEAF_TRIANGLES 4U
+#define NPH_PHYSICS_MESH_EDGE_0         (1U << 0U)
+#define NPH_PHYSICS_MESH_EDGE_1         (1U << 1U)
+#define NPH_PHYSICS_MESH_EDGE_2         (1U << 2U)
+#define NPH_PHYSICS_MESH_EDGES          7U
+#define NPH_PHYSICS_MESH_TWO_SIDED      (1U << 0U)
+#define NPH_PHYSICS_MESH_CHILD_LEAF     0x80000000U
+#define NPH_PHYSICS_MESH_CHILD_EMPTY    0xFFFFFFFFU
+#define NPH_PHYSICS_MESH_MAX_MANIFOLDS  4U // collide.h:152
+
+typedef struct NphPhysicsMeshNode {
+    f32 min_x[4];
+    f32 min_y[4];
+    f32 min_z[4];
+    f32 max_x[4];
+    f32 max_y[4];
+    f32 max_z[4];
+    u32 child[4];
+} NphPhysicsMeshNode; // 112 B (mesh.h:95)
+
+typedef struct NphPhysicsMesh {
+    u32 byte_size;
+    u32 vertex_count;
+    u32 triangle_count;
+    u32 node_count;
+    u32 nodes_offset;
+    u32 vertices_offset;
+    u32 triangles_offset;
+    u32 edges_offset;
+    u32 originals_offset;
+    u32 flags;
+    NphAABB bounds;
+} NphPhysicsMesh; // 64 B (mesh.h:113)
+
 typedef struct NphPhysicsHeightfield NphPhysicsHeightfield; // heightfield.h (not supported)
 
 // ---------------------------------------------------------------------------

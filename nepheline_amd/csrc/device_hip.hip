@@ -38,9 +38,10 @@
         }                                                                                          \
     } while (0)
 
+// A hull or mesh block and its device copy.
 typedef struct HullEntry {
-    const NphPhysicsHull *host;
-    const NphPhysicsHull *device;
+    const void *host;
+    const void *device;
 } HullEntry;
 
 // Record writes from the host (body and shape creation, velocity changes)
@@ -147,6 +148,7 @@ struct NphDevice {
     f32 *log_x, *log_t;  // log rows: 16 features, 4 targets
     u32 *log_count;
     u32 log_cap;
+    u32 mesh_shapes;     // mesh shapes made: the mesh pairs' kernel runs
 };
 
 const char *nph_dev_name(void) { return "hip:gfx950"; }
@@ -387,10 +389,10 @@ static NphResult reserve_groups(NphDevice *d, u32 groups)
     return NPH_RESULT_OK;
 }
 
-NphResult nph_dev_hull(NphDevice *d, const NphPhysicsHull *hull, const NphPhysicsHull **out)
+NphResult nph_dev_block(NphDevice *d, const void *block, u32 bytes, const void **out)
 {
     for (u32 i = 0U; i < d->hull_count; ++i) {
-        if (d->hulls[i].host == hull) {
+        if (d->hulls[i].host == block) {
             *out = d->hulls[i].device;
             return NPH_RESULT_OK;
         }
@@ -405,12 +407,12 @@ NphResult nph_dev_hull(NphDevice *d, const NphPhysicsHull *hull, const NphPhysic
         d->hull_cap = c;
     }
     void *p = NULL;
-    HIP_TRY(hipMalloc(&p, hull->byte_size));
-    HIP_TRY(hipMemcpy(p, hull, hull->byte_size, hipMemcpyHostToDevice));
-    d->hulls[d->hull_count].host = hull;
-    d->hulls[d->hull_count].device = (const NphPhysicsHull *)p;
+    HIP_TRY(hipMalloc(&p, bytes));
+    HIP_TRY(hipMemcpy(p, block, bytes, hipMemcpyHostToDevice));
+    d->hulls[d->hull_count].host = block;
+    d->hulls[d->hull_count].device = p;
     d->hull_count += 1U;
-    *out = (const NphPhysicsHull *)p;
+    *out = p;
     return NPH_RESULT_OK;
 }
 
@@ -649,6 +651,7 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_init_contacts(NphContactRec *cont
     c->body_a = in.body_a;
     c->body_b = in.body_b;
     c->normal = v3(0.0F, 1.0F, 0.0F);
+    c->child = in.child;
 }
 
 __global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide(NphContactRec *contacts, const NphShapeRec *shapes,
@@ -658,9 +661,24 @@ __global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide(NphContactRec *co
     if (i >= n) {
         return;
     }
-    NphContactRec *c = &contacts[list[i]];
-    contact_update(c, shapes, bodies, step_dt);
-    codes[i] = c->change;
+    if (contacts[list[i]].child != NPH_NULL32) {
+        return; // a mesh pair: k_collide_mesh
+    }
+    contact_update(contacts, list[i], shapes, bodies, step_dt);
+    codes[i] = contacts[list[i]].change;
+}
+
+// The mesh pairs of the same list, one a thread (their frames hold the
+// triangles' clusters, so they keep out of the convex pairs' kernel).
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide_mesh(NphContactRec *contacts, const NphShapeRec *shapes,
+                          const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || contacts[list[i]].child == NPH_NULL32) {
+        return;
+    }
+    contact_update_mesh(contacts, list[i], shapes, bodies, step_dt);
+    codes[i] = contacts[list[i]].change;
 }
 
 // Exclusive scan of one value per thread over a block of NPH_SCAN_BLOCK.
@@ -1410,10 +1428,17 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
     hipLaunchKernelGGL(k_collide, dim3((n + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
                        dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes, d->bodies,
                        d->list + first, d->codes + first, n, step_dt);
+    if (d->mesh_shapes > 0U) {
+        hipLaunchKernelGGL(k_collide_mesh, dim3((n + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
+                           dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes,
+                           d->bodies, d->list + first, d->codes + first, n, step_dt);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, d->stream));
     return NPH_RESULT_OK;
 }
+
+void nph_dev_note_mesh_shape(NphDevice *d) { d->mesh_shapes += 1U; }
 
 NphResult nph_dev_collide_begin(NphDevice *d, const u32 *awake, u32 count, f32 step_dt)
 {

@@ -283,7 +283,8 @@ static void graph_remove(NphPhysicsWorld *w, u32 contact)
     c->color_slot = NPH_NULL32;
 }
 
-// contact.c:239-263 (no mesh children)
+// contact.c:239-263: the pair's awake membership, then the graph membership
+// of its record and (a mesh pair's) children, in chain order.
 static NphResult contact_sync(NphPhysicsWorld *w, u32 contact)
 {
     const HContact *c = contact_at(w, contact);
@@ -301,15 +302,22 @@ static NphResult contact_sync(NphPhysicsWorld *w, u32 contact)
     }
     const int solvable = awake && (body_awake(a) || a->type != NPH_PHYSICS_BODY_DYNAMIC) &&
                          (body_awake(b) || b->type != NPH_PHYSICS_BODY_DYNAMIC);
-    const int in_graph = solvable && (c->flags & HC_TOUCHING) != 0U;
-    if (in_graph && c->color == NPH_NULL32) {
-        return graph_add(w, contact);
-    }
-    if (!in_graph && c->color != NPH_NULL32) {
-        graph_remove(w, contact);
+    for (u32 k = contact; k != NPH_NULL32; k = contact_at(w, k)->child) {
+        const HContact *rk = contact_at(w, k);
+        const int in_graph = solvable && (rk->flags & HC_TOUCHING) != 0U;
+        if (in_graph && rk->color == NPH_NULL32) {
+            r = graph_add(w, k);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+        } else if (!in_graph && rk->color != NPH_NULL32) {
+            graph_remove(w, k);
+        }
     }
     return NPH_RESULT_OK;
 }
+
+static int is_surface(const HShape *s) { return s->geometry.type == NPH_PHYSICS_SHAPE_MESH; }
 
 static NphResult contact_alloc(NphPhysicsWorld *w, u32 *out)
 {
@@ -333,41 +341,62 @@ static NphResult contact_alloc(NphPhysicsWorld *w, u32 *out)
     return NPH_RESULT_OK;
 }
 
-// contact.c:129-179 for a convex pair (one record).
+// contact.c:129-179: one record, or a mesh pair's four (the pair's, then
+// its children, each from the free list in turn).
 static NphResult contact_create(NphPhysicsWorld *w, u32 shape_a, u32 shape_b)
 {
-    u32 index = 0U;
-    NphResult r = contact_alloc(w, &index);
+    const HShape *sa = shape_at(w, shape_a);
+    const HShape *sb = shape_at(w, shape_b);
+    const u32 records =
+        is_surface(sa) || is_surface(sb) ? NPH_PHYSICS_MESH_MAX_MANIFOLDS : 1U;
+    u32 rec[NPH_PHYSICS_MESH_MAX_MANIFOLDS] = {NPH_NULL32, NPH_NULL32, NPH_NULL32, NPH_NULL32};
+    NphResult r = NPH_RESULT_OK;
+    u32 made = 0U;
+    while (made < records && r == NPH_RESULT_OK) {
+        r = contact_alloc(w, &rec[made]);
+        made += r == NPH_RESULT_OK ? 1U : 0U;
+    }
+    const u32 index = rec[0];
     if (r == NPH_RESULT_OK) {
         r = map_insert(w->alloc, &w->pairs, pair_key(shape_a, shape_b), index);
     }
     if (r == NPH_RESULT_OK) {
         r = vbuf_reserve(w->alloc, &w->awake_contacts, 4U, w->contacts.len);
     }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->inits, sizeof(NphContactInit), w->inits.len + records);
+    }
     if (r != NPH_RESULT_OK) {
+        for (u32 k = 0U; k < made; ++k) {
+            (void)vbuf_push(w->alloc, &w->free_contacts, 4U, &rec[k]); // reserved
+        }
         return r;
     }
-    const HShape *sa = shape_at(w, shape_a);
-    const HShape *sb = shape_at(w, shape_b);
+    // The shapes may have moved in memory (contact_alloc grows nothing of
+    // theirs), so read them again.
+    sa = shape_at(w, shape_a);
+    sb = shape_at(w, shape_b);
+    for (u32 k = 0U; k < records; ++k) {
+        HContact *c = contact_at(w, rec[k]);
+        memset(c, 0, sizeof *c);
+        c->shape_a = shape_a;
+        c->shape_b = shape_b;
+        c->body_a = sa->body;
+        c->body_b = sb->body;
+        c->color = NPH_NULL32;
+        c->color_slot = NPH_NULL32;
+        c->awake_slot = NPH_NULL32;
+        c->parent = k == 0U ? NPH_NULL32 : index;
+        c->child = k + 1U < records ? rec[k + 1U] : NPH_NULL32;
+        const NphContactInit init = {rec[k], shape_a, shape_b, c->body_a, c->body_b, c->child};
+        (void)vbuf_push(w->alloc, &w->inits, sizeof init, &init); // reserved
+    }
     HContact *c = contact_at(w, index);
-    memset(c, 0, sizeof *c);
-    c->shape_a = shape_a;
-    c->shape_b = shape_b;
-    c->body_a = sa->body;
-    c->body_b = sb->body;
-    c->color = NPH_NULL32;
-    c->color_slot = NPH_NULL32;
-    c->awake_slot = NPH_NULL32;
     if (((sa->flags | sb->flags) & HS_CONTACT_EVENTS) != 0U) {
         c->flags |= HC_EVENTS;
     }
     if (((sa->flags | sb->flags) & HS_HIT_EVENTS) != 0U) {
         c->flags |= HC_HIT_EVENTS;
-    }
-    const NphContactInit init = {index, shape_a, shape_b, c->body_a, c->body_b};
-    r = vbuf_push(w->alloc, &w->inits, sizeof init, &init);
-    if (r != NPH_RESULT_OK) {
-        return r;
     }
     list_insert(w, index, 0U, c->body_a);
     list_insert(w, index, 1U, c->body_b);
@@ -377,10 +406,26 @@ static NphResult contact_create(NphPhysicsWorld *w, u32 shape_a, u32 shape_b)
     return r;
 }
 
-// contact.c:181-237
+// contact.c:181-237: a mesh pair's children go first, each out of the
+// graph and back to the free list, then the pair's own record.
 static void contact_destroy(NphPhysicsWorld *w, u32 contact, int events)
 {
     HContact *c = contact_at(w, contact);
+    for (u32 k = c->child; k != NPH_NULL32;) {
+        HContact *child = contact_at(w, k);
+        const u32 next = child->child;
+        if (child->color != NPH_NULL32) {
+            graph_remove(w, k);
+        }
+        child->shape_a = NPH_NULL32;
+        child->shape_b = NPH_NULL32;
+        child->flags = 0U;
+        child->parent = NPH_NULL32;
+        child->child = NPH_NULL32;
+        (void)vbuf_push(w->alloc, &w->free_contacts, 4U, &k); // reserved when made
+        k = next;
+    }
+    c->child = NPH_NULL32;
     if ((c->flags & HC_TOUCHING) != 0U && events && (c->flags & HC_EVENTS) != 0U) {
         NphPhysicsContactEndEvent e;
         e.shape_a.value = shape_at(w, c->shape_a)->handle;
@@ -415,6 +460,20 @@ static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
         contact_destroy(w, contact, 1);
         return NPH_RESULT_OK;
     }
+    // A mesh pair's children touch as their manifolds say; their graph
+    // membership follows in contact_sync.
+    {
+        u32 i = 1U;
+        for (u32 k = c->child; k != NPH_NULL32; k = contact_at(w, k)->child, ++i) {
+            const u32 bits = (change >> (NPH_CHANGE_CHILD_SHIFT + 2U * (i - 1U))) & 3U;
+            HContact *child = contact_at(w, k);
+            if ((bits & NPH_CHANGE_STARTED) != 0U) {
+                child->flags |= HC_TOUCHING;
+            } else if ((bits & NPH_CHANGE_STOPPED) != 0U) {
+                child->flags &= ~(u32)HC_TOUCHING;
+            }
+        }
+    }
     if ((change & NPH_CHANGE_STARTED) != 0U) {
         c->flags |= HC_TOUCHING;
         if ((c->flags & HC_EVENTS) != 0U) {
@@ -439,6 +498,9 @@ static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
                 return r;
             }
         }
+        return contact_sync(w, contact);
+    }
+    if ((change & NPH_CHANGE_CHILDREN) != 0U) {
         return contact_sync(w, contact);
     }
     return NPH_RESULT_OK;

@@ -428,10 +428,19 @@ static NphResult grow_color_bits(NphPhysicsWorld *w, u32 slots)
 
 static HShape *shape_slot(NphPhysicsWorld *w, u32 s) { return &VB(w->shapes, HShape)[s]; }
 
-static NphGeom host_geom(const HShape *s)
+// The hull or mesh block a geometry points at (host memory), or NULL.
+static const void *geometry_block(const NphPhysicsGeometry *g)
 {
-    return geom_from(&s->geometry, s->geometry.type == NPH_PHYSICS_SHAPE_HULL ? s->geometry.hull : 0);
+    if (g->type == NPH_PHYSICS_SHAPE_HULL) {
+        return g->hull;
+    }
+    if (g->type == NPH_PHYSICS_SHAPE_MESH) {
+        return g->mesh;
+    }
+    return NULL;
 }
+
+static NphGeom host_geom(const HShape *s) { return geom_from(&s->geometry, geometry_block(&s->geometry)); }
 
 // body.c:392-448
 static void update_mass(NphPhysicsWorld *w, u32 slot)
@@ -666,13 +675,13 @@ bool nph_physics_body_is_awake(const NphPhysicsWorld *w, NphPhysicsBody body)
 // ---------------------------------------------------------------------------
 NphPhysicsMassData nph_physics_geometry_mass(const NphPhysicsGeometry *g, f32 density)
 {
-    const NphGeom gg = geom_from(g, g->type == NPH_PHYSICS_SHAPE_HULL ? g->hull : 0);
+    const NphGeom gg = geom_from(g, geometry_block(g));
     return geom_mass(&gg, density);
 }
 
 NphAABB nph_physics_geometry_aabb(const NphPhysicsGeometry *g, NphPhysicsPose pose)
 {
-    const NphGeom gg = geom_from(g, g->type == NPH_PHYSICS_SHAPE_HULL ? g->hull : 0);
+    const NphGeom gg = geom_from(g, geometry_block(g));
     return geom_aabb(&gg, pose);
 }
 
@@ -694,6 +703,8 @@ bool nph_physics_geometry_is_valid(const NphPhysicsGeometry *g)
                size_ok(g->box.half_extents.z);
     case NPH_PHYSICS_SHAPE_HULL:
         return g->hull != NULL && g->hull->vertex_count >= 4U;
+    case NPH_PHYSICS_SHAPE_MESH:
+        return g->mesh != NULL && g->mesh->triangle_count > 0U;
     default:
         return false;
     }
@@ -733,12 +744,12 @@ NphPhysicsShapeDesc nph_physics_shape_desc_default(NphPhysicsGeometry geometry)
     return d;
 }
 
-static NphShapeRec shape_record(NphPhysicsWorld *w, u32 s, const NphPhysicsHull *device_hull)
+static NphShapeRec shape_record(NphPhysicsWorld *w, u32 s, const void *device_block)
 {
     const HShape *h = shape_slot(w, s);
     NphShapeRec r;
     memset(&r, 0, sizeof r);
-    r.geom = geom_from(&h->geometry, device_hull);
+    r.geom = geom_from(&h->geometry, device_block);
     r.pose = h->pose;
     r.aabb = h->aabb;
     r.fat_aabb = h->fat_aabb;
@@ -750,17 +761,29 @@ static NphShapeRec shape_record(NphPhysicsWorld *w, u32 s, const NphPhysicsHull 
     return r;
 }
 
+// The device copy of a geometry's hull or mesh block (made once a block).
+static NphResult device_block_of(NphPhysicsWorld *w, const NphPhysicsGeometry *g,
+                                 const void **out)
+{
+    *out = NULL;
+    if (g->type == NPH_PHYSICS_SHAPE_HULL) {
+        return nph_dev_block(w->dev, g->hull, g->hull->byte_size, out);
+    }
+    if (g->type == NPH_PHYSICS_SHAPE_MESH) {
+        return nph_dev_block(w->dev, g->mesh, g->mesh->byte_size, out);
+    }
+    return NPH_RESULT_OK;
+}
+
 NphResult world_shape_record(NphPhysicsWorld *w, u32 s, NphShapeRec *out)
 {
     const HShape *h = shape_slot(w, s);
-    const NphPhysicsHull *device_hull = NULL;
-    if (h->geometry.type == NPH_PHYSICS_SHAPE_HULL) {
-        NphResult r = nph_dev_hull(w->dev, h->geometry.hull, &device_hull);
-        if (r != NPH_RESULT_OK) {
-            return r;
-        }
+    const void *device_block = NULL;
+    NphResult r = device_block_of(w, &h->geometry, &device_block);
+    if (r != NPH_RESULT_OK) {
+        return r;
     }
-    *out = shape_record(w, s, device_hull);
+    *out = shape_record(w, s, device_block);
     return NPH_RESULT_OK;
 }
 
@@ -782,6 +805,12 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
     if (hb == NULL) {
         return NPH_RESULT_NOT_FOUND;
     }
+    // A mesh bounds no volume, so it has no mass to move a body with
+    // (shape.c:196-201).
+    if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH &&
+        (hb->type == NPH_PHYSICS_BODY_DYNAMIC || desc->sensor != 0)) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
     if (desc->sensor != 0 || desc->ghost != 0) {
         return NPH_RESULT_UNSUPPORTED; // sensors and ghosts: not on this path
     }
@@ -789,10 +818,15 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
         desc->geometry.hull->vertex_count > NPH_HULL_MAX_VERTICES) {
         return NPH_RESULT_UNSUPPORTED;
     }
+    // Continuous sweeps against mesh triangles (continuous.c:211-300) are
+    // not built: a world that sweeps takes no meshes.
+    if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH && w->desc.enable_continuous != 0) {
+        return NPH_RESULT_UNSUPPORTED;
+    }
     NphResult r = world_sync_bodies(w);
-    const NphPhysicsHull *device_hull = NULL;
-    if (r == NPH_RESULT_OK && desc->geometry.type == NPH_PHYSICS_SHAPE_HULL) {
-        r = nph_dev_hull(w->dev, desc->geometry.hull, &device_hull);
+    const void *device_block = NULL;
+    if (r == NPH_RESULT_OK) {
+        r = device_block_of(w, &desc->geometry, &device_block);
     }
     const u32 body_slot = phys_slot(body.value);
     u32 slot = 0U;
@@ -839,7 +873,7 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
               : hb->type == NPH_PHYSICS_BODY_KINEMATIC ? NPH_TREE_KINEMATIC
                                                         : NPH_TREE_DYNAMIC;
     {
-        NphShapeRec tmp = shape_record(w, slot, device_hull);
+        NphShapeRec tmp = shape_record(w, slot, device_block);
         tmp.geom = host_geom(s); // the host's own hull address for the box
         s->aabb = shape_world_aabb(b, &tmp);
     }
@@ -851,8 +885,11 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
         r = move_buffer_push(w, slot);
     }
     if (r == NPH_RESULT_OK) {
-        const NphShapeRec rec = shape_record(w, slot, device_hull);
+        const NphShapeRec rec = shape_record(w, slot, device_block);
         r = nph_dev_write_shape(w->dev, slot, &rec);
+        if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH) {
+            nph_dev_note_mesh_shape(w->dev);
+        }
     }
     if (r == NPH_RESULT_OK && hb->type != NPH_PHYSICS_BODY_STATIC) {
         r = world_wake_body(w, body_slot);
@@ -1288,8 +1325,8 @@ u32 nph_physics_world_contacts(const NphPhysicsWorld *cw, NphPhysicsContactInfo 
             o->flags = hc->flags;
             o->color = hc->color;
             o->color_slot = hc->color_slot;
-            o->parent = NPH_NULL32;
-            o->child = NPH_NULL32;
+            o->parent = hc->parent;
+            o->child = hc->child;
             o->friction = c->friction;
             o->restitution = c->restitution;
             o->normal = c->normal;

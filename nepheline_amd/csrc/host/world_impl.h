@@ -134,6 +134,8 @@ typedef struct HContact {
     u32 color;
     u32 color_slot;
     u32 awake_slot;
+    u32 parent; // a mesh pair's child: its pair's record (else NPH_NULL32)
+    u32 child;  // a mesh pair's next record (else NPH_NULL32)
 } HContact;
 
 // joint.c / world_internal.h:280-330, less the step's numbers (the device's).

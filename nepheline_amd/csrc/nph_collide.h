@@ -1252,8 +1252,36 @@ NPH_HD int sat_query(const NphHullPair *hp, f32 limit, NphCache *cache, NphSat *
 
 // collide_hull.c:787-877: the reference face `ref` of R (A when !flip), the
 // incident face clipped against its side planes; at most `max_points`.
+// A convex shape's contact with one triangle, before its reduction: up to
+// sixteen points (physics_internal.h:65-89).
+#define NPH_TRI_POINTS 16U
+
+typedef struct NphTriContact {
+    NphVec3 normal; // from the shape towards the triangle
+    u32 point_count;
+    NphMPoint points[NPH_TRI_POINTS];
+} NphTriContact;
+
+NPH_HD void tri_clear(NphTriContact *c)
+{
+    c->normal = v3(0.0F, 1.0F, 0.0F);
+    c->point_count = 0U;
+}
+
+NPH_HD void tri_push(NphTriContact *c, NphVec3 point, f32 separation, u32 id)
+{
+    if (c->point_count < NPH_TRI_POINTS) {
+        NphMPoint *p = &c->points[c->point_count++];
+        p->point = point;
+        p->separation = separation;
+        p->id = id;
+    }
+}
+
+// collide_hull.c:786-877: more than `max_points` candidates are reduced to
+// the manifold's four.
 NPH_HD NphVec3 face_contact(const NphHullPair *hp, int flip, u32 ref, f32 speculative,
-                            NphMPoint *points, u32 *count)
+                            u32 max_points, NphMPoint *points, u32 *count)
 {
     const NphView *rh = flip ? hp->b : hp->a;
     const NphView *ih = flip ? hp->a : hp->b;
@@ -1330,7 +1358,7 @@ NPH_HD NphVec3 face_contact(const NphHullPair *hp, int flip, u32 ref, f32 specul
             n_out += 1U;
         }
     }
-    if (n_out > NPH_PHYSICS_MANIFOLD_MAX_POINTS) {
+    if (n_out > max_points) {
         n_out = manifold_reduce(cand, n_out, n);
     }
     for (u32 i = 0U; i < n_out; ++i) {
@@ -1340,8 +1368,12 @@ NPH_HD NphVec3 face_contact(const NphHullPair *hp, int flip, u32 ref, f32 specul
     return flip ? v3_neg(n) : n;
 }
 
-NPH_HD void collide_hulls(const NphView *a, NphPhysicsPose pa, const NphView *b,
-                          NphPhysicsPose pb, f32 speculative, NphCache *cache, NphManifold *out)
+// collide_hull.c:921-1002: the pair function, keeping up to `max_points`
+// points; `prefer_b` turns the tie between faces to B's (a triangle's face).
+// `out` is left as it was when nothing touches.
+NPH_HD void collide_hulls_into(const NphView *a, NphPhysicsPose pa, const NphView *b,
+                               NphPhysicsPose pb, f32 speculative, NphCache *cache, int prefer_b,
+                               u32 max_points, NphTriContact *out)
 {
     NphHullPair hp;
     hp.a = a;
@@ -1356,7 +1388,8 @@ NPH_HD void collide_hulls(const NphView *a, NphPhysicsPose pa, const NphView *b,
     }
     const f32 face_sep = nph_maxf(r.face_a_sep, r.face_b_sep);
     const int use_edge = r.edge_sep > face_sep + NPH_EDGE_BIAS;
-    const int flip = !use_edge && (r.face_b_sep > r.face_a_sep + NPH_FACE_BIAS);
+    const int flip = !use_edge && (prefer_b ? r.face_b_sep >= r.face_a_sep - NPH_FACE_BIAS
+                                            : r.face_b_sep > r.face_a_sep + NPH_FACE_BIAS);
     if (cache != 0) {
         cache->type = (u8)(use_edge ? NPH_CACHE_EDGES : (flip ? NPH_CACHE_FACE_B : NPH_CACHE_FACE_A));
         cache->index_a = (u16)(use_edge ? r.edge_a : r.face_a);
@@ -1376,7 +1409,7 @@ NPH_HD void collide_hulls(const NphView *a, NphPhysicsPose pa, const NphView *b,
             (use_edge || v3_dot(gd.normal, sat_axis) < NPH_FEATURE_AGREEMENT)) {
             const NphVec3 mid = v3_scale(v3_add(gd.point_a, gd.point_b), 0.5F);
             out->normal = pose_vector(pa, gd.normal);
-            manifold_push(out, pose_point(pa, mid), gd.distance, 0x40000000U);
+            tri_push(out, pose_point(pa, mid), gd.distance, 0x40000000U);
             return;
         }
     }
@@ -1392,17 +1425,53 @@ NPH_HD void collide_hulls(const NphView *a, NphPhysicsPose pa, const NphView *b,
         }
         const NphVec3 mid = v3_scale(v3_add(sd.closest1, sd.closest2), 0.5F);
         out->normal = pose_vector(pa, r.edge_axis);
-        manifold_push(out, pose_point(pa, mid), sep, nph_mix_id(0x20000000U | r.edge_a, r.edge_b));
+        tri_push(out, pose_point(pa, mid), sep, nph_mix_id(0x20000000U | r.edge_a, r.edge_b));
         return;
     }
-    NphMPoint pts[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+    NphMPoint pts[NPH_TRI_POINTS];
     u32 count = 0U;
-    const NphVec3 n = face_contact(&hp, flip, flip ? r.face_b : r.face_a, speculative, pts, &count);
-    if (count > 0U) {
-        out->normal = pose_vector(pa, n);
-        for (u32 i = 0U; i < count; ++i) {
-            manifold_push(out, pose_point(pa, pts[i].point), pts[i].separation, pts[i].id);
+    const NphVec3 n =
+        face_contact(&hp, flip, flip ? r.face_b : r.face_a, speculative, max_points, pts, &count);
+    out->normal = pose_vector(pa, n);
+    for (u32 i = 0U; i < count; ++i) {
+        tri_push(out, pose_point(pa, pts[i].point), pts[i].separation, pts[i].id);
+    }
+}
+
+// collide_hull.c:1004-1017: a convex pair, four points at most.
+NPH_HD void collide_hulls(const NphView *a, NphPhysicsPose pa, const NphView *b,
+                          NphPhysicsPose pb, f32 speculative, NphCache *cache, NphManifold *out)
+{
+    NphTriContact c;
+    tri_clear(&c);
+    collide_hulls_into(a, pa, b, pb, speculative, cache, 0, NPH_PHYSICS_MANIFOLD_MAX_POINTS, &c);
+    if (c.point_count > 0U) {
+        out->normal = c.normal;
+        for (u32 i = 0U; i < c.point_count; ++i) {
+            manifold_push(out, c.points[i].point, c.points[i].separation, c.points[i].id);
         }
+    }
+}
+
+// collide_hull.c:879-904: B's face `face` against A, as the face contact of
+// the pair with B's face the reference.
+NPH_HD void collide_hull_face(const NphView *a, NphPhysicsPose pa, const NphView *b,
+                              NphPhysicsPose pb, u32 face, f32 speculative, NphTriContact *out)
+{
+    NphHullPair hp;
+    hp.a = a;
+    hp.b = b;
+    hp.rel = pose_inv_mul(pa, pb);
+    for (u32 i = 0U; i < b->vertex_count; ++i) {
+        hp.bv[i] = pose_point(hp.rel, b->vertices[i]);
+    }
+    NphMPoint pts[NPH_TRI_POINTS];
+    u32 count = 0U;
+    const NphVec3 n = face_contact(&hp, 1, face, speculative, NPH_TRI_POINTS, pts, &count);
+    tri_clear(out);
+    out->normal = pose_vector(pa, n);
+    for (u32 i = 0U; i < count; ++i) {
+        tri_push(out, pose_point(pa, pts[i].point), pts[i].separation, pts[i].id);
     }
 }
 
@@ -1616,7 +1685,8 @@ NPH_HD void collide_boxes(NphVec3 half_a, NphPhysicsPose pa, NphVec3 half_b, Nph
     }
     NphMPoint pts[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
     u32 count = 0U;
-    const NphVec3 n = face_contact(&hp, flip, flip ? face_b : face_a, speculative, pts, &count);
+    const NphVec3 n = face_contact(&hp, flip, flip ? face_b : face_a, speculative,
+                                   NPH_PHYSICS_MANIFOLD_MAX_POINTS, pts, &count);
     out->normal = pose_vector(pa, n);
     for (u32 i = 0U; i < count; ++i) {
         manifold_push(out, pose_point(pa, pts[i].point), pts[i].separation, pts[i].id);

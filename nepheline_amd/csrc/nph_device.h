@@ -27,6 +27,7 @@ typedef struct NphContactInit {
     u32 shape_b;
     u32 body_a;
     u32 body_b;
+    u32 child; // a mesh pair's next record, or NPH_NULL32
 } NphContactInit;
 
 // A shape whose fat box grew in finalize, in the awake list's order and each
@@ -56,9 +57,11 @@ void nph_dev_destroy(NphDevice *d);
 const char *nph_dev_name(void);
 // Grows the record arrays (contents kept) to at least these many slots.
 NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts);
-// A device copy of a hull block, made once per block and kept for the
-// device's life; *out is its device address.
-NphResult nph_dev_hull(NphDevice *d, const NphPhysicsHull *hull, const NphPhysicsHull **out);
+// A device copy of a hull or mesh block of `bytes`, made once per block and
+// kept for the device's life; *out is its device address.
+NphResult nph_dev_block(NphDevice *d, const void *block, u32 bytes, const void **out);
+// A mesh shape exists: the narrow phase runs its mesh-pair kernel too.
+void nph_dev_note_mesh_shape(NphDevice *d);
 NphResult nph_dev_write_body(NphDevice *d, u32 slot, const NphBodyRec *b);
 // Joint records by slot: the configuration (when conf is not NULL) and, with
 // reset, accumulated impulses from zero; staged as the body writes are.

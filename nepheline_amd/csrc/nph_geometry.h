@@ -7,14 +7,18 @@
 #include "nph_math.h"
 
 // The geometry as the device keeps it: type, up to three sizes, and the hull
-// block (device address on the device, host address on the host).
+// or mesh block (device address on the device, host address on the host).
 typedef struct NphGeom {
     u32 type;
     f32 a, b, c; // sphere: r; capsule: half_height, radius; box: half extents
-    const NphPhysicsHull *hull;
+    union {
+        const NphPhysicsHull *hull;
+        const NphPhysicsMesh *mesh;
+    };
 } NphGeom;
 
-NPH_HD NphGeom geom_from(const NphPhysicsGeometry *g, const NphPhysicsHull *hull_address)
+// `block_address`: the hull or mesh block as the caller will read it.
+NPH_HD NphGeom geom_from(const NphPhysicsGeometry *g, const void *block_address)
 {
     NphGeom o;
     o.type = (u32)g->type;
@@ -35,8 +39,11 @@ NPH_HD NphGeom geom_from(const NphPhysicsGeometry *g, const NphPhysicsHull *hull
         o.b = g->box.half_extents.y;
         o.c = g->box.half_extents.z;
         break;
+    case NPH_PHYSICS_SHAPE_MESH:
+        o.mesh = (const NphPhysicsMesh *)block_address;
+        break;
     default:
-        o.hull = hull_address;
+        o.hull = (const NphPhysicsHull *)block_address;
         break;
     }
     return o;
@@ -165,6 +172,8 @@ NPH_HD NphAABB geom_aabb(const NphGeom *g, NphPhysicsPose pose)
         const NphVec3 h = v3(g->a, g->b, g->c);
         return transform_box(aabb_make(v3_neg(h), h), pose);
     }
+    case NPH_PHYSICS_SHAPE_MESH:
+        return transform_box(g->mesh->bounds, pose);
     default: {
         const NphPhysicsHull *hull = g->hull;
         const NphVec3 *v = hull_vertices(hull);
@@ -208,6 +217,11 @@ NPH_HD f32 geom_outer_radius(const NphGeom *g)
         return v3_len(v3(g->a, g->b, g->c));
     case NPH_PHYSICS_SHAPE_HULL:
         return g->hull->outer_radius;
+    case NPH_PHYSICS_SHAPE_MESH: {
+        // The farthest corner of the vertices' box.
+        const NphAABB b = g->mesh->bounds;
+        return v3_len(v3_max(v3_abs(b.min), v3_abs(b.max)));
+    }
     default:
         return 0.0F;
     }

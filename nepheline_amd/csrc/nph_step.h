@@ -17,7 +17,7 @@
 #ifndef NPH_STEP_H
 #define NPH_STEP_H
 
-#include "nph_collide.h"
+#include "nph_mesh.h"
 
 #define NPH_LANES 8U
 #define NPH_IMPACT_SPEED 1.0F
@@ -93,7 +93,11 @@ enum {
     NPH_CHANGE_STARTED = 1U,
     NPH_CHANGE_STOPPED = 2U,
     NPH_CHANGE_DISJOINT = 4U,
+    NPH_CHANGE_CHILDREN = 8U, // a mesh pair's child record started or stopped
 };
+// A mesh pair's child k (1..3) in its parent's change code: started at bit
+// NPH_CHANGE_CHILD_SHIFT + 2 (k - 1), stopped at the bit after.
+#define NPH_CHANGE_CHILD_SHIFT 16U
 
 typedef struct NphContactRec {
     u32 shape_a;
@@ -107,6 +111,7 @@ typedef struct NphContactRec {
     NphVec3 normal;
     u32 point_count;
     NphPointRec points[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+    u32 child; // a mesh pair's next record, or NPH_NULL32
 } NphContactRec;
 
 // The solver's body: four 16-byte rows, v | w | dp | dq (world_internal.h:387-400).
@@ -587,11 +592,154 @@ NPH_HD void pair_surface(const NphShapeRec *a, const NphShapeRec *b, f32 *fricti
         combine_values(a->restitution, a->combine >> 8U, b->restitution, b->combine >> 8U, 1);
 }
 
+// contact.c:378-483: a pair with a mesh, up to four manifolds, the deepest
+// in the pair's record and the rest in its children; impulses carried over
+// the whole chain, by id then by place (with the manifolds' normals within
+// the merge angle).  The fat-box test has been made on the parent.
+NPH_HD void mesh_update(NphContactRec *contacts, u32 contact, const NphShapeRec *shapes,
+                        const NphBodyRec *bodies, f32 step_dt)
+{
+    NphContactRec *c = &contacts[contact];
+    const NphShapeRec *sa = &shapes[c->shape_a];
+    const NphShapeRec *sb = &shapes[c->shape_b];
+    const int mesh_a = sa->geom.type == NPH_PHYSICS_SHAPE_MESH;
+    const NphShapeRec *sm = mesh_a ? sa : sb;
+    const NphShapeRec *sx = mesh_a ? sb : sa;
+    const NphBodyRec *bm = &bodies[sm->body];
+    const NphBodyRec *bx = &bodies[sx->body];
+    const NphQuat qm = q_mul(bm->rotation, sm->pose.rotation);
+    const NphQuat qx = q_mul(bx->rotation, sx->pose.rotation);
+    const NphDVec3 pm = dv3_add_v3(bm->origin, q_rotate(bm->rotation, sm->pose.position));
+    const NphDVec3 px = dv3_add_v3(bx->origin, q_rotate(bx->rotation, sx->pose.position));
+    const NphQuat rotation = q_mul(q_conj(qm), qx);
+    const NphDVec3 position = drotate_inverse(qm, dv3(px.x - pm.x, px.y - pm.y, px.z - pm.z));
+    NphSurfaceQuery q;
+    NphMeshManifolds mm;
+    collide_mesh_local(&sx->geom, rotation, position, sm->geom.mesh,
+                       pair_speculative(bm, bx, step_dt), &q, &mm);
+    const NphVec3 shape_from_x = dv3_relative(px, bx->center);
+    const NphVec3 x_from_m = dv3_relative(bx->center, bm->center);
+    // The chain's points of the last step.
+    NphPointRec old[NPH_PHYSICS_MESH_MAX_MANIFOLDS * NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+    NphVec3 old_normals[NPH_PHYSICS_MESH_MAX_MANIFOLDS * NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+    u32 old_count = 0U;
+    for (u32 k = contact; k != NPH_NULL32; k = contacts[k].child) {
+        const NphContactRec *r = &contacts[k];
+        for (u32 j = 0U; j < r->point_count; ++j) {
+            old_normals[old_count] = r->normal;
+            old[old_count++] = r->points[j];
+        }
+    }
+    u32 claimed = 0U;
+    u32 unmatched = 0U; // bit 4i + j: point j of manifold i
+    f32 friction = 0.0F;
+    f32 restitution = 0.0F;
+    pair_surface(sa, sb, &friction, &restitution);
+    u32 children = 0U;
+    u32 rec = contact;
+    for (u32 i = 0U; rec != NPH_NULL32; ++i) {
+        NphContactRec *r = &contacts[rec];
+        const int was = r->point_count > 0U;
+        r->point_count = 0U;
+        if (i < mm.count) {
+            const NphManifold *m = &mm.manifolds[i];
+            const NphVec3 n = q_rotate(qm, m->normal);
+            r->normal = mesh_a ? v3_neg(n) : n;
+            for (u32 j = 0U; j < m->point_count; ++j) {
+                NphPointRec *p = &r->points[j];
+                memset(p, 0, sizeof *p);
+                const NphVec3 ax = v3_add(shape_from_x, q_rotate(qm, m->points[j].point));
+                const NphVec3 am = v3_add(ax, x_from_m);
+                p->anchor_a = mesh_a ? am : ax;
+                p->anchor_b = mesh_a ? ax : am;
+                p->separation = m->points[j].separation;
+                p->id = m->points[j].id;
+                int found = 0;
+                for (u32 o = 0U; o < old_count; ++o) { // warm_by_id
+                    if (old[o].id == p->id) {
+                        p->normal_impulse = old[o].normal_impulse;
+                        p->tangent_impulse = old[o].tangent_impulse;
+                        claimed |= 1U << o;
+                        found = 1;
+                        break;
+                    }
+                }
+                if (!found) {
+                    unmatched |= 1U << (i * NPH_PHYSICS_MANIFOLD_MAX_POINTS + j);
+                }
+            }
+            r->point_count = m->point_count;
+        }
+        r->friction = friction;
+        r->restitution = restitution;
+        const int touching = r->point_count > 0U;
+        r->change = 0U;
+        if (touching && !was) {
+            r->change = NPH_CHANGE_STARTED;
+        } else if (!touching && was) {
+            r->change = NPH_CHANGE_STOPPED;
+        }
+        if (i > 0U && r->change != 0U) {
+            children |= r->change << (NPH_CHANGE_CHILD_SHIFT + 2U * (i - 1U));
+        }
+        rec = r->child;
+    }
+    if (unmatched != 0U && claimed != (1U << old_count) - 1U) {
+        rec = contact;
+        for (u32 i = 0U; rec != NPH_NULL32 && i < mm.count; ++i) {
+            NphContactRec *r = &contacts[rec];
+            for (u32 j = 0U; j < r->point_count; ++j) {
+                if ((unmatched & (1U << (i * NPH_PHYSICS_MANIFOLD_MAX_POINTS + j))) == 0U) {
+                    continue;
+                }
+                NphPointRec *p = &r->points[j];
+                u32 best = NPH_NULL32;
+                f32 best_d = NPH_WARM_DISTANCE * NPH_WARM_DISTANCE;
+                for (u32 o = 0U; o < old_count; ++o) { // warm_by_place
+                    if ((claimed & (1U << o)) != 0U ||
+                        v3_dot(old_normals[o], r->normal) < NPH_MESH_MERGE_COS) {
+                        continue;
+                    }
+                    const f32 d = nph_maxf(v3_dist_sq(old[o].anchor_a, p->anchor_a),
+                                           v3_dist_sq(old[o].anchor_b, p->anchor_b));
+                    if (d < best_d) {
+                        best_d = d;
+                        best = o;
+                    }
+                }
+                if (best != NPH_NULL32) {
+                    p->normal_impulse = old[best].normal_impulse;
+                    p->tangent_impulse = old[best].tangent_impulse;
+                    claimed |= 1U << best;
+                }
+            }
+            rec = r->child;
+        }
+    }
+    if (children != 0U) {
+        c->change |= NPH_CHANGE_CHILDREN | children;
+    }
+}
+
+// contact.c:516-530 for a mesh pair: the disjoint test, then its chain.
+NPH_HD void contact_update_mesh(NphContactRec *contacts, u32 index, const NphShapeRec *shapes,
+                                const NphBodyRec *bodies, f32 step_dt)
+{
+    NphContactRec *c = &contacts[index];
+    c->change = 0U;
+    if (!aabb_overlaps(shapes[c->shape_a].fat_aabb, shapes[c->shape_b].fat_aabb)) {
+        c->change = NPH_CHANGE_DISJOINT;
+        return;
+    }
+    mesh_update(contacts, index, shapes, bodies, step_dt);
+}
+
 // contact.c:516-577 for a convex pair: the disjoint test, the manifold in
 // A's centre-of-mass frame, impulses carried by id then by place.
-NPH_HD void contact_update(NphContactRec *c, const NphShapeRec *shapes, const NphBodyRec *bodies,
-                           f32 step_dt)
+NPH_HD void contact_update(NphContactRec *contacts, u32 index, const NphShapeRec *shapes,
+                           const NphBodyRec *bodies, f32 step_dt)
 {
+    NphContactRec *c = &contacts[index];
     const NphShapeRec *sa = &shapes[c->shape_a];
     const NphShapeRec *sb = &shapes[c->shape_b];
     c->change = 0U;

@@ -23,7 +23,7 @@ PRODUCT_LIB = os.path.join(HERE, "libnepheline_amd.so")
 OK, OUT_OF_MEMORY, INVALID_ARGUMENT, NOT_FOUND = 0, 1, 2, 3
 UNSUPPORTED = 8
 
-SHAPE_SPHERE, SHAPE_CAPSULE, SHAPE_BOX, SHAPE_HULL = 0, 1, 2, 3
+SHAPE_SPHERE, SHAPE_CAPSULE, SHAPE_BOX, SHAPE_HULL, SHAPE_MESH = 0, 1, 2, 3, 4
 BODY_STATIC, BODY_KINEMATIC, BODY_DYNAMIC = 0, 1, 2
 NULL32 = 0xFFFFFFFF
 
@@ -297,6 +297,29 @@ def geometry_hull(block_ptr):
     g.type = SHAPE_HULL
     g.u.ptr = block_ptr
     return g
+
+
+def geometry_mesh(block_ptr):
+    g = Geometry()
+    g.type = SHAPE_MESH
+    g.u.ptr = block_ptr
+    return g
+
+
+class MeshNode(C.Structure):
+    """mesh.h:88-96"""
+    _fields_ = [("min_x", C.c_float * 4), ("min_y", C.c_float * 4), ("min_z", C.c_float * 4),
+                ("max_x", C.c_float * 4), ("max_y", C.c_float * 4), ("max_z", C.c_float * 4),
+                ("child", C.c_uint32 * 4)]
+
+
+class Mesh(C.Structure):
+    """The mesh block's header (mesh.h:100-113)."""
+    _fields_ = [("byte_size", C.c_uint32), ("vertex_count", C.c_uint32),
+                ("triangle_count", C.c_uint32), ("node_count", C.c_uint32),
+                ("nodes_offset", C.c_uint32), ("vertices_offset", C.c_uint32),
+                ("triangles_offset", C.c_uint32), ("edges_offset", C.c_uint32),
+                ("originals_offset", C.c_uint32), ("flags", C.c_uint32), ("bounds", AABB)]
 
 
 class World:

@@ -133,13 +133,20 @@ def bin_drop(api, bodies: int, seed: int, hulls: HullBank | None, half: float = 
                            (0, 1, half + 1.0, 0.5), (0, -1, half + 1.0, 0.5)):
         wb = w.add_body(P.BODY_STATIC, (sx * (half + 0.5), wall_h, sz * (half + 0.5)))
         w.add_shape(wb, P.geometry_box(hx, wall_h, hz), friction=0.5)
-    r = Pcg32(seed)
+    drop_mixed(w, Pcg32(seed), bodies, hulls, -half + 1.0, half - 1.0, y_lo, height)
+    return w
+
+
+def drop_mixed(w: P.World, r: Pcg32, bodies: int, hulls: HullBank | None, lo: float, hi: float,
+               y_lo: float, y_hi: float):
+    """`bodies` dynamic bodies of mixed shapes (sphere, capsule, box, hull)
+    at x, z in [lo, hi], y in [y_lo, y_hi], random rotations, friction 0.5."""
     kinds = 4 if hulls is not None and len(hulls) > 0 else 3
     for _ in range(bodies):
         kind = r.next_bounded(kinds)
-        x = float(r.next_range(-half + 1.0, half - 1.0))
-        y = float(r.next_range(y_lo, height))
-        z = float(r.next_range(-half + 1.0, half - 1.0))
+        x = float(r.next_range(lo, hi))
+        y = float(r.next_range(y_lo, y_hi))
+        z = float(r.next_range(lo, hi))
         q = rotation(r)
         if kind == 0:
             g = P.geometry_sphere(float(r.next_range(0.2, 0.6)))
@@ -157,6 +164,85 @@ def bin_drop(api, bodies: int, seed: int, hulls: HullBank | None, half: float = 
         b = w.add_body(P.BODY_DYNAMIC, (x, y, z), q)
         w.add_shape(b, g, friction=0.5)
     w._keep.append(hulls)
+
+
+# ---------------------------------------------------------------------------
+# Meshes: blocks built by the reference's mesh builder (tests/golden), and
+# scenes of bodies dropped on them.
+# ---------------------------------------------------------------------------
+class MeshBank(HullBank):
+    """Mesh blocks kept alive, each 64-byte aligned (the blocks' nodes are
+    read with aligned four-wide loads)."""
+
+    def __init__(self, blob: bytes, offsets):
+        self._buf = C.create_string_buffer(len(blob) + 64)
+        base = (C.addressof(self._buf) + 63) & ~63
+        C.memmove(base, blob, len(blob))
+        self.ptrs = [base + int(o) for o in offsets]
+
+
+def load_mesh_bank(path: str) -> MeshBank:
+    """The bank format of load_hull_bank, blocks 64-byte aligned in the file."""
+    data = open(path, "rb").read()
+    n = int(np.frombuffer(data[:4], dtype=np.uint32)[0])
+    offsets = np.frombuffer(data[4:4 + 4 * n], dtype=np.uint32)
+    return MeshBank(data, offsets)
+
+
+def terrain_grid(r: Pcg32, nx: int, nz: int, cell: float, amp: float):
+    """A height grid centred on the origin: (nx + 1)(nz + 1) vertices, heights
+    amp * U[0, 1) row by row, two triangles a cell, counter-clockwise seen
+    from +y.  Returns (vertices float32 (n, 3), indices uint32 (m, 3))."""
+    verts = []
+    for j in range(nz + 1):
+        for i in range(nx + 1):
+            h = np.float32(np.float32(amp) * r.next_f32())
+            verts.append((np.float32(i * cell - nx * cell * 0.5), h,
+                          np.float32(j * cell - nz * cell * 0.5)))
+    idx = []
+    for j in range(nz):
+        for i in range(nx):
+            v00 = j * (nx + 1) + i
+            v10 = v00 + 1
+            v01 = v00 + nx + 1
+            v11 = v01 + 1
+            idx.append((v00, v01, v10))
+            idx.append((v10, v01, v11))
+    return np.array(verts, dtype=np.float32), np.array(idx, dtype=np.uint32)
+
+
+def mesh_terrain_drop(api, meshes: MeshBank, bodies: int, seed: int, hulls: HullBank | None,
+                      desc: dict | None = None) -> P.World:
+    """Mixed bodies dropped from 3-12 m onto the bank's first mesh (a
+    32 x 32 m height grid) on a static body at the origin."""
+    d = dict(DESC)
+    d.update(desc or {})
+    w = P.World(api, d)
+    ground = w.add_body(P.BODY_STATIC, (0.0, 0.0, 0.0))
+    w.add_shape(ground, P.geometry_mesh(meshes.ptrs[0]), friction=0.5)
+    drop_mixed(w, Pcg32(seed), bodies, hulls, -14.0, 14.0, 3.0, 12.0)
+    w._keep.append(meshes)
+    return w
+
+
+def mesh_field(api, meshes: MeshBank, bodies: int, seed: int, hulls: HullBank | None,
+               desc: dict | None = None) -> P.World:
+    """Four 64-triangle patches (the bank's meshes 1-4) on static bodies, one
+    tilted, one turned, the two-sided one upside down, over a ground slab;
+    mixed bodies dropped onto them."""
+    d = dict(DESC)
+    d.update(desc or {})
+    w = P.World(api, d)
+    ground(w, -2.0)
+    places = (((-7.0, 0.0, -7.0), (0.0, 0.0, 0.0, 1.0)),
+              ((7.0, 0.5, -7.0), quat_axis_angle((1.0, 0.0, 0.0), 0.35)),
+              ((-7.0, 0.0, 7.0), quat_axis_angle((0.0, 1.0, 0.0), 0.785398)),
+              ((7.0, 1.0, 7.0), quat_axis_angle((1.0, 0.0, 0.0), 3.14159265)))
+    for k, (pos, q) in enumerate(places):
+        b = w.add_body(P.BODY_STATIC, pos, q)
+        w.add_shape(b, P.geometry_mesh(meshes.ptrs[1 + k]), friction=0.5)
+    drop_mixed(w, Pcg32(seed), bodies, hulls, -12.0, 12.0, 4.0, 14.0)
+    w._keep.append(meshes)
     return w
 
 

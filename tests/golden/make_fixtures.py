@@ -9,6 +9,11 @@ oracle/_ref/libnph_ref.so (run in a container that has /root/reference).
   pyramid_hashes.json  nph_physics_world_hash (world.c:436-462) after each of
                        300 steps of the 20-row pyramid, dt 1/60, 4 substeps
   bin_hashes.json      the same for small bin drops (every shape type)
+  meshes.bin           5 mesh blocks built by the reference's builder
+                       (nph_physics_mesh_build, mesh.c) from seeded height
+                       grids: a 16 x 16-cell terrain and four 4 x 8-cell
+                       patches (the last two-sided)
+  mesh_hashes.json     world hashes of mixed drops onto them
 
 usage: python tests/golden/make_fixtures.py
 """
@@ -80,6 +85,42 @@ def make_hull_bank(api, n=256, seed=7, verts=12):
     return len(data)
 
 
+class MeshDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint32),
+                ("indices", C.c_void_p), ("triangle_count", C.c_uint32),
+                ("weld_tolerance", C.c_float), ("active_edge_angle", C.c_float),
+                ("two_sided", C.c_int32)]
+
+
+def make_mesh_bank(api, seed=31):
+    lib = api.lib
+    build = lib.nph_physics_mesh_build
+    build.restype = C.c_int
+    build.argtypes = [C.POINTER(MeshDesc), C.c_void_p, C.POINTER(C.c_void_p)]
+    r = S.Pcg32(seed)
+    grids = [(16, 16, 2.0, 1.5, 0)] + [(4, 8, 1.5, 0.8, 0)] * 3 + [(4, 8, 1.5, 0.8, 1)]
+    blocks = []
+    for nx, nz, cell, amp, two in grids:
+        v, idx = S.terrain_grid(r, nx, nz, cell, amp)
+        d = MeshDesc(v.ctypes.data, len(v), idx.ctypes.data, len(idx), 0.0, 0.0, two)
+        out = C.c_void_p()
+        assert build(C.byref(d), api.allocator(), C.byref(out)) == 0
+        size = C.cast(out, C.POINTER(C.c_uint32))[0]
+        blocks.append(C.string_at(out, size))
+    n = len(blocks)
+    head = (4 + 4 * n + 63) & ~63
+    offsets = []
+    body = b""
+    for b in blocks:
+        offsets.append(head + len(body))
+        body += b + b"\0" * ((64 - len(b) % 64) % 64)
+    data = struct.pack("<I", n) + struct.pack("<%dI" % n, *offsets)
+    data += b"\0" * (head - len(data)) + body
+    with open(os.path.join(HERE, "meshes.bin"), "wb") as f:
+        f.write(data)
+    return len(data)
+
+
 def hashes(world, steps, dt=1.0 / 60.0):
     out = []
     for _ in range(steps):
@@ -122,6 +163,14 @@ def main():
         json.dump({"scene": "continuous collision on: projectiles (40 shots at a thin wall, "
                             "every fifth a bullet) and a 300-body bin drop from 30 m",
                    "runs": ccd}, f)
+    print("meshes.bin", make_mesh_bank(api), "bytes")
+    meshes = S.load_mesh_bank(os.path.join(HERE, "meshes.bin"))
+    mesh_runs = {"terrain_300_41": hashes(S.mesh_terrain_drop(api, meshes, 300, 41, bank), 240),
+                 "field_200_42": hashes(S.mesh_field(api, meshes, 200, 42, bank), 240)}
+    with open(os.path.join(HERE, "mesh_hashes.json"), "w") as f:
+        json.dump({"scene": "mixed drops onto reference-built meshes (scenes.mesh_terrain_drop, "
+                            "scenes.mesh_field), dt=1/60 substeps=4",
+                   "runs": mesh_runs}, f)
     print("fixtures written")
 
 

@@ -46,7 +46,7 @@ LAYOUT_TYPES = {
     "NphPhysicsHull": None, "NphPhysicsEvents": None, "NphPhysicsContactHitEvent": None,
     "NphPhysicsBodyMoveEvent": None, "NphPhysicsJointDesc": P.JointDesc,
     "NphPhysicsBallJoint": P.BallJoint, "NphPhysicsHingeJoint": P.HingeJoint,
-    "NphPhysicsD6Joint": P.D6Joint,
+    "NphPhysicsD6Joint": P.D6Joint, "NphPhysicsMesh": P.Mesh, "NphPhysicsMeshNode": P.MeshNode,
 }
 
 
@@ -90,13 +90,13 @@ REF_INCLUDE = ["/root/reference/engine/physics/include", "/root/reference/engine
 def test_header_layouts_are_the_references(tmp_path):
     """Every public struct of include/nepheline_physics.h has the size and
     field offsets of the reference's (engine/physics/include/nepheline/
-    physics/{types,hull,world,body,shape}.h)."""
+    physics/{types,hull,world,body,shape,joint,mesh}.h)."""
     if not os.path.isdir(REF_INCLUDE[0]):
         pytest.skip("reference sources not present")
     mine = _layouts(tmp_path, [os.path.join(ROOT, "include")], "<nepheline_physics.h>")
     hdr = "<nepheline/physics/world.h>\n#include <nepheline/physics/body.h>\n" \
           "#include <nepheline/physics/shape.h>\n#include <nepheline/physics/hull.h>\n" \
-          "#include <nepheline/physics/joint.h>"
+          "#include <nepheline/physics/joint.h>\n#include <nepheline/physics/mesh.h>"
     ref = _layouts(tmp_path, REF_INCLUDE, hdr)
     assert mine == ref
 

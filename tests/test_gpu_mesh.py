@@ -1,0 +1,89 @@
+# SPDX-License-Identifier: Apache-2.0
+"""Convex shapes against triangle meshes on the GPU path (collide_mesh.c,
+contact.c:378-483): each mesh pair's up to four manifolds in its record and
+three child records, triangles collided one by one under the mesh's
+four-wide tree, merged by normal; bit for bit against the reference, on
+mesh blocks its own builder wrote (tests/golden/meshes.bin)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from nepheline_amd import physics as P
+from nepheline_amd import scenes as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mesh_bank():
+    return S.load_mesh_bank(os.path.join(GOLDEN, "meshes.bin"))
+
+
+def _runs():
+    with open(os.path.join(GOLDEN, "mesh_hashes.json")) as f:
+        return json.load(f)["runs"]
+
+
+def _check(w, want, name):
+    for i, h in enumerate(want):
+        w.step(1.0 / 60.0)
+        assert "%016x" % w.hash() == h, f"{name}: hash differs at step {i + 1}"
+
+
+def test_terrain_drop_bit_exact(gpu_api, hull_bank, mesh_bank):
+    print("mesh: 300 bodies onto a 512-triangle terrain, 240 steps", flush=True)
+    _check(S.mesh_terrain_drop(gpu_api, mesh_bank, 300, 41, hull_bank),
+           _runs()["terrain_300_41"], "terrain_300_41")
+
+
+def test_mesh_field_bit_exact(gpu_api, hull_bank, mesh_bank):
+    print("mesh: 200 bodies onto four patches (tilted, turned, two-sided), 240 steps",
+          flush=True)
+    _check(S.mesh_field(gpu_api, mesh_bank, 200, 42, hull_bank), _runs()["field_200_42"],
+           "field_200_42")
+
+
+def _records(w):
+    out = {}
+    for c in w.contacts():
+        pts = tuple((c.points[k].id, c.points[k].normal_impulse, c.points[k].separation,
+                     c.points[k].anchor_a.x, c.points[k].anchor_a.y, c.points[k].anchor_b.z,
+                     c.points[k].tangent_impulse.x) for k in range(c.point_count))
+        out[c.record] = (c.shape_a, c.shape_b, c.parent, c.child, c.color, c.point_count,
+                         c.normal.x, c.normal.y, c.normal.z, pts)
+    return out
+
+
+def test_mesh_chains_match_the_reference_record_by_record(gpu_api, ref_api, hull_bank,
+                                                          mesh_bank):
+    """Every contact record (the pairs' and their children, by record
+    index): shapes, chain links, colour, normal, points' ids, impulses,
+    separations and anchors; and every body's state, after 120 steps."""
+    print("mesh: live comparison with the reference, terrain 300 bodies, 120 steps", flush=True)
+    wg = S.mesh_terrain_drop(gpu_api, mesh_bank, 300, 41, hull_bank)
+    wr = S.mesh_terrain_drop(ref_api, mesh_bank, 300, 41, hull_bank)
+    for _ in range(120):
+        wg.step(1.0 / 60.0)
+        wr.step(1.0 / 60.0)
+    rg = _records(wg)
+    rr = _records(wr)
+    assert sorted(rg) == sorted(rr)
+    children_touching = 0
+    for k, v in rr.items():
+        assert rg[k] == v, k
+        children_touching += v[2] != P.NULL32 and v[5] > 0
+    assert children_touching > 50  # pairs with more than one manifold
+    np.testing.assert_array_equal(wg.body_state(), wr.body_state())
+
+
+def test_mesh_shapes_only_on_bodies_that_do_not_move_by_force(gpu_api, mesh_bank):
+    """shape.c:196-201: a mesh on a dynamic body is an invalid argument."""
+    w = P.World(gpu_api, S.DESC)
+    b = w.add_body(P.BODY_DYNAMIC, (0.0, 1.0, 0.0))
+    with pytest.raises(RuntimeError, match=str(P.INVALID_ARGUMENT)):
+        w.add_shape(b, P.geometry_mesh(mesh_bank.ptrs[1]))
+    w.close()
