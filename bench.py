@@ -16,7 +16,9 @@ the reference's own CPU step (oracle/_ref, its job system on up to 16 host
 threads) on a bounded sample of the same scene, and the learned warm start:
 a network trained on the device from plain bin drops of other seeds, the
 same scene and steps timed with it, the post-solve residual and penetration
-of both starts, and the network's forward kernel against the MFMA peak.
+of both starts, and the network's forward kernel against the MFMA peak; and
+the 100 000-body field (meshes, ragdolls, mixed bodies) on the GPU beside the
+reference's CPU step on a bounded sample of it.
 """
 import argparse
 import ctypes as C
@@ -230,6 +232,66 @@ def run_learned(args, bank):
     }
 
 
+def run_field(args, bank):
+    """The 100 000-body field (150 x 150 m: 200 mesh patches, 2 000 ragdolls
+    of 11 capsules and 10 joints, 78 000 mixed bodies): the GPU's contact
+    solves/s over the timed steps, and the reference's own CPU step on the
+    same scene over a bounded sample of the same steps."""
+    meshes = S.load_mesh_bank(os.path.join(ROOT, "tests", "golden", "meshes.bin"))
+    a = P.product_api()
+    a.lib.nph_amd_solved_contacts.restype = C.c_uint32
+    a.lib.nph_amd_solved_contacts.argtypes = [C.c_void_p]
+    t0 = time.perf_counter()
+    w = S.field(a, meshes, bank, seed=3)
+    setup = time.perf_counter() - t0
+    for _ in range(args.field_warmup):
+        w.step(DT)
+    solved = 0
+    t0 = time.perf_counter()
+    for _ in range(args.field_steps):
+        w.step(DT)
+        solved += a.lib.nph_amd_solved_contacts(w.ptr)
+    wall = time.perf_counter() - t0
+    bodies = len(w.bodies)
+    w.close()
+    out = {"workload": "field: %d bodies (200 mesh patches of 64 triangles, 2000 ragdolls, "
+                       "78000 mixed), 150 x 150 m, dt 1/60, 4 substeps" % bodies,
+           "value": solved * SUBSTEPS / wall, "unit": "contact-substeps/s",
+           "steps": args.field_steps, "warmup": args.field_warmup,
+           "ms_per_step": wall * 1e3 / args.field_steps, "setup_s": setup, "cpu_baseline": None}
+    path = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
+    if args.no_cpu or not os.path.exists(path):
+        return out
+    ra = P.api(path, "ref_shim_allocator")
+    ra.lib.ref_shim_graph_contacts.restype = C.c_uint32
+    ra.lib.ref_shim_graph_contacts.argtypes = [C.c_void_p]
+    ra.lib.ref_shim_jobs_create.restype = C.c_void_p
+    ra.lib.ref_shim_jobs_create.argtypes = [C.c_uint32]
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    jobs = ra.lib.ref_shim_jobs_create(max(cores - 1, 1)) if cores > 1 else None
+    wr = S.field(ra, meshes, bank, seed=3)
+    for _ in range(args.field_warmup):
+        wr.step(DT, jobs)
+    solved = 0
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < args.field_steps:
+        wr.step(DT, jobs)
+        solved += ra.lib.ref_shim_graph_contacts(wr.ptr)
+        steps += 1
+        if time.perf_counter() - t0 > args.cpu_seconds:
+            break
+    cwall = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": solved * SUBSTEPS / cwall, "unit": "contact-substeps/s",
+                           "cores": cores, "kind": "reference",
+                           "sample": "the reference's step (job system, %d threads) on the same "
+                                     "field, steps %d-%d (%.1f s)" % (
+                                         cores, args.field_warmup + 1,
+                                         args.field_warmup + steps, cwall)}
+    out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def run_cpu(args, bank):
     path = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
     if not os.path.exists(path):
@@ -291,6 +353,10 @@ def main():
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned warm start's training and runs")
+    ap.add_argument("--no-field", action="store_true",
+                    help="skip the 100 000-body field")
+    ap.add_argument("--field-steps", type=int, default=150)
+    ap.add_argument("--field-warmup", type=int, default=20)
     ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101])
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--train-epochs", type=int, default=8)
@@ -329,6 +395,7 @@ def main():
         t = json.load(open(tpath))
         traffic = t["hbm_bytes_per_contact_substep"] * k_cs / launches
     learned = None if args.no_learned else run_learned(args, bank)
+    field = None if args.no_field else run_field(args, bank)
     cpu = None if args.no_cpu else run_cpu(args, bank)
     line = {
         "metric": "contact solves/sec (contacts x substeps/s) plain vs learned warm start, "
@@ -368,6 +435,7 @@ def main():
                           cpu["cores"], args.warmup + 1, args.warmup + cpu["steps"], cpu["wall_s"]),
         },
         "learned_warm_start": learned,
+        "field_100k": field,
         "gpu_hash": g["hash"],
         "phases_ms_per_step": g["phases_ms"] if args.phases else None,
     }

@@ -419,6 +419,38 @@ def ragdoll_pile(api, count: int = 4, seed: int = 2026, desc: dict | None = None
     return w
 
 
+def field(api, meshes: MeshBank, hulls: HullBank | None, seed: int = 3, mixed: int = 78000,
+          ragdolls: int = 2000, patches: int = 200, half: float = 75.0,
+          desc: dict | None = None) -> P.World:
+    """A 150 x 150 m field: a ground slab; `patches` static mesh patches (the
+    bank's 64-triangle meshes 1-4 in turn) at random places and headings;
+    `ragdolls` eleven-body ragdolls dropped lying at random headings from
+    3-12 m; `mixed` bodies as the bin drop's from 2-20 m."""
+    d = dict(DESC)
+    d.update(desc or {})
+    w = P.World(api, d)
+    g = w.add_body(P.BODY_STATIC, (0.0, -1.0, 0.0))
+    w.add_shape(g, P.geometry_box(half + 1.0, 1.0, half + 1.0), friction=0.6)
+    r = Pcg32(seed)
+    for k in range(patches):
+        x = float(r.next_range(-half + 6.0, half - 6.0))
+        z = float(r.next_range(-half + 6.0, half - 6.0))
+        yaw = quat_axis_angle((0.0, 1.0, 0.0), _F(r.next_range(0.0, 6.2831853)))
+        b = w.add_body(P.BODY_STATIC, (x, 0.05, z), tuple(float(c) for c in yaw))
+        w.add_shape(b, P.geometry_mesh(meshes.ptrs[1 + k % 4]), friction=0.5)
+    lying = quat_axis_angle((1.0, 0.0, 0.0), _F(_F(-90.0) * DEG))
+    for _ in range(ragdolls):
+        x = float(r.next_range(-half + 3.0, half - 3.0))
+        y = float(r.next_range(3.0, 12.0))
+        z = float(r.next_range(-half + 3.0, half - 3.0))
+        turn = quat_mul(quat_axis_angle((0.0, 1.0, 0.0), _F(r.next_range(0.0, 6.2831853))), lying)
+        off = ragdoll_turn(turn, (0.0, 1.0, 0.0))
+        ragdoll(w, (x - off[0], y - off[1], z - off[2]), turn)
+    drop_mixed(w, r, mixed, hulls, -half + 3.0, half - 3.0, 2.0, 20.0)
+    w._keep.append(meshes)
+    return w
+
+
 def projectiles(api, count: int = 40, seed: int = 5, bullets: bool = True,
                 desc: dict | None = None) -> P.World:
     """Continuous collision: spheres, capsules and boxes shot at 60-150 m/s

@@ -13,7 +13,8 @@ oracle/_ref/libnph_ref.so (run in a container that has /root/reference).
                        (nph_physics_mesh_build, mesh.c) from seeded height
                        grids: a 16 x 16-cell terrain and four 4 x 8-cell
                        patches (the last two-sided)
-  mesh_hashes.json     world hashes of mixed drops onto them
+  mesh_hashes.json     world hashes of mixed drops onto them, and of a small
+                       field (scenes.field: meshes, ragdolls, mixed bodies)
 
 usage: python tests/golden/make_fixtures.py
 """
@@ -166,7 +167,9 @@ def main():
     print("meshes.bin", make_mesh_bank(api), "bytes")
     meshes = S.load_mesh_bank(os.path.join(HERE, "meshes.bin"))
     mesh_runs = {"terrain_300_41": hashes(S.mesh_terrain_drop(api, meshes, 300, 41, bank), 240),
-                 "field_200_42": hashes(S.mesh_field(api, meshes, 200, 42, bank), 240)}
+                 "field_200_42": hashes(S.mesh_field(api, meshes, 200, 42, bank), 240),
+                 "field_small_3": hashes(S.field(api, meshes, bank, seed=3, mixed=1500,
+                                                 ragdolls=40, patches=16, half=20.0), 200)}
     with open(os.path.join(HERE, "mesh_hashes.json"), "w") as f:
         json.dump({"scene": "mixed drops onto reference-built meshes (scenes.mesh_terrain_drop, "
                             "scenes.mesh_field), dt=1/60 substeps=4",

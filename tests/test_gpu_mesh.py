@@ -47,6 +47,15 @@ def test_mesh_field_bit_exact(gpu_api, hull_bank, mesh_bank):
            "field_200_42")
 
 
+def test_small_field_bit_exact(gpu_api, hull_bank, mesh_bank):
+    """The field scene at a small size: 16 mesh patches, 40 ragdolls (440
+    capsules, 400 ball and hinge joints), 1 500 mixed bodies."""
+    print("mesh: small field (patches, ragdolls, mixed), 200 steps", flush=True)
+    w = S.field(gpu_api, mesh_bank, hull_bank, seed=3, mixed=1500, ragdolls=40, patches=16,
+                half=20.0)
+    _check(w, _runs()["field_small_3"], "field_small_3")
+
+
 def _records(w):
     out = {}
     for c in w.contacts():
