@@ -425,6 +425,44 @@ void nph_physics_body_set_angular_velocity(NphPhysicsWorld *world, NphPhysicsBod
 NphPhysicsMassData nph_physics_body_mass_data(const NphPhysicsWorld *world, NphPhysicsBody body);
 u64 nph_physics_body_user_data(const NphPhysicsWorld *world, NphPhysicsBody body);
 bool nph_physics_body_is_awake(const NphPhysicsWorld *world, NphPhysicsBody body);
+// world.h:70-77 (inline there; exported here).
+bool nph_physics_body_is_null(NphPhysicsBody b);
+bool nph_physics_shape_is_null(NphPhysicsShape s);
+// body.c:132-180: its joints, contacts (neighbours that touched wake) and
+// shapes go first.
+void nph_physics_body_destroy(NphPhysicsWorld *world, NphPhysicsBody body);
+// body.c:236-262: a teleport; the shapes' proxies move now.
+void nph_physics_body_set_transform(NphPhysicsWorld *world, NphPhysicsBody body, NphDVec3 position,
+                                    NphQuat rotation);
+// body.c:316-386: forces and torques until the end of the next step;
+// impulses now.  Dynamic bodies only.
+void nph_physics_body_apply_force(NphPhysicsWorld *world, NphPhysicsBody body, NphVec3 force,
+                                  NphDVec3 point);
+void nph_physics_body_apply_force_at_center(NphPhysicsWorld *world, NphPhysicsBody body,
+                                            NphVec3 force);
+void nph_physics_body_apply_torque(NphPhysicsWorld *world, NphPhysicsBody body, NphVec3 torque);
+void nph_physics_body_apply_linear_impulse(NphPhysicsWorld *world, NphPhysicsBody body,
+                                           NphVec3 impulse, NphDVec3 point);
+void nph_physics_body_apply_angular_impulse(NphPhysicsWorld *world, NphPhysicsBody body,
+                                            NphVec3 impulse);
+// body.c:463-481: set mass properties (mass <= 0: back to the shapes').
+void nph_physics_body_set_mass_data(NphPhysicsWorld *world, NphPhysicsBody body,
+                                    const NphPhysicsMassData *mass);
+// body.c:483-604
+bool nph_physics_body_is_bullet(const NphPhysicsWorld *world, NphPhysicsBody body);
+void nph_physics_body_set_bullet(NphPhysicsWorld *world, NphPhysicsBody body, bool bullet);
+void nph_physics_body_set_damping(NphPhysicsWorld *world, NphPhysicsBody body, f32 linear,
+                                  f32 angular);
+void nph_physics_body_set_gravity_scale(NphPhysicsWorld *world, NphPhysicsBody body, f32 scale);
+void nph_physics_body_set_locks(NphPhysicsWorld *world, NphPhysicsBody body, u32 locks);
+void nph_physics_body_set_sleep(NphPhysicsWorld *world, NphPhysicsBody body, bool enable,
+                                f32 threshold);
+// Waking works as in the reference; putting a body to sleep is ignored (the
+// world runs with sleep off: its islands never sleep).
+void nph_physics_body_set_awake(NphPhysicsWorld *world, NphPhysicsBody body, bool awake);
+void nph_physics_body_set_user_data(NphPhysicsWorld *world, NphPhysicsBody body, u64 user_data);
+u32 nph_physics_body_shapes(const NphPhysicsWorld *world, NphPhysicsBody body, NphPhysicsShape *out,
+                            u32 capacity);
 
 // ---------------------------------------------------------------------------
 // Shapes (physics/shape.h:36-143)
@@ -470,6 +508,37 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *world, NphPhysicsBody body,
 bool nph_physics_shape_is_valid(const NphPhysicsWorld *world, NphPhysicsShape shape);
 NphPhysicsBody nph_physics_shape_body(const NphPhysicsWorld *world, NphPhysicsShape shape);
 NphAABB nph_physics_shape_aabb(const NphPhysicsWorld *world, NphPhysicsShape shape);
+// shape.c:29-42: the pair's friction and restitution for these values and modes.
+f32 nph_physics_combine_friction(f32 a, NphPhysicsCombine mode_a, f32 b, NphPhysicsCombine mode_b);
+f32 nph_physics_combine_restitution(f32 a, NphPhysicsCombine mode_a, f32 b,
+                                    NphPhysicsCombine mode_b);
+// shape.c:285-311: its contacts (touching neighbours wake) and proxy first.
+void nph_physics_shape_destroy(NphPhysicsWorld *world, NphPhysicsShape shape);
+// shape.c:320-440: the shape's settings; new surfaces apply from the next
+// step's narrow phase, a new density to the body's mass now, a new filter
+// ends the contacts it no longer allows now.
+NphPhysicsGeometry nph_physics_shape_geometry(const NphPhysicsWorld *world, NphPhysicsShape shape);
+NphPhysicsPose nph_physics_shape_pose(const NphPhysicsWorld *world, NphPhysicsShape shape);
+f32 nph_physics_shape_friction(const NphPhysicsWorld *world, NphPhysicsShape shape);
+f32 nph_physics_shape_restitution(const NphPhysicsWorld *world, NphPhysicsShape shape);
+NphPhysicsCombine nph_physics_shape_friction_combine(const NphPhysicsWorld *world,
+                                                     NphPhysicsShape shape);
+NphPhysicsCombine nph_physics_shape_restitution_combine(const NphPhysicsWorld *world,
+                                                        NphPhysicsShape shape);
+void nph_physics_shape_set_friction(NphPhysicsWorld *world, NphPhysicsShape shape, f32 friction);
+void nph_physics_shape_set_restitution(NphPhysicsWorld *world, NphPhysicsShape shape,
+                                       f32 restitution);
+void nph_physics_shape_set_friction_combine(NphPhysicsWorld *world, NphPhysicsShape shape,
+                                            NphPhysicsCombine mode);
+void nph_physics_shape_set_restitution_combine(NphPhysicsWorld *world, NphPhysicsShape shape,
+                                               NphPhysicsCombine mode);
+f32 nph_physics_shape_density(const NphPhysicsWorld *world, NphPhysicsShape shape);
+void nph_physics_shape_set_density(NphPhysicsWorld *world, NphPhysicsShape shape, f32 density);
+NphPhysicsFilter nph_physics_shape_filter(const NphPhysicsWorld *world, NphPhysicsShape shape);
+void nph_physics_shape_set_filter(NphPhysicsWorld *world, NphPhysicsShape shape,
+                                  NphPhysicsFilter filter);
+u64 nph_physics_shape_user_data(const NphPhysicsWorld *world, NphPhysicsShape shape);
+void nph_physics_shape_set_user_data(NphPhysicsWorld *world, NphPhysicsShape shape, u64 user_data);
 
 // ---------------------------------------------------------------------------
 // Joints (joint.h:51-272).  Ball, hinge and fixed joints are solved; the
@@ -597,6 +666,7 @@ void nph_physics_joint_frames_at(const NphPhysicsWorld *world, NphPhysicsBody a,
 NphResult nph_physics_joint_create(NphPhysicsWorld *world, const NphPhysicsJointDesc *desc,
                                    NphPhysicsJoint *out);                          // joint.c:357-443
 void nph_physics_joint_destroy(NphPhysicsWorld *world, NphPhysicsJoint joint);     // joint.c:466-473
+bool nph_physics_joint_is_null(NphPhysicsJoint j); // joint.h:55-58 (inline there)
 bool nph_physics_joint_is_valid(const NphPhysicsWorld *world, NphPhysicsJoint joint);
 NphPhysicsJointDesc nph_physics_joint_desc(const NphPhysicsWorld *world,
                                            NphPhysicsJoint joint);                 // joint.c:475-509

@@ -429,6 +429,10 @@ NphResult nph_physics_joint_create(NphPhysicsWorld *w, const NphPhysicsJointDesc
     return NPH_RESULT_OK;
 }
 
+void joints_destroy(NphPhysicsWorld *w, u32 joint, int wake) { joint_destroy(w, joint, wake); }
+
+bool nph_physics_joint_is_null(NphPhysicsJoint j) { return j.value == 0U; }
+
 void nph_physics_joint_destroy(NphPhysicsWorld *w, NphPhysicsJoint joint)
 {
     if (w == NULL || w->stepping || world_joint(w, joint) == NULL) {

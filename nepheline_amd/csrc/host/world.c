@@ -442,25 +442,29 @@ static const void *geometry_block(const NphPhysicsGeometry *g)
 
 static NphGeom host_geom(const HShape *s) { return geom_from(&s->geometry, geometry_block(&s->geometry)); }
 
-// body.c:392-448
+NphBodyRec *world_rec(NphPhysicsWorld *w, u32 slot) { return rec_at(w, slot); }
+
+// body.c:392-448 (the caller's mass properties kept when set)
 static void update_mass(NphPhysicsWorld *w, u32 slot)
 {
     HBody *hb = &VB(w->bodies, HBody)[slot];
     NphBodyRec *b = rec_at(w, slot);
-    NphPhysicsMassData total;
-    memset(&total, 0, sizeof total);
     f32 extent = 0.0F;
-    for (u32 s = hb->shape_head; s != NPH_NULL32; s = shape_slot(w, s)->next) {
-        const HShape *shape = shape_slot(w, s);
-        if (shape->density <= 0.0F || (shape->flags & (HS_SENSOR | HS_GHOST)) != 0U) {
-            continue;
+    if ((hb->flags & HB_MASS_SET) == 0U) {
+        NphPhysicsMassData total;
+        memset(&total, 0, sizeof total);
+        for (u32 s = hb->shape_head; s != NPH_NULL32; s = shape_slot(w, s)->next) {
+            const HShape *shape = shape_slot(w, s);
+            if (shape->density <= 0.0F || (shape->flags & (HS_SENSOR | HS_GHOST)) != 0U) {
+                continue;
+            }
+            const NphGeom g = host_geom(shape);
+            total = mass_add(total, mass_transform(geom_mass(&g, shape->density), shape->pose));
         }
-        const NphGeom g = host_geom(shape);
-        total = mass_add(total, mass_transform(geom_mass(&g, shape->density), shape->pose));
+        b->mass = total.mass;
+        b->local_center = total.center;
+        b->inertia = total.inertia;
     }
-    b->mass = total.mass;
-    b->local_center = total.center;
-    b->inertia = total.inertia;
     f32 inner = NPH_F32_MAX;
     for (u32 s = hb->shape_head; s != NPH_NULL32; s = shape_slot(w, s)->next) {
         const HShape *shape = shape_slot(w, s);
@@ -788,6 +792,15 @@ NphResult world_shape_record(NphPhysicsWorld *w, u32 s, NphShapeRec *out)
 }
 
 NphGeom world_host_geom(const HShape *s) { return host_geom(s); }
+
+void world_update_mass(NphPhysicsWorld *w, u32 slot) { update_mass(w, slot); }
+
+NphResult world_refresh_shape(NphPhysicsWorld *w, u32 s)
+{
+    NphShapeRec rec;
+    NphResult r = world_shape_record(w, s, &rec);
+    return r == NPH_RESULT_OK ? nph_dev_write_shape(w->dev, s, &rec) : r;
+}
 
 NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
                                    const NphPhysicsShapeDesc *desc, NphPhysicsShape *out)

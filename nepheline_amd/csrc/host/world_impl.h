@@ -65,6 +65,9 @@ void tree_enlarge_proxy(NphTree *t, u32 proxy, NphAABB aabb);
 void tree_query(const NphTree *t, NphAABB aabb, u64 mask, NphTreeQueryFn fn, void *ctx);
 NphResult tree_rebuild(NphTree *t, int full);
 u64 tree_user_data(const NphTree *t, u32 proxy);
+void tree_destroy_proxy(NphTree *t, u32 proxy);
+void tree_move_proxy(NphTree *t, u32 proxy, NphAABB aabb);
+void tree_set_category_bits(NphTree *t, u32 proxy, u64 category_bits);
 
 // ---------------------------------------------------------------------------
 // Records
@@ -74,6 +77,7 @@ enum {
     HB_ENABLE_SLEEP = 1U << 1U,
     HB_GYROSCOPIC = 1U << 2U,
     HB_BULLET = 1U << 5U,
+    HB_MASS_SET = 1U << 6U, // mass properties set by the caller
 };
 
 typedef struct HBody {
@@ -255,6 +259,10 @@ NphResult continuous_sweep(NphPhysicsWorld *w, const NphFastBody *fast, u32 coun
 // joint.c restated
 int joints_allow_contact(const NphPhysicsWorld *w, u32 body_a, u32 body_b);
 void joints_free(NphPhysicsWorld *w);
+void joints_destroy(NphPhysicsWorld *w, u32 joint, int wake);
+void world_update_mass(NphPhysicsWorld *w, u32 slot);
+NphBodyRec *world_rec(NphPhysicsWorld *w, u32 slot);
+NphResult world_refresh_shape(NphPhysicsWorld *w, u32 shape); // host record -> device
 
 NphResult pairmap_init(NphAllocator *a, NphPairMap *m);
 void pairmap_free(NphAllocator *a, NphPairMap *m);

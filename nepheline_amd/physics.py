@@ -232,6 +232,60 @@ class Api:
                                               [C.c_void_p, _U32, Vec3])
         self.body_set_angular_velocity = _bind(L, "nph_physics_body_set_angular_velocity", None,
                                                [C.c_void_p, _U32, Vec3])
+        self.body_destroy = _bind(L, "nph_physics_body_destroy", None, [C.c_void_p, _U32])
+        self.body_set_transform = _bind(L, "nph_physics_body_set_transform", None,
+                                        [C.c_void_p, _U32, DVec3, Quat])
+        self.body_apply_force = _bind(L, "nph_physics_body_apply_force", None,
+                                      [C.c_void_p, _U32, Vec3, DVec3])
+        self.body_apply_force_at_center = _bind(L, "nph_physics_body_apply_force_at_center", None,
+                                                [C.c_void_p, _U32, Vec3])
+        self.body_apply_torque = _bind(L, "nph_physics_body_apply_torque", None,
+                                       [C.c_void_p, _U32, Vec3])
+        self.body_apply_linear_impulse = _bind(L, "nph_physics_body_apply_linear_impulse", None,
+                                               [C.c_void_p, _U32, Vec3, DVec3])
+        self.body_apply_angular_impulse = _bind(L, "nph_physics_body_apply_angular_impulse", None,
+                                                [C.c_void_p, _U32, Vec3])
+        self.body_set_mass_data = _bind(L, "nph_physics_body_set_mass_data", None,
+                                        [C.c_void_p, _U32, C.POINTER(MassData)])
+        self.body_set_bullet = _bind(L, "nph_physics_body_set_bullet", None,
+                                     [C.c_void_p, _U32, C.c_bool])
+        self.body_is_bullet = _bind(L, "nph_physics_body_is_bullet", C.c_bool, [C.c_void_p, _U32])
+        self.body_set_damping = _bind(L, "nph_physics_body_set_damping", None,
+                                      [C.c_void_p, _U32, C.c_float, C.c_float])
+        self.body_set_gravity_scale = _bind(L, "nph_physics_body_set_gravity_scale", None,
+                                            [C.c_void_p, _U32, C.c_float])
+        self.body_set_locks = _bind(L, "nph_physics_body_set_locks", None,
+                                    [C.c_void_p, _U32, _U32])
+        self.body_set_sleep = _bind(L, "nph_physics_body_set_sleep", None,
+                                    [C.c_void_p, _U32, C.c_bool, C.c_float])
+        self.body_set_awake = _bind(L, "nph_physics_body_set_awake", None,
+                                    [C.c_void_p, _U32, C.c_bool])
+        self.body_set_user_data = _bind(L, "nph_physics_body_set_user_data", None,
+                                        [C.c_void_p, _U32, C.c_uint64])
+        self.body_user_data = _bind(L, "nph_physics_body_user_data", C.c_uint64,
+                                    [C.c_void_p, _U32])
+        self.body_shapes = _bind(L, "nph_physics_body_shapes", _U32,
+                                 [C.c_void_p, _U32, C.POINTER(_U32), _U32])
+        self.shape_destroy = _bind(L, "nph_physics_shape_destroy", None, [C.c_void_p, _U32])
+        self.shape_friction = _bind(L, "nph_physics_shape_friction", C.c_float, [C.c_void_p, _U32])
+        self.shape_set_friction = _bind(L, "nph_physics_shape_set_friction", None,
+                                        [C.c_void_p, _U32, C.c_float])
+        self.shape_set_restitution = _bind(L, "nph_physics_shape_set_restitution", None,
+                                           [C.c_void_p, _U32, C.c_float])
+        self.shape_set_friction_combine = _bind(L, "nph_physics_shape_set_friction_combine", None,
+                                                [C.c_void_p, _U32, C.c_int])
+        self.shape_friction_combine = _bind(L, "nph_physics_shape_friction_combine", C.c_int,
+                                            [C.c_void_p, _U32])
+        self.shape_set_density = _bind(L, "nph_physics_shape_set_density", None,
+                                       [C.c_void_p, _U32, C.c_float])
+        self.shape_density = _bind(L, "nph_physics_shape_density", C.c_float, [C.c_void_p, _U32])
+        self.shape_set_filter = _bind(L, "nph_physics_shape_set_filter", None,
+                                      [C.c_void_p, _U32, Filter])
+        self.shape_filter = _bind(L, "nph_physics_shape_filter", Filter, [C.c_void_p, _U32])
+        self.combine_friction = _bind(L, "nph_physics_combine_friction", C.c_float,
+                                      [C.c_float, C.c_int, C.c_float, C.c_int])
+        self.combine_restitution = _bind(L, "nph_physics_combine_restitution", C.c_float,
+                                         [C.c_float, C.c_int, C.c_float, C.c_int])
         self.shape_desc_default = _bind(L, "nph_physics_shape_desc_default", ShapeDesc,
                                         [Geometry])
         self.shape_create = _bind(L, "nph_physics_shape_create", C.c_int,

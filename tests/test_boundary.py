@@ -206,3 +206,20 @@ def test_golden_pyramid_hashes_are_the_references(ref_api):
     for i in range(40):
         w.step(1.0 / 60.0)
         assert "%016x" % w.hash() == want[i]
+
+
+def test_combine_rules_match_the_reference(ref_api):
+    """shape.c:29-42 (and world_internal.h:119-155): every pair of modes on
+    a grid of values, bit for bit."""
+    if not os.path.exists(P.PRODUCT_LIB):
+        pytest.skip("libnepheline_amd.so not built")
+    a = P.product_api()
+    vals = [0.0, 0.1, 0.3, 0.5, 0.77, 1.0]
+    for ma in range(5):
+        for mb in range(5):
+            for x in vals:
+                for y in vals:
+                    assert a.combine_friction(x * 2, ma, y * 2, mb) == \
+                        ref_api.combine_friction(x * 2, ma, y * 2, mb)
+                    assert a.combine_restitution(x, ma, y, mb) == \
+                        ref_api.combine_restitution(x, ma, y, mb)
