@@ -301,6 +301,7 @@ void nph_physics_world_destroy(NphPhysicsWorld *w)
     vbuf_free(a, &w->changed, 4U);
     vbuf_free(a, &w->codes, 4U);
     vbuf_free(a, &w->begin_events, sizeof(NphPhysicsContactBeginEvent));
+    vbuf_free(a, &w->hit_events, sizeof(NphPhysicsContactHitEvent));
     vbuf_free(a, &w->end_events, sizeof(NphPhysicsContactEndEvent));
     vbuf_free(a, &w->move_events, sizeof(NphPhysicsBodyMoveEvent));
     vbuf_free(a, &w->contact_view, sizeof(NphContactRec));
@@ -963,6 +964,78 @@ static f64 now_ms(void)
     return (f64)ts.tv_sec * 1e3 + (f64)ts.tv_nsec * 1e-6;
 }
 
+static NphResult read_contacts(NphPhysicsWorld *w);
+
+// solver.c:919-958: after the step, one hit a pair for the graph's
+// contacts that asked for them, in colour order: the pair's fastest
+// approaching point (over a mesh pair's chain) that carried load, when it
+// beats the world's threshold.  The records are read back only when some
+// contact in the graph wants hits.
+static NphResult hit_events(NphPhysicsWorld *w)
+{
+    int any = 0;
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS && !any; ++c) {
+        const u32 *list = VB(w->color_contacts[c], u32);
+        for (u32 i = 0U; i < w->color_contacts[c].len; ++i) {
+            if ((VB(w->contacts, HContact)[list[i]].flags & HC_HIT_EVENTS) != 0U) {
+                any = 1;
+                break;
+            }
+        }
+    }
+    if (!any) {
+        return NPH_RESULT_OK;
+    }
+    NphResult r = read_contacts(w);
+    if (r == NPH_RESULT_OK) {
+        r = world_sync_bodies(w);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    const NphContactRec *recs = VB(w->contact_view, NphContactRec);
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+        const u32 *list = VB(w->color_contacts[c], u32);
+        for (u32 i = 0U; i < w->color_contacts[c].len; ++i) {
+            const HContact *contact = &VB(w->contacts, HContact)[list[i]];
+            if ((contact->flags & HC_HIT_EVENTS) == 0U) {
+                continue;
+            }
+            f32 best = w->hit_threshold;
+            u32 hit = NPH_NULL32;
+            u32 best_point = NPH_NULL32;
+            for (u32 k = list[i]; k != NPH_NULL32; k = VB(w->contacts, HContact)[k].child) {
+                const NphContactRec *record = &recs[k];
+                for (u32 j = 0U; j < record->point_count; ++j) {
+                    const NphPointRec *p = &record->points[j];
+                    const f32 approach = -p->relative_velocity;
+                    if (approach > best && p->max_normal_impulse > 0.0F) {
+                        best = approach;
+                        hit = k;
+                        best_point = j;
+                    }
+                }
+            }
+            if (hit == NPH_NULL32) {
+                continue;
+            }
+            NphPhysicsContactHitEvent e;
+            memset(&e, 0, sizeof e);
+            e.shape_a.value = VB(w->shapes, HShape)[contact->shape_a].handle;
+            e.shape_b.value = VB(w->shapes, HShape)[contact->shape_b].handle;
+            e.point = dv3_add_v3(rec_at(w, contact->body_a)->center,
+                                 recs[hit].points[best_point].anchor_a);
+            e.normal = recs[hit].normal;
+            e.approach_speed = best;
+            r = vbuf_push(w->alloc, &w->hit_events, sizeof e, &e);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+        }
+    }
+    return NPH_RESULT_OK;
+}
+
 static NphResult solve(NphPhysicsWorld *w, f32 dt)
 {
     const u32 n = w->awake_bodies.len;
@@ -1051,7 +1124,8 @@ static NphResult solve(NphPhysicsWorld *w, f32 dt)
     if (r != NPH_RESULT_OK) {
         return r;
     }
-    return contacts_post_solve(w, grown, grown_count);
+    r = contacts_post_solve(w, grown, grown_count);
+    return r == NPH_RESULT_OK ? hit_events(w) : r;
 }
 
 NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
@@ -1065,6 +1139,7 @@ NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
     w->stepping = 1;
     w->step_dt = dt;
     w->begin_events.len = 0U;
+    w->hit_events.len = 0U;
     w->end_events.len = 0U;
     w->move_events.len = 0U;
     w->move_events_ready = 0;
@@ -1127,6 +1202,8 @@ NphPhysicsEvents nph_physics_world_events(const NphPhysicsWorld *cw)
     e.begin_count = w->begin_events.len;
     e.end = VB(w->end_events, NphPhysicsContactEndEvent);
     e.end_count = w->end_events.len;
+    e.hit = VB(w->hit_events, NphPhysicsContactHitEvent);
+    e.hit_count = w->hit_events.len;
     e.move = VB(w->move_events, NphPhysicsBodyMoveEvent);
     e.move_count = w->move_events.len;
     return e;

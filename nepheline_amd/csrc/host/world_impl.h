@@ -216,6 +216,7 @@ struct NphPhysicsWorld {
     NphVecBuf codes;   // u32
 
     NphVecBuf begin_events; // NphPhysicsContactBeginEvent
+    NphVecBuf hit_events;   // NphPhysicsContactHitEvent
     NphVecBuf end_events;
     NphVecBuf move_events; // NphPhysicsBodyMoveEvent, built on request
     int move_events_ready;

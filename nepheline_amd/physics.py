@@ -124,6 +124,36 @@ class ContactInfo(C.Structure):
                 ("points", ContactPointInfo * 4)]
 
 
+class PairEvent(C.Structure):
+    """Contact begin and end events (world.h:175-186)."""
+    _fields_ = [("shape_a", C.c_uint32), ("shape_b", C.c_uint32)]
+
+
+class HitEvent(C.Structure):
+    _fields_ = [("shape_a", C.c_uint32), ("shape_b", C.c_uint32), ("point", DVec3),
+                ("normal", Vec3), ("approach_speed", C.c_float)]
+
+
+class MoveEvent(C.Structure):
+    _fields_ = [("body", C.c_uint32), ("fell_asleep", C.c_uint32), ("user_data", C.c_uint64),
+                ("position", DVec3), ("rotation", Quat), ("linear_velocity", Vec3),
+                ("angular_velocity", Vec3)]
+
+
+class SensorEvent(C.Structure):
+    _fields_ = [("sensor", C.c_uint32), ("visitor", C.c_uint32),
+                ("sensor_user_data", C.c_uint64), ("visitor_user_data", C.c_uint64)]
+
+
+class Events(C.Structure):
+    _fields_ = [("begin", C.POINTER(PairEvent)), ("begin_count", C.c_uint32),
+                ("end", C.POINTER(PairEvent)), ("end_count", C.c_uint32),
+                ("hit", C.POINTER(HitEvent)), ("hit_count", C.c_uint32),
+                ("move", C.POINTER(MoveEvent)), ("move_count", C.c_uint32),
+                ("sensor_begin", C.POINTER(SensorEvent)), ("sensor_begin_count", C.c_uint32),
+                ("sensor_end", C.POINTER(SensorEvent)), ("sensor_end_count", C.c_uint32)]
+
+
 class BallJoint(C.Structure):
     _fields_ = [("enable_swing_limit", C.c_int32), ("swing_y", C.c_float), ("swing_z", C.c_float),
                 ("enable_twist_limit", C.c_int32), ("twist_lower", C.c_float),
@@ -212,6 +242,9 @@ class Api:
                                 [C.c_void_p, C.c_float, C.c_void_p])
         self.world_hash = _bind(L, "nph_physics_world_hash", C.c_uint64, [C.c_void_p])
         self.world_stats = _bind(L, "nph_physics_world_stats", Stats, [C.c_void_p])
+        self.world_events = _bind(L, "nph_physics_world_events", Events, [C.c_void_p])
+        self.world_set_hit_threshold = _bind(L, "nph_physics_world_set_hit_threshold", None,
+                                             [C.c_void_p, C.c_float])
         self.world_validate = _bind(L, "nph_physics_world_validate", C.c_bool, [C.c_void_p])
         self.world_contacts = _bind(L, "nph_physics_world_contacts", _U32,
                                     [C.c_void_p, C.POINTER(ContactInfo), _U32])
@@ -479,6 +512,18 @@ class World:
 
     def stats(self) -> Stats:
         return self.api.world_stats(self.ptr)
+
+    def events(self):
+        """The last step's events as tuples: (begin, end, hit, move)."""
+        e = self.api.world_events(self.ptr)
+        begin = [(e.begin[i].shape_a, e.begin[i].shape_b) for i in range(e.begin_count)]
+        end = [(e.end[i].shape_a, e.end[i].shape_b) for i in range(e.end_count)]
+        hit = [(h.shape_a, h.shape_b, h.point.x, h.point.y, h.point.z, h.normal.x, h.normal.y,
+                h.normal.z, h.approach_speed) for h in (e.hit[i] for i in range(e.hit_count))]
+        move = [(m.body, m.fell_asleep, m.position.x, m.position.y, m.position.z, m.rotation.w,
+                 m.linear_velocity.y, m.angular_velocity.x)
+                for m in (e.move[i] for i in range(e.move_count))]
+        return begin, end, hit, move
 
     def contacts(self):
         n = self.api.world_contacts(self.ptr, None, 0)

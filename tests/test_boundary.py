@@ -43,10 +43,11 @@ LAYOUT_TYPES = {
     "NphPhysicsPose": P.Pose, "NphPhysicsMassData": P.MassData, "NphPhysicsGeometry": P.Geometry,
     "NphPhysicsWorldDesc": P.WorldDesc, "NphPhysicsBodyDesc": P.BodyDesc,
     "NphPhysicsFilter": P.Filter, "NphPhysicsShapeDesc": P.ShapeDesc, "NphPhysicsStats": P.Stats,
-    "NphPhysicsHull": None, "NphPhysicsEvents": None, "NphPhysicsContactHitEvent": None,
-    "NphPhysicsBodyMoveEvent": None, "NphPhysicsJointDesc": P.JointDesc,
+    "NphPhysicsHull": None, "NphPhysicsEvents": P.Events,
+    "NphPhysicsContactHitEvent": P.HitEvent, "NphPhysicsBodyMoveEvent": P.MoveEvent, "NphPhysicsJointDesc": P.JointDesc,
     "NphPhysicsBallJoint": P.BallJoint, "NphPhysicsHingeJoint": P.HingeJoint,
     "NphPhysicsD6Joint": P.D6Joint, "NphPhysicsMesh": P.Mesh, "NphPhysicsMeshNode": P.MeshNode,
+    "NphPhysicsContactBeginEvent": P.PairEvent, "NphPhysicsSensorEvent": P.SensorEvent,
 }
 
 

@@ -108,3 +108,32 @@ def test_getters_report_the_settings(gpu_api, hull_bank):
     assert not a.lib.nph_physics_body_is_valid(w.ptr, b)
     w.step(1.0 / 60.0)
     w.close()
+
+
+def _eventful(api, hull_bank):
+    """A small bin drop whose shapes all ask for contact and hit events."""
+    w = P.World(api, S.DESC)
+    add = w.add_shape
+    w.add_shape = lambda b, g, **kw: add(b, g, hit_events=1, **kw)
+    floor = w.add_body(P.BODY_STATIC, (0.0, -1.0, 0.0))
+    w.add_shape(floor, P.geometry_box(5.0, 1.0, 5.0), friction=0.5)
+    S.drop_mixed(w, S.Pcg32(15), 120, hull_bank, -3.0, 3.0, 1.0, 8.0)
+    api.world_set_hit_threshold(w.ptr, 0.5)
+    return w
+
+
+def test_events_match_the_reference(gpu_api, ref_api, hull_bank):
+    """Begin, end, hit and move events of every step, in order, bit for bit."""
+    print("api: events of a 120-body drop with hit events, 150 steps", flush=True)
+    wg = _eventful(gpu_api, hull_bank)
+    wr = _eventful(ref_api, hull_bank)
+    hits = begins = 0
+    for i in range(150):
+        wg.step(1.0 / 60.0)
+        wr.step(1.0 / 60.0)
+        eg, er = wg.events(), wr.events()
+        for kind, a, b in zip(("begin", "end", "hit", "move"), eg, er):
+            assert a == b, f"{kind} events differ at step {i + 1}"
+        begins += len(eg[0])
+        hits += len(eg[2])
+    assert begins > 100 and hits > 20
