@@ -872,6 +872,62 @@ NphResult nph_amd_trainer_loss(NphAmdTrainer *t, f32 *loss);
 // NPH_AMD_MLP_PARAMS), the rest into *out (out->params = params).
 NphResult nph_amd_trainer_warm_start(const NphAmdTrainer *t, f32 *params, NphAmdWarmStart *out);
 
+// ---------------------------------------------------------------------------
+// Batched worlds (no counterpart in the reference; for callers that step
+// many small independent worlds, such as parameter sweeps and training
+// environments): a batch creates `world_count` worlds that share one
+// NphPhysicsWorldDesc and one device stream; each is filled and read through
+// the world API above, and nph_amd_batch_step steps all of them, each
+// stage of the step one launch over every world's items.  Each world's
+// results are bit for bit those it gets stepped alone.  The batch owns its
+// worlds: destroy the batch, not the worlds.  Continuous collision is not
+// batched (creation returns NPH_RESULT_UNSUPPORTED), nor is the learned warm
+// start (the step returns NPH_RESULT_UNSUPPORTED while a world has one, or
+// measures or logs).
+// ---------------------------------------------------------------------------
+typedef struct NphAmdBatch NphAmdBatch;
+
+// Launch choices of the batched step.
+typedef struct NphAmdBatchTuning {
+    u32 pass_block; // threads a block of the colour passes: 64, 128 or 256
+    u32 body_block; // threads a block of the per-body and per-lane stages
+    u32 use_graph;  // the substep loop launched as a captured HIP graph
+    u32 pad;
+} NphAmdBatchTuning;
+
+typedef struct NphAmdBatchStats {
+    u64 solved_contacts; // the last step's constraint-graph contacts, all worlds
+    u64 awake_bodies;    // the last step's awake bodies, all worlds
+    f64 step_ms;         // wall time of the last step
+    f64 pairs_ms;        // host: new pairs (worlds in parallel)
+    f64 changes_ms;      // host: narrow-phase changes and solver input
+    f64 solve_ms;        // the solve: lists up, device stages, grown proxies back
+    f64 post_ms;         // host: grown proxies into the trees, events
+    f64 device_solve_ms; // device time of the solve (events)
+    f64 device_passes_ms;
+    f64 device_collide_ms;
+} NphAmdBatchStats;
+
+NphResult nph_amd_batch_create(const NphPhysicsWorldDesc *desc, u32 world_count,
+                               NphAmdBatch **out);
+void nph_amd_batch_destroy(NphAmdBatch *batch);
+u32 nph_amd_batch_world_count(const NphAmdBatch *batch);
+NphPhysicsWorld *nph_amd_batch_world(NphAmdBatch *batch, u32 index);
+NphResult nph_amd_batch_step(NphAmdBatch *batch, f32 dt);
+NphAmdBatchStats nph_amd_batch_stats(const NphAmdBatch *batch);
+void nph_amd_batch_set_tuning(NphAmdBatch *batch, const NphAmdBatchTuning *tuning);
+NphAmdBatchTuning nph_amd_batch_tuning(const NphAmdBatch *batch);
+// The autotuner: steps the batch under every candidate launch choice in
+// turn, `rounds` times round-robin (the worlds advance rounds x candidates
+// steps), keeps the one whose solve took the least wall time in total, and
+// returns it in *chosen.
+NphResult nph_amd_batch_autotune(NphAmdBatch *batch, f32 dt, u32 rounds,
+                                 NphAmdBatchTuning *chosen);
+// HIP events around every colour-pass launch (not in graph mode): out[0] =
+// device milliseconds summed since enabling, out[1] = launches.
+void nph_amd_batch_time_kernels(NphAmdBatch *batch, bool enable);
+void nph_amd_batch_kernel_times(const NphAmdBatch *batch, f64 out[2]);
+
 #ifdef __cplusplus
 }
 #endif

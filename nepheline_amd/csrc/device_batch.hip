@@ -1,0 +1,1221 @@
+// SPDX-License-Identifier: Apache-2.0
+// Many independent worlds stepped together on one MI355X (gfx950).
+//
+// Each world keeps its own records (bodies, shapes, contacts, joints: its
+// NphDevice), and the host keeps each world's bookkeeping.  What is merged is
+// the launches: every stage of the step is one launch over the items of all
+// the worlds, so a batch of 1 024 worlds of 16 to 20 000 bodies costs the
+// launches of one world, and each colour pass fills the chip with the lanes
+// of every world instead of the few hundred of one.
+//
+// Work is laid out in waves.  A wave (64 threads) serves one world: a wave
+// table lists, per wave, the world and the world-local index range it
+// covers.  The wave index is wave-uniform, so the table entry and the
+// world's descriptor (NphStepData: the world's record arrays and its slices
+// of the batch's per-step arrays) load through the scalar unit, and the
+// per-thread code is the one-world step's (device_impl.h), operation for
+// operation: a world's results are bit for bit the ones it gets alone.
+//
+// Per-step arrays (solver bodies, lanes, points, groups) are the batch's,
+// each world a contiguous slice; lane and point fields stay structure of
+// arrays over the whole batch (one stride), so a world's slice is an offset
+// of the base pointer.  A world's groups never mix with another's: each
+// colour's contacts are grouped eight at a time per world, as the reference
+// groups them (solver.c:810-843).
+
+#include "device_impl.h"
+
+#define NPH_WAVE 64U
+#define NPH_COLLIDE_BLOCK 64U
+#define NPH_SCAN 1024U
+#define NPH_SLOTS (NPH_GRAPH_COLORS + 1U)
+
+typedef struct NphWave {
+    u32 world;
+    u32 base; // first index this wave covers
+    u32 end;  // the world's end of the range
+    u32 pad;
+} NphWave;
+
+// One world's share of a colour: groups [first, first + count), joints
+// [jfirst, jfirst + jcount), both world-local.
+typedef struct NphRange {
+    u32 first, count, jfirst, jcount;
+} NphRange;
+
+typedef struct NphCollideWorld {
+    NphContactRec *contacts;
+    const NphShapeRec *shapes;
+    const NphBodyRec *bodies;
+    u32 *gate;
+} NphCollideWorld;
+
+// A pinned host buffer and its device twin, filled on the host and sent up
+// with one copy.
+typedef struct Stage {
+    u8 *host;
+    u8 *dev;
+    u64 cap;
+    u64 used;
+} Stage;
+
+typedef struct Half {
+    Stage stage;
+    u32 *codes; // device: change codes per list entry
+    u32 cap_codes;
+    u32 list_count;
+} Half;
+
+enum {
+    T_SLOTS = 0,  // body slots (the solver map)
+    T_BODIES,     // awake bodies + the dummy
+    T_GROUPS,
+    T_LANES,
+    T_JOINTS,
+    T_COLOR0,     // + colour: that colour's joints (whole waves), then lanes
+    T_COUNT = T_COLOR0 + NPH_GRAPH_COLORS
+};
+
+struct NphDeviceBatch {
+    hipStream_t stream;
+    hipEvent_t ev[8]; // solve [0, 1], passes [2, 3], narrow phase halves [4, 5] and [6, 7]
+    NphBatchTuning tune;
+    Half half[2];
+    u32 *h_codes; // both halves' codes, read back
+    u64 cap_h_codes;
+    Stage solve;
+    // Per-step device arrays of the solve, the worlds' slices.
+    NphSolverBody *sbodies; // awake bodies + one dummy a world
+    NphSolverParams *params;
+    u32 cap_sbodies, cap_params;
+    u32 *lane_body_a, *lane_body_b, *lane_flags;
+    f32 *lanes;
+    u32 cap_lanes; // lanes (groups * 8): the lane fields' stride
+    f32 *points;
+    u32 cap_point_lanes; // point rows * 8: the point fields' stride
+    u32 *gcount, *gpos, *bsum; // grown shapes per awake body, their places
+    u32 cap_gbodies, cap_bsum;
+    NphGrown *grown;
+    u32 cap_grown;
+    u32 *wstart; // per world: first grown record; [n] = total
+    u32 cap_wstart;
+    NphGrown *h_grown;
+    u64 cap_h_grown;
+    u32 *h_wstart;
+    u64 cap_h_wstart;
+    // The substep loop as a graph, instantiated once and updated.
+    hipGraphExec_t exec;
+    // Timing.
+    f64 solve_ms, passes_ms, collide_ms;
+    int time_kernels;
+    hipEvent_t *kev;
+    u32 kev_pairs;
+    f64 kernel_ms;
+    u64 kernel_launches;
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// The wave's entry of a table: the wave index is read from the first lane,
+// so everything derived from it is uniform.
+#define WAVE_ITEM(tab, nwaves)                                                                     \
+    const u32 wv_ = (u32)__builtin_amdgcn_readfirstlane(                                           \
+        (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));                                      \
+    if (wv_ >= (nwaves)) {                                                                         \
+        return;                                                                                    \
+    }                                                                                              \
+    const NphWave e = (tab)[wv_];                                                                  \
+    const u32 i = e.base + (threadIdx.x & (NPH_WAVE - 1U));                                        \
+    if (i >= e.end) {                                                                              \
+        return;                                                                                    \
+    }
+
+typedef const NphWave *__restrict__ Tab;
+typedef const NphStepData *__restrict__ Worlds;
+
+__global__ void __launch_bounds__(256) k_b_init(Tab tab, u32 nw,
+                                                const NphCollideWorld *__restrict__ cw,
+                                                const NphContactInit *__restrict__ inits)
+{
+    WAVE_ITEM(tab, nw)
+    const NphCollideWorld c = cw[e.world];
+    const NphContactInit in = inits[i];
+    init_contact_record(c.contacts, c.gate, &in);
+}
+
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide(Tab tab, u32 nw,
+                                                  const NphCollideWorld *__restrict__ cw,
+                                                  const u32 *__restrict__ list, u32 *codes,
+                                                  f32 step_dt)
+{
+    WAVE_ITEM(tab, nw)
+    const NphCollideWorld c = cw[e.world];
+    const u32 rec = list[i];
+    if (c.contacts[rec].child != NPH_NULL32) {
+        return; // a mesh pair: k_b_collide_mesh
+    }
+    contact_update(c.contacts, rec, c.shapes, c.bodies, step_dt);
+    codes[i] = c.contacts[rec].change;
+}
+
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide_mesh(Tab tab, u32 nw,
+                                                       const NphCollideWorld *__restrict__ cw,
+                                                       const u32 *__restrict__ list, u32 *codes,
+                                                       f32 step_dt)
+{
+    WAVE_ITEM(tab, nw)
+    const NphCollideWorld c = cw[e.world];
+    const u32 rec = list[i];
+    if (c.contacts[rec].child == NPH_NULL32) {
+        return;
+    }
+    contact_update_mesh(c.contacts, rec, c.shapes, c.bodies, step_dt);
+    codes[i] = c.contacts[rec].change;
+}
+
+__global__ void __launch_bounds__(256) k_b_map_fill(Tab tab, u32 nw, Worlds ws)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    d.solver_of_body[i] = d.body_count;
+}
+
+__global__ void __launch_bounds__(256) k_b_map_set(Tab tab, u32 nw, Worlds ws)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    if (i < d.body_count) {
+        d.solver_of_body[d.awake_bodies[i]] = i;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_b_prepare_bodies(Tab tab, u32 nw, Worlds ws,
+                                                          NphStepParams sp)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    if (i < d.body_count) {
+        const u32 slot = d.awake_bodies[i];
+        prepare_body(&sp, &d.bodies[slot], slot, &d.sbodies[i], &d.params[i]);
+    } else {
+        solver_body_dummy(&d.sbodies[i]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_b_joint_prepare(Tab tab, u32 nw, Worlds ws,
+                                                         NphStepParams sp)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    const u32 slot = d.step_joints[i];
+    const NphJointConf *cf = &d.joint_conf[slot];
+    joint_prepare(cf, &d.joint_state[slot], &d.bodies[cf->body_a], &d.bodies[cf->body_b],
+                  d.solver_of_body[cf->body_a], d.solver_of_body[cf->body_b], d.params, sp.h);
+}
+
+__global__ void __launch_bounds__(256) k_b_group_points(Tab tab, u32 nw, Worlds ws)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    NphGroup *grp = &d.groups[i];
+    u32 p = 0U;
+    for (u32 l = 0U; l < grp->lane_count; ++l) {
+        p = nph_maxu(p, d.contacts[d.lane_contact[i * NPH_LANES + l]].point_count);
+    }
+    grp->point_count = p;
+}
+
+// Exclusive scan of one value a thread over a block of NPH_SCAN threads.
+__device__ static u32 scan_block(u32 v, u32 *total)
+{
+    __shared__ u32 s[NPH_SCAN];
+    const u32 t = threadIdx.x;
+    s[t] = v;
+    __syncthreads();
+    for (u32 off = 1U; off < NPH_SCAN; off <<= 1U) {
+        const u32 x = t >= off ? s[t - off] : 0U;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    const u32 incl = s[t];
+    *total = s[NPH_SCAN - 1U];
+    __syncthreads();
+    return incl - v;
+}
+
+// Each world's groups' first point rows, a block a world.
+__global__ void __launch_bounds__(NPH_SCAN) k_b_group_scan(Worlds ws, const u32 *wgroups)
+{
+    const NphStepData d = ws[blockIdx.x];
+    const u32 n = wgroups[blockIdx.x];
+    u32 base = 0U;
+    for (u32 start = 0U; start < n; start += NPH_SCAN) {
+        const u32 g = start + threadIdx.x;
+        const u32 p = g < n ? d.groups[g].point_count : 0U;
+        u32 total = 0U;
+        const u32 pos = scan_block(p, &total);
+        if (g < n) {
+            d.groups[g].first_point = base + pos;
+        }
+        base += total;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_b_lane_prepare(Tab tab, u32 nw, Worlds ws,
+                                                        NphStepParams sp)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    lane_prepare(&d, &sp, i / NPH_LANES, i % NPH_LANES);
+}
+
+__global__ void __launch_bounds__(256) k_b_group_impact(Tab tab, u32 nw, Worlds ws)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    group_finish_prepare(&d, i);
+}
+
+__global__ void __launch_bounds__(256) k_b_integrate_velocities(Tab tab, u32 nw, Worlds ws,
+                                                                NphStepParams sp)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    if (i < d.body_count) {
+        integrate_velocity(&sp, &d.params[i], &d.sbodies[i]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_b_integrate_positions(Tab tab, u32 nw, Worlds ws,
+                                                               NphStepParams sp)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    if (i < d.body_count) {
+        integrate_position(&sp, &d.params[i], &d.sbodies[i]);
+    }
+}
+
+// Colour c of every world: a world's threads are laid out as in its own
+// launch (joints rounded up to whole waves, then lanes); restitution runs
+// no joint rows.
+__global__ void __launch_bounds__(256) k_b_color_pass(Tab tab, u32 nw, Worlds ws,
+                                                      const NphRange *__restrict__ ranges,
+                                                      u32 color, NphStepParams sp, u32 pass)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    const NphRange r = ranges[e.world * NPH_SLOTS + color];
+    if (pass == PASS_RESTITUTION && i < ((r.jcount + 63U) & ~63U)) {
+        return;
+    }
+    color_pass_thread(&d, &sp, pass, r.first, r.count, r.jfirst, r.jcount, i);
+}
+
+// The overflow of every world that has one, a thread a world.
+__global__ void __launch_bounds__(64) k_b_overflow(const u32 *__restrict__ which, u32 n, Worlds ws,
+                                                   const NphRange *__restrict__ ranges,
+                                                   NphStepParams sp, u32 pass)
+{
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) {
+        return;
+    }
+    const u32 w = which[k];
+    const NphStepData d = ws[w];
+    const NphRange r = ranges[w * NPH_SLOTS + NPH_GRAPH_OVERFLOW];
+    overflow_pass(&d, &sp, pass, r.first, r.count, r.jfirst,
+                  pass == PASS_RESTITUTION ? 0U : r.jcount);
+}
+
+__global__ void __launch_bounds__(256) k_b_restitution_masks(Tab tab, u32 nw, Worlds ws,
+                                                             f32 threshold)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    group_restitution_masks(&d, threshold, i);
+}
+
+__global__ void __launch_bounds__(256) k_b_store(Tab tab, u32 nw, Worlds ws)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    lane_store(&d, i / NPH_LANES, i % NPH_LANES);
+}
+
+__global__ void __launch_bounds__(256) k_b_finalize(Tab tab, u32 nw, Worlds ws, NphStepParams sp)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    if (i < d.body_count) {
+        finalize_body(&sp, &d.params[i], &d.sbodies[i], &d.bodies[d.awake_bodies[i]], d.shapes);
+    }
+}
+
+// Grown shapes: a count per awake body of the batch (world k's bodies at
+// aw_off[k]), an exclusive scan over all of them, then each body writes its
+// shapes at its place: every world's grown records in its awake order, the
+// worlds one after another.
+__global__ void __launch_bounds__(256) k_b_grown_count(Tab tab, u32 nw, Worlds ws,
+                                                       const u32 *__restrict__ aw_off, u32 *gcount)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    if (i < d.body_count) {
+        gcount[aw_off[e.world] + i] = grown_of_body(&d, i);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_b_grown_scatter(Tab tab, u32 nw, Worlds ws,
+                                                         const u32 *__restrict__ aw_off,
+                                                         const u32 *__restrict__ gcount,
+                                                         const u32 *__restrict__ gpos,
+                                                         NphGrown *out)
+{
+    WAVE_ITEM(tab, nw)
+    const NphStepData d = ws[e.world];
+    if (i < d.body_count) {
+        const u32 a = aw_off[e.world] + i;
+        if (gcount[a] != 0U) {
+            grown_write(&d, i, &out[gpos[a]]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NPH_SCAN) k_scan_blocks(const u32 *in, u32 *out, u32 n,
+                                                          u32 *bsum)
+{
+    const u32 i = blockIdx.x * NPH_SCAN + threadIdx.x;
+    u32 total = 0U;
+    const u32 pos = scan_block(i < n ? in[i] : 0U, &total);
+    if (i < n) {
+        out[i] = pos;
+    }
+    if (threadIdx.x == 0U) {
+        bsum[blockIdx.x] = total;
+    }
+}
+
+// The block sums scanned in place (one block, in chunks); out[n] = total.
+__global__ void __launch_bounds__(NPH_SCAN) k_scan_sums(u32 *bsum, u32 nb, u32 *out, u32 n)
+{
+    u32 base = 0U;
+    for (u32 start = 0U; start < nb; start += NPH_SCAN) {
+        const u32 b = start + threadIdx.x;
+        const u32 v = b < nb ? bsum[b] : 0U;
+        u32 total = 0U;
+        const u32 pos = scan_block(v, &total);
+        if (b < nb) {
+            bsum[b] = base + pos;
+        }
+        base += total;
+    }
+    if (threadIdx.x == 0U) {
+        out[n] = base;
+    }
+}
+
+__global__ void __launch_bounds__(NPH_SCAN) k_scan_add(u32 *out, u32 n, const u32 *bsum)
+{
+    const u32 i = blockIdx.x * NPH_SCAN + threadIdx.x;
+    if (i < n) {
+        out[i] += bsum[blockIdx.x];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_b_world_starts(const u32 *aw_off, const u32 *gpos, u32 n,
+                                                        u32 *wstart)
+{
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k <= n) {
+        wstart[k] = gpos[aw_off[k]];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static NphResult stage_reserve(Stage *st, u64 bytes, hipStream_t s)
+{
+    if (bytes <= st->cap) {
+        return NPH_RESULT_OK;
+    }
+    u64 c = st->cap < 65536U ? 65536U : st->cap;
+    while (c < bytes) {
+        c *= 2U;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (st->host != NULL) {
+        HIP_TRY(hipHostFree(st->host));
+        st->host = NULL;
+    }
+    if (st->dev != NULL) {
+        HIP_TRY(hipFree(st->dev));
+        st->dev = NULL;
+    }
+    st->cap = 0U;
+    HIP_TRY(hipHostMalloc((void **)&st->host, c, hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void **)&st->dev, c));
+    st->cap = c;
+    return NPH_RESULT_OK;
+}
+
+// Byte offset of the next array of `bytes` in a stage being laid out.
+static u64 stage_take(u64 *at, u64 bytes)
+{
+    const u64 o = (*at + 255U) & ~(u64)255U;
+    *at = o + bytes;
+    return o;
+}
+
+static NphResult pinned_reserve(hipStream_t s, void **p, u64 *cap, u64 bytes)
+{
+    if (bytes <= *cap) {
+        return NPH_RESULT_OK;
+    }
+    u64 c = *cap < 4096U ? 4096U : *cap;
+    while (c < bytes) {
+        c *= 2U;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (*p != NULL) {
+        HIP_TRY(hipHostFree(*p));
+        *p = NULL;
+    }
+    *cap = 0U;
+    HIP_TRY(hipHostMalloc(p, c, hipHostMallocDefault));
+    *cap = c;
+    return NPH_RESULT_OK;
+}
+
+// Device array growth (contents dropped: per-step data).
+template <class T> static NphResult dev_reserve(T **p, u32 *cap, u64 need, hipStream_t s)
+{
+    if (need <= *cap) {
+        return NPH_RESULT_OK;
+    }
+    if (need > 0xFFFFFFFFULL / 2U) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    const u32 c = devi_grow_cap(*cap, (u32)need);
+    NphResult r = devi_grow_buffer((void **)p, 0, (u64)c * sizeof(T), s, 0);
+    if (r == NPH_RESULT_OK) {
+        *cap = c;
+    }
+    return r;
+}
+
+// Waves covering [0, count) of world w, appended to tab.
+static u32 add_waves(NphWave *tab, u32 at, u32 w, u32 count)
+{
+    for (u32 base = 0U; base < count; base += NPH_WAVE) {
+        tab[at].world = w;
+        tab[at].base = base;
+        tab[at].end = count;
+        tab[at].pad = 0U;
+        at += 1U;
+    }
+    return at;
+}
+
+static u32 waves_of(u32 count) { return (count + NPH_WAVE - 1U) / NPH_WAVE; }
+
+NphResult nph_devb_create(NphDeviceBatch **out)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+        fprintf(stderr, "nepheline_amd: no HIP device: the world step runs only on an MI355X\n");
+        return NPH_RESULT_UNSUPPORTED;
+    }
+    NphDeviceBatch *b = (NphDeviceBatch *)calloc(1, sizeof *b);
+    if (b == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        free(b);
+        return NPH_RESULT_DEVICE_LOST;
+    }
+    for (u32 i = 0U; i < 8U; ++i) {
+        if (hipEventCreate(&b->ev[i]) != hipSuccess) {
+            free(b);
+            return NPH_RESULT_DEVICE_LOST;
+        }
+    }
+    b->tune.pass_block = 256U;
+    b->tune.body_block = 256U;
+    b->tune.use_graph = 0U;
+    *out = b;
+    return NPH_RESULT_OK;
+}
+
+void nph_devb_destroy(NphDeviceBatch *b)
+{
+    if (b == NULL) {
+        return;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    if (b->exec != NULL) {
+        (void)hipGraphExecDestroy(b->exec);
+    }
+    Stage *st[3] = {&b->half[0].stage, &b->half[1].stage, &b->solve};
+    for (u32 i = 0U; i < 3U; ++i) {
+        if (st[i]->host != NULL) {
+            (void)hipHostFree(st[i]->host);
+        }
+        if (st[i]->dev != NULL) {
+            (void)hipFree(st[i]->dev);
+        }
+    }
+    void *bufs[] = {b->half[0].codes, b->half[1].codes, b->sbodies, b->params, b->lane_body_a,
+                    b->lane_body_b, b->lane_flags, b->lanes, b->points, b->gcount, b->gpos,
+                    b->bsum, b->grown, b->wstart};
+    for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
+        if (bufs[i] != NULL) {
+            (void)hipFree(bufs[i]);
+        }
+    }
+    void *pinned[] = {b->h_codes, b->h_grown, b->h_wstart};
+    for (u32 i = 0U; i < 3U; ++i) {
+        if (pinned[i] != NULL) {
+            (void)hipHostFree(pinned[i]);
+        }
+    }
+    for (u32 i = 0U; i < 8U; ++i) {
+        (void)hipEventDestroy(b->ev[i]);
+    }
+    for (u32 i = 0U; i < 2U * b->kev_pairs; ++i) {
+        (void)hipEventDestroy(b->kev[i]);
+    }
+    free(b->kev);
+    (void)hipStreamDestroy(b->stream);
+    free(b);
+}
+
+NphResult nph_devb_world(NphDeviceBatch *b, NphDevice **out) { return devi_create(out, b->stream); }
+
+void nph_devb_set_tuning(NphDeviceBatch *b, const NphBatchTuning *t)
+{
+    b->tune = *t;
+    if (b->tune.pass_block != 64U && b->tune.pass_block != 128U && b->tune.pass_block != 256U) {
+        b->tune.pass_block = 256U;
+    }
+    if (b->tune.body_block != 64U && b->tune.body_block != 128U && b->tune.body_block != 256U) {
+        b->tune.body_block = 256U;
+    }
+}
+
+NphBatchTuning nph_devb_tuning(const NphDeviceBatch *b) { return b->tune; }
+
+void nph_devb_times(const NphDeviceBatch *b, f64 out[3])
+{
+    out[0] = b->solve_ms;
+    out[1] = b->passes_ms;
+    out[2] = b->collide_ms;
+}
+
+void nph_devb_time_kernels(NphDeviceBatch *b, int enable)
+{
+    b->time_kernels = enable;
+    b->kernel_ms = 0.0;
+    b->kernel_launches = 0U;
+}
+
+void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[2])
+{
+    out[0] = b->kernel_ms;
+    out[1] = (f64)b->kernel_launches;
+}
+
+NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n, f32 step_dt,
+                           u32 half, const u32 *codes[2])
+{
+    if (half > 1U) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    Half *h = &b->half[half];
+    hipStream_t s = b->stream;
+    // Every world's staged record writes first, in stream order.
+    for (u32 k = 0U; k < n; ++k) {
+        const NphResult r = devi_flush_staged(parts[k].dev);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    u64 inits = 0U, list = 0U, iw = 0U, lw = 0U;
+    int mesh = 0;
+    for (u32 k = 0U; k < n; ++k) {
+        const NphCollidePart *p = &parts[k];
+        if (p->count < p->first) {
+            return NPH_RESULT_INVALID_ARGUMENT;
+        }
+        inits += p->init_count;
+        list += p->count - p->first;
+        iw += waves_of(p->init_count);
+        lw += waves_of(p->count - p->first);
+        mesh |= p->dev->mesh_shapes > 0U;
+    }
+    if (list > 0xFFFFFFF0ULL || inits > 0xFFFFFFF0ULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    u64 at = 0U;
+    const u64 o_cw = stage_take(&at, (u64)n * sizeof(NphCollideWorld));
+    const u64 o_init = stage_take(&at, inits * sizeof(NphContactInit));
+    const u64 o_list = stage_take(&at, list * 4U);
+    const u64 o_iw = stage_take(&at, iw * sizeof(NphWave));
+    const u64 o_lw = stage_take(&at, lw * sizeof(NphWave));
+    NphResult r = stage_reserve(&h->stage, at, s);
+    if (r == NPH_RESULT_OK) {
+        r = dev_reserve(&h->codes, &h->cap_codes, list > 0U ? list : 1U, s);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    u8 *hs = h->stage.host;
+    NphCollideWorld *cw = (NphCollideWorld *)(hs + o_cw);
+    NphContactInit *hi = (NphContactInit *)(hs + o_init);
+    u32 *hl = (u32 *)(hs + o_list);
+    NphWave *tiw = (NphWave *)(hs + o_iw);
+    NphWave *tlw = (NphWave *)(hs + o_lw);
+    u64 ai = 0U, al = 0U;
+    u32 ti = 0U, tl = 0U;
+    for (u32 k = 0U; k < n; ++k) {
+        const NphCollidePart *p = &parts[k];
+        cw[k].contacts = p->dev->contacts;
+        cw[k].shapes = p->dev->shapes;
+        cw[k].bodies = p->dev->bodies;
+        cw[k].gate = p->dev->gate;
+        if (p->init_count > 0U) {
+            memcpy(hi + ai, p->inits, (size_t)p->init_count * sizeof *hi);
+        }
+        const u32 m = p->count - p->first;
+        if (m > 0U) {
+            memcpy(hl + al, p->list + p->first, (size_t)m * 4U);
+        }
+        // Table entries carry batch-wide indices: base and end offset by the
+        // world's start in the concatenated arrays.
+        const u32 t0 = ti;
+        ti = add_waves(tiw, ti, k, p->init_count);
+        for (u32 j = t0; j < ti; ++j) {
+            tiw[j].base += (u32)ai;
+            tiw[j].end += (u32)ai;
+        }
+        const u32 t1 = tl;
+        tl = add_waves(tlw, tl, k, m);
+        for (u32 j = t1; j < tl; ++j) {
+            tlw[j].base += (u32)al;
+            tlw[j].end += (u32)al;
+        }
+        ai += p->init_count;
+        al += m;
+    }
+    h->list_count = (u32)list;
+    u8 *ds = h->stage.dev;
+    if (at > 0U) {
+        HIP_TRY(hipMemcpyAsync(ds, hs, at, hipMemcpyHostToDevice, s));
+    }
+    HIP_TRY(hipEventRecord(b->ev[4U + 2U * half], s));
+    if (iw > 0U) {
+        hipLaunchKernelGGL(k_b_init, dim3(devi_blocks((u32)iw * NPH_WAVE, 256U)), dim3(256), 0, s,
+                           (const NphWave *)(ds + o_iw), (u32)iw,
+                           (const NphCollideWorld *)(ds + o_cw),
+                           (const NphContactInit *)(ds + o_init));
+    }
+    if (lw > 0U) {
+        hipLaunchKernelGGL(k_b_collide, dim3((u32)lw), dim3(NPH_COLLIDE_BLOCK), 0, s,
+                           (const NphWave *)(ds + o_lw), (u32)lw,
+                           (const NphCollideWorld *)(ds + o_cw), (const u32 *)(ds + o_list),
+                           h->codes, step_dt);
+        if (mesh) {
+            hipLaunchKernelGGL(k_b_collide_mesh, dim3((u32)lw), dim3(NPH_COLLIDE_BLOCK), 0, s,
+                               (const NphWave *)(ds + o_lw), (u32)lw,
+                               (const NphCollideWorld *)(ds + o_cw), (const u32 *)(ds + o_list),
+                               h->codes, step_dt);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[5U + 2U * half], s));
+    if (half == 0U) {
+        return NPH_RESULT_OK;
+    }
+    // Both halves' codes back.
+    const u64 l0 = b->half[0].list_count;
+    const u64 l1 = h->list_count;
+    r = pinned_reserve(s, (void **)&b->h_codes, &b->cap_h_codes, (l0 + l1 + 1U) * 4U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    if (l0 > 0U) {
+        HIP_TRY(hipMemcpyAsync(b->h_codes, b->half[0].codes, l0 * 4U, hipMemcpyDeviceToHost, s));
+    }
+    if (l1 > 0U) {
+        HIP_TRY(hipMemcpyAsync(b->h_codes + l0, h->codes, l1 * 4U, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.0F;
+    b->collide_ms = 0.0;
+    if (hipEventElapsedTime(&ms, b->ev[4], b->ev[5]) == hipSuccess) {
+        b->collide_ms += ms;
+    }
+    if (hipEventElapsedTime(&ms, b->ev[6], b->ev[7]) == hipSuccess) {
+        b->collide_ms += ms;
+    }
+    codes[0] = b->h_codes;
+    codes[1] = b->h_codes + l0;
+    b->half[0].list_count = 0U;
+    return NPH_RESULT_OK;
+}
+
+// The launches of one constraint pass over the batch: every world's
+// overflow first, then colours 0..23 (solver.c:774-789).
+typedef struct PassPlan {
+    const NphWave *tab[T_COUNT];
+    u32 nw[T_COUNT];
+    const NphStepData *ws;
+    const NphRange *ranges;
+    const u32 *over;
+    u32 n_over;
+} PassPlan;
+
+static NphResult launch_pass(NphDeviceBatch *b, const PassPlan *pl, const NphStepParams *sp,
+                             u32 pass, int timed, u32 *timed_count)
+{
+    hipStream_t s = b->stream;
+    if (pl->n_over > 0U) {
+        hipLaunchKernelGGL(k_b_overflow, dim3(devi_blocks(pl->n_over, 64U)), dim3(64), 0, s,
+                           pl->over, pl->n_over, pl->ws, pl->ranges, *sp, pass);
+    }
+    const u32 pb = b->tune.pass_block;
+    for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
+        const u32 nw = pl->nw[T_COLOR0 + c];
+        if (nw == 0U) {
+            continue;
+        }
+        const int ev = timed && *timed_count < b->kev_pairs;
+        if (ev) {
+            HIP_TRY(hipEventRecord(b->kev[2U * *timed_count], s));
+        }
+        hipLaunchKernelGGL(k_b_color_pass, dim3(devi_blocks(nw * NPH_WAVE, pb)), dim3(pb), 0, s,
+                           pl->tab[T_COLOR0 + c], nw, pl->ws, pl->ranges, c, *sp, pass);
+        if (ev) {
+            HIP_TRY(hipEventRecord(b->kev[2U * *timed_count + 1U], s));
+            *timed_count += 1U;
+        }
+    }
+    return NPH_RESULT_OK;
+}
+
+static NphResult launch_substeps(NphDeviceBatch *b, const PassPlan *pl, const NphStepParams *sp,
+                                 int timed, u32 *timed_count)
+{
+    hipStream_t s = b->stream;
+    const u32 bb = b->tune.body_block;
+    const u32 nb = pl->nw[T_BODIES];
+    NphResult r = NPH_RESULT_OK;
+    for (u32 step = 0U; step < sp->substeps && r == NPH_RESULT_OK; ++step) {
+        hipLaunchKernelGGL(k_b_integrate_velocities, dim3(devi_blocks(nb * NPH_WAVE, bb)),
+                           dim3(bb), 0, s, pl->tab[T_BODIES], nb, pl->ws, *sp);
+        r = launch_pass(b, pl, sp, PASS_WARM, timed, timed_count);
+        if (r == NPH_RESULT_OK) {
+            r = launch_pass(b, pl, sp, PASS_SOLVE, timed, timed_count);
+        }
+        hipLaunchKernelGGL(k_b_integrate_positions, dim3(devi_blocks(nb * NPH_WAVE, bb)),
+                           dim3(bb), 0, s, pl->tab[T_BODIES], nb, pl->ws, *sp);
+        if (r == NPH_RESULT_OK) {
+            r = launch_pass(b, pl, sp, PASS_RELAX, timed, timed_count);
+        }
+    }
+    if (r == NPH_RESULT_OK && pl->nw[T_GROUPS] > 0U) {
+        hipLaunchKernelGGL(k_b_restitution_masks, dim3(devi_blocks(pl->nw[T_GROUPS] * NPH_WAVE, bb)),
+                           dim3(bb), 0, s, pl->tab[T_GROUPS], pl->nw[T_GROUPS], pl->ws,
+                           sp->restitution_threshold);
+        r = launch_pass(b, pl, sp, PASS_RESTITUTION, timed, timed_count);
+    }
+    if (r == NPH_RESULT_OK) {
+        HIP_TRY(hipGetLastError());
+    }
+    return r;
+}
+
+// The substep loop captured as a graph and launched; the executable graph
+// is kept and updated in place while the launch sequence keeps its shape.
+static NphResult launch_substeps_graph(NphDeviceBatch *b, const PassPlan *pl,
+                                       const NphStepParams *sp)
+{
+    hipStream_t s = b->stream;
+    hipGraph_t graph = NULL;
+    u32 none = 0U;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const NphResult r = launch_substeps(b, pl, sp, 0, &none);
+    const hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (r != NPH_RESULT_OK) {
+        if (graph != NULL) {
+            (void)hipGraphDestroy(graph);
+        }
+        return r;
+    }
+    HIP_TRY(ce);
+    int updated = 0;
+    if (b->exec != NULL) {
+        hipGraphNode_t bad = NULL;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(b->exec, graph, &bad, &res) == hipSuccess &&
+            res == hipGraphExecUpdateSuccess) {
+            updated = 1;
+        } else {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(b->exec);
+            b->exec = NULL;
+        }
+    }
+    if (!updated) {
+        const hipError_t ie = hipGraphInstantiate(&b->exec, graph, NULL, NULL, 0);
+        if (ie != hipSuccess) {
+            b->exec = NULL;
+            (void)hipGraphDestroy(graph);
+            HIP_TRY(ie);
+        }
+    }
+    const hipError_t le = hipGraphLaunch(b->exec, s);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(le);
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSolveInput *in,
+                         u32 n, const NphGrown **grown, const u32 **starts)
+{
+    *grown = NULL;
+    *starts = NULL;
+    hipStream_t s = b->stream;
+    for (u32 k = 0U; k < n; ++k) {
+        const NphResult r = devi_flush_staged(devs[k]);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    b->solve_ms = 0.0;
+    b->passes_ms = 0.0;
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    const NphStepParams sp = in[0].params;
+    for (u32 k = 1U; k < n; ++k) {
+        if (memcmp(&in[k].params, &sp, sizeof sp) != 0) {
+            return NPH_RESULT_INVALID_ARGUMENT; // one set of step numbers a batch
+        }
+    }
+    // Sizes and each world's offsets in the batch's arrays.
+    u64 A = 0U, G = 0U, J = 0U, W[T_COUNT];
+    memset(W, 0, sizeof W);
+    u64 shapes = 0U;
+    u32 n_over = 0U;
+    for (u32 k = 0U; k < n; ++k) {
+        const NphSolveInput *x = &in[k];
+        u32 g = 0U, j = 0U;
+        for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+            const u32 per = c == NPH_GRAPH_OVERFLOW ? 1U : NPH_LANES;
+            const u32 gc = (x->color_count[c] + per - 1U) / per;
+            g += gc;
+            j += x->joint_color_count[c];
+            if (c < NPH_GRAPH_COLORS) {
+                const u32 jc = x->joint_color_count[c];
+                W[T_COLOR0 + c] += waves_of(((jc + 63U) & ~63U) + gc * NPH_LANES);
+            }
+        }
+        if (x->color_count[NPH_GRAPH_OVERFLOW] > 0U ||
+            x->joint_color_count[NPH_GRAPH_OVERFLOW] > 0U) {
+            n_over += 1U;
+        }
+        A += x->awake_count;
+        G += g;
+        J += j;
+        W[T_SLOTS] += waves_of(x->body_slots);
+        W[T_BODIES] += waves_of(x->awake_count + 1U);
+        W[T_GROUPS] += waves_of(g);
+        W[T_LANES] += waves_of(g * NPH_LANES);
+        W[T_JOINTS] += waves_of(j);
+        shapes += devs[k]->cap_shapes;
+    }
+    if (A + n > 0x7FFFFFFFULL || G * NPH_LANES * NPH_PHYSICS_MANIFOLD_MAX_POINTS > 0x7FFFFFFFULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    const u64 lanes = G * NPH_LANES;
+    const u64 point_lanes = lanes * NPH_PHYSICS_MANIFOLD_MAX_POINTS; // four rows a group at most
+    NphResult r = dev_reserve(&b->sbodies, &b->cap_sbodies, A + n, s);
+    if (r == NPH_RESULT_OK) {
+        r = dev_reserve(&b->params, &b->cap_params, A > 0U ? A : 1U, s);
+    }
+    if (r == NPH_RESULT_OK && lanes > b->cap_lanes) {
+        const u32 c = devi_grow_cap(b->cap_lanes, (u32)lanes);
+        if ((r = devi_grow_buffer((void **)&b->lane_body_a, 0, (u64)c * 4U, s, 0)) ==
+                NPH_RESULT_OK &&
+            (r = devi_grow_buffer((void **)&b->lane_body_b, 0, (u64)c * 4U, s, 0)) ==
+                NPH_RESULT_OK &&
+            (r = devi_grow_buffer((void **)&b->lane_flags, 0, (u64)c * 4U, s, 0)) ==
+                NPH_RESULT_OK &&
+            (r = devi_grow_buffer((void **)&b->lanes, 0, (u64)c * NPH_L_FIELDS * 4U, s, 0)) ==
+                NPH_RESULT_OK) {
+            b->cap_lanes = c;
+        }
+    }
+    if (r == NPH_RESULT_OK && point_lanes > b->cap_point_lanes) {
+        const u32 c = devi_grow_cap(b->cap_point_lanes, (u32)point_lanes);
+        if ((r = devi_grow_buffer((void **)&b->points, 0, (u64)c * NPH_P_FIELDS * 4U, s, 0)) ==
+            NPH_RESULT_OK) {
+            b->cap_point_lanes = c;
+        }
+    }
+    const u32 nb_scan = (u32)((A + NPH_SCAN - 1U) / NPH_SCAN);
+    if (r == NPH_RESULT_OK && A + 1U > b->cap_gbodies) {
+        const u32 c = devi_grow_cap(b->cap_gbodies, (u32)(A + 1U));
+        if ((r = devi_grow_buffer((void **)&b->gcount, 0, (u64)c * 4U, s, 0)) == NPH_RESULT_OK &&
+            (r = devi_grow_buffer((void **)&b->gpos, 0, (u64)c * 4U, s, 0)) == NPH_RESULT_OK) {
+            b->cap_gbodies = c;
+        }
+    }
+    if (r == NPH_RESULT_OK) {
+        r = dev_reserve(&b->bsum, &b->cap_bsum, nb_scan + 1U, s);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = dev_reserve(&b->grown, &b->cap_grown, shapes + 1U, s);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = dev_reserve(&b->wstart, &b->cap_wstart, n + 1U, s);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    // The stage: descriptors, ranges, lists, group tables, wave tables.
+    u64 total_waves = 0U;
+    for (u32 t = 0U; t < T_COUNT; ++t) {
+        total_waves += W[t];
+    }
+    u64 at = 0U;
+    const u64 o_ws = stage_take(&at, (u64)n * sizeof(NphStepData));
+    const u64 o_rg = stage_take(&at, (u64)n * NPH_SLOTS * sizeof(NphRange));
+    const u64 o_wg = stage_take(&at, (u64)n * 4U);
+    const u64 o_ao = stage_take(&at, (u64)(n + 1U) * 4U);
+    const u64 o_ov = stage_take(&at, (u64)(n_over + 1U) * 4U);
+    const u64 o_aw = stage_take(&at, A * 4U);
+    const u64 o_lc = stage_take(&at, lanes * 4U);
+    const u64 o_gr = stage_take(&at, G * sizeof(NphGroup));
+    const u64 o_sj = stage_take(&at, J * 4U);
+    const u64 o_tab = stage_take(&at, total_waves * sizeof(NphWave));
+    r = stage_reserve(&b->solve, at, s);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    u8 *hs = b->solve.host;
+    u8 *ds = b->solve.dev;
+    NphStepData *ws = (NphStepData *)(hs + o_ws);
+    NphRange *rg = (NphRange *)(hs + o_rg);
+    u32 *wg = (u32 *)(hs + o_wg);
+    u32 *ao = (u32 *)(hs + o_ao);
+    u32 *ov = (u32 *)(hs + o_ov);
+    u32 *aw = (u32 *)(hs + o_aw);
+    u32 *lc = (u32 *)(hs + o_lc);
+    NphGroup *gr = (NphGroup *)(hs + o_gr);
+    u32 *sj = (u32 *)(hs + o_sj);
+    NphWave *tabs[T_COUNT];
+    u32 tfill[T_COUNT];
+    {
+        NphWave *t0 = (NphWave *)(hs + o_tab);
+        for (u32 t = 0U; t < T_COUNT; ++t) {
+            tabs[t] = t0;
+            t0 += W[t];
+            tfill[t] = 0U;
+        }
+    }
+    u64 a_off = 0U, g_off = 0U, j_off = 0U;
+    u32 over = 0U;
+    for (u32 k = 0U; k < n; ++k) {
+        const NphSolveInput *x = &in[k];
+        const NphDevice *d = devs[k];
+        NphStepData *sd = &ws[k];
+        memset(sd, 0, sizeof *sd);
+        sd->bodies = d->bodies;
+        sd->shapes = d->shapes;
+        sd->contacts = d->contacts;
+        sd->solver_of_body = d->solver_of_body;
+        sd->awake_bodies = (const u32 *)(ds + o_aw) + a_off;
+        sd->sbodies = b->sbodies + a_off + k;
+        sd->params = b->params + a_off;
+        sd->groups = (NphGroup *)(ds + o_gr) + g_off;
+        sd->lane_contact = (u32 *)(ds + o_lc) + g_off * NPH_LANES;
+        sd->lane_body_a = b->lane_body_a + g_off * NPH_LANES;
+        sd->lane_body_b = b->lane_body_b + g_off * NPH_LANES;
+        sd->lane_flags = b->lane_flags + g_off * NPH_LANES;
+        sd->lanes = b->lanes + g_off * NPH_LANES;
+        sd->points = b->points + g_off * NPH_LANES * NPH_PHYSICS_MANIFOLD_MAX_POINTS;
+        sd->lane_stride = b->cap_lanes;
+        sd->point_stride = b->cap_point_lanes;
+        sd->body_count = x->awake_count;
+        sd->joint_conf = d->joint_conf;
+        sd->joint_state = d->joint_state;
+        sd->step_joints = (const u32 *)(ds + o_sj) + j_off;
+        ao[k] = (u32)a_off;
+        memcpy(aw + a_off, x->awake_bodies, (size_t)x->awake_count * 4U);
+        // Groups, colour by colour then the overflow, eight contacts a group.
+        u32 g = 0U, at_c = 0U, j = 0U;
+        for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
+            const u32 per = c == NPH_GRAPH_OVERFLOW ? 1U : NPH_LANES;
+            const u32 m = x->color_count[c];
+            NphRange *rr = &rg[k * NPH_SLOTS + c];
+            rr->first = g;
+            rr->jfirst = j;
+            rr->jcount = x->joint_color_count[c];
+            for (u32 i = 0U; i < m; i += per) {
+                const u32 lanes_here = nph_minu(per, m - i);
+                NphGroup *grp = &gr[g_off + g];
+                memset(grp, 0, sizeof *grp);
+                grp->lane_count = lanes_here;
+                u32 *lcg = lc + (g_off + g) * NPH_LANES;
+                for (u32 l = 0U; l < NPH_LANES; ++l) {
+                    lcg[l] = l < lanes_here ? x->graph_contacts[at_c + i + l] : NPH_NULL32;
+                }
+                g += 1U;
+            }
+            rr->count = g - rr->first;
+            at_c += m;
+            j += rr->jcount;
+            if (c < NPH_GRAPH_COLORS) {
+                const u32 threads = ((rr->jcount + 63U) & ~63U) + rr->count * NPH_LANES;
+                tfill[T_COLOR0 + c] = add_waves(tabs[T_COLOR0 + c], tfill[T_COLOR0 + c], k, threads);
+            }
+        }
+        if (rg[k * NPH_SLOTS + NPH_GRAPH_OVERFLOW].count > 0U ||
+            rg[k * NPH_SLOTS + NPH_GRAPH_OVERFLOW].jcount > 0U) {
+            ov[over++] = k;
+        }
+        if (j > 0U) {
+            memcpy(sj + j_off, x->graph_joints, (size_t)j * 4U);
+        }
+        sd->joint_count = j;
+        wg[k] = g;
+        tfill[T_SLOTS] = add_waves(tabs[T_SLOTS], tfill[T_SLOTS], k, x->body_slots);
+        tfill[T_BODIES] = add_waves(tabs[T_BODIES], tfill[T_BODIES], k, x->awake_count + 1U);
+        tfill[T_GROUPS] = add_waves(tabs[T_GROUPS], tfill[T_GROUPS], k, g);
+        tfill[T_LANES] = add_waves(tabs[T_LANES], tfill[T_LANES], k, g * NPH_LANES);
+        tfill[T_JOINTS] = add_waves(tabs[T_JOINTS], tfill[T_JOINTS], k, j);
+        a_off += x->awake_count;
+        g_off += g;
+        j_off += j;
+    }
+    ao[n] = (u32)a_off;
+    HIP_TRY(hipMemcpyAsync(ds, hs, at, hipMemcpyHostToDevice, s));
+
+    PassPlan pl;
+    memset(&pl, 0, sizeof pl);
+    {
+        const NphWave *t0 = (const NphWave *)(ds + o_tab);
+        for (u32 t = 0U; t < T_COUNT; ++t) {
+            pl.tab[t] = t0;
+            pl.nw[t] = (u32)W[t];
+            t0 += W[t];
+        }
+    }
+    pl.ws = (const NphStepData *)(ds + o_ws);
+    pl.ranges = (const NphRange *)(ds + o_rg);
+    pl.over = (const u32 *)(ds + o_ov);
+    pl.n_over = over;
+    const NphStepData *dws = pl.ws;
+    const u32 *dao = (const u32 *)(ds + o_ao);
+    const u32 bb = b->tune.body_block;
+#define GRID(t) dim3(devi_blocks(pl.nw[t] * NPH_WAVE, bb)), dim3(bb), 0, s, pl.tab[t], pl.nw[t]
+
+    HIP_TRY(hipEventRecord(b->ev[0], s));
+    hipLaunchKernelGGL(k_b_map_fill, GRID(T_SLOTS), dws);
+    hipLaunchKernelGGL(k_b_map_set, GRID(T_BODIES), dws);
+    hipLaunchKernelGGL(k_b_prepare_bodies, GRID(T_BODIES), dws, sp);
+    if (J > 0U) {
+        hipLaunchKernelGGL(k_b_joint_prepare, GRID(T_JOINTS), dws, sp);
+    }
+    if (G > 0U) {
+        hipLaunchKernelGGL(k_b_group_points, GRID(T_GROUPS), dws);
+        hipLaunchKernelGGL(k_b_group_scan, dim3(n), dim3(NPH_SCAN), 0, s, dws,
+                           (const u32 *)(ds + o_wg));
+        hipLaunchKernelGGL(k_b_lane_prepare, GRID(T_LANES), dws, sp);
+        hipLaunchKernelGGL(k_b_group_impact, GRID(T_GROUPS), dws);
+    }
+    HIP_TRY(hipGetLastError());
+    u32 timed = 0U;
+    const int time_launches = b->time_kernels && !b->tune.use_graph;
+    if (time_launches) {
+        const u32 need = (3U * sp.substeps + 1U) * NPH_GRAPH_COLORS;
+        while (b->kev_pairs < need) {
+            hipEvent_t *ne = (hipEvent_t *)realloc(b->kev, (size_t)(2U * need) * sizeof *ne);
+            if (ne == NULL) {
+                return NPH_RESULT_OUT_OF_MEMORY;
+            }
+            b->kev = ne;
+            for (u32 i = 2U * b->kev_pairs; i < 2U * need; ++i) {
+                HIP_TRY(hipEventCreate(&b->kev[i]));
+            }
+            b->kev_pairs = need;
+        }
+    }
+    HIP_TRY(hipEventRecord(b->ev[2], s));
+    r = b->tune.use_graph ? launch_substeps_graph(b, &pl, &sp)
+                          : launch_substeps(b, &pl, &sp, time_launches, &timed);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    HIP_TRY(hipEventRecord(b->ev[3], s));
+    if (G > 0U) {
+        hipLaunchKernelGGL(k_b_store, GRID(T_LANES), dws);
+    }
+    hipLaunchKernelGGL(k_b_finalize, GRID(T_BODIES), dws, sp);
+    // Grown proxies: counts, one scan over the batch, placement.
+    hipLaunchKernelGGL(k_b_grown_count, GRID(T_BODIES), dws, dao, b->gcount);
+    if (A > 0U) {
+        hipLaunchKernelGGL(k_scan_blocks, dim3(nb_scan), dim3(NPH_SCAN), 0, s, b->gcount, b->gpos,
+                           (u32)A, b->bsum);
+    }
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(NPH_SCAN), 0, s, b->bsum, nb_scan, b->gpos,
+                       (u32)A);
+    if (A > 0U) {
+        hipLaunchKernelGGL(k_scan_add, dim3(nb_scan), dim3(NPH_SCAN), 0, s, b->gpos, (u32)A,
+                           b->bsum);
+    }
+    hipLaunchKernelGGL(k_b_grown_scatter, GRID(T_BODIES), dws, dao, b->gcount, b->gpos, b->grown);
+    hipLaunchKernelGGL(k_b_world_starts, dim3(devi_blocks(n + 1U, 256U)), dim3(256), 0, s, dao,
+                       b->gpos, n, b->wstart);
+    HIP_TRY(hipGetLastError());
+#undef GRID
+    HIP_TRY(hipEventRecord(b->ev[1], s));
+    r = pinned_reserve(s, (void **)&b->h_wstart, &b->cap_h_wstart, (u64)(n + 1U) * 4U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    HIP_TRY(hipMemcpyAsync(b->h_wstart, b->wstart, (u64)(n + 1U) * 4U, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const u32 m = b->h_wstart[n];
+    r = pinned_reserve(s, (void **)&b->h_grown, &b->cap_h_grown, (u64)(m + 1U) * sizeof(NphGrown));
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    if (m > 0U) {
+        HIP_TRY(hipMemcpyAsync(b->h_grown, b->grown, (u64)m * sizeof(NphGrown),
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    for (u32 i = 0U; i < timed; ++i) {
+        float kms = 0.0F;
+        HIP_TRY(hipEventElapsedTime(&kms, b->kev[2U * i], b->kev[2U * i + 1U]));
+        b->kernel_ms += kms;
+        b->kernel_launches += 1U;
+    }
+    float ms = 0.0F;
+    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+    b->solve_ms = ms;
+    ms = 0.0F;
+    (void)hipEventElapsedTime(&ms, b->ev[2], b->ev[3]);
+    b->passes_ms = ms;
+    *grown = b->h_grown;
+    *starts = b->h_wstart;
+    return NPH_RESULT_OK;
+}

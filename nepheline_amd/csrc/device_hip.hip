@@ -13,147 +13,13 @@
 // square root, and denormals kept, so each operation rounds as the
 // reference's SSE code does.
 
-#include <hip/hip_runtime.h>
 
-#include "nph_device.h"
-#include "nph_joint.h"
-#include "nph_learned.h"
 
-#include <stdio.h>
-#include <stdlib.h>
-#include <string.h>
-
-#define NPH_BLOCK 256U
-#define NPH_SCAN_BLOCK 1024U
-// The narrow phase keeps its clip buffers in private memory: one wave a block.
-#define NPH_COLLIDE_BLOCK 64U
-
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        const hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess) {                                                                    \
-            fprintf(stderr, "nepheline_amd: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_), \
-                    __FILE__, __LINE__);                                                           \
-            return e_ == hipErrorOutOfMemory ? NPH_RESULT_OUT_OF_MEMORY : NPH_RESULT_DEVICE_LOST;  \
-        }                                                                                          \
-    } while (0)
-
-// A hull or mesh block and its device copy.
-typedef struct HullEntry {
-    const void *host;
-    const void *device;
-} HullEntry;
-
-// Record writes from the host (body and shape creation, velocity changes)
-// wait here and reach HBM together, in one copy and one scatter, before the
-// next device read or stage.  A slot written twice keeps its last record.
-typedef struct Staged {
-    u32 rec_words;  // record size in 4-byte words
-    u32 *recs;      // count * rec_words
-    u32 *slots;     // count
-    u32 count, cap;
-    u32 *where;     // per slot: staged index + 1, or 0
-    u32 cap_where;
-    u32 *dev;       // device copy: slots then records
-    u64 dev_words;
-} Staged;
-
-struct NphDevice {
-    hipStream_t stream;
-    hipEvent_t ev[4];
-    f64 solve_ms;
-    f64 collide_ms;
-    f64 passes_ms;
-    // Records.
-    NphBodyRec *bodies;
-    NphShapeRec *shapes;
-    NphContactRec *contacts;
-    u32 cap_bodies, cap_shapes, cap_contacts;
-    // Per step.
-    u32 *solver_of_body; // cap_bodies
-    u32 *awake_bodies;   // cap_bodies
-    NphSolverBody *sbodies; // cap_bodies + 1
-    NphSolverParams *params; // cap_bodies
-    u32 *list;  // awake contacts / graph contacts (cap_list)
-    u32 cap_list;
-    u32 *codes; // change codes, per list entry
-    NphContactInit *inits;
-    u32 cap_inits;
-    NphGroup *groups;
-    u32 *lane_contact, *lane_body_a, *lane_body_b, *lane_flags;
-    f32 *lanes;
-    u32 cap_groups;
-    f32 *points;
-    u32 cap_point_rows;
-    NphGrown *grown; // cap_shapes + 1 (count first)
-    u32 *grown_blocks; // per NPH_BLOCK awake bodies: grown shapes (cap_bodies / NPH_BLOCK + 1)
-    // Host staging (pinned).
-    u32 *h_u32;
-    u32 cap_h_u32;
-    u32 *h_list;  // the awake list being collided
-    u64 cap_h_list;
-    u32 *h_codes; // change codes read back
-    u64 cap_h_codes;
-    NphContactInit *h_inits;
-    u64 cap_h_inits;
-    u32 collide_first; // entries [0, collide_first) queued by collide_begin
-    // Per-launch timing of the colour passes (off unless asked for).
-    int time_kernels;
-    hipEvent_t *kev;
-    u32 kev_pairs;
-    f64 kernel_ms;
-    u64 kernel_launches;
-    hipEvent_t cev[4];
-    NphGrown *h_grown;
-    u32 cap_h_grown;
-    HullEntry *hulls;
-    u32 hull_count, hull_cap;
-    Staged staged_bodies;
-    Staged staged_shapes;
-    // Joints by slot: host-written configuration, device-written state.
-    NphJointConf *joint_conf;
-    NphJointState *joint_state;
-    u32 cap_joints;
-    Staged staged_jconf;
-    Staged staged_jstate;
-    u32 *step_joints; // the graph's joints for a step
-    u32 cap_step_joints;
-    // Between nph_dev_solve and nph_dev_solve_finish.
-    NphStepData pending_sd;
-    u32 pending_timed;
-    // Continuous collision: fast bodies (count first) and their packed rows.
-    u32 *fast_list;
-    NphFastBody *fast_pack;
-    u32 cap_fast_pack;
-    u32 *h_fast_count;
-    u64 cap_h_fast_count;
-    NphFastBody *h_fast_pack;
-    u64 cap_h_fast_pack;
-    // The learned warm start (off unless a network is set), residual
-    // measurement and the contact log.
-    int learn_on;
-    int measure;
-    u32 learn_mode;
-    int gate_on;
-    f32 shift[16], scale[16], out_scale[4];
-    f32 *mlp_params;     // NPH_AMD_MLP_PARAMS
-    u16 *lx;             // lanes * 4 network rows of 16 BF16
-    f32 *lraw;           // lanes * 4 rows of 16 raw features
-    f32 *ly;             // lanes * 4 rows of 4 outputs
-    f32 *lres;           // per lane residual
-    u32 cap_learn_lanes;
-    u32 *gate;           // per contact record: 1 = plain warm start for good
-    u32 *wstats;         // NPH_WSTATS words
-    u32 *h_wstats;
-    f32 *log_x, *log_t;  // log rows: 16 features, 4 targets
-    u32 *log_count;
-    u32 log_cap;
-    u32 mesh_shapes;     // mesh shapes made: the mesh pairs' kernel runs
-};
+#include "device_impl.h"
 
 const char *nph_dev_name(void) { return "hip:gfx950"; }
 
-static NphResult grow_buffer(void **p, u64 old_bytes, u64 new_bytes, hipStream_t s, int keep)
+NphResult devi_grow_buffer(void **p, u64 old_bytes, u64 new_bytes, hipStream_t s, int keep)
 {
     void *n = NULL;
     HIP_TRY(hipMalloc(&n, new_bytes));
@@ -168,7 +34,7 @@ static NphResult grow_buffer(void **p, u64 old_bytes, u64 new_bytes, hipStream_t
     return NPH_RESULT_OK;
 }
 
-static u32 grow_cap(u32 cap, u32 need)
+u32 devi_grow_cap(u32 cap, u32 need)
 {
     u32 c = cap < 64U ? 64U : cap;
     while (c < need) {
@@ -177,7 +43,9 @@ static u32 grow_cap(u32 cap, u32 need)
     return c;
 }
 
-NphResult nph_dev_create(NphDevice **out)
+NphResult nph_dev_create(NphDevice **out) { return devi_create(out, NULL); }
+
+NphResult devi_create(NphDevice **out, hipStream_t stream)
 {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
@@ -188,7 +56,10 @@ NphResult nph_dev_create(NphDevice **out)
     if (d == NULL) {
         return NPH_RESULT_OUT_OF_MEMORY;
     }
-    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (stream != NULL) {
+        d->stream = stream;
+        d->shared_stream = 1;
+    } else if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
         free(d);
         return NPH_RESULT_DEVICE_LOST;
     }
@@ -263,7 +134,9 @@ void nph_dev_destroy(NphDevice *d)
         (void)hipEventDestroy(d->kev[i]);
     }
     free(d->kev);
-    (void)hipStreamDestroy(d->stream);
+    if (!d->shared_stream) {
+        (void)hipStreamDestroy(d->stream);
+    }
     free(d);
 }
 
@@ -287,43 +160,43 @@ void nph_dev_kernel_times(const NphDevice *d, f64 out[2])
 NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts)
 {
     if (bodies > d->cap_bodies) {
-        const u32 c = grow_cap(d->cap_bodies, bodies);
+        const u32 c = devi_grow_cap(d->cap_bodies, bodies);
         NphResult r;
-        if ((r = grow_buffer((void **)&d->bodies, (u64)d->cap_bodies * sizeof(NphBodyRec),
+        if ((r = devi_grow_buffer((void **)&d->bodies, (u64)d->cap_bodies * sizeof(NphBodyRec),
                              (u64)c * sizeof(NphBodyRec), d->stream, 1)) != NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->solver_of_body, 0, (u64)c * 4U, d->stream, 0)) !=
+            (r = devi_grow_buffer((void **)&d->solver_of_body, 0, (u64)c * 4U, d->stream, 0)) !=
                 NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->awake_bodies, 0, (u64)c * 4U, d->stream, 0)) !=
+            (r = devi_grow_buffer((void **)&d->awake_bodies, 0, (u64)c * 4U, d->stream, 0)) !=
                 NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->sbodies, 0, (u64)(c + 1U) * sizeof(NphSolverBody),
+            (r = devi_grow_buffer((void **)&d->sbodies, 0, (u64)(c + 1U) * sizeof(NphSolverBody),
                              d->stream, 0)) != NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->params, 0, (u64)c * sizeof(NphSolverParams), d->stream,
+            (r = devi_grow_buffer((void **)&d->params, 0, (u64)c * sizeof(NphSolverParams), d->stream,
                              0)) != NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->grown_blocks, 0, (u64)(c / NPH_BLOCK + 1U) * 4U,
+            (r = devi_grow_buffer((void **)&d->grown_blocks, 0, (u64)(c / NPH_BLOCK + 1U) * 4U,
                              d->stream, 0)) != NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->fast_list, 0, (u64)(c + 1U) * 4U, d->stream, 0)) !=
+            (r = devi_grow_buffer((void **)&d->fast_list, 0, (u64)(c + 1U) * 4U, d->stream, 0)) !=
                 NPH_RESULT_OK) {
             return r;
         }
         d->cap_bodies = c;
     }
     if (shapes > d->cap_shapes) {
-        const u32 c = grow_cap(d->cap_shapes, shapes);
+        const u32 c = devi_grow_cap(d->cap_shapes, shapes);
         NphResult r;
-        if ((r = grow_buffer((void **)&d->shapes, (u64)d->cap_shapes * sizeof(NphShapeRec),
+        if ((r = devi_grow_buffer((void **)&d->shapes, (u64)d->cap_shapes * sizeof(NphShapeRec),
                              (u64)c * sizeof(NphShapeRec), d->stream, 1)) != NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->grown, 0, (u64)(c + 1U) * sizeof(NphGrown), d->stream,
+            (r = devi_grow_buffer((void **)&d->grown, 0, (u64)(c + 1U) * sizeof(NphGrown), d->stream,
                              0)) != NPH_RESULT_OK) {
             return r;
         }
         d->cap_shapes = c;
     }
     if (contacts > d->cap_contacts) {
-        const u32 c = grow_cap(d->cap_contacts, contacts);
-        NphResult r = grow_buffer((void **)&d->contacts, (u64)d->cap_contacts * sizeof(NphContactRec),
+        const u32 c = devi_grow_cap(d->cap_contacts, contacts);
+        NphResult r = devi_grow_buffer((void **)&d->contacts, (u64)d->cap_contacts * sizeof(NphContactRec),
                                   (u64)c * sizeof(NphContactRec), d->stream, 1);
         if (r == NPH_RESULT_OK) {
-            r = grow_buffer((void **)&d->gate, (u64)d->cap_contacts * 4U, (u64)c * 4U, d->stream, 1);
+            r = devi_grow_buffer((void **)&d->gate, (u64)d->cap_contacts * 4U, (u64)c * 4U, d->stream, 1);
         }
         if (r != NPH_RESULT_OK) {
             return r;
@@ -338,10 +211,10 @@ static NphResult reserve_list(NphDevice *d, u32 n)
     if (n <= d->cap_list) {
         return NPH_RESULT_OK;
     }
-    const u32 c = grow_cap(d->cap_list, n);
+    const u32 c = devi_grow_cap(d->cap_list, n);
     NphResult r;
-    if ((r = grow_buffer((void **)&d->list, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->codes, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK) {
+    if ((r = devi_grow_buffer((void **)&d->list, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK ||
+        (r = devi_grow_buffer((void **)&d->codes, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK) {
         return r;
     }
     d->cap_list = c;
@@ -353,7 +226,7 @@ static NphResult reserve_host_u32(NphDevice *d, u32 n)
     if (n <= d->cap_h_u32) {
         return NPH_RESULT_OK;
     }
-    const u32 c = grow_cap(d->cap_h_u32, n);
+    const u32 c = devi_grow_cap(d->cap_h_u32, n);
     if (d->h_u32 != NULL) {
         HIP_TRY(hipHostFree(d->h_u32));
         d->h_u32 = NULL;
@@ -368,19 +241,19 @@ static NphResult reserve_groups(NphDevice *d, u32 groups)
     if (groups <= d->cap_groups) {
         return NPH_RESULT_OK;
     }
-    const u32 c = grow_cap(d->cap_groups, groups);
+    const u32 c = devi_grow_cap(d->cap_groups, groups);
     const u64 lanes = (u64)c * NPH_LANES;
     const u64 rows = (u64)c * NPH_PHYSICS_MANIFOLD_MAX_POINTS;
     NphResult r;
-    if ((r = grow_buffer((void **)&d->groups, 0, (u64)c * sizeof(NphGroup), d->stream, 0)) !=
+    if ((r = devi_grow_buffer((void **)&d->groups, 0, (u64)c * sizeof(NphGroup), d->stream, 0)) !=
             NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->lane_contact, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->lane_body_a, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->lane_body_b, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->lane_flags, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->lanes, 0, lanes * NPH_L_FIELDS * 4U, d->stream, 0)) !=
+        (r = devi_grow_buffer((void **)&d->lane_contact, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
+        (r = devi_grow_buffer((void **)&d->lane_body_a, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
+        (r = devi_grow_buffer((void **)&d->lane_body_b, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
+        (r = devi_grow_buffer((void **)&d->lane_flags, 0, lanes * 4U, d->stream, 0)) != NPH_RESULT_OK ||
+        (r = devi_grow_buffer((void **)&d->lanes, 0, lanes * NPH_L_FIELDS * 4U, d->stream, 0)) !=
             NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->points, 0, rows * NPH_LANES * NPH_P_FIELDS * 4U, d->stream,
+        (r = devi_grow_buffer((void **)&d->points, 0, rows * NPH_LANES * NPH_P_FIELDS * 4U, d->stream,
                          0)) != NPH_RESULT_OK) {
         return r;
     }
@@ -423,7 +296,7 @@ static_assert(sizeof(NphBodyRec) % 4U == 0U && sizeof(NphShapeRec) % 4U == 0U &&
 static NphResult stage_put(Staged *st, u32 slot, const void *rec)
 {
     if (slot >= st->cap_where) {
-        const u32 c = grow_cap(st->cap_where, slot + 1U);
+        const u32 c = devi_grow_cap(st->cap_where, slot + 1U);
         u32 *n = (u32 *)realloc(st->where, (size_t)c * 4U);
         if (n == NULL) {
             return NPH_RESULT_OUT_OF_MEMORY;
@@ -435,7 +308,7 @@ static NphResult stage_put(Staged *st, u32 slot, const void *rec)
     u32 i = st->where[slot];
     if (i == 0U) {
         if (st->count == st->cap) {
-            const u32 c = grow_cap(st->cap, st->count + 1U);
+            const u32 c = devi_grow_cap(st->cap, st->count + 1U);
             u32 *r = (u32 *)realloc(st->recs, (size_t)c * st->rec_words * 4U);
             if (r == NULL) {
                 return NPH_RESULT_OUT_OF_MEMORY;
@@ -479,7 +352,7 @@ static NphResult stage_flush(NphDevice *d, Staged *st, void *dst)
         while (c < words) {
             c *= 2U;
         }
-        NphResult r = grow_buffer((void **)&st->dev, 0, c * 4U, d->stream, 0);
+        NphResult r = devi_grow_buffer((void **)&st->dev, 0, c * 4U, d->stream, 0);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -504,7 +377,7 @@ static NphResult stage_flush(NphDevice *d, Staged *st, void *dst)
 }
 
 // Every device read and stage starts here.
-static NphResult flush_staged(NphDevice *d)
+NphResult devi_flush_staged(NphDevice *d)
 {
     NphResult r = stage_flush(d, &d->staged_bodies, d->bodies);
     if (r == NPH_RESULT_OK) {
@@ -524,11 +397,11 @@ NphResult nph_dev_reserve_joints(NphDevice *d, u32 joints)
     if (joints <= d->cap_joints) {
         return NPH_RESULT_OK;
     }
-    const u32 c = grow_cap(d->cap_joints, joints);
+    const u32 c = devi_grow_cap(d->cap_joints, joints);
     NphResult r;
-    if ((r = grow_buffer((void **)&d->joint_conf, (u64)d->cap_joints * sizeof(NphJointConf),
+    if ((r = devi_grow_buffer((void **)&d->joint_conf, (u64)d->cap_joints * sizeof(NphJointConf),
                          (u64)c * sizeof(NphJointConf), d->stream, 1)) != NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->joint_state, (u64)d->cap_joints * sizeof(NphJointState),
+        (r = devi_grow_buffer((void **)&d->joint_state, (u64)d->cap_joints * sizeof(NphJointState),
                          (u64)c * sizeof(NphJointState), d->stream, 1)) != NPH_RESULT_OK) {
         return r;
     }
@@ -558,7 +431,7 @@ NphResult nph_dev_read_joint(NphDevice *d, u32 slot, NphJointState *out)
     if (slot >= d->cap_joints) {
         return NPH_RESULT_INVALID_ARGUMENT;
     }
-    NphResult r = flush_staged(d);
+    NphResult r = devi_flush_staged(d);
     if (r != NPH_RESULT_OK) {
         return r;
     }
@@ -587,7 +460,7 @@ NphResult nph_dev_write_shape(NphDevice *d, u32 slot, const NphShapeRec *s)
 NphResult nph_dev_read_bodies(NphDevice *d, NphBodyRec *out, u32 count)
 {
     {
-        const NphResult r = flush_staged(d);
+        const NphResult r = devi_flush_staged(d);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -603,7 +476,7 @@ NphResult nph_dev_read_bodies(NphDevice *d, NphBodyRec *out, u32 count)
 NphResult nph_dev_read_shapes(NphDevice *d, NphShapeRec *out, u32 count)
 {
     {
-        const NphResult r = flush_staged(d);
+        const NphResult r = devi_flush_staged(d);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -619,7 +492,7 @@ NphResult nph_dev_read_shapes(NphDevice *d, NphShapeRec *out, u32 count)
 NphResult nph_dev_read_contacts(NphDevice *d, NphContactRec *out, u32 count)
 {
     {
-        const NphResult r = flush_staged(d);
+        const NphResult r = devi_flush_staged(d);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -643,15 +516,7 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_init_contacts(NphContactRec *cont
         return;
     }
     const NphContactInit in = list[i];
-    NphContactRec *c = &contacts[in.record];
-    gate[in.record] = 0U;
-    memset(c, 0, sizeof *c);
-    c->shape_a = in.shape_a;
-    c->shape_b = in.shape_b;
-    c->body_a = in.body_a;
-    c->body_b = in.body_b;
-    c->normal = v3(0.0F, 1.0F, 0.0F);
-    c->child = in.child;
+    init_contact_record(contacts, gate, &in);
 }
 
 __global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide(NphContactRec *contacts, const NphShapeRec *shapes,
@@ -788,55 +653,6 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_integrate_positions(NphStepData d
     }
 }
 
-enum { PASS_WARM = 0, PASS_SOLVE = 1, PASS_RELAX = 2, PASS_RESTITUTION = 3 };
-
-__device__ static void run_lane(const NphStepData *d, const NphStepParams *sp, u32 pass, u32 g,
-                                u32 l)
-{
-    switch (pass) {
-    case PASS_WARM:
-        lane_warm_start(d, g, l);
-        break;
-    case PASS_SOLVE:
-        lane_solve(d, g, l, sp->inv_h, sp->push_max, 0.0F, 1);
-        break;
-    case PASS_RELAX:
-        lane_solve(d, g, l, sp->inv_h, sp->push_max, 0.0F, 0);
-        break;
-    default:
-        lane_restitution(d, g, l);
-        break;
-    }
-}
-
-// A joint's pass: warm start, or the soft (solve) or rigid (relax) rows.
-// The joint's state is held in registers for the pass.
-__device__ static void run_joint(const NphStepData *d, const NphStepParams *sp, u32 pass, u32 slot)
-{
-    const NphJointConf cf = d->joint_conf[slot];
-    NphJointState st = d->joint_state[slot];
-    if (pass == PASS_WARM) {
-        joint_warm_start(d->sbodies, &cf, &st);
-    } else {
-        NphJointStep step;
-        step.h = sp->h;
-        step.inv_h = sp->inv_h;
-        step.soft.bias = sp->joint_bias;
-        step.soft.mass = sp->joint_mass;
-        step.soft.impulse = sp->joint_impulse;
-        joint_solve(d->sbodies, &step, &cf, &st, pass == PASS_SOLVE);
-    }
-    NphJointState *out = &d->joint_state[slot];
-    out->linear_impulse = st.linear_impulse;
-    out->angular_impulse = st.angular_impulse;
-    out->friction_impulse = st.friction_impulse;
-    out->lower_impulse = st.lower_impulse;
-    out->upper_impulse = st.upper_impulse;
-    out->swing_impulse = st.swing_impulse;
-    out->motor_impulse = st.motor_impulse;
-    out->spring_impulse = st.spring_impulse;
-}
-
 // The step's joints prepared (solver.c:1036, joint_solver.c:726-824).
 __global__ void __launch_bounds__(NPH_BLOCK) k_joint_prepare(NphStepData d, NphStepParams sp)
 {
@@ -855,18 +671,8 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_joint_prepare(NphStepData d, NphS
 // so their order inside the launch is free.
 __global__ void __launch_bounds__(NPH_BLOCK) k_color_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count, u32 jfirst, u32 jcount)
 {
-    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-    const u32 jpad = (jcount + 63U) & ~63U;
-    if (t < jpad) {
-        if (t < jcount) {
-            run_joint(&d, &sp, pass, d.step_joints[jfirst + t]);
-        }
-        return;
-    }
-    const u32 u = t - jpad;
-    if (u < count * NPH_LANES) {
-        run_lane(&d, &sp, pass, first + u / NPH_LANES, u % NPH_LANES);
-    }
+    color_pass_thread(&d, &sp, pass, first, count, jfirst, jcount,
+                      blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // The overflow: its joints, then its groups (one contact each), in order, on
@@ -874,12 +680,7 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_color_pass(NphStepData d, NphStep
 __global__ void __launch_bounds__(64) k_overflow_pass(NphStepData d, NphStepParams sp, u32 pass, u32 first, u32 count, u32 jfirst, u32 jcount)
 {
     if (blockIdx.x == 0U && threadIdx.x == 0U) {
-        for (u32 j = jfirst; j < jfirst + jcount; ++j) {
-            run_joint(&d, &sp, pass, d.step_joints[j]);
-        }
-        for (u32 g = first; g < first + count; ++g) {
-            run_lane(&d, &sp, pass, g, 0U);
-        }
+        overflow_pass(&d, &sp, pass, first, count, jfirst, jcount);
     }
 }
 
@@ -911,18 +712,6 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_finalize(NphStepData d, NphStepPa
 // list's order; their flags cleared.  Two launches: a count per block of
 // NPH_BLOCK awake bodies, then each block places its bodies' shapes after the
 // counts of the blocks before it.  out[0].shape = count.
-__device__ static u32 grown_of_body(const NphStepData *d, u32 i)
-{
-    u32 mine = 0U;
-    const NphBodyRec *b = &d->bodies[d->awake_bodies[i]];
-    if (b->enlarged != 0U) {
-        for (u32 s = b->shape_head; s != NPH_NULL32; s = d->shapes[s].next) {
-            mine += (d->shapes[s].flags & NPH_SHAPE_ENLARGED) != 0U ? 1U : 0U;
-        }
-    }
-    return mine;
-}
-
 __device__ static u32 block_sum(u32 v)
 {
     __shared__ u32 part[NPH_BLOCK / 64U];
@@ -973,20 +762,7 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_grown_scatter(NphStepData d,
     }
     const u32 pos = base + scan[threadIdx.x] - mine;
     if (mine != 0U) {
-        NphBodyRec *b = &d.bodies[d.awake_bodies[i]];
-        u32 k = 0U;
-        for (u32 s = b->shape_head; s != NPH_NULL32; s = d.shapes[s].next) {
-            NphShapeRec *sh = &d.shapes[s];
-            if ((sh->flags & NPH_SHAPE_ENLARGED) != 0U) {
-                NphGrown *g = &out[1U + pos + k];
-                g->shape = s;
-                g->pad = 0U;
-                g->fat_aabb = sh->fat_aabb;
-                sh->flags &= ~NPH_SHAPE_ENLARGED;
-                k += 1U;
-            }
-        }
-        b->enlarged = 0U;
+        grown_write(&d, i, &out[1U + pos]);
     }
     if (blockIdx.x == gridDim.x - 1U && threadIdx.x == NPH_BLOCK - 1U) {
         out[0].shape = base + scan[NPH_BLOCK - 1U];
@@ -1339,7 +1115,7 @@ static NphResult learned_after(NphDevice *d, const NphStepData *sd, u32 groups, 
 
 // A pinned host buffer of at least `bytes`; the stream is drained before an
 // old one is freed (a queued copy may still read it).
-static NphResult reserve_pinned(NphDevice *d, void **p, u64 *cap, u64 bytes)
+NphResult devi_reserve_pinned(NphDevice *d, void **p, u64 *cap, u64 bytes)
 {
     if (bytes <= *cap) {
         return NPH_RESULT_OK;
@@ -1364,7 +1140,7 @@ static NphResult reserve_pinned(NphDevice *d, void **p, u64 *cap, u64 bytes)
 NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 count)
 {
     {
-        const NphResult fr = flush_staged(d);
+        const NphResult fr = devi_flush_staged(d);
         if (fr != NPH_RESULT_OK) {
             return fr;
         }
@@ -1373,8 +1149,8 @@ NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 co
         return NPH_RESULT_OK;
     }
     if (count > d->cap_inits) {
-        const u32 c = grow_cap(d->cap_inits, count);
-        NphResult r = grow_buffer((void **)&d->inits, 0, (u64)c * sizeof(NphContactInit), d->stream, 0);
+        const u32 c = devi_grow_cap(d->cap_inits, count);
+        NphResult r = devi_grow_buffer((void **)&d->inits, 0, (u64)c * sizeof(NphContactInit), d->stream, 0);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -1382,7 +1158,7 @@ NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 co
     }
     // The previous step's copy out of the staging buffer is done: every step
     // ends with the stream drained.
-    NphResult r = reserve_pinned(d, (void **)&d->h_inits, &d->cap_h_inits,
+    NphResult r = devi_reserve_pinned(d, (void **)&d->h_inits, &d->cap_h_inits,
                                  (u64)count * sizeof *list);
     if (r != NPH_RESULT_OK) {
         return r;
@@ -1403,11 +1179,11 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
 {
     if (count > d->cap_list) {
         // Keep what an earlier range wrote (stream-ordered copies).
-        const u32 c = grow_cap(d->cap_list, count);
+        const u32 c = devi_grow_cap(d->cap_list, count);
         NphResult r;
-        if ((r = grow_buffer((void **)&d->list, (u64)d->cap_list * 4U, (u64)c * 4U, d->stream,
+        if ((r = devi_grow_buffer((void **)&d->list, (u64)d->cap_list * 4U, (u64)c * 4U, d->stream,
                              1)) != NPH_RESULT_OK ||
-            (r = grow_buffer((void **)&d->codes, (u64)d->cap_list * 4U, (u64)c * 4U, d->stream,
+            (r = devi_grow_buffer((void **)&d->codes, (u64)d->cap_list * 4U, (u64)c * 4U, d->stream,
                              1)) != NPH_RESULT_OK) {
             return r;
         }
@@ -1415,7 +1191,7 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
     }
     // Entries [0, first) went up with an earlier range: growing may drop them.
     {
-        NphResult r = reserve_pinned(d, (void **)&d->h_list, &d->cap_h_list, (u64)count * 4U);
+        NphResult r = devi_reserve_pinned(d, (void **)&d->h_list, &d->cap_h_list, (u64)count * 4U);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -1443,7 +1219,7 @@ void nph_dev_note_mesh_shape(NphDevice *d) { d->mesh_shapes += 1U; }
 NphResult nph_dev_collide_begin(NphDevice *d, const u32 *awake, u32 count, f32 step_dt)
 {
     {
-        const NphResult fr = flush_staged(d);
+        const NphResult fr = devi_flush_staged(d);
         if (fr != NPH_RESULT_OK) {
             return fr;
         }
@@ -1471,7 +1247,7 @@ NphResult nph_dev_collide_end(NphDevice *d, const u32 *awake, u32 count, f32 ste
         }
     }
     if (count > 0U) {
-        NphResult r = reserve_pinned(d, (void **)&d->h_codes, &d->cap_h_codes, (u64)count * 4U);
+        NphResult r = devi_reserve_pinned(d, (void **)&d->h_codes, &d->cap_h_codes, (u64)count * 4U);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -1499,7 +1275,7 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody
     *fast_count = 0U;
     d->pending_sd.body_count = 0U;
     {
-        const NphResult fr = flush_staged(d);
+        const NphResult fr = devi_flush_staged(d);
         if (fr != NPH_RESULT_OK) {
             return fr;
         }
@@ -1531,8 +1307,8 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody
         joints += jcount[c];
     }
     if (joints > d->cap_step_joints) {
-        const u32 c = grow_cap(d->cap_step_joints, joints);
-        NphResult gr = grow_buffer((void **)&d->step_joints, 0, (u64)c * 4U, d->stream, 0);
+        const u32 c = devi_grow_cap(d->cap_step_joints, joints);
+        NphResult gr = devi_grow_buffer((void **)&d->step_joints, 0, (u64)c * 4U, d->stream, 0);
         if (gr != NPH_RESULT_OK) {
             return gr;
         }
@@ -1721,7 +1497,7 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody
     hipLaunchKernelGGL(k_fast_scatter, dim3(blocks_for(n)), dim3(NPH_BLOCK), 0, s, sd,
                        d->grown_blocks, d->fast_list);
     HIP_TRY(hipGetLastError());
-    NphResult pr = reserve_pinned(d, (void **)&d->h_fast_count, &d->cap_h_fast_count, 64U);
+    NphResult pr = devi_reserve_pinned(d, (void **)&d->h_fast_count, &d->cap_h_fast_count, 64U);
     if (pr != NPH_RESULT_OK) {
         return pr;
     }
@@ -1732,14 +1508,14 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody
         return NPH_RESULT_OK;
     }
     if (f > d->cap_fast_pack) {
-        const u32 c = grow_cap(d->cap_fast_pack, f);
-        pr = grow_buffer((void **)&d->fast_pack, 0, (u64)c * sizeof(NphFastBody), s, 0);
+        const u32 c = devi_grow_cap(d->cap_fast_pack, f);
+        pr = devi_grow_buffer((void **)&d->fast_pack, 0, (u64)c * sizeof(NphFastBody), s, 0);
         if (pr != NPH_RESULT_OK) {
             return pr;
         }
         d->cap_fast_pack = c;
     }
-    pr = reserve_pinned(d, (void **)&d->h_fast_pack, &d->cap_h_fast_pack,
+    pr = devi_reserve_pinned(d, (void **)&d->h_fast_pack, &d->cap_h_fast_pack,
                         (u64)f * sizeof(NphFastBody));
     if (pr != NPH_RESULT_OK) {
         return pr;
@@ -1760,7 +1536,7 @@ NphResult nph_dev_solve_finish(NphDevice *d, const NphGrown **grown, u32 *grown_
     *grown = NULL;
     *grown_count = 0U;
     {
-        const NphResult fr = flush_staged(d); // the host's sweeps
+        const NphResult fr = devi_flush_staged(d); // the host's sweeps
         if (fr != NPH_RESULT_OK) {
             return fr;
         }
@@ -1786,7 +1562,7 @@ NphResult nph_dev_solve_finish(NphDevice *d, const NphGrown **grown, u32 *grown_
     HIP_TRY(hipStreamSynchronize(s));
     const u32 m = d->h_grown[0].shape;
     if (m + 1U > d->cap_h_grown) {
-        const u32 c = grow_cap(d->cap_h_grown, m + 1U);
+        const u32 c = devi_grow_cap(d->cap_h_grown, m + 1U);
         HIP_TRY(hipHostFree(d->h_grown));
         d->h_grown = NULL;
         HIP_TRY(hipHostMalloc((void **)&d->h_grown, (u64)c * sizeof(NphGrown), hipHostMallocDefault));
@@ -1863,6 +1639,11 @@ NphResult nph_dev_set_warm_start(NphDevice *d, const NphAmdWarmStart *ws)
 
 void nph_dev_measure_residual(NphDevice *d, int enable) { d->measure = enable; }
 
+int nph_dev_learned_active(const NphDevice *d)
+{
+    return d->learn_on || d->measure || d->log_cap > 0U;
+}
+
 NphResult nph_dev_warm_stats(NphDevice *d, NphAmdWarmStats *out)
 {
     memset(out, 0, sizeof *out);
@@ -1937,15 +1718,15 @@ static NphResult reserve_learn(NphDevice *d, u32 lanes)
     if (r != NPH_RESULT_OK || lanes <= d->cap_learn_lanes) {
         return r;
     }
-    const u32 c = grow_cap(d->cap_learn_lanes, lanes);
+    const u32 c = devi_grow_cap(d->cap_learn_lanes, lanes);
     const u64 rows = (u64)c * NPH_LEARNED_ROWS_PER_LANE;
-    if ((r = grow_buffer((void **)&d->lx, 0, rows * NPH_LEARNED_IN * 2U, d->stream, 0)) !=
+    if ((r = devi_grow_buffer((void **)&d->lx, 0, rows * NPH_LEARNED_IN * 2U, d->stream, 0)) !=
             NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->lraw, 0, rows * NPH_LEARNED_IN * 4U, d->stream, 0)) !=
+        (r = devi_grow_buffer((void **)&d->lraw, 0, rows * NPH_LEARNED_IN * 4U, d->stream, 0)) !=
             NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->ly, 0, rows * NPH_LEARNED_OUT * 4U, d->stream, 0)) !=
+        (r = devi_grow_buffer((void **)&d->ly, 0, rows * NPH_LEARNED_OUT * 4U, d->stream, 0)) !=
             NPH_RESULT_OK ||
-        (r = grow_buffer((void **)&d->lres, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK) {
+        (r = devi_grow_buffer((void **)&d->lres, 0, (u64)c * 4U, d->stream, 0)) != NPH_RESULT_OK) {
         return r;
     }
     d->cap_learn_lanes = c;

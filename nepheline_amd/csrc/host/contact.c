@@ -6,6 +6,8 @@
 // the per-contact narrow phase runs on the device (nph_dev_collide).
 #include "world_impl.h"
 
+#include <omp.h>
+
 // ---------------------------------------------------------------------------
 // The pair map
 // ---------------------------------------------------------------------------
@@ -535,6 +537,17 @@ NphResult contacts_collide_end(NphPhysicsWorld *w)
     if (r != NPH_RESULT_OK || n == 0U) {
         return r;
     }
+    return contacts_apply_codes(w, codes, n, NULL, 0U);
+}
+
+NphResult contacts_apply_codes(NphPhysicsWorld *w, const u32 *codes_a, u32 na, const u32 *codes_b,
+                               u32 nb)
+{
+    const u32 n = na + nb;
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    NphResult r;
     if ((r = vbuf_reserve(w->alloc, &w->changed, 4U, n)) != NPH_RESULT_OK ||
         (r = vbuf_reserve(w->alloc, &w->codes, 4U, n)) != NPH_RESULT_OK) {
         return r;
@@ -546,9 +559,10 @@ NphResult contacts_collide_end(NphPhysicsWorld *w)
     u32 *changed_codes = VB(w->codes, u32);
     u32 m = 0U;
     for (u32 i = 0U; i < n; ++i) {
-        if (codes[i] != 0U) {
+        const u32 code = i < na ? codes_a[i] : codes_b[i - na];
+        if (code != 0U) {
             changed[m] = list[i];
-            changed_codes[m] = codes[i];
+            changed_codes[m] = code;
             m += 1U;
         }
     }
@@ -668,7 +682,7 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
     }
     u32 *counts = VB(w->pair_counts, u32);
     const u32 *moved = VB(w->move_buffer, u32);
-#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U)
+#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U && !omp_in_parallel())
     for (u32 i = 0U; i < n; ++i) {
         counts[i] = query_moved(w, moved[i], NULL);
     }
@@ -686,7 +700,7 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
     w->pair_buffer.len = total;
     u32 *pairs = VB(w->pair_buffer, u32);
     if (total > 0U) {
-#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U)
+#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U && !omp_in_parallel())
         for (u32 i = 0U; i < n; ++i) {
             if (counts[i + 1U] > counts[i]) {
                 (void)query_moved(w, moved[i], &pairs[2U * counts[i]]);

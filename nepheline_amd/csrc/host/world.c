@@ -223,7 +223,8 @@ static int finite_nonneg(f32 v) { return nph_isfinitef(v) && v >= 0.0F; }
 
 void nph_physics_world_destroy(NphPhysicsWorld *w);
 
-NphResult nph_physics_world_create(const NphPhysicsWorldDesc *desc, NphPhysicsWorld **out)
+NphResult world_create(const NphPhysicsWorldDesc *desc, NphDeviceBatch *batch,
+                       NphPhysicsWorld **out)
 {
     if (desc == NULL || out == NULL || desc->alloc == NULL || desc->substeps == 0U ||
         (u32)desc->solver >= (u32)NPH_PHYSICS_SOLVER_COUNT || desc->substeps > 64U ||
@@ -249,7 +250,7 @@ NphResult nph_physics_world_create(const NphPhysicsWorldDesc *desc, NphPhysicsWo
     w->alloc = a;
     w->desc = *desc;
     w->hit_threshold = 1.0F;
-    NphResult r = nph_dev_create(&w->dev);
+    NphResult r = batch != NULL ? nph_devb_world(batch, &w->dev) : nph_dev_create(&w->dev);
     for (u32 t = 0U; t < 3U && r == NPH_RESULT_OK; ++t) {
         r = tree_init(&w->trees[t], a, 16U);
     }
@@ -266,6 +267,11 @@ NphResult nph_physics_world_create(const NphPhysicsWorldDesc *desc, NphPhysicsWo
     }
     *out = w;
     return NPH_RESULT_OK;
+}
+
+NphResult nph_physics_world_create(const NphPhysicsWorldDesc *desc, NphPhysicsWorld **out)
+{
+    return world_create(desc, NULL, out);
 }
 
 void nph_physics_world_destroy(NphPhysicsWorld *w)
@@ -1036,14 +1042,17 @@ static NphResult hit_events(NphPhysicsWorld *w)
     return NPH_RESULT_OK;
 }
 
-static NphResult solve(NphPhysicsWorld *w, f32 dt)
+NphResult world_solve_input(NphPhysicsWorld *w, f32 dt, NphSolveInput *out)
 {
-    const u32 n = w->awake_bodies.len;
-    if (n == 0U) {
-        return NPH_RESULT_OK;
-    }
     NphSolveInput in;
     memset(&in, 0, sizeof in);
+    const u32 n = w->awake_bodies.len;
+    w->stats.swept_bodies = 0U;
+    w->stats.impact_bodies = 0U;
+    if (n == 0U) {
+        *out = in;
+        return NPH_RESULT_OK;
+    }
     in.awake_bodies = VB(w->awake_bodies, u32);
     in.awake_count = n;
     in.body_slots = w->bodies.len;
@@ -1102,8 +1111,24 @@ static NphResult solve(NphPhysicsWorld *w, f32 dt)
               &sp->joint_bias, &sp->joint_mass, &sp->joint_impulse);
     w->last_substep = sp->h;
     sp->continuous = w->desc.enable_continuous != 0 ? 1U : 0U;
-    w->stats.swept_bodies = 0U;
-    w->stats.impact_bodies = 0U;
+    *out = in;
+    return NPH_RESULT_OK;
+}
+
+NphResult world_solve_post(NphPhysicsWorld *w, const NphGrown *grown, u32 grown_count)
+{
+    w->bodies_stale = 1;
+    NphResult r = contacts_post_solve(w, grown, grown_count);
+    return r == NPH_RESULT_OK ? hit_events(w) : r;
+}
+
+static NphResult solve(NphPhysicsWorld *w, f32 dt)
+{
+    NphSolveInput in;
+    NphResult r = world_solve_input(w, dt, &in);
+    if (r != NPH_RESULT_OK || in.awake_count == 0U) {
+        return r;
+    }
     const NphFastBody *fast = NULL;
     u32 fast_count = 0U;
     r = nph_dev_solve(w->dev, &in, &fast, &fast_count);
@@ -1124,9 +1149,21 @@ static NphResult solve(NphPhysicsWorld *w, f32 dt)
     if (r != NPH_RESULT_OK) {
         return r;
     }
-    r = contacts_post_solve(w, grown, grown_count);
-    return r == NPH_RESULT_OK ? hit_events(w) : r;
+    return world_solve_post(w, grown, grown_count);
 }
+
+void world_step_begin(NphPhysicsWorld *w, f32 dt)
+{
+    w->stepping = 1;
+    w->step_dt = dt;
+    w->begin_events.len = 0U;
+    w->hit_events.len = 0U;
+    w->end_events.len = 0U;
+    w->move_events.len = 0U;
+    w->move_events_ready = 0;
+}
+
+f64 world_now_ms(void) { return now_ms(); }
 
 NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
 {
@@ -1136,13 +1173,7 @@ NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
     }
     const f64 t0 = now_ms();
     NphResult r = NPH_RESULT_OK;
-    w->stepping = 1;
-    w->step_dt = dt;
-    w->begin_events.len = 0U;
-    w->hit_events.len = 0U;
-    w->end_events.len = 0U;
-    w->move_events.len = 0U;
-    w->move_events_ready = 0;
+    world_step_begin(w, dt);
     // The existing contacts' narrow phase runs on the device while the host
     // finds new pairs; the new contacts' follows (world.c:330-345 order of
     // results).

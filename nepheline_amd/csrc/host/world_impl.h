@@ -241,8 +241,22 @@ NphResult world_wake_body(NphPhysicsWorld *w, u32 slot);
 
 // contact.c restated
 NphResult contacts_update_pairs(NphPhysicsWorld *w);
+// The step in phases, for the batch that steps many worlds together.
+NphResult world_create(const NphPhysicsWorldDesc *desc, NphDeviceBatch *batch,
+                       NphPhysicsWorld **out);
+void world_step_begin(NphPhysicsWorld *w, f32 dt);
+// The solver's input (the awake list, the graph's colour lists, the step's
+// numbers); in->awake_count = 0 when nothing is awake.
+NphResult world_solve_input(NphPhysicsWorld *w, f32 dt, NphSolveInput *in);
+// After the device's solve: grown proxies, pairs' bookkeeping, hit events.
+NphResult world_solve_post(NphPhysicsWorld *w, const NphGrown *grown, u32 grown_count);
+f64 world_now_ms(void);
 NphResult contacts_collide_begin(NphPhysicsWorld *w);
 NphResult contacts_collide_end(NphPhysicsWorld *w);
+// The narrow phase's change codes of the awake list, entries [0, na) from
+// codes_a and [na, na + nb) from codes_b, applied in list order.
+NphResult contacts_apply_codes(NphPhysicsWorld *w, const u32 *codes_a, u32 na, const u32 *codes_b,
+                               u32 nb);
 NphResult contacts_post_solve(NphPhysicsWorld *w, const NphGrown *grown, u32 count);
 NphResult move_buffer_push(NphPhysicsWorld *w, u32 shape);
 u32 graph_pick_color(NphPhysicsWorld *w, u32 body_a, u32 body_b);

@@ -124,6 +124,63 @@ NphResult nph_dev_log_contacts(NphDevice *d, u32 capacity);
 NphResult nph_dev_contact_log(NphDevice *d, u32 *rows, const f32 **features,
                               const f32 **targets);
 
+// A learned warm start, residual measurement or contact log is on.
+int nph_dev_learned_active(const NphDevice *d);
+
+// ---------------------------------------------------------------------------
+// Many worlds stepped together: each world keeps its own records (its
+// NphDevice), their work goes on one stream, and every stage of the step is
+// one launch over all the worlds' items (device_batch.hip).
+// ---------------------------------------------------------------------------
+typedef struct NphDeviceBatch NphDeviceBatch;
+
+// Launch choices of the batched step (the autotuner picks them).
+typedef struct NphBatchTuning {
+    u32 pass_block;  // threads a block of the colour passes: 64, 128 or 256
+    u32 body_block;  // threads a block of the per-body and per-lane stages
+    u32 use_graph;   // the substep loop launched as a captured HIP graph
+    u32 pad;
+} NphBatchTuning;
+
+NphResult nph_devb_create(NphDeviceBatch **out);
+void nph_devb_destroy(NphDeviceBatch *b);
+// A world's device whose work goes on the batch's stream.
+NphResult nph_devb_world(NphDeviceBatch *b, NphDevice **out);
+void nph_devb_set_tuning(NphDeviceBatch *b, const NphBatchTuning *t);
+NphBatchTuning nph_devb_tuning(const NphDeviceBatch *b);
+
+// One world's part of a narrow-phase launch: the records made since the
+// last launch (inits), then entries [first, count) of its awake contacts.
+typedef struct NphCollidePart {
+    NphDevice *dev;
+    const NphContactInit *inits;
+    u32 init_count;
+    const u32 *list;
+    u32 first;
+    u32 count;
+} NphCollidePart;
+
+// Half 0 queues the contacts alive when the step starts and returns at once
+// (the lists are copied before it returns); half 1 queues the rest, waits,
+// and points codes[0] / codes[1] at the change codes of each half, the
+// parts' entries one after another in part order (valid until the next
+// call).
+NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n, f32 step_dt,
+                           u32 half, const u32 *codes[2]);
+// The solver stages and finalize for every world (all worlds share one
+// NphStepParams: in[0]'s); the grown proxies of world k are
+// (*grown)[(*starts)[k] .. (*starts)[k + 1]), in that world's awake order.
+// Continuous collision and the learned warm start are not batched.
+NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSolveInput *in,
+                         u32 n, const NphGrown **grown, const u32 **starts);
+// Device milliseconds of the last batched solve: out[0] the whole solve,
+// out[1] the constraint passes, out[2] the narrow phase (both halves).
+void nph_devb_times(const NphDeviceBatch *b, f64 out[3]);
+// Per-launch timing of the batched colour passes (not in graph mode):
+// out[0] = milliseconds summed, out[1] = launches.
+void nph_devb_time_kernels(NphDeviceBatch *b, int enable);
+void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[2]);
+
 #ifdef __cplusplus
 }
 #endif

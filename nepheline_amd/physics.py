@@ -409,31 +409,40 @@ class Mesh(C.Structure):
                 ("originals_offset", C.c_uint32), ("flags", C.c_uint32), ("bounds", AABB)]
 
 
-class World:
-    """A world of either library, with the calls the tests and the bench use."""
+def world_desc(a: Api, overrides: dict | None = None) -> WorldDesc:
+    d = a.world_desc_default()
+    d.alloc = a.allocator()
+    for k, v in (overrides or {}).items():
+        if k == "gravity":
+            d.gravity = Vec3(*v)
+        else:
+            setattr(d, k, v)
+    return d
 
-    def __init__(self, a: Api, desc_overrides: dict | None = None):
+
+class World:
+    """A world of either library, with the calls the tests and the bench use.
+    With `ptr`, it wraps a world a batch made and owns (close() leaves it)."""
+
+    def __init__(self, a: Api, desc_overrides: dict | None = None, ptr=None):
         self.api = a
-        d = a.world_desc_default()
-        d.alloc = a.allocator()
-        for k, v in (desc_overrides or {}).items():
-            if k == "gravity":
-                d.gravity = Vec3(*v)
-            else:
-                setattr(d, k, v)
-        self.desc = d
-        w = C.c_void_p()
-        r = a.world_create(C.byref(d), C.byref(w))
-        if r != OK:
-            raise RuntimeError(f"nph_physics_world_create failed: {r}")
-        self.ptr = w
+        self.desc = world_desc(a, desc_overrides)
+        self.owned = ptr is None
+        if ptr is None:
+            w = C.c_void_p()
+            r = a.world_create(C.byref(self.desc), C.byref(w))
+            if r != OK:
+                raise RuntimeError(f"nph_physics_world_create failed: {r}")
+            ptr = w
+        self.ptr = ptr
         self.bodies: list[int] = []
         self.shapes: list[int] = []
         self._keep = []  # hull blocks and other memory the world points at
 
     def close(self):
         if self.ptr:
-            self.api.world_destroy(self.ptr)
+            if self.owned:
+                self.api.world_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -562,3 +571,90 @@ def contacts_as_arrays(contacts):
         by_pair[(c.shape_a, c.shape_b)] = c
     ids.sort()
     return pairs, ids, by_pair
+
+
+# ---------------------------------------------------------------------------
+# Batched worlds (nph_amd_batch_*: this library's extension)
+# ---------------------------------------------------------------------------
+class BatchTuning(C.Structure):
+    _fields_ = [("pass_block", _U32), ("body_block", _U32), ("use_graph", _U32), ("pad", _U32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("solved_contacts", C.c_uint64), ("awake_bodies", C.c_uint64),
+                ("step_ms", C.c_double), ("pairs_ms", C.c_double), ("changes_ms", C.c_double),
+                ("solve_ms", C.c_double), ("post_ms", C.c_double),
+                ("device_solve_ms", C.c_double), ("device_passes_ms", C.c_double),
+                ("device_collide_ms", C.c_double)]
+
+
+def _bind_batch(a: Api):
+    if hasattr(a, "batch_create"):
+        return
+    L = a.lib
+    a.batch_create = _bind(L, "nph_amd_batch_create", C.c_int,
+                           [C.POINTER(WorldDesc), _U32, C.POINTER(C.c_void_p)])
+    a.batch_destroy = _bind(L, "nph_amd_batch_destroy", None, [C.c_void_p])
+    a.batch_world = _bind(L, "nph_amd_batch_world", C.c_void_p, [C.c_void_p, _U32])
+    a.batch_step = _bind(L, "nph_amd_batch_step", C.c_int, [C.c_void_p, C.c_float])
+    a.batch_stats = _bind(L, "nph_amd_batch_stats", BatchStats, [C.c_void_p])
+    a.batch_set_tuning = _bind(L, "nph_amd_batch_set_tuning", None,
+                               [C.c_void_p, C.POINTER(BatchTuning)])
+    a.batch_tuning = _bind(L, "nph_amd_batch_tuning", BatchTuning, [C.c_void_p])
+    a.batch_autotune = _bind(L, "nph_amd_batch_autotune", C.c_int,
+                             [C.c_void_p, C.c_float, _U32, C.POINTER(BatchTuning)])
+    a.batch_time_kernels = _bind(L, "nph_amd_batch_time_kernels", None, [C.c_void_p, C.c_bool])
+    a.batch_kernel_times = _bind(L, "nph_amd_batch_kernel_times", None,
+                                 [C.c_void_p, C.POINTER(C.c_double)])
+
+
+class Batch:
+    """`count` worlds of one description stepped together; `worlds[k]` is
+    filled and read like any World."""
+
+    def __init__(self, a: Api, count: int, desc_overrides: dict | None = None):
+        _bind_batch(a)
+        self.api = a
+        self.desc = world_desc(a, desc_overrides)
+        b = C.c_void_p()
+        r = a.batch_create(C.byref(self.desc), count, C.byref(b))
+        if r != OK:
+            raise RuntimeError(f"nph_amd_batch_create failed: {r}")
+        self.ptr = b
+        self.worlds = [World(a, desc_overrides, ptr=C.c_void_p(a.batch_world(b, k)))
+                       for k in range(count)]
+
+    def step(self, dt):
+        r = self.api.batch_step(self.ptr, dt)
+        if r != OK:
+            raise RuntimeError(f"nph_amd_batch_step failed: {r}")
+
+    def stats(self) -> BatchStats:
+        return self.api.batch_stats(self.ptr)
+
+    def tuning(self) -> BatchTuning:
+        return self.api.batch_tuning(self.ptr)
+
+    def set_tuning(self, pass_block=256, body_block=256, use_graph=0):
+        t = BatchTuning(pass_block, body_block, use_graph, 0)
+        self.api.batch_set_tuning(self.ptr, C.byref(t))
+
+    def autotune(self, dt, rounds=2) -> BatchTuning:
+        t = BatchTuning()
+        r = self.api.batch_autotune(self.ptr, dt, rounds, C.byref(t))
+        if r != OK:
+            raise RuntimeError(f"nph_amd_batch_autotune failed: {r}")
+        return t
+
+    def close(self):
+        if self.ptr:
+            for w in self.worlds:
+                w.ptr = None
+            self.api.batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -83,12 +83,20 @@ DESC = {"enable_sleep": 0, "enable_continuous": 0}
 DT = 1.0 / 60.0
 
 
-def pyramid(api, rows: int = 20, desc: dict | None = None) -> P.World:
+def _world(api, desc: dict | None, world: P.World | None) -> P.World:
+    """The world to fill: `world` (one a batch made) or a new one."""
+    if world is not None:
+        return world
     d = dict(DESC)
     d.update(desc or {})
-    w = P.World(api, d)
+    return P.World(api, d)
+
+
+def pyramid(api, rows: int = 20, desc: dict | None = None, world: P.World | None = None,
+            ground_half: float = 50.0) -> P.World:
+    w = _world(api, desc, world)
     g = w.add_body(P.BODY_STATIC, (0.0, -1.0, 0.0))
-    w.add_shape(g, P.geometry_box(50.0, 1.0, 50.0), friction=0.6)
+    w.add_shape(g, P.geometry_box(ground_half, 1.0, ground_half), friction=0.6)
     for row in range(rows):
         count = rows - row
         for i in range(count):
@@ -120,12 +128,11 @@ def load_hull_bank(path: str) -> HullBank:
 
 
 def bin_drop(api, bodies: int, seed: int, hulls: HullBank | None, half: float = 20.0,
-             height: float = 20.0, desc: dict | None = None, y_lo: float = 5.0) -> P.World:
+             height: float = 20.0, desc: dict | None = None, y_lo: float = 5.0,
+             world: P.World | None = None) -> P.World:
     """`bodies` mixed shapes dropped into an open bin of static boxes
     (floor and four walls), x, z in [-half, half], y in [y_lo, height]."""
-    d = dict(DESC)
-    d.update(desc or {})
-    w = P.World(api, d)
+    w = _world(api, desc, world)
     floor = w.add_body(P.BODY_STATIC, (0.0, -1.0, 0.0))
     w.add_shape(floor, P.geometry_box(half + 1.0, 1.0, half + 1.0), friction=0.5)
     wall_h = height * 0.5
@@ -212,12 +219,10 @@ def terrain_grid(r: Pcg32, nx: int, nz: int, cell: float, amp: float):
 
 
 def mesh_terrain_drop(api, meshes: MeshBank, bodies: int, seed: int, hulls: HullBank | None,
-                      desc: dict | None = None) -> P.World:
+                      desc: dict | None = None, world: P.World | None = None) -> P.World:
     """Mixed bodies dropped from 3-12 m onto the bank's first mesh (a
     32 x 32 m height grid) on a static body at the origin."""
-    d = dict(DESC)
-    d.update(desc or {})
-    w = P.World(api, d)
+    w = _world(api, desc, world)
     ground = w.add_body(P.BODY_STATIC, (0.0, 0.0, 0.0))
     w.add_shape(ground, P.geometry_mesh(meshes.ptrs[0]), friction=0.5)
     drop_mixed(w, Pcg32(seed), bodies, hulls, -14.0, 14.0, 3.0, 12.0)
@@ -395,20 +400,14 @@ def ground(w: P.World, top=0.0):
     return g
 
 
-def ragdoll_pile(api, count: int = 4, seed: int = 2026, desc: dict | None = None) -> P.World:
-    """`count` ragdolls lying criss-cross one above another, each a quarter
-    turn from the last and thrown at random (build_pile, test_ragdoll.c:
-    289-303), on a ground slab."""
-    d = dict(DESC)
-    d.update(desc or {})
-    w = P.World(api, d)
-    ground(w, 0.0)
-    r = Pcg32(seed)
+def _pile(w: P.World, r: Pcg32, count: int, x: float = 0.0, z: float = 0.0):
+    """`count` ragdolls lying criss-cross one above another at (x, z), each a
+    quarter turn from the last and thrown at random (build_pile,
+    test_ragdoll.c:289-303)."""
     lying = quat_axis_angle((1.0, 0.0, 0.0), _F(_F(-90.0) * DEG))
-    w.ragdolls = []
     for k in range(count):
         turn = quat_mul(quat_axis_angle((0.0, 1.0, 0.0), _F(_F(_F(k) * _F(90.0)) * DEG)), lying)
-        pelvis = (0.0, 0.15 + 0.3 * float(k), 0.0)
+        pelvis = (x, 0.15 + 0.3 * float(k), z)
         off = ragdoll_turn(turn, (0.0, 1.0, 0.0))
         base = (pelvis[0] - off[0], pelvis[1] - off[1], pelvis[2] - off[2])
         bodies, joints = ragdoll(w, base, turn)
@@ -416,6 +415,67 @@ def ragdoll_pile(api, count: int = 4, seed: int = 2026, desc: dict | None = None
         om = tuple(float(r.next_range(-1.0, 1.0)) for _ in range(3))
         throw_ragdoll(w, bodies, v, om)
         w.ragdolls.append((bodies, joints))
+
+
+def ragdoll_pile(api, count: int = 4, seed: int = 2026, desc: dict | None = None,
+                 world: P.World | None = None) -> P.World:
+    """`count` ragdolls in one criss-cross pile on a ground slab."""
+    w = _world(api, desc, world)
+    ground(w, 0.0)
+    w.ragdolls = []
+    _pile(w, Pcg32(seed), count)
+    return w
+
+
+def ragdoll_piles(api, ragdolls: int, seed: int, desc: dict | None = None,
+                  world: P.World | None = None, per_pile: int = 4,
+                  spacing: float = 4.0) -> P.World:
+    """`ragdolls` ragdolls in piles of `per_pile` (the pile above) on a
+    square grid `spacing` m apart, on one ground slab."""
+    w = _world(api, desc, world)
+    piles = (ragdolls + per_pile - 1) // per_pile
+    side = max(1, int(math.ceil(math.sqrt(piles))))
+    half = 0.5 * spacing * float(side)
+    g = w.add_body(P.BODY_STATIC, (0.0, -1.0, 0.0))
+    w.add_shape(g, P.geometry_box(half + 5.0, 1.0, half + 5.0), friction=0.6)
+    w.ragdolls = []
+    r = Pcg32(seed)
+    left = ragdolls
+    for p in range(piles):
+        x = -half + spacing * (float(p % side) + 0.5)
+        z = -half + spacing * (float(p // side) + 0.5)
+        _pile(w, r, min(per_pile, left), x, z)
+        left -= per_pile
+    return w
+
+
+def batch_plan(seed: int, worlds: int):
+    """Each world of the batched workload: (kind, bodies, seed). Bodies =
+    floor(16 * 1250^u), u uniform in [0, 1) (log-uniform in [16, 20 000]);
+    kind 0 pyramid, 1 bin drop, 2 ragdoll piles."""
+    r = Pcg32(seed)
+    out = []
+    for _ in range(worlds):
+        u = r.next_f32()
+        n = int(math.floor(16.0 * 1250.0 ** u))
+        kind = r.next_bounded(3)
+        out.append((kind, n, r.next_u32()))
+    return out
+
+
+def fill_batch_world(w: P.World, kind: int, bodies: int, seed: int, hulls: HullBank | None):
+    """A world of the batched workload with about `bodies` dynamic bodies: a
+    pyramid (the most rows whose boxes fit), a bin drop (bin area scaled
+    with the count, 40 x 40 m at 10 000), or ragdolls (bodies // 11) in
+    piles of four."""
+    if kind == 0:
+        rows = max(1, int((math.isqrt(8 * bodies + 1) - 1) // 2))
+        pyramid(None, rows, world=w, ground_half=max(50.0, 0.55 * rows + 5.0))
+    elif kind == 1:
+        half = max(3.0, 20.0 * math.sqrt(bodies / 10000.0))
+        bin_drop(None, bodies, seed, hulls, half=half, world=w)
+    else:
+        ragdoll_piles(None, max(1, bodies // 11), seed, world=w)
     return w
 
 

@@ -224,3 +224,35 @@ def test_combine_rules_match_the_reference(ref_api):
                         ref_api.combine_friction(x * 2, ma, y * 2, mb)
                     assert a.combine_restitution(x, ma, y, mb) == \
                         ref_api.combine_restitution(x, ma, y, mb)
+
+
+def test_batch_creation_rules():
+    """A batch needs worlds and refuses continuous collision; without a HIP
+    device it refuses to exist at all (no CPU fallback)."""
+    if not os.path.exists(P.PRODUCT_LIB):
+        pytest.skip("libnepheline_amd.so not built")
+    a = P.product_api()
+    P._bind_batch(a)
+    d = P.world_desc(a, S.DESC)
+    b = C.c_void_p()
+    assert a.batch_create(C.byref(d), 0, C.byref(b)) == P.INVALID_ARGUMENT
+    d.enable_continuous = 1
+    assert a.batch_create(C.byref(d), 4, C.byref(b)) == P.UNSUPPORTED
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return
+    except ImportError:
+        pass
+    d.enable_continuous = 0
+    assert a.batch_create(C.byref(d), 4, C.byref(b)) == P.UNSUPPORTED
+
+
+def test_batch_plan_is_log_uniform_over_the_three_kinds():
+    plan = S.batch_plan(4, 1024)
+    sizes = [n for _, n, _ in plan]
+    assert min(sizes) >= 16 and max(sizes) <= 20000
+    assert 2.4e6 < sum(sizes) < 3.3e6  # mean 16 * 1249 / ln 1250 = 2 803
+    assert {k for k, _, _ in plan} == {0, 1, 2}
+    small = sum(1 for n in sizes if n < 566)  # 16 * sqrt(1250): the median
+    assert 440 < small < 584
