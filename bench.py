@@ -16,9 +16,11 @@ the reference's own CPU step (oracle/_ref, its job system on up to 16 host
 threads) on a bounded sample of the same scene, and the learned warm start:
 a network trained on the device from plain bin drops of other seeds, the
 same scene and steps timed with it, the post-solve residual and penetration
-of both starts, and the network's forward kernel against the MFMA peak; and
-the 100 000-body field (meshes, ragdolls, mixed bodies) on the GPU beside the
-reference's CPU step on a bounded sample of it.
+of both starts, and the network's forward kernel against the MFMA peak; the
+100 000-body field (meshes, ragdolls, mixed bodies) on the GPU beside the
+reference's CPU step on a bounded sample of it; and 1 024 worlds of 16 to
+20 000 bodies stepped as one batch, beside the reference stepping a sample
+of them.
 """
 import argparse
 import ctypes as C
@@ -71,6 +73,29 @@ def graph_mix(world):
         tot += contact_bytes(c.point_count, e)
         n += 1
     return (tot / n if n else 0.0), n
+
+
+def graph_bytes(world, static_slots):
+    """Algorithmic bytes of one substep over the graph contacts the world
+    holds now, and their count (numpy over the contact readback). Bodies in
+    slots below `static_slots` are the scene's static ones."""
+    n = world.api.world_contacts(world.ptr, None, 0)
+    if n == 0:
+        return 0.0, 0
+    buf = (P.ContactInfo * n)()
+    world.api.world_contacts(world.ptr, buf, n)
+    arr = np.frombuffer(buf, dtype=np.dtype(P.ContactInfo), count=n)
+    sel = (arr["color"] != P.NULL32) & (arr["point_count"] > 0)
+    p = arr["point_count"][sel].astype(np.int64)
+    e = (arr["body_a"][sel] >= static_slots).astype(np.int64) + (
+        arr["body_b"][sel] >= static_slots).astype(np.int64)
+    b = 3 * (C_BYTES + p * (K_BYTES + 2 * M_BYTES)) + e * (3 * 2 * V_BYTES + 2 * X_BYTES)
+    return float(b.sum()), int(sel.sum())
+
+
+# Static bodies each kind of batched world makes first (ground; bin floor
+# and walls; ground).
+BATCH_STATIC = {0: 1, 1: 5, 2: 1}
 
 
 def build(api, bodies, seed, bank):
@@ -292,6 +317,139 @@ def run_field(args, bank):
     return out
 
 
+def run_batch(args, bank):
+    """The batched workload: 1 024 worlds (seed 4), each a pyramid, a bin
+    drop or ragdoll piles of floor(16 * 1250^u) bodies, u uniform, stepped
+    together (nph_amd_batch_step) after the autotuner picks the launch
+    choices; contact solves/s over the timed steps, the batched colour
+    pass's roofline, and the reference's CPU step on a sample of the
+    worlds."""
+    a = P.product_api()
+    plan = S.batch_plan(args.batch_seed, args.batch_worlds)
+    t0 = time.perf_counter()
+    batch = P.Batch(a, len(plan), S.DESC)
+    for (kind, n, seed), w in zip(plan, batch.worlds):
+        S.fill_batch_world(w, kind, n, seed, bank)
+    setup = time.perf_counter() - t0
+    bodies = sum(len(w.bodies) for w in batch.worlds)
+    print(f"# batch: {len(plan)} worlds, {bodies} bodies built in {setup:.1f} s",
+          file=sys.stderr, flush=True)
+    rounds = 1
+    tune = batch.autotune(DT, rounds=rounds)
+    tuned_steps = 12 * rounds
+    for _ in range(args.batch_warmup):
+        batch.step(DT)
+    warm = tuned_steps + args.batch_warmup
+    solved = 0
+    phases = np.zeros(8)
+    t0 = time.perf_counter()
+    for i in range(args.batch_steps):
+        batch.step(DT)
+        st = batch.stats()
+        solved += st.solved_contacts
+        phases += (st.step_ms, st.pairs_ms, st.changes_ms, st.solve_ms, st.post_ms,
+                   st.device_solve_ms, st.device_passes_ms, st.device_collide_ms)
+        if (i + 1) % 25 == 0:
+            print(f"# batch step {i + 1}/{args.batch_steps}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    # Roofline window: the tuned blocks, launched from the stream so each
+    # colour pass is bracketed by HIP events.
+    batch.set_tuning(tune.pass_block, tune.body_block, 0)
+    a.batch_time_kernels(batch.ptr, True)
+    k_solved = 0
+    for _ in range(args.batch_roofline_steps):
+        batch.step(DT)
+        k_solved += batch.stats().solved_contacts
+    kt = (C.c_double * 2)()
+    a.batch_kernel_times(batch.ptr, kt)
+    a.batch_time_kernels(batch.ptr, False)
+    tot_b = 0.0
+    tot_n = 0
+    for (kind, _, _), w in zip(plan, batch.worlds):
+        b, n = graph_bytes(w, BATCH_STATIC[kind])
+        tot_b += b
+        tot_n += n
+    batch.close()
+    mean_bytes = tot_b / tot_n if tot_n else 0.0
+    launches = max(int(kt[1]), 1)
+    bytes_per_launch = k_solved * SUBSTEPS * mean_bytes / launches
+    avg_launch_s = kt[0] * 1e-3 / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    names = ("step", "host_pairs", "host_changes", "solve", "host_post", "dev_solve",
+             "dev_passes", "dev_collide")
+    out = {"workload": "batch: %d worlds (seed %d; pyramids, bin drops, ragdoll piles of "
+                       "floor(16*1250^u) bodies), %d bodies, dt 1/60, 4 substeps, plain solver"
+                       % (len(plan), args.batch_seed, bodies),
+           "value": solved * SUBSTEPS / wall, "unit": "contact-substeps/s",
+           "steps": args.batch_steps, "warmup": warm, "ms_per_step": wall * 1e3 / args.batch_steps,
+           "setup_s": setup, "bodies": bodies, "worlds": len(plan),
+           "tuning": {"pass_block": tune.pass_block, "body_block": tune.body_block,
+                      "use_graph": bool(tune.use_graph)},
+           "phases_ms_per_step": {k: round(v / max(args.batch_steps, 1), 3)
+                                  for k, v in zip(names, phases)},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "k_b_color_pass (one colour of one constraint pass over every "
+                                  "world), HIP events around each launch over %d steps after "
+                                  "the timed region" % args.batch_roofline_steps,
+                        "launches": int(kt[1]), "avg_launch_us": avg_launch_s * 1e6,
+                        "bytes_per_launch": bytes_per_launch,
+                        "bytes_per_contact_substep": mean_bytes},
+           "cpu_baseline": None}
+    if not args.no_cpu:
+        cpu = run_batch_cpu(args, bank, plan, warm)
+        if cpu is not None:
+            out["cpu_baseline"] = cpu
+            out["vs_cpu"] = out["value"] / cpu["value"]
+    return out
+
+
+def run_batch_cpu(args, bank, plan, warm):
+    """The reference's step on every 16th world of the batch (worlds in
+    parallel on up to 16 host threads, each world's step on one), the same
+    steps as the GPU's timed region, stopped after `cpu_seconds`."""
+    from concurrent.futures import ThreadPoolExecutor
+    path = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
+    if not os.path.exists(path):
+        return None
+    ra = P.api(path, "ref_shim_allocator")
+    ra.lib.ref_shim_graph_contacts.restype = C.c_uint32
+    ra.lib.ref_shim_graph_contacts.argtypes = [C.c_void_p]
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    sample = plan[::16]
+    worlds = []
+    for kind, n, seed in sample:
+        w = P.World(ra, S.DESC)
+        S.fill_batch_world(w, kind, n, seed, bank)
+        worlds.append(w)
+    bodies = sum(len(w.bodies) for w in worlds)
+    order = sorted(range(len(worlds)), key=lambda k: -len(worlds[k].bodies))
+
+    def one(k):
+        worlds[k].step(DT)
+        return ra.lib.ref_shim_graph_contacts(worlds[k].ptr)
+
+    with ThreadPoolExecutor(max_workers=cores) as pool:
+        for _ in range(warm):
+            list(pool.map(one, order))
+        solved = 0
+        steps = 0
+        t0 = time.perf_counter()
+        while steps < args.batch_steps:
+            solved += sum(pool.map(one, order))
+            steps += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        cwall = time.perf_counter() - t0
+    for w in worlds:
+        w.close()
+    return {"value": solved * SUBSTEPS / cwall, "unit": "contact-substeps/s", "cores": cores,
+            "kind": "reference",
+            "sample": "the reference's step on every 16th world of the batch (%d worlds, %d "
+                      "bodies), worlds in parallel on %d threads, steps %d-%d (%.1f s)" % (
+                          len(worlds), bodies, cores, warm + 1, warm + steps, cwall)}
+
+
 def run_cpu(args, bank):
     path = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
     if not os.path.exists(path):
@@ -357,6 +515,13 @@ def main():
                     help="skip the 100 000-body field")
     ap.add_argument("--field-steps", type=int, default=150)
     ap.add_argument("--field-warmup", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 1 024-world batch")
+    ap.add_argument("--batch-worlds", type=int, default=1024)
+    ap.add_argument("--batch-seed", type=int, default=4)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-warmup", type=int, default=8)
+    ap.add_argument("--batch-roofline-steps", type=int, default=5)
     ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101])
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--train-epochs", type=int, default=8)
@@ -396,6 +561,7 @@ def main():
         traffic = t["hbm_bytes_per_contact_substep"] * k_cs / launches
     learned = None if args.no_learned else run_learned(args, bank)
     field = None if args.no_field else run_field(args, bank)
+    batched = None if args.no_batch else run_batch(args, bank)
     cpu = None if args.no_cpu else run_cpu(args, bank)
     line = {
         "metric": "contact solves/sec (contacts x substeps/s) plain vs learned warm start, "
@@ -436,6 +602,7 @@ def main():
         },
         "learned_warm_start": learned,
         "field_100k": field,
+        "batch_1024": batched,
         "gpu_hash": g["hash"],
         "phases_ms_per_step": g["phases_ms"] if args.phases else None,
     }

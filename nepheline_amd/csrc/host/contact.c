@@ -590,7 +590,9 @@ typedef struct PairQuery {
     NphPhysicsWorld *w;
     u32 shape;
     u32 key;
-    u32 *out; // NULL: the counting pass
+    u32 *out;          // the pairs' place; NULL: counted, or appended to `append`
+    NphVecBuf *append; // pairs of u32, grown as they come (one pass)
+    NphResult err;
     u32 count;
 } PairQuery;
 
@@ -632,13 +634,21 @@ static int pair_callback(u32 proxy, u64 user_data, void *context)
     if (q->out != NULL) {
         q->out[2U * q->count] = q->shape;
         q->out[2U * q->count + 1U] = other;
+    } else if (q->append != NULL) {
+        const u32 pair[2] = {q->shape, other};
+        q->err = vbuf_push(w->alloc, q->append, 8U, pair);
+        if (q->err != NPH_RESULT_OK) {
+            return 0;
+        }
     }
     q->count += 1U;
     return 1;
 }
 
-// contact.c:748-771: one moved proxy's new pairs, into `out` (NULL: counted).
-static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out)
+// contact.c:748-771: one moved proxy's new pairs, into `out`, or appended
+// to `append`, or (both NULL) only counted.
+static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out, NphVecBuf *append,
+                       NphResult *err)
 {
     const u32 tree = key & 3U;
     const u32 proxy = key >> 2U;
@@ -647,6 +657,8 @@ static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out)
     q.shape = (u32)tree_user_data(&w->trees[tree], proxy);
     q.key = key;
     q.out = out;
+    q.append = append;
+    q.err = NPH_RESULT_OK;
     q.count = 0U;
     const HShape *s = shape_at(w, q.shape);
     const NphAABB box = s->fat_aabb;
@@ -655,7 +667,12 @@ static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out)
         tree_query(&w->trees[NPH_TREE_STATIC], box, mask, pair_callback, &q);
         tree_query(&w->trees[NPH_TREE_KINEMATIC], box, mask, pair_callback, &q);
     }
-    tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, pair_callback, &q);
+    if (q.err == NPH_RESULT_OK) {
+        tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, pair_callback, &q);
+    }
+    if (err != NULL) {
+        *err = q.err;
+    }
     return q.count;
 }
 
@@ -673,37 +690,52 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
     if (n == 0U) {
         return NPH_RESULT_OK;
     }
-    // Count, place by prefix sums, fill (contact.c:820-843): each moved proxy
-    // writes its own range, so the pairs come out in the move buffer's order
-    // and each proxy's in its trees' order, however many threads query.
-    NphResult r = vbuf_reserve(w->alloc, &w->pair_counts, 4U, n + 1U);
-    if (r != NPH_RESULT_OK) {
-        return r;
-    }
-    u32 *counts = VB(w->pair_counts, u32);
     const u32 *moved = VB(w->move_buffer, u32);
-#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U && !omp_in_parallel())
-    for (u32 i = 0U; i < n; ++i) {
-        counts[i] = query_moved(w, moved[i], NULL);
-    }
     u32 total = 0U;
-    for (u32 i = 0U; i < n; ++i) {
-        const u32 c = counts[i];
-        counts[i] = total;
-        total += c;
-    }
-    counts[n] = total;
-    r = vbuf_reserve(w->alloc, &w->pair_buffer, 8U, total > 0U ? total : 1U);
-    if (r != NPH_RESULT_OK) {
-        return r;
-    }
-    w->pair_buffer.len = total;
-    u32 *pairs = VB(w->pair_buffer, u32);
-    if (total > 0U) {
-#pragma omp parallel for schedule(dynamic, 16) if (n >= 64U && !omp_in_parallel())
+    if (n < 64U || omp_in_parallel()) {
+        // One thread (a small move buffer, or this world is one of a
+        // batch's, stepped on a thread of its own): one pass, appending.
+        w->pair_buffer.len = 0U;
         for (u32 i = 0U; i < n; ++i) {
-            if (counts[i + 1U] > counts[i]) {
-                (void)query_moved(w, moved[i], &pairs[2U * counts[i]]);
+            NphResult r = NPH_RESULT_OK;
+            (void)query_moved(w, moved[i], NULL, &w->pair_buffer, &r);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+        }
+        total = w->pair_buffer.len;
+    } else {
+        // Count, place by prefix sums, fill (contact.c:820-843): each moved
+        // proxy writes its own range, so the pairs come out in the move
+        // buffer's order and each proxy's in its trees' order, however many
+        // threads query.
+        NphResult r = vbuf_reserve(w->alloc, &w->pair_counts, 4U, n + 1U);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        u32 *counts = VB(w->pair_counts, u32);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (u32 i = 0U; i < n; ++i) {
+            counts[i] = query_moved(w, moved[i], NULL, NULL, NULL);
+        }
+        for (u32 i = 0U; i < n; ++i) {
+            const u32 c = counts[i];
+            counts[i] = total;
+            total += c;
+        }
+        counts[n] = total;
+        r = vbuf_reserve(w->alloc, &w->pair_buffer, 8U, total > 0U ? total : 1U);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        w->pair_buffer.len = total;
+        u32 *pairs = VB(w->pair_buffer, u32);
+        if (total > 0U) {
+#pragma omp parallel for schedule(dynamic, 16)
+            for (u32 i = 0U; i < n; ++i) {
+                if (counts[i + 1U] > counts[i]) {
+                    (void)query_moved(w, moved[i], &pairs[2U * counts[i]], NULL, NULL);
+                }
             }
         }
     }
