@@ -881,9 +881,12 @@ NphResult nph_amd_trainer_warm_start(const NphAmdTrainer *t, f32 *params, NphAmd
 // stage of the step one launch over every world's items.  Each world's
 // results are bit for bit those it gets stepped alone.  The batch owns its
 // worlds: destroy the batch, not the worlds.  Continuous collision is not
-// batched (creation returns NPH_RESULT_UNSUPPORTED), nor is the learned warm
-// start (the step returns NPH_RESULT_UNSUPPORTED while a world has one, or
-// measures or logs).
+// batched (creation returns NPH_RESULT_UNSUPPORTED), nor is the contact log
+// (the step returns NPH_RESULT_UNSUPPORTED while a world logs).  The learned
+// warm start is: one network for the batch (nph_amd_batch_set_warm_start,
+// or the same one set on each world), its rows for every world in one
+// forward launch; the step returns NPH_RESULT_UNSUPPORTED when two worlds
+// have different networks.
 // ---------------------------------------------------------------------------
 typedef struct NphAmdBatch NphAmdBatch;
 
@@ -923,6 +926,25 @@ NphAmdBatchTuning nph_amd_batch_tuning(const NphAmdBatch *batch);
 // returns it in *chosen.
 NphResult nph_amd_batch_autotune(NphAmdBatch *batch, f32 dt, u32 rounds,
                                  NphAmdBatchTuning *chosen);
+// The learned warm start on every world of the batch (NULL: off), and
+// residual measurement on every world.
+NphResult nph_amd_batch_set_warm_start(NphAmdBatch *batch, const NphAmdWarmStart *ws);
+void nph_amd_batch_measure_residual(NphAmdBatch *batch, bool enable);
+// The residual statistics summed over every measured world and step since
+// the last reset (NphAmdWarmStats' quantities, as sums and maxima).
+typedef struct NphAmdBatchWarmStats {
+    f64 residual_sum_plain;   // contacts' post-solve residuals (m/s), plain start
+    f64 residual_sum_learned; // and learned start
+    f64 contacts_plain;       // contact-steps measured
+    f64 contacts_learned;
+    f64 residual_max_plain;
+    f64 residual_max_learned;
+    f64 penetration_max;
+    f64 gated;
+    u32 steps;
+    u32 pad;
+} NphAmdBatchWarmStats;
+NphResult nph_amd_batch_warm_stats(NphAmdBatch *batch, NphAmdBatchWarmStats *out, bool reset);
 // HIP events around every colour-pass launch (not in graph mode): out[0] =
 // device milliseconds summed since enabling, out[1] = launches.
 void nph_amd_batch_time_kernels(NphAmdBatch *batch, bool enable);

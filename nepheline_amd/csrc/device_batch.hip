@@ -43,6 +43,21 @@ typedef struct NphRange {
     u32 first, count, jfirst, jcount;
 } NphRange;
 
+// A world's part in the learned warm start: its gate and statistics, its
+// lanes' place in the batch's network rows.
+typedef struct NphLearnWorld {
+    u32 *gate;
+    u32 *stats;
+    u32 lane_base; // first lane of the world in the batch
+    u32 groups;
+    u32 learn;     // the network starts its points
+    u32 measure;   // residuals measured (learn implies it)
+    u32 pad[2];
+} NphLearnWorld;
+
+enum { ACC_SUM_PLAIN = 0, ACC_SUM_LEARNED, ACC_COUNT_PLAIN, ACC_COUNT_LEARNED, ACC_MAX_PLAIN,
+       ACC_MAX_LEARNED, ACC_MAX_PEN, ACC_GATED, ACC_WORDS };
+
 typedef struct NphCollideWorld {
     NphContactRec *contacts;
     const NphShapeRec *shapes;
@@ -103,6 +118,17 @@ struct NphDeviceBatch {
     u64 cap_h_grown;
     u32 *h_wstart;
     u64 cap_h_wstart;
+    // The learned warm start over the batch: network rows, outputs and
+    // residuals of every world's lanes (its slice at its first lane), and
+    // the residual statistics summed over worlds and steps (doubles:
+    // residual sums, contact counts, maxima, gated contacts).
+    u16 *lx;
+    f32 *ly;
+    f32 *lres;
+    u32 cap_learn_lanes;
+    f64 *acc;
+    f64 *h_acc;
+    u32 acc_steps;
     // The substep loop as a graph, instantiated once and updated.
     hipGraphExec_t exec;
     // Timing.
@@ -355,6 +381,147 @@ __global__ void __launch_bounds__(256) k_b_finalize(Tab tab, u32 nw, Worlds ws, 
     }
 }
 
+// The learned warm start: the batch's arguments narrowed to world w.
+__device__ static inline NphLearnArgs world_learn(const NphLearnArgs &a, const NphLearnWorld &lw)
+{
+    NphLearnArgs r = a;
+    r.x = a.x + (u64)lw.lane_base * NPH_LEARNED_ROWS_PER_LANE * NPH_LEARNED_IN;
+    r.y = a.y + (u64)lw.lane_base * NPH_LEARNED_ROWS_PER_LANE * NPH_LEARNED_OUT;
+    r.res = a.res + lw.lane_base;
+    r.gate = lw.gate;
+    r.stats = lw.stats;
+    r.group_count = lw.groups;
+    return r;
+}
+
+typedef const NphLearnWorld *__restrict__ LearnWorlds;
+
+__global__ void __launch_bounds__(256) k_b_learned_features(Tab tab, u32 nw, Worlds ws,
+                                                            LearnWorlds lws, NphLearnArgs a)
+{
+    WAVE_ITEM(tab, nw)
+    const NphLearnWorld lw = lws[e.world];
+    if (lw.learn == 0U) {
+        return;
+    }
+    const NphStepData d = ws[e.world];
+    const NphLearnArgs aw = world_learn(a, lw);
+    learned_features_at(&d, &aw, i);
+}
+
+__global__ void __launch_bounds__(256) k_b_learned_apply(Tab tab, u32 nw, Worlds ws,
+                                                         LearnWorlds lws, NphLearnArgs a)
+{
+    WAVE_ITEM(tab, nw)
+    const NphLearnWorld lw = lws[e.world];
+    if (lw.learn == 0U) {
+        return;
+    }
+    const NphStepData d = ws[e.world];
+    const NphLearnArgs aw = world_learn(a, lw);
+    learned_apply_at(&d, &aw, i);
+}
+
+// A thread a world: the statistics words cleared, or the gate's threshold.
+__global__ void __launch_bounds__(64) k_b_learned_clear(LearnWorlds lws, u32 n)
+{
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && lws[k].measure != 0U) {
+        for (u32 i = 0U; i < WS_CLEARED; ++i) {
+            lws[k].stats[i] = 0U;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_b_learned_threshold(LearnWorlds lws, u32 n)
+{
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && lws[k].learn != 0U) {
+        learned_threshold(lws[k].stats);
+    }
+}
+
+// Every lane of a wave reaches the wave's reductions, in range or not.
+__global__ void __launch_bounds__(256) k_b_learned_residual(Tab tab, u32 nw, Worlds ws,
+                                                            LearnWorlds lws, NphLearnArgs a)
+{
+    const u32 wv = (u32)__builtin_amdgcn_readfirstlane(
+        (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (wv >= nw) {
+        return;
+    }
+    const NphWave e = tab[wv];
+    const NphLearnWorld lw = lws[e.world];
+    if (lw.measure == 0U) {
+        return;
+    }
+    const u32 i = e.base + (threadIdx.x & (NPH_WAVE - 1U));
+    const NphStepData d = ws[e.world];
+    const NphLearnArgs aw = world_learn(a, lw);
+    learned_residual_at(&d, &aw, i, i < e.end);
+}
+
+__global__ void __launch_bounds__(256) k_b_learned_gate(Tab tab, u32 nw, Worlds ws,
+                                                        LearnWorlds lws, NphLearnArgs a)
+{
+    WAVE_ITEM(tab, nw)
+    const NphLearnWorld lw = lws[e.world];
+    if (lw.learn == 0U) {
+        return;
+    }
+    const NphStepData d = ws[e.world];
+    const NphLearnArgs aw = world_learn(a, lw);
+    learned_gate_at(&d, &aw, i);
+}
+
+// The step's statistics of every measured world added to the batch's
+// running totals (one block; double sums, in world order per thread, then
+// over the block's threads in a fixed order).
+__global__ void __launch_bounds__(256) k_b_learned_accumulate(LearnWorlds lws, u32 n, f64 *acc)
+{
+    __shared__ f64 part[ACC_WORDS][256];
+    f64 v[ACC_WORDS];
+    for (u32 j = 0U; j < ACC_WORDS; ++j) {
+        v[j] = 0.0;
+    }
+    for (u32 k = threadIdx.x; k < n; k += blockDim.x) {
+        if (lws[k].measure == 0U) {
+            continue;
+        }
+        const u32 *st = lws[k].stats;
+        v[ACC_SUM_PLAIN] += (f64)__uint_as_float(st[WS_SUM_PLAIN]);
+        v[ACC_SUM_LEARNED] += (f64)__uint_as_float(st[WS_SUM_LEARNED]);
+        v[ACC_COUNT_PLAIN] += (f64)st[WS_COUNT_PLAIN];
+        v[ACC_COUNT_LEARNED] += (f64)st[WS_COUNT_LEARNED];
+        v[ACC_MAX_PLAIN] = fmax(v[ACC_MAX_PLAIN], (f64)__uint_as_float(st[WS_MAX_PLAIN]));
+        v[ACC_MAX_LEARNED] = fmax(v[ACC_MAX_LEARNED], (f64)__uint_as_float(st[WS_MAX_LEARNED]));
+        v[ACC_MAX_PEN] = fmax(v[ACC_MAX_PEN], (f64)__uint_as_float(st[WS_MAX_PEN]));
+        v[ACC_GATED] += (f64)st[WS_GATED];
+    }
+    for (u32 j = 0U; j < ACC_WORDS; ++j) {
+        part[j][threadIdx.x] = v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0U) {
+        for (u32 t = 1U; t < blockDim.x; ++t) {
+            for (u32 j = 0U; j < ACC_WORDS; ++j) {
+                if (j == ACC_MAX_PLAIN || j == ACC_MAX_LEARNED || j == ACC_MAX_PEN) {
+                    v[j] = fmax(v[j], part[j][t]);
+                } else {
+                    v[j] += part[j][t];
+                }
+            }
+        }
+        for (u32 j = 0U; j < ACC_WORDS; ++j) {
+            if (j == ACC_MAX_PLAIN || j == ACC_MAX_LEARNED || j == ACC_MAX_PEN) {
+                acc[j] = fmax(acc[j], v[j]);
+            } else {
+                acc[j] += v[j];
+            }
+        }
+    }
+}
+
 // Grown shapes: a count per awake body of the batch (world k's bodies at
 // aw_off[k]), an exclusive scan over all of them, then each body writes its
 // shapes at its place: every world's grown records in its awake order, the
@@ -548,6 +715,13 @@ NphResult nph_devb_create(NphDeviceBatch **out)
     b->tune.pass_block = 256U;
     b->tune.body_block = 256U;
     b->tune.use_graph = 0U;
+    if (hipMalloc((void **)&b->acc, ACC_WORDS * sizeof(f64)) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_acc, ACC_WORDS * sizeof(f64), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipMemsetAsync(b->acc, 0, ACC_WORDS * sizeof(f64), b->stream) != hipSuccess) {
+        nph_devb_destroy(b);
+        return NPH_RESULT_DEVICE_LOST;
+    }
     *out = b;
     return NPH_RESULT_OK;
 }
@@ -578,8 +752,14 @@ void nph_devb_destroy(NphDeviceBatch *b)
             (void)hipFree(bufs[i]);
         }
     }
-    void *pinned[] = {b->h_codes, b->h_grown, b->h_wstart};
-    for (u32 i = 0U; i < 3U; ++i) {
+    void *lbufs[] = {b->lx, b->ly, b->lres, b->acc};
+    for (u32 i = 0U; i < 4U; ++i) {
+        if (lbufs[i] != NULL) {
+            (void)hipFree(lbufs[i]);
+        }
+    }
+    void *pinned[] = {b->h_codes, b->h_grown, b->h_wstart, b->h_acc};
+    for (u32 i = 0U; i < 4U; ++i) {
         if (pinned[i] != NULL) {
             (void)hipHostFree(pinned[i]);
         }
@@ -628,6 +808,21 @@ void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[2])
 {
     out[0] = b->kernel_ms;
     out[1] = (f64)b->kernel_launches;
+}
+
+NphResult nph_devb_warm_stats(NphDeviceBatch *b, f64 out[8], u32 *steps, int reset)
+{
+    HIP_TRY(hipMemcpyAsync(b->h_acc, b->acc, ACC_WORDS * sizeof(f64), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    memcpy(out, b->h_acc, ACC_WORDS * sizeof(f64));
+    *steps = b->acc_steps;
+    if (reset) {
+        HIP_TRY(hipMemsetAsync(b->acc, 0, ACC_WORDS * sizeof(f64), b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        b->acc_steps = 0U;
+    }
+    return NPH_RESULT_OK;
 }
 
 NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n, f32 step_dt,
@@ -908,6 +1103,30 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
             return NPH_RESULT_INVALID_ARGUMENT; // one set of step numbers a batch
         }
     }
+    // The learned warm start: one network for the batch (every world that
+    // has one has the same), residuals wherever measured; no contact log.
+    const NphDevice *teacher = NULL;
+    int any_measure = 0;
+    for (u32 k = 0U; k < n; ++k) {
+        NphDevice *d = devs[k];
+        if (d->log_cap > 0U) {
+            return NPH_RESULT_UNSUPPORTED;
+        }
+        if (d->learn_on) {
+            if (teacher == NULL) {
+                teacher = d;
+            } else if (d->learn_hash != teacher->learn_hash) {
+                return NPH_RESULT_UNSUPPORTED;
+            }
+        }
+        if (d->learn_on || d->measure) {
+            any_measure = 1;
+            const NphResult r = devi_reserve_stats(d);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
+        }
+    }
     // Sizes and each world's offsets in the batch's arrays.
     u64 A = 0U, G = 0U, J = 0U, W[T_COUNT];
     memset(W, 0, sizeof W);
@@ -986,6 +1205,17 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
     if (r == NPH_RESULT_OK) {
         r = dev_reserve(&b->wstart, &b->cap_wstart, n + 1U, s);
     }
+    if (r == NPH_RESULT_OK && any_measure && lanes > b->cap_learn_lanes) {
+        const u32 c = devi_grow_cap(b->cap_learn_lanes, (u32)lanes);
+        const u64 rows = (u64)c * NPH_LEARNED_ROWS_PER_LANE;
+        if ((r = devi_grow_buffer((void **)&b->lx, 0, rows * NPH_LEARNED_IN * 2U, s, 0)) ==
+                NPH_RESULT_OK &&
+            (r = devi_grow_buffer((void **)&b->ly, 0, rows * NPH_LEARNED_OUT * 4U, s, 0)) ==
+                NPH_RESULT_OK &&
+            (r = devi_grow_buffer((void **)&b->lres, 0, (u64)c * 4U, s, 0)) == NPH_RESULT_OK) {
+            b->cap_learn_lanes = c;
+        }
+    }
     if (r != NPH_RESULT_OK) {
         return r;
     }
@@ -1005,6 +1235,7 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
     const u64 o_gr = stage_take(&at, G * sizeof(NphGroup));
     const u64 o_sj = stage_take(&at, J * 4U);
     const u64 o_tab = stage_take(&at, total_waves * sizeof(NphWave));
+    const u64 o_lw = stage_take(&at, (u64)n * sizeof(NphLearnWorld));
     r = stage_reserve(&b->solve, at, s);
     if (r != NPH_RESULT_OK) {
         return r;
@@ -1020,6 +1251,7 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
     u32 *lc = (u32 *)(hs + o_lc);
     NphGroup *gr = (NphGroup *)(hs + o_gr);
     u32 *sj = (u32 *)(hs + o_sj);
+    NphLearnWorld *lws = (NphLearnWorld *)(hs + o_lw);
     NphWave *tabs[T_COUNT];
     u32 tfill[T_COUNT];
     {
@@ -1096,6 +1328,14 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
         }
         sd->joint_count = j;
         wg[k] = g;
+        NphLearnWorld *lw = &lws[k];
+        memset(lw, 0, sizeof *lw);
+        lw->gate = d->gate;
+        lw->stats = d->wstats;
+        lw->lane_base = (u32)(g_off * NPH_LANES);
+        lw->groups = g;
+        lw->learn = d->learn_on ? 1U : 0U;
+        lw->measure = (d->learn_on || d->measure) ? 1U : 0U;
         tfill[T_SLOTS] = add_waves(tabs[T_SLOTS], tfill[T_SLOTS], k, x->body_slots);
         tfill[T_BODIES] = add_waves(tabs[T_BODIES], tfill[T_BODIES], k, x->awake_count + 1U);
         tfill[T_GROUPS] = add_waves(tabs[T_GROUPS], tfill[T_GROUPS], k, g);
@@ -1142,6 +1382,33 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
         hipLaunchKernelGGL(k_b_group_impact, GRID(T_GROUPS), dws);
     }
     HIP_TRY(hipGetLastError());
+    NphLearnArgs la;
+    memset(&la, 0, sizeof la);
+    const NphLearnWorld *dlw = (const NphLearnWorld *)(ds + o_lw);
+    if (any_measure) {
+        la.x = teacher != NULL ? b->lx : NULL;
+        la.y = b->ly;
+        la.res = b->lres;
+        la.inv_h = sp.inv_h;
+        if (teacher != NULL) {
+            la.mode = teacher->learn_mode;
+            la.gate_on = teacher->gate_on ? 1U : 0U;
+            memcpy(la.shift, teacher->shift, sizeof la.shift);
+            memcpy(la.scale, teacher->scale, sizeof la.scale);
+            memcpy(la.out_scale, teacher->out_scale, sizeof la.out_scale);
+        }
+    }
+    if (teacher != NULL && G > 0U) {
+        hipLaunchKernelGGL(k_b_learned_features, GRID(T_LANES), dws, dlw, la);
+        HIP_TRY(hipGetLastError());
+        r = nph_amd_mlp_forward(b->lx, (u32)(lanes * NPH_LEARNED_ROWS_PER_LANE),
+                                teacher->mlp_params, b->ly, (void *)s);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        hipLaunchKernelGGL(k_b_learned_apply, GRID(T_LANES), dws, dlw, la);
+        HIP_TRY(hipGetLastError());
+    }
     u32 timed = 0U;
     const int time_launches = b->time_kernels && !b->tune.use_graph;
     if (time_launches) {
@@ -1167,6 +1434,20 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
     HIP_TRY(hipEventRecord(b->ev[3], s));
     if (G > 0U) {
         hipLaunchKernelGGL(k_b_store, GRID(T_LANES), dws);
+    }
+    if (any_measure) {
+        hipLaunchKernelGGL(k_b_learned_clear, dim3(devi_blocks(n, 64U)), dim3(64), 0, s, dlw, n);
+        if (G > 0U) {
+            hipLaunchKernelGGL(k_b_learned_residual, GRID(T_LANES), dws, dlw, la);
+            if (teacher != NULL && la.gate_on != 0U) {
+                hipLaunchKernelGGL(k_b_learned_threshold, dim3(devi_blocks(n, 64U)), dim3(64), 0,
+                                   s, dlw, n);
+                hipLaunchKernelGGL(k_b_learned_gate, GRID(T_LANES), dws, dlw, la);
+            }
+        }
+        hipLaunchKernelGGL(k_b_learned_accumulate, dim3(1), dim3(256), 0, s, dlw, n, b->acc);
+        HIP_TRY(hipGetLastError());
+        b->acc_steps += 1U;
     }
     hipLaunchKernelGGL(k_b_finalize, GRID(T_BODIES), dws, sp);
     // Grown proxies: counts, one scan over the batch, placement.

@@ -833,148 +833,20 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_fast_pack(NphStepData d, const u3
 // The learned warm start
 // ---------------------------------------------------------------------------
 
-// Statistics words: maxima as the bits of non-negative floats (ordered as
-// u32), sums as floats, counts; the gate's threshold persists across steps.
-enum {
-    WS_MAX_PLAIN = 0,
-    WS_MAX_LEARNED,
-    WS_COUNT_PLAIN,
-    WS_COUNT_LEARNED,
-    WS_GATED,
-    WS_MAX_PEN,
-    WS_SUM_PLAIN,
-    WS_SUM_LEARNED,
-    WS_CLEARED, // the words above are cleared each step
-    WS_THRESHOLD = WS_CLEARED,
-    NPH_WSTATS = 16
-};
-
-typedef struct NphLearnArgs {
-    u16 *x;          // network rows (NULL: no inference)
-    f32 *raw;        // raw feature rows (NULL: no log)
-    const f32 *y;    // network outputs
-    f32 *res;        // per lane residual
-    u32 *gate;       // per contact record
-    u32 *stats;
-    f32 *log_x;
-    f32 *log_t;
-    u32 *log_count;
-    u32 log_cap;
-    u32 mode;
-    u32 gate_on;
-    u32 group_count;
-    f32 inv_h;
-    f32 shift[16];
-    f32 scale[16];
-    f32 out_scale[4];
-} NphLearnArgs;
-
-__device__ static inline u16 bf16_bits(f32 f)
-{
-    u32 u = __float_as_uint(f); // round to nearest even (finite values)
-    u += 0x7FFFU + ((u >> 16U) & 1U);
-    return (u16)(u >> 16U);
-}
-
-__device__ static inline int lane_active(const NphStepData *d, const NphLearnArgs *a, u32 L,
-                                         const NphContactRec **c)
-{
-    const u32 g = L / NPH_LANES;
-    if (g >= a->group_count || L % NPH_LANES >= d->groups[g].lane_count) {
-        return 0;
-    }
-    *c = &d->contacts[d->lane_contact[L]];
-    return 1;
-}
-
-// Each lane's four network rows (zero where there is no point) and, when
-// logging, its points' raw features and carried-nothing flags.
+// The learned warm start's kernels: each a thread a lane of the one world.
 __global__ void __launch_bounds__(NPH_BLOCK) k_learned_features(NphStepData d, NphLearnArgs a)
 {
     const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= a.group_count * NPH_LANES) {
-        return;
-    }
-    const NphContactRec *c = NULL;
-    const u32 points = lane_active(&d, &a, L, &c) ? c->point_count : 0U;
-    const u32 g = L / NPH_LANES;
-    const u32 l = L % NPH_LANES;
-    for (u32 k = 0U; k < NPH_LEARNED_ROWS_PER_LANE; ++k) {
-        const u64 row = (u64)L * NPH_LEARNED_ROWS_PER_LANE + k;
-        f32 f[NPH_F_COUNT];
-        f32 fresh = 0.0F;
-        f32 carried = 0.0F;
-        if (k < points) {
-            learned_features(&d, g, l, k, f);
-            const u64 R = (u64)(d.groups[g].first_point + k) * NPH_LANES + l;
-            carried = PF(&d, NPH_P_NIMP, R);
-            fresh = (carried == 0.0F && PF(&d, NPH_P_TI1, R) == 0.0F &&
-                     PF(&d, NPH_P_TI2, R) == 0.0F)
-                        ? 1.0F
-                        : 0.0F;
-        } else {
-            for (u32 i = 0U; i < NPH_F_COUNT; ++i) {
-                f[i] = 0.0F;
-            }
-        }
-        if (a.x != NULL) {
-            u16 *xr = a.x + row * NPH_LEARNED_IN;
-            for (u32 i = 0U; i < NPH_LEARNED_IN; ++i) {
-                xr[i] = (i < NPH_F_COUNT && k < points)
-                            ? bf16_bits((f[i] - a.shift[i]) * a.scale[i])
-                            : (u16)0U;
-            }
-        }
-        if (a.raw != NULL) {
-            f32 *fr = a.raw + row * NPH_LEARNED_IN;
-            for (u32 i = 0U; i < NPH_F_COUNT; ++i) {
-                fr[i] = f[i];
-            }
-            fr[14] = carried;
-            fr[15] = fresh;
-        }
+    if (L < a.group_count * NPH_LANES) {
+        learned_features_at(&d, &a, L);
     }
 }
 
-// The network's impulses into the chosen points of every lane the gate
-// leaves to it; such lanes are flagged.
 __global__ void __launch_bounds__(NPH_BLOCK) k_learned_apply(NphStepData d, NphLearnArgs a)
 {
     const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
-    const NphContactRec *c = NULL;
-    if (L >= a.group_count * NPH_LANES || !lane_active(&d, &a, L, &c)) {
-        return;
-    }
-    const u32 record = d.lane_contact[L];
-    if (a.gate_on != 0U && a.gate[record] != 0U) {
-        return;
-    }
-    // With every point taken by the network, one contact record in
-    // NPH_LEARNED_CONTROL stays plain: the sample the gate measures the
-    // plain solver's residual on.
-    if (a.mode == NPH_AMD_WARM_ALL_POINTS && record % NPH_LEARNED_CONTROL == 0U) {
-        return;
-    }
-    const u32 g = L / NPH_LANES;
-    const u32 l = L % NPH_LANES;
-    const f32 friction = LF(&d, NPH_L_FRICTION, L);
-    u32 any = 0U;
-    for (u32 k = 0U; k < c->point_count; ++k) {
-        const u64 R = (u64)(d.groups[g].first_point + k) * NPH_LANES + l;
-        const int fresh = PF(&d, NPH_P_NIMP, R) == 0.0F && PF(&d, NPH_P_TI1, R) == 0.0F &&
-                          PF(&d, NPH_P_TI2, R) == 0.0F;
-        if (a.mode == NPH_AMD_WARM_ALL_POINTS || fresh) {
-            const f32 *y = a.y + ((u64)L * NPH_LEARNED_ROWS_PER_LANE + k) * NPH_LEARNED_OUT;
-            f32 n, t1, t2;
-            learned_impulses(y, a.out_scale, PF(&d, NPH_P_NMASS, R), friction, &n, &t1, &t2);
-            PF(&d, NPH_P_NIMP, R) = n;
-            PF(&d, NPH_P_TI1, R) = t1;
-            PF(&d, NPH_P_TI2, R) = t2;
-            any = 1U;
-        }
-    }
-    if (any != 0U) {
-        d.lane_flags[L] |= NPH_LANE_LEARNED;
+    if (L < a.group_count * NPH_LANES) {
+        learned_apply_at(&d, &a, L);
     }
 }
 
@@ -985,79 +857,24 @@ __global__ void __launch_bounds__(64) k_learned_clear(u32 *stats)
     }
 }
 
-// Every lane's residual and penetration after the step, into the
-// statistics by how the lane started: reduced across the wave first, so
-// each wave adds to the shared words once.
-__device__ static inline f32 wave_max(f32 v)
-{
-    for (u32 o = 32U; o > 0U; o >>= 1U) {
-        v = nph_maxf(v, __shfl_xor(v, (int)o, 64));
-    }
-    return v;
-}
-
-__device__ static inline f32 wave_sum(f32 v)
-{
-    for (u32 o = 32U; o > 0U; o >>= 1U) {
-        v += __shfl_xor(v, (int)o, 64);
-    }
-    return v;
-}
-
 __global__ void __launch_bounds__(NPH_BLOCK) k_learned_residual(NphStepData d, NphLearnArgs a)
 {
     const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
-    const NphContactRec *c = NULL;
-    f32 res = 0.0F;
-    f32 pen = 0.0F;
-    int active = 0;
-    int learned = 0;
-    if (L < a.group_count * NPH_LANES && lane_active(&d, &a, L, &c)) {
-        learned_residual(&d, a.inv_h, L / NPH_LANES, L % NPH_LANES, c, &res, &pen);
-        a.res[L] = res;
-        active = 1;
-        learned = (d.lane_flags[L] & NPH_LANE_LEARNED) != 0U;
-    }
-    const f32 rp = active && !learned ? res : 0.0F;
-    const f32 rl = active && learned ? res : 0.0F;
-    const f32 max_p = wave_max(rp);
-    const f32 max_l = wave_max(rl);
-    const f32 sum_p = wave_sum(rp);
-    const f32 sum_l = wave_sum(rl);
-    const f32 max_pen = wave_max(active ? nph_maxf(pen, 0.0F) : 0.0F);
-    const u64 bp = __ballot(active && !learned);
-    const u64 bl = __ballot(active && learned);
-    if ((threadIdx.x & 63U) == 0U && (bp | bl) != 0U) {
-        atomicMax(&a.stats[WS_MAX_PLAIN], __float_as_uint(max_p));
-        atomicMax(&a.stats[WS_MAX_LEARNED], __float_as_uint(max_l));
-        atomicAdd(&a.stats[WS_COUNT_PLAIN], (u32)__popcll(bp));
-        atomicAdd(&a.stats[WS_COUNT_LEARNED], (u32)__popcll(bl));
-        atomicAdd((f32 *)&a.stats[WS_SUM_PLAIN], sum_p);
-        atomicAdd((f32 *)&a.stats[WS_SUM_LEARNED], sum_l);
-        atomicMax(&a.stats[WS_MAX_PEN], __float_as_uint(max_pen));
-    }
+    learned_residual_at(&d, &a, L, L < a.group_count * NPH_LANES);
 }
 
-// The gate's threshold: the step's largest plain residual, or the last
-// one while no contact started plainly.
 __global__ void __launch_bounds__(64) k_learned_threshold(u32 *stats)
 {
-    if (threadIdx.x == 0U && stats[WS_COUNT_PLAIN] > 0U) {
-        stats[WS_THRESHOLD] = stats[WS_MAX_PLAIN];
+    if (threadIdx.x == 0U) {
+        learned_threshold(stats);
     }
 }
 
 __global__ void __launch_bounds__(NPH_BLOCK) k_learned_gate(NphStepData d, NphLearnArgs a)
 {
     const u32 L = blockIdx.x * blockDim.x + threadIdx.x;
-    const NphContactRec *c = NULL;
-    if (L >= a.group_count * NPH_LANES || !lane_active(&d, &a, L, &c) ||
-        (d.lane_flags[L] & NPH_LANE_LEARNED) == 0U) {
-        return;
-    }
-    if (a.res[L] > __uint_as_float(a.stats[WS_THRESHOLD])) {
-        a.gate[d.lane_contact[L]] = 1U;
-        atomicAdd(&a.stats[WS_GATED], 1U);
+    if (L < a.group_count * NPH_LANES) {
+        learned_gate_at(&d, &a, L);
     }
 }
 
@@ -1595,7 +1412,7 @@ NphResult nph_dev_solve_finish(NphDevice *d, const NphGrown **grown, u32 *grown_
 // The learned warm start: configuration, statistics, the contact log
 // ---------------------------------------------------------------------------
 
-static NphResult reserve_stats(NphDevice *d)
+NphResult devi_reserve_stats(NphDevice *d)
 {
     if (d->wstats != NULL) {
         return NPH_RESULT_OK;
@@ -1621,7 +1438,7 @@ NphResult nph_dev_set_warm_start(NphDevice *d, const NphAmdWarmStart *ws)
     if (d->mlp_params == NULL) {
         HIP_TRY(hipMalloc((void **)&d->mlp_params, NPH_AMD_MLP_PARAMS * 4U));
     }
-    NphResult r = reserve_stats(d);
+    NphResult r = devi_reserve_stats(d);
     if (r != NPH_RESULT_OK) {
         return r;
     }
@@ -1634,6 +1451,18 @@ NphResult nph_dev_set_warm_start(NphDevice *d, const NphAmdWarmStart *ws)
     d->learn_mode = ws->mode;
     d->gate_on = ws->gate ? 1 : 0;
     d->learn_on = 1;
+    // Worlds stepped in one batch share one network: this names it.
+    u64 h = 0xcbf29ce484222325ULL;
+    const u8 *parts[5] = {(const u8 *)ws->params, (const u8 *)d->shift, (const u8 *)d->scale,
+                          (const u8 *)d->out_scale, (const u8 *)&d->learn_mode};
+    const u64 sizes[5] = {NPH_AMD_MLP_PARAMS * 4U, sizeof d->shift, sizeof d->scale,
+                          sizeof d->out_scale, sizeof d->learn_mode};
+    for (u32 k = 0U; k < 5U; ++k) {
+        for (u64 i = 0U; i < sizes[k]; ++i) {
+            h = (h ^ parts[k][i]) * 0x100000001b3ULL;
+        }
+    }
+    d->learn_hash = h ^ (u64)d->gate_on;
     return NPH_RESULT_OK;
 }
 
@@ -1643,6 +1472,8 @@ int nph_dev_learned_active(const NphDevice *d)
 {
     return d->learn_on || d->measure || d->log_cap > 0U;
 }
+
+int nph_dev_logging(const NphDevice *d) { return d->log_cap > 0U; }
 
 NphResult nph_dev_warm_stats(NphDevice *d, NphAmdWarmStats *out)
 {
@@ -1714,7 +1545,7 @@ NphResult nph_dev_contact_log(NphDevice *d, u32 *rows, const f32 **features, con
 // The per-lane buffers of the learned kernels for `lanes` lanes.
 static NphResult reserve_learn(NphDevice *d, u32 lanes)
 {
-    NphResult r = reserve_stats(d);
+    NphResult r = devi_reserve_stats(d);
     if (r != NPH_RESULT_OK || lanes <= d->cap_learn_lanes) {
         return r;
     }
@@ -1791,14 +1622,25 @@ static NphResult learned_before(NphDevice *d, const NphStepData *sd, u32 groups,
 // After the store: residuals, the gate, the log.
 static NphResult learned_after(NphDevice *d, const NphStepData *sd, u32 groups, f32 inv_h)
 {
-    if (groups == 0U || (!d->learn_on && d->log_cap == 0U && !d->measure)) {
+    if (!d->learn_on && d->log_cap == 0U && !d->measure) {
+        return NPH_RESULT_OK;
+    }
+    hipStream_t s = d->stream;
+    if (d->learn_on || d->measure) {
+        // The statistics are this step's, also when nothing touches.
+        const NphResult r = devi_reserve_stats(d);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        hipLaunchKernelGGL(k_learned_clear, dim3(1), dim3(64), 0, s, d->wstats);
+    }
+    if (groups == 0U) {
+        HIP_TRY(hipGetLastError());
         return NPH_RESULT_OK;
     }
     const u32 lanes = groups * NPH_LANES;
     const NphLearnArgs a = learn_args(d, groups, inv_h);
-    hipStream_t s = d->stream;
     if (d->learn_on || d->measure) {
-        hipLaunchKernelGGL(k_learned_clear, dim3(1), dim3(64), 0, s, d->wstats);
         hipLaunchKernelGGL(k_learned_residual, dim3(blocks_for(lanes)), dim3(NPH_BLOCK), 0, s, *sd,
                            a);
         if (d->learn_on && d->gate_on) {

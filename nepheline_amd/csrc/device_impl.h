@@ -143,6 +143,7 @@ struct NphDevice {
     u32 *log_count;
     u32 log_cap;
     u32 mesh_shapes;     // mesh shapes made: the mesh pairs' kernel runs
+    u64 learn_hash;      // FNV-1a of the network's parameters and settings
 };
 
 // Buffer growth: a new allocation of new_bytes, the old contents copied
@@ -151,6 +152,8 @@ NphResult devi_grow_buffer(void **p, u64 old_bytes, u64 new_bytes, hipStream_t s
 u32 devi_grow_cap(u32 cap, u32 need);
 // A pinned host buffer of at least `bytes`.
 NphResult devi_reserve_pinned(NphDevice *d, void **p, u64 *cap, u64 bytes);
+// A world's statistics words of the learned warm start, allocated once.
+NphResult devi_reserve_stats(NphDevice *d);
 // A world's device on `stream` (a stream of its own when NULL).
 NphResult devi_create(NphDevice **out, hipStream_t stream);
 // The host's staged record writes of a world, to the device (stream-ordered).
@@ -287,6 +290,229 @@ __device__ static inline void grown_write(const NphStepData *d, u32 i, NphGrown 
         }
     }
     b->enlarged = 0U;
+}
+
+// ---------------------------------------------------------------------------
+// The learned warm start, a lane at a time (device_hip.hip launches it for one
+// world, device_batch.hip for many).
+// ---------------------------------------------------------------------------
+// Statistics words: maxima as the bits of non-negative floats (ordered as
+// u32), sums as floats, counts; the gate's threshold persists across steps.
+enum {
+    WS_MAX_PLAIN = 0,
+    WS_MAX_LEARNED,
+    WS_COUNT_PLAIN,
+    WS_COUNT_LEARNED,
+    WS_GATED,
+    WS_MAX_PEN,
+    WS_SUM_PLAIN,
+    WS_SUM_LEARNED,
+    WS_CLEARED, // the words above are cleared each step
+    WS_THRESHOLD = WS_CLEARED,
+    NPH_WSTATS = 16
+};
+
+typedef struct NphLearnArgs {
+    u16 *x;          // network rows (NULL: no inference)
+    f32 *raw;        // raw feature rows (NULL: no log)
+    const f32 *y;    // network outputs
+    f32 *res;        // per lane residual
+    u32 *gate;       // per contact record
+    u32 *stats;
+    f32 *log_x;
+    f32 *log_t;
+    u32 *log_count;
+    u32 log_cap;
+    u32 mode;
+    u32 gate_on;
+    u32 group_count;
+    f32 inv_h;
+    f32 shift[16];
+    f32 scale[16];
+    f32 out_scale[4];
+} NphLearnArgs;
+
+__device__ static inline u16 bf16_bits(f32 f)
+{
+    u32 u = __float_as_uint(f); // round to nearest even (finite values)
+    u += 0x7FFFU + ((u >> 16U) & 1U);
+    return (u16)(u >> 16U);
+}
+
+__device__ static inline int lane_active(const NphStepData *d, const NphLearnArgs *a, u32 L,
+                                         const NphContactRec **c)
+{
+    const u32 g = L / NPH_LANES;
+    if (g >= a->group_count || L % NPH_LANES >= d->groups[g].lane_count) {
+        return 0;
+    }
+    *c = &d->contacts[d->lane_contact[L]];
+    return 1;
+}
+
+// Each lane's four network rows (zero where there is no point) and, when
+// logging, its points' raw features and carried-nothing flags.  L < the
+// world's groups * 8.
+__device__ static inline void learned_features_at(const NphStepData *d, const NphLearnArgs *a,
+                                                  u32 L)
+{
+    const NphContactRec *c = NULL;
+    const u32 points = lane_active(d, a, L, &c) ? c->point_count : 0U;
+    const u32 g = L / NPH_LANES;
+    const u32 l = L % NPH_LANES;
+    for (u32 k = 0U; k < NPH_LEARNED_ROWS_PER_LANE; ++k) {
+        const u64 row = (u64)L * NPH_LEARNED_ROWS_PER_LANE + k;
+        f32 f[NPH_F_COUNT];
+        f32 fresh = 0.0F;
+        f32 carried = 0.0F;
+        if (k < points) {
+            learned_features(d, g, l, k, f);
+            const u64 R = (u64)(d->groups[g].first_point + k) * NPH_LANES + l;
+            carried = PF(d, NPH_P_NIMP, R);
+            fresh = (carried == 0.0F && PF(d, NPH_P_TI1, R) == 0.0F &&
+                     PF(d, NPH_P_TI2, R) == 0.0F)
+                        ? 1.0F
+                        : 0.0F;
+        } else {
+            for (u32 i = 0U; i < NPH_F_COUNT; ++i) {
+                f[i] = 0.0F;
+            }
+        }
+        if (a->x != NULL) {
+            u16 *xr = a->x + row * NPH_LEARNED_IN;
+            for (u32 i = 0U; i < NPH_LEARNED_IN; ++i) {
+                xr[i] = (i < NPH_F_COUNT && k < points)
+                            ? bf16_bits((f[i] - a->shift[i]) * a->scale[i])
+                            : (u16)0U;
+            }
+        }
+        if (a->raw != NULL) {
+            f32 *fr = a->raw + row * NPH_LEARNED_IN;
+            for (u32 i = 0U; i < NPH_F_COUNT; ++i) {
+                fr[i] = f[i];
+            }
+            fr[14] = carried;
+            fr[15] = fresh;
+        }
+    }
+}
+
+// The network's impulses into the chosen points of a lane the gate leaves
+// to it; such lanes are flagged.
+__device__ static inline void learned_apply_at(const NphStepData *d, const NphLearnArgs *a, u32 L)
+{
+    const NphContactRec *c = NULL;
+    if (!lane_active(d, a, L, &c)) {
+        return;
+    }
+    const u32 record = d->lane_contact[L];
+    if (a->gate_on != 0U && a->gate[record] != 0U) {
+        return;
+    }
+    // With every point taken by the network, one contact record in
+    // NPH_LEARNED_CONTROL stays plain: the sample the gate measures the
+    // plain solver's residual on.
+    if (a->mode == NPH_AMD_WARM_ALL_POINTS && record % NPH_LEARNED_CONTROL == 0U) {
+        return;
+    }
+    const u32 g = L / NPH_LANES;
+    const u32 l = L % NPH_LANES;
+    const f32 friction = LF(d, NPH_L_FRICTION, L);
+    u32 any = 0U;
+    for (u32 k = 0U; k < c->point_count; ++k) {
+        const u64 R = (u64)(d->groups[g].first_point + k) * NPH_LANES + l;
+        const int fresh = PF(d, NPH_P_NIMP, R) == 0.0F && PF(d, NPH_P_TI1, R) == 0.0F &&
+                          PF(d, NPH_P_TI2, R) == 0.0F;
+        if (a->mode == NPH_AMD_WARM_ALL_POINTS || fresh) {
+            const f32 *y = a->y + ((u64)L * NPH_LEARNED_ROWS_PER_LANE + k) * NPH_LEARNED_OUT;
+            f32 n, t1, t2;
+            learned_impulses(y, a->out_scale, PF(d, NPH_P_NMASS, R), friction, &n, &t1, &t2);
+            PF(d, NPH_P_NIMP, R) = n;
+            PF(d, NPH_P_TI1, R) = t1;
+            PF(d, NPH_P_TI2, R) = t2;
+            any = 1U;
+        }
+    }
+    if (any != 0U) {
+        d->lane_flags[L] |= NPH_LANE_LEARNED;
+    }
+}
+
+// Every lane's residual and penetration after the step, into the
+// statistics by how the lane started: reduced across the wave first, so
+// each wave adds to the shared words once.
+__device__ static inline f32 wave_max(f32 v)
+{
+    for (u32 o = 32U; o > 0U; o >>= 1U) {
+        v = nph_maxf(v, __shfl_xor(v, (int)o, 64));
+    }
+    return v;
+}
+
+__device__ static inline f32 wave_sum(f32 v)
+{
+    for (u32 o = 32U; o > 0U; o >>= 1U) {
+        v += __shfl_xor(v, (int)o, 64);
+    }
+    return v;
+}
+
+// Every lane of the wave calls this (the statistics are reduced across the
+// wave, then each wave adds to the shared words once); `in_range` marks the
+// lanes that hold a lane of the world.
+__device__ static inline void learned_residual_at(const NphStepData *d, const NphLearnArgs *a,
+                                                  u32 L, int in_range)
+{
+    const NphContactRec *c = NULL;
+    f32 res = 0.0F;
+    f32 pen = 0.0F;
+    int active = 0;
+    int learned = 0;
+    if (in_range && lane_active(d, a, L, &c)) {
+        learned_residual(d, a->inv_h, L / NPH_LANES, L % NPH_LANES, c, &res, &pen);
+        a->res[L] = res;
+        active = 1;
+        learned = (d->lane_flags[L] & NPH_LANE_LEARNED) != 0U;
+    }
+    const f32 rp = active && !learned ? res : 0.0F;
+    const f32 rl = active && learned ? res : 0.0F;
+    const f32 max_p = wave_max(rp);
+    const f32 max_l = wave_max(rl);
+    const f32 sum_p = wave_sum(rp);
+    const f32 sum_l = wave_sum(rl);
+    const f32 max_pen = wave_max(active ? nph_maxf(pen, 0.0F) : 0.0F);
+    const u64 bp = __ballot(active && !learned);
+    const u64 bl = __ballot(active && learned);
+    if ((threadIdx.x & 63U) == 0U && (bp | bl) != 0U) {
+        atomicMax(&a->stats[WS_MAX_PLAIN], __float_as_uint(max_p));
+        atomicMax(&a->stats[WS_MAX_LEARNED], __float_as_uint(max_l));
+        atomicAdd(&a->stats[WS_COUNT_PLAIN], (u32)__popcll(bp));
+        atomicAdd(&a->stats[WS_COUNT_LEARNED], (u32)__popcll(bl));
+        atomicAdd((f32 *)&a->stats[WS_SUM_PLAIN], sum_p);
+        atomicAdd((f32 *)&a->stats[WS_SUM_LEARNED], sum_l);
+        atomicMax(&a->stats[WS_MAX_PEN], __float_as_uint(max_pen));
+    }
+}
+
+// The gate's threshold: the step's largest plain residual, or the last
+// one while no contact started plainly.
+__device__ static inline void learned_threshold(u32 *stats)
+{
+    if (stats[WS_COUNT_PLAIN] > 0U) {
+        stats[WS_THRESHOLD] = stats[WS_MAX_PLAIN];
+    }
+}
+
+__device__ static inline void learned_gate_at(const NphStepData *d, const NphLearnArgs *a, u32 L)
+{
+    const NphContactRec *c = NULL;
+    if (!lane_active(d, a, L, &c) || (d->lane_flags[L] & NPH_LANE_LEARNED) == 0U) {
+        return;
+    }
+    if (a->res[L] > __uint_as_float(a->stats[WS_THRESHOLD])) {
+        a->gate[d->lane_contact[L]] = 1U;
+        atomicAdd(&a->stats[WS_GATED], 1U);
+    }
 }
 
 #endif // NPH_DEVICE_IMPL_H

@@ -206,7 +206,7 @@ NphResult nph_amd_batch_step(NphAmdBatch *b, f32 dt)
     }
     const u32 n = b->count;
     for (u32 k = 0U; k < n; ++k) {
-        if (nph_dev_learned_active(b->devs[k])) {
+        if (nph_dev_logging(b->devs[k])) {
             return NPH_RESULT_UNSUPPORTED;
         }
     }
@@ -378,5 +378,44 @@ NphResult nph_amd_batch_autotune(NphAmdBatch *b, f32 dt, u32 rounds, NphAmdBatch
     if (chosen != NULL) {
         *chosen = cand[best];
     }
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_amd_batch_set_warm_start(NphAmdBatch *b, const NphAmdWarmStart *ws)
+{
+    for (u32 k = 0U; k < b->count; ++k) {
+        const NphResult r = nph_amd_world_set_warm_start(b->worlds[k], ws);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    return NPH_RESULT_OK;
+}
+
+void nph_amd_batch_measure_residual(NphAmdBatch *b, bool enable)
+{
+    for (u32 k = 0U; k < b->count; ++k) {
+        nph_amd_world_measure_residual(b->worlds[k], enable);
+    }
+}
+
+NphResult nph_amd_batch_warm_stats(NphAmdBatch *b, NphAmdBatchWarmStats *out, bool reset)
+{
+    f64 v[8];
+    u32 steps = 0U;
+    const NphResult r = nph_devb_warm_stats(b->dev, v, &steps, reset ? 1 : 0);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    out->residual_sum_plain = v[0];
+    out->residual_sum_learned = v[1];
+    out->contacts_plain = v[2];
+    out->contacts_learned = v[3];
+    out->residual_max_plain = v[4];
+    out->residual_max_learned = v[5];
+    out->penetration_max = v[6];
+    out->gated = v[7];
+    out->steps = steps;
+    out->pad = 0U;
     return NPH_RESULT_OK;
 }

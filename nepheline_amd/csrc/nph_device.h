@@ -126,6 +126,8 @@ NphResult nph_dev_contact_log(NphDevice *d, u32 *rows, const f32 **features,
 
 // A learned warm start, residual measurement or contact log is on.
 int nph_dev_learned_active(const NphDevice *d);
+// The contact log is on.
+int nph_dev_logging(const NphDevice *d);
 
 // ---------------------------------------------------------------------------
 // Many worlds stepped together: each world keeps its own records (its
@@ -170,12 +172,18 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
 // The solver stages and finalize for every world (all worlds share one
 // NphStepParams: in[0]'s); the grown proxies of world k are
 // (*grown)[(*starts)[k] .. (*starts)[k + 1]), in that world's awake order.
-// Continuous collision and the learned warm start are not batched.
+// Continuous collision and the contact log are not batched; worlds with a
+// learned warm start must share one network.
 NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSolveInput *in,
                          u32 n, const NphGrown **grown, const u32 **starts);
 // Device milliseconds of the last batched solve: out[0] the whole solve,
 // out[1] the constraint passes, out[2] the narrow phase (both halves).
 void nph_devb_times(const NphDeviceBatch *b, f64 out[3]);
+// The learned warm start's statistics summed over the measured worlds and
+// steps since the last reset: out = residual sums (plain, learned), contact
+// counts (plain, learned), largest residuals (plain, learned), largest
+// penetration, gated contacts; *steps = steps measured.
+NphResult nph_devb_warm_stats(NphDeviceBatch *b, f64 out[8], u32 *steps, int reset);
 // Per-launch timing of the batched colour passes (not in graph mode):
 // out[0] = milliseconds summed, out[1] = launches.
 void nph_devb_time_kernels(NphDeviceBatch *b, int enable);

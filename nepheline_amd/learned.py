@@ -242,6 +242,51 @@ def warm_stats(world: P.World) -> WarmStats:
     return s
 
 
+class BatchWarmStats(C.Structure):
+    """nph_amd_batch_warm_stats: sums and maxima over worlds and steps."""
+    _fields_ = [("residual_sum_plain", C.c_double), ("residual_sum_learned", C.c_double),
+                ("contacts_plain", C.c_double), ("contacts_learned", C.c_double),
+                ("residual_max_plain", C.c_double), ("residual_max_learned", C.c_double),
+                ("penetration_max", C.c_double), ("gated", C.c_double),
+                ("steps", _U32), ("pad", _U32)]
+
+
+def _bind_batch(a: P.Api):
+    if hasattr(a, "batch_set_warm_start"):
+        return
+    L = a.lib
+    a.batch_set_warm_start = P._bind(L, "nph_amd_batch_set_warm_start", C.c_int,
+                                     [C.c_void_p, C.POINTER(WarmStart)])
+    a.batch_measure_residual = P._bind(L, "nph_amd_batch_measure_residual", None,
+                                       [C.c_void_p, C.c_bool])
+    a.batch_warm_stats = P._bind(L, "nph_amd_batch_warm_stats", C.c_int,
+                                 [C.c_void_p, C.POINTER(BatchWarmStats), C.c_bool])
+
+
+def batch_set_warm_start(batch: P.Batch, ws: WarmStart | None, params: np.ndarray | None = None):
+    _bind_batch(batch.api)
+    if ws is None:
+        _check(batch.api.batch_set_warm_start(batch.ptr, None), "batch warm start")
+        return
+    if params is not None:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        ws.params = params.ctypes.data_as(_FP)
+        batch._keep = getattr(batch, "_keep", []) + [params]
+    _check(batch.api.batch_set_warm_start(batch.ptr, C.byref(ws)), "batch warm start")
+
+
+def batch_measure_residual(batch: P.Batch, enable=True):
+    _bind_batch(batch.api)
+    batch.api.batch_measure_residual(batch.ptr, enable)
+
+
+def batch_warm_stats(batch: P.Batch, reset=False) -> BatchWarmStats:
+    _bind_batch(batch.api)
+    s = BatchWarmStats()
+    _check(batch.api.batch_warm_stats(batch.ptr, C.byref(s), reset), "batch warm stats")
+    return s
+
+
 def log_contacts(world: P.World, capacity: int):
     _check(_bind(world.api).world_log_contacts(world.ptr, capacity), "log contacts")
 

@@ -125,13 +125,74 @@ def test_a_batch_world_steps_alone_too(gpu_api, ref_api, hull_bank):
     batch.close()
 
 
-def test_the_learned_warm_start_is_not_batched(gpu_api, hull_bank):
+def _network(seed):
+    """A small fixed network (random weights, unit normalisation)."""
+    import numpy as np
+    from nepheline_amd import learned as L
+    rng = np.random.default_rng(seed)
+    params = (rng.standard_normal(L.PARAMS) * 0.05).astype(np.float32)
+    ws = L.WarmStart()
+    for i in range(16):
+        ws.feature_shift[i] = 0.0
+        ws.feature_scale[i] = 1.0
+    for i in range(4):
+        ws.output_scale[i] = 0.05
+    ws.mode = L.NEW_POINTS
+    ws.gate = True
+    return ws, params
+
+
+def test_learned_batch_matches_each_world_alone(gpu_api, hull_bank):
+    """One network for the batch: every world's hash after every step equals
+    the same world stepped alone with that network (the network's rows are
+    independent of one another), and the batch's residual statistics add up
+    the worlds' own."""
+    from nepheline_amd import learned as L
+    print("batch: learned warm start, 5 worlds, 80 steps against the same worlds alone",
+          flush=True)
+    ws, params = _network(5)
+    sizes = [(60, 51), (200, 52), (20, 53), (400, 54), (5, 55)]
+    batch = P.Batch(gpu_api, len(sizes), S.DESC)
+    alone = []
+    for (n, seed), w in zip(sizes, batch.worlds):
+        S.bin_drop(None, n, seed, hull_bank, half=3.0 + n / 100.0, height=8.0, y_lo=1.0, world=w)
+        a = S.bin_drop(gpu_api, n, seed, hull_bank, half=3.0 + n / 100.0, height=8.0, y_lo=1.0)
+        L.set_warm_start(a, ws, params)
+        L.measure_residual(a, True)
+        alone.append(a)
+    L.batch_set_warm_start(batch, ws, params)
+    L.batch_measure_residual(batch, True)
+    L.batch_warm_stats(batch, reset=True)
+    learned = 0
+    for step in range(80):
+        batch.step(S.DT)
+        tot_p = tot_l = 0
+        for k, (w, a) in enumerate(zip(batch.worlds, alone)):
+            a.step(S.DT)
+            assert w.hash() == a.hash(), f"world {k}: hash differs at step {step + 1}"
+            st = L.warm_stats(a)
+            tot_p += st.contacts_plain
+            tot_l += st.contacts_learned
+        learned += tot_l
+        bs = L.batch_warm_stats(batch, reset=True)
+        assert (bs.contacts_plain, bs.contacts_learned) == (tot_p, tot_l), step
+    assert learned > 100
+    # A second network on one world: the batch refuses to mix them.
+    ws2, params2 = _network(6)
+    L.set_warm_start(batch.worlds[1], ws2, params2)
+    assert gpu_api.batch_step(batch.ptr, S.DT) == P.UNSUPPORTED
+    for a in alone:
+        a.close()
+    batch.close()
+
+
+def test_the_contact_log_is_not_batched(gpu_api, hull_bank):
     from nepheline_amd import learned as L
     batch = P.Batch(gpu_api, 2, S.DESC)
     for w in batch.worlds:
         S.bin_drop(None, 30, 3, hull_bank, half=3.0, world=w)
-    L.measure_residual(batch.worlds[1], True)
+    L.log_contacts(batch.worlds[1], 1000)
     assert gpu_api.batch_step(batch.ptr, S.DT) == P.UNSUPPORTED
-    L.measure_residual(batch.worlds[1], False)
+    L.log_contacts(batch.worlds[1], 0)
     batch.step(S.DT)
     batch.close()
