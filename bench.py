@@ -53,6 +53,7 @@ C_BYTES, K_BYTES, M_BYTES, V_BYTES, X_BYTES = 68, 116, 16, 24, 28
 # HBM bytes per contact-substep of the colour passes, from rocprofv3 counters
 # (FETCH_SIZE and WRITE_SIZE, scaled by a device copy of known size).
 TRAFFIC_FILE = "k_color_pass_bin10k_traffic.json"
+BATCH_TRAFFIC_FILE = "k_b_color_pass_batch1024_traffic.json"
 
 
 def contact_bytes(points, endpoints):
@@ -203,6 +204,7 @@ def run_learned(args, bank):
     loss1 = tr.loss()
     rows = tr.rows
     ws, params = tr.warm_start(mode=L.NEW_POINTS, gate=True)
+    args.network = (ws, params)  # the batched worlds take the same network
     tr.close()
     train_s = time.perf_counter() - t0
     # Throughput with the network, same scene and steps as the headline.
@@ -317,29 +319,43 @@ def run_field(args, bank):
     return out
 
 
-def run_batch(args, bank):
-    """The batched workload: 1 024 worlds (seed 4), each a pyramid, a bin
-    drop or ragdoll piles of floor(16 * 1250^u) bodies, u uniform, stepped
-    together (nph_amd_batch_step) after the autotuner picks the launch
-    choices; contact solves/s over the timed steps, the batched colour
-    pass's roofline, and the reference's CPU step on a sample of the
-    worlds."""
+def _batch_residuals(st):
+    """Mean and largest post-solve residual and largest penetration over the
+    measured worlds and steps."""
+    n = st.contacts_plain + st.contacts_learned
+    return {"mean": (st.residual_sum_plain + st.residual_sum_learned) / n if n else 0.0,
+            "mean_plain_started": st.residual_sum_plain / st.contacts_plain
+            if st.contacts_plain else None,
+            "mean_learned_started": st.residual_sum_learned / st.contacts_learned
+            if st.contacts_learned else None,
+            "max": max(st.residual_max_plain, st.residual_max_learned),
+            "penetration_max": st.penetration_max, "contact_steps": n,
+            "learned_fraction": st.contacts_learned / n if n else 0.0,
+            "gated": st.gated, "steps": st.steps}
+
+
+def _batch_run(args, bank, plan, network=None, roofline=False):
+    """One batch of the plan: built, autotuned, warmed up, then the timed
+    steps with residual measurement on (the same for both starts)."""
+    from nepheline_amd import learned as L
     a = P.product_api()
-    plan = S.batch_plan(args.batch_seed, args.batch_worlds)
     t0 = time.perf_counter()
     batch = P.Batch(a, len(plan), S.DESC)
     for (kind, n, seed), w in zip(plan, batch.worlds):
         S.fill_batch_world(w, kind, n, seed, bank)
     setup = time.perf_counter() - t0
     bodies = sum(len(w.bodies) for w in batch.worlds)
-    print(f"# batch: {len(plan)} worlds, {bodies} bodies built in {setup:.1f} s",
-          file=sys.stderr, flush=True)
+    print(f"# batch: {len(plan)} worlds, {bodies} bodies built in {setup:.1f} s"
+          + (" (learned start)" if network else ""), file=sys.stderr, flush=True)
+    if network is not None:
+        L.batch_set_warm_start(batch, network[0], network[1])
     rounds = 1
     tune = batch.autotune(DT, rounds=rounds)
-    tuned_steps = 12 * rounds
     for _ in range(args.batch_warmup):
         batch.step(DT)
-    warm = tuned_steps + args.batch_warmup
+    warm = 12 * rounds + args.batch_warmup
+    L.batch_measure_residual(batch, True)
+    L.batch_warm_stats(batch, reset=True)
     solved = 0
     phases = np.zeros(8)
     t0 = time.perf_counter()
@@ -352,50 +368,79 @@ def run_batch(args, bank):
         if (i + 1) % 25 == 0:
             print(f"# batch step {i + 1}/{args.batch_steps}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
-    # Roofline window: the tuned blocks, launched from the stream so each
-    # colour pass is bracketed by HIP events.
-    batch.set_tuning(tune.pass_block, tune.body_block, 0)
-    a.batch_time_kernels(batch.ptr, True)
-    k_solved = 0
-    for _ in range(args.batch_roofline_steps):
-        batch.step(DT)
-        k_solved += batch.stats().solved_contacts
-    kt = (C.c_double * 2)()
-    a.batch_kernel_times(batch.ptr, kt)
-    a.batch_time_kernels(batch.ptr, False)
-    tot_b = 0.0
-    tot_n = 0
-    for (kind, _, _), w in zip(plan, batch.worlds):
-        b, n = graph_bytes(w, BATCH_STATIC[kind])
-        tot_b += b
-        tot_n += n
-    batch.close()
-    mean_bytes = tot_b / tot_n if tot_n else 0.0
-    launches = max(int(kt[1]), 1)
-    bytes_per_launch = k_solved * SUBSTEPS * mean_bytes / launches
-    avg_launch_s = kt[0] * 1e-3 / launches
-    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    res = _batch_residuals(L.batch_warm_stats(batch, reset=True))
+    L.batch_measure_residual(batch, False)
     names = ("step", "host_pairs", "host_changes", "solve", "host_post", "dev_solve",
              "dev_passes", "dev_collide")
-    out = {"workload": "batch: %d worlds (seed %d; pyramids, bin drops, ragdoll piles of "
-                       "floor(16*1250^u) bodies), %d bodies, dt 1/60, 4 substeps, plain solver"
-                       % (len(plan), args.batch_seed, bodies),
-           "value": solved * SUBSTEPS / wall, "unit": "contact-substeps/s",
+    out = {"value": solved * SUBSTEPS / wall, "unit": "contact-substeps/s",
            "steps": args.batch_steps, "warmup": warm, "ms_per_step": wall * 1e3 / args.batch_steps,
            "setup_s": setup, "bodies": bodies, "worlds": len(plan),
            "tuning": {"pass_block": tune.pass_block, "body_block": tune.body_block,
                       "use_graph": bool(tune.use_graph)},
            "phases_ms_per_step": {k: round(v / max(args.batch_steps, 1), 3)
                                   for k, v in zip(names, phases)},
-           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                        "kernel": "k_b_color_pass (one colour of one constraint pass over every "
-                                  "world), HIP events around each launch over %d steps after "
-                                  "the timed region" % args.batch_roofline_steps,
-                        "launches": int(kt[1]), "avg_launch_us": avg_launch_s * 1e6,
-                        "bytes_per_launch": bytes_per_launch,
-                        "bytes_per_contact_substep": mean_bytes},
-           "cpu_baseline": None}
+           "residual": res}
+    if roofline:
+        # The tuned blocks, launched from the stream so each colour pass is
+        # bracketed by HIP events.
+        batch.set_tuning(tune.pass_block, tune.body_block, 0)
+        a.batch_time_kernels(batch.ptr, True)
+        k_solved = 0
+        for _ in range(args.batch_roofline_steps):
+            batch.step(DT)
+            k_solved += batch.stats().solved_contacts
+        kt = (C.c_double * 2)()
+        a.batch_kernel_times(batch.ptr, kt)
+        a.batch_time_kernels(batch.ptr, False)
+        tot_b = 0.0
+        tot_n = 0
+        for (kind, _, _), w in zip(plan, batch.worlds):
+            b, n = graph_bytes(w, BATCH_STATIC[kind])
+            tot_b += b
+            tot_n += n
+        mean_bytes = tot_b / tot_n if tot_n else 0.0
+        launches = max(int(kt[1]), 1)
+        bytes_per_launch = k_solved * SUBSTEPS * mean_bytes / launches
+        avg_launch_s = kt[0] * 1e-3 / launches
+        achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", BATCH_TRAFFIC_FILE)
+        if os.path.exists(tpath):
+            t = json.load(open(tpath))
+            traffic = t["hbm_bytes_per_contact_substep"] * k_solved * SUBSTEPS / launches
+        out["roofline"] = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "k_b_color_pass (one colour of one constraint pass over every world), "
+                      "HIP events around each launch over %d steps after the timed region"
+                      % args.batch_roofline_steps,
+            "launches": int(kt[1]), "avg_launch_us": avg_launch_s * 1e6,
+            "bytes_per_launch": bytes_per_launch, "bytes_per_contact_substep": mean_bytes}
+    batch.close()
+    return out, warm
+
+
+def run_batch(args, bank):
+    """The batched workload: 1 024 worlds (seed 4), each a pyramid, a bin
+    drop or ragdoll piles of floor(16 * 1250^u) bodies, u uniform, stepped
+    together (nph_amd_batch_step) after the autotuner picks the launch
+    choices: contact solves/s over the timed steps with the plain start and
+    with the learned start (the network the learned leg trained), the
+    residuals of both, the batched colour pass's roofline, and the
+    reference's CPU step on a sample of the worlds."""
+    plan = S.batch_plan(args.batch_seed, args.batch_worlds)
+    out, warm = _batch_run(args, bank, plan, roofline=True)
+    out = dict({"workload": "batch: %d worlds (seed %d; pyramids, bin drops, ragdoll piles of "
+                            "floor(16*1250^u) bodies), %d bodies, dt 1/60, 4 substeps, plain "
+                            "start; residual measured in the timed steps"
+                            % (len(plan), args.batch_seed, out["bodies"])}, **out)
+    net = getattr(args, "network", None)
+    if net is not None:
+        lo, _ = _batch_run(args, bank, plan, network=net)
+        lo["vs_plain_batch"] = lo["value"] / out["value"]
+        lo["mode"] = "points that carry no impulse start from the network; residual gate on"
+        out["learned"] = lo
+    out["cpu_baseline"] = None
     if not args.no_cpu:
         cpu = run_batch_cpu(args, bank, plan, warm)
         if cpu is not None:
@@ -562,6 +607,9 @@ def main():
     learned = None if args.no_learned else run_learned(args, bank)
     field = None if args.no_field else run_field(args, bank)
     batched = None if args.no_batch else run_batch(args, bank)
+    if batched is not None and "learned" in batched:
+        # The batched learned start against the plain one-world solver above.
+        batched["learned"]["vs_plain_single_world"] = batched["learned"]["value"] / value
     cpu = None if args.no_cpu else run_cpu(args, bank)
     line = {
         "metric": "contact solves/sec (contacts x substeps/s) plain vs learned warm start, "

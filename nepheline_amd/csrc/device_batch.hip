@@ -81,6 +81,8 @@ typedef struct Half {
     u32 list_count;
 } Half;
 
+typedef struct NphSortedEntry NphSortedEntry;
+
 enum {
     T_SLOTS = 0,  // body slots (the solver map)
     T_BODIES,     // awake bodies + the dummy
@@ -98,6 +100,11 @@ struct NphDeviceBatch {
     Half half[2];
     u32 *h_codes; // both halves' codes, read back
     u64 cap_h_codes;
+    // The narrow phase's list sorted by pair type (shared by the halves,
+    // which run one after the other), and the key counters.
+    NphSortedEntry *sorted;
+    u32 cap_sorted;
+    u32 *keys; // counts [0, 64), cursors [64, 128)
     Stage solve;
     // Per-step device arrays of the solve, the worlds' slices.
     NphSolverBody *sbodies; // awake bodies + one dummy a world
@@ -171,34 +178,147 @@ __global__ void __launch_bounds__(256) k_b_init(Tab tab, u32 nw,
     init_contact_record(c.contacts, c.gate, &in);
 }
 
-__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide(Tab tab, u32 nw,
-                                                  const NphCollideWorld *__restrict__ cw,
-                                                  const u32 *__restrict__ list, u32 *codes,
-                                                  f32 step_dt)
+// The narrow phase runs over the list sorted by the pair's shape types, so
+// a wave runs one collide routine instead of every routine its mixed pairs
+// would call: a counting sort over NPH_PAIR_KEYS keys (per-block counts in
+// LDS, one global atomic a key a block).  The order inside a key is the
+// order the blocks reserve their ranges in; each contact's manifold depends
+// only on its own pair, and its change code goes back to its list index.
+#define NPH_PAIR_KEYS 64U
+
+typedef struct NphSortedEntry {
+    u32 world;
+    u32 record;
+    u32 index; // the entry's place in the half's list (its change code's)
+    u32 pad;
+} NphSortedEntry;
+
+__device__ static inline u32 pair_key_of(const NphCollideWorld &c, u32 rec)
 {
-    WAVE_ITEM(tab, nw)
-    const NphCollideWorld c = cw[e.world];
-    const u32 rec = list[i];
-    if (c.contacts[rec].child != NPH_NULL32) {
-        return; // a mesh pair: k_b_collide_mesh
-    }
-    contact_update(c.contacts, rec, c.shapes, c.bodies, step_dt);
-    codes[i] = c.contacts[rec].change;
+    const NphContactRec *cr = &c.contacts[rec];
+    const u32 ta = nph_minu(c.shapes[cr->shape_a].geom.type, 7U);
+    const u32 tb = nph_minu(c.shapes[cr->shape_b].geom.type, 7U);
+    return ta < tb ? ta * 8U + tb : tb * 8U + ta;
 }
 
-__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide_mesh(Tab tab, u32 nw,
-                                                       const NphCollideWorld *__restrict__ cw,
-                                                       const u32 *__restrict__ list, u32 *codes,
-                                                       f32 step_dt)
+__global__ void __launch_bounds__(64) k_b_keys_clear(u32 *count)
 {
-    WAVE_ITEM(tab, nw)
-    const NphCollideWorld c = cw[e.world];
-    const u32 rec = list[i];
-    if (c.contacts[rec].child == NPH_NULL32) {
+    if (threadIdx.x < NPH_PAIR_KEYS) {
+        count[threadIdx.x] = 0U;
+    }
+}
+
+// Entry i of the list (batch-wide index) in this thread, or none.
+__device__ static inline int list_item(Tab tab, u32 nw, u32 *world, u32 *i)
+{
+    const u32 wv = (u32)__builtin_amdgcn_readfirstlane(
+        (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (wv >= nw) {
+        return 0;
+    }
+    const NphWave e = tab[wv];
+    *world = e.world;
+    *i = e.base + (threadIdx.x & (NPH_WAVE - 1U));
+    return *i < e.end;
+}
+
+__global__ void __launch_bounds__(256) k_b_key_count(Tab tab, u32 nw,
+                                                     const NphCollideWorld *__restrict__ cw,
+                                                     const u32 *__restrict__ list, u32 *count)
+{
+    __shared__ u32 c[NPH_PAIR_KEYS];
+    if (threadIdx.x < NPH_PAIR_KEYS) {
+        c[threadIdx.x] = 0U;
+    }
+    __syncthreads();
+    u32 w = 0U, i = 0U;
+    if (list_item(tab, nw, &w, &i)) {
+        atomicAdd(&c[pair_key_of(cw[w], list[i])], 1U);
+    }
+    __syncthreads();
+    if (threadIdx.x < NPH_PAIR_KEYS && c[threadIdx.x] != 0U) {
+        atomicAdd(&count[threadIdx.x], c[threadIdx.x]);
+    }
+}
+
+// Each key's first place: the counts of the keys before it.
+__global__ void __launch_bounds__(64) k_b_key_scan(const u32 *count, u32 *cursor)
+{
+    if (threadIdx.x == 0U) {
+        u32 base = 0U;
+        for (u32 k = 0U; k < NPH_PAIR_KEYS; ++k) {
+            cursor[k] = base;
+            base += count[k];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_b_key_scatter(Tab tab, u32 nw,
+                                                       const NphCollideWorld *__restrict__ cw,
+                                                       const u32 *__restrict__ list, u32 *cursor,
+                                                       NphSortedEntry *out)
+{
+    __shared__ u32 c[NPH_PAIR_KEYS];
+    __shared__ u32 base[NPH_PAIR_KEYS];
+    if (threadIdx.x < NPH_PAIR_KEYS) {
+        c[threadIdx.x] = 0U;
+    }
+    __syncthreads();
+    u32 w = 0U, i = 0U, key = 0U, rank = 0U;
+    const int mine = list_item(tab, nw, &w, &i);
+    if (mine) {
+        key = pair_key_of(cw[w], list[i]);
+        rank = atomicAdd(&c[key], 1U);
+    }
+    __syncthreads();
+    if (threadIdx.x < NPH_PAIR_KEYS && c[threadIdx.x] != 0U) {
+        base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], c[threadIdx.x]);
+    }
+    __syncthreads();
+    if (mine) {
+        NphSortedEntry e;
+        e.world = w;
+        e.record = list[i];
+        e.index = i;
+        e.pad = 0U;
+        out[base[key] + rank] = e;
+    }
+}
+
+// The sorted list, a contact a thread: convex pairs, or (mesh) the mesh
+// pairs, whose frames hold the triangles' clusters.
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide(const NphSortedEntry *__restrict__ sorted, u32 n,
+                                                  const NphCollideWorld *__restrict__ cw,
+                                                  u32 *codes, f32 step_dt)
+{
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) {
         return;
     }
-    contact_update_mesh(c.contacts, rec, c.shapes, c.bodies, step_dt);
-    codes[i] = c.contacts[rec].change;
+    const NphSortedEntry e = sorted[j];
+    const NphCollideWorld c = cw[e.world];
+    if (c.contacts[e.record].child != NPH_NULL32) {
+        return; // a mesh pair: k_b_collide_mesh
+    }
+    contact_update(c.contacts, e.record, c.shapes, c.bodies, step_dt);
+    codes[e.index] = c.contacts[e.record].change;
+}
+
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide_mesh(const NphSortedEntry *__restrict__ sorted, u32 n,
+                                                       const NphCollideWorld *__restrict__ cw,
+                                                       u32 *codes, f32 step_dt)
+{
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) {
+        return;
+    }
+    const NphSortedEntry e = sorted[j];
+    const NphCollideWorld c = cw[e.world];
+    if (c.contacts[e.record].child == NPH_NULL32) {
+        return;
+    }
+    contact_update_mesh(c.contacts, e.record, c.shapes, c.bodies, step_dt);
+    codes[e.index] = c.contacts[e.record].change;
 }
 
 __global__ void __launch_bounds__(256) k_b_map_fill(Tab tab, u32 nw, Worlds ws)
@@ -715,7 +835,8 @@ NphResult nph_devb_create(NphDeviceBatch **out)
     b->tune.pass_block = 256U;
     b->tune.body_block = 256U;
     b->tune.use_graph = 0U;
-    if (hipMalloc((void **)&b->acc, ACC_WORDS * sizeof(f64)) != hipSuccess ||
+    if (hipMalloc((void **)&b->keys, 2U * NPH_PAIR_KEYS * 4U) != hipSuccess ||
+        hipMalloc((void **)&b->acc, ACC_WORDS * sizeof(f64)) != hipSuccess ||
         hipHostMalloc((void **)&b->h_acc, ACC_WORDS * sizeof(f64), hipHostMallocDefault) !=
             hipSuccess ||
         hipMemsetAsync(b->acc, 0, ACC_WORDS * sizeof(f64), b->stream) != hipSuccess) {
@@ -752,8 +873,8 @@ void nph_devb_destroy(NphDeviceBatch *b)
             (void)hipFree(bufs[i]);
         }
     }
-    void *lbufs[] = {b->lx, b->ly, b->lres, b->acc};
-    for (u32 i = 0U; i < 4U; ++i) {
+    void *lbufs[] = {b->lx, b->ly, b->lres, b->acc, b->sorted, b->keys};
+    for (u32 i = 0U; i < 6U; ++i) {
         if (lbufs[i] != NULL) {
             (void)hipFree(lbufs[i]);
         }
@@ -866,6 +987,9 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
     if (r == NPH_RESULT_OK) {
         r = dev_reserve(&h->codes, &h->cap_codes, list > 0U ? list : 1U, s);
     }
+    if (r == NPH_RESULT_OK) {
+        r = dev_reserve(&b->sorted, &b->cap_sorted, list > 0U ? list : 1U, s);
+    }
     if (r != NPH_RESULT_OK) {
         return r;
     }
@@ -920,15 +1044,23 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
                            (const NphContactInit *)(ds + o_init));
     }
     if (lw > 0U) {
-        hipLaunchKernelGGL(k_b_collide, dim3((u32)lw), dim3(NPH_COLLIDE_BLOCK), 0, s,
-                           (const NphWave *)(ds + o_lw), (u32)lw,
-                           (const NphCollideWorld *)(ds + o_cw), (const u32 *)(ds + o_list),
-                           h->codes, step_dt);
+        const NphWave *tab = (const NphWave *)(ds + o_lw);
+        const NphCollideWorld *dcw = (const NphCollideWorld *)(ds + o_cw);
+        const u32 *dl = (const u32 *)(ds + o_list);
+        const u32 m = (u32)list;
+        const dim3 grid(devi_blocks((u32)lw * NPH_WAVE, 256U));
+        hipLaunchKernelGGL(k_b_keys_clear, dim3(1), dim3(64), 0, s, b->keys);
+        hipLaunchKernelGGL(k_b_key_count, grid, dim3(256), 0, s, tab, (u32)lw, dcw, dl, b->keys);
+        hipLaunchKernelGGL(k_b_key_scan, dim3(1), dim3(64), 0, s, b->keys,
+                           b->keys + NPH_PAIR_KEYS);
+        hipLaunchKernelGGL(k_b_key_scatter, grid, dim3(256), 0, s, tab, (u32)lw, dcw, dl,
+                           b->keys + NPH_PAIR_KEYS, b->sorted);
+        hipLaunchKernelGGL(k_b_collide, dim3(devi_blocks(m, NPH_COLLIDE_BLOCK)),
+                           dim3(NPH_COLLIDE_BLOCK), 0, s, b->sorted, m, dcw, h->codes, step_dt);
         if (mesh) {
-            hipLaunchKernelGGL(k_b_collide_mesh, dim3((u32)lw), dim3(NPH_COLLIDE_BLOCK), 0, s,
-                               (const NphWave *)(ds + o_lw), (u32)lw,
-                               (const NphCollideWorld *)(ds + o_cw), (const u32 *)(ds + o_list),
-                               h->codes, step_dt);
+            hipLaunchKernelGGL(k_b_collide_mesh, dim3(devi_blocks(m, NPH_COLLIDE_BLOCK)),
+                               dim3(NPH_COLLIDE_BLOCK), 0, s, b->sorted, m, dcw, h->codes,
+                               step_dt);
         }
     }
     HIP_TRY(hipGetLastError());
