@@ -349,41 +349,45 @@ def _batch_run(args, bank, plan, network=None, roofline=False):
           + (" (learned start)" if network else ""), file=sys.stderr, flush=True)
     if network is not None:
         L.batch_set_warm_start(batch, network[0], network[1])
-    rounds = 1
-    tune = batch.autotune(DT, rounds=rounds)
+    # Warm-up steps first, so the autotuner sees the pile the timed steps
+    # will (the choice that wins while bodies fall is not the one that wins
+    # once they rest).
     for _ in range(args.batch_warmup):
         batch.step(DT)
-    warm = 12 * rounds + args.batch_warmup
+    rounds = 2
+    tune = batch.autotune(DT, rounds=rounds)  # 32 candidates a round, one step each
+    warm = args.batch_warmup + 32 * rounds
     L.batch_measure_residual(batch, True)
     L.batch_warm_stats(batch, reset=True)
     solved = 0
-    phases = np.zeros(8)
+    phases = np.zeros(11)
     t0 = time.perf_counter()
     for i in range(args.batch_steps):
         batch.step(DT)
         st = batch.stats()
         solved += st.solved_contacts
-        phases += (st.step_ms, st.pairs_ms, st.changes_ms, st.solve_ms, st.post_ms,
-                   st.device_solve_ms, st.device_passes_ms, st.device_collide_ms)
+        phases += (st.step_ms, st.narrow_ms, st.pairs_ms, st.changes_ms, st.solve_ms, st.post_ms,
+                   st.device_solve_ms, st.device_passes_ms, st.device_collide_ms,
+                   st.wait_narrow_ms, st.wait_solve_ms)
         if (i + 1) % 25 == 0:
             print(f"# batch step {i + 1}/{args.batch_steps}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     res = _batch_residuals(L.batch_warm_stats(batch, reset=True))
     L.batch_measure_residual(batch, False)
-    names = ("step", "host_pairs", "host_changes", "solve", "host_post", "dev_solve",
-             "dev_passes", "dev_collide")
+    names = ("step", "narrow", "host_pairs", "host_changes", "solve", "host_post", "dev_solve",
+             "dev_passes", "dev_collide", "wait_narrow", "wait_solve")
     out = {"value": solved * SUBSTEPS / wall, "unit": "contact-substeps/s",
            "steps": args.batch_steps, "warmup": warm, "ms_per_step": wall * 1e3 / args.batch_steps,
            "setup_s": setup, "bodies": bodies, "worlds": len(plan),
            "tuning": {"pass_block": tune.pass_block, "body_block": tune.body_block,
-                      "use_graph": bool(tune.use_graph)},
+                      "use_graph": bool(tune.use_graph), "groups": tune.groups},
            "phases_ms_per_step": {k: round(v / max(args.batch_steps, 1), 3)
                                   for k, v in zip(names, phases)},
            "residual": res}
     if roofline:
         # The tuned blocks, launched from the stream so each colour pass is
         # bracketed by HIP events.
-        batch.set_tuning(tune.pass_block, tune.body_block, 0)
+        batch.set_tuning(tune.pass_block, tune.body_block, 0, tune.groups)
         a.batch_time_kernels(batch.ptr, True)
         k_solved = 0
         for _ in range(args.batch_roofline_steps):
@@ -565,7 +569,7 @@ def main():
     ap.add_argument("--batch-worlds", type=int, default=1024)
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--batch-steps", type=int, default=200)
-    ap.add_argument("--batch-warmup", type=int, default=8)
+    ap.add_argument("--batch-warmup", type=int, default=40)
     ap.add_argument("--batch-roofline-steps", type=int, default=5)
     ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101])
     ap.add_argument("--train-steps", type=int, default=600)

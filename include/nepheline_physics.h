@@ -895,13 +895,15 @@ typedef struct NphAmdBatchTuning {
     u32 pass_block; // threads a block of the colour passes: 64, 128 or 256
     u32 body_block; // threads a block of the per-body and per-lane stages
     u32 use_graph;  // the substep loop launched as a captured HIP graph
-    u32 pad;
+    u32 groups;     // world groups the step is pipelined over (1 to 4), each
+                    // on its own stream, balanced by awake bodies every step
 } NphAmdBatchTuning;
 
 typedef struct NphAmdBatchStats {
     u64 solved_contacts; // the last step's constraint-graph contacts, all worlds
     u64 awake_bodies;    // the last step's awake bodies, all worlds
     f64 step_ms;         // wall time of the last step
+    f64 narrow_ms;       // both narrow-phase calls: staging, launches, waiting for codes
     f64 pairs_ms;        // host: new pairs (worlds in parallel)
     f64 changes_ms;      // host: narrow-phase changes and solver input
     f64 solve_ms;        // the solve: lists up, device stages, grown proxies back
@@ -909,6 +911,8 @@ typedef struct NphAmdBatchStats {
     f64 device_solve_ms; // device time of the solve (events)
     f64 device_passes_ms;
     f64 device_collide_ms;
+    f64 wait_narrow_ms;  // host blocked on the device for the change codes
+    f64 wait_solve_ms;   // and for the solve's grown proxies
 } NphAmdBatchStats;
 
 NphResult nph_amd_batch_create(const NphPhysicsWorldDesc *desc, u32 world_count,
@@ -922,7 +926,7 @@ void nph_amd_batch_set_tuning(NphAmdBatch *batch, const NphAmdBatchTuning *tunin
 NphAmdBatchTuning nph_amd_batch_tuning(const NphAmdBatch *batch);
 // The autotuner: steps the batch under every candidate launch choice in
 // turn, `rounds` times round-robin (the worlds advance rounds x candidates
-// steps), keeps the one whose solve took the least wall time in total, and
+// steps), keeps the one whose steps took the least wall time in total, and
 // returns it in *chosen.
 NphResult nph_amd_batch_autotune(NphAmdBatch *batch, f32 dt, u32 rounds,
                                  NphAmdBatchTuning *chosen);

@@ -25,6 +25,15 @@
 
 #include "device_impl.h"
 
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
 #define NPH_WAVE 64U
 #define NPH_COLLIDE_BLOCK 64U
 #define NPH_SCAN 1024U
@@ -93,6 +102,93 @@ enum {
     T_COUNT = T_COLOR0 + NPH_GRAPH_COLORS
 };
 
+// Host threads that fill the stages world by world (each world's slice of
+// every staged array is known from prefix sums first, so the worlds are
+// independent): a fixed pool, woken per call, worlds taken one at a time.
+class FillPool {
+  public:
+    FillPool()
+    {
+        unsigned n = std::thread::hardware_concurrency();
+        n = n == 0U ? 1U : (n > 16U ? 16U : n);
+        for (unsigned t = 1U; t < n; ++t) {
+            workers_.emplace_back([this] { loop(); });
+        }
+    }
+    ~FillPool()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+            ++epoch_;
+        }
+        cv_.notify_all();
+        for (auto &w : workers_) {
+            w.join();
+        }
+    }
+    // body(k) for every k in [0, n), on the pool and the calling thread.
+    void run(u32 n, const std::function<void(u32)> &body)
+    {
+        if (n < 8U || workers_.empty()) {
+            for (u32 k = 0U; k < n; ++k) {
+                body(k);
+            }
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            body_ = &body;
+            n_ = n;
+            next_.store(0U);
+            busy_ = (u32)workers_.size();
+            ++epoch_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [this] { return busy_ == 0U; });
+        body_ = nullptr;
+    }
+
+  private:
+    void drain()
+    {
+        for (u32 k = next_.fetch_add(1U); k < n_; k = next_.fetch_add(1U)) {
+            (*body_)(k);
+        }
+    }
+    void loop()
+    {
+        u64 seen = 0U;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return epoch_ != seen; });
+                seen = epoch_;
+                if (stop_) {
+                    return;
+                }
+            }
+            drain();
+            {
+                std::lock_guard<std::mutex> g(m_);
+                busy_ -= 1U;
+            }
+            done_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(u32)> *body_ = nullptr;
+    u32 n_ = 0U;
+    std::atomic<u32> next_{0U};
+    u32 busy_ = 0U;
+    u64 epoch_ = 0U;
+    bool stop_ = false;
+};
+
 struct NphDeviceBatch {
     hipStream_t stream;
     hipEvent_t ev[8]; // solve [0, 1], passes [2, 3], narrow phase halves [4, 5] and [6, 7]
@@ -138,8 +234,13 @@ struct NphDeviceBatch {
     u32 acc_steps;
     // The substep loop as a graph, instantiated once and updated.
     hipGraphExec_t exec;
+    FillPool *pool;
+    std::vector<u64> offs; // per-world prefix sums of the staged arrays
     // Timing.
     f64 solve_ms, passes_ms, collide_ms;
+    f64 wait_ms[2]; // host blocked on the device: narrow phase's codes, solve's grown proxies
+    u32 pending_n;  // worlds of the solve begun and not yet ended
+    u32 pending_timed;
     int time_kernels;
     hipEvent_t *kev;
     u32 kev_pairs;
@@ -796,13 +897,14 @@ template <class T> static NphResult dev_reserve(T **p, u32 *cap, u64 need, hipSt
     return r;
 }
 
-// Waves covering [0, count) of world w, appended to tab.
-static u32 add_waves(NphWave *tab, u32 at, u32 w, u32 count)
+// Waves covering [0, count) of world w, from tab[at]; their indices shifted
+// by `shift` (the world's start in a concatenated array, or 0).
+static u32 add_waves(NphWave *tab, u32 at, u32 w, u32 count, u32 shift = 0U)
 {
     for (u32 base = 0U; base < count; base += NPH_WAVE) {
         tab[at].world = w;
-        tab[at].base = base;
-        tab[at].end = count;
+        tab[at].base = base + shift;
+        tab[at].end = count + shift;
         tab[at].pad = 0U;
         at += 1U;
     }
@@ -818,17 +920,23 @@ NphResult nph_devb_create(NphDeviceBatch **out)
         fprintf(stderr, "nepheline_amd: no HIP device: the world step runs only on an MI355X\n");
         return NPH_RESULT_UNSUPPORTED;
     }
-    NphDeviceBatch *b = (NphDeviceBatch *)calloc(1, sizeof *b);
+    NphDeviceBatch *b = new (std::nothrow) NphDeviceBatch();
     if (b == NULL) {
         return NPH_RESULT_OUT_OF_MEMORY;
     }
+    b->pool = new (std::nothrow) FillPool();
+    if (b->pool == NULL) {
+        delete b;
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        free(b);
+        delete b->pool;
+        delete b;
         return NPH_RESULT_DEVICE_LOST;
     }
     for (u32 i = 0U; i < 8U; ++i) {
         if (hipEventCreate(&b->ev[i]) != hipSuccess) {
-            free(b);
+            nph_devb_destroy(b);
             return NPH_RESULT_DEVICE_LOST;
         }
     }
@@ -893,10 +1001,13 @@ void nph_devb_destroy(NphDeviceBatch *b)
     }
     free(b->kev);
     (void)hipStreamDestroy(b->stream);
-    free(b);
+    delete b->pool;
+    delete b;
 }
 
 NphResult nph_devb_world(NphDeviceBatch *b, NphDevice **out) { return devi_create(out, b->stream); }
+
+void nph_devb_adopt(NphDeviceBatch *b, NphDevice *d) { d->stream = b->stream; }
 
 void nph_devb_set_tuning(NphDeviceBatch *b, const NphBatchTuning *t)
 {
@@ -911,11 +1022,13 @@ void nph_devb_set_tuning(NphDeviceBatch *b, const NphBatchTuning *t)
 
 NphBatchTuning nph_devb_tuning(const NphDeviceBatch *b) { return b->tune; }
 
-void nph_devb_times(const NphDeviceBatch *b, f64 out[3])
+void nph_devb_times(const NphDeviceBatch *b, f64 out[5])
 {
     out[0] = b->solve_ms;
     out[1] = b->passes_ms;
     out[2] = b->collide_ms;
+    out[3] = b->wait_ms[0];
+    out[4] = b->wait_ms[1];
 }
 
 void nph_devb_time_kernels(NphDeviceBatch *b, int enable)
@@ -929,6 +1042,12 @@ void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[2])
 {
     out[0] = b->kernel_ms;
     out[1] = (f64)b->kernel_launches;
+}
+
+NphResult nph_devb_follow(NphDeviceBatch *b, const NphDeviceBatch *first)
+{
+    HIP_TRY(hipStreamWaitEvent(b->stream, first->ev[5], 0));
+    return NPH_RESULT_OK;
 }
 
 NphResult nph_devb_warm_stats(NphDeviceBatch *b, f64 out[8], u32 *steps, int reset)
@@ -999,10 +1118,27 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
     u32 *hl = (u32 *)(hs + o_list);
     NphWave *tiw = (NphWave *)(hs + o_iw);
     NphWave *tlw = (NphWave *)(hs + o_lw);
-    u64 ai = 0U, al = 0U;
-    u32 ti = 0U, tl = 0U;
-    for (u32 k = 0U; k < n; ++k) {
+    // Each world's place in the concatenated arrays and tables, then the
+    // worlds filled in parallel.
+    b->offs.resize(4U * (size_t)n);
+    u64 *off = b->offs.data();
+    {
+        u64 ai = 0U, al = 0U, ti = 0U, tl = 0U;
+        for (u32 k = 0U; k < n; ++k) {
+            const NphCollidePart *p = &parts[k];
+            off[4U * k] = ai;
+            off[4U * k + 1U] = al;
+            off[4U * k + 2U] = ti;
+            off[4U * k + 3U] = tl;
+            ai += p->init_count;
+            al += p->count - p->first;
+            ti += waves_of(p->init_count);
+            tl += waves_of(p->count - p->first);
+        }
+    }
+    b->pool->run(n, [&](u32 k) {
         const NphCollidePart *p = &parts[k];
+        const u64 ai = off[4U * k], al = off[4U * k + 1U];
         cw[k].contacts = p->dev->contacts;
         cw[k].shapes = p->dev->shapes;
         cw[k].bodies = p->dev->bodies;
@@ -1014,23 +1150,10 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
         if (m > 0U) {
             memcpy(hl + al, p->list + p->first, (size_t)m * 4U);
         }
-        // Table entries carry batch-wide indices: base and end offset by the
-        // world's start in the concatenated arrays.
-        const u32 t0 = ti;
-        ti = add_waves(tiw, ti, k, p->init_count);
-        for (u32 j = t0; j < ti; ++j) {
-            tiw[j].base += (u32)ai;
-            tiw[j].end += (u32)ai;
-        }
-        const u32 t1 = tl;
-        tl = add_waves(tlw, tl, k, m);
-        for (u32 j = t1; j < tl; ++j) {
-            tlw[j].base += (u32)al;
-            tlw[j].end += (u32)al;
-        }
-        ai += p->init_count;
-        al += m;
-    }
+        // Table entries carry batch-wide indices.
+        (void)add_waves(tiw, (u32)off[4U * k + 2U], k, p->init_count, (u32)ai);
+        (void)add_waves(tlw, (u32)off[4U * k + 3U], k, m, (u32)al);
+    });
     h->list_count = (u32)list;
     u8 *ds = h->stage.dev;
     if (at > 0U) {
@@ -1081,7 +1204,10 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
     if (l1 > 0U) {
         HIP_TRY(hipMemcpyAsync(b->h_codes + l0, h->codes, l1 * 4U, hipMemcpyDeviceToHost, s));
     }
+    const auto w0 = std::chrono::steady_clock::now();
     HIP_TRY(hipStreamSynchronize(s));
+    b->wait_ms[0] =
+        std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - w0).count();
     float ms = 0.0F;
     b->collide_ms = 0.0;
     if (hipEventElapsedTime(&ms, b->ev[4], b->ev[5]) == hipSuccess) {
@@ -1215,9 +1341,16 @@ static NphResult launch_substeps_graph(NphDeviceBatch *b, const PassPlan *pl,
 NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSolveInput *in,
                          u32 n, const NphGrown **grown, const u32 **starts)
 {
-    *grown = NULL;
-    *starts = NULL;
+    const NphResult r = nph_devb_solve_begin(b, devs, in, n);
+    return r == NPH_RESULT_OK ? nph_devb_solve_end(b, grown, starts) : r;
+}
+
+NphResult nph_devb_solve_begin(NphDeviceBatch *b, NphDevice *const *devs, const NphSolveInput *in,
+                               u32 n)
+{
     hipStream_t s = b->stream;
+    b->pending_n = 0U;
+    b->pending_timed = 0U;
     for (u32 k = 0U; k < n; ++k) {
         const NphResult r = devi_flush_staged(devs[k]);
         if (r != NPH_RESULT_OK) {
@@ -1264,8 +1397,21 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
     memset(W, 0, sizeof W);
     u64 shapes = 0U;
     u32 n_over = 0U;
+    // Per world: its awake, group and joint offsets, its overflow slot, and
+    // its first wave in each table.
+    enum { O_A = 0, O_G, O_J, O_OVER, O_T, O_STRIDE = O_T + T_COUNT };
+    b->offs.resize((size_t)O_STRIDE * n);
+    u64 *off = b->offs.data();
     for (u32 k = 0U; k < n; ++k) {
         const NphSolveInput *x = &in[k];
+        u64 *o = off + (size_t)O_STRIDE * k;
+        o[O_A] = A;
+        o[O_G] = G;
+        o[O_J] = J;
+        o[O_OVER] = n_over;
+        for (u32 t = 0U; t < T_COUNT; ++t) {
+            o[O_T + t] = W[t];
+        }
         u32 g = 0U, j = 0U;
         for (u32 c = 0U; c <= NPH_GRAPH_COLORS; ++c) {
             const u32 per = c == NPH_GRAPH_OVERFLOW ? 1U : NPH_LANES;
@@ -1385,20 +1531,18 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
     u32 *sj = (u32 *)(hs + o_sj);
     NphLearnWorld *lws = (NphLearnWorld *)(hs + o_lw);
     NphWave *tabs[T_COUNT];
-    u32 tfill[T_COUNT];
     {
         NphWave *t0 = (NphWave *)(hs + o_tab);
         for (u32 t = 0U; t < T_COUNT; ++t) {
             tabs[t] = t0;
             t0 += W[t];
-            tfill[t] = 0U;
         }
     }
-    u64 a_off = 0U, g_off = 0U, j_off = 0U;
-    u32 over = 0U;
-    for (u32 k = 0U; k < n; ++k) {
+    b->pool->run(n, [&](u32 k) {
         const NphSolveInput *x = &in[k];
         const NphDevice *d = devs[k];
+        const u64 *o = off + (size_t)O_STRIDE * k;
+        const u64 a_off = o[O_A], g_off = o[O_G], j_off = o[O_J];
         NphStepData *sd = &ws[k];
         memset(sd, 0, sizeof *sd);
         sd->bodies = d->bodies;
@@ -1448,12 +1592,12 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
             j += rr->jcount;
             if (c < NPH_GRAPH_COLORS) {
                 const u32 threads = ((rr->jcount + 63U) & ~63U) + rr->count * NPH_LANES;
-                tfill[T_COLOR0 + c] = add_waves(tabs[T_COLOR0 + c], tfill[T_COLOR0 + c], k, threads);
+                (void)add_waves(tabs[T_COLOR0 + c], (u32)o[O_T + T_COLOR0 + c], k, threads);
             }
         }
         if (rg[k * NPH_SLOTS + NPH_GRAPH_OVERFLOW].count > 0U ||
             rg[k * NPH_SLOTS + NPH_GRAPH_OVERFLOW].jcount > 0U) {
-            ov[over++] = k;
+            ov[o[O_OVER]] = k;
         }
         if (j > 0U) {
             memcpy(sj + j_off, x->graph_joints, (size_t)j * 4U);
@@ -1468,15 +1612,14 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
         lw->groups = g;
         lw->learn = d->learn_on ? 1U : 0U;
         lw->measure = (d->learn_on || d->measure) ? 1U : 0U;
-        tfill[T_SLOTS] = add_waves(tabs[T_SLOTS], tfill[T_SLOTS], k, x->body_slots);
-        tfill[T_BODIES] = add_waves(tabs[T_BODIES], tfill[T_BODIES], k, x->awake_count + 1U);
-        tfill[T_GROUPS] = add_waves(tabs[T_GROUPS], tfill[T_GROUPS], k, g);
-        tfill[T_LANES] = add_waves(tabs[T_LANES], tfill[T_LANES], k, g * NPH_LANES);
-        tfill[T_JOINTS] = add_waves(tabs[T_JOINTS], tfill[T_JOINTS], k, j);
-        a_off += x->awake_count;
-        g_off += g;
-        j_off += j;
-    }
+        (void)add_waves(tabs[T_SLOTS], (u32)o[O_T + T_SLOTS], k, x->body_slots);
+        (void)add_waves(tabs[T_BODIES], (u32)o[O_T + T_BODIES], k, x->awake_count + 1U);
+        (void)add_waves(tabs[T_GROUPS], (u32)o[O_T + T_GROUPS], k, g);
+        (void)add_waves(tabs[T_LANES], (u32)o[O_T + T_LANES], k, g * NPH_LANES);
+        (void)add_waves(tabs[T_JOINTS], (u32)o[O_T + T_JOINTS], k, j);
+    });
+    const u64 a_off = A;
+    const u32 over = n_over;
     ao[n] = (u32)a_off;
     HIP_TRY(hipMemcpyAsync(ds, hs, at, hipMemcpyHostToDevice, s));
 
@@ -1605,7 +1748,29 @@ NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSol
         return r;
     }
     HIP_TRY(hipMemcpyAsync(b->h_wstart, b->wstart, (u64)(n + 1U) * 4U, hipMemcpyDeviceToHost, s));
+    b->pending_n = n;
+    b->pending_timed = timed;
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_devb_solve_end(NphDeviceBatch *b, const NphGrown **grown, const u32 **starts)
+{
+    *grown = NULL;
+    *starts = NULL;
+    hipStream_t s = b->stream;
+    const u32 n = b->pending_n;
+    const u32 timed = b->pending_timed;
+    b->pending_n = 0U;
+    b->pending_timed = 0U;
+    if (n == 0U) {
+        b->wait_ms[1] = 0.0;
+        return NPH_RESULT_OK;
+    }
+    NphResult r = NPH_RESULT_OK;
+    const auto w0 = std::chrono::steady_clock::now();
     HIP_TRY(hipStreamSynchronize(s));
+    b->wait_ms[1] =
+        std::chrono::duration<f64, std::milli>(std::chrono::steady_clock::now() - w0).count();
     const u32 m = b->h_wstart[n];
     r = pinned_reserve(s, (void **)&b->h_grown, &b->cap_h_grown, (u64)(m + 1U) * sizeof(NphGrown));
     if (r != NPH_RESULT_OK) {

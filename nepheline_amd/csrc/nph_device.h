@@ -148,6 +148,9 @@ NphResult nph_devb_create(NphDeviceBatch **out);
 void nph_devb_destroy(NphDeviceBatch *b);
 // A world's device whose work goes on the batch's stream.
 NphResult nph_devb_world(NphDeviceBatch *b, NphDevice **out);
+// A world's device (made by any batch of the same owner) moved to b's
+// stream; every earlier operation of the world must have completed.
+void nph_devb_adopt(NphDeviceBatch *b, NphDevice *d);
 void nph_devb_set_tuning(NphDeviceBatch *b, const NphBatchTuning *t);
 NphBatchTuning nph_devb_tuning(const NphDeviceBatch *b);
 
@@ -176,9 +179,19 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
 // learned warm start must share one network.
 NphResult nph_devb_solve(NphDeviceBatch *b, NphDevice *const *devs, const NphSolveInput *in,
                          u32 n, const NphGrown **grown, const u32 **starts);
+// b's later work waits for the first narrow-phase half `first` queued
+// last (so that half runs with the whole device before b's).
+NphResult nph_devb_follow(NphDeviceBatch *b, const NphDeviceBatch *first);
+// The same in two calls: begin queues the solve and returns (the inputs may
+// change at once); end waits for it and returns the grown proxies.
+NphResult nph_devb_solve_begin(NphDeviceBatch *b, NphDevice *const *devs, const NphSolveInput *in,
+                               u32 n);
+NphResult nph_devb_solve_end(NphDeviceBatch *b, const NphGrown **grown, const u32 **starts);
 // Device milliseconds of the last batched solve: out[0] the whole solve,
-// out[1] the constraint passes, out[2] the narrow phase (both halves).
-void nph_devb_times(const NphDeviceBatch *b, f64 out[3]);
+// out[1] the constraint passes, out[2] the narrow phase (both halves); and
+// the host's milliseconds blocked on the device: out[3] for the narrow
+// phase's change codes, out[4] for the solve's grown proxies.
+void nph_devb_times(const NphDeviceBatch *b, f64 out[5]);
 // The learned warm start's statistics summed over the measured worlds and
 // steps since the last reset: out = residual sums (plain, learned), contact
 // counts (plain, learned), largest residuals (plain, learned), largest

@@ -577,15 +577,18 @@ def contacts_as_arrays(contacts):
 # Batched worlds (nph_amd_batch_*: this library's extension)
 # ---------------------------------------------------------------------------
 class BatchTuning(C.Structure):
-    _fields_ = [("pass_block", _U32), ("body_block", _U32), ("use_graph", _U32), ("pad", _U32)]
+    _fields_ = [("pass_block", _U32), ("body_block", _U32), ("use_graph", _U32),
+                ("groups", _U32)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("solved_contacts", C.c_uint64), ("awake_bodies", C.c_uint64),
-                ("step_ms", C.c_double), ("pairs_ms", C.c_double), ("changes_ms", C.c_double),
+                ("step_ms", C.c_double), ("narrow_ms", C.c_double), ("pairs_ms", C.c_double),
+                ("changes_ms", C.c_double),
                 ("solve_ms", C.c_double), ("post_ms", C.c_double),
                 ("device_solve_ms", C.c_double), ("device_passes_ms", C.c_double),
-                ("device_collide_ms", C.c_double)]
+                ("device_collide_ms", C.c_double), ("wait_narrow_ms", C.c_double),
+                ("wait_solve_ms", C.c_double)]
 
 
 def _bind_batch(a: Api):
@@ -635,8 +638,8 @@ class Batch:
     def tuning(self) -> BatchTuning:
         return self.api.batch_tuning(self.ptr)
 
-    def set_tuning(self, pass_block=256, body_block=256, use_graph=0):
-        t = BatchTuning(pass_block, body_block, use_graph, 0)
+    def set_tuning(self, pass_block=256, body_block=256, use_graph=0, groups=2):
+        t = BatchTuning(pass_block, body_block, use_graph, groups)
         self.api.batch_set_tuning(self.ptr, C.byref(t))
 
     def autotune(self, dt, rounds=2) -> BatchTuning:

@@ -48,7 +48,9 @@ def _worlds(hull_bank, mesh_bank):
     ]
 
 
-TUNINGS = [(256, 256, 0), (64, 64, 1), (128, 256, 1), (256, 64, 0)]
+# (pass block, body block, graph, groups): the worlds change groups and
+# streams as the tuning changes.
+TUNINGS = [(256, 256, 0, 2), (64, 64, 1, 1), (128, 256, 1, 4), (256, 64, 0, 3)]
 
 
 def test_batch_matches_each_world_alone(gpu_api, ref_api, hull_bank, mesh_bank):
@@ -91,12 +93,13 @@ def test_autotune_picks_a_candidate_and_results_stay_exact(gpu_api, ref_api, hul
         S.bin_drop(None, 40 + 60 * k, 30 + k, hull_bank, half=4.0, height=10.0, y_lo=1.0, world=w)
         refs.append(S.bin_drop(ref_api, 40 + 60 * k, 30 + k, hull_bank, half=4.0, height=10.0,
                                y_lo=1.0))
-    t = batch.autotune(S.DT, rounds=1)  # 12 candidates, one step each
-    assert t.pass_block in (64, 128, 256) and t.body_block in (64, 256) and t.use_graph in (0, 1)
+    t = batch.autotune(S.DT, rounds=1)  # 32 candidates, one step each
+    assert t.pass_block in (128, 256) and t.body_block in (64, 256) and t.use_graph in (0, 1)
+    assert 1 <= t.groups <= 4
     got = batch.tuning()
-    assert (got.pass_block, got.body_block, got.use_graph) == (t.pass_block, t.body_block,
-                                                               t.use_graph)
-    for _ in range(12):
+    assert (got.pass_block, got.body_block, got.use_graph, got.groups) == (
+        t.pass_block, t.body_block, t.use_graph, t.groups)
+    for _ in range(32):
         for r in refs:
             r.step(S.DT)
     for step in range(20):
