@@ -591,6 +591,7 @@ typedef struct PairQuery {
     u32 shape;
     u32 key;
     u32 *out;          // the pairs' place; NULL: counted, or appended to `append`
+    u32 *cache;        // counting: the first NPH_PAIR_CACHE pairs kept here
     NphVecBuf *append; // pairs of u32, grown as they come (one pass)
     NphResult err;
     u32 count;
@@ -634,6 +635,11 @@ static int pair_callback(u32 proxy, u64 user_data, void *context)
     if (q->out != NULL) {
         q->out[2U * q->count] = q->shape;
         q->out[2U * q->count + 1U] = other;
+    } else if (q->cache != NULL) {
+        if (q->count < NPH_PAIR_CACHE) {
+            q->cache[2U * q->count] = q->shape;
+            q->cache[2U * q->count + 1U] = other;
+        }
     } else if (q->append != NULL) {
         const u32 pair[2] = {q->shape, other};
         q->err = vbuf_push(w->alloc, q->append, 8U, pair);
@@ -647,7 +653,7 @@ static int pair_callback(u32 proxy, u64 user_data, void *context)
 
 // contact.c:748-771: one moved proxy's new pairs, into `out`, or appended
 // to `append`, or (both NULL) only counted.
-static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out, NphVecBuf *append,
+static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out, u32 *cache, NphVecBuf *append,
                        NphResult *err)
 {
     const u32 tree = key & 3U;
@@ -657,6 +663,7 @@ static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out, NphVecBuf *append,
     q.shape = (u32)tree_user_data(&w->trees[tree], proxy);
     q.key = key;
     q.out = out;
+    q.cache = cache;
     q.append = append;
     q.err = NPH_RESULT_OK;
     q.count = 0U;
@@ -698,7 +705,7 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
         w->pair_buffer.len = 0U;
         for (u32 i = 0U; i < n; ++i) {
             NphResult r = NPH_RESULT_OK;
-            (void)query_moved(w, moved[i], NULL, &w->pair_buffer, &r);
+            (void)query_moved(w, moved[i], NULL, NULL, &w->pair_buffer, &r);
             if (r != NPH_RESULT_OK) {
                 return r;
             }
@@ -710,13 +717,18 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
         // buffer's order and each proxy's in its trees' order, however many
         // threads query.
         NphResult r = vbuf_reserve(w->alloc, &w->pair_counts, 4U, n + 1U);
+        if (r == NPH_RESULT_OK) {
+            r = vbuf_reserve(w->alloc, &w->pair_cache, 8U * NPH_PAIR_CACHE, n);
+        }
         if (r != NPH_RESULT_OK) {
             return r;
         }
         u32 *counts = VB(w->pair_counts, u32);
+        u32 *cache = VB(w->pair_cache, u32);
 #pragma omp parallel for schedule(dynamic, 16)
         for (u32 i = 0U; i < n; ++i) {
-            counts[i] = query_moved(w, moved[i], NULL, NULL, NULL);
+            counts[i] = query_moved(w, moved[i], NULL, cache + (u64)i * 2U * NPH_PAIR_CACHE,
+                                    NULL, NULL);
         }
         for (u32 i = 0U; i < n; ++i) {
             const u32 c = counts[i];
@@ -731,10 +743,14 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
         w->pair_buffer.len = total;
         u32 *pairs = VB(w->pair_buffer, u32);
         if (total > 0U) {
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 64)
             for (u32 i = 0U; i < n; ++i) {
-                if (counts[i + 1U] > counts[i]) {
-                    (void)query_moved(w, moved[i], &pairs[2U * counts[i]], NULL, NULL);
+                const u32 c = counts[i + 1U] - counts[i];
+                if (c > NPH_PAIR_CACHE) {
+                    (void)query_moved(w, moved[i], &pairs[2U * counts[i]], NULL, NULL, NULL);
+                } else if (c > 0U) {
+                    memcpy(&pairs[2U * counts[i]], cache + (u64)i * 2U * NPH_PAIR_CACHE,
+                           (size_t)c * 8U);
                 }
             }
         }

@@ -296,6 +296,7 @@ void nph_physics_world_destroy(NphPhysicsWorld *w)
     vbuf_free(a, &w->move_buffer, 4U);
     vbuf_free(a, &w->pair_buffer, 8U);
     vbuf_free(a, &w->pair_counts, 4U);
+    vbuf_free(a, &w->pair_cache, 8U * NPH_PAIR_CACHE);
     for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
         vbuf_free(a, &w->color_bits[c], 8U);
     }

@@ -200,6 +200,7 @@ struct NphPhysicsWorld {
     NphTree trees[3];
     NphVecBuf move_buffer; // u32 proxy keys
     NphVecBuf pair_counts; // u32 per moved proxy, then their prefix sums
+    NphVecBuf pair_cache;  // u32 pairs: the counting pass's first pairs of each moved proxy
     NphVecBuf pair_buffer; // u32 pairs (a, b)
 
     NphVecBuf color_bits[NPH_GRAPH_COLORS]; // u64 words over body slots
@@ -240,6 +241,9 @@ NphResult world_upload_body(NphPhysicsWorld *w, u32 slot);
 NphResult world_wake_body(NphPhysicsWorld *w, u32 slot);
 
 // contact.c restated
+// New pairs a moved proxy's counting pass keeps, so that the filling pass
+// queries again only the proxies with more.
+#define NPH_PAIR_CACHE 8U
 NphResult contacts_update_pairs(NphPhysicsWorld *w);
 // The step in phases, for the batch that steps many worlds together.
 NphResult world_create(const NphPhysicsWorldDesc *desc, NphDeviceBatch *batch,
