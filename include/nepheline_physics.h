@@ -247,6 +247,56 @@ NphAABB nph_physics_geometry_aabb(const NphPhysicsGeometry *g, NphPhysicsPose po
 bool nph_physics_geometry_is_valid(const NphPhysicsGeometry *g);
 
 // ---------------------------------------------------------------------------
+// Pair queries (physics/collide.h:62-121): the narrow phase of one pair,
+// run on the device by the same code as the world step's.
+// ---------------------------------------------------------------------------
+typedef struct NphPhysicsManifoldPoint {
+    NphVec3 point;  // midway between the two surfaces
+    f32 separation; // along the normal; negative when the shapes overlap
+    u32 id;         // the features that made the point
+} NphPhysicsManifoldPoint;
+
+typedef struct NphPhysicsManifold {
+    NphVec3 normal;  // unit, from shape A towards shape B
+    u32 point_count; // 0: farther apart than the speculative distance
+    NphPhysicsManifoldPoint points[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
+} NphPhysicsManifold;
+
+// What the previous call on the pair found (zero: no history).
+typedef struct NphPhysicsCollideCache {
+    u8 type;
+    u8 reserved;
+    u16 index_a;
+    u16 index_b;
+    u16 reserved2;
+} NphPhysicsCollideCache;
+
+// collide.h:96: every pair of convex types.
+bool nph_physics_collide_supported(NphPhysicsShapeType a, NphPhysicsShapeType b);
+// collide.h:103-106: the manifold of a at pose_a and b at pose_b, points
+// at most speculative_distance apart, at most four; cache may be NULL.
+// Without a HIP device: no points, and a message on stderr.
+void nph_physics_collide(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                         const NphPhysicsGeometry *b, NphPhysicsPose pose_b,
+                         f32 speculative_distance, NphPhysicsCollideCache *cache,
+                         NphPhysicsManifold *out);
+// collide.h:121: `count` candidates reduced to at most four, in place.
+u32 nph_physics_manifold_reduce(NphPhysicsManifoldPoint *points, u32 count, NphVec3 normal);
+
+// No counterpart in the reference: many pairs in one launch (pairs of
+// convex shapes only); caches, when given, are read and written per pair.
+typedef struct NphAmdCollideQuery {
+    NphPhysicsGeometry a;
+    NphPhysicsPose pose_a;
+    NphPhysicsGeometry b;
+    NphPhysicsPose pose_b;
+    f32 speculative_distance;
+    u32 pad;
+} NphAmdCollideQuery;
+NphResult nph_amd_collide_many(const NphAmdCollideQuery *queries, u32 count,
+                               NphPhysicsManifold *out, NphPhysicsCollideCache *caches);
+
+// ---------------------------------------------------------------------------
 // World (physics/world.h:57-326)
 // ---------------------------------------------------------------------------
 typedef struct NphPhysicsWorld NphPhysicsWorld;

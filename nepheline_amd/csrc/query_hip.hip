@@ -1,0 +1,300 @@
+// SPDX-License-Identifier: Apache-2.0
+// The reference's standalone narrow-phase queries (collide.h:97-121) on the
+// device: nph_physics_collide and nph_physics_manifold_reduce run the same
+// device code the world step's narrow phase runs (nph_collide.h), one
+// thread a pair, and nph_amd_collide_many runs many pairs in one launch.
+//
+// The queries own a device context of their own (stream, buffers, device
+// copies of the hull blocks they have seen), made on first use and guarded
+// by a mutex: the reference's functions are free functions, callable from
+// any thread.  There is no CPU fallback: without a HIP device the calls
+// report no contact and say why on stderr.
+
+#include "device_impl.h"
+
+#include <mutex>
+
+namespace {
+
+typedef struct QueryPair {
+    NphGeom a;
+    NphGeom b;
+    NphPhysicsPose pose_a;
+    NphPhysicsPose pose_b;
+    f32 speculative;
+    u32 pad;
+    NphCache cache;
+} QueryPair;
+
+struct QueryContext {
+    std::mutex lock;
+    int ready = 0;
+    int failed = 0;
+    hipStream_t stream = NULL;
+    QueryPair *pairs = NULL;
+    NphManifold *manifolds = NULL;
+    NphCache *caches = NULL;
+    u32 cap = 0U;
+    NphMPoint *points = NULL;
+    u32 cap_points = 0U;
+    u32 *count = NULL;
+    HullEntry *blocks = NULL;
+    u32 block_count = 0U, block_cap = 0U;
+};
+
+QueryContext g_query;
+
+NphResult query_ready(QueryContext *q)
+{
+    if (q->ready) {
+        return NPH_RESULT_OK;
+    }
+    if (q->failed) {
+        return NPH_RESULT_UNSUPPORTED;
+    }
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1 ||
+        hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&q->count, 4U) != hipSuccess) {
+        fprintf(stderr, "nepheline_amd: no HIP device: the collision queries run only on an "
+                        "MI355X\n");
+        q->failed = 1;
+        return NPH_RESULT_UNSUPPORTED;
+    }
+    q->ready = 1;
+    return NPH_RESULT_OK;
+}
+
+// The device copy of a hull block (made once per block).
+NphResult query_block(QueryContext *q, const void *block, u32 bytes, const void **out)
+{
+    for (u32 i = 0U; i < q->block_count; ++i) {
+        if (q->blocks[i].host == block) {
+            *out = q->blocks[i].device;
+            return NPH_RESULT_OK;
+        }
+    }
+    if (q->block_count == q->block_cap) {
+        const u32 c = q->block_cap == 0U ? 16U : 2U * q->block_cap;
+        HullEntry *n = (HullEntry *)realloc(q->blocks, (size_t)c * sizeof(HullEntry));
+        if (n == NULL) {
+            return NPH_RESULT_OUT_OF_MEMORY;
+        }
+        q->blocks = n;
+        q->block_cap = c;
+    }
+    void *p = NULL;
+    HIP_TRY(hipMalloc(&p, bytes));
+    HIP_TRY(hipMemcpy(p, block, bytes, hipMemcpyHostToDevice));
+    q->blocks[q->block_count].host = block;
+    q->blocks[q->block_count].device = p;
+    q->block_count += 1U;
+    *out = p;
+    return NPH_RESULT_OK;
+}
+
+NphResult query_geom(QueryContext *q, const NphPhysicsGeometry *g, NphGeom *out)
+{
+    const void *block = NULL;
+    if (g->type == NPH_PHYSICS_SHAPE_HULL) {
+        const NphResult r = query_block(q, g->hull, g->hull->byte_size, &block);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+    }
+    *out = geom_from(g, block);
+    return NPH_RESULT_OK;
+}
+
+int convex(NphPhysicsShapeType t)
+{
+    return t == NPH_PHYSICS_SHAPE_SPHERE || t == NPH_PHYSICS_SHAPE_CAPSULE ||
+           t == NPH_PHYSICS_SHAPE_BOX || t == NPH_PHYSICS_SHAPE_HULL;
+}
+
+} // namespace
+
+__global__ void __launch_bounds__(64) k_query_collide(const QueryPair *pairs, u32 n,
+                                                      NphManifold *out, NphCache *caches)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const QueryPair p = pairs[i];
+    NphCache cache = p.cache;
+    NphManifold m;
+    nph_collide(&p.a, p.pose_a, &p.b, p.pose_b, p.speculative, &cache, &m);
+    out[i] = m;
+    caches[i] = cache;
+}
+
+__global__ void __launch_bounds__(64) k_query_reduce(NphMPoint *points, u32 count, NphVec3 normal,
+                                                     u32 *kept)
+{
+    if (blockIdx.x == 0U && threadIdx.x == 0U) {
+        *kept = manifold_reduce(points, count, normal);
+    }
+}
+
+extern "C" {
+
+bool nph_physics_collide_supported(NphPhysicsShapeType a, NphPhysicsShapeType b)
+{
+    return convex(a) && convex(b);
+}
+
+NphResult nph_amd_collide_many(const NphAmdCollideQuery *queries, u32 n, NphPhysicsManifold *out,
+                               NphPhysicsCollideCache *caches)
+{
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    if (queries == NULL || out == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    QueryContext *q = &g_query;
+    std::lock_guard<std::mutex> guard(q->lock);
+    NphResult r = query_ready(q);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    if (n > q->cap) {
+        u32 c = q->cap < 64U ? 64U : q->cap;
+        while (c < n) {
+            c *= 2U;
+        }
+        HIP_TRY(hipStreamSynchronize(q->stream));
+        void *old[3] = {q->pairs, q->manifolds, q->caches};
+        for (u32 i = 0U; i < 3U; ++i) {
+            if (old[i] != NULL) {
+                HIP_TRY(hipFree(old[i]));
+            }
+        }
+        q->pairs = NULL;
+        q->manifolds = NULL;
+        q->caches = NULL;
+        q->cap = 0U;
+        HIP_TRY(hipMalloc((void **)&q->pairs, (u64)c * sizeof(QueryPair)));
+        HIP_TRY(hipMalloc((void **)&q->manifolds, (u64)c * sizeof(NphManifold)));
+        HIP_TRY(hipMalloc((void **)&q->caches, (u64)c * sizeof(NphCache)));
+        q->cap = c;
+    }
+    QueryPair *h = (QueryPair *)malloc((size_t)n * sizeof(QueryPair));
+    if (h == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    for (u32 i = 0U; i < n && r == NPH_RESULT_OK; ++i) {
+        const NphAmdCollideQuery *x = &queries[i];
+        memset(&h[i], 0, sizeof h[i]);
+        if (!convex(x->a.type) || !convex(x->b.type)) {
+            r = NPH_RESULT_INVALID_ARGUMENT;
+            break;
+        }
+        r = query_geom(q, &x->a, &h[i].a);
+        if (r == NPH_RESULT_OK) {
+            r = query_geom(q, &x->b, &h[i].b);
+        }
+        h[i].pose_a = x->pose_a;
+        h[i].pose_b = x->pose_b;
+        h[i].speculative = x->speculative_distance;
+        if (caches != NULL) {
+            memcpy(&h[i].cache, &caches[i], sizeof h[i].cache);
+        }
+    }
+    if (r == NPH_RESULT_OK) {
+        const hipError_t e1 = hipMemcpyAsync(q->pairs, h, (u64)n * sizeof(QueryPair),
+                                             hipMemcpyHostToDevice, q->stream);
+        if (e1 == hipSuccess) {
+            hipLaunchKernelGGL(k_query_collide, dim3(devi_blocks(n, 64U)), dim3(64), 0, q->stream,
+                               q->pairs, n, q->manifolds, q->caches);
+        }
+        const hipError_t e2 = e1 == hipSuccess ? hipGetLastError() : e1;
+        const hipError_t e3 =
+            e2 == hipSuccess ? hipMemcpyAsync(out, q->manifolds, (u64)n * sizeof(NphManifold),
+                                              hipMemcpyDeviceToHost, q->stream)
+                             : e2;
+        const hipError_t e4 = (e3 == hipSuccess && caches != NULL)
+                                  ? hipMemcpyAsync(caches, q->caches, (u64)n * sizeof(NphCache),
+                                                   hipMemcpyDeviceToHost, q->stream)
+                                  : e3;
+        const hipError_t e5 = e4 == hipSuccess ? hipStreamSynchronize(q->stream) : e4;
+        if (e5 != hipSuccess) {
+            fprintf(stderr, "nepheline_amd: collision query failed: %s\n", hipGetErrorString(e5));
+            r = NPH_RESULT_DEVICE_LOST;
+        }
+    }
+    free(h);
+    return r;
+}
+
+void nph_physics_collide(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                         const NphPhysicsGeometry *b, NphPhysicsPose pose_b,
+                         f32 speculative_distance, NphPhysicsCollideCache *cache,
+                         NphPhysicsManifold *out)
+{
+    NphAmdCollideQuery qy;
+    qy.a = *a;
+    qy.pose_a = pose_a;
+    qy.b = *b;
+    qy.pose_b = pose_b;
+    qy.speculative_distance = speculative_distance;
+    qy.pad = 0U;
+    NphPhysicsCollideCache local;
+    memset(&local, 0, sizeof local);
+    NphPhysicsCollideCache *c = cache != NULL ? cache : &local;
+    if (nph_amd_collide_many(&qy, 1U, out, c) != NPH_RESULT_OK) {
+        out->normal = v3(0.0F, 1.0F, 0.0F);
+        out->point_count = 0U;
+    }
+}
+
+u32 nph_physics_manifold_reduce(NphPhysicsManifoldPoint *points, u32 count, NphVec3 normal)
+{
+    if (count <= NPH_PHYSICS_MANIFOLD_MAX_POINTS || points == NULL) {
+        return count;
+    }
+    QueryContext *q = &g_query;
+    std::lock_guard<std::mutex> guard(q->lock);
+    if (query_ready(q) != NPH_RESULT_OK) {
+        return count;
+    }
+    if (count > q->cap_points) {
+        u32 c = q->cap_points < 256U ? 256U : q->cap_points;
+        while (c < count) {
+            c *= 2U;
+        }
+        if (hipStreamSynchronize(q->stream) != hipSuccess) {
+            return count;
+        }
+        if (q->points != NULL) {
+            (void)hipFree(q->points);
+            q->points = NULL;
+            q->cap_points = 0U;
+        }
+        if (hipMalloc((void **)&q->points, (u64)c * sizeof(NphMPoint)) != hipSuccess) {
+            return count;
+        }
+        q->cap_points = c;
+    }
+    u32 kept = count;
+    if (hipMemcpyAsync(q->points, points, (u64)count * sizeof(NphMPoint), hipMemcpyHostToDevice,
+                       q->stream) != hipSuccess) {
+        return count;
+    }
+    hipLaunchKernelGGL(k_query_reduce, dim3(1), dim3(64), 0, q->stream, q->points, count, normal,
+                       q->count);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&kept, q->count, 4U, hipMemcpyDeviceToHost, q->stream) != hipSuccess ||
+        hipStreamSynchronize(q->stream) != hipSuccess) {
+        fprintf(stderr, "nepheline_amd: manifold reduction failed\n");
+        return count;
+    }
+    if (hipMemcpy(points, q->points, (u64)kept * sizeof(NphMPoint), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+        return count;
+    }
+    return kept;
+}
+
+} // extern "C"

@@ -65,6 +65,46 @@ class Geometry(C.Structure):
     _fields_ = [("type", C.c_int), ("u", _GeomUnion)]
 
 
+class ManifoldPoint(C.Structure):
+    """collide.h:66-73"""
+    _fields_ = [("point", Vec3), ("separation", C.c_float), ("id", C.c_uint32)]
+
+
+class Manifold(C.Structure):
+    """collide.h:75-79"""
+    _fields_ = [("normal", Vec3), ("point_count", C.c_uint32), ("points", ManifoldPoint * 4)]
+
+
+class CollideCache(C.Structure):
+    """collide.h:85-91"""
+    _fields_ = [("type", C.c_uint8), ("reserved", C.c_uint8), ("index_a", C.c_uint16),
+                ("index_b", C.c_uint16), ("reserved2", C.c_uint16)]
+
+
+class CollideQuery(C.Structure):
+    """nph_amd_collide_many's pair (this library's extension)."""
+    _fields_ = [("a", Geometry), ("pose_a", Pose), ("b", Geometry), ("pose_b", Pose),
+                ("speculative_distance", C.c_float), ("pad", C.c_uint32)]
+
+
+def bind_collide(a):
+    """The pair queries of either library (collide.h:96-121)."""
+    if hasattr(a, "collide"):
+        return a
+    L = a.lib
+    a.collide_supported = _bind(L, "nph_physics_collide_supported", C.c_bool, [C.c_int, C.c_int])
+    a.collide = _bind(L, "nph_physics_collide", None,
+                      [C.POINTER(Geometry), Pose, C.POINTER(Geometry), Pose, C.c_float,
+                       C.POINTER(CollideCache), C.POINTER(Manifold)])
+    a.manifold_reduce = _bind(L, "nph_physics_manifold_reduce", C.c_uint32,
+                              [C.POINTER(ManifoldPoint), C.c_uint32, Vec3])
+    if hasattr(L, "nph_amd_collide_many"):
+        a.collide_many = _bind(L, "nph_amd_collide_many", C.c_int,
+                               [C.POINTER(CollideQuery), C.c_uint32, C.POINTER(Manifold),
+                                C.POINTER(CollideCache)])
+    return a
+
+
 class WorldDesc(C.Structure):
     _fields_ = [("alloc", C.c_void_p), ("gravity", Vec3), ("solver", C.c_int),
                 ("substeps", C.c_uint32), ("contact_hertz", C.c_float),

@@ -48,6 +48,8 @@ LAYOUT_TYPES = {
     "NphPhysicsBallJoint": P.BallJoint, "NphPhysicsHingeJoint": P.HingeJoint,
     "NphPhysicsD6Joint": P.D6Joint, "NphPhysicsMesh": P.Mesh, "NphPhysicsMeshNode": P.MeshNode,
     "NphPhysicsContactBeginEvent": P.PairEvent, "NphPhysicsSensorEvent": P.SensorEvent,
+    "NphPhysicsManifold": P.Manifold, "NphPhysicsManifoldPoint": P.ManifoldPoint,
+    "NphPhysicsCollideCache": P.CollideCache,
 }
 
 
@@ -97,7 +99,8 @@ def test_header_layouts_are_the_references(tmp_path):
     mine = _layouts(tmp_path, [os.path.join(ROOT, "include")], "<nepheline_physics.h>")
     hdr = "<nepheline/physics/world.h>\n#include <nepheline/physics/body.h>\n" \
           "#include <nepheline/physics/shape.h>\n#include <nepheline/physics/hull.h>\n" \
-          "#include <nepheline/physics/joint.h>\n#include <nepheline/physics/mesh.h>"
+          "#include <nepheline/physics/joint.h>\n#include <nepheline/physics/mesh.h>\n" \
+          "#include <nepheline/physics/collide.h>"
     ref = _layouts(tmp_path, REF_INCLUDE, hdr)
     assert mine == ref
 
@@ -256,3 +259,20 @@ def test_batch_plan_is_log_uniform_over_the_three_kinds():
     assert {k for k, _, _ in plan} == {0, 1, 2}
     small = sum(1 for n in sizes if n < 566)  # 16 * sqrt(1250): the median
     assert 440 < small < 584
+
+
+def test_collide_query_layout():
+    """The extension's query struct as the header lays it out."""
+    import tempfile, pathlib
+    with tempfile.TemporaryDirectory() as d:
+        src = pathlib.Path(d) / "q.c"
+        src.write_text("#include <nepheline_physics.h>\n#include <stdio.h>\n#include <stddef.h>\n"
+                       "int main(void){printf(\"%zu %zu %zu\\n\", sizeof(NphAmdCollideQuery), "
+                       "offsetof(NphAmdCollideQuery, b), offsetof(NphAmdCollideQuery, "
+                       "speculative_distance));return 0;}")
+        exe = pathlib.Path(d) / "q"
+        subprocess.run(["gcc", "-std=c17", f"-I{os.path.join(ROOT, 'include')}", "-o", str(exe),
+                        str(src)], check=True, capture_output=True)
+        out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert [int(x) for x in out] == [C.sizeof(P.CollideQuery), P.CollideQuery.b.offset,
+                                     P.CollideQuery.speculative_distance.offset]
