@@ -385,9 +385,9 @@ def _batch_run(args, bank, plan, network=None, roofline=False):
                                   for k, v in zip(names, phases)},
            "residual": res}
     if roofline:
-        # The tuned blocks, launched from the stream so each colour pass is
-        # bracketed by HIP events.
-        batch.set_tuning(tune.pass_block, tune.body_block, 0, tune.groups)
+        # The tuned blocks, launched from one stream (one group, so no two
+        # colour passes overlap) with HIP events around each.
+        batch.set_tuning(tune.pass_block, tune.body_block, 0, 1)
         a.batch_time_kernels(batch.ptr, True)
         k_solved = 0
         for _ in range(args.batch_roofline_steps):
@@ -416,8 +416,8 @@ def _batch_run(args, bank, plan, network=None, roofline=False):
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_b_color_pass (one colour of one constraint pass over every world), "
-                      "HIP events around each launch over %d steps after the timed region"
-                      % args.batch_roofline_steps,
+                      "HIP events around each launch over %d steps after the timed region, "
+                      "one group" % args.batch_roofline_steps,
             "launches": int(kt[1]), "avg_launch_us": avg_launch_s * 1e6,
             "bytes_per_launch": bytes_per_launch, "bytes_per_contact_substep": mean_bytes}
     batch.close()
