@@ -354,9 +354,14 @@ def _batch_run(args, bank, plan, network=None, roofline=False):
     # once they rest).
     for _ in range(args.batch_warmup):
         batch.step(DT)
-    rounds = 2
-    tune = batch.autotune(DT, rounds=rounds)  # 32 candidates a round, one step each
-    warm = args.batch_warmup + 32 * rounds
+    if args.batch_fixed:
+        batch.set_tuning(256, 64, 0, 1)
+        tune = batch.tuning()
+        warm = args.batch_warmup
+    else:
+        rounds = 2
+        tune = batch.autotune(DT, rounds=rounds)  # 32 candidates a round, one step each
+        warm = args.batch_warmup + 32 * rounds
     L.batch_measure_residual(batch, True)
     L.batch_warm_stats(batch, reset=True)
     solved = 0
@@ -571,6 +576,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-warmup", type=int, default=40)
     ap.add_argument("--batch-roofline-steps", type=int, default=5)
+    ap.add_argument("--batch-fixed", action="store_true",
+                    help="no autotuner: colour-pass blocks of 256, body blocks of 64, stream "
+                         "launches, one group (one stream), as the profiles are taken")
     ap.add_argument("--train-seeds", type=int, nargs="+", default=[100, 101])
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--train-epochs", type=int, default=8)
