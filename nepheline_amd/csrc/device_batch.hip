@@ -25,6 +25,8 @@
 
 #include "device_impl.h"
 
+#include <hip/hip_ext.h>
+
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -1248,14 +1250,16 @@ static NphResult launch_pass(NphDeviceBatch *b, const PassPlan *pl, const NphSte
             continue;
         }
         const int ev = timed && *timed_count < b->kev_pairs;
+        const dim3 grid(devi_blocks(nw * NPH_WAVE, pb));
         if (ev) {
-            HIP_TRY(hipEventRecord(b->kev[2U * *timed_count], s));
-        }
-        hipLaunchKernelGGL(k_b_color_pass, dim3(devi_blocks(nw * NPH_WAVE, pb)), dim3(pb), 0, s,
-                           pl->tab[T_COLOR0 + c], nw, pl->ws, pl->ranges, c, *sp, pass);
-        if (ev) {
-            HIP_TRY(hipEventRecord(b->kev[2U * *timed_count + 1U], s));
+            // The dispatch packet itself stamps the kernel's start and end.
+            hipExtLaunchKernelGGL(k_b_color_pass, grid, dim3(pb), 0, s,
+                                  b->kev[2U * *timed_count], b->kev[2U * *timed_count + 1U], 0,
+                                  pl->tab[T_COLOR0 + c], nw, pl->ws, pl->ranges, c, *sp, pass);
             *timed_count += 1U;
+        } else {
+            hipLaunchKernelGGL(k_b_color_pass, grid, dim3(pb), 0, s, pl->tab[T_COLOR0 + c], nw,
+                               pl->ws, pl->ranges, c, *sp, pass);
         }
     }
     return NPH_RESULT_OK;

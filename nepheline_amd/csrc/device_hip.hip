@@ -17,6 +17,8 @@
 
 #include "device_impl.h"
 
+#include <hip/hip_ext.h>
+
 const char *nph_dev_name(void) { return "hip:gfx950"; }
 
 NphResult devi_grow_buffer(void **p, u64 old_bytes, u64 new_bytes, hipStream_t s, int keep)
@@ -1246,16 +1248,16 @@ NphResult nph_dev_solve(NphDevice *d, const NphSolveInput *in, const NphFastBody
             const u32 jc_ = joints_ ? jcount[c] : 0U;                                              \
             if (count[c] > 0U || jc_ > 0U) {                                                       \
                 const int ev_ = d->time_kernels && timed < d->kev_pairs;                           \
+                const dim3 grid_(blocks_for(((jc_ + 63U) & ~63U) + count[c] * NPH_LANES));         \
                 if (ev_) {                                                                         \
-                    HIP_TRY(hipEventRecord(d->kev[2U * timed], s));                                \
-                }                                                                                  \
-                hipLaunchKernelGGL(k_color_pass,                                                   \
-                                   dim3(blocks_for(((jc_ + 63U) & ~63U) + count[c] * NPH_LANES)),  \
-                                   dim3(NPH_BLOCK), 0, s, sd, sp, (u32)(kind), first[c], count[c],  \
-                                   jfirst[c], jc_);                                                \
-                if (ev_) {                                                                         \
-                    HIP_TRY(hipEventRecord(d->kev[2U * timed + 1U], s));                           \
+                    /* the dispatch packet itself stamps the kernel's start and end */             \
+                    hipExtLaunchKernelGGL(k_color_pass, grid_, dim3(NPH_BLOCK), 0, s,              \
+                                          d->kev[2U * timed], d->kev[2U * timed + 1U], 0, sd, sp,  \
+                                          (u32)(kind), first[c], count[c], jfirst[c], jc_);        \
                     timed += 1U;                                                                   \
+                } else {                                                                           \
+                    hipLaunchKernelGGL(k_color_pass, grid_, dim3(NPH_BLOCK), 0, s, sd, sp,         \
+                                       (u32)(kind), first[c], count[c], jfirst[c], jc_);           \
                 }                                                                                  \
             }                                                                                      \
         }                                                                                          \
