@@ -163,6 +163,37 @@ typedef struct NphPhysicsHull {
     u32 reserved[2];
 } NphPhysicsHull; // 128 B (hull.h:78)
 
+// The builder and the block queries (hull.h:41-170): host code, the same
+// bytes as the reference's for the same input.
+#define NPH_PHYSICS_HULL_MAX_VERTICES         256U
+#define NPH_PHYSICS_HULL_DEFAULT_MAX_VERTICES 64U
+
+typedef struct NphPhysicsHullDesc {
+    const NphVec3 *points;
+    u32 point_count;  // at least 4
+    u32 max_vertices; // 0: NPH_PHYSICS_HULL_DEFAULT_MAX_VERTICES; 4..256
+    f32 tolerance;    // 0: 3 * FLT_EPSILON * (max|x| + max|y| + max|z|)
+} NphPhysicsHullDesc; // hull.h:111-132
+
+// Quickhull over desc->points into one block from `alloc` (which also holds
+// the build's temporaries); built again at 4x, then 16x the tolerance when
+// the first hull fails validation. Replaces hull.h:135-136 (hull.c:1141-1197).
+NphResult nph_physics_hull_build(const NphPhysicsHullDesc *desc, NphAllocator *alloc,
+                                 NphPhysicsHull **out);
+// The box hull, made directly (hull.h:142-143, hull.c:321-346).
+NphResult nph_physics_hull_make_box(NphVec3 half_extents, NphAllocator *alloc,
+                                    NphPhysicsHull **out);
+// Releases a block of the two above; NULL is ignored (hull.h:147, hull.c:311-319).
+void nph_physics_hull_free(NphPhysicsHull *hull, NphAllocator *alloc);
+// The first vertex farthest along `direction` (hull.h:150, hull.c:1203-1216).
+u32 nph_physics_hull_support(const NphPhysicsHull *hull, NphVec3 direction);
+// The vertex count of face `face` (hull.h:153, hull.c:1218-1230).
+u32 nph_physics_hull_face_vertex_count(const NphPhysicsHull *hull, u32 face);
+// Checks a block of `size` bytes: layout, counts, Euler, header numbers,
+// twins, closed convex loops, every vertex behind every plane, within
+// `tolerance` (0: the hull's own). Replaces hull.h:166 (hull.c:1254-1377).
+bool nph_physics_hull_validate(const void *block, u32 size, f32 tolerance);
+
 // ---------------------------------------------------------------------------
 // Mesh blocks (physics/mesh.h:44-150): caller-owned, immutable, pointer-free;
 // the same bytes the reference's nph_physics_mesh_build writes (a four-wide

@@ -105,6 +105,47 @@ def bind_collide(a):
     return a
 
 
+class HullDesc(C.Structure):
+    """hull.h:111-132."""
+    _fields_ = [("points", C.c_void_p), ("point_count", C.c_uint32),
+                ("max_vertices", C.c_uint32), ("tolerance", C.c_float)]
+
+
+def bind_hull(a):
+    """The hull builder and block queries of either library (hull.h:135-166)."""
+    if hasattr(a, "hull_build"):
+        return a
+    L = a.lib
+    a.hull_build = _bind(L, "nph_physics_hull_build", C.c_int,
+                         [C.POINTER(HullDesc), C.c_void_p, C.POINTER(C.c_void_p)])
+    a.hull_make_box = _bind(L, "nph_physics_hull_make_box", C.c_int,
+                            [Vec3, C.c_void_p, C.POINTER(C.c_void_p)])
+    a.hull_free = _bind(L, "nph_physics_hull_free", None, [C.c_void_p, C.c_void_p])
+    a.hull_support = _bind(L, "nph_physics_hull_support", C.c_uint32, [C.c_void_p, Vec3])
+    a.hull_face_vertex_count = _bind(L, "nph_physics_hull_face_vertex_count", C.c_uint32,
+                                     [C.c_void_p, C.c_uint32])
+    a.hull_validate = _bind(L, "nph_physics_hull_validate", C.c_bool,
+                            [C.c_void_p, C.c_uint32, C.c_float])
+    return a
+
+
+def build_hull(a, points, max_vertices: int = 0, tolerance: float = 0.0):
+    """(result code, the block's bytes or None) of the library's
+    nph_physics_hull_build over an (n, 3) float32 array; the block is freed."""
+    import numpy as np
+    bind_hull(a)
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    d = HullDesc(pts.ctypes.data, int(pts.shape[0]), int(max_vertices), float(tolerance))
+    out = C.c_void_p()
+    rc = a.hull_build(C.byref(d), a.allocator(), C.byref(out))
+    if rc != 0:
+        return rc, None
+    size = C.cast(out, C.POINTER(C.c_uint32))[0]
+    block = C.string_at(out, size)
+    a.hull_free(out, a.allocator())
+    return rc, block
+
+
 class WorldDesc(C.Structure):
     _fields_ = [("alloc", C.c_void_p), ("gravity", Vec3), ("solver", C.c_int),
                 ("substeps", C.c_uint32), ("contact_hertz", C.c_float),

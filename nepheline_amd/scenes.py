@@ -106,6 +106,22 @@ def pyramid(api, rows: int = 20, desc: dict | None = None, world: P.World | None
     return w
 
 
+def hull_points(r: Pcg32, count: int):
+    """`count` points in a ball of radius U[0.3, 0.6]: a uniform direction
+    (two draws, z = 1 - 2u, phi = 2 pi v) times radius * sqrt(0.5 + 0.5 w)."""
+    radius = float(r.next_range(0.3, 0.6))
+    pts = []
+    for _ in range(count):
+        u = float(r.next_f32())
+        v = float(r.next_f32())
+        z = 1.0 - 2.0 * u
+        s = math.sqrt(max(0.0, 1.0 - z * z))
+        phi = 2.0 * math.pi * v
+        d = radius * math.sqrt(float(r.next_f32()) * 0.5 + 0.5)
+        pts.append((s * math.cos(phi) * d, s * math.sin(phi) * d, z * d))
+    return np.array(pts, dtype=np.float32)
+
+
 class HullBank:
     """Hull blocks (the reference's byte format) kept alive for the worlds
     that point at them."""

@@ -20,7 +20,6 @@ usage: python tests/golden/make_fixtures.py
 """
 import ctypes as C
 import json
-import math
 import os
 import struct
 import sys
@@ -41,22 +40,6 @@ class HullDesc(C.Structure):
                 ("max_vertices", C.c_uint32), ("tolerance", C.c_float)]
 
 
-def hull_points(r: S.Pcg32, count: int):
-    """`count` points in a ball of radius U[0.3, 0.6]: a uniform direction
-    (two draws, z = 1 - 2u, phi = 2 pi v) times radius * sqrt(0.5 + 0.5 w)."""
-    radius = float(r.next_range(0.3, 0.6))
-    pts = []
-    for _ in range(count):
-        u = float(r.next_f32())
-        v = float(r.next_f32())
-        z = 1.0 - 2.0 * u
-        s = math.sqrt(max(0.0, 1.0 - z * z))
-        phi = 2.0 * math.pi * v
-        d = radius * math.sqrt(float(r.next_f32()) * 0.5 + 0.5)
-        pts.append((s * math.cos(phi) * d, s * math.sin(phi) * d, z * d))
-    return np.array(pts, dtype=np.float32)
-
-
 def make_hull_bank(api, n=256, seed=7, verts=12):
     lib = api.lib
     build = lib.nph_physics_hull_build
@@ -65,7 +48,7 @@ def make_hull_bank(api, n=256, seed=7, verts=12):
     r = S.Pcg32(seed)
     blocks = []
     while len(blocks) < n:
-        pts = hull_points(r, verts)
+        pts = S.hull_points(r, verts)
         d = HullDesc(pts.ctypes.data, verts, 0, 0.0)
         out = C.c_void_p()
         if build(C.byref(d), api.allocator(), C.byref(out)) != 0:
