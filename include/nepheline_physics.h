@@ -234,6 +234,33 @@ typedef struct NphPhysicsMesh {
     NphAABB bounds;
 } NphPhysicsMesh; // 64 B (mesh.h:113)
 
+// The builder (mesh.h:154-201): host code, the same bytes as the
+// reference's for the same input.
+typedef struct NphPhysicsMeshDesc {
+    const NphVec3 *vertices;
+    u32 vertex_count;
+    const u32 *indices; // three per triangle
+    u32 triangle_count;
+    f32 weld_tolerance;    // 0: 1e-6 of the largest coordinate; negative: exact copies only
+    f32 active_edge_angle; // radians; 0: NPH_PHYSICS_MESH_DEFAULT_ACTIVE_ANGLE
+    b32 two_sided;
+} NphPhysicsMeshDesc; // mesh.h:158-176
+#define NPH_PHYSICS_MESH_DEFAULT_ACTIVE_ANGLE (5.0F * 3.14159265F / 180.0F)
+
+// Welds, drops degenerate triangles, marks active edges, builds the binned
+// SAH tree and collapses it four wide, into one block from `alloc`.
+// Replaces mesh.h:189-190 (mesh.c:675-798).
+NphResult nph_physics_mesh_build(const NphPhysicsMeshDesc *desc, NphAllocator *alloc,
+                                 NphPhysicsMesh **out);
+// Releases a block of nph_physics_mesh_build; NULL is ignored (mesh.h:193).
+void nph_physics_mesh_free(NphPhysicsMesh *mesh, NphAllocator *alloc);
+// Checks a block of `size` bytes: layout, counts, indices, flags, the tree
+// (order, boxes, leaves covering the triangles in order, depth). Replaces
+// mesh.h:205 (mesh.c:872-996).
+bool nph_physics_mesh_validate(const void *block, u32 size);
+// Triangle t's vertices in leaf order (mesh.h:151, mesh.c:34-41).
+void nph_physics_mesh_triangle(const NphPhysicsMesh *m, u32 t, NphVec3 out[3]);
+
 typedef struct NphPhysicsHeightfield NphPhysicsHeightfield; // heightfield.h (not supported)
 
 // ---------------------------------------------------------------------------

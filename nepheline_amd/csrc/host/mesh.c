@@ -1,0 +1,1082 @@
+// SPDX-License-Identifier: Apache-2.0
+// Mesh blocks on the host (engine/physics/include/nepheline/physics/mesh.h):
+// the builder and the block validator.
+//
+// The world step's device narrow phase walks a mesh block's four-wide tree
+// and reads its triangles in leaf order; triangle indices and active-edge
+// flags go into contact ids and normals. So the builder writes the
+// reference's bytes for the same input (mesh.c:190-798): vertices welded
+// into the lowest-numbered earlier representative within the tolerance,
+// triangles without three distinct vertices or without area dropped, edges
+// marked active as the reference marks them (open, shared by more than two,
+// back to back, or bent convex beyond the angle), a binned SAH tree (Wald
+// 2007; 16 bins on the centres' longest axis, median splits past depth 64,
+// leaves of at most four) built depth first with the left half first, and
+// its collapse into four-wide nodes by opening the largest inner child
+// (Dammertz et al. 2008). The code is this library's own.
+
+#include "world_impl.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+
+#define MNONE 0xFFFFFFFFu
+#define SAH_BINS 16U
+#define SAH_DEPTH 64U      // binned SAH above this binary depth, medians below
+#define CHECK_STACK 160U   // the validator's (and the queries') stack bound
+#define SLIVER_SIN_SQ 1e-12 // a triangle's squared sine below this has no area
+
+static void *mesh_take(NphAllocator *a, u64 size)
+{
+    NphCallsite cs = {__FILE__, (u32)__LINE__};
+    return a->vtable->alloc(a, size > 0U ? size : 16U, 16U, cs);
+}
+
+static void mesh_give(NphAllocator *a, void *p, u64 size)
+{
+    NphCallsite cs = {__FILE__, (u32)__LINE__};
+    if (p != NULL) {
+        a->vtable->free(a, p, size > 0U ? size : 16U, 16U, cs);
+    }
+}
+
+// f32 helpers in the reference's operation order (vec.h, geometry.h).
+static f32 m_min(f32 a, f32 b)
+{
+    return a < b ? a : b;
+}
+static f32 m_max(f32 a, f32 b)
+{
+    return a > b ? a : b;
+}
+static NphVec3 m_v3(f32 x, f32 y, f32 z)
+{
+    NphVec3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+static NphVec3 m_sub(NphVec3 a, NphVec3 b)
+{
+    return m_v3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+static NphVec3 m_vmin(NphVec3 a, NphVec3 b)
+{
+    return m_v3(m_min(a.x, b.x), m_min(a.y, b.y), m_min(a.z, b.z));
+}
+static NphVec3 m_vmax(NphVec3 a, NphVec3 b)
+{
+    return m_v3(m_max(a.x, b.x), m_max(a.y, b.y), m_max(a.z, b.z));
+}
+static f32 m_dot(NphVec3 a, NphVec3 b)
+{
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+static f32 m_at(NphVec3 v, u32 i)
+{
+    return i == 0U ? v.x : (i == 1U ? v.y : v.z);
+}
+static bool m_finite(NphVec3 v)
+{
+    return isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
+}
+// The largest |coordinate| within the limit (false for NaN).
+static bool m_in_range(NphVec3 v)
+{
+    return m_max(fabsf(v.x), m_max(fabsf(v.y), fabsf(v.z))) <= NPH_PHYSICS_MAX_COORDINATE;
+}
+// The unit normal of triangle (a, b, c), zero when it has none: the
+// normalisation of vec.h:325-355 (1 / sqrt, a rescale when the square
+// overflows).
+static NphVec3 m_unit_normal(NphVec3 a, NphVec3 b, NphVec3 c)
+{
+    const NphVec3 u = m_sub(b, a);
+    const NphVec3 v = m_sub(c, a);
+    const NphVec3 n = m_v3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+    const f32 sq = m_dot(n, n);
+    if (sq > 0.0F && sq <= FLT_MAX) {
+        const f32 s = 1.0F / sqrtf(sq);
+        return m_v3(n.x * s, n.y * s, n.z * s);
+    }
+    if (!(sq > FLT_MAX) || !m_finite(n)) {
+        return m_v3(0.0F, 0.0F, 0.0F);
+    }
+    const f32 m = m_max(fabsf(n.x), m_max(fabsf(n.y), fabsf(n.z)));
+    const f32 inv = 1.0F / m;
+    const NphVec3 w = m_v3(n.x * inv, n.y * inv, n.z * inv);
+    const f32 s = 1.0F / sqrtf(m_dot(w, w));
+    return m_v3(w.x * s, w.y * s, w.z * s);
+}
+static f32 box_area(NphAABB b)
+{
+    const NphVec3 d = m_sub(b.max, b.min);
+    return d.x * d.y + d.y * d.z + d.z * d.x;
+}
+static NphAABB box_join(NphAABB a, NphAABB b)
+{
+    NphAABB r;
+    r.min = m_vmin(a.min, b.min);
+    r.max = m_vmax(a.max, b.max);
+    return r;
+}
+static NphAABB box_none(void)
+{
+    NphAABB r;
+    r.min = m_v3(INFINITY, INFINITY, INFINITY);
+    r.max = m_v3(-INFINITY, -INFINITY, -INFINITY);
+    return r;
+}
+
+static u32 leaf_code(u32 first, u32 count)
+{
+    return NPH_PHYSICS_MESH_CHILD_LEAF | (first << 2U) | (count - 1U);
+}
+static bool is_leaf(u32 child)
+{
+    return child != NPH_PHYSICS_MESH_CHILD_EMPTY && (child & NPH_PHYSICS_MESH_CHILD_LEAF) != 0U;
+}
+
+// ---------------------------------------------------------------------------
+// Welding (mesh.c:165-249): a grid of cells 2 tol wide, each holding a chain
+// of the representatives in it; a vertex looks in the cells its tol-ball
+// meets and joins the lowest-numbered representative within tol.
+// ---------------------------------------------------------------------------
+typedef struct WeldCell {
+    i32 key[3];
+    u32 head; // MNONE: a free slot
+} WeldCell;
+
+typedef struct Weld {
+    WeldCell *cells;
+    u32 mask;
+} Weld;
+
+static u32 cell_hash(const i32 k[3])
+{
+    u64 h = (u64)(u32)k[0] * 0x9E3779B97F4A7C15ULL;
+    h ^= (u64)(u32)k[1] * 0xC2B2AE3D27D4EB4FULL;
+    h ^= (u64)(u32)k[2] * 0x165667B19E3779F9ULL;
+    h ^= h >> 29U;
+    h *= 0xBF58476D1CE4E5B9ULL;
+    h ^= h >> 32U;
+    return (u32)h;
+}
+
+static WeldCell *cell_find(Weld *w, const i32 k[3], bool insert)
+{
+    for (u32 i = cell_hash(k) & w->mask;; i = (i + 1U) & w->mask) {
+        WeldCell *c = &w->cells[i];
+        if (c->head == MNONE) {
+            if (!insert) {
+                return NULL;
+            }
+            memcpy(c->key, k, sizeof c->key);
+            return c;
+        }
+        if (memcmp(c->key, k, sizeof c->key) == 0) {
+            return c;
+        }
+    }
+}
+
+static void cell_at(NphVec3 p, f32 inv, i32 k[3])
+{
+    k[0] = (i32)floorf(p.x * inv);
+    k[1] = (i32)floorf(p.y * inv);
+    k[2] = (i32)floorf(p.z * inv);
+}
+
+// An exact copy's key: the coordinates' bits, -0 folded into +0.
+static void bits_at(NphVec3 p, i32 k[3])
+{
+    const f32 c[3] = {p.x + 0.0F, p.y + 0.0F, p.z + 0.0F};
+    memcpy(k, c, sizeof c);
+}
+
+typedef struct MeshWork {
+    NphAllocator *a;
+    u32 nv_in;
+    u32 nt_in;
+    u32 *remap;   // [nv_in] input vertex -> welded
+    u32 *chain;   // [nv_in] next representative in the same cell
+    NphVec3 *pos; // [nv_in] representatives
+    u32 *tris;    // [3 nt_in] kept triangles over representatives
+    u32 *orig;    // [nt_in]
+    u8 *flags;    // [nt_in] active edges
+    u64 *ekeys;   // [3 nt_in] edge sort keys
+    struct MItem *items;
+    struct MBin *bins;
+    u32 *stack; // [8 nt_in]
+    u32 *out_of; // [nv_in] representative -> output vertex
+    NphPhysicsMeshNode *nodes;
+} MeshWork;
+
+typedef struct MItem {
+    NphAABB box;
+    NphVec3 center;
+    u32 tri;
+} MItem;
+
+// A binary node: inner (count 0) or a leaf of `count` items from `first`.
+typedef struct MBin {
+    NphAABB box;
+    u32 left;
+    u32 right;
+    u32 first;
+    u32 count;
+} MBin;
+
+static u32 node_cap(u32 t)
+{
+    return (t > 2U ? t : 2U) - 1U;
+}
+
+static NphResult weld_vertices(MeshWork *s, const NphVec3 *v, f32 tol, u32 *out_count)
+{
+    u32 cap = 16U;
+    while ((u64)cap < 2ULL * s->nv_in && cap < 0x80000000U) {
+        cap *= 2U;
+    }
+    Weld w;
+    w.cells = (WeldCell *)mesh_take(s->a, (u64)cap * sizeof(WeldCell));
+    w.mask = cap - 1U;
+    if (w.cells == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    for (u32 i = 0U; i < cap; ++i) {
+        w.cells[i].head = MNONE;
+    }
+    const bool exact = !(tol > 0.0F);
+    const f32 inv = exact ? 0.0F : 0.5F / tol;
+    const f32 tol_sq = tol * tol;
+    u32 count = 0U;
+    for (u32 i = 0U; i < s->nv_in; ++i) {
+        const NphVec3 p = v[i];
+        u32 best = MNONE;
+        i32 k[3];
+        if (exact) {
+            bits_at(p, k);
+            const WeldCell *c = cell_find(&w, k, false);
+            if (c != NULL) {
+                best = c->head;
+            }
+        } else {
+            i32 lo[3];
+            i32 hi[3];
+            cell_at(m_v3(p.x - tol, p.y - tol, p.z - tol), inv, lo);
+            cell_at(m_v3(p.x + tol, p.y + tol, p.z + tol), inv, hi);
+            for (i32 x = lo[0]; x <= hi[0]; ++x) {
+                for (i32 y = lo[1]; y <= hi[1]; ++y) {
+                    for (i32 z = lo[2]; z <= hi[2]; ++z) {
+                        const i32 q[3] = {x, y, z};
+                        const WeldCell *c = cell_find(&w, q, false);
+                        for (u32 r = c != NULL ? c->head : MNONE; r != MNONE; r = s->chain[r]) {
+                            const NphVec3 d = m_sub(s->pos[r], p);
+                            if (r < best && m_dot(d, d) <= tol_sq) {
+                                best = r;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (best != MNONE) {
+            s->remap[i] = best;
+            continue;
+        }
+        const u32 rep = count++;
+        s->pos[rep] = p;
+        s->remap[i] = rep;
+        if (exact) {
+            bits_at(p, k);
+        } else {
+            cell_at(p, inv, k);
+        }
+        WeldCell *c = cell_find(&w, k, true);
+        s->chain[rep] = c->head; // MNONE for a new cell
+        c->head = rep;
+    }
+    mesh_give(s->a, w.cells, (u64)cap * sizeof(WeldCell));
+    *out_count = count;
+    return NPH_RESULT_OK;
+}
+
+// Keeps the triangles with three distinct representatives and an area
+// (mesh.c:261-294; the sine test in f64, where squared lengths cannot
+// overflow).
+static u32 keep_triangles(MeshWork *s, const u32 *idx)
+{
+    u32 kept = 0U;
+    for (u32 t = 0U; t < s->nt_in; ++t) {
+        const u32 a = s->remap[idx[3ULL * t]];
+        const u32 b = s->remap[idx[3ULL * t + 1U]];
+        const u32 c = s->remap[idx[3ULL * t + 2U]];
+        if (a == b || b == c || a == c) {
+            continue;
+        }
+        const NphVec3 pa = s->pos[a];
+        const NphVec3 pb = s->pos[b];
+        const NphVec3 pc = s->pos[c];
+        const f64 ux = (f64)pb.x - (f64)pa.x, uy = (f64)pb.y - (f64)pa.y, uz = (f64)pb.z - (f64)pa.z;
+        const f64 vx = (f64)pc.x - (f64)pa.x, vy = (f64)pc.y - (f64)pa.y, vz = (f64)pc.z - (f64)pa.z;
+        const f64 cx = uy * vz - uz * vy;
+        const f64 cy = uz * vx - ux * vz;
+        const f64 cz = ux * vy - uy * vx;
+        const f64 cross_sq = cx * cx + cy * cy + cz * cz;
+        const f64 lu = ux * ux + uy * uy + uz * uz;
+        const f64 lv = vx * vx + vy * vy + vz * vz;
+        if (!(cross_sq > SLIVER_SIN_SQ * lu * lv)) {
+            continue;
+        }
+        s->tris[3ULL * kept] = a;
+        s->tris[3ULL * kept + 1U] = b;
+        s->tris[3ULL * kept + 2U] = c;
+        s->orig[kept] = t;
+        kept += 1U;
+    }
+    return kept;
+}
+
+// Edge sort records: the undirected edge (lower, higher representative)
+// then the triangle and the edge's slot in it; equal edges are adjacent,
+// by triangle.
+typedef struct MEdge {
+    u64 key;
+    u32 tri;
+    u32 slot;
+} MEdge;
+
+static int edge_order(const void *pa, const void *pb)
+{
+    const MEdge *a = (const MEdge *)pa;
+    const MEdge *b = (const MEdge *)pb;
+    if (a->key != b->key) {
+        return a->key < b->key ? -1 : 1;
+    }
+    if (a->tri != b->tri) {
+        return a->tri < b->tri ? -1 : 1;
+    }
+    return a->slot < b->slot ? -1 : (a->slot > b->slot ? 1 : 0);
+}
+
+static NphVec3 tri_normal(const MeshWork *s, u32 t)
+{
+    const u32 *v = s->tris + 3ULL * t;
+    return m_unit_normal(s->pos[v[0]], s->pos[v[1]], s->pos[v[2]]);
+}
+
+// An edge two triangles share is active when they wind it the same way, meet
+// back to back, or bend beyond the angle (either way for a two-sided mesh,
+// convex only otherwise); flat and concave edges are not (mesh.c:313-338).
+static bool shared_edge_active(const MeshWork *s, const MEdge *e1, const MEdge *e2, f32 cos_limit,
+                               bool two_sided)
+{
+    const u32 *t1 = s->tris + 3ULL * e1->tri;
+    const u32 *t2 = s->tris + 3ULL * e2->tri;
+    if (t1[e1->slot] == t2[e2->slot]) {
+        return true;
+    }
+    const NphVec3 n1 = tri_normal(s, e1->tri);
+    const NphVec3 n2 = tri_normal(s, e2->tri);
+    const f32 cos_angle = m_dot(n1, n2);
+    if (cos_angle < -0.99985F) {
+        return true;
+    }
+    if (!(cos_angle < cos_limit)) {
+        return false;
+    }
+    if (two_sided) {
+        return true;
+    }
+    const NphVec3 start = s->pos[t1[e1->slot]];
+    const NphVec3 apex = s->pos[t2[(e2->slot + 2U) % 3U]];
+    return m_dot(n1, m_sub(apex, start)) < 0.0F;
+}
+
+static void mark_edges(MeshWork *s, u32 kept, f32 cos_limit, bool two_sided)
+{
+    MEdge *ek = (MEdge *)s->ekeys;
+    const u64 n = 3ULL * kept;
+    for (u32 t = 0U; t < kept; ++t) {
+        for (u32 e = 0U; e < 3U; ++e) {
+            const u32 a = s->tris[3ULL * t + e];
+            const u32 b = s->tris[3ULL * t + (e + 1U) % 3U];
+            MEdge *k = &ek[3ULL * t + e];
+            k->key = a < b ? ((u64)a << 32U) | b : ((u64)b << 32U) | a;
+            k->tri = t;
+            k->slot = e;
+        }
+    }
+    qsort(ek, (size_t)n, sizeof(MEdge), edge_order);
+    for (u64 i = 0U; i < n;) {
+        u64 j = i + 1U;
+        while (j < n && ek[j].key == ek[i].key) {
+            j += 1U;
+        }
+        if (j - i != 2U || shared_edge_active(s, &ek[i], &ek[i + 1U], cos_limit, two_sided)) {
+            for (u64 k = i; k < j; ++k) {
+                s->flags[ek[k].tri] |= (u8)(1U << ek[k].slot);
+            }
+        }
+        i = j;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The tree (mesh.c:376-649)
+// ---------------------------------------------------------------------------
+static u32 bin_index(const MItem *it, u32 axis, f32 lo, f32 scale)
+{
+    const f32 x = (m_at(it->center, axis) - lo) * scale;
+    if (!(x > 0.0F)) {
+        return 0U;
+    }
+    const u32 b = (u32)x;
+    return b < SAH_BINS - 1U ? b : SAH_BINS - 1U;
+}
+
+static int center_order_axis(const MItem *a, const MItem *b, u32 axis)
+{
+    const f32 x = m_at(a->center, axis);
+    const f32 y = m_at(b->center, axis);
+    if (x != y) {
+        return x < y ? -1 : 1;
+    }
+    return a->tri < b->tri ? -1 : (a->tri > b->tri ? 1 : 0);
+}
+static int center_order_x(const void *a, const void *b)
+{
+    return center_order_axis((const MItem *)a, (const MItem *)b, 0U);
+}
+static int center_order_y(const void *a, const void *b)
+{
+    return center_order_axis((const MItem *)a, (const MItem *)b, 1U);
+}
+static int center_order_z(const void *a, const void *b)
+{
+    return center_order_axis((const MItem *)a, (const MItem *)b, 2U);
+}
+
+// Splits items [lo, hi) (more than a leaf) and returns the split point, both
+// sides non-empty: the cheapest binned-SAH plane along the centres' longest
+// axis, or the median past SAH_DEPTH (sorted by centre, then triangle).
+static u32 split_items(MItem *items, u32 lo, u32 hi, u32 depth)
+{
+    const u32 count = hi - lo;
+    NphAABB cb;
+    cb.min = items[lo].center;
+    cb.max = items[lo].center;
+    for (u32 i = lo + 1U; i < hi; ++i) {
+        cb.min = m_vmin(cb.min, items[i].center);
+        cb.max = m_vmax(cb.max, items[i].center);
+    }
+    const NphVec3 ext = m_sub(cb.max, cb.min);
+    const u32 axis = ext.x >= ext.y ? (ext.x >= ext.z ? 0U : 2U) : (ext.y >= ext.z ? 1U : 2U);
+    const f32 axis_lo = m_at(cb.min, axis);
+    const f32 axis_len = m_at(ext, axis);
+    const u32 mid = lo + count / 2U;
+    if (!(axis_len > 0.0F)) {
+        return mid;
+    }
+    if (depth >= SAH_DEPTH) {
+        qsort(items + lo, count, sizeof(MItem),
+              axis == 0U ? center_order_x : (axis == 1U ? center_order_y : center_order_z));
+        return mid;
+    }
+    u32 n_in[SAH_BINS];
+    NphAABB b_box[SAH_BINS];
+    for (u32 b = 0U; b < SAH_BINS; ++b) {
+        n_in[b] = 0U;
+        b_box[b] = box_none();
+    }
+    const f32 scale = (f32)SAH_BINS / axis_len;
+    for (u32 i = lo; i < hi; ++i) {
+        const u32 b = bin_index(&items[i], axis, axis_lo, scale);
+        n_in[b] += 1U;
+        b_box[b] = box_join(b_box[b], items[i].box);
+    }
+    // Sweep from the right: the area and count of bins b.. for each b.
+    f32 r_area[SAH_BINS];
+    u32 r_count[SAH_BINS];
+    NphAABB acc = box_none();
+    u32 n = 0U;
+    for (u32 b = SAH_BINS - 1U; b > 0U; --b) {
+        acc = box_join(acc, b_box[b]);
+        n += n_in[b];
+        r_area[b] = n > 0U ? box_area(acc) : 0.0F;
+        r_count[b] = n;
+    }
+    acc = box_none();
+    n = 0U;
+    f32 best = FLT_MAX;
+    u32 plane = 0U;
+    for (u32 b = 0U; b + 1U < SAH_BINS; ++b) {
+        acc = box_join(acc, b_box[b]);
+        n += n_in[b];
+        if (n == 0U || r_count[b + 1U] == 0U) {
+            continue;
+        }
+        const f32 cost = box_area(acc) * (f32)n + r_area[b + 1U] * (f32)r_count[b + 1U];
+        if (cost < best) {
+            best = cost;
+            plane = b + 1U;
+        }
+    }
+    if (!(best < FLT_MAX)) {
+        return mid;
+    }
+    // Partition in place: left of the plane to the front, swapping each
+    // other item with the last unexamined one.
+    u32 i = lo;
+    u32 j = hi;
+    while (i < j) {
+        if (bin_index(&items[i], axis, axis_lo, scale) < plane) {
+            i += 1U;
+        } else {
+            j -= 1U;
+            const MItem t = items[i];
+            items[i] = items[j];
+            items[j] = t;
+        }
+    }
+    return i > lo && i < hi ? i : mid;
+}
+
+// The binary tree over items [0, count), depth first with the left half
+// built next (so its nodes and leaves come first); pending ranges wait on
+// an explicit stack of (lo, hi, parent link, depth).
+static void build_bins(MeshWork *s, u32 count)
+{
+    u32 made = 0U;
+    u32 *st = s->stack;
+    u32 top = 1U;
+    st[0] = 0U;
+    st[1] = count;
+    st[2] = MNONE;
+    st[3] = 0U;
+    while (top > 0U) {
+        top -= 1U;
+        const u32 lo = st[4U * top];
+        const u32 hi = st[4U * top + 1U];
+        const u32 link = st[4U * top + 2U];
+        const u32 depth = st[4U * top + 3U];
+        const u32 me = made++;
+        MBin *node = &s->bins[me];
+        node->box = s->items[lo].box;
+        for (u32 i = lo + 1U; i < hi; ++i) {
+            node->box = box_join(node->box, s->items[i].box);
+        }
+        if (link != MNONE) {
+            MBin *parent = &s->bins[link >> 1U];
+            if ((link & 1U) == 0U) {
+                parent->left = me;
+            } else {
+                parent->right = me;
+            }
+        }
+        if (hi - lo <= NPH_PHYSICS_MESH_LEAF_TRIANGLES) {
+            node->first = lo;
+            node->count = hi - lo;
+            node->left = MNONE;
+            node->right = MNONE;
+            continue;
+        }
+        const u32 mid = split_items(s->items, lo, hi, depth);
+        node->first = 0U;
+        node->count = 0U;
+        u32 *r = &st[4U * top];
+        r[0] = mid;
+        r[1] = hi;
+        r[2] = (me << 1U) | 1U;
+        r[3] = depth + 1U;
+        r[4] = lo;
+        r[5] = mid;
+        r[6] = me << 1U;
+        r[7] = depth + 1U;
+        top += 2U;
+    }
+}
+
+static void put_child(NphPhysicsMeshNode *n, u32 k, NphAABB b, u32 child)
+{
+    n->min_x[k] = b.min.x;
+    n->min_y[k] = b.min.y;
+    n->min_z[k] = b.min.z;
+    n->max_x[k] = b.max.x;
+    n->max_y[k] = b.max.y;
+    n->max_z[k] = b.max.z;
+    n->child[k] = child;
+}
+
+// An unused slot: the inverted box no query box overlaps.
+static void put_empty(NphPhysicsMeshNode *n, u32 k)
+{
+    NphAABB b;
+    b.min = m_v3(FLT_MAX, FLT_MAX, FLT_MAX);
+    b.max = m_v3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    put_child(n, k, b, NPH_PHYSICS_MESH_CHILD_EMPTY);
+}
+
+// Four-wide nodes from the binary tree: a node takes its two children and
+// opens the one of largest area among its inner children (first on ties)
+// until it holds four, keeping their order; nodes are numbered depth first
+// in child order. Returns the count.
+static u32 collapse_bins(const MeshWork *s, NphPhysicsMeshNode *out)
+{
+    const MBin *bins = s->bins;
+    if (bins[0].count > 0U) {
+        put_child(&out[0], 0U, bins[0].box, leaf_code(bins[0].first, bins[0].count));
+        for (u32 k = 1U; k < 4U; ++k) {
+            put_empty(&out[0], k);
+        }
+        return 1U;
+    }
+    u32 made = 1U;
+    u32 *st = s->stack; // (binary node, output node)
+    u32 top = 1U;
+    st[0] = 0U;
+    st[1] = 0U;
+    while (top > 0U) {
+        top -= 1U;
+        const u32 b = st[2U * top];
+        const u32 o = st[2U * top + 1U];
+        u32 kids[4] = {bins[b].left, bins[b].right, MNONE, MNONE};
+        u32 n = 2U;
+        while (n < 4U) {
+            u32 pick = MNONE;
+            f32 widest = -1.0F;
+            for (u32 k = 0U; k < n; ++k) {
+                const MBin *c = &bins[kids[k]];
+                if (c->count == 0U && box_area(c->box) > widest) {
+                    widest = box_area(c->box);
+                    pick = k;
+                }
+            }
+            if (pick == MNONE) {
+                break;
+            }
+            const MBin *c = &bins[kids[pick]];
+            for (u32 k = n; k > pick + 1U; --k) {
+                kids[k] = kids[k - 1U];
+            }
+            kids[pick] = c->left;
+            kids[pick + 1U] = c->right;
+            n += 1U;
+        }
+        u32 code[4];
+        for (u32 k = 0U; k < n; ++k) {
+            const MBin *c = &bins[kids[k]];
+            code[k] = c->count > 0U ? leaf_code(c->first, c->count) : made++;
+            put_child(&out[o], k, c->box, code[k]);
+        }
+        for (u32 k = n; k < 4U; ++k) {
+            put_empty(&out[o], k);
+        }
+        for (u32 k = n; k > 0U; --k) {
+            if (!is_leaf(code[k - 1U])) {
+                st[2U * top] = kids[k - 1U];
+                st[2U * top + 1U] = code[k - 1U];
+                top += 1U;
+            }
+        }
+    }
+    return made;
+}
+
+// ---------------------------------------------------------------------------
+// Building (mesh.c:651-798)
+// ---------------------------------------------------------------------------
+static bool mesh_desc_ok(const NphPhysicsMeshDesc *d)
+{
+    if (d == NULL || d->vertices == NULL || d->indices == NULL || d->vertex_count < 3U ||
+        d->triangle_count == 0U || d->triangle_count > NPH_PHYSICS_MESH_MAX_TRIANGLES ||
+        !isfinite(d->weld_tolerance) || !isfinite(d->active_edge_angle) ||
+        d->active_edge_angle < 0.0F || d->active_edge_angle > 3.14159265F) {
+        return false;
+    }
+    for (u32 i = 0U; i < d->vertex_count; ++i) {
+        if (!m_finite(d->vertices[i]) || !m_in_range(d->vertices[i])) {
+            return false;
+        }
+    }
+    for (u64 i = 0U; i < 3ULL * d->triangle_count; ++i) {
+        if (d->indices[i] >= d->vertex_count) {
+            return false;
+        }
+    }
+    return true;
+}
+
+static void work_release(MeshWork *s)
+{
+    NphAllocator *a = s->a;
+    const u64 n = s->nv_in;
+    const u64 t = s->nt_in;
+    mesh_give(a, s->remap, n * 4U);
+    mesh_give(a, s->chain, n * 4U);
+    mesh_give(a, s->pos, n * sizeof(NphVec3));
+    mesh_give(a, s->tris, 3U * t * 4U);
+    mesh_give(a, s->orig, t * 4U);
+    mesh_give(a, s->flags, t);
+    mesh_give(a, s->ekeys, 3U * t * sizeof(MEdge));
+    mesh_give(a, s->items, t * sizeof(MItem));
+    mesh_give(a, s->bins, 2U * t * sizeof(MBin));
+    mesh_give(a, s->stack, 8U * t * 4U);
+    mesh_give(a, s->out_of, n * 4U);
+    mesh_give(a, s->nodes, (u64)node_cap(s->nt_in) * sizeof(NphPhysicsMeshNode));
+}
+
+static NphResult work_reserve(MeshWork *s)
+{
+    NphAllocator *a = s->a;
+    const u64 n = s->nv_in;
+    const u64 t = s->nt_in;
+    s->remap = (u32 *)mesh_take(a, n * 4U);
+    s->chain = (u32 *)mesh_take(a, n * 4U);
+    s->pos = (NphVec3 *)mesh_take(a, n * sizeof(NphVec3));
+    s->tris = (u32 *)mesh_take(a, 3U * t * 4U);
+    s->orig = (u32 *)mesh_take(a, t * 4U);
+    s->flags = (u8 *)mesh_take(a, t);
+    s->ekeys = (u64 *)mesh_take(a, 3U * t * sizeof(MEdge));
+    s->items = (MItem *)mesh_take(a, t * sizeof(MItem));
+    s->bins = (MBin *)mesh_take(a, 2U * t * sizeof(MBin));
+    s->stack = (u32 *)mesh_take(a, 8U * t * 4U);
+    s->out_of = (u32 *)mesh_take(a, n * 4U);
+    s->nodes = (NphPhysicsMeshNode *)mesh_take(a, (u64)node_cap(s->nt_in) *
+                                                       sizeof(NphPhysicsMeshNode));
+    if (s->remap == NULL || s->chain == NULL || s->pos == NULL || s->tris == NULL ||
+        s->orig == NULL || s->flags == NULL || s->ekeys == NULL || s->items == NULL ||
+        s->bins == NULL || s->stack == NULL || s->out_of == NULL || s->nodes == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    memset(s->flags, 0, (size_t)t);
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_physics_mesh_build(const NphPhysicsMeshDesc *desc, NphAllocator *alloc,
+                                 NphPhysicsMesh **out)
+{
+    if (alloc == NULL || out == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    *out = NULL;
+    if (!mesh_desc_ok(desc)) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    f32 extent = 0.0F;
+    for (u32 i = 0U; i < desc->vertex_count; ++i) {
+        const NphVec3 p = desc->vertices[i];
+        extent = m_max(extent, m_max(fabsf(p.x), m_max(fabsf(p.y), fabsf(p.z))));
+    }
+    // At least 2^-29 of the extent, so cell indices fit 32 bits.
+    f32 tol = desc->weld_tolerance;
+    if (tol == 0.0F) {
+        tol = 1e-6F * extent;
+    }
+    if (tol > 0.0F) {
+        tol = m_max(tol, extent * 1.8626451e-9F);
+    }
+    const f32 angle = desc->active_edge_angle > 0.0F ? desc->active_edge_angle
+                                                     : NPH_PHYSICS_MESH_DEFAULT_ACTIVE_ANGLE;
+    const bool two_sided = desc->two_sided != 0;
+
+    MeshWork s;
+    memset(&s, 0, sizeof s);
+    s.a = alloc;
+    s.nv_in = desc->vertex_count;
+    s.nt_in = desc->triangle_count;
+    NphResult r = work_reserve(&s);
+    u32 reps = 0U;
+    if (r == NPH_RESULT_OK) {
+        r = weld_vertices(&s, desc->vertices, tol, &reps);
+    }
+    const u32 kept = r == NPH_RESULT_OK ? keep_triangles(&s, desc->indices) : 0U;
+    if (r == NPH_RESULT_OK && kept == 0U) {
+        r = NPH_RESULT_INVALID_ARGUMENT;
+    }
+    if (r != NPH_RESULT_OK) {
+        work_release(&s);
+        return r;
+    }
+    mark_edges(&s, kept, cosf(angle), two_sided);
+    for (u32 t = 0U; t < kept; ++t) {
+        const u32 *v = s.tris + 3ULL * t;
+        MItem *it = &s.items[t];
+        it->box.min = m_vmin(m_vmin(s.pos[v[0]], s.pos[v[1]]), s.pos[v[2]]);
+        it->box.max = m_vmax(m_vmax(s.pos[v[0]], s.pos[v[1]]), s.pos[v[2]]);
+        it->center = m_v3((it->box.min.x + it->box.max.x) * 0.5F,
+                          (it->box.min.y + it->box.max.y) * 0.5F,
+                          (it->box.min.z + it->box.max.z) * 0.5F);
+        it->tri = t;
+    }
+    build_bins(&s, kept);
+    // Output vertices: the used representatives, numbered by first use in
+    // leaf order.
+    for (u32 i = 0U; i < reps; ++i) {
+        s.out_of[i] = MNONE;
+    }
+    u32 nv = 0U;
+    for (u32 i = 0U; i < kept; ++i) {
+        const u32 *v = s.tris + 3ULL * s.items[i].tri;
+        for (u32 k = 0U; k < 3U; ++k) {
+            if (s.out_of[v[k]] == MNONE) {
+                s.out_of[v[k]] = nv++;
+            }
+        }
+    }
+    const u32 nn = collapse_bins(&s, s.nodes);
+    const u64 o_nodes = sizeof(NphPhysicsMesh);
+    const u64 o_verts = o_nodes + (u64)nn * sizeof(NphPhysicsMeshNode);
+    const u64 o_tris = o_verts + (u64)nv * sizeof(NphVec3);
+    const u64 o_edges = o_tris + 12ULL * kept;
+    const u64 o_orig = (o_edges + kept + 3U) & ~3ULL;
+    const u64 size = o_orig + 4ULL * kept;
+    if (size > 0xFFFFFFFFULL) {
+        work_release(&s);
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    NphPhysicsMesh *m = (NphPhysicsMesh *)mesh_take(alloc, size);
+    if (m == NULL) {
+        work_release(&s);
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    memset(m, 0, (size_t)size);
+    memcpy((u8 *)m + o_nodes, s.nodes, (size_t)nn * sizeof(NphPhysicsMeshNode));
+    NphVec3 *verts = (NphVec3 *)((u8 *)m + o_verts);
+    u32 *tris = (u32 *)((u8 *)m + o_tris);
+    u8 *edges = (u8 *)m + o_edges;
+    u32 *orig = (u32 *)((u8 *)m + o_orig);
+    for (u32 i = 0U; i < reps; ++i) {
+        if (s.out_of[i] != MNONE) {
+            verts[s.out_of[i]] = s.pos[i];
+        }
+    }
+    NphAABB bounds;
+    bounds.min = verts[0];
+    bounds.max = verts[0];
+    for (u32 i = 1U; i < nv; ++i) {
+        bounds.min = m_vmin(bounds.min, verts[i]);
+        bounds.max = m_vmax(bounds.max, verts[i]);
+    }
+    for (u32 i = 0U; i < kept; ++i) {
+        const u32 t = s.items[i].tri;
+        for (u32 k = 0U; k < 3U; ++k) {
+            tris[3ULL * i + k] = s.out_of[s.tris[3ULL * t + k]];
+        }
+        edges[i] = s.flags[t];
+        orig[i] = s.orig[t];
+    }
+    m->byte_size = (u32)size;
+    m->vertex_count = nv;
+    m->triangle_count = kept;
+    m->node_count = nn;
+    m->nodes_offset = (u32)o_nodes;
+    m->vertices_offset = (u32)o_verts;
+    m->triangles_offset = (u32)o_tris;
+    m->edges_offset = (u32)o_edges;
+    m->originals_offset = (u32)o_orig;
+    m->flags = two_sided ? NPH_PHYSICS_MESH_TWO_SIDED : 0U;
+    m->bounds = bounds;
+    work_release(&s);
+    *out = m;
+    return NPH_RESULT_OK;
+}
+
+void nph_physics_mesh_free(NphPhysicsMesh *mesh, NphAllocator *alloc)
+{
+    if (mesh != NULL && alloc != NULL) {
+        mesh_give(alloc, mesh, mesh->byte_size);
+    }
+}
+
+void nph_physics_mesh_triangle(const NphPhysicsMesh *m, u32 t, NphVec3 out[3])
+{
+    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
+    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset) + 3ULL * t;
+    out[0] = v[tri[0]];
+    out[1] = v[tri[1]];
+    out[2] = v[tri[2]];
+}
+
+// ---------------------------------------------------------------------------
+// Validation (mesh.c:829-996): the same checks, so a block is accepted
+// exactly when the reference accepts it.
+// ---------------------------------------------------------------------------
+#define MREJECT(...)                                                                               \
+    do {                                                                                           \
+        fprintf(stderr, "nepheline_amd: mesh validate: " __VA_ARGS__);                             \
+        fputc('\n', stderr);                                                                       \
+        return false;                                                                              \
+    } while (0)
+
+typedef struct MPending {
+    u32 child;
+    u32 parent;
+    u32 slot;
+    u32 depth;
+} MPending;
+
+static bool span_ok(u64 offset, u64 count, u64 elem, u64 align, u64 size)
+{
+    return offset >= sizeof(NphPhysicsMesh) && offset % align == 0U && count <= size &&
+           offset + count * elem <= size;
+}
+
+static bool slot_holds_point(const NphPhysicsMeshNode *n, u32 k, NphVec3 p)
+{
+    return p.x >= n->min_x[k] && p.x <= n->max_x[k] && p.y >= n->min_y[k] && p.y <= n->max_y[k] &&
+           p.z >= n->min_z[k] && p.z <= n->max_z[k];
+}
+
+static bool slot_holds_node(const NphPhysicsMeshNode *n, u32 k, const NphPhysicsMeshNode *c)
+{
+    for (u32 j = 0U; j < 4U; ++j) {
+        if (c->child[j] == NPH_PHYSICS_MESH_CHILD_EMPTY) {
+            continue;
+        }
+        if (c->min_x[j] < n->min_x[k] || c->max_x[j] > n->max_x[k] || c->min_y[j] < n->min_y[k] ||
+            c->max_y[j] > n->max_y[k] || c->min_z[j] < n->min_z[k] || c->max_z[j] > n->max_z[k]) {
+            return false;
+        }
+    }
+    return true;
+}
+
+static bool slot_is_empty_box(const NphPhysicsMeshNode *n, u32 k)
+{
+    return n->min_x[k] == FLT_MAX && n->min_y[k] == FLT_MAX && n->min_z[k] == FLT_MAX &&
+           n->max_x[k] == -FLT_MAX && n->max_y[k] == -FLT_MAX && n->max_z[k] == -FLT_MAX;
+}
+
+bool nph_physics_mesh_validate(const void *block, u32 size)
+{
+    const u64 head = sizeof(NphPhysicsMesh);
+    if (block == NULL || size < head || ((uintptr_t)block & 15U) != 0U) {
+        MREJECT("no 16-byte aligned header in %u bytes", size);
+    }
+    const NphPhysicsMesh *m = (const NphPhysicsMesh *)block;
+    const u64 bytes = m->byte_size;
+    if (bytes < head || bytes > size) {
+        MREJECT("byte_size %u outside the %u-byte block", m->byte_size, size);
+    }
+    const u32 nt = m->triangle_count;
+    const u32 nv = m->vertex_count;
+    const u32 nn = m->node_count;
+    if (nt == 0U || nt > NPH_PHYSICS_MESH_MAX_TRIANGLES || nv < 3U || nn == 0U ||
+        nn > node_cap(nt) || (m->flags & ~(u32)NPH_PHYSICS_MESH_TWO_SIDED) != 0U) {
+        MREJECT("counts triangles=%u vertices=%u nodes=%u flags=%u", nt, nv, nn, m->flags);
+    }
+    if (!span_ok(m->nodes_offset, nn, sizeof(NphPhysicsMeshNode), 16U, bytes) ||
+        !span_ok(m->vertices_offset, nv, sizeof(NphVec3), 4U, bytes) ||
+        !span_ok(m->triangles_offset, 3ULL * nt, 4U, 4U, bytes) ||
+        !span_ok(m->edges_offset, nt, 1U, 1U, bytes) ||
+        !span_ok(m->originals_offset, nt, 4U, 4U, bytes)) {
+        MREJECT("arrays outside the %u-byte block", m->byte_size);
+    }
+    if (!m_in_range(m->bounds.min) || !m_in_range(m->bounds.max)) {
+        MREJECT("bounds not finite or beyond the coordinate limit");
+    }
+    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
+    const NphAABB bx = m->bounds;
+    for (u32 i = 0U; i < nv; ++i) {
+        const NphVec3 p = v[i];
+        if (!m_finite(p) || !(p.x >= bx.min.x && p.x <= bx.max.x && p.y >= bx.min.y &&
+                              p.y <= bx.max.y && p.z >= bx.min.z && p.z <= bx.max.z)) {
+            MREJECT("vertex %u not finite or outside the bounds", i);
+        }
+    }
+    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset);
+    const u8 *edges = (const u8 *)m + m->edges_offset;
+    for (u64 i = 0U; i < 3ULL * nt; ++i) {
+        if (tri[i] >= nv) {
+            MREJECT("triangle %u's vertex %u out of range", (u32)(i / 3U), tri[i]);
+        }
+    }
+    for (u32 t = 0U; t < nt; ++t) {
+        if ((edges[t] & ~(u32)NPH_PHYSICS_MESH_EDGES) != 0U) {
+            MREJECT("triangle %u's edge flags %u", t, (u32)edges[t]);
+        }
+    }
+    // Depth first in child order: each node once, after its parent; leaves
+    // covering the triangles in order; every box holding what it contains;
+    // no deeper than a query's stack (three siblings a level plus four).
+    const NphPhysicsMeshNode *nodes =
+        (const NphPhysicsMeshNode *)((const u8 *)m + m->nodes_offset);
+    MPending st[CHECK_STACK];
+    u32 top = 0U;
+    st[top].child = 0U;
+    st[top].parent = MNONE;
+    st[top].slot = 0U;
+    st[top].depth = 0U;
+    top += 1U;
+    u32 seen = 0U;
+    u32 next_tri = 0U;
+    while (top > 0U) {
+        const MPending e = st[--top];
+        if (is_leaf(e.child)) {
+            const u32 first = (e.child & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
+            const u32 count = (e.child & 3U) + 1U;
+            if (first != next_tri || (u64)first + count > nt) {
+                MREJECT("leaf %u+%u out of order", first, count);
+            }
+            for (u32 t = first; t < first + count; ++t) {
+                for (u32 j = 0U; j < 3U; ++j) {
+                    if (!slot_holds_point(&nodes[e.parent], e.slot, v[tri[3ULL * t + j]])) {
+                        MREJECT("triangle %u outside its leaf's box", t);
+                    }
+                }
+            }
+            next_tri = first + count;
+            continue;
+        }
+        const u32 idx = e.child;
+        if (idx >= nn || (e.parent != MNONE && (idx <= e.parent || !slot_holds_node(
+                                                                      &nodes[e.parent], e.slot,
+                                                                      &nodes[idx])))) {
+            MREJECT("node %u out of order or outside its parent's box", idx);
+        }
+        seen += 1U;
+        if (seen > nn) {
+            MREJECT("a node reached twice");
+        }
+        const NphPhysicsMeshNode *n = &nodes[idx];
+        u32 used = 0U;
+        for (u32 k = 0U; k < 4U; ++k) {
+            if (n->child[k] == NPH_PHYSICS_MESH_CHILD_EMPTY) {
+                if (!slot_is_empty_box(n, k)) {
+                    MREJECT("node %u: empty slot %u's box is not the empty box", idx, k);
+                }
+                continue;
+            }
+            if (used != k) {
+                MREJECT("node %u: a child after an empty slot", idx);
+            }
+            const NphVec3 lo = m_v3(n->min_x[k], n->min_y[k], n->min_z[k]);
+            const NphVec3 hi = m_v3(n->max_x[k], n->max_y[k], n->max_z[k]);
+            if (!m_in_range(lo) || !m_in_range(hi) || !(lo.x <= hi.x) || !(lo.y <= hi.y) ||
+                !(lo.z <= hi.z)) {
+                MREJECT("node %u: child %u's box inverted or not finite", idx, k);
+            }
+            used += 1U;
+        }
+        if (used == 0U) {
+            MREJECT("node %u is empty", idx);
+        }
+        if (3U * e.depth + 4U > CHECK_STACK || top + used > CHECK_STACK) {
+            MREJECT("deeper than the query stack");
+        }
+        for (u32 k = used; k > 0U; --k) {
+            st[top].child = n->child[k - 1U];
+            st[top].parent = idx;
+            st[top].slot = k - 1U;
+            st[top].depth = e.depth + 1U;
+            top += 1U;
+        }
+    }
+    if (seen != nn || next_tri != nt) {
+        MREJECT("%u of %u nodes reached, %u of %u triangles in leaves", seen, nn, next_tri, nt);
+    }
+    return true;
+}

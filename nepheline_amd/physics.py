@@ -129,10 +129,49 @@ def bind_hull(a):
     return a
 
 
+class MeshDesc(C.Structure):
+    """mesh.h:158-176."""
+    _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint32),
+                ("indices", C.c_void_p), ("triangle_count", C.c_uint32),
+                ("weld_tolerance", C.c_float), ("active_edge_angle", C.c_float),
+                ("two_sided", C.c_int32)]
+
+
+def bind_mesh(a):
+    """The mesh builder and validator of either library (mesh.h:189-205)."""
+    if hasattr(a, "mesh_build"):
+        return a
+    L = a.lib
+    a.mesh_build = _bind(L, "nph_physics_mesh_build", C.c_int,
+                         [C.POINTER(MeshDesc), C.c_void_p, C.POINTER(C.c_void_p)])
+    a.mesh_free = _bind(L, "nph_physics_mesh_free", None, [C.c_void_p, C.c_void_p])
+    a.mesh_validate = _bind(L, "nph_physics_mesh_validate", C.c_bool, [C.c_void_p, C.c_uint32])
+    return a
+
+
+def build_mesh(a, vertices, indices, weld_tolerance: float = 0.0, active_edge_angle: float = 0.0,
+               two_sided: bool = False):
+    """(result code, the block's bytes or None) of the library's
+    nph_physics_mesh_build over float32 (n, 3) vertices and uint32 (m, 3)
+    indices; the block is freed."""
+    bind_mesh(a)
+    v = np.ascontiguousarray(vertices, dtype=np.float32)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    d = MeshDesc(v.ctypes.data, int(v.shape[0]), idx.ctypes.data, int(idx.size // 3),
+                 float(weld_tolerance), float(active_edge_angle), int(bool(two_sided)))
+    out = C.c_void_p()
+    rc = a.mesh_build(C.byref(d), a.allocator(), C.byref(out))
+    if rc != 0:
+        return rc, None
+    size = C.cast(out, C.POINTER(C.c_uint32))[0]
+    block = C.string_at(out, size)
+    a.mesh_free(out, a.allocator())
+    return rc, block
+
+
 def build_hull(a, points, max_vertices: int = 0, tolerance: float = 0.0):
     """(result code, the block's bytes or None) of the library's
     nph_physics_hull_build over an (n, 3) float32 array; the block is freed."""
-    import numpy as np
     bind_hull(a)
     pts = np.ascontiguousarray(points, dtype=np.float32)
     d = HullDesc(pts.ctypes.data, int(pts.shape[0]), int(max_vertices), float(tolerance))

@@ -35,9 +35,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
 
 
-class HullDesc(C.Structure):
-    _fields_ = [("points", C.c_void_p), ("point_count", C.c_uint32),
-                ("max_vertices", C.c_uint32), ("tolerance", C.c_float)]
+HullDesc = P.HullDesc
 
 
 def make_hull_bank(api, n=256, seed=7, verts=12):
@@ -69,11 +67,7 @@ def make_hull_bank(api, n=256, seed=7, verts=12):
     return len(data)
 
 
-class MeshDesc(C.Structure):
-    _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint32),
-                ("indices", C.c_void_p), ("triangle_count", C.c_uint32),
-                ("weld_tolerance", C.c_float), ("active_edge_angle", C.c_float),
-                ("two_sided", C.c_int32)]
+MeshDesc = P.MeshDesc
 
 
 def make_mesh_bank(api, seed=31):
