@@ -1,0 +1,84 @@
+// SPDX-License-Identifier: Apache-2.0
+// The narrow phase for convex pairs with a hull of more than 32 vertices:
+// the same collide code as the step's kernels (nph_collide.h), built here
+// with its private arrays sized for the reference's largest hull (256
+// vertices, hull.h:45) and its clip capacity (256 + 8, collide_hull.c:26),
+// so faces clip to the reference's points. The step's convex kernels skip
+// these pairs (NPH_SHAPE_BIG_HULL on a shape, `big` on a query) and these
+// kernels take only them, so each pair runs in exactly one. Their frames
+// are larger (15 KB a thread of private memory), which is why the common
+// hulls' kernels keep the 32-vertex build. Mesh pairs need none of this:
+// against a triangle the per-shape arrays hold the triangle, and a clipped
+// face of at most 64 vertices (the host's limit for hull faces) fits the
+// 32-vertex build's 68 points.
+#define NPH_HULL_MAX_VERTICES 256U
+#define NPH_CLIP_CAPACITY     264U
+
+#include "device_impl.h"
+
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide_big(NphContactRec *contacts, const NphShapeRec *shapes,
+                              const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || contacts[list[i]].child != NPH_NULL32 ||
+        !contact_has_big_hull(&contacts[list[i]], shapes)) {
+        return;
+    }
+    contact_update(contacts, list[i], shapes, bodies, step_dt);
+    codes[i] = contacts[list[i]].change;
+}
+
+__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide_big(const NphSortedEntry *__restrict__ sorted, u32 n,
+                                                      const NphCollideWorld *__restrict__ cw,
+                                                      u32 *codes, f32 step_dt)
+{
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) {
+        return;
+    }
+    const NphSortedEntry e = sorted[j];
+    const NphCollideWorld c = cw[e.world];
+    if (c.contacts[e.record].child != NPH_NULL32 ||
+        !contact_has_big_hull(&c.contacts[e.record], c.shapes)) {
+        return;
+    }
+    contact_update(c.contacts, e.record, c.shapes, c.bodies, step_dt);
+    codes[e.index] = c.contacts[e.record].change;
+}
+
+__global__ void __launch_bounds__(64) k_query_collide_big(const NphQueryPair *pairs, u32 n,
+                                                          NphManifold *out, NphCache *caches)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || pairs[i].big == 0U) {
+        return;
+    }
+    const NphQueryPair p = pairs[i];
+    NphCache cache = p.cache;
+    NphManifold m;
+    nph_collide(&p.a, p.pose_a, &p.b, p.pose_b, p.speculative, &cache, &m);
+    out[i] = m;
+    caches[i] = cache;
+}
+
+void devi_collide_big(hipStream_t s, NphContactRec *contacts, const NphShapeRec *shapes,
+                      const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt)
+{
+    hipLaunchKernelGGL(k_collide_big, dim3(devi_blocks(n, NPH_COLLIDE_BLOCK)),
+                       dim3(NPH_COLLIDE_BLOCK), 0, s, contacts, shapes, bodies, list, codes, n,
+                       step_dt);
+}
+
+void devi_b_collide_big(hipStream_t s, const NphSortedEntry *sorted, u32 n,
+                        const NphCollideWorld *cw, u32 *codes, f32 step_dt)
+{
+    hipLaunchKernelGGL(k_b_collide_big, dim3(devi_blocks(n, NPH_COLLIDE_BLOCK)),
+                       dim3(NPH_COLLIDE_BLOCK), 0, s, sorted, n, cw, codes, step_dt);
+}
+
+void devi_query_collide_big(hipStream_t s, const NphQueryPair *pairs, u32 n, NphManifold *out,
+                            NphCache *caches)
+{
+    hipLaunchKernelGGL(k_query_collide_big, dim3(devi_blocks(n, 64U)), dim3(64), 0, s, pairs, n,
+                       out, caches);
+}

@@ -69,13 +69,6 @@ typedef struct NphLearnWorld {
 enum { ACC_SUM_PLAIN = 0, ACC_SUM_LEARNED, ACC_COUNT_PLAIN, ACC_COUNT_LEARNED, ACC_MAX_PLAIN,
        ACC_MAX_LEARNED, ACC_MAX_PEN, ACC_GATED, ACC_WORDS };
 
-typedef struct NphCollideWorld {
-    NphContactRec *contacts;
-    const NphShapeRec *shapes;
-    const NphBodyRec *bodies;
-    u32 *gate;
-} NphCollideWorld;
-
 // A pinned host buffer and its device twin, filled on the host and sent up
 // with one copy.
 typedef struct Stage {
@@ -91,8 +84,6 @@ typedef struct Half {
     u32 cap_codes;
     u32 list_count;
 } Half;
-
-typedef struct NphSortedEntry NphSortedEntry;
 
 enum {
     T_SLOTS = 0,  // body slots (the solver map)
@@ -289,13 +280,6 @@ __global__ void __launch_bounds__(256) k_b_init(Tab tab, u32 nw,
 // only on its own pair, and its change code goes back to its list index.
 #define NPH_PAIR_KEYS 64U
 
-typedef struct NphSortedEntry {
-    u32 world;
-    u32 record;
-    u32 index; // the entry's place in the half's list (its change code's)
-    u32 pad;
-} NphSortedEntry;
-
 __device__ static inline u32 pair_key_of(const NphCollideWorld &c, u32 rec)
 {
     const NphContactRec *cr = &c.contacts[rec];
@@ -402,6 +386,9 @@ __global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide(const NphSorted
     const NphCollideWorld c = cw[e.world];
     if (c.contacts[e.record].child != NPH_NULL32) {
         return; // a mesh pair: k_b_collide_mesh
+    }
+    if (contact_has_big_hull(&c.contacts[e.record], c.shapes)) {
+        return; // a hull beyond 32 vertices: k_b_collide_big
     }
     contact_update(c.contacts, e.record, c.shapes, c.bodies, step_dt);
     codes[e.index] = c.contacts[e.record].change;
@@ -1084,6 +1071,7 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
     }
     u64 inits = 0U, list = 0U, iw = 0U, lw = 0U;
     int mesh = 0;
+    int big = 0;
     for (u32 k = 0U; k < n; ++k) {
         const NphCollidePart *p = &parts[k];
         if (p->count < p->first) {
@@ -1094,6 +1082,7 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
         iw += waves_of(p->init_count);
         lw += waves_of(p->count - p->first);
         mesh |= p->dev->mesh_shapes > 0U;
+        big |= p->dev->big_hull_shapes > 0U;
     }
     if (list > 0xFFFFFFF0ULL || inits > 0xFFFFFFF0ULL) {
         return NPH_RESULT_OUT_OF_MEMORY;
@@ -1186,6 +1175,9 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
             hipLaunchKernelGGL(k_b_collide_mesh, dim3(devi_blocks(m, NPH_COLLIDE_BLOCK)),
                                dim3(NPH_COLLIDE_BLOCK), 0, s, b->sorted, m, dcw, h->codes,
                                step_dt);
+        }
+        if (big) {
+            devi_b_collide_big(s, b->sorted, m, dcw, h->codes, step_dt);
         }
     }
     HIP_TRY(hipGetLastError());

@@ -531,6 +531,9 @@ __global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide(NphContactRec *co
     if (contacts[list[i]].child != NPH_NULL32) {
         return; // a mesh pair: k_collide_mesh
     }
+    if (contact_has_big_hull(&contacts[list[i]], shapes)) {
+        return; // a hull beyond 32 vertices: k_collide_big
+    }
     contact_update(contacts, list[i], shapes, bodies, step_dt);
     codes[i] = contacts[list[i]].change;
 }
@@ -1028,12 +1031,18 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
                            dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes,
                            d->bodies, d->list + first, d->codes + first, n, step_dt);
     }
+    if (d->big_hull_shapes > 0U) {
+        devi_collide_big(d->stream, d->contacts, d->shapes, d->bodies, d->list + first,
+                         d->codes + first, n, step_dt);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, d->stream));
     return NPH_RESULT_OK;
 }
 
 void nph_dev_note_mesh_shape(NphDevice *d) { d->mesh_shapes += 1U; }
+
+void nph_dev_note_big_hull_shape(NphDevice *d) { d->big_hull_shapes += 1U; }
 
 NphResult nph_dev_collide_begin(NphDevice *d, const u32 *awake, u32 count, f32 step_dt)
 {

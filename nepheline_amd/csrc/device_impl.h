@@ -143,6 +143,7 @@ struct NphDevice {
     u32 *log_count;
     u32 log_cap;
     u32 mesh_shapes;     // mesh shapes made: the mesh pairs' kernel runs
+    u32 big_hull_shapes; // shapes of hulls beyond 32 vertices: the large-hull kernels run
     u64 learn_hash;      // FNV-1a of the network's parameters and settings
 };
 
@@ -160,6 +161,43 @@ NphResult devi_create(NphDevice **out, hipStream_t stream);
 NphResult devi_flush_staged(NphDevice *d);
 
 static inline u32 devi_blocks(u32 n, u32 block) { return (n + block - 1U) / block; }
+
+// A world's narrow-phase records, for the batched narrow phase.
+typedef struct NphCollideWorld {
+    NphContactRec *contacts;
+    const NphShapeRec *shapes;
+    const NphBodyRec *bodies;
+    u32 *gate;
+} NphCollideWorld;
+
+// One contact of the batched narrow phase's list, sorted by pair type.
+typedef struct NphSortedEntry {
+    u32 world;
+    u32 record;
+    u32 index; // the entry's place in the half's list (its change code's)
+    u32 pad;
+} NphSortedEntry;
+
+// One pair of the standalone collision queries.
+typedef struct NphQueryPair {
+    NphGeom a;
+    NphGeom b;
+    NphPhysicsPose pose_a;
+    NphPhysicsPose pose_b;
+    f32 speculative;
+    u32 big; // a hull beyond 32 vertices: the large-hull kernel takes it
+    NphCache cache;
+} NphQueryPair;
+
+// The large-hull narrow phase (collide_big.hip: the same code built for
+// hulls of up to 256 vertices): the convex pairs the 32-vertex kernels
+// skip. Each launches on `s` and returns at once.
+void devi_collide_big(hipStream_t s, NphContactRec *contacts, const NphShapeRec *shapes,
+                      const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt);
+void devi_b_collide_big(hipStream_t s, const NphSortedEntry *sorted, u32 n,
+                        const NphCollideWorld *cw, u32 *codes, f32 step_dt);
+void devi_query_collide_big(hipStream_t s, const NphQueryPair *pairs, u32 n, NphManifold *out,
+                            NphCache *caches);
 
 enum { PASS_WARM = 0, PASS_SOLVE = 1, PASS_RELAX = 2, PASS_RESTITUTION = 3 };
 

@@ -770,6 +770,10 @@ static NphShapeRec shape_record(NphPhysicsWorld *w, u32 s, const void *device_bl
     r.body = h->body;
     r.next = h->next;
     r.combine = h->combine;
+    if (h->geometry.type == NPH_PHYSICS_SHAPE_HULL &&
+        h->geometry.hull->vertex_count > NPH_HULL_MAX_VERTICES) {
+        r.flags = NPH_SHAPE_BIG_HULL;
+    }
     return r;
 }
 
@@ -835,9 +839,16 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
     if (desc->sensor != 0 || desc->ghost != 0) {
         return NPH_RESULT_UNSUPPORTED; // sensors and ghosts: not on this path
     }
-    if (desc->geometry.type == NPH_PHYSICS_SHAPE_HULL &&
-        desc->geometry.hull->vertex_count > NPH_HULL_MAX_VERTICES) {
-        return NPH_RESULT_UNSUPPORTED;
+    if (desc->geometry.type == NPH_PHYSICS_SHAPE_HULL) {
+        // A clipped face against a mesh triangle must fit the narrow
+        // phase's 68 points, as the reference's 264 hold it (faces of at
+        // most 64 vertices; any number of vertices up to 256).
+        const NphPhysicsHull *h = desc->geometry.hull;
+        for (u32 f = 0U; f < h->face_count; ++f) {
+            if (nph_physics_hull_face_vertex_count(h, f) > NPH_HULL_MAX_FACE_VERTICES) {
+                return NPH_RESULT_UNSUPPORTED;
+            }
+        }
     }
     // Continuous sweeps against mesh triangles (continuous.c:211-300) are
     // not built: a world that sweeps takes no meshes.
@@ -910,6 +921,9 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
         r = nph_dev_write_shape(w->dev, slot, &rec);
         if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH) {
             nph_dev_note_mesh_shape(w->dev);
+        }
+        if ((rec.flags & NPH_SHAPE_BIG_HULL) != 0U) {
+            nph_dev_note_big_hull_shape(w->dev);
         }
     }
     if (r == NPH_RESULT_OK && hb->type != NPH_PHYSICS_BODY_STATIC) {

@@ -12,15 +12,26 @@
 //   GJK distance                gjk.c:26-416 (signed volumes, Montanari 2017)
 //
 // Everything is one thread's work over private arrays sized for hulls of at
-// most NPH_HULL_MAX_VERTICES vertices; the host rejects larger hulls.
+// most NPH_HULL_MAX_VERTICES vertices. The step's kernels are built at 32
+// (the arrays of the common hulls stay small); pairs with a larger hull run
+// in kernels built from the same code at the reference's 256 with its clip
+// capacity of 264 (collide_hull.c:25-26). At 32 a clipped face holds at
+// most 32 + 32 points, so 68 never truncates, as 264 never does there; a
+// mesh pair clips a triangle and a hull face, at most 64 + 3 points when the
+// hull's faces have at most 64 vertices, so mesh pairs of any hull of such
+// faces keep the 32-vertex build too.
 #ifndef NPH_COLLIDE_H
 #define NPH_COLLIDE_H
 
 #include "nph_geometry.h"
 
+#ifndef NPH_HULL_MAX_VERTICES
 #define NPH_HULL_MAX_VERTICES 32U
-#define NPH_HULL_MAX_FACES    (2U * NPH_HULL_MAX_VERTICES)
 #define NPH_CLIP_CAPACITY     68U
+#endif
+#define NPH_HULL_MAX_FACES    (2U * NPH_HULL_MAX_VERTICES)
+// The largest hull face a world takes (see the mesh pairs above).
+#define NPH_HULL_MAX_FACE_VERTICES 64U
 
 #define NPH_FACE_BIAS         (0.1F * NPH_PHYSICS_LINEAR_SLOP)
 #define NPH_EDGE_BIAS         (0.5F * NPH_PHYSICS_LINEAR_SLOP)

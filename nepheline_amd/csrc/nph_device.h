@@ -62,6 +62,9 @@ NphResult nph_dev_reserve(NphDevice *d, u32 bodies, u32 shapes, u32 contacts);
 NphResult nph_dev_block(NphDevice *d, const void *block, u32 bytes, const void **out);
 // A mesh shape exists: the narrow phase runs its mesh-pair kernel too.
 void nph_dev_note_mesh_shape(NphDevice *d);
+// A shape of a hull beyond 32 vertices was made: the large-hull narrow
+// phase runs from now on.
+void nph_dev_note_big_hull_shape(NphDevice *d);
 NphResult nph_dev_write_body(NphDevice *d, u32 slot, const NphBodyRec *b);
 // Joint records by slot: the configuration (when conf is not NULL) and, with
 // reset, accumulated impulses from zero; staged as the body writes are.

@@ -63,7 +63,9 @@ typedef struct NphBodyRec {
     u32 enlarged;   // set by finalize when a shape left its fat box
 } NphBodyRec;
 
-enum { NPH_SHAPE_ENLARGED = 1U };
+// NPH_SHAPE_BIG_HULL: a hull of more vertices than the step's kernels are
+// built for; its convex pairs run in the large-hull kernels.
+enum { NPH_SHAPE_ENLARGED = 1U, NPH_SHAPE_BIG_HULL = 2U };
 
 typedef struct NphShapeRec {
     NphGeom geom;
@@ -75,7 +77,7 @@ typedef struct NphShapeRec {
     u32 body;
     u32 next; // the body's next shape
     u32 combine; // friction_combine | restitution_combine << 8
-    u32 flags;   // NPH_SHAPE_ENLARGED
+    u32 flags;   // NPH_SHAPE_ENLARGED | NPH_SHAPE_BIG_HULL
 } NphShapeRec;
 
 typedef struct NphPointRec {
@@ -732,6 +734,13 @@ NPH_HD void contact_update_mesh(NphContactRec *contacts, u32 index, const NphSha
         return;
     }
     mesh_update(contacts, index, shapes, bodies, step_dt);
+}
+
+// Whether a contact's pair has a hull beyond the step kernels' capacity
+// (the large-hull kernels take it).
+NPH_HD int contact_has_big_hull(const NphContactRec *c, const NphShapeRec *shapes)
+{
+    return ((shapes[c->shape_a].flags | shapes[c->shape_b].flags) & NPH_SHAPE_BIG_HULL) != 0U;
 }
 
 // contact.c:516-577 for a convex pair: the disjoint test, the manifold in

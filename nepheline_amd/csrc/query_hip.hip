@@ -16,22 +16,12 @@
 
 namespace {
 
-typedef struct QueryPair {
-    NphGeom a;
-    NphGeom b;
-    NphPhysicsPose pose_a;
-    NphPhysicsPose pose_b;
-    f32 speculative;
-    u32 pad;
-    NphCache cache;
-} QueryPair;
-
 struct QueryContext {
     std::mutex lock;
     int ready = 0;
     int failed = 0;
     hipStream_t stream = NULL;
-    QueryPair *pairs = NULL;
+    NphQueryPair *pairs = NULL;
     NphManifold *manifolds = NULL;
     NphCache *caches = NULL;
     u32 cap = 0U;
@@ -106,6 +96,11 @@ NphResult query_geom(QueryContext *q, const NphPhysicsGeometry *g, NphGeom *out)
     return NPH_RESULT_OK;
 }
 
+bool big_hull(const NphPhysicsGeometry *g)
+{
+    return g->type == NPH_PHYSICS_SHAPE_HULL && g->hull->vertex_count > NPH_HULL_MAX_VERTICES;
+}
+
 int convex(NphPhysicsShapeType t)
 {
     return t == NPH_PHYSICS_SHAPE_SPHERE || t == NPH_PHYSICS_SHAPE_CAPSULE ||
@@ -114,14 +109,14 @@ int convex(NphPhysicsShapeType t)
 
 } // namespace
 
-__global__ void __launch_bounds__(64) k_query_collide(const QueryPair *pairs, u32 n,
+__global__ void __launch_bounds__(64) k_query_collide(const NphQueryPair *pairs, u32 n,
                                                       NphManifold *out, NphCache *caches)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) {
-        return;
+    if (i >= n || pairs[i].big != 0U) {
+        return; // a hull beyond 32 vertices: the large-hull kernel
     }
-    const QueryPair p = pairs[i];
+    const NphQueryPair p = pairs[i];
     NphCache cache = p.cache;
     NphManifold m;
     nph_collide(&p.a, p.pose_a, &p.b, p.pose_b, p.speculative, &cache, &m);
@@ -175,15 +170,16 @@ NphResult nph_amd_collide_many(const NphAmdCollideQuery *queries, u32 n, NphPhys
         q->manifolds = NULL;
         q->caches = NULL;
         q->cap = 0U;
-        HIP_TRY(hipMalloc((void **)&q->pairs, (u64)c * sizeof(QueryPair)));
+        HIP_TRY(hipMalloc((void **)&q->pairs, (u64)c * sizeof(NphQueryPair)));
         HIP_TRY(hipMalloc((void **)&q->manifolds, (u64)c * sizeof(NphManifold)));
         HIP_TRY(hipMalloc((void **)&q->caches, (u64)c * sizeof(NphCache)));
         q->cap = c;
     }
-    QueryPair *h = (QueryPair *)malloc((size_t)n * sizeof(QueryPair));
+    NphQueryPair *h = (NphQueryPair *)malloc((size_t)n * sizeof(NphQueryPair));
     if (h == NULL) {
         return NPH_RESULT_OUT_OF_MEMORY;
     }
+    bool any_big = false;
     for (u32 i = 0U; i < n && r == NPH_RESULT_OK; ++i) {
         const NphAmdCollideQuery *x = &queries[i];
         memset(&h[i], 0, sizeof h[i]);
@@ -198,16 +194,21 @@ NphResult nph_amd_collide_many(const NphAmdCollideQuery *queries, u32 n, NphPhys
         h[i].pose_a = x->pose_a;
         h[i].pose_b = x->pose_b;
         h[i].speculative = x->speculative_distance;
+        h[i].big = big_hull(&x->a) || big_hull(&x->b) ? 1U : 0U;
+        any_big |= h[i].big != 0U;
         if (caches != NULL) {
             memcpy(&h[i].cache, &caches[i], sizeof h[i].cache);
         }
     }
     if (r == NPH_RESULT_OK) {
-        const hipError_t e1 = hipMemcpyAsync(q->pairs, h, (u64)n * sizeof(QueryPair),
+        const hipError_t e1 = hipMemcpyAsync(q->pairs, h, (u64)n * sizeof(NphQueryPair),
                                              hipMemcpyHostToDevice, q->stream);
         if (e1 == hipSuccess) {
             hipLaunchKernelGGL(k_query_collide, dim3(devi_blocks(n, 64U)), dim3(64), 0, q->stream,
                                q->pairs, n, q->manifolds, q->caches);
+            if (any_big) {
+                devi_query_collide_big(q->stream, q->pairs, n, q->manifolds, q->caches);
+            }
         }
         const hipError_t e2 = e1 == hipSuccess ? hipGetLastError() : e1;
         const hipError_t e3 =

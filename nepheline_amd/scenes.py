@@ -135,6 +135,33 @@ class HullBank:
         return len(self.ptrs)
 
 
+def built_hull_bank(api, vertex_counts, seed: int = 5) -> HullBank:
+    """Hulls built by the library's own nph_physics_hull_build: for each
+    budget, the hull of 4 * budget points on a sphere of radius U[0.3, 0.6]
+    (slightly jittered so faces stay triangles), the budget filled."""
+    r = Pcg32(seed)
+    blocks = []
+    for n in vertex_counts:
+        radius = float(r.next_range(0.3, 0.6))
+        pts = []
+        for _ in range(4 * n):
+            u, v = float(r.next_f32()), float(r.next_f32())
+            z = 1.0 - 2.0 * u
+            sxy = math.sqrt(max(0.0, 1.0 - z * z))
+            phi = 2.0 * math.pi * v
+            d = radius * (1.0 + 0.01 * float(r.next_f32()))
+            pts.append((sxy * math.cos(phi) * d, sxy * math.sin(phi) * d, z * d))
+        rc, block = P.build_hull(api, np.array(pts, dtype=np.float32), n)
+        if rc != 0:
+            raise RuntimeError(f"hull build failed: {rc}")
+        blocks.append(block)
+    offsets, blob = [], b""
+    for b in blocks:
+        offsets.append(len(blob))
+        blob += b + b"\0" * ((16 - len(b) % 16) % 16)
+    return HullBank(blob, offsets)
+
+
 def load_hull_bank(path: str) -> HullBank:
     """A bank file: u32 count, u32 offsets[count], then 16-byte aligned blocks."""
     data = open(path, "rb").read()
