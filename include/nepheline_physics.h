@@ -261,6 +261,25 @@ bool nph_physics_mesh_validate(const void *block, u32 size);
 // Triangle t's vertices in leaf order (mesh.h:151, mesh.c:34-41).
 void nph_physics_mesh_triangle(const NphPhysicsMesh *m, u32 t, NphVec3 out[3]);
 
+// The mesh's box and ray queries (mesh.h:207-227), host code.
+typedef bool (*NphPhysicsMeshVisit)(void *user, u32 triangle);
+// Visits the triangles whose boxes overlap `box`, in tree order, until a
+// visit returns false. Replaces mesh.h:213-214 (mesh.c:1013-1049).
+void nph_physics_mesh_overlap(const NphPhysicsMesh *m, NphAABB box, NphPhysicsMeshVisit visit,
+                              void *user);
+typedef struct NphPhysicsMeshHit {
+    f32 fraction;
+    NphVec3 normal; // towards the side the ray came from
+    u32 triangle;   // in leaf order
+    u32 original;   // the input's index
+    b32 back;       // hit from behind (two-sided meshes only)
+} NphPhysicsMeshHit; // mesh.h:216-222
+// The closest triangle the segment origin -> origin + delta crosses
+// (watertight test; one-sided meshes from the front only). Replaces
+// mesh.h:224-225 (mesh.c:1104-1192).
+bool nph_physics_mesh_raycast(const NphPhysicsMesh *m, NphVec3 origin, NphVec3 delta,
+                              NphPhysicsMeshHit *hit);
+
 typedef struct NphPhysicsHeightfield NphPhysicsHeightfield; // heightfield.h (not supported)
 
 // ---------------------------------------------------------------------------

@@ -1080,3 +1080,236 @@ bool nph_physics_mesh_validate(const void *block, u32 size)
     }
     return true;
 }
+
+// ---------------------------------------------------------------------------
+// Queries (mesh.c:998-1199): the box overlap in tree order, and the closest
+// ray hit by the watertight test of Woop, Benthin and Wald ("Watertight
+// Ray/Triangle Intersection", JCGT 2013), nearest child first.
+// ---------------------------------------------------------------------------
+void nph_physics_mesh_overlap(const NphPhysicsMesh *m, NphAABB box, NphPhysicsMeshVisit visit,
+                              void *user)
+{
+    const NphPhysicsMeshNode *nodes =
+        (const NphPhysicsMeshNode *)((const u8 *)m + m->nodes_offset);
+    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset);
+    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
+    u32 st[CHECK_STACK];
+    u32 top = 0U;
+    st[top++] = 0U;
+    while (top > 0U) {
+        const u32 code = st[--top];
+        if (!is_leaf(code)) {
+            const NphPhysicsMeshNode *n = &nodes[code];
+            // Pushed last to first, so the children come off in order; an
+            // empty slot's inverted box overlaps nothing.
+            for (u32 k = 4U; k > 0U; --k) {
+                const u32 j = k - 1U;
+                if (n->min_x[j] <= box.max.x && n->max_x[j] >= box.min.x &&
+                    n->min_y[j] <= box.max.y && n->max_y[j] >= box.min.y &&
+                    n->min_z[j] <= box.max.z && n->max_z[j] >= box.min.z &&
+                    n->child[j] != NPH_PHYSICS_MESH_CHILD_EMPTY && top < CHECK_STACK) {
+                    st[top++] = n->child[j];
+                }
+            }
+            continue;
+        }
+        const u32 first = (code & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
+        const u32 last = first + (code & 3U) + 1U;
+        for (u32 t = first; t < last; ++t) {
+            const NphVec3 a = v[tri[3ULL * t]];
+            const NphVec3 b = v[tri[3ULL * t + 1U]];
+            const NphVec3 c = v[tri[3ULL * t + 2U]];
+            const NphVec3 lo = m_vmin(m_vmin(a, b), c);
+            const NphVec3 hi = m_vmax(m_vmax(a, b), c);
+            if (lo.x <= box.max.x && hi.x >= box.min.x && lo.y <= box.max.y &&
+                hi.y >= box.min.y && lo.z <= box.max.z && hi.z >= box.min.z && !visit(user, t)) {
+                return;
+            }
+        }
+    }
+}
+
+// The ray in Woop et al.'s frame: kz the axis of the direction's largest
+// magnitude, kx and ky the next two (swapped for a negative dz, keeping the
+// winding), and the shear that makes the ray run along +z.
+typedef struct WRay {
+    NphVec3 origin;
+    u32 kx, ky, kz;
+    f32 sx, sy, sz;
+} WRay;
+
+static WRay wray_make(NphVec3 origin, NphVec3 delta)
+{
+    WRay r;
+    r.origin = origin;
+    const f32 ax = fabsf(delta.x), ay = fabsf(delta.y), az = fabsf(delta.z);
+    r.kz = ax >= ay ? (ax >= az ? 0U : 2U) : (ay >= az ? 1U : 2U);
+    r.kx = (r.kz + 1U) % 3U;
+    r.ky = (r.kx + 1U) % 3U;
+    const f32 dz = m_at(delta, r.kz);
+    if (dz < 0.0F) {
+        const u32 t = r.kx;
+        r.kx = r.ky;
+        r.ky = t;
+    }
+    r.sx = m_at(delta, r.kx) / dz;
+    r.sy = m_at(delta, r.ky) / dz;
+    r.sz = 1.0F / dz;
+    return r;
+}
+
+// The fraction where the ray crosses triangle (pa, pb, pc), negative for a
+// miss; *back when it crosses from behind. An edge through the ray (a zero
+// barycentric) is decided in f64, so two triangles sharing it never both
+// miss (Woop et al., section 3).
+static f32 wray_hit(const WRay *r, NphVec3 pa, NphVec3 pb, NphVec3 pc, bool *back)
+{
+    const NphVec3 a = m_sub(pa, r->origin);
+    const NphVec3 b = m_sub(pb, r->origin);
+    const NphVec3 c = m_sub(pc, r->origin);
+    const f32 az = m_at(a, r->kz), bz = m_at(b, r->kz), cz = m_at(c, r->kz);
+    const f32 ax = m_at(a, r->kx) - r->sx * az;
+    const f32 ay = m_at(a, r->ky) - r->sy * az;
+    const f32 bx = m_at(b, r->kx) - r->sx * bz;
+    const f32 by = m_at(b, r->ky) - r->sy * bz;
+    const f32 cx = m_at(c, r->kx) - r->sx * cz;
+    const f32 cy = m_at(c, r->ky) - r->sy * cz;
+    f32 u = cx * by - cy * bx;
+    f32 v = ax * cy - ay * cx;
+    f32 w = bx * ay - by * ax;
+    if (u == 0.0F || v == 0.0F || w == 0.0F) {
+        u = (f32)((f64)cx * (f64)by - (f64)cy * (f64)bx);
+        v = (f32)((f64)ax * (f64)cy - (f64)ay * (f64)cx);
+        w = (f32)((f64)bx * (f64)ay - (f64)by * (f64)ax);
+    }
+    if ((u < 0.0F || v < 0.0F || w < 0.0F) && (u > 0.0F || v > 0.0F || w > 0.0F)) {
+        return -1.0F;
+    }
+    const f32 det = u + v + w;
+    if (det == 0.0F) {
+        return -1.0F;
+    }
+    const f32 t = (u * r->sz * az + v * r->sz * bz + w * r->sz * cz) / det;
+    if (!(t >= 0.0F && t <= 1.0F)) {
+        return -1.0F;
+    }
+    *back = det < 0.0F;
+    return t;
+}
+
+// A finite reciprocal: an axis-parallel ray gets +-1e30, whose products
+// stay ordered.
+static f32 recip_finite(f32 d)
+{
+    if (fabsf(d) < 1e-30F) {
+        return d < 0.0F ? -1e30F : 1e30F;
+    }
+    return 1.0F / d;
+}
+
+// The slab test of the ray against a node's four child boxes: each child's
+// entry fraction, FLT_MAX for a miss (enter after leave, or beyond `limit`).
+static void slab_enter(const NphPhysicsMeshNode *n, NphVec3 o, NphVec3 inv, f32 limit,
+                       f32 out[4])
+{
+    for (u32 k = 0U; k < 4U; ++k) {
+        const f32 t1x = (n->min_x[k] - o.x) * inv.x;
+        const f32 t2x = (n->max_x[k] - o.x) * inv.x;
+        const f32 t1y = (n->min_y[k] - o.y) * inv.y;
+        const f32 t2y = (n->max_y[k] - o.y) * inv.y;
+        const f32 t1z = (n->min_z[k] - o.z) * inv.z;
+        const f32 t2z = (n->max_z[k] - o.z) * inv.z;
+        const f32 enter = m_max(m_max(m_min(t1x, t2x), m_min(t1y, t2y)),
+                                m_max(m_min(t1z, t2z), 0.0F));
+        const f32 leave = m_min(m_min(m_max(t1x, t2x), m_max(t1y, t2y)),
+                                m_min(m_max(t1z, t2z), limit));
+        out[k] = enter <= leave ? enter : FLT_MAX;
+    }
+}
+
+bool nph_physics_mesh_raycast(const NphPhysicsMesh *m, NphVec3 origin, NphVec3 delta,
+                              NphPhysicsMeshHit *hit)
+{
+    if (!(m_dot(delta, delta) > 0.0F)) {
+        return false;
+    }
+    const NphPhysicsMeshNode *nodes =
+        (const NphPhysicsMeshNode *)((const u8 *)m + m->nodes_offset);
+    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset);
+    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
+    const bool two_sided = (m->flags & NPH_PHYSICS_MESH_TWO_SIDED) != 0U;
+    const WRay ray = wray_make(origin, delta);
+    const NphVec3 inv = m_v3(recip_finite(delta.x), recip_finite(delta.y), recip_finite(delta.z));
+    f32 best = FLT_MAX;
+    u32 best_tri = MNONE;
+    bool best_back = false;
+    u32 st[CHECK_STACK];
+    f32 st_enter[CHECK_STACK];
+    u32 top = 0U;
+    st[top] = 0U;
+    st_enter[top] = 0.0F;
+    top += 1U;
+    while (top > 0U) {
+        top -= 1U;
+        const u32 code = st[top];
+        if (st_enter[top] > best) {
+            continue;
+        }
+        if (is_leaf(code)) {
+            const u32 first = (code & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
+            const u32 last = first + (code & 3U) + 1U;
+            for (u32 t = first; t < last; ++t) {
+                bool back = false;
+                const f32 f = wray_hit(&ray, v[tri[3ULL * t]], v[tri[3ULL * t + 1U]],
+                                       v[tri[3ULL * t + 2U]], &back);
+                if (f >= 0.0F && f < best && (two_sided || !back)) {
+                    best = f; // equal fractions keep the first found
+                    best_tri = t;
+                    best_back = back;
+                }
+            }
+            continue;
+        }
+        const NphPhysicsMeshNode *n = &nodes[code];
+        f32 enter[4];
+        slab_enter(n, origin, inv, m_min(best, 1.0F), enter);
+        for (u32 k = 0U; k < 4U; ++k) {
+            if (n->child[k] == NPH_PHYSICS_MESH_CHILD_EMPTY) {
+                enter[k] = FLT_MAX;
+            }
+        }
+        // Farthest first onto the stack (an insertion sort, later children
+        // after earlier ones on ties), so the nearest comes off next.
+        u32 order[4] = {0U, 1U, 2U, 3U};
+        for (u32 i = 1U; i < 4U; ++i) {
+            for (u32 j = i; j > 0U && enter[order[j]] > enter[order[j - 1U]]; --j) {
+                const u32 t = order[j];
+                order[j] = order[j - 1U];
+                order[j - 1U] = t;
+            }
+        }
+        for (u32 i = 0U; i < 4U; ++i) {
+            const u32 k = order[i];
+            if (enter[k] < FLT_MAX && enter[k] <= best && top < CHECK_STACK) {
+                st[top] = n->child[k];
+                st_enter[top] = enter[k];
+                top += 1U;
+            }
+        }
+    }
+    if (best_tri == MNONE) {
+        return false;
+    }
+    NphVec3 p[3];
+    nph_physics_mesh_triangle(m, best_tri, p);
+    NphVec3 nrm = m_unit_normal(p[0], p[1], p[2]);
+    if (best_back) {
+        nrm = m_v3(-nrm.x, -nrm.y, -nrm.z);
+    }
+    hit->fraction = best;
+    hit->normal = nrm;
+    hit->triangle = best_tri;
+    hit->original = ((const u32 *)((const u8 *)m + m->originals_offset))[best_tri];
+    hit->back = best_back ? 1 : 0;
+    return true;
+}

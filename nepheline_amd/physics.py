@@ -146,7 +146,35 @@ def bind_mesh(a):
                          [C.POINTER(MeshDesc), C.c_void_p, C.POINTER(C.c_void_p)])
     a.mesh_free = _bind(L, "nph_physics_mesh_free", None, [C.c_void_p, C.c_void_p])
     a.mesh_validate = _bind(L, "nph_physics_mesh_validate", C.c_bool, [C.c_void_p, C.c_uint32])
+    a.mesh_overlap = _bind(L, "nph_physics_mesh_overlap", None,
+                           [C.c_void_p, AABB, MESH_VISIT, C.c_void_p])
+    a.mesh_raycast = _bind(L, "nph_physics_mesh_raycast", C.c_bool,
+                           [C.c_void_p, Vec3, Vec3, C.POINTER(MeshHit)])
     return a
+
+
+# mesh.h:207-222: the overlap's visit and the ray's hit.
+MESH_VISIT = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_uint32)
+
+
+class MeshHit(C.Structure):
+    _fields_ = [("fraction", C.c_float), ("normal", Vec3), ("triangle", C.c_uint32),
+                ("original", C.c_uint32), ("back", C.c_int32)]
+
+
+def mesh_overlap(a, mesh_ptr, lo, hi, stop_after=None):
+    """The triangles nph_physics_mesh_overlap visits for the box (lo, hi),
+    in order; with `stop_after`, the visit returns false at that count."""
+    bind_mesh(a)
+    seen = []
+
+    def visit(_user, t):
+        seen.append(int(t))
+        return stop_after is None or len(seen) < stop_after
+
+    cb = MESH_VISIT(visit)
+    a.mesh_overlap(mesh_ptr, AABB(Vec3(*lo), Vec3(*hi)), cb, None)
+    return seen
 
 
 def build_mesh(a, vertices, indices, weld_tolerance: float = 0.0, active_edge_angle: float = 0.0,

@@ -306,3 +306,69 @@ def test_validate_agrees_with_the_reference_on_corrupted_blocks(api, ref_api):
         b, q = _aligned(bytes(arr))
         assert not api.mesh_validate(q, good.size), k
         assert not ref.mesh_validate(q, good.size), k
+
+
+def _query_meshes(api):
+    """Aligned copies of the golden meshes and of seeded soups (one- and
+    two-sided), for the query tests."""
+    data = open(os.path.join(GOLDEN, "meshes.bin"), "rb").read()
+    n = int(np.frombuffer(data[:4], np.uint32)[0])
+    offs = list(np.frombuffer(data[4:4 + 4 * n], np.uint32)) + [len(data)]
+    blocks = [data[offs[k]:offs[k + 1]] for k in range(n)]
+    rng = np.random.default_rng(5)
+    for two in (False, True):
+        v = rng.uniform(-4, 4, (120, 3)).astype(np.float32)
+        idx = rng.integers(0, 120, (300, 3)).astype(np.uint32)
+        blocks.append(P.build_mesh(api, v, idx, two_sided=two)[1])
+    return [_aligned(b) for b in blocks]
+
+
+def test_overlap_queries_match_the_reference(api, ref_api):
+    """Seeded boxes over each mesh: the triangles visited, in order, and the
+    early stop, as the reference's nph_physics_mesh_overlap."""
+    ref = P.bind_mesh(ref_api)
+    rng = np.random.default_rng(8)
+    visited = 0
+    for buf, p in _query_meshes(api):
+        for _ in range(150):
+            c = rng.uniform(-16, 16, 3)
+            e = rng.uniform(0.05, 6.0, 3)
+            lo, hi = tuple(c - e), tuple(c + e)
+            mine = P.mesh_overlap(api, p, lo, hi)
+            assert mine == P.mesh_overlap(ref, p, lo, hi)
+            visited += len(mine)
+            if len(mine) > 3:
+                assert P.mesh_overlap(api, p, lo, hi, 3) == P.mesh_overlap(ref, p, lo, hi, 3)
+    assert visited > 1000
+
+
+def test_raycasts_match_the_reference(api, ref_api):
+    """Seeded segments, axis-parallel ones, and rays aimed at vertices and
+    edge midpoints (where the watertight test decides in f64): hit or miss,
+    fraction, normal, triangle, original and side equal the reference's."""
+    ref = P.bind_mesh(ref_api)
+    rng = np.random.default_rng(9)
+    hits = 0
+    for buf, p in _query_meshes(api):
+        m = _Mesh(C.string_at(p, C.cast(p, C.POINTER(C.c_uint32))[0]))
+        for k in range(300):
+            o = rng.uniform(-18, 18, 3)
+            if k % 3 == 0:
+                d = rng.uniform(-30, 30, 3)
+            elif k % 3 == 1:
+                d = np.zeros(3)
+                d[k % 3] = rng.choice([-40.0, 40.0])
+            else:
+                t = m.tris[rng.integers(0, m.nt)]
+                a, b = m.verts[t[rng.integers(0, 3)]], m.verts[t[rng.integers(0, 3)]]
+                d = (a.astype(np.float64) + b) * 0.5 - o
+                d *= 1.5
+            o32, d32 = P.Vec3(*o.astype(np.float32)), P.Vec3(*d.astype(np.float32))
+            hm, hr = P.MeshHit(), P.MeshHit()
+            rm = api.mesh_raycast(p, o32, d32, C.byref(hm))
+            rr = ref.mesh_raycast(p, o32, d32, C.byref(hr))
+            assert rm == rr, k
+            if rm:
+                assert bytes(hm) == bytes(hr), k
+                hits += 1
+    assert hits > 300
