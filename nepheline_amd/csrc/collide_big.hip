@@ -1,18 +1,21 @@
 // SPDX-License-Identifier: Apache-2.0
 // The narrow phase for convex pairs with a hull of more than 32 vertices:
 // the same collide code as the step's kernels (nph_collide.h), built here
-// with its private arrays sized for the reference's largest hull (256
-// vertices, hull.h:45) and its clip capacity (256 + 8, collide_hull.c:26),
-// so faces clip to the reference's points. The step's convex kernels skip
-// these pairs (NPH_SHAPE_BIG_HULL on a shape, `big` on a query) and these
-// kernels take only them, so each pair runs in exactly one. Their frames
-// are larger (15 KB a thread of private memory), which is why the common
-// hulls' kernels keep the 32-vertex build. Mesh pairs need none of this:
+// with B's vertex array sized for the reference's largest hull (256
+// vertices, hull.h:45), B's face normals recomputed rather than kept, and
+// a clip capacity of 132: hull faces have at most 64 vertices (the host's
+// limit), so a clipped face holds at most 64 + 64 points and neither this
+// capacity nor the reference's 264 (collide_hull.c:26) ever truncates. The
+// step's convex kernels skip these pairs (NPH_SHAPE_BIG_HULL on a shape,
+// `big` on a query) and these kernels take only them, so each pair runs in
+// exactly one. The frame stays near the mesh kernel's: a 15 KB one (B's
+// normals kept, capacity 264) could not get its scratch on batch streams
+// (HSA_STATUS_ERROR_OUT_OF_RESOURCES). Mesh pairs need none of this:
 // against a triangle the per-shape arrays hold the triangle, and a clipped
-// face of at most 64 vertices (the host's limit for hull faces) fits the
-// 32-vertex build's 68 points.
+// face fits the 32-vertex build's 68 points.
 #define NPH_HULL_MAX_VERTICES 256U
-#define NPH_CLIP_CAPACITY     264U
+#define NPH_CLIP_CAPACITY     132U
+#define NPH_CACHE_B_NORMALS   0
 
 #include "device_impl.h"
 

@@ -30,6 +30,12 @@
 #define NPH_CLIP_CAPACITY     68U
 #endif
 #define NPH_HULL_MAX_FACES    (2U * NPH_HULL_MAX_VERTICES)
+// 1: the SAT's edge loop reads B's face normals (in A's frame) from a private
+// array filled by the face loop; 0: it recomputes them (the same function of
+// the same inputs, so the same bits) to keep the private frame small.
+#ifndef NPH_CACHE_B_NORMALS
+#define NPH_CACHE_B_NORMALS 1
+#endif
 // The largest hull face a world takes (see the mesh pairs above).
 #define NPH_HULL_MAX_FACE_VERTICES 64U
 
@@ -1197,12 +1203,16 @@ NPH_HD int sat_query(const NphHullPair *hp, f32 limit, NphCache *cache, NphSat *
         }
         return 0;
     }
+#if NPH_CACHE_B_NORMALS
     NphVec3 bn[NPH_HULL_MAX_FACES];
+#endif
     r->face_b_sep = -NPH_F32_MAX;
     r->face_b = 0U;
     for (u32 f = 0U; f < b->face_count; ++f) {
         const NphPlane pl = plane_b_in_a(hp, f);
+#if NPH_CACHE_B_NORMALS
         bn[f] = pl.normal;
+#endif
         const f32 s = min_projection4(a->vertices, a->vertex_count, pl.normal) + pl.d;
         if (s > r->face_b_sep) {
             r->face_b_sep = s;
@@ -1227,8 +1237,13 @@ NPH_HD int sat_query(const NphHullPair *hp, f32 limit, NphCache *cache, NphSat *
         const NphVec3 pa = a->vertices[a->edges[ea].origin];
         const NphVec3 ua = v3_sub(edge_head(a, a->vertices, ea), pa);
         for (u32 eb = 0U; eb < b->edge_count; eb += 2U) {
+#if NPH_CACHE_B_NORMALS
             const NphVec3 c = v3_neg(bn[b->edges[eb].face]);
             const NphVec3 d = v3_neg(bn[b->edges[eb + 1U].face]);
+#else
+            const NphVec3 c = v3_neg(plane_b_in_a(hp, b->edges[eb].face).normal);
+            const NphVec3 d = v3_neg(plane_b_in_a(hp, b->edges[eb + 1U].face).normal);
+#endif
             const NphVec3 pb = hp->bv[b->edges[eb].origin];
             const NphVec3 hb = edge_head(b, hp->bv, eb);
             const NphVec3 u = arc_plane(c, d, pb, hb);
