@@ -9,12 +9,24 @@
 // by a mutex: the reference's functions are free functions, callable from
 // any thread.  There is no CPU fallback: without a HIP device the calls
 // report no contact and say why on stderr.
+//
+// Hulls with a face of more than NPH_HULL_MAX_FACE_VERTICES vertices are
+// refused here as the world refuses them: their clipped faces would not fit
+// the large-hull kernel's clip capacity (the reference's holds them).
 
 #include "device_impl.h"
 
 #include <mutex>
 
 namespace {
+
+// A hull block the queries have seen: its device copy, and whether a face
+// of it is too wide for the clip buffers.
+struct QueryBlock {
+    const void *host;
+    const void *device;
+    int wide_face;
+};
 
 struct QueryContext {
     std::mutex lock;
@@ -28,7 +40,7 @@ struct QueryContext {
     NphMPoint *points = NULL;
     u32 cap_points = 0U;
     u32 *count = NULL;
-    HullEntry *blocks = NULL;
+    QueryBlock *blocks = NULL;
     u32 block_count = 0U, block_cap = 0U;
 };
 
@@ -55,39 +67,56 @@ NphResult query_ready(QueryContext *q)
     return NPH_RESULT_OK;
 }
 
+// True when a face of the hull has more vertices than the clip buffers of
+// the narrow phase take (the world refuses such a shape at creation).
+int hull_has_wide_face(const NphPhysicsHull *h)
+{
+    for (u32 f = 0U; f < h->face_count; ++f) {
+        if (nph_physics_hull_face_vertex_count(h, f) > NPH_HULL_MAX_FACE_VERTICES) {
+            return 1;
+        }
+    }
+    return 0;
+}
+
 // The device copy of a hull block (made once per block).
-NphResult query_block(QueryContext *q, const void *block, u32 bytes, const void **out)
+NphResult query_block(QueryContext *q, const NphPhysicsHull *block, const void **out)
 {
     for (u32 i = 0U; i < q->block_count; ++i) {
         if (q->blocks[i].host == block) {
             *out = q->blocks[i].device;
-            return NPH_RESULT_OK;
+            return q->blocks[i].wide_face ? NPH_RESULT_UNSUPPORTED : NPH_RESULT_OK;
         }
     }
+    const u32 bytes = block->byte_size;
     if (q->block_count == q->block_cap) {
         const u32 c = q->block_cap == 0U ? 16U : 2U * q->block_cap;
-        HullEntry *n = (HullEntry *)realloc(q->blocks, (size_t)c * sizeof(HullEntry));
+        QueryBlock *n = (QueryBlock *)realloc(q->blocks, (size_t)c * sizeof(QueryBlock));
         if (n == NULL) {
             return NPH_RESULT_OUT_OF_MEMORY;
         }
         q->blocks = n;
         q->block_cap = c;
     }
+    const int wide = hull_has_wide_face(block);
     void *p = NULL;
-    HIP_TRY(hipMalloc(&p, bytes));
-    HIP_TRY(hipMemcpy(p, block, bytes, hipMemcpyHostToDevice));
+    if (!wide) {
+        HIP_TRY(hipMalloc(&p, bytes));
+        HIP_TRY(hipMemcpy(p, block, bytes, hipMemcpyHostToDevice));
+    }
     q->blocks[q->block_count].host = block;
     q->blocks[q->block_count].device = p;
+    q->blocks[q->block_count].wide_face = wide;
     q->block_count += 1U;
     *out = p;
-    return NPH_RESULT_OK;
+    return wide ? NPH_RESULT_UNSUPPORTED : NPH_RESULT_OK;
 }
 
 NphResult query_geom(QueryContext *q, const NphPhysicsGeometry *g, NphGeom *out)
 {
     const void *block = NULL;
     if (g->type == NPH_PHYSICS_SHAPE_HULL) {
-        const NphResult r = query_block(q, g->hull, g->hull->byte_size, &block);
+        const NphResult r = query_block(q, g->hull, &block);
         if (r != NPH_RESULT_OK) {
             return r;
         }

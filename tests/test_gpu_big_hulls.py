@@ -167,3 +167,19 @@ def test_hull_faces_beyond_64_vertices_are_refused(gpu_api):
     w.add_shape(b, P.geometry_hull(shapes[60].ptrs[0]))
     w.step(S.DT)
     w.close()
+    # The pair queries refuse the 100-gon caps too (an empty manifold, and
+    # NPH_RESULT_UNSUPPORTED from the batched queries), and take the 60-gon's.
+    g = P.bind_collide(gpu_api)
+    low = P.Pose(P.Vec3(0.0, 0.0, 0.0), P.Quat(0.0, 0.0, 0.0, 1.0))
+    high = P.Pose(P.Vec3(0.0, 0.95, 0.0), P.Quat(0.0, 0.0, 0.0, 1.0))
+    for seg, touching in ((100, False), (60, True)):
+        geom = P.geometry_hull(shapes[seg].ptrs[0])
+        m = P.Manifold()
+        g.collide(C.byref(geom), low, C.byref(geom), high, 0.02, None, C.byref(m))
+        assert (m.point_count > 0) == touching, seg
+        q = (P.CollideQuery * 1)()
+        q[0].a, q[0].pose_a, q[0].b, q[0].pose_b = geom, low, geom, high
+        q[0].speculative_distance = 0.02
+        out = (P.Manifold * 1)()
+        rc = g.collide_many(q, 1, out, None)
+        assert rc == (P.OK if touching else P.UNSUPPORTED), seg
