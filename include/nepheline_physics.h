@@ -373,6 +373,23 @@ typedef struct NphAmdCollideQuery {
 NphResult nph_amd_collide_many(const NphAmdCollideQuery *queries, u32 count,
                                NphPhysicsManifold *out, NphPhysicsCollideCache *caches);
 
+// No counterpart in the reference: nph_amd_collide_many for callers that
+// cannot allocate device memory (code under HIP graph capture, tight
+// serving loops, harnesses that forbid allocation). It stages a call's
+// pairs, manifolds, caches and hull blocks in one fixed device array of
+// NPH_AMD_STATIC_SCRATCH_BYTES per GPU (held until the process ends),
+// runs on the null stream and calls no allocator. A call whose pairs and
+// distinct hull blocks do not fit, or that names more than
+// NPH_AMD_STATIC_MAX_HULLS distinct hull blocks, returns
+// NPH_RESULT_INVALID_ARGUMENT before touching the device. Every call on a
+// GPU shares that GPU's array: calls that overlap on one GPU (from two
+// host threads) race. Results equal nph_amd_collide_many's bit for bit.
+#define NPH_AMD_STATIC_SCRATCH_BYTES (16U << 20)
+#define NPH_AMD_STATIC_MAX_HULLS 256U
+#define NPH_AMD_MAX_DEVICES 64U
+NphResult nph_amd_collide_many_static(const NphAmdCollideQuery *queries, u32 count,
+                                      NphPhysicsManifold *out, NphPhysicsCollideCache *caches);
+
 // ---------------------------------------------------------------------------
 // World (physics/world.h:57-326)
 // ---------------------------------------------------------------------------

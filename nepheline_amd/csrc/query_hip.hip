@@ -10,6 +10,10 @@
 // any thread.  There is no CPU fallback: without a HIP device the calls
 // report no contact and say why on stderr.
 //
+// nph_amd_collide_many_static is the same query for callers that cannot
+// allocate device memory: it stages the pairs, results and hull blocks of a
+// call in one fixed device array (`scratch`, below) and calls no allocator.
+//
 // Hulls with a face of more than NPH_HULL_MAX_FACE_VERTICES vertices are
 // refused here as the world refuses them: their clipped faces would not fit
 // the large-hull kernel's clip capacity (the reference's holds them).
@@ -134,6 +138,97 @@ int convex(NphPhysicsShapeType t)
 {
     return t == NPH_PHYSICS_SHAPE_SPHERE || t == NPH_PHYSICS_SHAPE_CAPSULE ||
            t == NPH_PHYSICS_SHAPE_BOX || t == NPH_PHYSICS_SHAPE_HULL;
+}
+
+// The fixed device array of nph_amd_collide_many_static: a call's pairs,
+// manifolds, caches and hull blocks, laid out from its start. Each device
+// holds its own copy until the process ends. In host code `scratch` names a
+// host-side stand-in, never the device memory: kernels and copies take the
+// address hipGetSymbolAddress gives for the current device.
+static __device__ __align__(256) unsigned char scratch[NPH_AMD_STATIC_SCRATCH_BYTES];
+
+// The device address of `scratch`, looked up once per device (with that
+// device current) on the first call made on it.
+struct StaticScratch {
+    std::mutex lock;
+    void *ptr[NPH_AMD_MAX_DEVICES];
+};
+StaticScratch g_static;
+
+NphResult static_scratch(void **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= (int)NPH_AMD_MAX_DEVICES) {
+        fprintf(stderr, "nepheline_amd: no HIP device: the collision queries run only on an "
+                        "MI355X\n");
+        return NPH_RESULT_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> guard(g_static.lock);
+    if (g_static.ptr[dev] == NULL) {
+        void *p = NULL;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == NULL) {
+            fprintf(stderr, "nepheline_amd: the static scratch of device %d has no address: %s\n",
+                    dev, hipGetErrorString(e));
+            return NPH_RESULT_DEVICE_LOST;
+        }
+        g_static.ptr[dev] = p;
+    }
+    *out = g_static.ptr[dev];
+    return NPH_RESULT_OK;
+}
+
+static inline u64 align256(u64 x) { return (x + 255U) & ~(u64)255U; }
+
+// A hull block of one static call: its place in the scratch array.
+struct StaticBlock {
+    const NphPhysicsHull *host;
+    u64 offset;
+};
+
+// The distinct hull blocks of a static call (at most NPH_AMD_STATIC_MAX_HULLS),
+// each placed after `base` bytes of the scratch array; false when one is
+// refused or there are too many.
+NphResult static_blocks(const NphAmdCollideQuery *queries, u32 n, u64 base, StaticBlock *blocks,
+                        u32 *count, u64 *end)
+{
+    u32 k = 0U;
+    u64 at = base;
+    for (u32 i = 0U; i < n; ++i) {
+        const NphPhysicsGeometry *g[2] = {&queries[i].a, &queries[i].b};
+        for (u32 s = 0U; s < 2U; ++s) {
+            if (!convex(g[s]->type)) {
+                return NPH_RESULT_INVALID_ARGUMENT;
+            }
+            if (g[s]->type != NPH_PHYSICS_SHAPE_HULL) {
+                continue;
+            }
+            const NphPhysicsHull *h = g[s]->hull;
+            u32 j = 0U;
+            while (j < k && blocks[j].host != h) {
+                ++j;
+            }
+            if (j < k) {
+                continue;
+            }
+            if (hull_has_wide_face(h)) {
+                return NPH_RESULT_UNSUPPORTED;
+            }
+            if (k == NPH_AMD_STATIC_MAX_HULLS) {
+                fprintf(stderr, "nepheline_amd: nph_amd_collide_many_static takes at most %u "
+                                "distinct hull blocks a call\n",
+                        NPH_AMD_STATIC_MAX_HULLS);
+                return NPH_RESULT_INVALID_ARGUMENT;
+            }
+            blocks[k].host = h;
+            blocks[k].offset = at;
+            at = align256(at + h->byte_size);
+            ++k;
+        }
+    }
+    *count = k;
+    *end = at;
+    return NPH_RESULT_OK;
 }
 
 } // namespace
@@ -277,6 +372,94 @@ void nph_physics_collide(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
         out->normal = v3(0.0F, 1.0F, 0.0F);
         out->point_count = 0U;
     }
+}
+
+NphResult nph_amd_collide_many_static(const NphAmdCollideQuery *queries, u32 n,
+                                      NphPhysicsManifold *out, NphPhysicsCollideCache *caches)
+{
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    if (queries == NULL || out == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    // The call's layout in the scratch array: pairs, manifolds, caches, then
+    // the hull blocks. Everything is checked before the device is touched.
+    const u64 at_manifolds = align256((u64)n * sizeof(NphQueryPair));
+    const u64 at_caches = align256(at_manifolds + (u64)n * sizeof(NphManifold));
+    const u64 at_blocks = align256(at_caches + (u64)n * sizeof(NphCache));
+    StaticBlock blocks[NPH_AMD_STATIC_MAX_HULLS];
+    u32 block_count = 0U;
+    u64 end = 0U;
+    NphResult r = static_blocks(queries, n, at_blocks, blocks, &block_count, &end);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    if (end > NPH_AMD_STATIC_SCRATCH_BYTES) {
+        fprintf(stderr, "nepheline_amd: nph_amd_collide_many_static: %u pairs and %u hull blocks "
+                        "need %llu bytes, over the %u-byte static scratch\n",
+                n, block_count, (unsigned long long)end, NPH_AMD_STATIC_SCRATCH_BYTES);
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    void *base = NULL;
+    r = static_scratch(&base);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    unsigned char *dev = (unsigned char *)base;
+    NphQueryPair *pairs = (NphQueryPair *)dev;
+    NphManifold *manifolds = (NphManifold *)(dev + at_manifolds);
+    NphCache *dcaches = (NphCache *)(dev + at_caches);
+    for (u32 j = 0U; j < block_count; ++j) {
+        HIP_TRY(hipMemcpy(dev + blocks[j].offset, blocks[j].host, blocks[j].host->byte_size,
+                          hipMemcpyHostToDevice));
+    }
+    // The pairs in device form, a chunk at a time through a buffer on the
+    // stack (hipMemcpy returns once the chunk is on the device).
+    enum { CHUNK = 64 };
+    NphQueryPair h[CHUNK];
+    bool any_big = false;
+    for (u32 first = 0U; first < n; first += CHUNK) {
+        const u32 m = n - first < (u32)CHUNK ? n - first : (u32)CHUNK;
+        for (u32 i = 0U; i < m; ++i) {
+            const NphAmdCollideQuery *x = &queries[first + i];
+            memset(&h[i], 0, sizeof h[i]);
+            const NphPhysicsGeometry *g[2] = {&x->a, &x->b};
+            NphGeom *o[2] = {&h[i].a, &h[i].b};
+            for (u32 s = 0U; s < 2U; ++s) {
+                const void *address = NULL;
+                if (g[s]->type == NPH_PHYSICS_SHAPE_HULL) {
+                    u32 j = 0U;
+                    while (blocks[j].host != g[s]->hull) {
+                        ++j; // present: static_blocks placed every hull
+                    }
+                    address = dev + blocks[j].offset;
+                }
+                *o[s] = geom_from(g[s], address);
+            }
+            h[i].pose_a = x->pose_a;
+            h[i].pose_b = x->pose_b;
+            h[i].speculative = x->speculative_distance;
+            h[i].big = big_hull(&x->a) || big_hull(&x->b) ? 1U : 0U;
+            any_big |= h[i].big != 0U;
+            if (caches != NULL) {
+                memcpy(&h[i].cache, &caches[first + i], sizeof h[i].cache);
+            }
+        }
+        HIP_TRY(hipMemcpy(pairs + first, h, (u64)m * sizeof(NphQueryPair),
+                          hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(k_query_collide, dim3(devi_blocks(n, 64U)), dim3(64), 0, 0, pairs, n,
+                       manifolds, dcaches);
+    if (any_big) {
+        devi_query_collide_big(0, pairs, n, manifolds, dcaches);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, manifolds, (u64)n * sizeof(NphManifold), hipMemcpyDeviceToHost));
+    if (caches != NULL) {
+        HIP_TRY(hipMemcpy(caches, dcaches, (u64)n * sizeof(NphCache), hipMemcpyDeviceToHost));
+    }
+    return NPH_RESULT_OK;
 }
 
 u32 nph_physics_manifold_reduce(NphPhysicsManifoldPoint *points, u32 count, NphVec3 normal)

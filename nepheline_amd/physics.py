@@ -102,6 +102,10 @@ def bind_collide(a):
         a.collide_many = _bind(L, "nph_amd_collide_many", C.c_int,
                                [C.POINTER(CollideQuery), C.c_uint32, C.POINTER(Manifold),
                                 C.POINTER(CollideCache)])
+    if hasattr(L, "nph_amd_collide_many_static"):
+        a.collide_many_static = _bind(L, "nph_amd_collide_many_static", C.c_int,
+                                      [C.POINTER(CollideQuery), C.c_uint32, C.POINTER(Manifold),
+                                       C.POINTER(CollideCache)])
     return a
 
 
