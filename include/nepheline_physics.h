@@ -885,6 +885,10 @@ const char *nph_amd_backend_name(const NphPhysicsWorld *world);
 // overflow's (the lanes of solver.c:810-843); times substeps, the step's
 // contact solves.
 u32 nph_amd_solved_contacts(const NphPhysicsWorld *world);
+// EXTENSION: the mechanical energy of the world's dynamic bodies now, in
+// f64 joules: out[0] linear kinetic, out[1] angular kinetic (body-axis
+// inertia), out[2] potential in the world's gravity (zero at the origin).
+void nph_amd_world_energy(const NphPhysicsWorld *world, f64 out[3]);
 
 // EXTENSION: device time of the last step's stages, in milliseconds, from HIP
 // events on the world's stream: [0] narrow phase, [1] solver and finalize,
@@ -1097,6 +1101,10 @@ NphResult nph_amd_batch_warm_stats(NphAmdBatch *batch, NphAmdBatchWarmStats *out
 // device milliseconds summed since enabling, out[1] = launches.
 void nph_amd_batch_time_kernels(NphAmdBatch *batch, bool enable);
 void nph_amd_batch_kernel_times(const NphAmdBatch *batch, f64 out[2]);
+// With kernel timing on: the convex narrow-phase kernel's dispatch-stamped
+// durations since enabling: out[0] = device milliseconds, out[1] = launches,
+// out[2] = pairs those launches took.
+void nph_amd_batch_collide_times(const NphAmdBatch *batch, f64 out[3]);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,7 @@
 // groups them (solver.c:810-843).
 
 #include "device_impl.h"
+#include "device_narrow.h"
 
 #include <hip/hip_ext.h>
 
@@ -193,6 +194,12 @@ struct NphDeviceBatch {
     // which run one after the other), and the key counters.
     NphSortedEntry *sorted;
     u32 cap_sorted;
+    // The narrow phase's team lists (32-vertex and large-hull pairs) and
+    // their counts.
+    NphSortedEntry *team_small;
+    NphSortedEntry *team_big;
+    u32 cap_team_small, cap_team_big;
+    u32 *team_counts;
     u32 *keys; // counts [0, 64), cursors [64, 128)
     Stage solve;
     // Per-step device arrays of the solve, the worlds' slices.
@@ -239,6 +246,14 @@ struct NphDeviceBatch {
     u32 kev_pairs;
     f64 kernel_ms;
     u64 kernel_launches;
+    // The convex narrow-phase kernel's own duration per half (dispatch-
+    // stamped events), its launches and the pairs they took.
+    hipEvent_t cev[4];
+    int cev_made;
+    u32 cev_pairs[2];
+    f64 collide_kernel_ms;
+    u64 collide_kernel_launches;
+    u64 collide_kernel_pairs;
 };
 
 // ---------------------------------------------------------------------------
@@ -374,24 +389,40 @@ __global__ void __launch_bounds__(256) k_b_key_scatter(Tab tab, u32 nw,
 
 // The sorted list, a contact a thread: convex pairs, or (mesh) the mesh
 // pairs, whose frames hold the triangles' clusters.
-__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide(const NphSortedEntry *__restrict__ sorted, u32 n,
-                                                  const NphCollideWorld *__restrict__ cw,
-                                                  u32 *codes, f32 step_dt)
+// The convex narrow phase over the sorted list (device_narrow.h): lanes,
+// then teams; a wave of the lane pass covers pairs of one type, and so do
+// most teams of a wave of the team pass, whose list keeps the waves' order.
+__global__ void __launch_bounds__(64) k_b_narrow_clear(u32 *counts) { narrow_counts_clear(counts); }
+
+__global__ void __launch_bounds__(NPH_BLOCK) k_b_narrow_lanes(const NphSortedEntry *__restrict__ sorted, u32 n,
+                                                              NphManyWorlds w, u32 *codes,
+                                                              f32 step_dt, NphTeamLists lists)
 {
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) {
-        return;
+    NphSortedEntry e;
+    if (j < n) {
+        e = sorted[j];
+    } else {
+        e.world = e.record = e.index = e.pad = 0U;
     }
-    const NphSortedEntry e = sorted[j];
-    const NphCollideWorld c = cw[e.world];
-    if (c.contacts[e.record].child != NPH_NULL32) {
-        return; // a mesh pair: k_b_collide_mesh
-    }
-    if (contact_has_big_hull(&c.contacts[e.record], c.shapes)) {
-        return; // a hull beyond 32 vertices: k_b_collide_big
-    }
-    contact_update(c.contacts, e.record, c.shapes, c.bodies, step_dt);
-    codes[e.index] = c.contacts[e.record].change;
+    narrow_lane(j < n, e, w, codes, step_dt, lists);
+}
+
+__global__ void __launch_bounds__(NPH_TEAM_BLOCK) k_b_narrow_teams(const NphSortedEntry *list, const u32 *count,
+                                                                   NphManyWorlds w, u32 *codes,
+                                                                   f32 step_dt)
+{
+    narrow_teams<NPH_HULL_MAX_VERTICES, NPH_CLIP_CAPACITY, NPH_TEAM_BLOCK>(list, count, w, codes,
+                                                                            step_dt);
+}
+
+__global__ void __launch_bounds__(NPH_BIG_TEAM_BLOCK) k_b_narrow_big_teams(const NphSortedEntry *list,
+                                                                           const u32 *count,
+                                                                           NphManyWorlds w, u32 *codes,
+                                                                           f32 step_dt)
+{
+    narrow_teams<NPH_BIG_HULL_VERTICES, NPH_BIG_CLIP_CAPACITY, NPH_BIG_TEAM_BLOCK>(
+        list, count, w, codes, step_dt);
 }
 
 __global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide_mesh(const NphSortedEntry *__restrict__ sorted, u32 n,
@@ -970,8 +1001,10 @@ void nph_devb_destroy(NphDeviceBatch *b)
             (void)hipFree(bufs[i]);
         }
     }
-    void *lbufs[] = {b->lx, b->ly, b->lres, b->acc, b->sorted, b->keys};
-    for (u32 i = 0U; i < 6U; ++i) {
+    void *lbufs[] = {b->lx,     b->ly,         b->lres,      b->acc,
+                     b->sorted, b->keys,       b->team_small, b->team_big,
+                     b->team_counts};
+    for (u32 i = 0U; i < sizeof lbufs / sizeof lbufs[0]; ++i) {
         if (lbufs[i] != NULL) {
             (void)hipFree(lbufs[i]);
         }
@@ -987,6 +1020,11 @@ void nph_devb_destroy(NphDeviceBatch *b)
     }
     for (u32 i = 0U; i < 2U * b->kev_pairs; ++i) {
         (void)hipEventDestroy(b->kev[i]);
+    }
+    if (b->cev_made) {
+        for (u32 i = 0U; i < 4U; ++i) {
+            (void)hipEventDestroy(b->cev[i]);
+        }
     }
     free(b->kev);
     (void)hipStreamDestroy(b->stream);
@@ -1025,12 +1063,19 @@ void nph_devb_time_kernels(NphDeviceBatch *b, int enable)
     b->time_kernels = enable;
     b->kernel_ms = 0.0;
     b->kernel_launches = 0U;
+    b->collide_kernel_ms = 0.0;
+    b->collide_kernel_launches = 0U;
+    b->collide_kernel_pairs = 0U;
+    b->cev_pairs[0] = b->cev_pairs[1] = 0U;
 }
 
-void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[2])
+void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[5])
 {
     out[0] = b->kernel_ms;
     out[1] = (f64)b->kernel_launches;
+    out[2] = b->collide_kernel_ms;
+    out[3] = (f64)b->collide_kernel_launches;
+    out[4] = (f64)b->collide_kernel_pairs;
 }
 
 NphResult nph_devb_follow(NphDeviceBatch *b, const NphDeviceBatch *first)
@@ -1169,15 +1214,56 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
                            b->keys + NPH_PAIR_KEYS);
         hipLaunchKernelGGL(k_b_key_scatter, grid, dim3(256), 0, s, tab, (u32)lw, dcw, dl,
                            b->keys + NPH_PAIR_KEYS, b->sorted);
-        hipLaunchKernelGGL(k_b_collide, dim3(devi_blocks(m, NPH_COLLIDE_BLOCK)),
-                           dim3(NPH_COLLIDE_BLOCK), 0, s, b->sorted, m, dcw, h->codes, step_dt);
+        if (b->time_kernels && !b->cev_made) {
+            for (u32 e = 0U; e < 4U; ++e) {
+                HIP_TRY(hipEventCreate(&b->cev[e]));
+            }
+            b->cev_made = 1;
+        }
+        if (b->team_counts == NULL) {
+            HIP_TRY(hipMalloc((void **)&b->team_counts, 64U));
+        }
+        r = dev_reserve(&b->team_small, &b->cap_team_small, m, s);
+        if (r == NPH_RESULT_OK) {
+            r = dev_reserve(&b->team_big, &b->cap_team_big, m, s);
+        }
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        NphManyWorlds w;
+        w.cw = dcw;
+        NphTeamLists lists;
+        lists.small = b->team_small;
+        lists.big = b->team_big;
+        lists.counts = b->team_counts;
+        hipLaunchKernelGGL(k_b_narrow_clear, dim3(1), dim3(64), 0, s, b->team_counts);
+        const dim3 lane_grid(devi_blocks(m, NPH_BLOCK));
+        const dim3 team_grid(devi_team_grid(m, NPH_TEAM_BLOCK));
+        if (b->time_kernels) {
+            // The dispatch packets stamp the lane pass's start and the team
+            // pass's end.
+            hipExtLaunchKernelGGL(k_b_narrow_lanes, lane_grid, dim3(NPH_BLOCK), 0, s,
+                                  b->cev[2U * half], NULL, 0, b->sorted, m, w, h->codes, step_dt,
+                                  lists);
+            hipExtLaunchKernelGGL(k_b_narrow_teams, team_grid, dim3(NPH_TEAM_BLOCK), 0, s, NULL,
+                                  b->cev[2U * half + 1U], 0, b->team_small, b->team_counts, w,
+                                  h->codes, step_dt);
+            b->cev_pairs[half] = m;
+        } else {
+            hipLaunchKernelGGL(k_b_narrow_lanes, lane_grid, dim3(NPH_BLOCK), 0, s, b->sorted, m, w,
+                               h->codes, step_dt, lists);
+            hipLaunchKernelGGL(k_b_narrow_teams, team_grid, dim3(NPH_TEAM_BLOCK), 0, s,
+                               b->team_small, b->team_counts, w, h->codes, step_dt);
+        }
+        if (big) {
+            hipLaunchKernelGGL(k_b_narrow_big_teams, dim3(devi_team_grid(m, NPH_BIG_TEAM_BLOCK)),
+                               dim3(NPH_BIG_TEAM_BLOCK), 0, s, b->team_big, b->team_counts + 1, w,
+                               h->codes, step_dt);
+        }
         if (mesh) {
             hipLaunchKernelGGL(k_b_collide_mesh, dim3(devi_blocks(m, NPH_COLLIDE_BLOCK)),
                                dim3(NPH_COLLIDE_BLOCK), 0, s, b->sorted, m, dcw, h->codes,
                                step_dt);
-        }
-        if (big) {
-            devi_b_collide_big(s, b->sorted, m, dcw, h->codes, step_dt);
         }
     }
     HIP_TRY(hipGetLastError());
@@ -1209,6 +1295,16 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
     }
     if (hipEventElapsedTime(&ms, b->ev[6], b->ev[7]) == hipSuccess) {
         b->collide_ms += ms;
+    }
+    for (u32 k = 0U; k < 2U; ++k) {
+        if (b->cev_pairs[k] > 0U) {
+            float kms = 0.0F;
+            HIP_TRY(hipEventElapsedTime(&kms, b->cev[2U * k], b->cev[2U * k + 1U]));
+            b->collide_kernel_ms += kms;
+            b->collide_kernel_launches += 1U;
+            b->collide_kernel_pairs += b->cev_pairs[k];
+            b->cev_pairs[k] = 0U;
+        }
     }
     codes[0] = b->h_codes;
     codes[1] = b->h_codes + l0;

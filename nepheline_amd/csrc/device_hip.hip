@@ -16,6 +16,7 @@
 
 
 #include "device_impl.h"
+#include "device_narrow.h"
 
 #include <hip/hip_ext.h>
 
@@ -96,7 +97,8 @@ void nph_dev_destroy(NphDevice *d)
                     d->lane_contact, d->lane_body_a, d->lane_body_b, d->lane_flags, d->lanes,
                     d->points, d->grown, d->grown_blocks, d->joint_conf, d->joint_state, d->step_joints,
                     d->fast_list, d->fast_pack, d->gate, d->mlp_params, d->lx, d->lraw, d->ly,
-                    d->lres, d->wstats, d->log_x, d->log_t, d->log_count};
+                    d->lres, d->wstats, d->log_x, d->log_t, d->log_count, d->team_small,
+                    d->team_big, d->team_counts};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
             (void)hipFree(bufs[i]);
@@ -521,21 +523,35 @@ __global__ void __launch_bounds__(NPH_BLOCK) k_init_contacts(NphContactRec *cont
     init_contact_record(contacts, gate, &in);
 }
 
-__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide(NphContactRec *contacts, const NphShapeRec *shapes,
-                          const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt)
+// The convex narrow phase (device_narrow.h): lanes, then teams.
+__global__ void __launch_bounds__(64) k_narrow_clear(u32 *counts) { narrow_counts_clear(counts); }
+
+__global__ void __launch_bounds__(NPH_BLOCK) k_narrow_lanes(NphOneWorld w, const u32 *list, u32 *codes, u32 n,
+                                                            f32 step_dt, NphTeamLists lists)
 {
     const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) {
-        return;
-    }
-    if (contacts[list[i]].child != NPH_NULL32) {
-        return; // a mesh pair: k_collide_mesh
-    }
-    if (contact_has_big_hull(&contacts[list[i]], shapes)) {
-        return; // a hull beyond 32 vertices: k_collide_big
-    }
-    contact_update(contacts, list[i], shapes, bodies, step_dt);
-    codes[i] = contacts[list[i]].change;
+    NphSortedEntry e;
+    e.world = 0U;
+    e.record = i < n ? list[i] : 0U;
+    e.index = i;
+    e.pad = 0U;
+    narrow_lane(i < n, e, w, codes, step_dt, lists);
+}
+
+__global__ void __launch_bounds__(NPH_TEAM_BLOCK) k_narrow_teams(NphOneWorld w, const NphSortedEntry *list,
+                                                                 const u32 *count, u32 *codes,
+                                                                 f32 step_dt)
+{
+    narrow_teams<NPH_HULL_MAX_VERTICES, NPH_CLIP_CAPACITY, NPH_TEAM_BLOCK>(list, count, w, codes,
+                                                                            step_dt);
+}
+
+__global__ void __launch_bounds__(NPH_BIG_TEAM_BLOCK) k_narrow_big_teams(NphOneWorld w, const NphSortedEntry *list,
+                                                                         const u32 *count, u32 *codes,
+                                                                         f32 step_dt)
+{
+    narrow_teams<NPH_BIG_HULL_VERTICES, NPH_BIG_CLIP_CAPACITY, NPH_BIG_TEAM_BLOCK>(
+        list, count, w, codes, step_dt);
 }
 
 // The mesh pairs of the same list, one a thread (their frames hold the
@@ -1022,18 +1038,45 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
     memcpy(d->h_list + first, awake + first, (size_t)n * 4U);
     HIP_TRY(hipMemcpyAsync(d->list + first, d->h_list + first, (u64)n * 4U, hipMemcpyHostToDevice,
                            d->stream));
+    if (d->team_counts == NULL) {
+        HIP_TRY(hipMalloc((void **)&d->team_counts, 64U));
+    }
+    if (n > d->cap_team) {
+        const u32 c = devi_grow_cap(d->cap_team, n);
+        NphResult r;
+        if ((r = devi_grow_buffer((void **)&d->team_small, 0, (u64)c * sizeof(NphSortedEntry),
+                                  d->stream, 0)) != NPH_RESULT_OK ||
+            (r = devi_grow_buffer((void **)&d->team_big, 0, (u64)c * sizeof(NphSortedEntry),
+                                  d->stream, 0)) != NPH_RESULT_OK) {
+            return r;
+        }
+        d->cap_team = c;
+    }
     HIP_TRY(hipEventRecord(e0, d->stream));
-    hipLaunchKernelGGL(k_collide, dim3((n + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
-                       dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes, d->bodies,
-                       d->list + first, d->codes + first, n, step_dt);
+    NphOneWorld w;
+    w.c.contacts = d->contacts;
+    w.c.shapes = d->shapes;
+    w.c.bodies = d->bodies;
+    w.c.gate = d->gate;
+    NphTeamLists lists;
+    lists.small = d->team_small;
+    lists.big = d->team_big;
+    lists.counts = d->team_counts;
+    hipLaunchKernelGGL(k_narrow_clear, dim3(1), dim3(64), 0, d->stream, d->team_counts);
+    hipLaunchKernelGGL(k_narrow_lanes, dim3(devi_blocks(n, NPH_BLOCK)), dim3(NPH_BLOCK), 0,
+                       d->stream, w, d->list + first, d->codes + first, n, step_dt, lists);
+    hipLaunchKernelGGL(k_narrow_teams, dim3(devi_team_grid(n, NPH_TEAM_BLOCK)),
+                       dim3(NPH_TEAM_BLOCK), 0, d->stream, w, d->team_small, d->team_counts,
+                       d->codes + first, step_dt);
+    if (d->big_hull_shapes > 0U) {
+        hipLaunchKernelGGL(k_narrow_big_teams, dim3(devi_team_grid(n, NPH_BIG_TEAM_BLOCK)),
+                           dim3(NPH_BIG_TEAM_BLOCK), 0, d->stream, w, d->team_big,
+                           d->team_counts + 1, d->codes + first, step_dt);
+    }
     if (d->mesh_shapes > 0U) {
         hipLaunchKernelGGL(k_collide_mesh, dim3((n + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
                            dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes,
                            d->bodies, d->list + first, d->codes + first, n, step_dt);
-    }
-    if (d->big_hull_shapes > 0U) {
-        devi_collide_big(d->stream, d->contacts, d->shapes, d->bodies, d->list + first,
-                         d->codes + first, n, step_dt);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, d->stream));

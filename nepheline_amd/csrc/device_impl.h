@@ -71,6 +71,12 @@ struct NphDevice {
     u32 *list;  // awake contacts / graph contacts (cap_list)
     u32 cap_list;
     u32 *codes; // change codes, per list entry
+    // The narrow phase's team lists (32-vertex and large-hull pairs) and
+    // their counts.
+    struct NphSortedEntry *team_small;
+    struct NphSortedEntry *team_big;
+    u32 cap_team;
+    u32 *team_counts;
     NphContactInit *inits;
     u32 cap_inits;
     NphGroup *groups;
@@ -189,15 +195,6 @@ typedef struct NphQueryPair {
     NphCache cache;
 } NphQueryPair;
 
-// The large-hull narrow phase (collide_big.hip: the same code built for
-// hulls of up to 256 vertices): the convex pairs the 32-vertex kernels
-// skip. Each launches on `s` and returns at once.
-void devi_collide_big(hipStream_t s, NphContactRec *contacts, const NphShapeRec *shapes,
-                      const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt);
-void devi_b_collide_big(hipStream_t s, const NphSortedEntry *sorted, u32 n,
-                        const NphCollideWorld *cw, u32 *codes, f32 step_dt);
-void devi_query_collide_big(hipStream_t s, const NphQueryPair *pairs, u32 n, NphManifold *out,
-                            NphCache *caches);
 
 enum { PASS_WARM = 0, PASS_SOLVE = 1, PASS_RELAX = 2, PASS_RESTITUTION = 3 };
 

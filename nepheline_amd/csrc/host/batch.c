@@ -211,10 +211,22 @@ void nph_amd_batch_kernel_times(const NphAmdBatch *b, f64 out[2])
 {
     out[0] = out[1] = 0.0;
     for (u32 g = 0U; g < b->streams; ++g) {
-        f64 t[2];
+        f64 t[5];
         nph_devb_kernel_times(b->dev[g], t);
         out[0] += t[0];
         out[1] += t[1];
+    }
+}
+
+void nph_amd_batch_collide_times(const NphAmdBatch *b, f64 out[3])
+{
+    out[0] = out[1] = out[2] = 0.0;
+    for (u32 g = 0U; g < b->streams; ++g) {
+        f64 t[5];
+        nph_devb_kernel_times(b->dev[g], t);
+        out[0] += t[2];
+        out[1] += t[3];
+        out[2] += t[4];
     }
 }
 

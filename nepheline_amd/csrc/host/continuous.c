@@ -113,101 +113,40 @@ static Proxy geometry_proxy(const NphPhysicsGeometry *g, NphVec3 storage[8])
 
 static u32 support(const Proxy *p, NphVec3 d) { return points_support(p->points, p->count, d); }
 
-static NphGjkVertex posed_vertex(const Proxy *a, NphPhysicsPose pa, u32 ia, const Proxy *b,
-                                 NphPhysicsPose pb, u32 ib)
-{
-    NphGjkVertex v;
-    v.a = pose_point(pa, a->points[ia]);
-    v.b = pose_point(pb, b->points[ib]);
-    v.w = v3_sub(v.a, v.b);
-    v.ia = ia;
-    v.ib = ib;
-    return v;
-}
-
-// nph_physics_distance(a, pose_a, b, pose_b, use_radii = false, cache): the
-// cores' distance, the normal from A towards B (+Y when they overlap).
+// nph_physics_distance(a, pose_a, b, pose_b, use_radii = false, cache) on
+// the GJK stepper (nph_gjk.h): supports searched in each proxy's own frame,
+// vertices posed into the common one; the simplex starts from the cache
+// when it names valid points, and the cache keeps the final simplex.
 static void distance(const Proxy *a, NphPhysicsPose pa, const Proxy *b, NphPhysicsPose pb,
                      SimplexCache *cache, NphDistance *out)
 {
-    NphGjkSimplex s;
-    s.count = 0U;
-    if (cache->count >= 1U && cache->count <= 4U) {
-        for (u32 i = 0U; i < cache->count; ++i) {
-            const u32 ia = cache->index_a[i];
-            const u32 ib = cache->index_b[i];
-            if (ia >= a->count || ib >= b->count) {
-                s.count = 0U;
-                break;
-            }
-            s.v[s.count++] = posed_vertex(a, pa, ia, b, pb, ib);
+    NphGjk g;
+    gjk_reset(&g);
+    const int cached = cache->count >= 1U && cache->count <= 4U;
+    for (u32 i = 0U; cached && i < cache->count; ++i) {
+        const u32 ia = cache->index_a[i];
+        const u32 ib = cache->index_b[i];
+        if (ia >= a->count || ib >= b->count) {
+            gjk_reset(&g);
+            break;
         }
+        gjk_seed(&g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
     }
-    if (s.count == 0U) {
-        s.v[0] = posed_vertex(a, pa, 0U, b, pb, 0U);
-        s.count = 1U;
+    if (g.count == 0U) {
+        gjk_seed(&g, 0U, pose_point(pa, a->points[0]), 0U, pose_point(pb, b->points[0]));
     }
-    const f32 scale =
-        nph_maxf(v3_len_sq(s.v[0].a), v3_len_sq(s.v[0].b)) + v3_len_sq(s.v[0].w) + 1e-12F;
-    int inside = 0;
-    NphVec3 v = gjk_simplex_solve(&s, &inside);
-    f32 v_sq = v3_len_sq(v);
-    u32 iterations = 0U;
-    while (!inside && iterations < 40U) {
-        if (v_sq <= 1e-12F * scale) {
-            inside = 1;
-            break;
-        }
-        iterations += 1U;
-        const NphVec3 d = v3_neg(v);
-        const u32 ia = support(a, q_rotate_inv(pa.rotation, d));
-        const u32 ib = support(b, q_rotate_inv(pb.rotation, v));
-        int duplicate = 0;
-        for (u32 i = 0U; i < s.count; ++i) {
-            duplicate = duplicate || (s.v[i].ia == ia && s.v[i].ib == ib);
-        }
-        if (duplicate) {
-            break;
-        }
-        const NphGjkVertex w = posed_vertex(a, pa, ia, b, pb, ib);
-        if (v_sq - v3_dot(v, w.w) <= 1e-6F * v_sq) {
-            break;
-        }
-        s.v[s.count++] = w;
-        const NphVec3 v_new = gjk_simplex_solve(&s, &inside);
-        const f32 v_new_sq = v3_len_sq(v_new);
-        if (!inside && v_new_sq >= v_sq) {
-            break;
-        }
-        v = v_new;
-        v_sq = v_new_sq;
+    gjk_begin(&g);
+    while (!g.done) {
+        const u32 ia = support(a, q_rotate_inv(pa.rotation, v3_neg(g.dir)));
+        const u32 ib = support(b, q_rotate_inv(pb.rotation, g.dir));
+        gjk_feed(&g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
     }
-    cache->count = s.count;
+    cache->count = g.count;
     for (u32 i = 0U; i < 4U; ++i) {
-        cache->index_a[i] = i < s.count ? (u8)s.v[i].ia : 0U;
-        cache->index_b[i] = i < s.count ? (u8)s.v[i].ib : 0U;
+        cache->index_a[i] = i < g.count ? (u8)g.v[i].ia : 0U;
+        cache->index_b[i] = i < g.count ? (u8)g.v[i].ib : 0U;
     }
-    NphVec3 pa_ = v3_zero();
-    NphVec3 pb_ = v3_zero();
-    f32 total = 0.0F;
-    for (u32 i = 0U; i < s.count; ++i) {
-        pa_ = v3_mul_add(pa_, s.v[i].a, s.lambda[i]);
-        pb_ = v3_mul_add(pb_, s.v[i].b, s.lambda[i]);
-        total += s.lambda[i];
-    }
-    if (total > 0.0F && total != 1.0F) {
-        pa_ = v3_scale(pa_, 1.0F / total);
-        pb_ = v3_scale(pb_, 1.0F / total);
-    }
-    const f32 dist = inside ? 0.0F : v3_dist(pa_, pb_);
-    NphVec3 normal = v3(0.0F, 1.0F, 0.0F);
-    if (dist > 0.0F) {
-        normal = v3_scale(v3_sub(pb_, pa_), 1.0F / dist);
-    }
-    out->point_a = pa_;
-    out->point_b = pb_;
-    out->normal = normal;
-    out->distance = dist;
+    gjk_finish(&g, out);
 }
 
 // The gap between m and t along n (cast.c:262-272).

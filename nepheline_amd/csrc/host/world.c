@@ -1409,6 +1409,46 @@ u64 nph_physics_world_hash(const NphPhysicsWorld *cw)
     return h;
 }
 
+void nph_amd_world_energy(const NphPhysicsWorld *cw, f64 out[3])
+{
+    // Mechanical energy of the dynamic bodies, in f64: linear kinetic
+    // 1/2 m |v|^2, angular kinetic 1/2 w_b . I w_b with w_b the angular
+    // velocity in body axes, and potential -m s (g . c) with s the body's
+    // gravity scale and c its centre of mass.
+    NphPhysicsWorld *w = (NphPhysicsWorld *)cw;
+    out[0] = out[1] = out[2] = 0.0;
+    if (world_sync_bodies(w) != NPH_RESULT_OK) {
+        return;
+    }
+    const NphVec3 g = w->desc.gravity;
+    for (u32 i = 0U; i < w->bodies.len; ++i) {
+        const HBody *hb = &VB(w->bodies, HBody)[i];
+        const NphBodyRec *b = rec_at(w, i);
+        if (hb->handle == 0U || b->type != (u32)NPH_PHYSICS_BODY_DYNAMIC || !(b->mass > 0.0F)) {
+            continue;
+        }
+        const f64 m = b->mass;
+        const f64 v[3] = {b->v.x, b->v.y, b->v.z};
+        out[0] += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        // Body-axis angular velocity: the rotation's inverse applied to w.
+        const f64 qx = -b->rotation.x, qy = -b->rotation.y, qz = -b->rotation.z,
+                  qw = b->rotation.w;
+        const f64 wx = b->w.x, wy = b->w.y, wz = b->w.z;
+        const f64 tx = 2.0 * (qy * wz - qz * wy), ty = 2.0 * (qz * wx - qx * wz),
+                  tz = 2.0 * (qx * wy - qy * wx);
+        const f64 wb[3] = {wx + qw * tx + (qy * tz - qz * ty), wy + qw * ty + (qz * tx - qx * tz),
+                           wz + qw * tz + (qx * ty - qy * tx)};
+        const NphMat3 *I = &b->inertia;
+        for (u32 r = 0U; r < 3U; ++r) {
+            const f64 row = (f64)(&I->c[0].x)[r] * wb[0] + (f64)(&I->c[1].x)[r] * wb[1] +
+                            (f64)(&I->c[2].x)[r] * wb[2];
+            out[1] += 0.5 * wb[r] * row;
+        }
+        out[2] -= m * (f64)b->gravity_scale *
+                  ((f64)g.x * b->center.x + (f64)g.y * b->center.y + (f64)g.z * b->center.z);
+    }
+}
+
 bool nph_physics_world_validate(const NphPhysicsWorld *w)
 {
     // The bookkeeping's invariants (world.c:482-695, the parts kept here):

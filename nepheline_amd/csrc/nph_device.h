@@ -203,7 +203,9 @@ NphResult nph_devb_warm_stats(NphDeviceBatch *b, f64 out[8], u32 *steps, int res
 // Per-launch timing of the batched colour passes (not in graph mode):
 // out[0] = milliseconds summed, out[1] = launches.
 void nph_devb_time_kernels(NphDeviceBatch *b, int enable);
-void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[2]);
+// out: colour-pass milliseconds and launches, convex narrow-phase kernel
+// milliseconds, launches and pairs, since timing was enabled.
+void nph_devb_kernel_times(const NphDeviceBatch *b, f64 out[5]);
 
 #ifdef __cplusplus
 }

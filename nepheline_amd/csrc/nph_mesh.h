@@ -14,7 +14,7 @@
 #ifndef NPH_MESH_H
 #define NPH_MESH_H
 
-#include "nph_collide.h"
+#include "nph_polytope.h"
 
 #define NPH_MESH_MERGE_COS 0.99619470F
 #define NPH_MESH_MERGE_DISTANCE 1e-4F
@@ -490,6 +490,9 @@ NPH_HD NphView triangle_view(const NphVec3 v[3], NphVec3 n, NphTriView *s)
     return h;
 }
 
+// The pair functions below run the polytope code (C++, nph_polytope.h).
+#if defined(__cplusplus)
+
 NPH_HD void hull_triangle(const NphView *a, NphPhysicsPose pa, const NphTri *t, f32 speculative,
                           NphTriContact *out)
 {
@@ -926,5 +929,7 @@ NPH_HD void collide_mesh_local(const NphGeom *a, NphQuat rotation, NphDVec3 posi
     mesh_visit_box(mesh, box, position, q);
     surface_finish(q, out);
 }
+
+#endif // __cplusplus
 
 #endif

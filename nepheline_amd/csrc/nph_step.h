@@ -594,6 +594,7 @@ NPH_HD void pair_surface(const NphShapeRec *a, const NphShapeRec *b, f32 *fricti
         combine_values(a->restitution, a->combine >> 8U, b->restitution, b->combine >> 8U, 1);
 }
 
+#if defined(__cplusplus) // the narrow phase of a pair (nph_polytope.h is C++)
 // contact.c:378-483: a pair with a mesh, up to four manifolds, the deepest
 // in the pair's record and the rest in its children; impulses carried over
 // the whole chain, by id then by place (with the manifolds' normals within
@@ -743,28 +744,54 @@ NPH_HD int contact_has_big_hull(const NphContactRec *c, const NphShapeRec *shape
     return ((shapes[c->shape_a].flags | shapes[c->shape_b].flags) & NPH_SHAPE_BIG_HULL) != 0U;
 }
 
-// contact.c:516-577 for a convex pair: the disjoint test, the manifold in
-// A's centre-of-mass frame, impulses carried by id then by place.
-NPH_HD void contact_update(NphContactRec *contacts, u32 index, const NphShapeRec *shapes,
-                           const NphBodyRec *bodies, f32 step_dt)
+// A convex pair's narrow-phase input (contact.c:516-530): both shapes'
+// geometry and poses in A's centre-of-mass frame, the speculative distance,
+// B's centre in that frame and the pair's cache.
+typedef struct NphPairJob {
+    NphGeom ga;
+    NphGeom gb;
+    NphPhysicsPose pa;
+    NphPhysicsPose pb;
+    NphVec3 ab;
+    f32 speculative;
+    NphCache cache;
+} NphPairJob;
+
+// False when the pair's fat boxes are apart (its change is then
+// NPH_CHANGE_DISJOINT and the record is left as it is).
+NPH_HD int contact_job(const NphContactRec *c, const NphShapeRec *shapes, const NphBodyRec *bodies,
+                       f32 step_dt, NphPairJob *job)
 {
-    NphContactRec *c = &contacts[index];
     const NphShapeRec *sa = &shapes[c->shape_a];
     const NphShapeRec *sb = &shapes[c->shape_b];
-    c->change = 0U;
-    const int was_touching = c->point_count > 0U;
     if (!aabb_overlaps(sa->fat_aabb, sb->fat_aabb)) {
-        c->change = NPH_CHANGE_DISJOINT;
-        return;
+        return 0;
     }
     const NphBodyRec *ba = &bodies[c->body_a];
     const NphBodyRec *bb = &bodies[c->body_b];
     const NphDVec3 origin = ba->center;
-    const NphPhysicsPose pa = shape_pose_relative(ba, sa, origin);
-    const NphPhysicsPose pb = shape_pose_relative(bb, sb, origin);
-    NphManifold m;
-    nph_collide(&sa->geom, pa, &sb->geom, pb, pair_speculative(ba, bb, step_dt), &c->cache, &m);
-    const NphVec3 ab = dv3_relative(bb->center, origin);
+    job->ga = sa->geom;
+    job->gb = sb->geom;
+    job->pa = shape_pose_relative(ba, sa, origin);
+    job->pb = shape_pose_relative(bb, sb, origin);
+    job->ab = dv3_relative(bb->center, origin);
+    job->speculative = pair_speculative(ba, bb, step_dt);
+    job->cache = c->cache;
+    return 1;
+}
+
+// contact.c:531-577: the new manifold into the record, impulses carried by
+// id then by place, and the change of touching.
+NPH_HD void contact_store(NphContactRec *c, const NphShapeRec *shapes, const NphPairJob *job,
+                          const NphManifold *mp)
+{
+    const NphManifold m = *mp;
+    const NphShapeRec *sa = &shapes[c->shape_a];
+    const NphShapeRec *sb = &shapes[c->shape_b];
+    const NphVec3 ab = job->ab;
+    const int was_touching = c->point_count > 0U;
+    c->change = 0U;
+    c->cache = job->cache;
     NphPointRec points[NPH_PHYSICS_MANIFOLD_MAX_POINTS];
     const u32 old_count = c->point_count;
     u32 claimed = 0U;
@@ -829,6 +856,7 @@ NPH_HD void contact_update(NphContactRec *contacts, u32 index, const NphShapeRec
         c->change = NPH_CHANGE_STOPPED;
     }
 }
+#endif // __cplusplus
 
 // ---------------------------------------------------------------------------
 // The soft-step lanes (solver_impl.h:107-668), one lane of a group

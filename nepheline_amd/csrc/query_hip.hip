@@ -19,6 +19,7 @@
 // the large-hull kernel's clip capacity (the reference's holds them).
 
 #include "device_impl.h"
+#include "device_narrow.h"
 
 #include <mutex>
 
@@ -233,19 +234,32 @@ NphResult static_blocks(const NphAmdCollideQuery *queries, u32 n, u64 base, Stat
 
 } // namespace
 
-__global__ void __launch_bounds__(64) k_query_collide(const NphQueryPair *pairs, u32 n,
-                                                      NphManifold *out, NphCache *caches)
+// A team a pair (device_narrow.h); hulls beyond 32 vertices in the
+// large-hull build.
+__global__ void __launch_bounds__(NPH_TEAM_BLOCK) k_query_teams(const NphQueryPair *pairs, u32 n,
+                                                                NphManifold *out, NphCache *caches)
 {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || pairs[i].big != 0U) {
-        return; // a hull beyond 32 vertices: the large-hull kernel
+    query_teams<NPH_HULL_MAX_VERTICES, NPH_CLIP_CAPACITY, NPH_TEAM_BLOCK>(pairs, n, 0, out, caches);
+}
+
+__global__ void __launch_bounds__(NPH_BIG_TEAM_BLOCK) k_query_big_teams(const NphQueryPair *pairs, u32 n,
+                                                                        NphManifold *out,
+                                                                        NphCache *caches)
+{
+    query_teams<NPH_BIG_HULL_VERTICES, NPH_BIG_CLIP_CAPACITY, NPH_BIG_TEAM_BLOCK>(pairs, n, 1, out,
+                                                                                  caches);
+}
+
+// Launches the pair queries of `pairs` on `s`.
+static void query_launch(hipStream_t s, const NphQueryPair *pairs, u32 n, bool any_big,
+                         NphManifold *out, NphCache *caches)
+{
+    hipLaunchKernelGGL(k_query_teams, dim3(devi_team_grid(n, NPH_TEAM_BLOCK)), dim3(NPH_TEAM_BLOCK),
+                       0, s, pairs, n, out, caches);
+    if (any_big) {
+        hipLaunchKernelGGL(k_query_big_teams, dim3(devi_team_grid(n, NPH_BIG_TEAM_BLOCK)),
+                           dim3(NPH_BIG_TEAM_BLOCK), 0, s, pairs, n, out, caches);
     }
-    const NphQueryPair p = pairs[i];
-    NphCache cache = p.cache;
-    NphManifold m;
-    nph_collide(&p.a, p.pose_a, &p.b, p.pose_b, p.speculative, &cache, &m);
-    out[i] = m;
-    caches[i] = cache;
 }
 
 __global__ void __launch_bounds__(64) k_query_reduce(NphMPoint *points, u32 count, NphVec3 normal,
@@ -328,11 +342,7 @@ NphResult nph_amd_collide_many(const NphAmdCollideQuery *queries, u32 n, NphPhys
         const hipError_t e1 = hipMemcpyAsync(q->pairs, h, (u64)n * sizeof(NphQueryPair),
                                              hipMemcpyHostToDevice, q->stream);
         if (e1 == hipSuccess) {
-            hipLaunchKernelGGL(k_query_collide, dim3(devi_blocks(n, 64U)), dim3(64), 0, q->stream,
-                               q->pairs, n, q->manifolds, q->caches);
-            if (any_big) {
-                devi_query_collide_big(q->stream, q->pairs, n, q->manifolds, q->caches);
-            }
+            query_launch(q->stream, q->pairs, n, any_big, q->manifolds, q->caches);
         }
         const hipError_t e2 = e1 == hipSuccess ? hipGetLastError() : e1;
         const hipError_t e3 =
@@ -449,11 +459,7 @@ NphResult nph_amd_collide_many_static(const NphAmdCollideQuery *queries, u32 n,
         HIP_TRY(hipMemcpy(pairs + first, h, (u64)m * sizeof(NphQueryPair),
                           hipMemcpyHostToDevice));
     }
-    hipLaunchKernelGGL(k_query_collide, dim3(devi_blocks(n, 64U)), dim3(64), 0, 0, pairs, n,
-                       manifolds, dcaches);
-    if (any_big) {
-        devi_query_collide_big(0, pairs, n, manifolds, dcaches);
-    }
+    query_launch(0, pairs, n, any_big, manifolds, dcaches);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(out, manifolds, (u64)n * sizeof(NphManifold), hipMemcpyDeviceToHost));
     if (caches != NULL) {

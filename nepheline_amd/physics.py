@@ -725,6 +725,34 @@ def contacts_as_arrays(contacts):
     return pairs, ids, by_pair
 
 
+def canonical_digests(world) -> dict:
+    """The canonical pair and contact-id lists of contacts_as_arrays, built
+    with numpy from one contact readback (for worlds of 10^5 bodies), and the
+    SHA-256 of each as little-endian u32 rows: pairs (min, max shape slot) of
+    parent records, sorted; ids (shape_a, shape_b, chained, point id) of
+    every touching manifold's points, sorted."""
+    import hashlib
+    n = world.api.world_contacts(world.ptr, None, 0)
+    buf = (ContactInfo * max(n, 1))()
+    world.api.world_contacts(world.ptr, buf, n)
+    arr = np.frombuffer(buf, dtype=np.dtype(ContactInfo), count=n)
+    parent = arr["parent"] == NULL32
+    a, b = arr["shape_a"].astype(np.uint32), arr["shape_b"].astype(np.uint32)
+    pairs = np.stack([np.minimum(a, b), np.maximum(a, b)], 1)[parent]
+    pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))] if len(pairs) else pairs
+    rows = []
+    for k in range(4):
+        sel = arr["point_count"] > k
+        ids = arr["points"]["id"][sel, k].astype(np.uint32)
+        rows.append(np.stack([a[sel], b[sel], (~parent[sel]).astype(np.uint32), ids], 1))
+    ids = np.concatenate(rows) if rows else np.zeros((0, 4), np.uint32)
+    if len(ids):
+        ids = ids[np.lexsort((ids[:, 3], ids[:, 2], ids[:, 1], ids[:, 0]))]
+    return {"pairs": int(len(pairs)), "ids": int(len(ids)),
+            "pairs_sha256": hashlib.sha256(np.ascontiguousarray(pairs, "<u4").tobytes()).hexdigest(),
+            "ids_sha256": hashlib.sha256(np.ascontiguousarray(ids, "<u4").tobytes()).hexdigest()}
+
+
 # ---------------------------------------------------------------------------
 # Batched worlds (nph_amd_batch_*: this library's extension)
 # ---------------------------------------------------------------------------
