@@ -14,8 +14,9 @@ post-solve residual measured in it, as it is for the learned start.
 The same line carries: the learned warm start on the same batch and steps
 (a network trained on the device from plain bin drops of other seeds), with
 the residual, largest penetration and mechanical energy of both starts; the
-roofline of the two heaviest kernels of the batch, the narrow phase
-(k_b_collide) and the colour pass (k_b_color_pass), from dispatch-stamped HIP
+roofline of the two heaviest kernels of the batch, the convex narrow phase
+(k_b_narrow_lanes then k_b_narrow_teams) and the colour pass
+(k_b_color_pass), from dispatch-stamped HIP
 events on the stream they run on, in a window of steps right after the
 timed region; the reference's own CPU step (oracle/_ref) on a sample of the
 batch's worlds, on every host thread the run may use and on one thread; and
@@ -227,7 +228,8 @@ def batch_rooflines(a, batch, plan, steps):
     cavg_s = ct[0] * 1e-3 / claunches
     cbytes = ct[2] * COLLIDE_PAIR_BYTES / claunches
     collide = roofline("hbm", cbytes / cavg_s / 1e9 if cavg_s > 0 else 0.0, HBM_PEAK_GBS,
-                       "GB/s", kernel="k_b_collide",
+                       "GB/s", kernel="k_b_narrow_lanes + k_b_narrow_teams (the lane "
+                       "pass's start to the team pass's end)",
                        window="%d steps after the timed region, one stream" % steps,
                        launches=int(ct[1]), avg_launch_us=cavg_s * 1e6, device_ms=ct[0],
                        pairs_per_launch=ct[2] / claunches, bytes_per_pair=COLLIDE_PAIR_BYTES,
@@ -586,7 +588,7 @@ def main():
         "vs_cpu": plain["value"] / cpu_line["value"] if cpu_line else None,
         "plain": plain,
         "learned_warm_start": learned,
-        "rooflines": {"k_b_collide": collide, "k_b_color_pass": color},
+        "rooflines": {"narrow_phase": collide, "k_b_color_pass": color},
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
