@@ -99,13 +99,33 @@ __device__ inline void narrow_lane(bool valid, const NphSortedEntry &e, const Wo
             if (!contact_job(rec, c.shapes, c.bodies, step_dt, &job)) {
                 rec->change = NPH_CHANGE_DISJOINT;
                 codes[e.index] = NPH_CHANGE_DISJOINT;
-            } else if (!pair_has_closed_form(job.ga.type, job.gb.type)) {
-                route = contact_has_big_hull(rec, c.shapes) ? 2 : 1;
             } else {
                 NphManifold m;
-                collide_closed_form(&job.ga, job.pa, &job.gb, job.pb, job.speculative, &m);
-                contact_store(rec, c.shapes, &job, &m);
-                codes[e.index] = rec->change;
+                int done = 1;
+                if (pair_wants_team(job.ga.type, job.gb.type)) {
+                    // A cached axis that still separates ends the pair here,
+                    // as it would end it on the team.
+                    done = cached_axis_separates(&job.ga, job.pa, &job.gb, job.pb,
+                                                 job.speculative, &job.cache);
+                    if (done) {
+                        manifold_clear(&m);
+                        if (job.ga.type < job.gb.type) {
+                            m.normal = v3_neg(m.normal);
+                        }
+                    } else {
+                        route = contact_has_big_hull(rec, c.shapes) ? 2 : 1;
+                    }
+                } else if (pair_has_closed_form(job.ga.type, job.gb.type)) {
+                    collide_closed_form(&job.ga, job.pa, &job.gb, job.pb, job.speculative, &m);
+                } else {
+                    NphLaneScratch sc;
+                    lane_collide(&job.ga, job.pa, &job.gb, job.pb, job.speculative, &job.cache,
+                                 &sc, &m);
+                }
+                if (done) {
+                    contact_store(rec, c.shapes, &job, &m);
+                    codes[e.index] = rec->change;
+                }
             }
         }
     }

@@ -1118,6 +1118,109 @@ NPH_TD void team_collide(const NphTeam<T> &tm, NphPolyScratch<V, CAP> *sc, const
     out->normal = v3_neg(out->normal);
 }
 
+// ---------------------------------------------------------------------------
+// The lane pass's share of the polytope pairs
+// ---------------------------------------------------------------------------
+
+// Whether a convex pair runs on a team: a hull against a box or a hull (the
+// pairs whose edge-pair search is long). Boxes against boxes and capsules,
+// and hulls against spheres and capsules, are a lane's work.
+NPH_TD int pair_wants_team(u32 ta, u32 tb)
+{
+    const int hull_a = ta == NPH_PHYSICS_SHAPE_HULL, hull_b = tb == NPH_PHYSICS_SHAPE_HULL;
+    const int poly_a = hull_a || ta == NPH_PHYSICS_SHAPE_BOX;
+    const int poly_b = hull_b || tb == NPH_PHYSICS_SHAPE_BOX;
+    return (hull_a || hull_b) && poly_a && poly_b;
+}
+
+// For a pair that runs on a team: whether the axis in the cache still
+// separates it by more than the speculative distance, which ends the pair
+// with no points and the cache as it is (collide_hull.c:571-594). The same
+// arithmetic as team_sat's first test, one lane.
+NPH_TD int cached_axis_separates(const NphGeom *ga, NphPhysicsPose pose_a, const NphGeom *gb,
+                                 NphPhysicsPose pose_b, f32 speculative_distance,
+                                 const NphCache *cache)
+{
+    if (cache->type == NPH_CACHE_NONE) {
+        return 0;
+    }
+    // The team orders the pair by type (hulls first).
+    const int swap = ga->type < gb->type;
+    const NphGeom *a = swap ? gb : ga;
+    const NphGeom *b = swap ? ga : gb;
+    const NphPhysicsPose pa = swap ? pose_b : pose_a;
+    const NphPhysicsPose pb = swap ? pose_a : pose_b;
+    const f32 limit = nph_maxf(speculative_distance, 0.0F);
+    NphBoxStorage box;
+    const NphView va = a->type == NPH_PHYSICS_SHAPE_BOX ? box_view(v3(a->a, a->b, a->c), &box)
+                                                        : hull_view(a->hull);
+    NphBoxStorage boxb;
+    const NphView vb = b->type == NPH_PHYSICS_SHAPE_BOX ? box_view(v3(b->a, b->b, b->c), &boxb)
+                                                        : hull_view(b->hull);
+    NphPolyPair pp;
+    pp.a = &va;
+    pp.b = &vb;
+    pp.rel = pose_inv_mul(pa, pb);
+    f32 s = -NPH_F32_MAX;
+    if (cache->type == NPH_CACHE_FACE_A && cache->index_a < va.face_count) {
+        const NphPlane pl = va.planes[cache->index_a];
+        f32 m = v3_dot(pose_point(pp.rel, vb.vertices[0]), pl.normal);
+        for (u32 i = 1U; i < vb.vertex_count; ++i) {
+            m = nph_minf(m, v3_dot(pose_point(pp.rel, vb.vertices[i]), pl.normal));
+        }
+        s = m + pl.d;
+    } else if (cache->type == NPH_CACHE_FACE_B && cache->index_b < vb.face_count) {
+        const NphPlane pl = face_b_plane(pp, cache->index_b);
+        s = lowest_projection(va.vertices, va.vertex_count, pl.normal) + pl.d;
+    } else if (cache->type == NPH_CACHE_EDGES && cache->index_a < va.edge_count &&
+               cache->index_b < vb.edge_count) {
+        const u32 ea = cache->index_a;
+        const u32 eb = cache->index_b;
+        const NphVec3 p_a = va.vertices[va.edges[ea].origin];
+        const NphVec3 p_b = pose_point(pp.rel, vb.vertices[vb.edges[eb].origin]);
+        const NphVec3 h_b = pose_point(pp.rel, vb.vertices[edge_end(&vb, eb)]);
+        NphVec3 axis;
+        s = edge_axis_separation(p_a, v3_sub(va.vertices[edge_end(&va, ea)], p_a), p_b,
+                                 v3_sub(h_b, p_b), va.centroid, &axis);
+    }
+    return s > limit;
+}
+
+// nph_physics_collide for the pairs the lane pass keeps: closed forms,
+// boxes against boxes and capsules, hulls against spheres and capsules.
+// The scratch holds two box views and one box's vertices and clip polygons.
+typedef NphPolyScratch<8U, 12U> NphLaneScratch;
+
+NPH_TD void lane_collide(const NphGeom *a, NphPhysicsPose pa, const NphGeom *b, NphPhysicsPose pb,
+                         f32 speculative_distance, NphCache *cache, NphLaneScratch *sc,
+                         NphManifold *out)
+{
+    const NphTeam<1U> one;
+    manifold_clear(out);
+    const f32 speculative = nph_maxf(speculative_distance, 0.0F);
+    const int swap = a->type < b->type; // as team_collide: the higher type first
+    const NphGeom *x = swap ? b : a;
+    const NphGeom *y = swap ? a : b;
+    const NphPhysicsPose px = swap ? pb : pa;
+    const NphPhysicsPose py = swap ? pa : pb;
+    if (!closed_form_ordered(x, px, y, py, speculative, out)) {
+        if (x->type == NPH_PHYSICS_SHAPE_BOX && y->type == NPH_PHYSICS_SHAPE_BOX) {
+            team_boxes(one, sc, v3(x->a, x->b, x->c), px, v3(y->a, y->b, y->c), py, speculative,
+                       cache, out);
+        } else {
+            const NphView vx = team_view(one, sc, x, 0U);
+            if (y->type == NPH_PHYSICS_SHAPE_SPHERE) {
+                team_hull_sphere(one, &vx, px, y->a, py, speculative, out);
+            } else {
+                team_hull_capsule(one, &vx, px, y->a, y->b, py, speculative, out);
+            }
+        }
+    }
+    if (swap) {
+        out->normal = v3_neg(out->normal);
+    }
+}
+
 // One lane, its own scratch (private memory).
 NPH_TD void nph_collide(const NphGeom *a, NphPhysicsPose pa, const NphGeom *b, NphPhysicsPose pb,
                         f32 speculative, NphCache *cache, NphManifold *out)
