@@ -561,7 +561,7 @@ f32 nph_physics_joint_hinge_angle(const NphPhysicsWorld *w, NphPhysicsJoint join
     if (j == NULL) {
         return 0.0F;
     }
-    return jnt_twist_angle(q_mul(jrec(w, j->body_a)->rotation, j->frame_a.rotation),
+    return joint_twist_angle(q_mul(jrec(w, j->body_a)->rotation, j->frame_a.rotation),
                            q_mul(jrec(w, j->body_b)->rotation, j->frame_b.rotation));
 }
 

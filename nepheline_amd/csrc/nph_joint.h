@@ -1,16 +1,25 @@
 // SPDX-License-Identifier: Apache-2.0
-// Joints on the device: the ball, hinge and fixed joints' preparation, warm
-// start and soft/rigid solve (joint_solver.c:32-394, 726-907), one thread per
-// joint, in the colour passes beside the contacts.
+// Joints on the device: ball, hinge and fixed joints as ordered lists of
+// constraint rows, solved one joint a thread in the colour passes beside the
+// contacts, with the reference's arithmetic (joint_solver.c:32-394,726-907).
 //
 // A joint's record is two parts kept by joint slot in HBM: the host-written
 // configuration (type, bodies, frames, limits, motor, spring) and the
-// device-written state (accumulated impulses, kept between steps for the warm
-// start, and what the step's preparation fixes: solver indices, masses,
-// anchors and frames at the step's start).  The rows run in the reference's
-// order (motor or spring, limits, locked axes, the anchors last) with its
-// arithmetic, operation for operation; joints are scalar in the reference
-// too, so nothing here is lane-shaped.
+// device-written state (accumulated impulses, kept between steps for the
+// warm start, and what the step's preparation fixes: solver indices,
+// masses, anchors and frames at the step's start).
+//
+// A solve first measures the joint's geometry from the bodies' substep
+// rotations (the hinge's axis and angle, the ball's swing and twist axes
+// and gaps; positions do not change within a pass), then runs the rows of
+// the joint's type in order. Each row applies its impulse to the bodies'
+// velocities at once, so a later row sees the earlier rows' effect:
+//   hinge  spring, motor, lower limit, upper limit, the two locked axes,
+//          the anchors
+//   ball   friction, swing cone, lower twist, upper twist, the anchors
+//   fixed  the orientation, the anchors
+// Rows are soft while solving (the world's joint softness), rigid while
+// relaxing, and limits are speculative while their gap is open.
 #ifndef NPH_JOINT_H
 #define NPH_JOINT_H
 
@@ -65,126 +74,8 @@ struct NphJointState {
     f32 spring_impulse_scale;
 };
 
-// ---------------------------------------------------------------------------
-// Small linear algebra (joint_solver.c:32-96)
-// ---------------------------------------------------------------------------
-
-// K x = b for a symmetric K by columns, by Cramer's rule; zero unless the
-// determinant is positive.
-NPH_HD NphVec3 jnt_solve3(const NphVec3 k[3], NphVec3 b)
-{
-    const NphVec3 c12 = v3_cross(k[1], k[2]);
-    const f32 det = v3_dot(k[0], c12);
-    if (!(det > 0.0F)) {
-        return v3_zero();
-    }
-    const f32 inv = 1.0F / det;
-    return v3(v3_dot(b, c12) * inv, v3_dot(k[0], v3_cross(b, k[2])) * inv,
-              v3_dot(k[0], v3_cross(k[1], b)) * inv);
-}
-
-// (mA + mB) I - [rA]x IA^-1 [rA]x - [rB]x IB^-1 [rB]x, column by column.
-NPH_HD void jnt_point_mass(const NphJointState *j, NphVec3 ra, NphVec3 rb, NphVec3 k[3])
-{
-    const f32 m = j->inv_mass_a + j->inv_mass_b;
-    for (u32 c = 0U; c < 3U; ++c) {
-        const NphVec3 e = v3(c == 0U ? 1.0F : 0.0F, c == 1U ? 1.0F : 0.0F, c == 2U ? 1.0F : 0.0F);
-        const NphVec3 ta = m3_mul_v3(j->inv_inertia_a, v3_cross(ra, e));
-        const NphVec3 tb = m3_mul_v3(j->inv_inertia_b, v3_cross(rb, e));
-        k[c] = v3_sub(v3_sub(v3_scale(e, m), v3_cross(ra, ta)), v3_cross(rb, tb));
-    }
-}
-
-NPH_HD void jnt_angular_mass(const NphJointState *j, NphVec3 k[3])
-{
-    for (u32 c = 0U; c < 3U; ++c) {
-        k[c] = v3_add(j->inv_inertia_a.c[c], j->inv_inertia_b.c[c]);
-    }
-}
-
-NPH_HD f32 jnt_axial_mass(const NphJointState *j, NphVec3 axis)
-{
-    const f32 k = v3_dot(axis, m3_mul_v3(j->inv_inertia_a, axis)) +
-                  v3_dot(axis, m3_mul_v3(j->inv_inertia_b, axis));
-    return k > 0.0F ? 1.0F / k : 0.0F;
-}
-
-NPH_HD NphQuat jnt_qneg(NphQuat q) { return q4(-q.x, -q.y, -q.z, -q.w); }
-
-// B's twist from A's frame about A's X axis, in [-pi, pi].
-NPH_HD f32 jnt_twist_angle(NphQuat qa, NphQuat qb)
-{
-    NphQuat r = q_mul(q_conj(qa), qb);
-    if (r.w < 0.0F) {
-        r = jnt_qneg(r);
-    }
-    return 2.0F * nph_atan2f(r.x, r.w);
-}
-
-NPH_HD f32 jnt_unwind(f32 a)
-{
-    const f32 pi = 3.14159265F;
-    if (a > pi) {
-        return a - 2.0F * pi;
-    }
-    if (a < -pi) {
-        return a + 2.0F * pi;
-    }
-    return a;
-}
-
-// ---------------------------------------------------------------------------
-// Bodies (joint_solver.c:114-165)
-// ---------------------------------------------------------------------------
-typedef struct NphJointNow {
-    NphSolverBody *a;
-    NphSolverBody *b;
-    NphVec3 ra; // anchors from the centres of mass, now
-    NphVec3 rb;
-    NphQuat qa; // frames, now
-    NphQuat qb;
-} NphJointNow;
-
-NPH_HD NphJointNow jnt_now(NphSolverBody *bodies, const NphJointState *j)
-{
-    NphJointNow s;
-    s.a = &bodies[j->solver_a];
-    s.b = &bodies[j->solver_b];
-    s.ra = q_rotate(sb_dq(s.a), j->anchor_a);
-    s.rb = q_rotate(sb_dq(s.b), j->anchor_b);
-    s.qa = q_mul(sb_dq(s.a), j->frame_q_a);
-    s.qb = q_mul(sb_dq(s.b), j->frame_q_b);
-    return s;
-}
-
-// A linear impulse p at the anchors: B gains it, A loses it.
-NPH_HD void jnt_apply_linear(const NphJointState *j, NphJointNow *s, NphVec3 p)
-{
-    if ((j->write & NPH_JOINT_WRITE_A) != 0U) {
-        sb_set_v(s->a, v3_mul_add(sb_v(s->a), p, -j->inv_mass_a));
-        sb_set_w(s->a, v3_sub(sb_w(s->a), m3_mul_v3(j->inv_inertia_a, v3_cross(s->ra, p))));
-    }
-    if ((j->write & NPH_JOINT_WRITE_B) != 0U) {
-        sb_set_v(s->b, v3_mul_add(sb_v(s->b), p, j->inv_mass_b));
-        sb_set_w(s->b, v3_add(sb_w(s->b), m3_mul_v3(j->inv_inertia_b, v3_cross(s->rb, p))));
-    }
-}
-
-NPH_HD void jnt_apply_angular(const NphJointState *j, NphJointNow *s, NphVec3 l)
-{
-    if ((j->write & NPH_JOINT_WRITE_A) != 0U) {
-        sb_set_w(s->a, v3_sub(sb_w(s->a), m3_mul_v3(j->inv_inertia_a, l)));
-    }
-    if ((j->write & NPH_JOINT_WRITE_B) != 0U) {
-        sb_set_w(s->b, v3_add(sb_w(s->b), m3_mul_v3(j->inv_inertia_b, l)));
-    }
-}
-
-NPH_HD NphVec3 jnt_relative_angular(const NphJointNow *s) { return v3_sub(sb_w(s->b), sb_w(s->a)); }
-
-// ---------------------------------------------------------------------------
-// Rows (joint_solver.c:170-394)
-// ---------------------------------------------------------------------------
+// A row's soft-constraint coefficients: velocity bias per unit of error,
+// mass scale and accumulated-impulse scale.
 typedef struct NphJointSoft {
     f32 bias;
     f32 mass;
@@ -194,209 +85,443 @@ typedef struct NphJointSoft {
 typedef struct NphJointStep {
     f32 h;
     f32 inv_h;
-    NphJointSoft soft; // the world's joint stiffness at the substep
+    NphJointSoft soft; // the world's joint softness at the substep
 } NphJointStep;
 
-// Speculative while the gap c is open, soft while solving, rigid relaxing.
-NPH_HD NphJointSoft jnt_limit_soft(f32 c, const NphJointStep *step, int use_bias)
+// ---------------------------------------------------------------------------
+// Effective masses
+// ---------------------------------------------------------------------------
+
+// x with K x = rhs for a symmetric positive K given by columns (Cramer's
+// rule); zero when the determinant is not positive.
+NPH_HD NphVec3 joint_solve_sym3(const NphVec3 k[3], NphVec3 rhs)
 {
-    NphJointSoft r = {0.0F, 1.0F, 0.0F};
-    if (c > 0.0F) {
-        r.bias = c * step->inv_h;
-    } else if (use_bias) {
-        r.bias = step->soft.bias * c;
-        r.mass = step->soft.mass;
-        r.impulse = step->soft.impulse;
+    const NphVec3 c12 = v3_cross(k[1], k[2]);
+    const f32 det = v3_dot(k[0], c12);
+    if (!(det > 0.0F)) {
+        return v3_zero();
     }
-    return r;
+    const f32 inv = 1.0F / det;
+    return v3(v3_dot(rhs, c12) * inv, v3_dot(k[0], v3_cross(rhs, k[2])) * inv,
+              v3_dot(k[0], v3_cross(k[1], rhs)) * inv);
 }
 
-// One-sided angular row: the gap c closes at dot(wA - wB, axis).
-NPH_HD void jnt_angular_limit(const NphJointState *j, NphJointNow *s, NphVec3 axis, f32 c,
-                              f32 *accumulated, const NphJointStep *step, int use_bias)
+// The anchors' inverse mass matrix, (mA + mB) 1 - [rA] IA^-1 [rA] - [rB] IB^-1
+// [rB] (cross-product matrices), by columns.
+NPH_HD void joint_anchor_matrix(const NphJointState *j, NphVec3 ra, NphVec3 rb, NphVec3 k[3])
 {
-    const NphJointSoft r = jnt_limit_soft(c, step, use_bias);
-    const f32 cdot = -v3_dot(jnt_relative_angular(s), axis);
-    const f32 impulse =
-        -jnt_axial_mass(j, axis) * r.mass * (cdot + r.bias) - r.impulse * *accumulated;
-    const f32 total = nph_maxf(*accumulated + impulse, 0.0F);
-    const f32 delta = total - *accumulated;
-    *accumulated = total;
-    jnt_apply_angular(j, s, v3_scale(axis, -delta));
-}
-
-// The anchors meet: three rows at once.
-NPH_HD void jnt_point(NphJointState *j, NphJointNow *s, const NphJointStep *step, int use_bias)
-{
-    const NphVec3 va = v3_add(sb_v(s->a), v3_cross(sb_w(s->a), s->ra));
-    const NphVec3 vb = v3_add(sb_v(s->b), v3_cross(sb_w(s->b), s->rb));
-    const NphVec3 cdot = v3_sub(vb, va);
-    NphVec3 bias = v3_zero();
-    f32 mass_scale = 1.0F;
-    f32 impulse_scale = 0.0F;
-    if (use_bias) {
-        const NphVec3 c = v3_add(v3_add(v3_sub(sb_dp(s->b), sb_dp(s->a)), v3_sub(s->rb, s->ra)),
-                                 j->delta_center);
-        bias = v3_scale(c, step->soft.bias);
-        mass_scale = step->soft.mass;
-        impulse_scale = step->soft.impulse;
-    }
-    NphVec3 k[3];
-    jnt_point_mass(j, s->ra, s->rb, k);
-    const NphVec3 x = jnt_solve3(k, v3_add(cdot, bias));
-    const NphVec3 impulse =
-        v3_sub(v3_scale(x, -mass_scale), v3_scale(j->linear_impulse, impulse_scale));
-    j->linear_impulse = v3_add(j->linear_impulse, impulse);
-    jnt_apply_linear(j, s, impulse);
-}
-
-// The frames align (fixed): the rotation vector from A's frame to B's.
-NPH_HD void jnt_rotation_lock(NphJointState *j, NphJointNow *s, const NphJointStep *step,
-                              int use_bias)
-{
-    const NphVec3 cdot = jnt_relative_angular(s);
-    NphVec3 bias = v3_zero();
-    f32 mass_scale = 1.0F;
-    f32 impulse_scale = 0.0F;
-    if (use_bias) {
-        NphQuat e = q_mul(s->qb, q_conj(s->qa));
-        if (e.w < 0.0F) {
-            e = jnt_qneg(e);
+    const f32 m = j->inv_mass_a + j->inv_mass_b;
+    for (u32 col = 0U; col < 3U; ++col) {
+        NphVec3 e = v3_zero();
+        if (col == 0U) {
+            e.x = 1.0F;
+        } else if (col == 1U) {
+            e.y = 1.0F;
+        } else {
+            e.z = 1.0F;
         }
-        bias = v3_scale(v3(e.x, e.y, e.z), 2.0F * step->soft.bias);
+        const NphVec3 turn_a = m3_mul_v3(j->inv_inertia_a, v3_cross(ra, e));
+        const NphVec3 turn_b = m3_mul_v3(j->inv_inertia_b, v3_cross(rb, e));
+        k[col] = v3_sub(v3_sub(v3_scale(e, m), v3_cross(ra, turn_a)), v3_cross(rb, turn_b));
+    }
+}
+
+// The rotations' inverse mass matrix IA^-1 + IB^-1, by columns.
+NPH_HD void joint_rotation_matrix(const NphJointState *j, NphVec3 k[3])
+{
+    for (u32 col = 0U; col < 3U; ++col) {
+        k[col] = v3_add(j->inv_inertia_a.c[col], j->inv_inertia_b.c[col]);
+    }
+}
+
+// The effective mass of a rotation about a unit axis (0 when immovable).
+NPH_HD f32 joint_axial_mass(const NphJointState *j, NphVec3 axis)
+{
+    const f32 inv = v3_dot(axis, m3_mul_v3(j->inv_inertia_a, axis)) +
+                    v3_dot(axis, m3_mul_v3(j->inv_inertia_b, axis));
+    return inv > 0.0F ? 1.0F / inv : 0.0F;
+}
+
+NPH_HD NphQuat joint_flip(NphQuat q) { return q4(-q.x, -q.y, -q.z, -q.w); }
+
+// The angle of B's frame about A's frame's X axis, in [-pi, pi].
+NPH_HD f32 joint_twist_angle(NphQuat qa, NphQuat qb)
+{
+    NphQuat rel = q_mul(q_conj(qa), qb);
+    if (rel.w < 0.0F) {
+        rel = joint_flip(rel);
+    }
+    return 2.0F * nph_atan2f(rel.x, rel.w);
+}
+
+// An angle difference brought back into [-pi, pi] (one turn at most).
+NPH_HD f32 joint_wrap(f32 a)
+{
+    const f32 pi = 3.14159265F;
+    return a > pi ? a - 2.0F * pi : (a < -pi ? a + 2.0F * pi : a);
+}
+
+// ---------------------------------------------------------------------------
+// The bodies as a pass sees them
+// ---------------------------------------------------------------------------
+typedef struct NphJointPose {
+    NphSolverBody *a;
+    NphSolverBody *b;
+    NphVec3 ra; // anchors from the centres of mass, rotated by the substep
+    NphVec3 rb;
+    NphQuat qa; // the frames, rotated by the substep
+    NphQuat qb;
+} NphJointPose;
+
+NPH_HD NphJointPose joint_pose(NphSolverBody *bodies, const NphJointState *j)
+{
+    NphJointPose p;
+    p.a = &bodies[j->solver_a];
+    p.b = &bodies[j->solver_b];
+    p.ra = q_rotate(sb_dq(p.a), j->anchor_a);
+    p.rb = q_rotate(sb_dq(p.b), j->anchor_b);
+    p.qa = q_mul(sb_dq(p.a), j->frame_q_a);
+    p.qb = q_mul(sb_dq(p.b), j->frame_q_b);
+    return p;
+}
+
+// An impulse p at the anchors: B gains it, A loses it (only moving bodies
+// are written).
+NPH_HD void joint_push(const NphJointState *j, NphJointPose *p, NphVec3 impulse)
+{
+    if ((j->write & NPH_JOINT_WRITE_A) != 0U) {
+        sb_set_v(p->a, v3_mul_add(sb_v(p->a), impulse, -j->inv_mass_a));
+        sb_set_w(p->a, v3_sub(sb_w(p->a), m3_mul_v3(j->inv_inertia_a, v3_cross(p->ra, impulse))));
+    }
+    if ((j->write & NPH_JOINT_WRITE_B) != 0U) {
+        sb_set_v(p->b, v3_mul_add(sb_v(p->b), impulse, j->inv_mass_b));
+        sb_set_w(p->b, v3_add(sb_w(p->b), m3_mul_v3(j->inv_inertia_b, v3_cross(p->rb, impulse))));
+    }
+}
+
+// An angular impulse: B gains it, A loses it.
+NPH_HD void joint_twist(const NphJointState *j, NphJointPose *p, NphVec3 impulse)
+{
+    if ((j->write & NPH_JOINT_WRITE_A) != 0U) {
+        sb_set_w(p->a, v3_sub(sb_w(p->a), m3_mul_v3(j->inv_inertia_a, impulse)));
+    }
+    if ((j->write & NPH_JOINT_WRITE_B) != 0U) {
+        sb_set_w(p->b, v3_add(sb_w(p->b), m3_mul_v3(j->inv_inertia_b, impulse)));
+    }
+}
+
+NPH_HD NphVec3 joint_spin(const NphJointPose *p) { return v3_sub(sb_w(p->b), sb_w(p->a)); }
+
+// ---------------------------------------------------------------------------
+// The joint's geometry at a pass (positions only)
+// ---------------------------------------------------------------------------
+typedef struct NphJointShape {
+    // Hinge: the axis (A's X), the angle about it and its effective mass.
+    NphVec3 hinge_axis;
+    f32 hinge_angle;
+    f32 hinge_mass;
+    // Ball: the swing row's axis and its gap to the cone (swing_on when the
+    // X axes are apart), the twist rows' axis and angle (twist_on when the
+    // X axes are not opposite).
+    int swing_on;
+    NphVec3 swing_axis;
+    f32 swing_gap;
+    int twist_on;
+    NphVec3 twist_axis;
+    f32 twist_angle;
+} NphJointShape;
+
+NPH_HD NphJointShape joint_shape(const NphJointConf *cf, const NphJointState *j,
+                                 const NphJointPose *p)
+{
+    NphJointShape g;
+    g.swing_on = 0;
+    g.twist_on = 0;
+    g.hinge_angle = g.hinge_mass = g.swing_gap = g.twist_angle = 0.0F;
+    g.hinge_axis = g.swing_axis = g.twist_axis = v3_zero();
+    if (cf->type == (u32)NPH_PHYSICS_JOINT_HINGE) {
+        g.hinge_axis = q_axis_x(p->qa);
+        g.hinge_angle = joint_twist_angle(p->qa, p->qb);
+        g.hinge_mass = joint_axial_mass(j, g.hinge_axis);
+    } else if (cf->type == (u32)NPH_PHYSICS_JOINT_BALL) {
+        const NphPhysicsBallJoint *ball = &cf->s.ball;
+        const NphVec3 xa = q_axis_x(p->qa);
+        const NphVec3 xb = q_axis_x(p->qb);
+        if (ball->enable_swing_limit != 0) {
+            // The cone is the ellipse of half-angles swing_y, swing_z about
+            // A's X; the swing turns about xa x xb.
+            const NphVec3 across = v3_cross(xa, xb);
+            const f32 s = v3_len(across);
+            if (s > 1e-6F) {
+                g.swing_on = 1;
+                g.swing_axis = v3_scale(across, 1.0F / s);
+                const NphVec3 in_a = q_rotate_inv(p->qa, g.swing_axis);
+                const f32 uy = in_a.y / ball->swing_y;
+                const f32 uz = in_a.z / ball->swing_z;
+                const f32 cone = 1.0F / nph_sqrtf(uy * uy + uz * uz);
+                g.swing_gap = cone - nph_atan2f(s, v3_dot(xa, xb));
+            }
+        }
+        if (ball->enable_twist_limit != 0) {
+            const NphVec3 mid = v3_add(xa, xb);
+            const f32 len = v3_len(mid);
+            if (len > 1e-3F) {
+                g.twist_on = 1;
+                g.twist_axis = v3_scale(mid, 1.0F / len);
+                g.twist_angle = joint_twist_angle(p->qa, p->qb);
+            }
+        }
+    }
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Rows
+// ---------------------------------------------------------------------------
+
+// A one-sided angular row: the gap closes at -(wB - wA).axis; its
+// accumulated impulse stays non-negative. Speculative while the gap is
+// open (it may close by the end of the substep), soft or rigid when shut.
+NPH_HD void joint_limit_row(const NphJointState *j, NphJointPose *p, NphVec3 axis, f32 gap,
+                            f32 *sum, const NphJointStep *step, int soft)
+{
+    NphJointSoft k = {0.0F, 1.0F, 0.0F};
+    if (gap > 0.0F) {
+        k.bias = gap * step->inv_h;
+    } else if (soft) {
+        k.bias = step->soft.bias * gap;
+        k.mass = step->soft.mass;
+        k.impulse = step->soft.impulse;
+    }
+    const f32 closing = -v3_dot(joint_spin(p), axis);
+    const f32 delta = -joint_axial_mass(j, axis) * k.mass * (closing + k.bias) - k.impulse * *sum;
+    const f32 now = nph_maxf(*sum + delta, 0.0F);
+    const f32 applied = now - *sum;
+    *sum = now;
+    joint_twist(j, p, v3_scale(axis, -applied));
+}
+
+// The hinge's spring towards its target angle (once a substep: solve only).
+NPH_HD void joint_spring_row(const NphPhysicsHingeJoint *hinge, NphJointState *j, NphJointPose *p,
+                             const NphJointShape *g)
+{
+    const f32 error = joint_wrap(g->hinge_angle - hinge->spring_target);
+    const f32 rate = v3_dot(joint_spin(p), g->hinge_axis);
+    const f32 delta = -j->spring_mass * g->hinge_mass * (rate + j->spring_bias * error) -
+                      j->spring_impulse_scale * j->spring_impulse;
+    j->spring_impulse += delta;
+    joint_twist(j, p, v3_scale(g->hinge_axis, delta));
+}
+
+// The hinge's motor: the relative speed about the axis driven towards the
+// motor speed, within the torque limit.
+NPH_HD void joint_motor_row(const NphPhysicsHingeJoint *hinge, NphJointState *j, NphJointPose *p,
+                            const NphJointShape *g, const NphJointStep *step)
+{
+    const f32 error = v3_dot(joint_spin(p), g->hinge_axis) - hinge->motor_speed;
+    const f32 cap = step->h * hinge->max_motor_torque;
+    const f32 before = j->motor_impulse;
+    const f32 asked = before - g->hinge_mass * error;
+    j->motor_impulse = nph_clampf(asked, -cap, cap);
+    joint_twist(j, p, v3_scale(g->hinge_axis, j->motor_impulse - before));
+}
+
+// Three-row blocks solve K x = error + bias at once; their impulse is
+// -mass_scale x - impulse_scale (accumulated).
+NPH_HD NphVec3 joint_block_impulse(const NphVec3 k[3], NphVec3 error, NphVec3 bias, NphVec3 sum,
+                                   f32 mass_scale, f32 impulse_scale)
+{
+    const NphVec3 x = joint_solve_sym3(k, v3_add(error, bias));
+    return v3_sub(v3_scale(x, -mass_scale), v3_scale(sum, impulse_scale));
+}
+
+// The anchors meet (every joint, last).
+NPH_HD void joint_anchor_rows(NphJointState *j, NphJointPose *p, const NphJointStep *step, int soft)
+{
+    const NphVec3 at_a = v3_add(sb_v(p->a), v3_cross(sb_w(p->a), p->ra));
+    const NphVec3 at_b = v3_add(sb_v(p->b), v3_cross(sb_w(p->b), p->rb));
+    NphVec3 bias = v3_zero();
+    f32 mass_scale = 1.0F, impulse_scale = 0.0F;
+    if (soft) {
+        const NphVec3 apart =
+            v3_add(v3_add(v3_sub(sb_dp(p->b), sb_dp(p->a)), v3_sub(p->rb, p->ra)), j->delta_center);
+        bias = v3_scale(apart, step->soft.bias);
         mass_scale = step->soft.mass;
         impulse_scale = step->soft.impulse;
     }
     NphVec3 k[3];
-    jnt_angular_mass(j, k);
-    const NphVec3 x = jnt_solve3(k, v3_add(cdot, bias));
-    const NphVec3 impulse =
-        v3_sub(v3_scale(x, -mass_scale), v3_scale(j->angular_impulse, impulse_scale));
-    j->angular_impulse = v3_add(j->angular_impulse, impulse);
-    jnt_apply_angular(j, s, impulse);
+    joint_anchor_matrix(j, p->ra, p->rb, k);
+    const NphVec3 delta = joint_block_impulse(k, v3_sub(at_b, at_a), bias, j->linear_impulse,
+                                              mass_scale, impulse_scale);
+    j->linear_impulse = v3_add(j->linear_impulse, delta);
+    joint_push(j, p, delta);
 }
 
-// The X axes align (hinge): two rows about A's Y and Z axes, their impulses
-// kept as scalars in angular_impulse.x and .y.
-NPH_HD void jnt_axis_lock(NphJointState *j, NphJointNow *s, const NphJointStep *step,
-                          int use_bias)
+// The frames align (fixed joints): the error is the rotation vector from
+// A's frame to B's (twice the vector part of qb qa*, the short way).
+NPH_HD void joint_orientation_rows(NphJointState *j, NphJointPose *p, const NphJointStep *step,
+                                   int soft)
 {
-    const NphVec3 p1 = q_axis_y(s->qa);
-    const NphVec3 p2 = q_axis_z(s->qa);
-    const NphVec3 dw = jnt_relative_angular(s);
-    f32 c1 = 0.0F;
-    f32 c2 = 0.0F;
-    f32 mass_scale = 1.0F;
-    f32 impulse_scale = 0.0F;
-    if (use_bias) {
-        const NphVec3 e = v3_cross(q_axis_x(s->qa), q_axis_x(s->qb));
-        c1 = step->soft.bias * v3_dot(e, p1);
-        c2 = step->soft.bias * v3_dot(e, p2);
+    NphVec3 bias = v3_zero();
+    f32 mass_scale = 1.0F, impulse_scale = 0.0F;
+    if (soft) {
+        NphQuat off = q_mul(p->qb, q_conj(p->qa));
+        if (off.w < 0.0F) {
+            off = joint_flip(off);
+        }
+        bias = v3_scale(q_xyz(off), 2.0F * step->soft.bias);
         mass_scale = step->soft.mass;
         impulse_scale = step->soft.impulse;
     }
-    const NphVec3 ip1 = v3_add(m3_mul_v3(j->inv_inertia_a, p1), m3_mul_v3(j->inv_inertia_b, p1));
-    const NphVec3 ip2 = v3_add(m3_mul_v3(j->inv_inertia_a, p2), m3_mul_v3(j->inv_inertia_b, p2));
-    const f32 k11 = v3_dot(p1, ip1);
-    const f32 k12 = v3_dot(p1, ip2);
-    const f32 k22 = v3_dot(p2, ip2);
-    const f32 det = k11 * k22 - k12 * k12;
+    NphVec3 k[3];
+    joint_rotation_matrix(j, k);
+    const NphVec3 delta = joint_block_impulse(k, joint_spin(p), bias, j->angular_impulse,
+                                              mass_scale, impulse_scale);
+    j->angular_impulse = v3_add(j->angular_impulse, delta);
+    joint_twist(j, p, delta);
+}
+
+// The hinge's two locked axes (A's Y and Z): a 2 x 2 block whose impulses
+// accumulate in angular_impulse.x and .y; the error is how far B's X has
+// turned from A's, xa x xb, on each.
+NPH_HD void joint_hinge_axes_rows(NphJointState *j, NphJointPose *p, const NphJointStep *step,
+                                  int soft)
+{
+    const NphVec3 u = q_axis_y(p->qa);
+    const NphVec3 v = q_axis_z(p->qa);
+    f32 bu = 0.0F, bv = 0.0F, mass_scale = 1.0F, impulse_scale = 0.0F;
+    if (soft) {
+        const NphVec3 tilt = v3_cross(q_axis_x(p->qa), q_axis_x(p->qb));
+        bu = step->soft.bias * v3_dot(tilt, u);
+        bv = step->soft.bias * v3_dot(tilt, v);
+        mass_scale = step->soft.mass;
+        impulse_scale = step->soft.impulse;
+    }
+    const NphVec3 iu = v3_add(m3_mul_v3(j->inv_inertia_a, u), m3_mul_v3(j->inv_inertia_b, u));
+    const NphVec3 iv = v3_add(m3_mul_v3(j->inv_inertia_a, v), m3_mul_v3(j->inv_inertia_b, v));
+    const f32 kuu = v3_dot(u, iu);
+    const f32 kuv = v3_dot(u, iv);
+    const f32 kvv = v3_dot(v, iv);
+    const f32 det = kuu * kvv - kuv * kuv;
     if (!(det > 0.0F)) {
         return;
     }
     const f32 inv = 1.0F / det;
-    const f32 b1 = v3_dot(dw, p1) + c1;
-    const f32 b2 = v3_dot(dw, p2) + c2;
-    const f32 x1 = (k22 * b1 - k12 * b2) * inv;
-    const f32 x2 = (k11 * b2 - k12 * b1) * inv;
-    const f32 i1 = -mass_scale * x1 - impulse_scale * j->angular_impulse.x;
-    const f32 i2 = -mass_scale * x2 - impulse_scale * j->angular_impulse.y;
-    j->angular_impulse.x += i1;
-    j->angular_impulse.y += i2;
-    jnt_apply_angular(j, s, v3_add(v3_scale(p1, i1), v3_scale(p2, i2)));
+    const NphVec3 spin = joint_spin(p);
+    const f32 eu = v3_dot(spin, u) + bu;
+    const f32 ev = v3_dot(spin, v) + bv;
+    const f32 du = -mass_scale * ((kvv * eu - kuv * ev) * inv) - impulse_scale * j->angular_impulse.x;
+    const f32 dv = -mass_scale * ((kuu * ev - kuv * eu) * inv) - impulse_scale * j->angular_impulse.y;
+    j->angular_impulse.x += du;
+    j->angular_impulse.y += dv;
+    joint_twist(j, p, v3_add(v3_scale(u, du), v3_scale(v, dv)));
 }
 
-NPH_HD void jnt_hinge(const NphPhysicsHingeJoint *h, NphJointState *j, NphJointNow *s,
-                      const NphJointStep *step, int use_bias)
+// The ball's friction: the relative spin stopped, within the friction
+// torque (the accumulated impulse clamped to a ball).
+NPH_HD void joint_friction_rows(NphJointState *j, NphJointPose *p, const NphJointStep *step,
+                                f32 torque)
 {
-    const NphVec3 axis = q_axis_x(s->qa);
-    const f32 angle = jnt_twist_angle(s->qa, s->qb);
-    const f32 mass = jnt_axial_mass(j, axis);
-    // The spring once a substep, in the solve.
-    if (h->enable_spring != 0 && use_bias) {
-        const f32 c = jnt_unwind(angle - h->spring_target);
-        const f32 cdot = v3_dot(jnt_relative_angular(s), axis);
-        const f32 impulse = -j->spring_mass * mass * (cdot + j->spring_bias * c) -
-                            j->spring_impulse_scale * j->spring_impulse;
-        j->spring_impulse += impulse;
-        jnt_apply_angular(j, s, v3_scale(axis, impulse));
+    NphVec3 k[3];
+    joint_rotation_matrix(j, k);
+    const NphVec3 stop = joint_solve_sym3(k, joint_spin(p));
+    const NphVec3 before = j->friction_impulse;
+    NphVec3 now = v3_sub(before, stop);
+    const f32 cap = step->h * torque;
+    const f32 size_sq = v3_len_sq(now);
+    if (size_sq > cap * cap) {
+        now = v3_scale(now, cap / nph_sqrtf(size_sq));
     }
-    if (h->enable_motor != 0) {
-        const f32 cdot = v3_dot(jnt_relative_angular(s), axis) - h->motor_speed;
-        const f32 max_impulse = step->h * h->max_motor_torque;
-        const f32 old = j->motor_impulse;
-        const f32 want = old - mass * cdot;
-        j->motor_impulse = want < -max_impulse ? -max_impulse : (want > max_impulse ? max_impulse : want);
-        jnt_apply_angular(j, s, v3_scale(axis, j->motor_impulse - old));
-    }
-    if (h->enable_limit != 0) {
-        jnt_angular_limit(j, s, v3_neg(axis), angle - h->lower, &j->lower_impulse, step, use_bias);
-        jnt_angular_limit(j, s, axis, h->upper - angle, &j->upper_impulse, step, use_bias);
-    }
-    jnt_axis_lock(j, s, step, use_bias);
+    j->friction_impulse = now;
+    joint_twist(j, p, v3_sub(now, before));
 }
 
-// The ball's cone: the X axes' angle within the ellipse of swing_y, swing_z.
-NPH_HD void jnt_swing(const NphPhysicsBallJoint *b, NphJointState *j, NphJointNow *s,
-                      const NphJointStep *step, int use_bias)
+// The rows of each type, in the order they are solved.
+enum {
+    NPH_ROW_END = 0,
+    NPH_ROW_SPRING,
+    NPH_ROW_MOTOR,
+    NPH_ROW_LOWER,
+    NPH_ROW_UPPER,
+    NPH_ROW_HINGE_AXES,
+    NPH_ROW_FRICTION,
+    NPH_ROW_SWING,
+    NPH_ROW_TWIST_LOWER,
+    NPH_ROW_TWIST_UPPER,
+    NPH_ROW_ORIENTATION,
+    NPH_ROW_ANCHORS,
+};
+
+NPH_HD u32 joint_row(u32 type, u32 k)
 {
-    const NphVec3 xa = q_axis_x(s->qa);
-    const NphVec3 xb = q_axis_x(s->qb);
-    const NphVec3 n = v3_cross(xa, xb);
-    const f32 sin_phi = v3_len(n);
-    if (!(sin_phi > 1e-6F)) {
+    if (type == (u32)NPH_PHYSICS_JOINT_HINGE) { // spring .. hinge axes, then the anchors
+        return k < 5U ? NPH_ROW_SPRING + k : (k == 5U ? NPH_ROW_ANCHORS : NPH_ROW_END);
+    }
+    if (type == (u32)NPH_PHYSICS_JOINT_BALL) {
+        return k < 4U ? NPH_ROW_FRICTION + k : (k == 4U ? NPH_ROW_ANCHORS : NPH_ROW_END);
+    }
+    return k == 0U ? NPH_ROW_ORIENTATION : (k == 1U ? NPH_ROW_ANCHORS : NPH_ROW_END);
+}
+
+NPH_HD void joint_run_row(u32 row, const NphJointConf *cf, NphJointState *j, NphJointPose *p,
+                          const NphJointShape *g, const NphJointStep *step, int soft)
+{
+    const NphPhysicsHingeJoint *hinge = &cf->s.hinge;
+    const NphPhysicsBallJoint *ball = &cf->s.ball;
+    switch (row) {
+    case NPH_ROW_SPRING:
+        if (hinge->enable_spring != 0 && soft) {
+            joint_spring_row(hinge, j, p, g);
+        }
         return;
-    }
-    const NphVec3 axis = v3_scale(n, 1.0F / sin_phi);
-    const NphVec3 local = q_rotate_inv(s->qa, axis);
-    const f32 ey = local.y / b->swing_y;
-    const f32 ez = local.z / b->swing_z;
-    const f32 limit = 1.0F / nph_sqrtf(ey * ey + ez * ez);
-    const f32 phi = nph_atan2f(sin_phi, v3_dot(xa, xb));
-    jnt_angular_limit(j, s, axis, limit - phi, &j->swing_impulse, step, use_bias);
-}
-
-NPH_HD void jnt_ball(const NphPhysicsBallJoint *b, NphJointState *j, NphJointNow *s,
-                     const NphJointStep *step, int use_bias)
-{
-    if (b->friction_torque > 0.0F) {
-        NphVec3 k[3];
-        jnt_angular_mass(j, k);
-        const NphVec3 x = jnt_solve3(k, jnt_relative_angular(s));
-        const NphVec3 old = j->friction_impulse;
-        NphVec3 total = v3_sub(old, x);
-        const f32 max_impulse = step->h * b->friction_torque;
-        const f32 len_sq = v3_len_sq(total);
-        if (len_sq > max_impulse * max_impulse) {
-            total = v3_scale(total, max_impulse / nph_sqrtf(len_sq));
+    case NPH_ROW_MOTOR:
+        if (hinge->enable_motor != 0) {
+            joint_motor_row(hinge, j, p, g, step);
         }
-        j->friction_impulse = total;
-        jnt_apply_angular(j, s, v3_sub(total, old));
-    }
-    if (b->enable_swing_limit != 0) {
-        jnt_swing(b, j, s, step, use_bias);
-    }
-    if (b->enable_twist_limit != 0) {
-        const NphVec3 sum = v3_add(q_axis_x(s->qa), q_axis_x(s->qb));
-        const f32 len = v3_len(sum);
-        if (len > 1e-3F) {
-            const NphVec3 axis = v3_scale(sum, 1.0F / len);
-            const f32 twist = jnt_twist_angle(s->qa, s->qb);
-            jnt_angular_limit(j, s, v3_neg(axis), twist - b->twist_lower, &j->lower_impulse, step,
-                              use_bias);
-            jnt_angular_limit(j, s, axis, b->twist_upper - twist, &j->upper_impulse, step,
-                              use_bias);
+        return;
+    case NPH_ROW_LOWER:
+        if (hinge->enable_limit != 0) {
+            joint_limit_row(j, p, v3_neg(g->hinge_axis), g->hinge_angle - hinge->lower,
+                            &j->lower_impulse, step, soft);
         }
+        return;
+    case NPH_ROW_UPPER:
+        if (hinge->enable_limit != 0) {
+            joint_limit_row(j, p, g->hinge_axis, hinge->upper - g->hinge_angle, &j->upper_impulse,
+                            step, soft);
+        }
+        return;
+    case NPH_ROW_HINGE_AXES:
+        joint_hinge_axes_rows(j, p, step, soft);
+        return;
+    case NPH_ROW_FRICTION:
+        if (ball->friction_torque > 0.0F) {
+            joint_friction_rows(j, p, step, ball->friction_torque);
+        }
+        return;
+    case NPH_ROW_SWING:
+        if (g->swing_on) {
+            joint_limit_row(j, p, g->swing_axis, g->swing_gap, &j->swing_impulse, step, soft);
+        }
+        return;
+    case NPH_ROW_TWIST_LOWER:
+        if (g->twist_on) {
+            joint_limit_row(j, p, v3_neg(g->twist_axis), g->twist_angle - ball->twist_lower,
+                            &j->lower_impulse, step, soft);
+        }
+        return;
+    case NPH_ROW_TWIST_UPPER:
+        if (g->twist_on) {
+            joint_limit_row(j, p, g->twist_axis, ball->twist_upper - g->twist_angle,
+                            &j->upper_impulse, step, soft);
+        }
+        return;
+    case NPH_ROW_ORIENTATION:
+        joint_orientation_rows(j, p, step, soft);
+        return;
+    default:
+        joint_anchor_rows(j, p, step, soft);
+        return;
     }
 }
 
@@ -405,20 +530,20 @@ NPH_HD void jnt_ball(const NphPhysicsBallJoint *b, NphJointState *j, NphJointNow
 // ---------------------------------------------------------------------------
 
 // Before the passes: masses, anchors and frames at the step's start; what is
-// switched off starts from nothing.
+// switched off starts from nothing when switched on again.
 NPH_HD void joint_prepare(const NphJointConf *cf, NphJointState *j, const NphBodyRec *a,
                           const NphBodyRec *b, u32 solver_a, u32 solver_b,
                           const NphSolverParams *params, f32 h)
 {
-    const int dyn_a = a->type == (u32)NPH_PHYSICS_BODY_DYNAMIC;
-    const int dyn_b = b->type == (u32)NPH_PHYSICS_BODY_DYNAMIC;
+    const int moves_a = a->type == (u32)NPH_PHYSICS_BODY_DYNAMIC;
+    const int moves_b = b->type == (u32)NPH_PHYSICS_BODY_DYNAMIC;
     j->solver_a = solver_a;
     j->solver_b = solver_b;
-    j->write = (dyn_a ? NPH_JOINT_WRITE_A : 0U) | (dyn_b ? NPH_JOINT_WRITE_B : 0U);
-    j->inv_mass_a = dyn_a ? a->inv_mass : 0.0F;
-    j->inv_mass_b = dyn_b ? b->inv_mass : 0.0F;
-    j->inv_inertia_a = dyn_a ? params[solver_a].inv_inertia : m3_zero();
-    j->inv_inertia_b = dyn_b ? params[solver_b].inv_inertia : m3_zero();
+    j->write = (moves_a ? NPH_JOINT_WRITE_A : 0U) | (moves_b ? NPH_JOINT_WRITE_B : 0U);
+    j->inv_mass_a = moves_a ? a->inv_mass : 0.0F;
+    j->inv_mass_b = moves_b ? b->inv_mass : 0.0F;
+    j->inv_inertia_a = moves_a ? params[solver_a].inv_inertia : m3_zero();
+    j->inv_inertia_b = moves_b ? params[solver_b].inv_inertia : m3_zero();
     j->anchor_a = q_rotate(a->rotation, v3_sub(cf->frame_a.position, a->local_center));
     j->anchor_b = q_rotate(b->rotation, v3_sub(cf->frame_b.position, b->local_center));
     j->delta_center = dv3_relative(b->center, a->center);
@@ -428,81 +553,84 @@ NPH_HD void joint_prepare(const NphJointConf *cf, NphJointState *j, const NphBod
     j->spring_mass = 1.0F;
     j->spring_impulse_scale = 0.0F;
     if (cf->type == (u32)NPH_PHYSICS_JOINT_HINGE) {
-        const NphPhysicsHingeJoint *hj = &cf->s.hinge;
-        if (hj->enable_limit == 0) {
-            j->lower_impulse = 0.0F;
-            j->upper_impulse = 0.0F;
+        const NphPhysicsHingeJoint *hinge = &cf->s.hinge;
+        if (hinge->enable_limit == 0) {
+            j->lower_impulse = j->upper_impulse = 0.0F;
         }
-        if (hj->enable_motor == 0) {
+        if (hinge->enable_motor == 0) {
             j->motor_impulse = 0.0F;
         }
-        if (hj->enable_spring == 0) {
+        if (hinge->enable_spring == 0) {
             j->spring_impulse = 0.0F;
         }
-        make_soft(hj->enable_spring != 0 ? hj->spring_hertz : 0.0F, hj->spring_damping_ratio, h,
-                  &j->spring_bias, &j->spring_mass, &j->spring_impulse_scale);
+        make_soft(hinge->enable_spring != 0 ? hinge->spring_hertz : 0.0F,
+                  hinge->spring_damping_ratio, h, &j->spring_bias, &j->spring_mass,
+                  &j->spring_impulse_scale);
     } else if (cf->type == (u32)NPH_PHYSICS_JOINT_BALL) {
-        const NphPhysicsBallJoint *bj = &cf->s.ball;
-        if (bj->enable_swing_limit == 0) {
+        const NphPhysicsBallJoint *ball = &cf->s.ball;
+        if (ball->enable_swing_limit == 0) {
             j->swing_impulse = 0.0F;
         }
-        if (bj->enable_twist_limit == 0) {
-            j->lower_impulse = 0.0F;
-            j->upper_impulse = 0.0F;
+        if (ball->enable_twist_limit == 0) {
+            j->lower_impulse = j->upper_impulse = 0.0F;
         }
-        if (!(bj->friction_torque > 0.0F)) {
+        if (!(ball->friction_torque > 0.0F)) {
             j->friction_impulse = v3_zero();
         }
     }
 }
 
-// The accumulated angular impulse as a world vector (joint_solver.c:826-853).
+// The accumulated angular impulse as one world vector, for the warm start
+// and the reaction torque (joint_solver.c:826-853): the hinge's locked axes
+// and its axial rows; the ball's friction, swing and twist rows.
 NPH_HD NphVec3 joint_angular_impulse(const NphJointConf *cf, const NphJointState *j, NphQuat qa,
                                      NphQuat qb)
 {
-    NphVec3 angular = j->angular_impulse;
     if (cf->type == (u32)NPH_PHYSICS_JOINT_HINGE) {
-        angular = v3_add(v3_scale(q_axis_y(qa), j->angular_impulse.x),
-                         v3_scale(q_axis_z(qa), j->angular_impulse.y));
+        const NphVec3 locked = v3_add(v3_scale(q_axis_y(qa), j->angular_impulse.x),
+                                      v3_scale(q_axis_z(qa), j->angular_impulse.y));
         const f32 axial = j->lower_impulse - j->upper_impulse + j->motor_impulse + j->spring_impulse;
-        angular = v3_mul_add(angular, q_axis_x(qa), axial);
-    } else if (cf->type == (u32)NPH_PHYSICS_JOINT_BALL) {
-        angular = j->friction_impulse;
+        return v3_mul_add(locked, q_axis_x(qa), axial);
+    }
+    if (cf->type == (u32)NPH_PHYSICS_JOINT_BALL) {
+        NphVec3 total = j->friction_impulse;
         const NphVec3 xa = q_axis_x(qa);
         const NphVec3 xb = q_axis_x(qb);
-        const NphVec3 n = v3_cross(xa, xb);
-        const f32 sin_phi = v3_len(n);
-        if (j->swing_impulse != 0.0F && sin_phi > 1e-6F) {
-            angular = v3_mul_add(angular, n, -j->swing_impulse / sin_phi);
+        const NphVec3 across = v3_cross(xa, xb);
+        const f32 s = v3_len(across);
+        if (j->swing_impulse != 0.0F && s > 1e-6F) {
+            total = v3_mul_add(total, across, -j->swing_impulse / s);
         }
-        const NphVec3 sum = v3_add(xa, xb);
-        const f32 len = v3_len(sum);
+        const NphVec3 mid = v3_add(xa, xb);
+        const f32 len = v3_len(mid);
         if (len > 1e-3F) {
-            angular = v3_mul_add(angular, sum, (j->lower_impulse - j->upper_impulse) / len);
+            total = v3_mul_add(total, mid, (j->lower_impulse - j->upper_impulse) / len);
         }
+        return total;
     }
-    return angular;
+    return j->angular_impulse;
 }
 
 NPH_HD void joint_warm_start(NphSolverBody *bodies, const NphJointConf *cf, NphJointState *j)
 {
-    NphJointNow s = jnt_now(bodies, j);
-    jnt_apply_angular(j, &s, joint_angular_impulse(cf, j, s.qa, s.qb));
-    jnt_apply_linear(j, &s, j->linear_impulse);
+    NphJointPose p = joint_pose(bodies, j);
+    joint_twist(j, &p, joint_angular_impulse(cf, j, p.qa, p.qb));
+    joint_push(j, &p, j->linear_impulse);
 }
 
+// One pass of the joint's rows: soft (solve) or rigid (relax).
 NPH_HD void joint_solve(NphSolverBody *bodies, const NphJointStep *step, const NphJointConf *cf,
-                        NphJointState *j, int use_bias)
+                        NphJointState *j, int soft)
 {
-    NphJointNow s = jnt_now(bodies, j);
-    if (cf->type == (u32)NPH_PHYSICS_JOINT_BALL) {
-        jnt_ball(&cf->s.ball, j, &s, step, use_bias);
-    } else if (cf->type == (u32)NPH_PHYSICS_JOINT_HINGE) {
-        jnt_hinge(&cf->s.hinge, j, &s, step, use_bias);
-    } else {
-        jnt_rotation_lock(j, &s, step, use_bias);
+    NphJointPose p = joint_pose(bodies, j);
+    const NphJointShape g = joint_shape(cf, j, &p);
+    for (u32 k = 0U;; ++k) {
+        const u32 row = joint_row(cf->type, k);
+        if (row == NPH_ROW_END) {
+            break;
+        }
+        joint_run_row(row, cf, j, &p, &g, step, soft);
     }
-    jnt_point(j, &s, step, use_bias);
 }
 
 #endif // NPH_JOINT_H
