@@ -601,8 +601,8 @@ void nph_physics_body_set_gravity_scale(NphPhysicsWorld *world, NphPhysicsBody b
 void nph_physics_body_set_locks(NphPhysicsWorld *world, NphPhysicsBody body, u32 locks);
 void nph_physics_body_set_sleep(NphPhysicsWorld *world, NphPhysicsBody body, bool enable,
                                 f32 threshold);
-// Waking works as in the reference; putting a body to sleep is ignored (the
-// world runs with sleep off: its islands never sleep).
+// body.c:567-593: awake wakes the body's island; asleep puts its whole
+// island to sleep now (a kinematic body alone).
 void nph_physics_body_set_awake(NphPhysicsWorld *world, NphPhysicsBody body, bool awake);
 void nph_physics_body_set_user_data(NphPhysicsWorld *world, NphPhysicsBody body, u64 user_data);
 u32 nph_physics_body_shapes(const NphPhysicsWorld *world, NphPhysicsBody body, NphPhysicsShape *out,
@@ -1028,8 +1028,8 @@ NphResult nph_amd_trainer_warm_start(const NphAmdTrainer *t, f32 *params, NphAmd
 // the world API above, and nph_amd_batch_step steps all of them, each
 // stage of the step one launch over every world's items.  Each world's
 // results are bit for bit those it gets stepped alone.  The batch owns its
-// worlds: destroy the batch, not the worlds.  Continuous collision is not
-// batched (creation returns NPH_RESULT_UNSUPPORTED), nor is the contact log
+// worlds: destroy the batch, not the worlds.  Continuous collision and sleep
+// are not batched (creation returns NPH_RESULT_UNSUPPORTED), nor is the contact log
 // (the step returns NPH_RESULT_UNSUPPORTED while a world logs).  The learned
 // warm start is: one network for the batch (nph_amd_batch_set_warm_start,
 // or the same one set on each world), its rows for every world in one

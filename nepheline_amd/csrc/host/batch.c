@@ -460,6 +460,13 @@ NphResult nph_amd_batch_step(NphAmdBatch *b, f32 dt)
         const NphResult rg = group_finish(b, g); // every begun solve is ended
         r = r == NPH_RESULT_OK ? rg : r;
     }
+    if (r == NPH_RESULT_OK) {
+#pragma omp parallel for schedule(dynamic, 8)
+        for (u32 k = 0U; k < n; ++k) {
+            b->results[k] = world_step_end(b->worlds[k]); // host lists only
+        }
+        r = first_error(b);
+    }
     st->device_solve_ms = st->device_passes_ms = st->device_collide_ms = 0.0;
     st->wait_narrow_ms = st->wait_solve_ms = 0.0;
     for (u32 g = 0U; g < b->groups; ++g) {

@@ -42,27 +42,6 @@ static void wake_and_upload(NphPhysicsWorld *w, NphPhysicsBody body)
     (void)world_upload_body(w, slot);
 }
 
-// island.c:167-197: out of the awake list by swap, at rest.
-static void awake_list_remove_body(NphPhysicsWorld *w, u32 slot)
-{
-    HBody *b = hbody(w, slot);
-    if (b->awake_slot == NPH_NULL32) {
-        return;
-    }
-    const u32 at = b->awake_slot;
-    const u32 last = VB(w->awake_bodies, u32)[w->awake_bodies.len - 1U];
-    VB(w->awake_bodies, u32)[at] = last;
-    hbody(w, last)->awake_slot = at;
-    w->awake_bodies.len -= 1U;
-    b->awake_slot = NPH_NULL32;
-    b->flags &= ~(u32)HB_AWAKE;
-    NphBodyRec *r = world_rec(w, slot);
-    r->v = v3_zero();
-    r->w = v3_zero();
-    r->force = v3_zero();
-    r->torque = v3_zero();
-}
-
 // shape.c:148-166: the proxy out of its tree, and out of the move buffer
 // with the buffer's order kept.
 static void destroy_proxy(NphPhysicsWorld *w, u32 s)
@@ -143,7 +122,11 @@ void nph_physics_body_destroy(NphPhysicsWorld *w, NphPhysicsBody body)
     }
     b->shape_count = 0U;
     world_rec(w, slot)->shape_head = NPH_NULL32;
-    awake_list_remove_body(w, slot);
+    if (b->awake_slot != NPH_NULL32) {
+        (void)world_set_awake_list(w, slot, 0);
+    }
+    islands_remove_body(w, slot);
+    b = hbody(w, slot);
     b->handle = 0U;
     slots_release(w->alloc, &w->body_slots, slot);
     (void)world_upload_body(w, slot);
@@ -361,11 +344,24 @@ void nph_physics_body_set_sleep(NphPhysicsWorld *w, NphPhysicsBody body, bool en
 void nph_physics_body_set_awake(NphPhysicsWorld *w, NphPhysicsBody body, bool awake)
 {
     HBody *hb = world_body(w, body);
-    if (hb == NULL || hb->type == NPH_PHYSICS_BODY_STATIC || !awake) {
+    if (w == NULL || w->stepping || hb == NULL || hb->type == NPH_PHYSICS_BODY_STATIC ||
+        world_sync_bodies(w) != NPH_RESULT_OK) {
         return;
     }
-    if (world_sync_bodies(w) == NPH_RESULT_OK) {
+    const u32 slot = phys_slot(body.value);
+    if (awake) {
         wake_and_upload(w, body);
+        return;
+    }
+    if ((hb->flags & HB_AWAKE) == 0U) {
+        return;
+    }
+    // Asleep on request: the whole island (what touches it would keep it
+    // awake otherwise); a kinematic body alone.
+    if (hb->island != NPH_NULL32) {
+        (void)islands_sleep_now(w, islands_root(w, hb->island));
+    } else {
+        (void)world_set_awake_list(w, slot, 0);
     }
 }
 

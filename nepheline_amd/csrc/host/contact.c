@@ -388,6 +388,9 @@ static NphResult contact_create(NphPhysicsWorld *w, u32 shape_a, u32 shape_b)
         c->color = NPH_NULL32;
         c->color_slot = NPH_NULL32;
         c->awake_slot = NPH_NULL32;
+        c->island = NPH_NULL32;
+        c->island_prev = NPH_NULL32;
+        c->island_next = NPH_NULL32;
         c->parent = k == 0U ? NPH_NULL32 : index;
         c->child = k + 1U < records ? rec[k + 1U] : NPH_NULL32;
         const NphContactInit init = {rec[k], shape_a, shape_b, c->body_a, c->body_b, c->child};
@@ -437,6 +440,9 @@ static void contact_destroy(NphPhysicsWorld *w, u32 contact, int events)
     if (c->color != NPH_NULL32) {
         graph_remove(w, contact);
     }
+    if ((c->flags & HC_ISLAND) != 0U) {
+        islands_unlink_contact(w, contact);
+    }
     awake_list_remove(w, contact);
     list_remove(w, contact, 0U, c->body_a);
     list_remove(w, contact, 1U, c->body_b);
@@ -452,9 +458,33 @@ void contacts_destroy(NphPhysicsWorld *w, u32 contact, int events)
     contact_destroy(w, contact, events);
 }
 
+// contact.c:265-281: the body's contacts, then its joints.
+NphResult contacts_sync_body(NphPhysicsWorld *w, u32 body)
+{
+    for (u32 key = body_at(w, body)->contact_head; key != NPH_NULL32;) {
+        const u32 c = key >> 1U;
+        const u32 next = contact_at(w, c)->next[key & 1U];
+        NphResult r = contact_sync(w, c);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        key = next;
+    }
+    for (u32 key = body_at(w, body)->joint_head; key != NPH_NULL32;) {
+        const u32 j = key >> 1U;
+        const u32 next = VB(w->joints, HJoint)[j].next[key & 1U];
+        NphResult r = joints_sync(w, j);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
+        key = next;
+    }
+    return NPH_RESULT_OK;
+}
+
 HContact *contacts_at(NphPhysicsWorld *w, u32 contact) { return contact_at(w, contact); }
 
-// contact.c:591-655 (no islands: the world runs without sleep)
+// contact.c:591-655
 static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
 {
     HContact *c = contact_at(w, contact);
@@ -487,6 +517,22 @@ static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
                 return r;
             }
         }
+        // A sleeping dynamic body touched by an awake one wakes with its
+        // island; the two islands then link.
+        const u32 ends[2] = {c->body_a, c->body_b};
+        for (u32 k = 0U; k < 2U; ++k) {
+            const HBody *b = body_at(w, ends[k]);
+            if (b->type == NPH_PHYSICS_BODY_DYNAMIC && !body_awake(b)) {
+                NphResult r = world_wake_body(w, ends[k]);
+                if (r != NPH_RESULT_OK) {
+                    return r;
+                }
+            }
+        }
+        if (body_at(w, c->body_a)->type == NPH_PHYSICS_BODY_DYNAMIC &&
+            body_at(w, c->body_b)->type == NPH_PHYSICS_BODY_DYNAMIC) {
+            islands_link_contact(w, contact);
+        }
         return contact_sync(w, contact);
     }
     if ((change & NPH_CHANGE_STOPPED) != 0U) {
@@ -499,6 +545,9 @@ static NphResult apply_change(NphPhysicsWorld *w, u32 contact, u32 change)
             if (r != NPH_RESULT_OK) {
                 return r;
             }
+        }
+        if ((c->flags & HC_ISLAND) != 0U) {
+            islands_unlink_contact(w, contact);
         }
         return contact_sync(w, contact);
     }

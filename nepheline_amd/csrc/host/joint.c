@@ -251,7 +251,7 @@ static void graph_remove_joint(NphPhysicsWorld *w, u32 joint)
 }
 
 // joint.c:257-271
-static NphResult joint_sync(NphPhysicsWorld *w, u32 joint)
+NphResult joints_sync(NphPhysicsWorld *w, u32 joint)
 {
     HJoint *j = joint_at(w, joint);
     const HBody *a = jbody(w, j->body_a);
@@ -309,6 +309,9 @@ static void joint_destroy(NphPhysicsWorld *w, u32 joint, int wake)
     const u32 b = j->body_b;
     if (j->color != NPH_NULL32) {
         graph_remove_joint(w, joint);
+    }
+    if ((j->flags & HJ_ISLAND) != 0U) {
+        islands_unlink_joint(w, joint);
     }
     list_remove(w, joint, 0U, a);
     list_remove(w, joint, 1U, b);
@@ -382,6 +385,9 @@ NphResult nph_physics_joint_create(NphPhysicsWorld *w, const NphPhysicsJointDesc
     j->flags = desc->collide_connected != 0 ? NPH_JOINT_COLLIDE_CONNECTED : 0U;
     j->color = NPH_NULL32;
     j->color_slot = NPH_NULL32;
+    j->island = NPH_NULL32;
+    j->island_prev = NPH_NULL32;
+    j->island_next = NPH_NULL32;
     j->frame_a = desc->frame_a;
     j->frame_b = desc->frame_b;
     j->frame_a.rotation = q_normalize(desc->frame_a.rotation);
@@ -408,12 +414,17 @@ NphResult nph_physics_joint_create(NphPhysicsWorld *w, const NphPhysicsJointDesc
             }
         }
     }
+    // Both ends awake, their islands linked (merged at the step's end).
     r = world_wake_body(w, slot_a);
     if (r == NPH_RESULT_OK) {
         r = world_wake_body(w, slot_b);
     }
+    if (r == NPH_RESULT_OK && jbody(w, slot_a)->type == NPH_PHYSICS_BODY_DYNAMIC &&
+        jbody(w, slot_b)->type == NPH_PHYSICS_BODY_DYNAMIC) {
+        islands_link_joint(w, slot);
+    }
     if (r == NPH_RESULT_OK) {
-        r = joint_sync(w, slot);
+        r = joints_sync(w, slot);
     }
     if (r == NPH_RESULT_OK) {
         r = upload_conf(w, slot);

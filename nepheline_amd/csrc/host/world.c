@@ -161,37 +161,53 @@ HShape *world_shape(const NphPhysicsWorld *w, NphPhysicsShape h)
 
 static NphBodyRec *rec_at(NphPhysicsWorld *w, u32 slot) { return &VB(w->body_recs, NphBodyRec)[slot]; }
 
+// solver.c:886-896: the step's move events, one per body of the awake list
+// as the step left it (its first `move_pending` entries), in its order, with
+// the poses and velocities the step wrote. Built from the host copy the
+// first time it is brought up to date after the step, so before any change
+// to a body or to the awake list.
+static NphResult build_move_events(NphPhysicsWorld *w)
+{
+    const u32 n = w->move_pending;
+    NphResult r = vbuf_reserve(w->alloc, &w->move_events, sizeof(NphPhysicsBodyMoveEvent),
+                               n > 0U ? n : 1U);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    for (u32 i = 0U; i < n; ++i) {
+        const u32 slot = VB(w->awake_bodies, u32)[i];
+        HBody *hb = &VB(w->bodies, HBody)[slot];
+        const NphBodyRec *b = rec_at(w, slot);
+        NphPhysicsBodyMoveEvent *m = &VB(w->move_events, NphPhysicsBodyMoveEvent)[i];
+        m->body.value = hb->handle;
+        m->fell_asleep = 0U;
+        m->user_data = hb->user_data;
+        m->position = b->origin;
+        m->rotation = b->rotation;
+        m->linear_velocity = b->v;
+        m->angular_velocity = b->w;
+        hb->move_event = i;
+    }
+    w->move_events.len = n;
+    w->move_events_ready = 1;
+    return NPH_RESULT_OK;
+}
+
 NphResult world_sync_bodies(NphPhysicsWorld *w)
 {
-    if (!w->bodies_stale) {
-        return NPH_RESULT_OK;
-    }
-    NphResult r = nph_dev_read_bodies(w->dev, VB(w->body_recs, NphBodyRec), w->bodies.len);
-    if (r == NPH_RESULT_OK) {
+    if (w->bodies_stale) {
+        NphResult r = nph_dev_read_bodies(w->dev, VB(w->body_recs, NphBodyRec), w->bodies.len);
+        if (r != NPH_RESULT_OK) {
+            return r;
+        }
         w->bodies_stale = 0;
     }
-    return r;
+    return w->move_events_ready ? NPH_RESULT_OK : build_move_events(w);
 }
 
 NphResult world_upload_body(NphPhysicsWorld *w, u32 slot)
 {
     return nph_dev_write_body(w->dev, slot, rec_at(w, slot));
-}
-
-NphResult world_wake_body(NphPhysicsWorld *w, u32 slot)
-{
-    HBody *b = &VB(w->bodies, HBody)[slot];
-    if (b->type == NPH_PHYSICS_BODY_STATIC || (b->flags & HB_AWAKE) != 0U) {
-        return NPH_RESULT_OK;
-    }
-    NphResult r = vbuf_push(w->alloc, &w->awake_bodies, 4U, &slot);
-    if (r != NPH_RESULT_OK) {
-        return r;
-    }
-    b->awake_slot = w->awake_bodies.len - 1U;
-    b->flags |= HB_AWAKE;
-    rec_at(w, slot)->sleep_time = 0.0F;
-    return NPH_RESULT_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -237,7 +253,9 @@ NphResult world_create(const NphPhysicsWorldDesc *desc, NphDeviceBatch *batch,
         !nph_isfinitef(desc->max_linear_speed) || !finite_nonneg(desc->time_to_sleep)) {
         return NPH_RESULT_INVALID_ARGUMENT;
     }
-    if (desc->solver != NPH_PHYSICS_SOLVER_SOFT_STEP || desc->enable_sleep != 0) {
+    // A batch's worlds step without sleep (their records are read back only
+    // when a step's events need them).
+    if (desc->solver != NPH_PHYSICS_SOLVER_SOFT_STEP || (batch != NULL && desc->enable_sleep != 0)) {
         return NPH_RESULT_UNSUPPORTED;
     }
     NphAllocator *a = desc->alloc;
@@ -250,6 +268,7 @@ NphResult world_create(const NphPhysicsWorldDesc *desc, NphDeviceBatch *batch,
     w->alloc = a;
     w->desc = *desc;
     w->hit_threshold = 1.0F;
+    w->move_events_ready = 1;
     NphResult r = batch != NULL ? nph_devb_world(batch, &w->dev) : nph_dev_create(&w->dev);
     for (u32 t = 0U; t < 3U && r == NPH_RESULT_OK; ++t) {
         r = tree_init(&w->trees[t], a, 16U);
@@ -312,6 +331,7 @@ void nph_physics_world_destroy(NphPhysicsWorld *w)
     vbuf_free(a, &w->end_events, sizeof(NphPhysicsContactEndEvent));
     vbuf_free(a, &w->move_events, sizeof(NphPhysicsBodyMoveEvent));
     vbuf_free(a, &w->contact_view, sizeof(NphContactRec));
+    islands_free(w);
     for (u32 t = 0U; t < 3U; ++t) {
         tree_free(&w->trees[t]);
     }
@@ -558,16 +578,26 @@ NphResult nph_physics_body_create(NphPhysicsWorld *w, const NphPhysicsBodyDesc *
     hb->contact_head = NPH_NULL32;
     hb->joint_head = NPH_NULL32;
     hb->awake_slot = NPH_NULL32;
+    hb->island = NPH_NULL32;
+    hb->island_prev = NPH_NULL32;
+    hb->island_next = NPH_NULL32;
+    hb->move_event = NPH_NULL32;
     hb->user_data = desc->user_data;
     hb->handle = phys_handle(slot, generation);
     update_mass(w, slot);
-    if (desc->type != NPH_PHYSICS_BODY_STATIC && desc->awake != 0) {
-        r = world_wake_body(w, slot);
+    r = world_upload_body(w, slot);
+    if (r == NPH_RESULT_OK && desc->type == NPH_PHYSICS_BODY_DYNAMIC) {
+        r = islands_add_body(w, slot);
     }
-    if (r == NPH_RESULT_OK) {
-        r = world_upload_body(w, slot);
+    if (r == NPH_RESULT_OK && desc->type != NPH_PHYSICS_BODY_STATIC) {
+        // A body made asleep sleeps alone: its own island, already still.
+        r = world_set_awake_list(w, slot, desc->awake != 0);
     }
     if (r != NPH_RESULT_OK) {
+        hb = &VB(w->bodies, HBody)[slot];
+        if (hb->island != NPH_NULL32) {
+            islands_remove_body(w, slot);
+        }
         hb->handle = 0U;
         slots_release(w->alloc, &w->body_slots, slot);
         return r;
@@ -1130,11 +1160,100 @@ NphResult world_solve_input(NphPhysicsWorld *w, f32 dt, NphSolveInput *out)
     return NPH_RESULT_OK;
 }
 
+static int joint_is_driven(const HJoint *j)
+{
+    // joint_solver.c:1410-1433, for the joint types built here.
+    return j->type == (u32)NPH_PHYSICS_JOINT_HINGE && j->s.hinge.enable_motor != 0 &&
+           j->s.hinge.motor_speed != 0.0F && j->s.hinge.max_motor_torque > 0.0F;
+}
+
+static NphResult wake_unless_awake(NphPhysicsWorld *w, u32 body)
+{
+    const HBody *b = &VB(w->bodies, HBody)[body];
+    return (b->flags & HB_AWAKE) == 0U ? world_wake_body(w, body) : NPH_RESULT_OK;
+}
+
+// solver.c:845-910, after the device's finalize, in the awake list's order:
+// a motor driving towards a speed keeps its bodies' sleep clocks at zero;
+// the move events cover the awake list as it stands; a kinematic body that
+// moved wakes the sleepers it touches or is jointed to. The host copy of
+// the records is fetched only when some joint is driven or some kinematic
+// body is awake.
+static NphResult post_finalize(NphPhysicsWorld *w)
+{
+    w->move_pending = w->awake_bodies.len;
+    w->move_events_ready = 0;
+    w->move_events.len = 0U;
+    int driven = 0;
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS && !driven; ++c) {
+        for (u32 i = 0U; i < w->color_joints[c].len && !driven; ++i) {
+            driven = joint_is_driven(&VB(w->joints, HJoint)[VB(w->color_joints[c], u32)[i]]);
+        }
+    }
+    int kinematic = 0;
+    for (u32 i = 0U; i < w->awake_bodies.len && !kinematic; ++i) {
+        kinematic = VB(w->bodies, HBody)[VB(w->awake_bodies, u32)[i]].type ==
+                    NPH_PHYSICS_BODY_KINEMATIC;
+    }
+    if (!driven && !kinematic) {
+        return NPH_RESULT_OK;
+    }
+    NphResult r = world_sync_bodies(w); // the move events too
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS && driven && r == NPH_RESULT_OK; ++c) {
+        for (u32 i = 0U; i < w->color_joints[c].len && r == NPH_RESULT_OK; ++i) {
+            const HJoint *j = &VB(w->joints, HJoint)[VB(w->color_joints[c], u32)[i]];
+            if (joint_is_driven(j)) {
+                rec_at(w, j->body_a)->sleep_time = 0.0F;
+                rec_at(w, j->body_b)->sleep_time = 0.0F;
+                r = world_upload_body(w, j->body_a);
+                if (r == NPH_RESULT_OK) {
+                    r = world_upload_body(w, j->body_b);
+                }
+            }
+        }
+    }
+    for (u32 i = 0U; i < w->awake_bodies.len && kinematic && r == NPH_RESULT_OK; ++i) {
+        const u32 slot = VB(w->awake_bodies, u32)[i];
+        const HBody *b = &VB(w->bodies, HBody)[slot];
+        if (b->type != NPH_PHYSICS_BODY_KINEMATIC || rec_at(w, slot)->sleep_time > 0.0F) {
+            continue;
+        }
+        for (u32 key = b->contact_head; key != NPH_NULL32 && r == NPH_RESULT_OK;) {
+            const HContact *c = &VB(w->contacts, HContact)[key >> 1U];
+            const u32 other = (key & 1U) == 0U ? c->body_b : c->body_a;
+            key = c->next[key & 1U];
+            if ((c->flags & HC_TOUCHING) != 0U) {
+                r = wake_unless_awake(w, other);
+            }
+        }
+        for (u32 key = b->joint_head; key != NPH_NULL32 && r == NPH_RESULT_OK;) {
+            const HJoint *j = &VB(w->joints, HJoint)[key >> 1U];
+            const u32 other = (key & 1U) == 0U ? j->body_b : j->body_a;
+            key = j->next[key & 1U];
+            r = wake_unless_awake(w, other);
+        }
+    }
+    return r;
+}
+
 NphResult world_solve_post(NphPhysicsWorld *w, const NphGrown *grown, u32 grown_count)
 {
     w->bodies_stale = 1;
     NphResult r = contacts_post_solve(w, grown, grown_count);
+    if (r == NPH_RESULT_OK) {
+        r = post_finalize(w);
+    }
     return r == NPH_RESULT_OK ? hit_events(w) : r;
+}
+
+NphResult world_step_end(NphPhysicsWorld *w)
+{
+    // Islands merge whether or not they may sleep (world.c:351-359).
+    NphResult r = islands_merge_pending(w);
+    if (r == NPH_RESULT_OK && w->desc.enable_sleep != 0) {
+        r = islands_sleep_and_split(w);
+    }
+    return r;
 }
 
 static NphResult solve(NphPhysicsWorld *w, f32 dt)
@@ -1175,7 +1294,8 @@ void world_step_begin(NphPhysicsWorld *w, f32 dt)
     w->hit_events.len = 0U;
     w->end_events.len = 0U;
     w->move_events.len = 0U;
-    w->move_events_ready = 0;
+    w->move_events_ready = 1; // none until the solve's end
+    w->move_pending = 0U;
 }
 
 f64 world_now_ms(void) { return now_ms(); }
@@ -1210,6 +1330,11 @@ NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
         r = solve(w, dt);
         w->stats.solve_ms = now_ms() - t;
     }
+    if (r == NPH_RESULT_OK) {
+        t = now_ms();
+        r = world_step_end(w);
+        w->stats.sleep_ms = now_ms() - t;
+    }
     w->stepping = 0;
     w->stats.steps += 1U;
     w->stats.step_ms = now_ms() - t0;
@@ -1224,25 +1349,8 @@ NphPhysicsEvents nph_physics_world_events(const NphPhysicsWorld *cw)
     NphPhysicsWorld *w = (NphPhysicsWorld *)cw;
     NphPhysicsEvents e;
     memset(&e, 0, sizeof e);
-    if (!w->move_events_ready && w->stats.steps > 0U && world_sync_bodies(w) == NPH_RESULT_OK &&
-        vbuf_reserve(w->alloc, &w->move_events, sizeof(NphPhysicsBodyMoveEvent),
-                     w->awake_bodies.len) == NPH_RESULT_OK) {
-        // solver.c:886-896: one per awake body, in the awake list's order.
-        for (u32 i = 0U; i < w->awake_bodies.len; ++i) {
-            const u32 slot = VB(w->awake_bodies, u32)[i];
-            const HBody *hb = &VB(w->bodies, HBody)[slot];
-            const NphBodyRec *b = rec_at(w, slot);
-            NphPhysicsBodyMoveEvent *m = &VB(w->move_events, NphPhysicsBodyMoveEvent)[i];
-            m->body.value = hb->handle;
-            m->fell_asleep = 0U;
-            m->user_data = hb->user_data;
-            m->position = b->origin;
-            m->rotation = b->rotation;
-            m->linear_velocity = b->v;
-            m->angular_velocity = b->w;
-        }
-        w->move_events.len = w->awake_bodies.len;
-        w->move_events_ready = 1;
+    if (!w->move_events_ready) {
+        (void)world_sync_bodies(w); // builds them
     }
     e.begin = VB(w->begin_events, NphPhysicsContactBeginEvent);
     e.begin_count = w->begin_events.len;
@@ -1287,6 +1395,8 @@ NphPhysicsStats nph_physics_world_stats(const NphPhysicsWorld *cw)
     s.touching_contacts = touching;
     s.manifold_points = points;
     s.joints = w->joint_count;
+    s.islands = w->islands.len - w->free_islands.len;
+    s.awake_islands = w->awake_islands.len;
     u32 colors = 0U;
     for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
         colors += w->color_contacts[c].len > 0U ? 1U : 0U;
@@ -1471,6 +1581,69 @@ bool nph_physics_world_validate(const NphPhysicsWorld *w)
     for (u32 i = 0U; i < w->awake_bodies.len; ++i) {
         const u32 s = VB(w->awake_bodies, u32)[i];
         if (VB(w->bodies, HBody)[s].awake_slot != i) {
+            return false;
+        }
+    }
+    // Islands: dynamic bodies only; linked contacts touch between two
+    // dynamic bodies, linked joints join two; every list agrees with its
+    // members and its count; merges are done by the step's end.
+    const HBody *bodies = VB(w->bodies, HBody);
+    for (u32 i = 0U; i < w->bodies.len; ++i) {
+        const HBody *b = &bodies[i];
+        if (b->handle != 0U && ((b->type == NPH_PHYSICS_BODY_DYNAMIC) != (b->island != NPH_NULL32) ||
+                                ((b->flags & HB_AWAKE) != 0U) != (b->awake_slot != NPH_NULL32))) {
+            return false;
+        }
+    }
+    for (u32 i = 0U; i < w->contacts.len; ++i) {
+        const HContact *c = &VB(w->contacts, HContact)[i];
+        if (c->shape_a == NPH_NULL32 || c->parent != NPH_NULL32) {
+            continue;
+        }
+        const int linked = (c->flags & HC_TOUCHING) != 0U &&
+                           bodies[c->body_a].type == NPH_PHYSICS_BODY_DYNAMIC &&
+                           bodies[c->body_b].type == NPH_PHYSICS_BODY_DYNAMIC;
+        if (linked != ((c->flags & HC_ISLAND) != 0U) || linked != (c->island != NPH_NULL32)) {
+            return false;
+        }
+    }
+    for (u32 i = 0U; i < w->joints.len; ++i) {
+        const HJoint *j = &VB(w->joints, HJoint)[i];
+        if (j->handle == 0U) {
+            continue;
+        }
+        const int linked = bodies[j->body_a].type == NPH_PHYSICS_BODY_DYNAMIC &&
+                           bodies[j->body_b].type == NPH_PHYSICS_BODY_DYNAMIC;
+        if (linked != ((j->flags & HJ_ISLAND) != 0U) || linked != (j->island != NPH_NULL32)) {
+            return false;
+        }
+    }
+    for (u32 i = 0U; i < w->islands.len; ++i) {
+        const HIsland *is = &VB(w->islands, HIsland)[i];
+        if (is->live == 0U) {
+            continue;
+        }
+        u32 n = 0U;
+        for (u32 b = is->body_head; b != NPH_NULL32; b = bodies[b].island_next, ++n) {
+            if (bodies[b].island != i) {
+                return false;
+            }
+        }
+        u32 m = 0U;
+        for (u32 c = is->contact_head; c != NPH_NULL32;
+             c = VB(w->contacts, HContact)[c].island_next, ++m) {
+            if (VB(w->contacts, HContact)[c].island != i) {
+                return false;
+            }
+        }
+        u32 nj = 0U;
+        for (u32 j = is->joint_head; j != NPH_NULL32; j = VB(w->joints, HJoint)[j].island_next, ++nj) {
+            if (VB(w->joints, HJoint)[j].island != i) {
+                return false;
+            }
+        }
+        if (n != is->body_count || n == 0U || m != is->contact_count || nj != is->joint_count ||
+            is->parent != NPH_NULL32 || (is->awake != 0U) != (is->awake_slot != NPH_NULL32)) {
             return false;
         }
     }

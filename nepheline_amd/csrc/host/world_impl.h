@@ -78,6 +78,7 @@ enum {
     HB_GYROSCOPIC = 1U << 2U,
     HB_BULLET = 1U << 5U,
     HB_MASS_SET = 1U << 6U, // mass properties set by the caller
+    HB_VISITED = 1U << 8U,  // an island split's mark
 };
 
 typedef struct HBody {
@@ -91,6 +92,10 @@ typedef struct HBody {
     u32 joint_head; // joint << 1 | side
     u32 joint_count;
     u32 awake_slot;
+    u32 island; // its island (dynamic bodies), else NPH_NULL32
+    u32 island_prev;
+    u32 island_next;
+    u32 move_event; // this step's move event, when it has one
     u64 user_data;
 } HBody;
 
@@ -125,6 +130,8 @@ enum {
     HC_TOUCHING = 1U << 0U,
     HC_EVENTS = 1U << 1U,
     HC_HIT_EVENTS = 1U << 2U,
+    HC_ISLAND = 1U << 3U,  // links its bodies' islands
+    HC_VISITED = 1U << 8U, // an island split's mark
 };
 
 typedef struct HContact {
@@ -140,6 +147,9 @@ typedef struct HContact {
     u32 awake_slot;
     u32 parent; // a mesh pair's child: its pair's record (else NPH_NULL32)
     u32 child;  // a mesh pair's next record (else NPH_NULL32)
+    u32 island; // the island it links, when HC_ISLAND
+    u32 island_prev;
+    u32 island_next;
 } HContact;
 
 // joint.c / world_internal.h:280-330, less the step's numbers (the device's).
@@ -153,11 +163,34 @@ typedef struct HJoint {
     u32 next[2];
     u32 color;
     u32 color_slot;
+    u32 island; // the island it links, when HJ_ISLAND
+    u32 island_prev;
+    u32 island_next;
     NphPhysicsPose frame_a;
     NphPhysicsPose frame_b;
     NphJointSettings s;
     u64 user_data;
 } HJoint;
+
+enum {
+    HJ_ISLAND = 1U << 4U,  // links its bodies' islands (HJoint.flags)
+    HJ_VISITED = 1U << 5U, // an island split's mark
+};
+
+// A set of bodies joined by touching contacts and joints, which sleeps and
+// wakes as one (island.c): its bodies', contacts' and joints' lists, the
+// union-find parent of a merge not yet carried out, its awake list slot.
+typedef struct HIsland {
+    u32 live;
+    u32 awake;
+    u32 body_head, body_tail, body_count;
+    u32 contact_head, contact_tail, contact_count;
+    u32 joint_head, joint_tail, joint_count;
+    u32 parent;
+    u32 awake_slot;
+    u32 removed; // constraints removed since the last split: it may have come apart
+    f32 sleep_time;
+} HIsland;
 
 typedef struct NphSlots {
     NphVecBuf generation; // u32
@@ -216,11 +249,19 @@ struct NphPhysicsWorld {
     NphVecBuf changed; // u32
     NphVecBuf codes;   // u32
 
+    NphVecBuf islands;        // HIsland
+    NphVecBuf free_islands;   // u32
+    NphVecBuf awake_islands;  // u32
+    NphVecBuf island_work;    // u32: merge pairs, split seeds
+    NphVecBuf island_stack;   // u32: a split's depth-first stack
+    NphVecBuf sleep_list;     // u32: islands that fall asleep this step
+
     NphVecBuf begin_events; // NphPhysicsContactBeginEvent
     NphVecBuf hit_events;   // NphPhysicsContactHitEvent
     NphVecBuf end_events;
-    NphVecBuf move_events; // NphPhysicsBodyMoveEvent, built on request
+    NphVecBuf move_events; // NphPhysicsBodyMoveEvent, built at the first sync after the step
     int move_events_ready;
+    u32 move_pending; // the awake bodies the step's move events cover
     NphVecBuf contact_view; // NphContactRec, readback scratch
 
     NphPhysicsStats stats;
@@ -240,6 +281,26 @@ NphResult world_sync_bodies(NphPhysicsWorld *w); // device -> host copy when sta
 NphResult world_upload_body(NphPhysicsWorld *w, u32 slot);
 NphResult world_wake_body(NphPhysicsWorld *w, u32 slot);
 
+// island.c restated: islands, waking and sleep.
+NphResult islands_add_body(NphPhysicsWorld *w, u32 body);
+void islands_remove_body(NphPhysicsWorld *w, u32 body);
+void islands_link_contact(NphPhysicsWorld *w, u32 contact);
+void islands_unlink_contact(NphPhysicsWorld *w, u32 contact);
+void islands_link_joint(NphPhysicsWorld *w, u32 joint);
+void islands_unlink_joint(NphPhysicsWorld *w, u32 joint);
+u32 islands_root(NphPhysicsWorld *w, u32 island);
+NphResult islands_merge_pending(NphPhysicsWorld *w);
+NphResult islands_sleep_and_split(NphPhysicsWorld *w);
+NphResult islands_sleep_now(NphPhysicsWorld *w, u32 island);
+// A body into (awake) or out of the awake list, then its contacts' and
+// joints' awake and graph membership.
+NphResult world_set_awake_list(NphPhysicsWorld *w, u32 body, int awake);
+void islands_free(NphPhysicsWorld *w);
+// contact.c / joint.c: a body's contacts' and joints' membership brought in
+// line with their bodies.
+NphResult contacts_sync_body(NphPhysicsWorld *w, u32 body);
+NphResult joints_sync(NphPhysicsWorld *w, u32 joint);
+
 // contact.c restated
 // New pairs a moved proxy's counting pass keeps, so that the filling pass
 // queries again only the proxies with more.
@@ -249,6 +310,8 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w);
 NphResult world_create(const NphPhysicsWorldDesc *desc, NphDeviceBatch *batch,
                        NphPhysicsWorld **out);
 void world_step_begin(NphPhysicsWorld *w, f32 dt);
+// The step's end: island merges, then sleep and splits when sleep is on.
+NphResult world_step_end(NphPhysicsWorld *w);
 // The solver's input (the awake list, the graph's colour lists, the step's
 // numbers); in->awake_count = 0 when nothing is awake.
 NphResult world_solve_input(NphPhysicsWorld *w, f32 dt, NphSolveInput *in);

@@ -445,6 +445,7 @@ class Api:
                                     [C.c_void_p, _U32, C.c_bool, C.c_float])
         self.body_set_awake = _bind(L, "nph_physics_body_set_awake", None,
                                     [C.c_void_p, _U32, C.c_bool])
+        self.body_is_awake = _bind(L, "nph_physics_body_is_awake", C.c_bool, [C.c_void_p, _U32])
         self.body_set_user_data = _bind(L, "nph_physics_body_set_user_data", None,
                                         [C.c_void_p, _U32, C.c_uint64])
         self.body_user_data = _bind(L, "nph_physics_body_user_data", C.c_uint64,

@@ -139,8 +139,7 @@ def test_unsupported_settings_are_rejected_like_the_reference_rejects_bad_ones()
     d.enable_sleep = 0
     d.enable_continuous = 0
     assert a.world_create(C.byref(d), C.byref(w)) == P.UNSUPPORTED
-    d.solver = 0
-    d.enable_sleep = 1
+    d.solver = 2  # XPBD
     assert a.world_create(C.byref(d), C.byref(w)) == P.UNSUPPORTED
 
 
