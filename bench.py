@@ -269,7 +269,7 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
     L.batch_warm_stats(batch, reset=True)
     e0 = energy(batch.worlds)
     solved = 0
-    phases = np.zeros(11)
+    phases = np.zeros(16)
     t0 = time.perf_counter()
     for i in range(args.steps):
         batch.step(DT)
@@ -277,7 +277,8 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
         solved += st.solved_contacts
         phases += (st.step_ms, st.narrow_ms, st.pairs_ms, st.changes_ms, st.solve_ms, st.post_ms,
                    st.device_solve_ms, st.device_passes_ms, st.device_collide_ms,
-                   st.wait_narrow_ms, st.wait_solve_ms)
+                   st.wait_narrow_ms, st.wait_solve_ms, st.device_pairs_ms, st.pairs_trees_ms,
+                   st.pairs_search_ms, st.pairs_create_ms, st.rebuild_cpu_ms)
         if (i + 1) % 10 == 0:
             log(f"batch step {i + 1}/{args.steps}")
     wall = time.perf_counter() - t0
@@ -285,8 +286,9 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
     st = L.batch_warm_stats(batch, reset=True)
     L.batch_measure_residual(batch, False)
     n = st.contacts_plain + st.contacts_learned
-    names = ("step", "narrow", "host_pairs", "host_changes", "solve", "host_post", "dev_solve",
-             "dev_passes", "dev_collide", "wait_narrow", "wait_solve")
+    names = ("step", "narrow", "pairs", "host_changes", "solve", "host_post", "dev_solve",
+             "dev_passes", "dev_collide", "wait_narrow", "wait_solve", "dev_pairs",
+             "pairs_trees", "pairs_search", "pairs_create", "rebuild_cpu")
     out = {"value": solved * SUBSTEPS / wall, "ms_per_step": wall * 1e3 / args.steps,
            "contact_substeps": solved * SUBSTEPS, "setup_s": setup, "bodies": bodies,
            "tuning": {"how": how, "pass_block": tune.pass_block, "body_block": tune.body_block,

@@ -1052,7 +1052,7 @@ typedef struct NphAmdBatchStats {
     u64 awake_bodies;    // the last step's awake bodies, all worlds
     f64 step_ms;         // wall time of the last step
     f64 narrow_ms;       // both narrow-phase calls: staging, launches, waiting for codes
-    f64 pairs_ms;        // host: new pairs (worlds in parallel)
+    f64 pairs_ms;        // new pairs: trees and change lists (host), the search (device)
     f64 changes_ms;      // host: narrow-phase changes and solver input
     f64 solve_ms;        // the solve: lists up, device stages, grown proxies back
     f64 post_ms;         // host: grown proxies into the trees, events
@@ -1061,6 +1061,11 @@ typedef struct NphAmdBatchStats {
     f64 device_collide_ms;
     f64 wait_narrow_ms;  // host blocked on the device for the change codes
     f64 wait_solve_ms;   // and for the solve's grown proxies
+    f64 device_pairs_ms; // device time of the pair searches (events)
+    f64 pairs_trees_ms;  // of pairs_ms: the trees' rebuilds and the change lists (wall)
+    f64 pairs_search_ms; // the searches: upload, launches, pairs back (wall)
+    f64 pairs_create_ms; // the new contacts (wall)
+    f64 rebuild_cpu_ms;  // the trees' rebuilds, summed over the worlds' threads
 } NphAmdBatchStats;
 
 NphResult nph_amd_batch_create(const NphPhysicsWorldDesc *desc, u32 world_count,

@@ -98,7 +98,8 @@ void nph_dev_destroy(NphDevice *d)
                     d->points, d->grown, d->grown_blocks, d->joint_conf, d->joint_state, d->step_joints,
                     d->fast_list, d->fast_pack, d->gate, d->mlp_params, d->lx, d->lraw, d->ly,
                     d->lres, d->wstats, d->log_x, d->log_t, d->log_count, d->team_small,
-                    d->team_big, d->team_counts};
+                    d->team_big, d->team_counts, d->pf_nodes[0], d->pf_nodes[1],
+                    d->pf_nodes[2], d->pf_shapes, d->pf_stamp, d->pf_set};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
             (void)hipFree(bufs[i]);

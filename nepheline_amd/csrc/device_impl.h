@@ -151,6 +151,16 @@ struct NphDevice {
     u32 mesh_shapes;     // mesh shapes made: the mesh pairs' kernel runs
     u32 big_hull_shapes; // shapes of hulls beyond 32 vertices: the large-hull kernels run
     u64 learn_hash;      // FNV-1a of the network's parameters and settings
+    // Pair finding (pairs_hip.hip): the trees' copies, the shapes' filter
+    // data and moved stamps, the filter set.
+    NphTreeNode *pf_nodes[3];
+    u32 pf_node_cap[3];
+    NphPairShape *pf_shapes;
+    u32 *pf_stamp;
+    u32 pf_shape_cap;
+    u64 *pf_set;
+    u32 pf_set_cap;
+    u32 pf_search; // searches so far: the moved stamps' value
 };
 
 // Buffer growth: a new allocation of new_bytes, the old contents copied

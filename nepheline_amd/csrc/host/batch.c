@@ -59,6 +59,9 @@ struct NphAmdBatch {
     u32 begun[NPH_BATCH_GROUPS]; // the group's solve was queued this step
     u64 *order;  // worlds by descending awake count: (~count << 32) | world
     NphResult *results;
+    NphPairSearch *pairs; // one search a group, over the group's worlds that moved
+    NphPairInput *pin;    // by position
+    u32 *pin_pos;         // the search's worlds: their positions
     NphAmdBatchStats stats;
 };
 
@@ -92,9 +95,12 @@ void nph_amd_batch_destroy(NphAmdBatch *b)
             nph_physics_world_destroy(b->worlds[k]);
         }
     }
+    nph_pairs_destroy(b->pairs);
     for (u32 g = 0U; g < NPH_BATCH_GROUPS; ++g) {
         nph_devb_destroy(b->dev[g]);
     }
+    batch_free(a, b->pin, n * sizeof *b->pin);
+    batch_free(a, b->pin_pos, n * sizeof *b->pin_pos);
     batch_free(a, b->worlds, n * sizeof *b->worlds);
     batch_free(a, b->world_at, n * sizeof *b->world_at);
     batch_free(a, b->devs, n * sizeof *b->devs);
@@ -139,11 +145,13 @@ NphResult nph_amd_batch_create(const NphPhysicsWorldDesc *desc, u32 world_count,
     b->active = (u32 *)batch_alloc(a, n * sizeof *b->active);
     b->order = (u64 *)batch_alloc(a, n * sizeof *b->order);
     b->results = (NphResult *)batch_alloc(a, n * sizeof *b->results);
+    b->pin = (NphPairInput *)batch_alloc(a, n * sizeof *b->pin);
+    b->pin_pos = (u32 *)batch_alloc(a, n * sizeof *b->pin_pos);
     NphResult r = NPH_RESULT_OK;
     if (b->worlds == NULL || b->world_at == NULL || b->devs == NULL || b->parts == NULL ||
         b->inputs == NULL ||
         b->first == NULL || b->off_a == NULL || b->off_b == NULL || b->active == NULL ||
-        b->order == NULL || b->results == NULL) {
+        b->order == NULL || b->results == NULL || b->pin == NULL || b->pin_pos == NULL) {
         r = NPH_RESULT_OUT_OF_MEMORY;
     }
     b->streams = world_count < NPH_BATCH_GROUPS ? world_count : NPH_BATCH_GROUPS;
@@ -276,12 +284,55 @@ static NphResult group_to_solve(NphAmdBatch *b, u32 g, f32 dt)
 {
     const u32 s0 = b->start[g], m = b->start[g + 1U] - s0;
     f64 t = world_now_ms();
+    // New pairs: each world's trees and change lists (worlds in parallel),
+    // one device search over the worlds whose move buffers hold proxies,
+    // then each world's new contacts.
 #pragma omp parallel for schedule(dynamic, 1)
     for (u32 i = 0U; i < m; ++i) {
         const u32 pos = s0 + i;
-        b->results[pos] = contacts_update_pairs(b->worlds[b->world_at[pos]]);
+        NphPhysicsWorld *w = b->worlds[b->world_at[pos]];
+        b->results[pos] = w->move_buffer.len == 0U ? contacts_update_pairs(w)
+                                                   : pairs_prepare(w, &b->pin[pos]);
     }
     NphResult r = first_error(b);
+    f64 t1 = world_now_ms();
+    b->stats.pairs_trees_ms += t1 - t;
+    for (u32 i = s0; i < s0 + m; ++i) {
+        b->stats.rebuild_cpu_ms += b->worlds[b->world_at[i]]->rebuild_ms;
+    }
+    u32 searched = 0U;
+    for (u32 i = s0; i < s0 + m && r == NPH_RESULT_OK; ++i) {
+        if (b->worlds[b->world_at[i]]->move_buffer.len > 0U) {
+            b->pin[searched] = b->pin[i]; // packed to the front (searched <= i)
+            b->pin_pos[searched++] = i;
+        }
+    }
+    const u32 *pairs = NULL;
+    const u32 *starts = NULL;
+    if (r == NPH_RESULT_OK && searched > 0U) {
+        if (b->pairs == NULL) {
+            r = nph_pairs_create(&b->pairs);
+        }
+        if (r == NPH_RESULT_OK) {
+            r = nph_pairs_find(b->pairs, b->pin, searched, &pairs, &starts);
+        }
+        if (r == NPH_RESULT_OK) {
+            b->stats.device_pairs_ms += nph_pairs_last_ms(b->pairs);
+        }
+    }
+    const f64 t2 = world_now_ms();
+    b->stats.pairs_search_ms += t2 - t1;
+    if (r == NPH_RESULT_OK && searched > 0U) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (u32 k = 0U; k < searched; ++k) {
+            NphPhysicsWorld *w = b->worlds[b->world_at[b->pin_pos[k]]];
+            pairs_sent(w);
+            b->results[b->pin_pos[k]] =
+                pairs_finish(w, pairs + 2ULL * starts[k], starts[k + 1U] - starts[k]);
+        }
+        r = first_error(b);
+    }
+    b->stats.pairs_create_ms += world_now_ms() - t2;
     b->stats.pairs_ms += world_now_ms() - t;
     if (r != NPH_RESULT_OK) {
         return r;
@@ -398,6 +449,8 @@ NphResult nph_amd_batch_step(NphAmdBatch *b, f32 dt)
     st->solved_contacts = 0U;
     st->awake_bodies = 0U;
     st->narrow_ms = st->pairs_ms = st->changes_ms = st->solve_ms = st->post_ms = 0.0;
+    st->device_pairs_ms = st->pairs_trees_ms = st->pairs_search_ms = st->pairs_create_ms = 0.0;
+    st->rebuild_cpu_ms = 0.0;
     // The groups: worlds by descending awake bodies, each to the group with
     // the fewest so far; inside a group, positions keep that order (the host
     // phases take the largest worlds first).

@@ -562,6 +562,7 @@ void nph_physics_shape_set_filter(NphPhysicsWorld *w, NphPhysicsShape shape,
     const u32 slot = phys_slot(shape.value);
     s->filter = filter;
     tree_set_category_bits(&w->trees[s->tree], s->proxy, filter.category);
+    pairs_mark_shape(w, slot);
     const u32 body = s->body;
     for (u32 key = hbody(w, body)->contact_head; key != NPH_NULL32;) {
         const u32 c = key >> 1U;

@@ -14,11 +14,8 @@
 #define MAP_EMPTY 0ULL
 #define MAP_GONE (~0ULL)
 
-static u64 pair_key(u32 a, u32 b)
-{
-    // contact.c:12-15; +1 keeps key 0 free for empty buckets.
-    return (a < b ? ((u64)a << 32U) | b : ((u64)b << 32U) | a) + 1ULL;
-}
+// contact.c:12-15; +1 keeps key 0 free for empty buckets.
+static u64 pair_key(u32 a, u32 b) { return nph_pair_key(a, b); }
 
 static u64 mix64(u64 x)
 {
@@ -363,6 +360,9 @@ static NphResult contact_create(NphPhysicsWorld *w, u32 shape_a, u32 shape_b)
         r = map_insert(w->alloc, &w->pairs, pair_key(shape_a, shape_b), index);
     }
     if (r == NPH_RESULT_OK) {
+        pairs_touch_key(w, pair_key(shape_a, shape_b));
+    }
+    if (r == NPH_RESULT_OK) {
         r = vbuf_reserve(w->alloc, &w->awake_contacts, 4U, w->contacts.len);
     }
     if (r == NPH_RESULT_OK) {
@@ -447,6 +447,7 @@ static void contact_destroy(NphPhysicsWorld *w, u32 contact, int events)
     list_remove(w, contact, 0U, c->body_a);
     list_remove(w, contact, 1U, c->body_b);
     map_remove(&w->pairs, pair_key(c->shape_a, c->shape_b));
+    pairs_touch_key(w, pair_key(c->shape_a, c->shape_b));
     c->shape_a = NPH_NULL32;
     c->shape_b = NPH_NULL32;
     c->flags = 0U;
@@ -625,195 +626,272 @@ NphResult contacts_apply_codes(NphPhysicsWorld *w, const u32 *codes_a, u32 na, c
 }
 
 // ---------------------------------------------------------------------------
-// New pairs (contact.c:691-854)
+// New pairs (contact.c:691-854): the queries run on the device
+// (pairs_hip.hip); the host keeps the trees, tells the device what changed,
+// and makes the contacts of the pairs that come back, in their order.
 // ---------------------------------------------------------------------------
-static int filter_should_collide(NphPhysicsFilter a, NphPhysicsFilter b)
+void pairs_mark_shape(NphPhysicsWorld *w, u32 shape)
 {
-    if (a.group == b.group && a.group != 0) {
-        return a.group > 0;
+    HShape *s = shape_at(w, shape);
+    if ((s->flags & HS_PAIR_DIRTY) != 0U) {
+        return;
     }
-    return (a.category & b.mask) != 0U && (b.category & a.mask) != 0U;
+    if (vbuf_push(w->alloc, &w->pf_dirty_shapes, 4U, &shape) == NPH_RESULT_OK) {
+        s->flags |= HS_PAIR_DIRTY;
+    } else {
+        w->pf_lost = 1; // every shape goes again
+    }
 }
 
-typedef struct PairQuery {
-    NphPhysicsWorld *w;
-    u32 shape;
-    u32 key;
-    u32 *out;          // the pairs' place; NULL: counted, or appended to `append`
-    u32 *cache;        // counting: the first NPH_PAIR_CACHE pairs kept here
-    NphVecBuf *append; // pairs of u32, grown as they come (one pass)
-    NphResult err;
-    u32 count;
-} PairQuery;
-
-static int pair_callback(u32 proxy, u64 user_data, void *context)
+void pairs_mark_body(NphPhysicsWorld *w, u32 body)
 {
-    PairQuery *q = (PairQuery *)context;
-    NphPhysicsWorld *w = q->w;
-    const u32 other = (u32)user_data;
-    if (other == q->shape) {
-        return 1;
+    for (u32 s = body_at(w, body)->shape_head; s != NPH_NULL32; s = shape_at(w, s)->next) {
+        pairs_mark_shape(w, s);
     }
-    const HShape *s = shape_at(w, q->shape);
-    const HShape *o = shape_at(w, other);
-    const u32 other_key = (proxy << 2U) | o->tree;
-    if ((o->flags & HS_IN_MOVE_BUFFER) != 0U && other_key < q->key) {
-        return 1;
+}
+
+void pairs_touch_key(NphPhysicsWorld *w, u64 key)
+{
+    if (vbuf_push(w->alloc, &w->pf_touched, 8U, &key) != NPH_RESULT_OK) {
+        w->pf_lost = 1; // the set starts over
     }
-    if (s->body == o->body) {
-        return 1;
+}
+
+// contact.c:703-746's shape data, as the device's filters read it.
+static NphPairShape pair_shape_of(const NphPhysicsWorld *w, u32 slot)
+{
+    NphPairShape p;
+    memset(&p, 0, sizeof p);
+    const HShape *s = &VB(w->shapes, HShape)[slot];
+    if (s->handle == 0U) {
+        return p;
     }
-    if (((s->flags | o->flags) & (HS_SENSOR | HS_GHOST)) != 0U) {
-        return 1;
+    const HBody *b = &VB(w->bodies, HBody)[s->body];
+    p.category = s->filter.category;
+    p.mask = s->filter.mask;
+    p.group = s->filter.group;
+    p.body = s->body;
+    p.flags = ((s->flags & (HS_SENSOR | HS_GHOST)) != 0U ? NPH_PAIR_UNTOUCHABLE : 0U) |
+              (b->type == NPH_PHYSICS_BODY_DYNAMIC ? NPH_PAIR_DYNAMIC : 0U) |
+              (b->joint_count != 0U ? NPH_PAIR_JOINTED : 0U);
+    return p;
+}
+
+// Whether the set holds the key now: a shape pair with a contact, or a
+// body pair a joint keeps apart (joint.c:302-318).
+static int key_in_set(const NphPhysicsWorld *w, u64 key)
+{
+    if ((key & NPH_PAIR_BODY_KEY) == 0U) {
+        return map_find(&w->pairs, key) != NPH_NULL32;
     }
-    const HBody *bs = body_at(w, s->body);
-    const HBody *bo = body_at(w, o->body);
-    if (bs->type != NPH_PHYSICS_BODY_DYNAMIC && bo->type != NPH_PHYSICS_BODY_DYNAMIC) {
-        return 1;
+    const u64 k = (key & ~NPH_PAIR_BODY_KEY) - 1ULL;
+    const u32 a = (u32)(k >> 32U);
+    const u32 b = (u32)k;
+    if (a >= w->bodies.len || b >= w->bodies.len) {
+        return 0;
     }
-    if (!filter_should_collide(s->filter, o->filter)) {
-        return 1;
-    }
-    if (bs->joint_count != 0U && bo->joint_count != 0U &&
-        !joints_allow_contact(w, s->body, o->body)) {
-        return 1;
-    }
-    if (map_find(&w->pairs, pair_key(q->shape, other)) != NPH_NULL32) {
-        return 1;
-    }
-    if (q->out != NULL) {
-        q->out[2U * q->count] = q->shape;
-        q->out[2U * q->count + 1U] = other;
-    } else if (q->cache != NULL) {
-        if (q->count < NPH_PAIR_CACHE) {
-            q->cache[2U * q->count] = q->shape;
-            q->cache[2U * q->count + 1U] = other;
+    const HBody *ba = &VB(w->bodies, HBody)[a];
+    const HBody *bb = &VB(w->bodies, HBody)[b];
+    return ba->joint_count != 0U && bb->joint_count != 0U && !joints_allow_contact(w, a, b);
+}
+
+static int u64_order(const void *x, const void *y)
+{
+    const u64 a = *(const u64 *)x;
+    const u64 b = *(const u64 *)y;
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+// The keys the set holds now, for a set that starts over.
+static NphResult all_set_keys(NphPhysicsWorld *w)
+{
+    w->pf_keys.len = 0U;
+    for (u32 i = 0U; i < w->pairs.cap; ++i) {
+        const u64 k = w->pairs.keys[i];
+        if (k != MAP_EMPTY && k != MAP_GONE) {
+            NphResult r = vbuf_push(w->alloc, &w->pf_keys, 8U, &k);
+            if (r != NPH_RESULT_OK) {
+                return r;
+            }
         }
-    } else if (q->append != NULL) {
-        const u32 pair[2] = {q->shape, other};
-        q->err = vbuf_push(w->alloc, q->append, 8U, pair);
-        if (q->err != NPH_RESULT_OK) {
-            return 0;
+    }
+    const u32 shape_keys = w->pf_keys.len;
+    for (u32 j = 0U; j < w->joints.len; ++j) {
+        const HJoint *jt = &VB(w->joints, HJoint)[j];
+        if (jt->handle == 0U || (jt->flags & NPH_JOINT_COLLIDE_CONNECTED) != 0U) {
+            continue;
+        }
+        const u64 k = nph_pair_key(jt->body_a, jt->body_b) | NPH_PAIR_BODY_KEY;
+        NphResult r = vbuf_push(w->alloc, &w->pf_keys, 8U, &k);
+        if (r != NPH_RESULT_OK) {
+            return r;
         }
     }
-    q->count += 1U;
-    return 1;
+    // Two joints may keep one pair apart: each key once.
+    u64 *keys = VB(w->pf_keys, u64);
+    const u32 n = w->pf_keys.len - shape_keys;
+    if (n > 1U) {
+        qsort(keys + shape_keys, n, 8U, u64_order);
+        u32 m = 1U;
+        for (u32 i = 1U; i < n; ++i) {
+            if (keys[shape_keys + i] != keys[shape_keys + m - 1U]) {
+                keys[shape_keys + m++] = keys[shape_keys + i];
+            }
+        }
+        w->pf_keys.len = shape_keys + m;
+    }
+    return NPH_RESULT_OK;
 }
 
-// contact.c:748-771: one moved proxy's new pairs, into `out`, or appended
-// to `append`, or (both NULL) only counted.
-static u32 query_moved(NphPhysicsWorld *w, u32 key, u32 *out, u32 *cache, NphVecBuf *append,
-                       NphResult *err)
+static u32 pow2_at_least(u32 n)
 {
-    const u32 tree = key & 3U;
-    const u32 proxy = key >> 2U;
-    PairQuery q;
-    q.w = w;
-    q.shape = (u32)tree_user_data(&w->trees[tree], proxy);
-    q.key = key;
-    q.out = out;
-    q.cache = cache;
-    q.append = append;
-    q.err = NPH_RESULT_OK;
-    q.count = 0U;
-    const HShape *s = shape_at(w, q.shape);
-    const NphAABB box = s->fat_aabb;
-    const u64 mask = s->filter.group > 0 ? ~0ULL : s->filter.mask;
-    if (tree == NPH_TREE_DYNAMIC) {
-        tree_query(&w->trees[NPH_TREE_STATIC], box, mask, pair_callback, &q);
-        tree_query(&w->trees[NPH_TREE_KINEMATIC], box, mask, pair_callback, &q);
+    u32 c = 1024U;
+    while (c < n) {
+        c *= 2U;
     }
-    if (q.err == NPH_RESULT_OK) {
-        tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, pair_callback, &q);
-    }
-    if (err != NULL) {
-        *err = q.err;
-    }
-    return q.count;
+    return c;
 }
 
-NphResult contacts_update_pairs(NphPhysicsWorld *w)
+// The trees' rebuild (contact.c:811-818), and the stats of the move buffer.
+static NphResult pairs_begin(NphPhysicsWorld *w)
 {
+    const f64 t0 = world_now_ms();
     for (u32 t = NPH_TREE_KINEMATIC; t <= NPH_TREE_DYNAMIC; ++t) {
         NphResult r = tree_rebuild(&w->trees[t], 0);
         if (r != NPH_RESULT_OK) {
             return r;
         }
     }
-    const u32 n = w->move_buffer.len;
-    w->stats.moved_proxies = n;
+    w->rebuild_ms = world_now_ms() - t0;
+    w->stats.moved_proxies = w->move_buffer.len;
     w->stats.new_pairs = 0U;
-    if (n == 0U) {
-        return NPH_RESULT_OK;
+    return NPH_RESULT_OK;
+}
+
+NphResult pairs_prepare(NphPhysicsWorld *w, NphPairInput *in)
+{
+    NphResult r = pairs_begin(w);
+    if (r != NPH_RESULT_OK) {
+        return r;
     }
-    const u32 *moved = VB(w->move_buffer, u32);
-    u32 total = 0U;
-    if (n < 64U || omp_in_parallel()) {
-        // One thread (a small move buffer, or this world is one of a
-        // batch's, stepped on a thread of its own): one pass, appending.
-        w->pair_buffer.len = 0U;
-        for (u32 i = 0U; i < n; ++i) {
-            NphResult r = NPH_RESULT_OK;
-            (void)query_moved(w, moved[i], NULL, NULL, &w->pair_buffer, &r);
-            if (r != NPH_RESULT_OK) {
-                return r;
+    memset(in, 0, sizeof *in);
+    in->dev = w->dev;
+    for (u32 t = 0U; t < 3U; ++t) {
+        const NphTree *tr = &w->trees[t];
+        in->tree_nodes[t] = tr->nodes;
+        in->tree_dirty[t] = tr->dirty;
+        in->tree_dirty_count[t] = tr->dirty_count;
+        in->tree_root[t] = tr->root;
+        in->tree_capacity[t] = tr->node_capacity;
+        in->tree_height[t] = tree_height(tr);
+    }
+    // Shapes whose filter data changed (every live one after a lost mark).
+    if (w->pf_lost) {
+        for (u32 s = 0U; s < w->shapes.len; ++s) {
+            if (shape_at(w, s)->handle != 0U) {
+                pairs_mark_shape(w, s);
             }
         }
-        total = w->pair_buffer.len;
+    }
+    const u32 ns = w->pf_dirty_shapes.len;
+    if ((r = vbuf_reserve(w->alloc, &w->pf_shape_info, sizeof(NphPairShape), ns + 1U)) !=
+        NPH_RESULT_OK) {
+        return r;
+    }
+    for (u32 i = 0U; i < ns; ++i) {
+        const u32 slot = VB(w->pf_dirty_shapes, u32)[i];
+        VB(w->pf_shape_info, NphPairShape)[i] = pair_shape_of(w, slot);
+    }
+    in->shape_slots = VB(w->pf_dirty_shapes, u32);
+    in->shape_info = VB(w->pf_shape_info, NphPairShape);
+    in->shape_count = ns;
+    in->shape_capacity = w->shapes.len;
+    // The set's keys that changed, each once, and whether each is in; or,
+    // when the set would pass half full, every key it holds, in a new set.
+    u64 *touched = VB(w->pf_touched, u64);
+    u32 nt = w->pf_touched.len;
+    if (nt > 1U) {
+        qsort(touched, nt, 8U, u64_order);
+        u32 m = 1U;
+        for (u32 i = 1U; i < nt; ++i) {
+            if (touched[i] != touched[m - 1U]) {
+                touched[m++] = touched[i];
+            }
+        }
+        nt = m;
+    }
+    u32 adds = 0U;
+    for (u32 i = 0U; i < nt; ++i) {
+        adds += key_in_set(w, touched[i]) ? 1U : 0U;
+    }
+    if (w->pf_lost || w->pf_set_cap == 0U || 2ULL * ((u64)w->pf_set_used + adds) > w->pf_set_cap) {
+        if ((r = all_set_keys(w)) != NPH_RESULT_OK) {
+            return r;
+        }
+        w->pf_set_cap = pow2_at_least(4U * w->pf_keys.len);
+        w->pf_set_used = w->pf_keys.len;
+        if ((r = vbuf_reserve(w->alloc, &w->pf_key_in, 1U, w->pf_keys.len + 1U)) != NPH_RESULT_OK) {
+            return r;
+        }
+        memset(w->pf_key_in.data, 1, (size_t)w->pf_keys.len + 1U);
+        in->set_keys = VB(w->pf_keys, u64);
+        in->set_count = w->pf_keys.len;
+        in->set_reset = 1U;
     } else {
-        // Count, place by prefix sums, fill (contact.c:820-843): each moved
-        // proxy writes its own range, so the pairs come out in the move
-        // buffer's order and each proxy's in its trees' order, however many
-        // threads query.
-        NphResult r = vbuf_reserve(w->alloc, &w->pair_counts, 4U, n + 1U);
-        if (r == NPH_RESULT_OK) {
-            r = vbuf_reserve(w->alloc, &w->pair_cache, 8U * NPH_PAIR_CACHE, n);
-        }
-        if (r != NPH_RESULT_OK) {
+        if ((r = vbuf_reserve(w->alloc, &w->pf_key_in, 1U, nt + 1U)) != NPH_RESULT_OK) {
             return r;
         }
-        u32 *counts = VB(w->pair_counts, u32);
-        u32 *cache = VB(w->pair_cache, u32);
-#pragma omp parallel for schedule(dynamic, 16)
-        for (u32 i = 0U; i < n; ++i) {
-            counts[i] = query_moved(w, moved[i], NULL, cache + (u64)i * 2U * NPH_PAIR_CACHE,
-                                    NULL, NULL);
+        for (u32 i = 0U; i < nt; ++i) {
+            VB(w->pf_key_in, u8)[i] = key_in_set(w, touched[i]) ? 1U : 0U;
         }
-        for (u32 i = 0U; i < n; ++i) {
-            const u32 c = counts[i];
-            counts[i] = total;
-            total += c;
-        }
-        counts[n] = total;
-        r = vbuf_reserve(w->alloc, &w->pair_buffer, 8U, total > 0U ? total : 1U);
-        if (r != NPH_RESULT_OK) {
-            return r;
-        }
-        w->pair_buffer.len = total;
-        u32 *pairs = VB(w->pair_buffer, u32);
-        if (total > 0U) {
-#pragma omp parallel for schedule(dynamic, 64)
-            for (u32 i = 0U; i < n; ++i) {
-                const u32 c = counts[i + 1U] - counts[i];
-                if (c > NPH_PAIR_CACHE) {
-                    (void)query_moved(w, moved[i], &pairs[2U * counts[i]], NULL, NULL, NULL);
-                } else if (c > 0U) {
-                    memcpy(&pairs[2U * counts[i]], cache + (u64)i * 2U * NPH_PAIR_CACHE,
-                           (size_t)c * 8U);
-                }
-            }
-        }
+        w->pf_set_used += adds;
+        in->set_keys = touched;
+        in->set_count = nt;
+    }
+    in->set_in = VB(w->pf_key_in, u8);
+    in->set_capacity = w->pf_set_cap;
+    // The move buffer and its shapes.
+    const u32 n = w->move_buffer.len;
+    if ((r = vbuf_reserve(w->alloc, &w->pf_moved_shapes, 4U, n + 1U)) != NPH_RESULT_OK) {
+        return r;
     }
     for (u32 i = 0U; i < n; ++i) {
         const u32 key = VB(w->move_buffer, u32)[i];
-        const u32 s = (u32)tree_user_data(&w->trees[key & 3U], key >> 2U);
-        shape_at(w, s)->flags &= ~(u32)HS_IN_MOVE_BUFFER;
+        VB(w->pf_moved_shapes, u32)[i] = (u32)tree_user_data(&w->trees[key & 3U], key >> 2U);
+    }
+    in->moved_keys = VB(w->move_buffer, u32);
+    in->moved_shapes = VB(w->pf_moved_shapes, u32);
+    in->moved_count = n;
+    return NPH_RESULT_OK;
+}
+
+// The device holds what the input sent: the change lists start over.
+void pairs_sent(NphPhysicsWorld *w)
+{
+    for (u32 t = 0U; t < 3U; ++t) {
+        tree_dirty_clear(&w->trees[t]);
+    }
+    for (u32 i = 0U; i < w->pf_dirty_shapes.len; ++i) {
+        shape_at(w, VB(w->pf_dirty_shapes, u32)[i])->flags &= ~(u32)HS_PAIR_DIRTY;
+    }
+    w->pf_dirty_shapes.len = 0U;
+    w->pf_touched.len = 0U;
+    w->pf_lost = 0;
+}
+
+// contact.c:844-853: the move buffer emptied, then a contact for each new
+// pair in order.
+NphResult pairs_finish(NphPhysicsWorld *w, const u32 *pairs, u32 count)
+{
+    for (u32 i = 0U; i < w->move_buffer.len; ++i) {
+        shape_at(w, VB(w->pf_moved_shapes, u32)[i])->flags &= ~(u32)HS_IN_MOVE_BUFFER;
     }
     w->move_buffer.len = 0U;
-    for (u32 i = 0U; i < total; ++i) {
-        const u32 *p = &VB(w->pair_buffer, u32)[2U * i];
-        if (map_find(&w->pairs, pair_key(p[0], p[1])) == NPH_NULL32) {
-            NphResult r = contact_create(w, p[0], p[1]);
+    for (u32 i = 0U; i < count; ++i) {
+        const u32 a = pairs[2U * i];
+        const u32 b = pairs[2U * i + 1U];
+        if (map_find(&w->pairs, pair_key(a, b)) == NPH_NULL32) {
+            NphResult r = contact_create(w, a, b);
             if (r != NPH_RESULT_OK) {
                 return r;
             }
@@ -821,6 +899,29 @@ NphResult contacts_update_pairs(NphPhysicsWorld *w)
         }
     }
     return NPH_RESULT_OK;
+}
+
+NphResult contacts_update_pairs(NphPhysicsWorld *w)
+{
+    if (w->move_buffer.len == 0U) {
+        return pairs_begin(w);
+    }
+    NphPairInput in;
+    NphResult r = pairs_prepare(w, &in);
+    if (r == NPH_RESULT_OK && w->pair_search == NULL) {
+        r = nph_pairs_create(&w->pair_search);
+    }
+    const u32 *pairs = NULL;
+    const u32 *starts = NULL;
+    if (r == NPH_RESULT_OK) {
+        r = nph_pairs_find(w->pair_search, &in, 1U, &pairs, &starts);
+    }
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    pairs_sent(w);
+    w->pairs_device_ms = nph_pairs_last_ms(w->pair_search);
+    return pairs_finish(w, pairs + 2U * starts[0], starts[1] - starts[0]);
 }
 
 NphResult move_buffer_push(NphPhysicsWorld *w, u32 shape)

@@ -192,6 +192,7 @@ static void list_insert(NphPhysicsWorld *w, u32 joint, u32 side, u32 body)
     }
     b->joint_head = key;
     b->joint_count += 1U;
+    pairs_mark_body(w, body);
 }
 
 static void list_remove(NphPhysicsWorld *w, u32 joint, u32 side, u32 body)
@@ -209,6 +210,7 @@ static void list_remove(NphPhysicsWorld *w, u32 joint, u32 side, u32 body)
         joint_at(w, next >> 1U)->prev[next & 1U] = prev;
     }
     b->joint_count -= 1U;
+    pairs_mark_body(w, body);
 }
 
 static int body_awake(const HBody *b) { return (b->flags & HB_AWAKE) != 0U; }
@@ -315,6 +317,7 @@ static void joint_destroy(NphPhysicsWorld *w, u32 joint, int wake)
     }
     list_remove(w, joint, 0U, a);
     list_remove(w, joint, 1U, b);
+    pairs_touch_key(w, nph_pair_key(a, b) | NPH_PAIR_BODY_KEY);
     if ((j->flags & NPH_JOINT_COLLIDE_CONNECTED) == 0U) {
         // The shapes may touch now: A's look for pairs next step.
         for (u32 s = jbody(w, a)->shape_head; s != NPH_NULL32;
@@ -402,6 +405,7 @@ NphResult nph_physics_joint_create(NphPhysicsWorld *w, const NphPhysicsJointDesc
     w->joint_count += 1U;
     list_insert(w, slot, 0U, slot_a);
     list_insert(w, slot, 1U, slot_b);
+    pairs_touch_key(w, nph_pair_key(slot_a, slot_b) | NPH_PAIR_BODY_KEY);
     // Joined bodies keep apart unless asked: their contacts go now.
     if (desc->collide_connected == 0) {
         for (u32 key = jbody(w, slot_a)->contact_head; key != NPH_NULL32;) {

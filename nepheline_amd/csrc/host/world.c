@@ -313,9 +313,13 @@ void nph_physics_world_destroy(NphPhysicsWorld *w)
     vbuf_free(a, &w->awake_contacts, 4U);
     vbuf_free(a, &w->inits, sizeof(NphContactInit));
     vbuf_free(a, &w->move_buffer, 4U);
-    vbuf_free(a, &w->pair_buffer, 8U);
-    vbuf_free(a, &w->pair_counts, 4U);
-    vbuf_free(a, &w->pair_cache, 8U * NPH_PAIR_CACHE);
+    nph_pairs_destroy(w->pair_search);
+    vbuf_free(a, &w->pf_dirty_shapes, 4U);
+    vbuf_free(a, &w->pf_touched, 8U);
+    vbuf_free(a, &w->pf_keys, 8U);
+    vbuf_free(a, &w->pf_key_in, 1U);
+    vbuf_free(a, &w->pf_shape_info, sizeof(NphPairShape));
+    vbuf_free(a, &w->pf_moved_shapes, 4U);
     for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
         vbuf_free(a, &w->color_bits[c], 8U);
     }
@@ -944,6 +948,7 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
     s->fat_aabb = aabb_grow(s->aabb, margin);
     r = tree_create_proxy(&w->trees[s->tree], s->fat_aabb, s->filter.category, slot, &s->proxy);
     if (r == NPH_RESULT_OK) {
+        pairs_mark_shape(w, slot);
         r = move_buffer_push(w, slot);
     }
     if (r == NPH_RESULT_OK) {

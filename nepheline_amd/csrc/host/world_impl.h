@@ -31,24 +31,19 @@ void vbuf_free(NphAllocator *a, NphVecBuf *v, u32 elem);
 // and in what order.
 // ---------------------------------------------------------------------------
 #define NPH_TREE_NULL NPH_NULL32
-typedef struct NphTreeNode {
-    NphAABB aabb;
-    u64 category_bits;
-    u32 child1; // a leaf's user data, low word
-    u32 child2; // a leaf's user data, high word
-    u32 parent; // the next free node in the free list
-    u16 height;
-    u16 flags;
-} NphTreeNode; // 48 B (aabb_tree.h:52-67)
 
 typedef struct NphTree {
     NphAllocator *alloc;
-    NphTreeNode *nodes;
+    NphTreeNode *nodes; // node_capacity
     u32 root;
     u32 node_count;
     u32 node_capacity;
-    u32 free_list;
     u32 proxy_count;
+    u32 *free_ids; // free node ids, the next one taken last
+    u32 free_count;
+    u32 *dirty;    // nodes whose box, categories, children or height changed
+    u32 dirty_count;
+    u8 *dirty_mark;
     u32 rebuild_capacity;
     u32 *leaf_indices;
     NphAABB *leaf_boxes;
@@ -68,6 +63,9 @@ u64 tree_user_data(const NphTree *t, u32 proxy);
 void tree_destroy_proxy(NphTree *t, u32 proxy);
 void tree_move_proxy(NphTree *t, u32 proxy, NphAABB aabb);
 void tree_set_category_bits(NphTree *t, u32 proxy, u64 category_bits);
+// The write list taken (the device's copy is up to date): cleared.
+void tree_dirty_clear(NphTree *t);
+u32 tree_height(const NphTree *t);
 
 // ---------------------------------------------------------------------------
 // Records
@@ -105,6 +103,7 @@ enum {
     HS_IN_MOVE_BUFFER = 1U << 3U,
     HS_SENSOR = 1U << 4U,
     HS_GHOST = 1U << 5U,
+    HS_PAIR_DIRTY = 1U << 6U, // its pair-filter data waits for the device
 };
 
 typedef struct HShape {
@@ -206,9 +205,6 @@ typedef struct NphPairMap {
     u32 used; // count + removed
 } NphPairMap;
 
-#define NPH_TREE_STATIC 0U
-#define NPH_TREE_KINEMATIC 1U
-#define NPH_TREE_DYNAMIC 2U
 
 struct NphPhysicsWorld {
     NphAllocator *alloc;
@@ -232,9 +228,6 @@ struct NphPhysicsWorld {
 
     NphTree trees[3];
     NphVecBuf move_buffer; // u32 proxy keys
-    NphVecBuf pair_counts; // u32 per moved proxy, then their prefix sums
-    NphVecBuf pair_cache;  // u32 pairs: the counting pass's first pairs of each moved proxy
-    NphVecBuf pair_buffer; // u32 pairs (a, b)
 
     NphVecBuf color_bits[NPH_GRAPH_COLORS]; // u64 words over body slots
     NphVecBuf color_contacts[NPH_GRAPH_COLORS + 1U]; // u32; [24] = overflow
@@ -263,6 +256,21 @@ struct NphPhysicsWorld {
     int move_events_ready;
     u32 move_pending; // the awake bodies the step's move events cover
     NphVecBuf contact_view; // NphContactRec, readback scratch
+
+    // Pair finding on the device: the search, what changed since the last
+    // one (shapes' filter data, filter-set keys), and the set's size.
+    NphPairSearch *pair_search;
+    NphVecBuf pf_dirty_shapes; // u32 shape slots
+    NphVecBuf pf_touched;      // u64 set keys
+    NphVecBuf pf_keys;         // u64: a set that starts over
+    NphVecBuf pf_key_in;       // u8
+    NphVecBuf pf_shape_info;   // NphPairShape
+    NphVecBuf pf_moved_shapes; // u32
+    u32 pf_set_cap;            // 0: none yet
+    u32 pf_set_used;           // slots taken (removed ones too), at most
+    int pf_lost;               // a change went unrecorded: everything goes again
+    f64 pairs_device_ms;
+    f64 rebuild_ms; // the last step's tree rebuilds
 
     NphPhysicsStats stats;
     f32 step_dt;
@@ -299,6 +307,13 @@ void islands_free(NphPhysicsWorld *w);
 // contact.c / joint.c: a body's contacts' and joints' membership brought in
 // line with their bodies.
 NphResult contacts_sync_body(NphPhysicsWorld *w, u32 body);
+// contact.c: the device pair search's change records and its host steps.
+void pairs_mark_shape(NphPhysicsWorld *w, u32 shape);
+void pairs_mark_body(NphPhysicsWorld *w, u32 body);
+void pairs_touch_key(NphPhysicsWorld *w, u64 key);
+NphResult pairs_prepare(NphPhysicsWorld *w, NphPairInput *in);
+void pairs_sent(NphPhysicsWorld *w);
+NphResult pairs_finish(NphPhysicsWorld *w, const u32 *pairs, u32 count);
 NphResult joints_sync(NphPhysicsWorld *w, u32 joint);
 
 // contact.c restated

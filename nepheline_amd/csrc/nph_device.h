@@ -52,6 +52,88 @@ typedef struct NphSolveInput {
     NphStepParams params;
 } NphSolveInput;
 
+// ---------------------------------------------------------------------------
+// Pair finding (contact.c:703-854): the host keeps the three AABB trees as
+// the reference builds them; the device holds a copy of each, brought up to
+// date with the nodes the host wrote since the last search, and runs every
+// moved proxy's query over them, count -> scan -> fill, so the new pairs
+// come out in the move buffer's order and each proxy's in its trees' order
+// (pairs_hip.hip).
+// ---------------------------------------------------------------------------
+// The trees, in the order a dynamic proxy's query visits them.
+#define NPH_TREE_STATIC 0U
+#define NPH_TREE_KINEMATIC 1U
+#define NPH_TREE_DYNAMIC 2U
+
+typedef struct NphTreeNode {
+    NphAABB aabb;
+    u64 category_bits;
+    u32 child1; // a leaf's user data, low word
+    u32 child2; // a leaf's user data, high word
+    u32 parent;
+    u16 height; // 0: a leaf
+    u16 flags;
+} NphTreeNode; // 48 B (aabb_tree.h:52-67)
+
+// What the pair filters read of a shape (contact.c:703-746, shape.c:21-27).
+typedef struct NphPairShape {
+    u64 category;
+    u64 mask;
+    i32 group;
+    u32 body;
+    u32 flags; // NPH_PAIR_*
+    u32 pad;
+} NphPairShape; // 32 B
+
+enum {
+    NPH_PAIR_UNTOUCHABLE = 1U, // a sensor or a ghost
+    NPH_PAIR_DYNAMIC = 2U,     // its body is dynamic
+    NPH_PAIR_JOINTED = 4U,     // its body has joints
+};
+
+// The filter set's keys: shape pairs that have a contact, and body pairs
+// that a joint keeps from colliding.
+#define NPH_PAIR_BODY_KEY (1ULL << 63U)
+NPH_HD u64 nph_pair_key(u32 a, u32 b)
+{
+    return (a < b ? ((u64)a << 32U) | b : ((u64)b << 32U) | a) + 1ULL; // contact.c:12-15
+}
+
+// One world's part of a search: what the host changed since the world's
+// last search, and its move buffer.
+typedef struct NphPairInput {
+    NphDevice *dev;
+    const NphTreeNode *tree_nodes[3];
+    const u32 *tree_dirty[3]; // nodes written since the last search
+    u32 tree_dirty_count[3];
+    u32 tree_root[3];
+    u32 tree_capacity[3];
+    u32 tree_height[3];
+    const u32 *shape_slots; // shapes whose filter data changed, and that data
+    const NphPairShape *shape_info;
+    u32 shape_count;
+    u32 shape_capacity; // shape slots the device must hold
+    const u64 *set_keys; // filter-set keys to bring up to date, and whether each is in
+    const u8 *set_in;
+    u32 set_count;
+    u32 set_capacity; // a power of two
+    u32 set_reset;    // the set starts over (empty, then set_keys)
+    const u32 *moved_keys; // proxy << 2 | tree, the move buffer's order
+    const u32 *moved_shapes;
+    u32 moved_count;
+} NphPairInput;
+
+typedef struct NphPairSearch NphPairSearch;
+NphResult nph_pairs_create(NphPairSearch **out);
+void nph_pairs_destroy(NphPairSearch *p);
+// The new pairs of n worlds in one search: world k's are pairs
+// [(*starts)[k], (*starts)[k + 1]) of *pairs, each (shape a, shape b), in
+// order; valid until the next call.
+NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const u32 **pairs,
+                         const u32 **starts);
+// Device milliseconds of the last search (its stream's events).
+f64 nph_pairs_last_ms(const NphPairSearch *p);
+
 NphResult nph_dev_create(NphDevice **out);
 void nph_dev_destroy(NphDevice *d);
 const char *nph_dev_name(void);

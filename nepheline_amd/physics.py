@@ -769,7 +769,9 @@ class BatchStats(C.Structure):
                 ("solve_ms", C.c_double), ("post_ms", C.c_double),
                 ("device_solve_ms", C.c_double), ("device_passes_ms", C.c_double),
                 ("device_collide_ms", C.c_double), ("wait_narrow_ms", C.c_double),
-                ("wait_solve_ms", C.c_double)]
+                ("wait_solve_ms", C.c_double), ("device_pairs_ms", C.c_double),
+                ("pairs_trees_ms", C.c_double), ("pairs_search_ms", C.c_double),
+                ("pairs_create_ms", C.c_double), ("rebuild_cpu_ms", C.c_double)]
 
 
 def _bind_batch(a: Api):
