@@ -193,17 +193,25 @@ def train_network(args, bank):
 # ---------------------------------------------------------------------------
 # The batch (headline)
 # ---------------------------------------------------------------------------
-def batch_rooflines(a, batch, plan, steps):
-    """The window after the timed region: the tuned blocks launched from one
-    stream (one group, so no two kernels of the batch overlap), with
-    dispatch-stamped HIP events around every colour-pass launch and every
-    convex narrow-phase launch."""
-    batch.set_tuning(batch.tuning().pass_block, batch.tuning().body_block, 0, 1)
-    a.batch_time_kernels(batch.ptr, True)
-    solved = 0
-    for _ in range(steps):
-        batch.step(DT)
-        solved += batch.stats().solved_contacts
+def measured_traffic(kernel, key):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), when
+    it was taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "%s_batch1024_traffic.json" % kernel)
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("kernel") != kernel or d.get("workload_key") != key:
+        return None
+    return d
+
+
+def batch_rooflines(a, batch, plan, solved, window, key=None):
+    """The colour pass's and the convex narrow phase's rooflines from the
+    dispatch-stamped HIP events around every launch of `window` (kernel
+    timing was on for its steps, which solved `solved` graph contacts)."""
     kt = (C.c_double * 2)()
     a.batch_kernel_times(batch.ptr, kt)
     ct = (C.c_double * 3)()
@@ -220,18 +228,22 @@ def batch_rooflines(a, batch, plan, steps):
     avg_s = kt[0] * 1e-3 / launches
     color = roofline("hbm", bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0, HBM_PEAK_GBS,
                      "GB/s", kernel="k_b_color_pass",
-                     window="%d steps after the timed region, one stream" % steps,
-                     launches=int(kt[1]), avg_launch_us=avg_s * 1e6, device_ms=kt[0],
+                     window=window, launches=int(kt[1]), avg_launch_us=avg_s * 1e6, device_ms=kt[0],
                      bytes_per_launch=bytes_per_launch, bytes_per_contact_substep=mean_bytes,
                      contact_substeps=solved * SUBSTEPS)
+    counters = measured_traffic("k_b_color_pass", key)
+    if counters is not None:
+        color["traffic"] = counters["hbm_bytes_per_launch"]
+        color["traffic_source"] = ("profiles/k_b_color_pass_batch1024_traffic.json (%s)"
+                                   % counters["window"])
+        color["traffic_over_algorithmic"] = counters["hbm_bytes_per_launch"] / bytes_per_launch
     claunches = max(int(ct[1]), 1)
     cavg_s = ct[0] * 1e-3 / claunches
     cbytes = ct[2] * COLLIDE_PAIR_BYTES / claunches
     collide = roofline("hbm", cbytes / cavg_s / 1e9 if cavg_s > 0 else 0.0, HBM_PEAK_GBS,
                        "GB/s", kernel="k_b_narrow_lanes + k_b_narrow_teams (the lane "
                        "pass's start to the team pass's end)",
-                       window="%d steps after the timed region, one stream" % steps,
-                       launches=int(ct[1]), avg_launch_us=cavg_s * 1e6, device_ms=ct[0],
+                       window=window, launches=int(ct[1]), avg_launch_us=cavg_s * 1e6, device_ms=ct[0],
                        pairs_per_launch=ct[2] / claunches, bytes_per_pair=COLLIDE_PAIR_BYTES,
                        bytes_per_launch=cbytes)
     return color, collide
@@ -267,6 +279,12 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
         how = "fixed (the warm-up is too short for the autotuner)"
     L.batch_measure_residual(batch, True)
     L.batch_warm_stats(batch, reset=True)
+    # The rooflines' kernel timing runs over the timed steps themselves
+    # (dispatch-stamped events on each launch's own stream), unless the
+    # substep loop is a captured graph: then over a window after them.
+    timed_kernels = rooflines and not tune.use_graph
+    if timed_kernels:
+        a.batch_time_kernels(batch.ptr, True)
     e0 = energy(batch.worlds)
     solved = 0
     phases = np.zeros(16)
@@ -301,7 +319,21 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
            "learned_fraction": st.contacts_learned / n if n else 0.0, "gated": st.gated,
            "energy": drift(e0, e1)}
     if rooflines:
-        out["rooflines"] = batch_rooflines(a, batch, plan, args.roofline_steps)
+        window = "the %d timed steps" % args.steps
+        solved_window = solved
+        if not timed_kernels:
+            batch.set_tuning(tune.pass_block, tune.body_block, 0, tune.groups)
+            a.batch_time_kernels(batch.ptr, True)
+            solved_window = 0
+            for _ in range(args.roofline_steps):
+                batch.step(DT)
+                solved_window += batch.stats().solved_contacts
+            window = "%d steps after the timed region (stream launches)" % args.roofline_steps
+        key = {"worlds": len(plan), "seed": args.batch_seed, "steps": args.steps,
+               "warmup": args.warmup}
+        out["rooflines"] = batch_rooflines(a, batch, plan, solved_window, window,
+                                           key if timed_kernels else None)
+        a.batch_time_kernels(batch.ptr, False)
     batch.close()
     return out
 
@@ -309,8 +341,9 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
 def batch_cpu(args, bank, plan, threads, stride):
     """The reference's step on every `stride`-th world of the batch, worlds
     side by side on `threads` host threads (each world's step on one), after
-    the same warm-up as the GPU's, over the GPU's timed steps or until
-    `cpu_seconds` of stepping."""
+    the same warm-up as the GPU's, over at least the GPU's timed steps and
+    at least `cpu_seconds` of stepping (the sample's steps are then the
+    GPU's and the ones after them)."""
     from concurrent.futures import ThreadPoolExecutor
     path = os.path.join(ROOT, "oracle", "_ref", "libnph_ref.so")
     ra = P.api(path, "ref_shim_allocator")
@@ -334,7 +367,8 @@ def batch_cpu(args, bank, plan, threads, stride):
             list(pool.map(one, order))
         solved, steps = 0, 0
         t0 = time.perf_counter()
-        while steps < args.steps and time.perf_counter() - t0 < args.cpu_seconds:
+        while (steps < args.steps or time.perf_counter() - t0 < args.cpu_seconds) and \
+                steps < 100 * max(args.steps, 1):
             solved += sum(pool.map(one, order))
             steps += 1
         cwall = time.perf_counter() - t0
@@ -548,7 +582,7 @@ def main():
         learned["mode"] = "points that carry no impulse start from the network; residual gate on"
         learned["training"] = training
     color, collide = plain.pop("rooflines")
-    dominant = collide if collide["device_ms"] >= color["device_ms"] else color
+    dominant = color  # the batch's largest kernel by device time
     cpu_line = None
     if cpu is not None:
         log(f"CPU baseline: {cpu['threads']} threads ({cpu['cpu_model']})")
