@@ -360,6 +360,31 @@ void nph_physics_collide(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
 // collide.h:121: `count` candidates reduced to at most four, in place.
 u32 nph_physics_manifold_reduce(NphPhysicsManifoldPoint *points, u32 count, NphVec3 normal);
 
+// ---------------------------------------------------------------------------
+// Casts and overlaps against one geometry (collide.h:240-318, cast.c), host
+// code.  Meshes: rays and overlaps; a shape cast against a mesh is not
+// built here (it returns false).
+// ---------------------------------------------------------------------------
+#define NPH_PHYSICS_NO_TRIANGLE 0xFFFFFFFFU
+
+typedef struct NphPhysicsCastHit {
+    NphVec3 point;  // on the geometry's surface
+    NphVec3 normal; // unit: the surface's there, out of it (against the motion)
+    f32 fraction;   // of the motion, in [0, 1]
+    u32 triangle;   // a mesh's index in its input, or NPH_PHYSICS_NO_TRIANGLE
+} NphPhysicsCastHit;
+
+bool nph_physics_raycast(const NphPhysicsGeometry *g, NphPhysicsPose pose, NphVec3 origin,
+                         NphVec3 delta, NphPhysicsCastHit *out);                 // cast.c:241-257
+bool nph_physics_shape_cast(const NphPhysicsGeometry *a, NphPhysicsPose pose_a, NphVec3 translation,
+                            const NphPhysicsGeometry *b, NphPhysicsPose pose_b,
+                            NphPhysicsCastHit *out);                             // cast.c:525-530
+bool nph_physics_shape_cast_margin(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                                   NphVec3 translation, f32 margin, const NphPhysicsGeometry *b,
+                                   NphPhysicsPose pose_b, NphPhysicsCastHit *out); // :532-564
+bool nph_physics_overlap(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                         const NphPhysicsGeometry *b, NphPhysicsPose pose_b);    // :570-590
+
 // No counterpart in the reference: many pairs in one launch (pairs of
 // convex shapes only); caches, when given, are read and written per pair.
 typedef struct NphAmdCollideQuery {
@@ -683,6 +708,89 @@ void nph_physics_shape_set_filter(NphPhysicsWorld *world, NphPhysicsShape shape,
                                   NphPhysicsFilter filter);
 u64 nph_physics_shape_user_data(const NphPhysicsWorld *world, NphPhysicsShape shape);
 void nph_physics_shape_set_user_data(NphPhysicsWorld *world, NphPhysicsShape shape, u64 user_data);
+
+// ---------------------------------------------------------------------------
+// World queries (physics/query.h:51-172, query.c): the broadphase's trees
+// walked (static, kinematic, then dynamic shapes) and each shape found
+// tested exactly in a frame centred on the query's origin; hits in the
+// trees' order.  Host code over the host's trees; collide_shape's manifolds
+// come from the device's narrow phase, all of a query's pairs in one
+// launch.  Against mesh shapes: rays, box and shape overlaps; casts and
+// collide_shape pass them over.
+// ---------------------------------------------------------------------------
+enum {
+    NPH_PHYSICS_QUERY_STATIC = 1U << NPH_PHYSICS_BODY_STATIC,
+    NPH_PHYSICS_QUERY_KINEMATIC = 1U << NPH_PHYSICS_BODY_KINEMATIC,
+    NPH_PHYSICS_QUERY_DYNAMIC = 1U << NPH_PHYSICS_BODY_DYNAMIC,
+    NPH_PHYSICS_QUERY_ALL_BODIES = 7U,
+};
+
+typedef struct NphPhysicsQueryFilter {
+    u64 category;
+    u64 mask;
+    u32 body_types;        // NPH_PHYSICS_QUERY_*
+    NphPhysicsBody ignore; // 0: none
+    b32 sensors;
+} NphPhysicsQueryFilter; // query.h:61-67
+
+typedef struct NphPhysicsQueryHit {
+    NphPhysicsShape shape;
+    NphPhysicsBody body;
+    NphDVec3 point;
+    NphVec3 normal;
+    f32 fraction;
+    u32 triangle;
+} NphPhysicsQueryHit; // query.h:73-80
+
+typedef f32 (*NphPhysicsQueryCastFn)(void *user, const NphPhysicsQueryHit *hit);
+typedef bool (*NphPhysicsQueryShapeFn)(void *user, NphPhysicsShape shape, NphPhysicsBody body);
+
+typedef struct NphPhysicsQueryContact {
+    NphPhysicsShape shape;
+    NphPhysicsBody body;
+    NphPhysicsManifold manifold;
+} NphPhysicsQueryContact; // query.h:91-100
+typedef bool (*NphPhysicsQueryContactFn)(void *user, const NphPhysicsQueryContact *contact);
+
+NphPhysicsQueryFilter nph_physics_query_filter_default(void);                    // query.c:13-22
+bool nph_physics_world_raycast_closest(const NphPhysicsWorld *world, NphDVec3 origin,
+                                       NphVec3 translation, const NphPhysicsQueryFilter *filter,
+                                       NphPhysicsQueryHit *out);                // query.c:271-283
+void nph_physics_world_raycast(const NphPhysicsWorld *world, NphDVec3 origin, NphVec3 translation,
+                               const NphPhysicsQueryFilter *filter, NphPhysicsQueryCastFn fn,
+                               void *user);                                     // query.c:249-269
+bool nph_physics_world_cast_shape_closest(const NphPhysicsWorld *world,
+                                          const NphPhysicsGeometry *geometry, NphDVec3 position,
+                                          NphQuat rotation, NphVec3 translation,
+                                          const NphPhysicsQueryFilter *filter,
+                                          NphPhysicsQueryHit *out);
+void nph_physics_world_cast_shape(const NphPhysicsWorld *world, const NphPhysicsGeometry *geometry,
+                                  NphDVec3 position, NphQuat rotation, NphVec3 translation,
+                                  const NphPhysicsQueryFilter *filter, NphPhysicsQueryCastFn fn,
+                                  void *user);
+bool nph_physics_world_cast_shape_margin_closest(const NphPhysicsWorld *world,
+                                                 const NphPhysicsGeometry *geometry,
+                                                 NphDVec3 position, NphQuat rotation,
+                                                 NphVec3 translation, f32 margin,
+                                                 const NphPhysicsQueryFilter *filter,
+                                                 NphPhysicsQueryHit *out);
+void nph_physics_world_cast_shape_margin(const NphPhysicsWorld *world,
+                                         const NphPhysicsGeometry *geometry, NphDVec3 position,
+                                         NphQuat rotation, NphVec3 translation, f32 margin,
+                                         const NphPhysicsQueryFilter *filter,
+                                         NphPhysicsQueryCastFn fn, void *user); // query.c:294-329
+void nph_physics_world_overlap_shape(const NphPhysicsWorld *world,
+                                     const NphPhysicsGeometry *geometry, NphDVec3 position,
+                                     NphQuat rotation, const NphPhysicsQueryFilter *filter,
+                                     NphPhysicsQueryShapeFn fn, void *user);   // query.c:492-508
+void nph_physics_world_overlap_aabb(const NphPhysicsWorld *world, NphAABB box,
+                                    const NphPhysicsQueryFilter *filter, NphPhysicsQueryShapeFn fn,
+                                    void *user);                                // query.c:533-546
+void nph_physics_world_collide_shape(const NphPhysicsWorld *world,
+                                     const NphPhysicsGeometry *geometry, NphDVec3 position,
+                                     NphQuat rotation, f32 speculative_distance,
+                                     const NphPhysicsQueryFilter *filter,
+                                     NphPhysicsQueryContactFn fn, void *user);  // query.c:548-566
 
 // ---------------------------------------------------------------------------
 // Joints (joint.h:51-272).  Ball, hinge and fixed joints are solved; the

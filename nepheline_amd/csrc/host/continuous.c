@@ -60,170 +60,6 @@ static NphQuat slerp(NphQuat a, NphQuat b, f32 t)
 static f32 quat_angle(NphQuat q) { return 2.0F * atan2f(v3_len(q_xyz(q)), nph_absf(q.w)); }
 
 // ---------------------------------------------------------------------------
-// Proxies and GJK with poses and a simplex cache (geometry.c:446-508,
-// gjk.c:306-416, cast.c:262-458)
-// ---------------------------------------------------------------------------
-typedef struct Proxy {
-    const NphVec3 *points;
-    u32 count;
-    f32 radius;
-} Proxy;
-
-typedef struct SimplexCache {
-    u32 count;
-    u8 index_a[4];
-    u8 index_b[4];
-} SimplexCache;
-
-static Proxy geometry_proxy(const NphPhysicsGeometry *g, NphVec3 storage[8])
-{
-    Proxy p;
-    switch (g->type) {
-    case NPH_PHYSICS_SHAPE_SPHERE:
-        storage[0] = v3_zero();
-        p.points = storage;
-        p.count = 1U;
-        p.radius = g->sphere.radius;
-        return p;
-    case NPH_PHYSICS_SHAPE_CAPSULE:
-        storage[0] = v3(0.0F, -g->capsule.half_height, 0.0F);
-        storage[1] = v3(0.0F, g->capsule.half_height, 0.0F);
-        p.points = storage;
-        p.count = g->capsule.half_height > 0.0F ? 2U : 1U;
-        p.radius = g->capsule.radius;
-        return p;
-    case NPH_PHYSICS_SHAPE_BOX: {
-        const NphVec3 h = g->box.half_extents;
-        for (u32 i = 0U; i < 8U; ++i) {
-            storage[i] = v3((i & 1U) != 0U ? h.x : -h.x, (i & 2U) != 0U ? h.y : -h.y,
-                            (i & 4U) != 0U ? h.z : -h.z);
-        }
-        p.points = storage;
-        p.count = 8U;
-        p.radius = 0.0F;
-        return p;
-    }
-    default: // NPH_PHYSICS_SHAPE_HULL (meshes and heightfields are rejected)
-        p.points = hull_vertices(g->hull);
-        p.count = g->hull->vertex_count;
-        p.radius = 0.0F;
-        return p;
-    }
-}
-
-static u32 support(const Proxy *p, NphVec3 d) { return points_support(p->points, p->count, d); }
-
-// nph_physics_distance(a, pose_a, b, pose_b, use_radii = false, cache) on
-// the GJK stepper (nph_gjk.h): supports searched in each proxy's own frame,
-// vertices posed into the common one; the simplex starts from the cache
-// when it names valid points, and the cache keeps the final simplex.
-static void distance(const Proxy *a, NphPhysicsPose pa, const Proxy *b, NphPhysicsPose pb,
-                     SimplexCache *cache, NphDistance *out)
-{
-    NphGjk g;
-    gjk_reset(&g);
-    const int cached = cache->count >= 1U && cache->count <= 4U;
-    for (u32 i = 0U; cached && i < cache->count; ++i) {
-        const u32 ia = cache->index_a[i];
-        const u32 ib = cache->index_b[i];
-        if (ia >= a->count || ib >= b->count) {
-            gjk_reset(&g);
-            break;
-        }
-        gjk_seed(&g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
-    }
-    if (g.count == 0U) {
-        gjk_seed(&g, 0U, pose_point(pa, a->points[0]), 0U, pose_point(pb, b->points[0]));
-    }
-    gjk_begin(&g);
-    while (!g.done) {
-        const u32 ia = support(a, q_rotate_inv(pa.rotation, v3_neg(g.dir)));
-        const u32 ib = support(b, q_rotate_inv(pb.rotation, g.dir));
-        gjk_feed(&g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
-    }
-    cache->count = g.count;
-    for (u32 i = 0U; i < 4U; ++i) {
-        cache->index_a[i] = i < g.count ? (u8)g.v[i].ia : 0U;
-        cache->index_b[i] = i < g.count ? (u8)g.v[i].ib : 0U;
-    }
-    gjk_finish(&g, out);
-}
-
-// The gap between m and t along n (cast.c:262-272).
-static f32 proxy_gap(const Proxy *m, NphPhysicsPose pose_m, const Proxy *t, NphPhysicsPose pose_t,
-                     NphVec3 n)
-{
-    const u32 im = support(m, q_rotate_inv(pose_m.rotation, v3_neg(n)));
-    const u32 it = support(t, q_rotate_inv(pose_t.rotation, n));
-    const f32 low = v3_dot(pose_point(pose_m, m->points[im]), n) - m->radius;
-    const f32 high = v3_dot(pose_point(pose_t, t->points[it]), n) + t->radius;
-    return low - high;
-}
-
-// A face or edge-edge normal from the simplex's features (cast.c:395-446).
-static int feature_normal(const Proxy *m, NphPhysicsPose pose_m, const Proxy *t,
-                          NphPhysicsPose pose_t, const SimplexCache *c, NphVec3 hint,
-                          NphVec3 *out)
-{
-    u32 ti[4];
-    u32 mi[4];
-    u32 nt = 0U;
-    u32 nm = 0U;
-    for (u32 k = 0U; k < c->count && k < 4U; ++k) {
-        int seen_t = 0;
-        int seen_m = 0;
-        for (u32 j = 0U; j < nt; ++j) {
-            seen_t = seen_t || ti[j] == c->index_a[k];
-        }
-        for (u32 j = 0U; j < nm; ++j) {
-            seen_m = seen_m || mi[j] == c->index_b[k];
-        }
-        if (!seen_t && c->index_a[k] < t->count) {
-            ti[nt++] = c->index_a[k];
-        }
-        if (!seen_m && c->index_b[k] < m->count) {
-            mi[nm++] = c->index_b[k];
-        }
-    }
-    NphVec3 tp[4];
-    NphVec3 mp[4];
-    for (u32 j = 0U; j < nt; ++j) {
-        tp[j] = pose_point(pose_t, t->points[ti[j]]);
-    }
-    for (u32 j = 0U; j < nm; ++j) {
-        mp[j] = pose_point(pose_m, m->points[mi[j]]);
-    }
-    NphVec3 n;
-    if (nt >= 3U && nm == 1U) {
-        n = v3_cross(v3_sub(tp[1], tp[0]), v3_sub(tp[2], tp[0]));
-    } else if (nm >= 3U && nt == 1U) {
-        n = v3_cross(v3_sub(mp[1], mp[0]), v3_sub(mp[2], mp[0]));
-    } else if (nt == 2U && nm == 2U) {
-        n = v3_cross(v3_sub(tp[1], tp[0]), v3_sub(mp[1], mp[0]));
-    } else {
-        return 0;
-    }
-    const f32 len = v3_len(n);
-    if (!(len > 1e-12F)) {
-        return 0;
-    }
-    n = v3_scale(n, 1.0F / len);
-    *out = v3_dot(n, hint) < 0.0F ? v3_neg(n) : n;
-    return 1;
-}
-
-static NphVec3 contact_axis(const Proxy *m, NphPhysicsPose pose_m, const Proxy *t,
-                            NphPhysicsPose pose_t, const SimplexCache *features, NphVec3 hint)
-{
-    NphVec3 n;
-    if (feature_normal(m, pose_m, t, pose_t, features, hint, &n) &&
-        proxy_gap(m, pose_m, t, pose_t, n) >= proxy_gap(m, pose_m, t, pose_t, hint)) {
-        return n;
-    }
-    return hint;
-}
-
-// ---------------------------------------------------------------------------
 // The sweep (continuous.c:64-219)
 // ---------------------------------------------------------------------------
 typedef struct Sweep {
@@ -241,10 +77,10 @@ static void sweep_at(const Sweep *s, f32 t, NphVec3 *centre, NphQuat *rotation)
 }
 
 typedef struct Mover {
-    Proxy proxy;
+    NphProxy proxy;
     NphVec3 storage[8];
-    Proxy inset;
-    Proxy core;
+    NphProxy inset;
+    NphProxy core;
     NphVec3 core_point;
     NphVec3 local;
     NphQuat local_q;
@@ -252,7 +88,7 @@ typedef struct Mover {
 
 static void mover_init(Mover *mv, const NphBodyRec *b, const HShape *s)
 {
-    mv->proxy = geometry_proxy(&s->geometry, mv->storage);
+    mv->proxy = proxy_of(&s->geometry, mv->storage);
     const NphGeom g = world_host_geom(s);
     const f32 inner = geom_inner_radius(&g);
     mv->inset = mv->proxy;
@@ -274,26 +110,27 @@ static NphPhysicsPose mover_at(const Mover *mv, const Sweep *s, f32 t)
     return pose_make(v3_add(c, q_rotate(q, mv->local)), q_mul(q, mv->local_q));
 }
 
-static f32 advance(const Mover *mv, const Proxy *m, const Sweep *s, const Proxy *t,
+static f32 advance(const Mover *mv, const NphProxy *m, const Sweep *s, const NphProxy *t,
                    NphPhysicsPose pose_t, f32 limit, NphVec3 *axis)
 {
     const f32 radius = m->radius + t->radius;
-    SimplexCache cache;
+    NphSimplexCache cache;
     memset(&cache, 0, sizeof cache);
     f32 alpha = 0.0F;
     for (u32 step = 0U; step < TOI_STEPS; ++step) {
         const NphPhysicsPose pm = mover_at(mv, s, alpha);
         const NphPhysicsPose at_m = pose_make(v3_zero(), pm.rotation);
         const NphPhysicsPose at_t = pose_make(v3_sub(pose_t.position, pm.position), pose_t.rotation);
-        NphDistance d;
-        distance(t, at_t, m, at_m, &cache, &d);
+        NphProxyDistance pd;
+        proxy_distance(t, at_t, m, at_m, 0, &cache, &pd);
+        const NphDistance d = pd.d;
         const f32 separation = d.distance - radius;
         if (!(d.distance > 0.0F)) {
             return alpha;
         }
         *axis = d.normal;
         if (separation <= TOI_TARGET + TOI_TOLERANCE) {
-            *axis = contact_axis(m, at_m, t, at_t, &cache, d.normal);
+            *axis = proxy_contact_axis(m, at_m, t, at_t, &cache, d.normal);
             return alpha;
         }
         const f32 closing = -v3_dot(s->dc, d.normal) + s->spin;
@@ -308,7 +145,7 @@ static f32 advance(const Mover *mv, const Proxy *m, const Sweep *s, const Proxy 
     return alpha;
 }
 
-static f32 gap_at(const Mover *mv, const Proxy *m, const Sweep *s, const Proxy *t,
+static f32 gap_at(const Mover *mv, const NphProxy *m, const Sweep *s, const NphProxy *t,
                   NphPhysicsPose pose_t, NphVec3 axis, f32 alpha)
 {
     const NphPhysicsPose pm = mover_at(mv, s, alpha);
@@ -316,7 +153,7 @@ static f32 gap_at(const Mover *mv, const Proxy *m, const Sweep *s, const Proxy *
                      pose_make(v3_sub(pose_t.position, pm.position), pose_t.rotation), axis);
 }
 
-static f32 shape_toi(const Mover *mv, const Sweep *s, const Proxy *t, NphPhysicsPose pose_t,
+static f32 shape_toi(const Mover *mv, const Sweep *s, const NphProxy *t, NphPhysicsPose pose_t,
                      f32 limit)
 {
     NphVec3 axis = v3(0.0F, 1.0F, 0.0F);
@@ -363,7 +200,7 @@ static void sweep_target(BodySweep *bs, u32 target_body, const HShape *ts)
         return;
     }
     NphVec3 storage[8];
-    const Proxy tp = geometry_proxy(&ts->geometry, storage);
+    const NphProxy tp = proxy_of(&ts->geometry, storage);
     NphShapeRec trec;
     memset(&trec, 0, sizeof trec);
     trec.pose = ts->pose;
@@ -492,6 +329,7 @@ static NphResult finish_body(NphPhysicsWorld *w, const NphFastBody *f, NphBodyRe
     u32 i = 0U;
     for (u32 s = hb->shape_head; s != NPH_NULL32 && r == NPH_RESULT_OK && i < n;
          s = VB(w->shapes, HShape)[s].next, ++i) {
+        w->shape_view_ok = 0;
         r = nph_dev_write_shape(w->dev, s, &recs[i]);
     }
     return r;

@@ -335,6 +335,7 @@ void nph_physics_world_destroy(NphPhysicsWorld *w)
     vbuf_free(a, &w->end_events, sizeof(NphPhysicsContactEndEvent));
     vbuf_free(a, &w->move_events, sizeof(NphPhysicsBodyMoveEvent));
     vbuf_free(a, &w->contact_view, sizeof(NphContactRec));
+    vbuf_free(a, &w->shape_view, sizeof(NphShapeRec));
     islands_free(w);
     for (u32 t = 0U; t < 3U; ++t) {
         tree_free(&w->trees[t]);
@@ -845,6 +846,7 @@ NphResult world_refresh_shape(NphPhysicsWorld *w, u32 s)
 {
     NphShapeRec rec;
     NphResult r = world_shape_record(w, s, &rec);
+    w->shape_view_ok = 0;
     return r == NPH_RESULT_OK ? nph_dev_write_shape(w->dev, s, &rec) : r;
 }
 
@@ -953,6 +955,7 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
     }
     if (r == NPH_RESULT_OK) {
         const NphShapeRec rec = shape_record(w, slot, device_block);
+        w->shape_view_ok = 0;
         r = nph_dev_write_shape(w->dev, slot, &rec);
         if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH) {
             nph_dev_note_mesh_shape(w->dev);
@@ -1293,6 +1296,7 @@ static NphResult solve(NphPhysicsWorld *w, f32 dt)
 
 void world_step_begin(NphPhysicsWorld *w, f32 dt)
 {
+    w->shape_view_ok = 0; // the step moves the tight boxes
     w->stepping = 1;
     w->step_dt = dt;
     w->begin_events.len = 0U;

@@ -256,6 +256,8 @@ struct NphPhysicsWorld {
     int move_events_ready;
     u32 move_pending; // the awake bodies the step's move events cover
     NphVecBuf contact_view; // NphContactRec, readback scratch
+    NphVecBuf shape_view;   // NphShapeRec as the device holds them, for queries
+    int shape_view_ok;      // shape_view is current
 
     // Pair finding on the device: the search, what changed since the last
     // one (shapes' filter data, filter-set keys), and the set's size.
@@ -307,6 +309,39 @@ void islands_free(NphPhysicsWorld *w);
 // contact.c / joint.c: a body's contacts' and joints' membership brought in
 // line with their bodies.
 NphResult contacts_sync_body(NphPhysicsWorld *w, u32 body);
+// cast.c: convex proxies, GJK with poses and a simplex cache, casts and
+// overlaps against one geometry.
+typedef struct NphProxy {
+    const NphVec3 *points;
+    u32 count;
+    f32 radius;
+} NphProxy;
+typedef struct NphSimplexCache {
+    u32 count;
+    u8 index_a[4];
+    u8 index_b[4];
+} NphSimplexCache;
+typedef struct NphProxyDistance {
+    NphDistance d;
+    u32 simplex_count; // 4: the cores overlap
+} NphProxyDistance;
+NphProxy proxy_of(const NphPhysicsGeometry *g, NphVec3 storage[8]);
+void proxy_distance(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, NphPhysicsPose pb,
+                    int rounded, NphSimplexCache *cache, NphProxyDistance *out);
+f32 proxy_gap(const NphProxy *m, NphPhysicsPose pose_m, const NphProxy *t, NphPhysicsPose pose_t,
+              NphVec3 n);
+NphVec3 proxy_contact_axis(const NphProxy *m, NphPhysicsPose pose_m, const NphProxy *t,
+                           NphPhysicsPose pose_t, const NphSimplexCache *features, NphVec3 hint);
+int raycast_local(const NphPhysicsGeometry *g, NphVec3 origin, NphVec3 delta,
+                  NphPhysicsCastHit *out);
+int cast_convex(const NphPhysicsGeometry *a, NphPhysicsPose pose_a, NphVec3 delta, f32 margin,
+                const NphPhysicsGeometry *b, NphPhysicsPose pose_b, f32 limit,
+                NphPhysicsCastHit *out);
+int overlap_convex(const NphPhysicsGeometry *a, NphPhysicsPose pose_a, const NphPhysicsGeometry *b,
+                   NphPhysicsPose pose_b);
+int overlap_mesh_local(const NphPhysicsGeometry *a, NphQuat rotation, NphDVec3 position,
+                       const NphPhysicsMesh *mesh);
+
 // contact.c: the device pair search's change records and its host steps.
 void pairs_mark_shape(NphPhysicsWorld *w, u32 shape);
 void pairs_mark_body(NphPhysicsWorld *w, u32 body);

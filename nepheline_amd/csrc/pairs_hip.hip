@@ -24,6 +24,8 @@
 #include "device_impl.h"
 
 #define PF_BLOCK 256U
+#define PF_QBLOCK 64U // a query's walk is long and branchy: small blocks spread it over the CUs
+#define PF_SMALL 64U  // moved proxies a one-block search takes
 #define PF_SCAN 1024U
 #define PF_STACK 64U        // depth-first stack of a tree of height up to 62
 #define PF_STACK_DEEP 1024U // aabb_tree.h:50, for taller trees
@@ -297,9 +299,9 @@ __global__ void __launch_bounds__(PF_BLOCK) k_pf_insert(PfArgs a)
 }
 
 template <u32 STACK>
-__global__ void __launch_bounds__(PF_BLOCK) k_pf_count(PfArgs a)
+__global__ void __launch_bounds__(PF_QBLOCK) k_pf_count(PfArgs a)
 {
-    const u32 t = blockIdx.x * PF_BLOCK + threadIdx.x;
+    const u32 t = blockIdx.x * PF_QBLOCK + threadIdx.x;
     if (t < a.moved_count) {
         const PfMoved m = a.moved[t];
         a.counts[t] = pf_query<STACK, false>(a.worlds[m.world], m, NULL);
@@ -376,9 +378,9 @@ __global__ void __launch_bounds__(PF_SCAN) k_pf_scan_bases(PfArgs a, u32 chunk_c
 
 // Global offsets, then the pairs where they fit the buffer.
 template <u32 STACK>
-__global__ void __launch_bounds__(PF_BLOCK) k_pf_fill(PfArgs a)
+__global__ void __launch_bounds__(PF_QBLOCK) k_pf_fill(PfArgs a)
 {
-    const u32 t = blockIdx.x * PF_BLOCK + threadIdx.x;
+    const u32 t = blockIdx.x * PF_QBLOCK + threadIdx.x;
     if (t < a.moved_count) {
         const u32 off = a.chunks[t / PF_SCAN] + a.offsets[t];
         a.offsets[t] = off;
@@ -391,9 +393,9 @@ __global__ void __launch_bounds__(PF_BLOCK) k_pf_fill(PfArgs a)
 
 // The pairs again, every one (after the buffer grew).
 template <u32 STACK>
-__global__ void __launch_bounds__(PF_BLOCK) k_pf_refill(PfArgs a)
+__global__ void __launch_bounds__(PF_QBLOCK) k_pf_refill(PfArgs a)
 {
-    const u32 t = blockIdx.x * PF_BLOCK + threadIdx.x;
+    const u32 t = blockIdx.x * PF_QBLOCK + threadIdx.x;
     if (t < a.moved_count) {
         const PfMoved m = a.moved[t];
         (void)pf_query<STACK, true>(a.worlds[m.world], m, a.pairs + 2ULL * a.offsets[t]);
@@ -745,8 +747,8 @@ NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const 
         hipLaunchKernelGGL(k_pf_insert, dim3(devi_blocks((u32)nk, PF_BLOCK)), dim3(PF_BLOCK), 0, s, a);
         HIP_TRY(hipGetLastError());
     }
-    const u32 blocks = devi_blocks(moved > 0U ? moved : 1U, PF_BLOCK);
-    if (moved <= PF_SCAN) {
+    const u32 blocks = devi_blocks(moved > 0U ? moved : 1U, PF_QBLOCK);
+    if (moved <= PF_SMALL) {
         if (deep) {
             hipLaunchKernelGGL(k_pf_small<PF_STACK_DEEP>, dim3(1), dim3(PF_SCAN), 0, s, a);
         } else {
@@ -755,16 +757,16 @@ NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const 
         HIP_TRY(hipGetLastError());
     } else {
         if (deep) {
-            hipLaunchKernelGGL(k_pf_count<PF_STACK_DEEP>, dim3(blocks), dim3(PF_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(k_pf_count<PF_STACK_DEEP>, dim3(blocks), dim3(PF_QBLOCK), 0, s, a);
         } else {
-            hipLaunchKernelGGL(k_pf_count<PF_STACK>, dim3(blocks), dim3(PF_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(k_pf_count<PF_STACK>, dim3(blocks), dim3(PF_QBLOCK), 0, s, a);
         }
         hipLaunchKernelGGL(k_pf_scan_chunk, dim3(chunk_count), dim3(PF_SCAN), 0, s, a);
         hipLaunchKernelGGL(k_pf_scan_bases, dim3(1), dim3(PF_SCAN), 0, s, a, chunk_count);
         if (deep) {
-            hipLaunchKernelGGL(k_pf_fill<PF_STACK_DEEP>, dim3(blocks), dim3(PF_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(k_pf_fill<PF_STACK_DEEP>, dim3(blocks), dim3(PF_QBLOCK), 0, s, a);
         } else {
-            hipLaunchKernelGGL(k_pf_fill<PF_STACK>, dim3(blocks), dim3(PF_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(k_pf_fill<PF_STACK>, dim3(blocks), dim3(PF_QBLOCK), 0, s, a);
         }
         hipLaunchKernelGGL(k_pf_starts, dim3(devi_blocks(n + 1U, PF_BLOCK)), dim3(PF_BLOCK), 0, s, a,
                            chunk_count);
@@ -783,9 +785,9 @@ NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const 
         a.pairs = p->d_pairs;
         a.pair_cap = c;
         if (deep) {
-            hipLaunchKernelGGL(k_pf_refill<PF_STACK_DEEP>, dim3(blocks), dim3(PF_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(k_pf_refill<PF_STACK_DEEP>, dim3(blocks), dim3(PF_QBLOCK), 0, s, a);
         } else {
-            hipLaunchKernelGGL(k_pf_refill<PF_STACK>, dim3(blocks), dim3(PF_BLOCK), 0, s, a);
+            hipLaunchKernelGGL(k_pf_refill<PF_STACK>, dim3(blocks), dim3(PF_QBLOCK), 0, s, a);
         }
         HIP_TRY(hipGetLastError());
     }

@@ -166,6 +166,148 @@ class MeshHit(C.Structure):
                 ("original", C.c_uint32), ("back", C.c_int32)]
 
 
+class CastHit(C.Structure):
+    """collide.h:246-253"""
+    _fields_ = [("point", Vec3), ("normal", Vec3), ("fraction", C.c_float),
+                ("triangle", C.c_uint32)]
+
+
+class QueryFilter(C.Structure):
+    """query.h:61-67"""
+    _fields_ = [("category", C.c_uint64), ("mask", C.c_uint64), ("body_types", C.c_uint32),
+                ("ignore", C.c_uint32), ("sensors", C.c_int32)]
+
+
+class QueryHit(C.Structure):
+    """query.h:73-80"""
+    _fields_ = [("shape", C.c_uint32), ("body", C.c_uint32), ("point", DVec3), ("normal", Vec3),
+                ("fraction", C.c_float), ("triangle", C.c_uint32)]
+
+
+class QueryContact(C.Structure):
+    """query.h:91-100"""
+    _fields_ = [("shape", C.c_uint32), ("body", C.c_uint32), ("manifold", Manifold)]
+
+
+QUERY_CAST_FN = C.CFUNCTYPE(C.c_float, C.c_void_p, C.POINTER(QueryHit))
+QUERY_SHAPE_FN = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.c_uint32, C.c_uint32)
+QUERY_CONTACT_FN = C.CFUNCTYPE(C.c_bool, C.c_void_p, C.POINTER(QueryContact))
+
+
+def bind_query(a):
+    """The casts, overlaps and world queries of either library (collide.h:240-318,
+    query.h:51-172)."""
+    if hasattr(a, "world_raycast"):
+        return a
+    L = a.lib
+    PG = C.POINTER(Geometry)
+    PF = C.POINTER(QueryFilter)
+    a.raycast = _bind(L, "nph_physics_raycast", C.c_bool, [PG, Pose, Vec3, Vec3, C.POINTER(CastHit)])
+    a.shape_cast_margin = _bind(L, "nph_physics_shape_cast_margin", C.c_bool,
+                                [PG, Pose, Vec3, C.c_float, PG, Pose, C.POINTER(CastHit)])
+    a.overlap = _bind(L, "nph_physics_overlap", C.c_bool, [PG, Pose, PG, Pose])
+    a.query_filter_default = _bind(L, "nph_physics_query_filter_default", QueryFilter, [])
+    a.world_raycast = _bind(L, "nph_physics_world_raycast", None,
+                            [C.c_void_p, DVec3, Vec3, PF, QUERY_CAST_FN, C.c_void_p])
+    a.world_raycast_closest = _bind(L, "nph_physics_world_raycast_closest", C.c_bool,
+                                    [C.c_void_p, DVec3, Vec3, PF, C.POINTER(QueryHit)])
+    a.world_cast_shape_margin = _bind(L, "nph_physics_world_cast_shape_margin", None,
+                                      [C.c_void_p, PG, DVec3, Quat, Vec3, C.c_float, PF,
+                                       QUERY_CAST_FN, C.c_void_p])
+    a.world_cast_shape_closest = _bind(L, "nph_physics_world_cast_shape_closest", C.c_bool,
+                                       [C.c_void_p, PG, DVec3, Quat, Vec3, PF,
+                                        C.POINTER(QueryHit)])
+    a.world_overlap_shape = _bind(L, "nph_physics_world_overlap_shape", None,
+                                  [C.c_void_p, PG, DVec3, Quat, PF, QUERY_SHAPE_FN, C.c_void_p])
+    a.world_overlap_aabb = _bind(L, "nph_physics_world_overlap_aabb", None,
+                                 [C.c_void_p, AABB, PF, QUERY_SHAPE_FN, C.c_void_p])
+    a.world_collide_shape = _bind(L, "nph_physics_world_collide_shape", None,
+                                  [C.c_void_p, PG, DVec3, Quat, C.c_float, PF, QUERY_CONTACT_FN,
+                                   C.c_void_p])
+    return a
+
+
+def _hit_tuple(h):
+    return (h.shape, h.body, h.point.x, h.point.y, h.point.z, h.normal.x, h.normal.y, h.normal.z,
+            h.fraction, h.triangle)
+
+
+def world_raycast_all(a, world_ptr, origin, translation, flt=None, answer=1.0, stop_after=None):
+    """Every hit nph_physics_world_raycast reports, in order, as tuples; the
+    callback answers `answer` (1: all hits; use the hit's fraction for the
+    closest's search), or -1 after `stop_after` hits."""
+    bind_query(a)
+    out = []
+
+    def fn(_u, hp):
+        h = hp.contents
+        out.append(_hit_tuple(h))
+        if stop_after is not None and len(out) >= stop_after:
+            return -1.0
+        return h.fraction if answer == "fraction" else answer
+
+    cb = QUERY_CAST_FN(fn)
+    a.world_raycast(world_ptr, DVec3(*origin), Vec3(*translation),
+                    C.byref(flt) if flt is not None else None, cb, None)
+    return out
+
+
+def world_cast_all(a, world_ptr, geometry, position, rotation, translation, margin=0.0, flt=None,
+                   answer=1.0):
+    bind_query(a)
+    out = []
+
+    def fn(_u, hp):
+        h = hp.contents
+        out.append(_hit_tuple(h))
+        return h.fraction if answer == "fraction" else answer
+
+    cb = QUERY_CAST_FN(fn)
+    a.world_cast_shape_margin(world_ptr, C.byref(geometry), DVec3(*position), Quat(*rotation),
+                              Vec3(*translation), margin,
+                              C.byref(flt) if flt is not None else None, cb, None)
+    return out
+
+
+def world_overlaps(a, world_ptr, geometry=None, position=None, rotation=None, box=None, flt=None):
+    """The (shape, body) pairs an overlap query reports, in order: a shape's
+    with `geometry`, else the box's."""
+    bind_query(a)
+    out = []
+
+    def fn(_u, shape, body):
+        out.append((shape, body))
+        return True
+
+    cb = QUERY_SHAPE_FN(fn)
+    f = C.byref(flt) if flt is not None else None
+    if geometry is not None:
+        a.world_overlap_shape(world_ptr, C.byref(geometry), DVec3(*position), Quat(*rotation), f,
+                              cb, None)
+    else:
+        a.world_overlap_aabb(world_ptr, AABB(Vec3(*box[0]), Vec3(*box[1])), f, cb, None)
+    return out
+
+
+def world_contacts_with(a, world_ptr, geometry, position, rotation, speculative, flt=None):
+    """The manifolds nph_physics_world_collide_shape reports, in order."""
+    bind_query(a)
+    out = []
+
+    def fn(_u, cp):
+        c = cp.contents
+        m = c.manifold
+        pts = tuple((m.points[k].point.x, m.points[k].point.y, m.points[k].point.z,
+                     m.points[k].separation, m.points[k].id) for k in range(m.point_count))
+        out.append((c.shape, c.body, m.normal.x, m.normal.y, m.normal.z, pts))
+        return True
+
+    cb = QUERY_CONTACT_FN(fn)
+    a.world_collide_shape(world_ptr, C.byref(geometry), DVec3(*position), Quat(*rotation),
+                          speculative, C.byref(flt) if flt is not None else None, cb, None)
+    return out
+
+
 def mesh_overlap(a, mesh_ptr, lo, hi, stop_after=None):
     """The triangles nph_physics_mesh_overlap visits for the box (lo, hi),
     in order; with `stop_after`, the visit returns false at that count."""
