@@ -1,20 +1,26 @@
 // SPDX-License-Identifier: Apache-2.0
 // Hull blocks on the host (engine/physics/include/nepheline/physics/hull.h):
-// the block packer, the box hull, the Quickhull builder, and the block
-// queries and validator.
+// the packer that turns face loops into a block, the box hull, the Quickhull
+// builder, the two block queries and the validator.
 //
-// A hull's topology is part of every contact it makes: face and half-edge
-// numbers go into the manifold points' ids, which key the warm start. So the
-// builder must write the reference's bytes for the same cloud, not merely
-// some valid hull of it. It takes the reference's decisions in the
-// reference's order (hull.c:350-1197): the initial tetrahedron from the
-// extreme points, the farthest outside point first, the depth-first horizon,
-// John Lloyd's two merge passes (QuickHull3D: non-convex with respect to the
-// larger face, then with respect to either), orphaned points re-assigned to
-// the new faces, every distance in f64 on the f32 input; and it packs faces,
-// half-edges and vertices in the same order with the same f64 arithmetic
-// for planes and mass (hull.c:128-309). The block is host data: worlds copy
-// it to the device when a shape first uses it.
+// A hull's numbering is part of every contact it makes: face and half-edge
+// numbers go into the manifold points' ids, which key the warm start. So a
+// build must give the reference's bytes for the same cloud (hull.c:128-1197),
+// not merely some valid hull of it. The decisions that fix those bytes:
+//   - the seed tetrahedron from the extreme points of the widest axis;
+//   - the next point is the highest outside point over all faces (ties: the
+//     lowest face number), until the vertex budget is spent;
+//   - the faces that see it are found depth first from its face, and the new
+//     faces fan from the horizon in the order that walk meets it;
+//   - John Lloyd's two merge passes (QuickHull3D): across edges non-convex
+//     with respect to the larger face, then with respect to either;
+//   - outside sets are last-in-first-out lists, and points orphaned by
+//     deleted faces go to the new face they are highest above;
+//   - all of it in f64 on the f32 input; the block lists live faces in the
+//     order they were made, each loop from its lead half-edge, and vertices
+//     in the order the loops first reach them.
+// Blocks are host data: worlds copy them to the device when a shape first
+// uses one.
 
 #include "world_impl.h"
 
@@ -22,15 +28,15 @@
 #include <math.h>
 #include <stdio.h>
 
-#define HNONE 0xFFFFFFFFu
+#define NIL 0xFFFFFFFFu
 
-static void *hull_take(NphAllocator *a, u64 size)
+static void *grab(NphAllocator *a, u64 size)
 {
     NphCallsite cs = {__FILE__, (u32)__LINE__};
     return a->vtable->alloc(a, size > 0U ? size : 16U, 16U, cs);
 }
 
-static void hull_give(NphAllocator *a, void *p, u64 size)
+static void drop(NphAllocator *a, void *p, u64 size)
 {
     NphCallsite cs = {__FILE__, (u32)__LINE__};
     if (p != NULL) {
@@ -38,16 +44,17 @@ static void hull_give(NphAllocator *a, void *p, u64 size)
     }
 }
 
-static u32 up16(u32 n)
+static u32 round16(u32 n)
 {
     return (n + 15U) & ~15U;
 }
 
 // ---------------------------------------------------------------------------
-// f64 vectors (vec.h:899-964: component-wise, dot left to right, scaling by
-// a reciprocal)
+// Vector arithmetic. f64: component-wise, dot products summed x, y, z,
+// scaling by a reciprocal (engine/math vec.h:899-964); f32 likewise
+// (vec.h:300-368); a plane's height is n . p + d (geometry.h:289).
 // ---------------------------------------------------------------------------
-static NphDVec3 d3(f64 x, f64 y, f64 z)
+static NphDVec3 dv(f64 x, f64 y, f64 z)
 {
     NphDVec3 r;
     r.x = x;
@@ -55,11 +62,11 @@ static NphDVec3 d3(f64 x, f64 y, f64 z)
     r.z = z;
     return r;
 }
-static NphDVec3 d3_of(NphVec3 v)
+static NphDVec3 dv_wide(NphVec3 v)
 {
-    return d3((f64)v.x, (f64)v.y, (f64)v.z);
+    return dv((f64)v.x, (f64)v.y, (f64)v.z);
 }
-static NphVec3 d3_narrow(NphDVec3 v)
+static NphVec3 dv_thin(NphDVec3 v)
 {
     NphVec3 r;
     r.x = (f32)v.x;
@@ -67,53 +74,53 @@ static NphVec3 d3_narrow(NphDVec3 v)
     r.z = (f32)v.z;
     return r;
 }
-static NphDVec3 d3_add(NphDVec3 a, NphDVec3 b)
+static NphDVec3 dv_add(NphDVec3 a, NphDVec3 b)
 {
-    return d3(a.x + b.x, a.y + b.y, a.z + b.z);
+    return dv(a.x + b.x, a.y + b.y, a.z + b.z);
 }
-static NphDVec3 d3_sub(NphDVec3 a, NphDVec3 b)
+static NphDVec3 dv_sub(NphDVec3 a, NphDVec3 b)
 {
-    return d3(a.x - b.x, a.y - b.y, a.z - b.z);
+    return dv(a.x - b.x, a.y - b.y, a.z - b.z);
 }
-static NphDVec3 d3_scale(NphDVec3 v, f64 s)
+static NphDVec3 dv_mul(NphDVec3 v, f64 s)
 {
-    return d3(v.x * s, v.y * s, v.z * s);
+    return dv(v.x * s, v.y * s, v.z * s);
 }
-static f64 d3_dot(NphDVec3 a, NphDVec3 b)
+static f64 dv_dot(NphDVec3 a, NphDVec3 b)
 {
     return a.x * b.x + a.y * b.y + a.z * b.z;
 }
-static NphDVec3 d3_cross(NphDVec3 a, NphDVec3 b)
+static NphDVec3 dv_cross(NphDVec3 a, NphDVec3 b)
 {
-    return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+    return dv(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-static f64 d3_comp(NphDVec3 v, u32 i)
+static f64 dv_axis(NphDVec3 v, u32 k)
 {
-    return i == 0U ? v.x : (i == 1U ? v.y : v.z);
+    return k == 0U ? v.x : (k == 1U ? v.y : v.z);
 }
-// The unit vector along v; zero for a zero or non-finite v; a v whose
-// squared length overflows is scaled down by its largest component first.
-static NphDVec3 d3_unit_or_zero(NphDVec3 v)
+// v's direction; zero for a zero or non-finite v; a finite v whose squared
+// length overflows is first divided by its largest magnitude.
+static NphDVec3 dv_unit(NphDVec3 v)
 {
-    const f64 sq = d3_dot(v, v);
+    const f64 sq = dv_dot(v, v);
     if (sq > 0.0 && sq <= DBL_MAX) {
-        return d3_scale(v, 1.0 / sqrt(sq));
+        return dv_mul(v, 1.0 / sqrt(sq));
     }
     if (!(sq > DBL_MAX) || !isfinite(v.x) || !isfinite(v.y) || !isfinite(v.z)) {
-        return d3(0.0, 0.0, 0.0);
+        return dv(0.0, 0.0, 0.0);
     }
-    const f64 ax = fabs(v.x), ay = fabs(v.y), az = fabs(v.z);
-    const f64 m = ax > (ay > az ? ay : az) ? ax : (ay > az ? ay : az);
-    const NphDVec3 u = d3_scale(v, 1.0 / m);
-    return d3_scale(u, 1.0 / sqrt(d3_dot(u, u)));
+    const f64 ay = fabs(v.y), az = fabs(v.z);
+    const f64 yz = ay > az ? ay : az;
+    const f64 big = fabs(v.x) > yz ? fabs(v.x) : yz;
+    const NphDVec3 w = dv_mul(v, 1.0 / big);
+    return dv_mul(w, 1.0 / sqrt(dv_dot(w, w)));
 }
 
-// f32 helpers in the reference's order (vec.h:300-368, geometry.h:289).
-static f32 f3_dot(NphVec3 a, NphVec3 b)
+static f32 fv_dot(NphVec3 a, NphVec3 b)
 {
     return a.x * b.x + a.y * b.y + a.z * b.z;
 }
-static NphVec3 f3_sub(NphVec3 a, NphVec3 b)
+static NphVec3 fv_sub(NphVec3 a, NphVec3 b)
 {
     NphVec3 r;
     r.x = a.x - b.x;
@@ -121,7 +128,7 @@ static NphVec3 f3_sub(NphVec3 a, NphVec3 b)
     r.z = a.z - b.z;
     return r;
 }
-static NphVec3 f3_cross(NphVec3 a, NphVec3 b)
+static NphVec3 fv_cross(NphVec3 a, NphVec3 b)
 {
     NphVec3 r;
     r.x = a.y * b.z - a.z * b.y;
@@ -129,268 +136,330 @@ static NphVec3 f3_cross(NphVec3 a, NphVec3 b)
     r.z = a.x * b.y - a.y * b.x;
     return r;
 }
-static f32 f3_len(NphVec3 v)
+static f32 fv_len(NphVec3 v)
 {
-    return sqrtf(f3_dot(v, v));
+    return sqrtf(fv_dot(v, v));
 }
-static f32 f_min(f32 a, f32 b)
+static f32 lesser(f32 a, f32 b)
 {
     return a < b ? a : b;
 }
-static f32 f_max(f32 a, f32 b)
+static f32 greater(f32 a, f32 b)
 {
     return a > b ? a : b;
 }
-static f32 plane_height(NphPlane p, NphVec3 x)
+static f32 above(NphPlane p, NphVec3 x)
 {
-    return f3_dot(p.normal, x) + p.d;
+    return fv_dot(p.normal, x) + p.d;
 }
-static bool f3_finite(NphVec3 v)
+static bool fv_finite(NphVec3 v)
 {
     return isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
 }
-
-// ---------------------------------------------------------------------------
-// Mass properties of a closed polyhedron: Eberly, "Polyhedral Mass
-// Properties (Revisited)", Geometric Tools 2002 -- the published
-// subexpressions per coordinate and the ten integrals (1, x, y, z, x^2, y^2,
-// z^2, xy, yz, zx) summed over a fan of triangles per face. The operation
-// order is the publication's, which the reference also keeps (hull.c:69-126).
-// ---------------------------------------------------------------------------
-typedef struct Eberly {
-    f64 f1, f2, f3, g0, g1, g2;
-} Eberly;
-
-static Eberly eberly(f64 w0, f64 w1, f64 w2)
+// |x| within the coordinate limit; false for NaN.
+static bool fv_bounded(NphVec3 v)
 {
-    Eberly t;
+    const f32 lim = NPH_PHYSICS_MAX_COORDINATE;
+    return fabsf(v.x) <= lim && fabsf(v.y) <= lim && fabsf(v.z) <= lim;
+}
+
+// ---------------------------------------------------------------------------
+// Block layout: header, vertices (16-byte padded), planes, face leads
+// (padded), half-edges (padded), all offsets from the block's start.
+// ---------------------------------------------------------------------------
+typedef struct Layout {
+    u32 verts, planes, leads, edges, size;
+} Layout;
+
+static Layout layout_of(u32 nv, u32 nf, u32 ne)
+{
+    Layout l;
+    l.verts = (u32)sizeof(NphPhysicsHull);
+    l.planes = l.verts + round16(nv * (u32)sizeof(NphVec3));
+    l.leads = l.planes + nf * (u32)sizeof(NphPlane);
+    l.edges = l.leads + round16(nf * (u32)sizeof(u16));
+    l.size = l.edges + round16(ne * (u32)sizeof(NphPhysicsHullEdge));
+    return l;
+}
+
+#define AT(block, off, T) ((T *)((u8 *)(block) + (off)))
+#define AT_C(block, off, T) ((const T *)((const u8 *)(block) + (off)))
+
+// ---------------------------------------------------------------------------
+// Mass properties of a closed polyhedron by Eberly's method ("Polyhedral
+// Mass Properties (Revisited)", Geometric Tools, 2002): each triangle of a
+// fan adds its share of the ten integrals of 1, x, y, z, x^2, y^2, z^2, xy,
+// yz and zx, from the per-coordinate subexpressions f1, f2, f3, g0, g1, g2
+// the paper derives. The sums keep the paper's order of operations.
+// ---------------------------------------------------------------------------
+enum { M_VOL, M_X, M_Y, M_Z, M_XX, M_YY, M_ZZ, M_XY, M_YZ, M_ZX, M_COUNT };
+
+static void eberly_terms(f64 w0, f64 w1, f64 w2, f64 out[6])
+{
     const f64 s01 = w0 + w1;
-    const f64 w0sq = w0 * w0;
-    t.f1 = s01 + w2;
-    const f64 q = w0sq + w1 * s01;
-    t.f2 = q + w2 * t.f1;
-    t.f3 = w0 * w0sq + w1 * q + w2 * t.f2;
-    t.g0 = t.f2 + w0 * (t.f1 + w0);
-    t.g1 = t.f2 + w1 * (t.f1 + w1);
-    t.g2 = t.f2 + w2 * (t.f1 + w2);
-    return t;
+    const f64 sq0 = w0 * w0;
+    const f64 f1 = s01 + w2;
+    const f64 part = sq0 + w1 * s01;
+    const f64 f2 = part + w2 * f1;
+    out[0] = f1;
+    out[1] = f2;
+    out[2] = w0 * sq0 + w1 * part + w2 * f2;
+    out[3] = f2 + w0 * (f1 + w0);
+    out[4] = f2 + w1 * (f1 + w1);
+    out[5] = f2 + w2 * (f1 + w2);
 }
 
-static void mass_triangle(f64 acc[10], NphDVec3 p0, NphDVec3 p1, NphDVec3 p2)
+static void integrate_triangle(f64 m[M_COUNT], NphDVec3 p0, NphDVec3 p1, NphDVec3 p2)
 {
-    const NphDVec3 u = d3_sub(p1, p0);
-    const NphDVec3 v = d3_sub(p2, p0);
-    // The unnormalised triangle normal (b1 c2 - b2 c1, a2 c1 - a1 c2, a1 b2 - a2 b1).
-    const f64 nx = u.y * v.z - v.y * u.z;
-    const f64 ny = v.x * u.z - u.x * v.z;
-    const f64 nz = u.x * v.y - v.x * u.y;
-    const Eberly ex = eberly(p0.x, p1.x, p2.x);
-    const Eberly ey = eberly(p0.y, p1.y, p2.y);
-    const Eberly ez = eberly(p0.z, p1.z, p2.z);
-    acc[0] += nx * ex.f1;
-    acc[1] += nx * ex.f2;
-    acc[2] += ny * ey.f2;
-    acc[3] += nz * ez.f2;
-    acc[4] += nx * ex.f3;
-    acc[5] += ny * ey.f3;
-    acc[6] += nz * ez.f3;
-    acc[7] += nx * (p0.y * ex.g0 + p1.y * ex.g1 + p2.y * ex.g2);
-    acc[8] += ny * (p0.z * ey.g0 + p1.z * ey.g1 + p2.z * ey.g2);
-    acc[9] += nz * (p0.x * ez.g0 + p1.x * ez.g1 + p2.x * ez.g2);
+    // The unnormalised normal: its x, y and z weigh the x, y and z integrals.
+    const NphDVec3 n = dv_cross(dv_sub(p1, p0), dv_sub(p2, p0));
+    f64 tx[6], ty[6], tz[6];
+    eberly_terms(p0.x, p1.x, p2.x, tx);
+    eberly_terms(p0.y, p1.y, p2.y, ty);
+    eberly_terms(p0.z, p1.z, p2.z, tz);
+    m[M_VOL] += n.x * tx[0];
+    m[M_X] += n.x * tx[1];
+    m[M_Y] += n.y * ty[1];
+    m[M_Z] += n.z * tz[1];
+    m[M_XX] += n.x * tx[2];
+    m[M_YY] += n.y * ty[2];
+    m[M_ZZ] += n.z * tz[2];
+    m[M_XY] += n.x * (p0.y * tx[3] + p1.y * tx[4] + p2.y * tx[5]);
+    m[M_YZ] += n.y * (p0.z * ty[3] + p1.z * ty[4] + p2.z * ty[5]);
+    m[M_ZX] += n.z * (p0.x * tz[3] + p1.x * tz[4] + p2.x * tz[5]);
 }
+
+// The paper's closing factors: 1/6, 1/24 (first moments), 1/60 (squares),
+// 1/120 (products).
+static const f64 eberly_scale[M_COUNT] = {1.0 / 6.0,  1.0 / 24.0, 1.0 / 24.0,  1.0 / 24.0,
+                                          1.0 / 60.0, 1.0 / 60.0, 1.0 / 60.0,  1.0 / 120.0,
+                                          1.0 / 120.0, 1.0 / 120.0};
 
 // ---------------------------------------------------------------------------
-// Packing (hull.c:128-309): vertices and counter-clockwise face loops into a
-// block. Face f's loop is loop[offsets[f]] .. loop[offsets[f + 1] - 1].
+// The packer: `nv` vertices and `nf` loops, each counter-clockwise seen from
+// outside (face f is loop[offsets[f]] .. loop[offsets[f + 1] - 1]), into a
+// block. The loops must close into a two-manifold: every directed edge once,
+// its reverse once.
 // ---------------------------------------------------------------------------
-static int key_order(const void *pa, const void *pb)
+
+// The vertex a loop's half-edge h ends at.
+static u32 loop_head(const u32 *loop, const u32 *offsets, u32 f, u32 h)
 {
-    const u64 a = *(const u64 *)pa;
-    const u64 b = *(const u64 *)pb;
-    return a < b ? -1 : (a > b ? 1 : 0);
+    return h + 1U == offsets[f + 1U] ? loop[offsets[f]] : loop[h + 1U];
 }
 
-static NphResult hull_pack(const NphVec3 *vertices, u32 nv, const u32 *loop, const u32 *offsets,
-                           u32 nf, f32 tolerance, NphAllocator *a, NphPhysicsHull **out)
+// mate[h] = the half-edge running the other way along h's edge, found in
+// the list of half-edges leaving h's head (each vertex's outgoing star).
+// False unless every half-edge has exactly one such mate.
+static bool pair_mates(const u32 *loop, const u32 *offsets, u32 nv, u32 nf, u32 *mate,
+                       u32 *star_at, u32 *star, u32 *head)
+{
+    const u32 ne = offsets[nf];
+    for (u32 v = 0U; v <= nv; ++v) {
+        star_at[v] = 0U;
+    }
+    for (u32 f = 0U; f < nf; ++f) {
+        for (u32 h = offsets[f]; h < offsets[f + 1U]; ++h) {
+            head[h] = loop_head(loop, offsets, f, h);
+            if (loop[h] >= nv || head[h] >= nv || loop[h] == head[h]) {
+                return false;
+            }
+            star_at[loop[h] + 1U] += 1U;
+        }
+    }
+    for (u32 v = 0U; v < nv; ++v) {
+        star_at[v + 1U] += star_at[v];
+    }
+    for (u32 h = 0U; h < ne; ++h) {
+        star[star_at[loop[h]]++] = h;
+    }
+    // star_at[v] now ends v's star; the star before it begins at star_at[v - 1].
+    for (u32 h = 0U; h < ne; ++h) {
+        const u32 b = head[h];
+        u32 found = NIL;
+        u32 hits = 0U;
+        for (u32 k = b == 0U ? 0U : star_at[b - 1U]; k < star_at[b]; ++k) {
+            if (head[star[k]] == loop[h]) {
+                found = star[k];
+                hits += 1U;
+            }
+        }
+        if (hits != 1U) {
+            return false;
+        }
+        mate[h] = found;
+    }
+    return true;
+}
+
+// The face planes (Newell's normal, through the loop's vertex mean) and the
+// half-edge records. Coordinates are taken relative to `mid`, the vertex
+// mean, for precision far from the origin.
+static void write_faces(NphPhysicsHull *hull, const Layout *l, const NphVec3 *vertices,
+                        const u32 *loop, const u32 *offsets, const u32 *mate, const u16 *slot,
+                        NphDVec3 mid)
+{
+    NphPlane *planes = AT(hull, l->planes, NphPlane);
+    u16 *leads = AT(hull, l->leads, u16);
+    NphPhysicsHullEdge *edges = AT(hull, l->edges, NphPhysicsHullEdge);
+    for (u32 f = 0U; f < hull->face_count; ++f) {
+        const u32 first = offsets[f];
+        const u32 n = offsets[f + 1U] - first;
+        NphDVec3 newell = dv(0.0, 0.0, 0.0);
+        NphDVec3 mean = dv(0.0, 0.0, 0.0);
+        for (u32 h = first; h < first + n; ++h) {
+            const NphDVec3 p = dv_sub(dv_wide(vertices[loop[h]]), mid);
+            const NphDVec3 q = dv_sub(dv_wide(vertices[loop_head(loop, offsets, f, h)]), mid);
+            newell.x += (p.y - q.y) * (p.z + q.z);
+            newell.y += (p.z - q.z) * (p.x + q.x);
+            newell.z += (p.x - q.x) * (p.y + q.y);
+            mean = dv_add(mean, p);
+        }
+        const NphDVec3 unit = dv_unit(newell);
+        const NphDVec3 at = dv_add(dv_mul(mean, 1.0 / (f64)n), mid);
+        planes[f].normal = dv_thin(unit);
+        planes[f].d = (f32)(-dv_dot(unit, at));
+        leads[f] = slot[first];
+        for (u32 h = first; h < first + n; ++h) {
+            NphPhysicsHullEdge *rec = &edges[slot[h]];
+            rec->next = slot[h + 1U == first + n ? first : h + 1U];
+            rec->twin = slot[mate[h]];
+            rec->origin = (u16)loop[h];
+            rec->face = (u16)f;
+        }
+    }
+}
+
+// Volume, centroid, inertia about the centroid and the two radii, from the
+// fans of the loops (each from its first vertex).
+static bool write_mass(NphPhysicsHull *hull, const Layout *l, const NphVec3 *vertices,
+                       const u32 *loop, const u32 *offsets, NphDVec3 mid)
+{
+    f64 m[M_COUNT];
+    for (u32 i = 0U; i < M_COUNT; ++i) {
+        m[i] = 0.0;
+    }
+    for (u32 f = 0U; f < hull->face_count; ++f) {
+        const NphDVec3 hub = dv_sub(dv_wide(vertices[loop[offsets[f]]]), mid);
+        for (u32 h = offsets[f] + 1U; h + 1U < offsets[f + 1U]; ++h) {
+            integrate_triangle(m, hub, dv_sub(dv_wide(vertices[loop[h]]), mid),
+                               dv_sub(dv_wide(vertices[loop[h + 1U]]), mid));
+        }
+    }
+    for (u32 i = 0U; i < M_COUNT; ++i) {
+        m[i] *= eberly_scale[i];
+    }
+    const f64 vol = m[M_VOL];
+    if (!(vol > 0.0)) {
+        return false;
+    }
+    const NphDVec3 c = dv(m[M_X] / vol, m[M_Y] / vol, m[M_Z] / vol);
+    // Second moments moved to the centroid (parallel axes), products negated.
+    const f64 ixx = m[M_YY] + m[M_ZZ] - vol * (c.y * c.y + c.z * c.z);
+    const f64 iyy = m[M_XX] + m[M_ZZ] - vol * (c.z * c.z + c.x * c.x);
+    const f64 izz = m[M_XX] + m[M_YY] - vol * (c.x * c.x + c.y * c.y);
+    const f64 ixy = -(m[M_XY] - vol * c.x * c.y);
+    const f64 iyz = -(m[M_YZ] - vol * c.y * c.z);
+    const f64 izx = -(m[M_ZX] - vol * c.z * c.x);
+    hull->volume = (f32)vol;
+    hull->centroid = dv_thin(dv_add(c, mid));
+    hull->inertia.c[0] = dv_thin(dv(ixx, ixy, izx));
+    hull->inertia.c[1] = dv_thin(dv(ixy, iyy, iyz));
+    hull->inertia.c[2] = dv_thin(dv(izx, iyz, izz));
+    const NphPlane *planes = AT_C(hull, l->planes, NphPlane);
+    f32 inner = FLT_MAX;
+    for (u32 f = 0U; f < hull->face_count; ++f) {
+        inner = lesser(inner, -above(planes[f], hull->centroid));
+    }
+    f32 outer = 0.0F;
+    for (u32 v = 0U; v < hull->vertex_count; ++v) {
+        outer = greater(outer, fv_len(fv_sub(vertices[v], hull->centroid)));
+    }
+    hull->inner_radius = greater(inner, 0.0F);
+    hull->outer_radius = outer;
+    return true;
+}
+
+static NphResult pack(const NphVec3 *vertices, u32 nv, const u32 *loop, const u32 *offsets, u32 nf,
+                      f32 tolerance, NphAllocator *a, NphPhysicsHull **out)
 {
     const u32 ne = offsets[nf];
     if (nv < 4U || nf < 4U || ne > 0xFFFFU || nv > NPH_PHYSICS_HULL_MAX_VERTICES ||
         (ne & 1U) != 0U) {
         return NPH_RESULT_INVALID_ARGUMENT;
     }
-    // Twins: sort the half-edges by (lower vertex, higher vertex, direction),
-    // so each undirected edge is two neighbours, the one leaving the lower
-    // vertex first; anything else is a loop set that is not a closed
-    // two-manifold.
-    u64 *keys = (u64 *)hull_take(a, (u64)ne * 8U);
-    u32 *twin = (u32 *)hull_take(a, (u64)ne * 4U);
-    u16 *slot = (u16 *)hull_take(a, (u64)ne * 2U);
-    NphResult r = NPH_RESULT_OK;
-    if (keys == NULL || twin == NULL || slot == NULL) {
-        r = NPH_RESULT_OUT_OF_MEMORY;
-    } else {
-        for (u32 f = 0U; f < nf; ++f) {
-            const u32 first = offsets[f];
-            const u32 n = offsets[f + 1U] - first;
-            for (u32 i = 0U; i < n; ++i) {
-                const u32 tail = loop[first + i];
-                const u32 head = loop[first + (i + 1U) % n];
-                const u64 lo = tail < head ? tail : head;
-                const u64 hi = tail < head ? head : tail;
-                keys[first + i] = (lo << 44) | (hi << 24) | ((u64)(tail != lo) << 16) |
-                                  (u64)(first + i);
-            }
-        }
-        qsort(keys, ne, sizeof(u64), key_order);
-        for (u32 k = 0U; k < ne; k += 2U) {
-            const u64 x = keys[k];
-            const u64 y = keys[k + 1U];
-            if ((x >> 24) != (y >> 24) || ((x >> 16) & 1U) != 0U || ((y >> 16) & 1U) != 1U) {
-                r = NPH_RESULT_INVALID_ARGUMENT;
-                break;
-            }
-            twin[x & 0xFFFFU] = (u32)(y & 0xFFFFU);
-            twin[y & 0xFFFFU] = (u32)(x & 0xFFFFU);
-        }
+    // One scratch allocation: mate, star, head (ne each), star_at (nv + 1),
+    // then the output numbering (ne u16).
+    const u64 words = 3ULL * ne + nv + 1U;
+    const u64 scratch_bytes = words * 4U + (u64)ne * 2U;
+    u32 *scratch = (u32 *)grab(a, scratch_bytes);
+    if (scratch == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
     }
-    hull_give(a, keys, (u64)ne * 8U);
-    if (r != NPH_RESULT_OK) {
-        hull_give(a, twin, (u64)ne * 4U);
-        hull_give(a, slot, (u64)ne * 2U);
-        return r;
+    u32 *mate = scratch;
+    u32 *star = mate + ne;
+    u32 *head = star + ne;
+    u32 *star_at = head + ne;
+    u16 *slot = (u16 *)(star_at + nv + 1U);
+    if (!pair_mates(loop, offsets, nv, nf, mate, star_at, star, head)) {
+        drop(a, scratch, scratch_bytes);
+        return NPH_RESULT_INVALID_ARGUMENT;
     }
-    // Output numbering: each edge's first half-edge in face and loop order
-    // gets the next even index, its twin the odd one after it.
+    // The block's numbering: walking the loops in order, each edge's first
+    // half-edge met takes the next even number, its mate the odd one after.
     for (u32 h = 0U; h < ne; ++h) {
         slot[h] = 0xFFFFU;
     }
-    u32 pair = 0U;
+    u32 next = 0U;
     for (u32 h = 0U; h < ne; ++h) {
         if (slot[h] == 0xFFFFU) {
-            slot[h] = (u16)pair;
-            slot[twin[h]] = (u16)(pair + 1U);
-            pair += 2U;
+            slot[h] = (u16)next;
+            slot[mate[h]] = (u16)(next + 1U);
+            next += 2U;
         }
     }
 
-    const u32 v_off = (u32)sizeof(NphPhysicsHull);
-    const u32 p_off = v_off + up16(nv * (u32)sizeof(NphVec3));
-    const u32 f_off = p_off + nf * (u32)sizeof(NphPlane);
-    const u32 e_off = f_off + up16(nf * 2U);
-    const u32 size = e_off + up16(ne * (u32)sizeof(NphPhysicsHullEdge));
-    NphPhysicsHull *hull = (NphPhysicsHull *)hull_take(a, size);
+    const Layout l = layout_of(nv, nf, ne);
+    NphPhysicsHull *hull = (NphPhysicsHull *)grab(a, l.size);
     if (hull == NULL) {
-        hull_give(a, twin, (u64)ne * 4U);
-        hull_give(a, slot, (u64)ne * 2U);
+        drop(a, scratch, scratch_bytes);
         return NPH_RESULT_OUT_OF_MEMORY;
     }
-    memset(hull, 0, size);
-    hull->byte_size = size;
+    memset(hull, 0, l.size);
+    hull->byte_size = l.size;
     hull->vertex_count = nv;
     hull->edge_count = ne;
     hull->face_count = nf;
-    hull->vertices_offset = v_off;
-    hull->planes_offset = p_off;
-    hull->faces_offset = f_off;
-    hull->edges_offset = e_off;
+    hull->vertices_offset = l.verts;
+    hull->planes_offset = l.planes;
+    hull->faces_offset = l.leads;
+    hull->edges_offset = l.edges;
     hull->tolerance = tolerance;
-    NphVec3 *vo = (NphVec3 *)((u8 *)hull + v_off);
-    NphPlane *po = (NphPlane *)((u8 *)hull + p_off);
-    u16 *fo = (u16 *)((u8 *)hull + f_off);
-    NphPhysicsHullEdge *eo = (NphPhysicsHullEdge *)((u8 *)hull + e_off);
+    memcpy(AT(hull, l.verts, NphVec3), vertices, (size_t)nv * sizeof(NphVec3));
 
-    memcpy(vo, vertices, (size_t)nv * sizeof(NphVec3));
     NphAABB box;
     box.min = vertices[0];
     box.max = vertices[0];
-    NphDVec3 mid = d3(0.0, 0.0, 0.0);
+    NphDVec3 mid = dv(0.0, 0.0, 0.0);
     for (u32 v = 0U; v < nv; ++v) {
-        box.min.x = f_min(box.min.x, vertices[v].x);
-        box.min.y = f_min(box.min.y, vertices[v].y);
-        box.min.z = f_min(box.min.z, vertices[v].z);
-        box.max.x = f_max(box.max.x, vertices[v].x);
-        box.max.y = f_max(box.max.y, vertices[v].y);
-        box.max.z = f_max(box.max.z, vertices[v].z);
-        mid = d3_add(mid, d3_of(vertices[v]));
+        const NphVec3 p = vertices[v];
+        box.min.x = lesser(box.min.x, p.x);
+        box.min.y = lesser(box.min.y, p.y);
+        box.min.z = lesser(box.min.z, p.z);
+        box.max.x = greater(box.max.x, p.x);
+        box.max.y = greater(box.max.y, p.y);
+        box.max.z = greater(box.max.z, p.z);
+        mid = dv_add(mid, dv_wide(p));
     }
-    // Planes and integrals are taken relative to the vertices' mean, for
-    // precision far from the origin.
-    mid = d3_scale(mid, 1.0 / (f64)nv);
+    mid = dv_mul(mid, 1.0 / (f64)nv);
     hull->bounds = box;
 
-    f64 acc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (u32 f = 0U; f < nf; ++f) {
-        const u32 first = offsets[f];
-        const u32 n = offsets[f + 1U] - first;
-        // Newell's normal of the loop and the loop's mean.
-        NphDVec3 nrm = d3(0.0, 0.0, 0.0);
-        NphDVec3 mean = d3(0.0, 0.0, 0.0);
-        for (u32 i = 0U; i < n; ++i) {
-            const NphDVec3 c = d3_sub(d3_of(vertices[loop[first + i]]), mid);
-            const NphDVec3 d = d3_sub(d3_of(vertices[loop[first + (i + 1U) % n]]), mid);
-            nrm.x += (c.y - d.y) * (c.z + d.z);
-            nrm.y += (c.z - d.z) * (c.x + d.x);
-            nrm.z += (c.x - d.x) * (c.y + d.y);
-            mean = d3_add(mean, c);
-        }
-        mean = d3_scale(mean, 1.0 / (f64)n);
-        nrm = d3_unit_or_zero(nrm);
-        po[f].normal = d3_narrow(nrm);
-        po[f].d = (f32)(-d3_dot(nrm, d3_add(mean, mid)));
-        fo[f] = slot[first];
-        for (u32 i = 0U; i < n; ++i) {
-            const u32 h = first + i;
-            NphPhysicsHullEdge *e = &eo[slot[h]];
-            e->next = slot[first + (i + 1U) % n];
-            e->twin = slot[twin[h]];
-            e->origin = (u16)loop[h];
-            e->face = (u16)f;
-        }
-        const NphDVec3 p0 = d3_sub(d3_of(vertices[loop[first]]), mid);
-        for (u32 i = 1U; i + 1U < n; ++i) {
-            mass_triangle(acc, p0, d3_sub(d3_of(vertices[loop[first + i]]), mid),
-                          d3_sub(d3_of(vertices[loop[first + i + 1U]]), mid));
-        }
-    }
-    hull_give(a, twin, (u64)ne * 4U);
-    hull_give(a, slot, (u64)ne * 2U);
-
-    // Eberly's final scale factors.
-    static const f64 scale[10] = {1.0 / 6.0,  1.0 / 24.0, 1.0 / 24.0,  1.0 / 24.0,  1.0 / 60.0,
-                                  1.0 / 60.0, 1.0 / 60.0, 1.0 / 120.0, 1.0 / 120.0, 1.0 / 120.0};
-    for (u32 i = 0U; i < 10U; ++i) {
-        acc[i] *= scale[i];
-    }
-    const f64 vol = acc[0];
-    if (!(vol > 0.0)) {
-        hull_give(a, hull, size);
+    write_faces(hull, &l, vertices, loop, offsets, mate, slot, mid);
+    drop(a, scratch, scratch_bytes);
+    if (!write_mass(hull, &l, vertices, loop, offsets, mid)) {
+        drop(a, hull, l.size);
         return NPH_RESULT_INVALID_ARGUMENT;
     }
-    const f64 cx = acc[1] / vol;
-    const f64 cy = acc[2] / vol;
-    const f64 cz = acc[3] / vol;
-    // Moments about the centroid (parallel axis), products negated.
-    const f64 ixx = acc[5] + acc[6] - vol * (cy * cy + cz * cz);
-    const f64 iyy = acc[4] + acc[6] - vol * (cz * cz + cx * cx);
-    const f64 izz = acc[4] + acc[5] - vol * (cx * cx + cy * cy);
-    const f64 ixy = -(acc[7] - vol * cx * cy);
-    const f64 iyz = -(acc[8] - vol * cy * cz);
-    const f64 ixz = -(acc[9] - vol * cz * cx);
-    hull->volume = (f32)vol;
-    hull->centroid = d3_narrow(d3_add(d3(cx, cy, cz), mid));
-    hull->inertia.c[0] = d3_narrow(d3(ixx, ixy, ixz));
-    hull->inertia.c[1] = d3_narrow(d3(ixy, iyy, iyz));
-    hull->inertia.c[2] = d3_narrow(d3(ixz, iyz, izz));
-    f32 inner = FLT_MAX;
-    for (u32 f = 0U; f < nf; ++f) {
-        inner = f_min(inner, -plane_height(po[f], hull->centroid));
-    }
-    f32 outer = 0.0F;
-    for (u32 v = 0U; v < nv; ++v) {
-        outer = f_max(outer, f3_len(f3_sub(vertices[v], hull->centroid)));
-    }
-    hull->inner_radius = f_max(inner, 0.0F);
-    hull->outer_radius = outer;
     *out = hull;
     return NPH_RESULT_OK;
 }
@@ -398,7 +467,7 @@ static NphResult hull_pack(const NphVec3 *vertices, u32 nv, const u32 *loop, con
 void nph_physics_hull_free(NphPhysicsHull *hull, NphAllocator *alloc)
 {
     if (hull != NULL && alloc != NULL) {
-        hull_give(alloc, hull, hull->byte_size);
+        drop(alloc, hull, hull->byte_size);
     }
 }
 
@@ -410,8 +479,8 @@ NphResult nph_physics_hull_make_box(NphVec3 h, NphAllocator *alloc, NphPhysicsHu
         return NPH_RESULT_INVALID_ARGUMENT;
     }
     // Corner k: bit 0 +x, bit 1 +y, bit 2 +z. Faces -x, +x, -y, +y, -z, +z,
-    // each loop counter-clockwise seen from outside (the reference's box
-    // numbering, hull.c:333-341, which the box's contact ids depend on).
+    // each counter-clockwise seen from outside: the reference's box
+    // numbering (hull.c:333-341), which every box contact id depends on.
     NphVec3 corner[8];
     for (u32 k = 0U; k < 8U; ++k) {
         corner[k].x = (k & 1U) != 0U ? h.x : -h.x;
@@ -421,732 +490,712 @@ NphResult nph_physics_hull_make_box(NphVec3 h, NphAllocator *alloc, NphPhysicsHu
     static const u32 loops[24] = {0, 4, 6, 2, 1, 3, 7, 5, 0, 1, 5, 4,
                                   2, 6, 7, 3, 0, 2, 3, 1, 4, 5, 7, 6};
     static const u32 offsets[7] = {0, 4, 8, 12, 16, 20, 24};
+    // The builder's default tolerance for these eight points.
     const f32 tol = 3.0F * FLT_EPSILON * (h.x + h.y + h.z);
-    return hull_pack(corner, 8U, loops, offsets, 6U, tol, alloc, out);
+    return pack(corner, 8U, loops, offsets, 6U, tol, alloc, out);
 }
 
 // ---------------------------------------------------------------------------
-// Quickhull (hull.c:350-1139)
+// Quickhull (Barber, Dobkin and Huhdanpaa 1996) over a half-edge surface in
+// the manner of John Lloyd's QuickHull3D: each half-edge records the vertex
+// it ENDS at, so a vertex that merging makes redundant leaves by relinking.
 // ---------------------------------------------------------------------------
+typedef struct Arc {
+    u32 to;    // the vertex it ends at; it starts where `back` ends
+    u32 fwd;   // the next half-edge round its face
+    u32 back;  // the previous one
+    u32 mate;  // the half-edge the other way along the same edge
+    u32 owner; // its face
+} Arc;
 
-// A half-edge of the working mesh. It stores the vertex it ENDS at; its start
-// is the previous half-edge's end, so dropping a vertex while merging is a
-// relinking only.
-typedef struct QEdge {
-    u32 head;
-    u32 next;
-    u32 prev;
-    u32 twin;
-    u32 face;
-} QEdge;
+enum { LIVE = 0, CONCAVE = 1, DEAD = 2 };
 
-enum { QF_LIVE = 0, QF_NONCONVEX = 1, QF_GONE = 2 };
+typedef struct Poly {
+    NphDVec3 n;   // unit normal
+    NphDVec3 mid; // the mean of its vertices
+    f64 lift;     // n . mid: a point p is n . p - lift above the face
+    f64 size;     // the length of its fan's cross-product sum (twice the area)
+    f64 peak;     // how far `apex` is above it
+    u32 apex;     // its highest outside point, NIL when it has none
+    u32 lead;     // the half-edge its loop starts from
+    u32 degree;   // vertices on its loop
+    u32 crowd;    // its first outside point (the rest through Build.after)
+    u32 fate;     // LIVE, CONCAVE (first merge pass) or DEAD
+} Poly;
 
-typedef struct QFace {
-    NphDVec3 normal;
-    NphDVec3 centroid;
-    f64 offset; // height(p) = normal . p - offset
-    f64 area;
-    f64 far_dist;
-    u32 far_point; // the outside point farthest above the face (HNONE: none)
-    u32 edge;      // a half-edge of the face
-    u32 nverts;
-    u32 outside; // the face's outside set, linked through Qh.link
-    u32 state;
-    u32 pad;
-} QFace;
-
-typedef struct QFrame {
-    u32 face;
-    u32 stop; // the frame ends when its cursor comes back here
-    u32 at;
-} QFrame;
-
-typedef struct Qh {
+typedef struct Build {
     NphAllocator *a;
-    NphDVec3 *pts;
-    u32 n;
-    u32 *link; // per point: the next point of the same outside set
-    NphVecBuf edges, faces, horizon, fresh, orphans, frames;
-    f64 tol;
-    int oom;
-} Qh;
+    NphDVec3 *pt;
+    u32 npt;
+    u32 *after; // per point: the next point of the same outside set
+    NphVecBuf arcs, polys;
+    NphVecBuf rim;   // horizon half-edges (on the dying side), in walk order
+    NphVecBuf cone;  // the faces made for the current point
+    NphVecBuf loose; // outside points whose faces died
+    f64 eps;
+    bool broke; // an allocation failed
+} Build;
 
-#define QE(q) VB((q)->edges, QEdge)
-#define QF(q) VB((q)->faces, QFace)
+#define ARC(b) VB((b)->arcs, Arc)
+#define POLY(b) VB((b)->polys, Poly)
+#define U32S(v) VB(v, u32)
 
-static f64 height(const QFace *f, NphDVec3 p)
+static void note(Build *b, NphVecBuf *list, u32 value)
 {
-    return d3_dot(f->normal, p) - f->offset;
+    if (vbuf_push(b->a, list, 4U, &value) != NPH_RESULT_OK) {
+        b->broke = true;
+    }
 }
 
-static u32 tail_of(const Qh *q, u32 e)
+static f64 height(const Poly *f, NphDVec3 p)
 {
-    return QE(q)[QE(q)[e].prev].head;
+    return dv_dot(f->n, p) - f->lift;
 }
 
-// The face's plane from its loop: the fan of cross products from the loop's
-// first vertex, the mean of its vertices, its area (hull.c:414-465). A face
-// of almost no area has an unreliable normal; its component along the
-// face's longest edge is removed.
-static void face_refit(Qh *q, u32 fi)
+static u32 start_of(const Build *b, u32 e)
 {
-    const QEdge *E = QE(q);
-    QFace *f = &QF(q)[fi];
-    const u32 first = f->edge;
-    const NphDVec3 origin = q->pts[E[first].head];
-    u32 e = E[first].next;
-    NphDVec3 last = d3_sub(q->pts[E[e].head], origin);
-    NphDVec3 sum = d3_add(origin, q->pts[E[e].head]);
-    NphDVec3 nrm = d3(0.0, 0.0, 0.0);
+    return ARC(b)[ARC(b)[e].back].to;
+}
+
+static u32 across(const Build *b, u32 e)
+{
+    return ARC(b)[ARC(b)[e].mate].owner;
+}
+
+// The face's plane from its loop (QuickHull3D's computeNormal): normal from
+// the fan of cross products about the lead's end vertex, the vertex mean,
+// and the size. A face of almost no area has an unreliable normal, so its
+// component along the longest edge is taken out.
+static void refit(Build *b, u32 fi)
+{
+    const Arc *A = ARC(b);
+    Poly *f = &POLY(b)[fi];
+    const u32 lead = f->lead;
+    const NphDVec3 hub = b->pt[A[lead].to];
+    u32 e = A[lead].fwd;
+    NphDVec3 spoke = dv_sub(b->pt[A[e].to], hub);
+    NphDVec3 sum = dv_add(hub, b->pt[A[e].to]);
+    NphDVec3 n = dv(0.0, 0.0, 0.0);
     u32 count = 2U;
-    for (e = E[e].next; e != first; e = E[e].next) {
-        const NphDVec3 p = q->pts[E[e].head];
-        const NphDVec3 d = d3_sub(p, origin);
-        nrm = d3_add(nrm, d3_cross(last, d));
-        sum = d3_add(sum, p);
-        last = d;
+    for (e = A[e].fwd; e != lead; e = A[e].fwd) {
+        const NphDVec3 p = b->pt[A[e].to];
+        const NphDVec3 next = dv_sub(p, hub);
+        n = dv_add(n, dv_cross(spoke, next));
+        sum = dv_add(sum, p);
+        spoke = next;
         count += 1U;
     }
-    f->area = sqrt(d3_dot(nrm, nrm));
-    f->nverts = count;
-    f->centroid = d3_scale(sum, 1.0 / (f64)count);
-    if (f->area > 0.0) {
-        nrm = d3_scale(nrm, 1.0 / f->area);
+    f->size = sqrt(dv_dot(n, n));
+    f->degree = count;
+    f->mid = dv_mul(sum, 1.0 / (f64)count);
+    if (f->size > 0.0) {
+        n = dv_mul(n, 1.0 / f->size);
     }
-    if (f->area < q->tol * q->tol * 16.0 || !(f->area > 0.0)) {
-        u32 longest = first;
+    if (!(f->size > 0.0) || f->size < b->eps * b->eps * 16.0) {
+        u32 longest = lead;
         f64 best = 0.0;
-        e = first;
+        e = lead;
         do {
-            const NphDVec3 s = d3_sub(q->pts[E[e].head], q->pts[tail_of(q, e)]);
-            const f64 len = d3_dot(s, s);
-            if (len > best) {
-                best = len;
+            const NphDVec3 s = dv_sub(b->pt[A[e].to], b->pt[start_of(b, e)]);
+            const f64 sq = dv_dot(s, s);
+            if (sq > best) {
+                best = sq;
                 longest = e;
             }
-            e = E[e].next;
-        } while (e != first);
+            e = A[e].fwd;
+        } while (e != lead);
         if (best > 0.0) {
-            const NphDVec3 s = d3_sub(q->pts[E[longest].head], q->pts[tail_of(q, longest)]);
-            const NphDVec3 u = d3_scale(s, 1.0 / sqrt(best));
-            nrm = d3_unit_or_zero(d3_sub(nrm, d3_scale(u, d3_dot(nrm, u))));
+            const NphDVec3 s = dv_sub(b->pt[A[longest].to], b->pt[start_of(b, longest)]);
+            const NphDVec3 u = dv_mul(s, 1.0 / sqrt(best));
+            n = dv_unit(dv_sub(n, dv_mul(u, dv_dot(n, u))));
         }
     }
-    f->normal = nrm;
-    f->offset = d3_dot(nrm, f->centroid);
+    f->n = n;
+    f->lift = dv_dot(n, f->mid);
 }
 
-// A triangle whose loop ends its half-edges at v0, v1, v2 in turn; its
-// first half-edge ends at v0.
-static u32 add_tri(Qh *q, u32 v0, u32 v1, u32 v2)
+// A new triangle whose half-edges end at v0, v1 and v2 in turn; its lead is
+// the one ending at v0. NIL when out of memory.
+static u32 triangle(Build *b, u32 v0, u32 v1, u32 v2)
 {
-    const u32 e = q->edges.len;
-    const u32 fi = q->faces.len;
-    if (vbuf_reserve(q->a, &q->edges, (u32)sizeof(QEdge), e + 3U) != NPH_RESULT_OK ||
-        vbuf_reserve(q->a, &q->faces, (u32)sizeof(QFace), fi + 1U) != NPH_RESULT_OK) {
-        q->oom = 1;
-        return HNONE;
+    const u32 e = b->arcs.len;
+    const u32 fi = b->polys.len;
+    if (vbuf_reserve(b->a, &b->arcs, (u32)sizeof(Arc), e + 3U) != NPH_RESULT_OK ||
+        vbuf_reserve(b->a, &b->polys, (u32)sizeof(Poly), fi + 1U) != NPH_RESULT_OK) {
+        b->broke = true;
+        return NIL;
     }
-    const u32 heads[3] = {v0, v1, v2};
-    QEdge *E = QE(q);
-    for (u32 i = 0U; i < 3U; ++i) {
-        E[e + i].head = heads[i];
-        E[e + i].next = e + (i + 1U) % 3U;
-        E[e + i].prev = e + (i + 2U) % 3U;
-        E[e + i].twin = HNONE;
-        E[e + i].face = fi;
+    Arc *A = ARC(b);
+    const u32 ends[3] = {v0, v1, v2};
+    for (u32 k = 0U; k < 3U; ++k) {
+        A[e + k].to = ends[k];
+        A[e + k].fwd = e + (k == 2U ? 0U : k + 1U);
+        A[e + k].back = e + (k == 0U ? 2U : k - 1U);
+        A[e + k].mate = NIL;
+        A[e + k].owner = fi;
     }
-    q->edges.len = e + 3U;
-    QFace *f = &QF(q)[fi];
+    b->arcs.len = e + 3U;
+    Poly *f = &POLY(b)[fi];
     memset(f, 0, sizeof *f);
-    f->edge = e;
-    f->outside = HNONE;
-    f->far_point = HNONE;
-    f->state = QF_LIVE;
-    q->faces.len = fi + 1U;
-    face_refit(q, fi);
+    f->lead = e;
+    f->crowd = NIL;
+    f->apex = NIL;
+    f->fate = LIVE;
+    b->polys.len = fi + 1U;
+    refit(b, fi);
     return fi;
 }
 
-// Outside sets: singly linked, newest first; the far point is the first of
-// the largest heights in list order.
-static void outside_push(Qh *q, u32 fi, u32 p, f64 d)
+static void mate(Build *b, u32 x, u32 y)
 {
-    QFace *f = &QF(q)[fi];
-    q->link[p] = f->outside;
-    f->outside = p;
-    if (f->far_point == HNONE || d > f->far_dist) {
-        f->far_point = p;
-        f->far_dist = d;
+    ARC(b)[x].mate = y;
+    ARC(b)[y].mate = x;
+}
+
+// ----- outside sets: last in, first out -----
+
+static void enlist(Build *b, u32 fi, u32 p, f64 h)
+{
+    Poly *f = &POLY(b)[fi];
+    b->after[p] = f->crowd;
+    f->crowd = p;
+    if (f->apex == NIL || h > f->peak) {
+        f->apex = p;
+        f->peak = h;
     }
 }
 
-static void outside_rescan(Qh *q, u32 fi)
+// The highest point of the set again, after the plane or the set changed;
+// the first of equals in list order.
+static void rescan(Build *b, u32 fi)
 {
-    QFace *f = &QF(q)[fi];
-    f->far_point = HNONE;
-    f->far_dist = 0.0;
-    for (u32 p = f->outside; p != HNONE; p = q->link[p]) {
-        const f64 d = height(f, q->pts[p]);
-        if (f->far_point == HNONE || d > f->far_dist) {
-            f->far_point = p;
-            f->far_dist = d;
+    Poly *f = &POLY(b)[fi];
+    f->apex = NIL;
+    f->peak = 0.0;
+    for (u32 p = f->crowd; p != NIL; p = b->after[p]) {
+        const f64 h = height(f, b->pt[p]);
+        if (f->apex == NIL || h > f->peak) {
+            f->apex = p;
+            f->peak = h;
         }
     }
 }
 
-static void outside_remove(Qh *q, u32 fi, u32 p)
+static void unlist(Build *b, u32 fi, u32 p)
 {
-    u32 *at = &QF(q)[fi].outside;
-    while (*at != HNONE && *at != p) {
-        at = &q->link[*at];
+    u32 *at = &POLY(b)[fi].crowd;
+    while (*at != NIL && *at != p) {
+        at = &b->after[*at];
     }
     if (*at == p) {
-        *at = q->link[p];
+        *at = b->after[p];
     }
-    outside_rescan(q, fi);
 }
 
-// Empties a face's outside set, in list order: into face `into` for the
-// points above it by more than the tolerance, onto the orphans otherwise.
-static void outside_release(Qh *q, u32 fi, u32 into)
+// Empties a dying face's set: each point moves to `heir` when it is outside
+// it, otherwise it is loose until the new faces are done.
+static void scatter(Build *b, u32 fi, u32 heir)
 {
-    u32 p = QF(q)[fi].outside;
-    QF(q)[fi].outside = HNONE;
-    QF(q)[fi].far_point = HNONE;
-    while (p != HNONE) {
-        const u32 after = q->link[p];
-        bool kept = false;
-        if (into != HNONE) {
-            const f64 d = height(&QF(q)[into], q->pts[p]);
-            if (d > q->tol) {
-                outside_push(q, into, p, d);
-                kept = true;
+    u32 p = POLY(b)[fi].crowd;
+    POLY(b)[fi].crowd = NIL;
+    POLY(b)[fi].apex = NIL;
+    while (p != NIL) {
+        const u32 later = b->after[p];
+        const f64 h = heir == NIL ? 0.0 : height(&POLY(b)[heir], b->pt[p]);
+        if (heir != NIL && h > b->eps) {
+            enlist(b, heir, p, h);
+        } else {
+            note(b, &b->loose, p);
+        }
+        p = later;
+    }
+}
+
+// ----- the visible region and the cone -----
+
+// Kills face fi, seen from `eye`, and every face beyond it that also sees
+// the eye, depth first; the half-edges where a dying face meets a living one
+// go to the rim in the order met. `entry` is fi's half-edge the walk came in
+// through (NIL for the first face): the walk round fi starts after it.
+static void dig(Build *b, NphDVec3 eye, u32 fi, u32 entry)
+{
+    scatter(b, fi, NIL);
+    POLY(b)[fi].fate = DEAD;
+    const u32 stop = entry == NIL ? POLY(b)[fi].lead : entry;
+    u32 e = entry == NIL ? stop : ARC(b)[entry].fwd;
+    do {
+        const u32 other = ARC(b)[e].mate;
+        const u32 g = ARC(b)[other].owner;
+        if (POLY(b)[g].fate != DEAD) {
+            if (height(&POLY(b)[g], eye) > b->eps) {
+                dig(b, eye, g, other);
+            } else {
+                note(b, &b->rim, e);
             }
         }
-        if (!kept && vbuf_push(q->a, &q->orphans, 4U, &p) != NPH_RESULT_OK) {
-            q->oom = 1;
+        e = ARC(b)[e].fwd;
+    } while (e != stop && !b->broke);
+}
+
+// A triangle from each rim half-edge to the eye, stitched to its living
+// neighbour and to the triangles either side.
+static void fan(Build *b, u32 eye)
+{
+    b->cone.len = 0U;
+    u32 first_side = NIL;
+    u32 last_side = NIL;
+    for (u32 k = 0U; k < b->rim.len; ++k) {
+        const u32 r = U32S(b->rim)[k];
+        const u32 fi = triangle(b, eye, start_of(b, r), ARC(b)[r].to);
+        if (fi == NIL) {
+            return;
         }
-        p = after;
+        // Its half-edges: ends at the eye (lead), at the rim's start, at the
+        // rim's end (along the rim).
+        const u32 to_eye = POLY(b)[fi].lead;
+        const u32 from_eye = ARC(b)[to_eye].fwd;
+        mate(b, ARC(b)[from_eye].fwd, ARC(b)[r].mate);
+        if (last_side == NIL) {
+            first_side = to_eye;
+        } else {
+            mate(b, from_eye, last_side);
+        }
+        last_side = to_eye;
+        note(b, &b->cone, fi);
+        if (b->broke) {
+            return;
+        }
+    }
+    if (first_side != NIL) {
+        mate(b, ARC(b)[first_side].fwd, last_side);
     }
 }
 
-static u32 across(const Qh *q, u32 e)
+// ----- merging -----
+
+// How far the centroid of the face across e is above e's own face.
+static f64 ridge(const Build *b, u32 e)
 {
-    return QE(q)[QE(q)[e].twin].face;
+    const Arc *A = ARC(b);
+    const Poly *P = POLY(b);
+    return height(&P[A[e].owner], P[A[A[e].mate].owner].mid);
 }
 
-// The height of the centroid of the face across e above e's own face.
-static f64 across_height(const Qh *q, u32 e)
+// Inside face f, makes `after` follow `before` (QuickHull3D's
+// connectHalfEdges). When both border the same other face g, the vertex
+// between them is redundant: `after` takes over `before`'s start, and g's
+// two half-edges there become one, or g disappears if it was a triangle.
+// Returns the face that disappeared, or NIL.
+static u32 splice(Build *b, u32 f, u32 before, u32 after)
 {
-    return height(&QF(q)[QE(q)[e].face], QF(q)[across(q, e)].centroid);
-}
-
-// Rejoins face fi's loop between `before` and `at` after a merge
-// (hull.c:581-621). When both border the same face, the vertex between them
-// is redundant: the two half-edges become one, and a triangle across them
-// vanishes (returned; HNONE otherwise).
-static u32 rejoin(Qh *q, u32 fi, u32 before, u32 at)
-{
-    QEdge *E = QE(q);
-    QFace *F = QF(q);
-    if (across(q, before) != across(q, at)) {
-        E[before].next = at;
-        E[at].prev = before;
-        return HNONE;
+    Arc *A = ARC(b);
+    Poly *P = POLY(b);
+    const u32 g = A[A[after].mate].owner;
+    if (A[A[before].mate].owner != g) {
+        A[before].fwd = after;
+        A[after].back = before;
+        return NIL;
     }
-    const u32 other = across(q, at);
-    u32 gone = HNONE;
-    u32 mate;
-    if (before == F[fi].edge) {
-        F[fi].edge = at;
+    if (P[f].lead == before) {
+        P[f].lead = after;
     }
-    if (F[other].nverts == 3U) {
-        mate = E[E[E[at].twin].prev].twin;
-        F[other].state = QF_GONE;
-        gone = other;
+    u32 keep;
+    u32 gone = NIL;
+    if (P[g].degree == 3U) {
+        keep = A[A[A[after].mate].back].mate;
+        P[g].fate = DEAD;
+        gone = g;
     } else {
-        mate = E[E[at].twin].next;
-        if (F[other].edge == E[mate].prev) {
-            F[other].edge = mate;
+        keep = A[A[after].mate].fwd;
+        if (P[g].lead == A[keep].back) {
+            P[g].lead = keep;
         }
-        E[mate].prev = E[E[mate].prev].prev;
-        E[E[mate].prev].next = mate;
+        A[keep].back = A[A[keep].back].back;
+        A[A[keep].back].fwd = keep;
     }
-    E[at].prev = E[before].prev;
-    E[E[at].prev].next = at;
-    E[at].twin = mate;
-    E[mate].twin = at;
-    if (F[other].state != QF_GONE) {
-        face_refit(q, other);
-        outside_rescan(q, other);
+    A[after].back = A[before].back;
+    A[A[after].back].fwd = after;
+    mate(b, after, keep);
+    if (gone == NIL) {
+        refit(b, g);
+        rescan(b, g);
     }
     return gone;
 }
 
-// Face fi takes in the face across its half-edge e, with every further
-// half-edge the two share (hull.c:623-664). The faces that vanish go to
-// `gone`; returns their count (1 to 3).
-static u32 absorb(Qh *q, u32 fi, u32 e, u32 gone[3])
+// Face f takes in the face g across its half-edge e, with every edge the two
+// share in a run through e (QuickHull3D's mergeAdjacentFace). Writes the
+// faces that disappeared to `gone` (g first) and returns how many.
+static u32 swallow(Build *b, u32 f, u32 e, u32 gone[3])
 {
-    QEdge *E = QE(q);
-    QFace *F = QF(q);
-    const u32 other = across(q, e);
+    Arc *A = ARC(b);
+    const u32 g = across(b, e);
     u32 n = 0U;
-    gone[n++] = other;
-    F[other].state = QF_GONE;
-    const u32 mate = E[e].twin;
-    u32 own_prev = E[e].prev;
-    u32 own_next = E[e].next;
-    u32 mate_prev = E[mate].prev;
-    u32 mate_next = E[mate].next;
-    while (across(q, own_prev) == other) {
-        own_prev = E[own_prev].prev;
-        mate_next = E[mate_next].next;
+    gone[n++] = g;
+    POLY(b)[g].fate = DEAD;
+    const u32 m = A[e].mate;
+    u32 f_back = A[e].back;
+    u32 f_fwd = A[e].fwd;
+    u32 g_back = A[m].back;
+    u32 g_fwd = A[m].fwd;
+    while (across(b, f_back) == g) {
+        f_back = A[f_back].back;
+        g_fwd = A[g_fwd].fwd;
     }
-    while (across(q, own_next) == other) {
-        mate_prev = E[mate_prev].prev;
-        own_next = E[own_next].next;
+    while (across(b, f_fwd) == g) {
+        g_back = A[g_back].back;
+        f_fwd = A[f_fwd].fwd;
     }
-    for (u32 h = mate_next; h != E[mate_prev].next; h = E[h].next) {
-        E[h].face = fi;
+    for (u32 h = g_fwd; h != A[g_back].fwd; h = A[h].fwd) {
+        A[h].owner = f;
     }
-    if (e == F[fi].edge) {
-        F[fi].edge = own_next;
+    if (POLY(b)[f].lead == e) {
+        POLY(b)[f].lead = f_fwd;
     }
-    u32 g = rejoin(q, fi, mate_prev, own_next);
-    if (g != HNONE) {
-        gone[n++] = g;
+    u32 d = splice(b, f, g_back, f_fwd);
+    if (d != NIL) {
+        gone[n++] = d;
     }
-    g = rejoin(q, fi, own_prev, mate_next);
-    if (g != HNONE) {
-        gone[n++] = g;
+    d = splice(b, f, f_back, g_fwd);
+    if (d != NIL) {
+        gone[n++] = d;
     }
-    face_refit(q, fi);
-    outside_rescan(q, fi);
+    refit(b, f);
+    rescan(b, f);
     return n;
 }
 
-enum { MERGE_WRT_LARGER = 0, MERGE_WRT_EITHER = 1 };
-
-// One merge of a non-convex neighbour into fi, if any (hull.c:671-711);
-// true when it merged, so the caller looks again. A face left with a
-// neighbour non-convex only with respect to the neighbour is marked for the
-// second pass.
-static bool merge_once(Qh *q, u32 fi, int rule)
+// One merge across the first edge of f, from its lead, that calls for it;
+// true when one was made (f's loop changed: look again). In the first pass
+// the larger face of each pair decides: the smaller's centroid not clearly
+// below it means merge; otherwise the larger's centroid not clearly below
+// the smaller marks f concave for the second pass, where a centroid not
+// clearly below either face means merge (QuickHull3D's doAdjacentMerge).
+static bool merge_step(Build *b, u32 f, bool second_pass)
 {
-    const f64 lim = -q->tol;
-    u32 e = QF(q)[fi].edge;
-    bool convex = true;
+    const f64 floor = -b->eps;
+    const u32 lead = POLY(b)[f].lead;
+    bool concave = false;
+    u32 e = lead;
     do {
-        const u32 other = across(q, e);
-        const u32 t = QE(q)[e].twin;
-        bool merge = false;
-        if (rule == MERGE_WRT_EITHER) {
-            merge = across_height(q, e) > lim || across_height(q, t) > lim;
-        } else if (QF(q)[fi].area > QF(q)[other].area) {
-            if (across_height(q, e) > lim) {
-                merge = true;
-            } else if (across_height(q, t) > lim) {
-                convex = false;
-            }
-        } else {
-            if (across_height(q, t) > lim) {
-                merge = true;
-            } else if (across_height(q, e) > lim) {
-                convex = false;
-            }
+        f64 first = ridge(b, e);
+        f64 other = ridge(b, ARC(b)[e].mate);
+        if (!second_pass && !(POLY(b)[f].size > POLY(b)[across(b, e)].size)) {
+            const f64 t = first;
+            first = other;
+            other = t;
         }
+        const bool merge = second_pass ? (first > floor || other > floor) : first > floor;
         if (merge) {
             u32 gone[3];
-            const u32 n = absorb(q, fi, e, gone);
-            for (u32 i = 0U; i < n; ++i) {
-                outside_release(q, gone[i], fi);
+            const u32 n = swallow(b, f, e, gone);
+            for (u32 k = 0U; k < n; ++k) {
+                scatter(b, gone[k], f);
             }
             return true;
         }
-        e = QE(q)[e].next;
-    } while (e != QF(q)[fi].edge);
-    if (!convex) {
-        QF(q)[fi].state = QF_NONCONVEX;
+        concave = concave || other > floor;
+        e = ARC(b)[e].fwd;
+    } while (e != lead);
+    if (concave) {
+        POLY(b)[f].fate = CONCAVE;
     }
     return false;
 }
 
-// Removes every face that sees `eye`, depth first from `start`, and lists
-// the horizon: the half-edges of removed faces whose twins stay, in order
-// around the eye (hull.c:717-752). An explicit stack of frames stands in for
-// the recursion; a frame walks its face's loop from the half-edge after the
-// one it was entered by (the start face: from its first half-edge).
-static void carve(Qh *q, NphDVec3 eye, u32 start)
+// Each loose point to the living new face it is highest above, if any.
+static void rehome(Build *b)
 {
-    q->horizon.len = 0U;
-    q->frames.len = 0U;
-    outside_release(q, start, HNONE);
-    QF(q)[start].state = QF_GONE;
-    const QFrame root = {start, QF(q)[start].edge, QF(q)[start].edge};
-    if (vbuf_push(q->a, &q->frames, (u32)sizeof(QFrame), &root) != NPH_RESULT_OK) {
-        q->oom = 1;
+    for (u32 k = 0U; k < b->loose.len; ++k) {
+        const u32 p = U32S(b->loose)[k];
+        f64 best = b->eps;
+        u32 to = NIL;
+        for (u32 c = 0U; c < b->cone.len; ++c) {
+            const u32 fi = U32S(b->cone)[c];
+            if (POLY(b)[fi].fate == LIVE) {
+                const f64 h = height(&POLY(b)[fi], b->pt[p]);
+                if (h > best) {
+                    best = h;
+                    to = fi;
+                }
+            }
+        }
+        if (to != NIL) {
+            enlist(b, to, p, best);
+        }
+    }
+    b->loose.len = 0U;
+}
+
+// Adds point `eye`, the apex of face fi, to the surface.
+static void expand(Build *b, u32 eye, u32 fi)
+{
+    unlist(b, fi, eye);
+    b->loose.len = 0U;
+    b->rim.len = 0U;
+    dig(b, b->pt[eye], fi, NIL);
+    if (b->broke) {
         return;
     }
-    while (q->frames.len > 0U && !q->oom) {
-        QFrame *top = &VB(q->frames, QFrame)[q->frames.len - 1U];
-        const u32 e = top->at;
-        top->at = QE(q)[e].next;
-        if (top->at == top->stop) {
-            q->frames.len -= 1U;
-        }
-        const u32 t = QE(q)[e].twin;
-        const u32 nb = QE(q)[t].face;
-        if (QF(q)[nb].state == QF_GONE) {
-            continue;
-        }
-        if (height(&QF(q)[nb], eye) > q->tol) {
-            outside_release(q, nb, HNONE);
-            QF(q)[nb].state = QF_GONE;
-            const QFrame child = {nb, t, QE(q)[t].next};
-            if (vbuf_push(q->a, &q->frames, (u32)sizeof(QFrame), &child) != NPH_RESULT_OK) {
-                q->oom = 1;
-            }
-        } else if (vbuf_push(q->a, &q->horizon, 4U, &e) != NPH_RESULT_OK) {
-            q->oom = 1;
-        }
-    }
-}
-
-// A triangle from each horizon half-edge to the eye, twinned to the face
-// beyond the horizon and to its neighbours in the cone (hull.c:756-804).
-static void cone(Qh *q, u32 eye)
-{
-    q->fresh.len = 0U;
-    u32 side_prev = HNONE;
-    u32 side_first = HNONE;
-    for (u32 i = 0U; i < q->horizon.len && !q->oom; ++i) {
-        const u32 h = VB(q->horizon, u32)[i];
-        const u32 tail = tail_of(q, h);
-        const u32 head = QE(q)[h].head;
-        const u32 fi = add_tri(q, eye, tail, head);
-        if (fi == HNONE) {
-            return;
-        }
-        QEdge *E = QE(q);
-        const u32 to_eye = QF(q)[fi].edge;   // head -> eye
-        const u32 from_eye = E[to_eye].next; // eye -> tail
-        const u32 base = E[from_eye].next;   // tail -> head, on the horizon
-        const u32 beyond = E[h].twin;
-        E[base].twin = beyond;
-        E[beyond].twin = base;
-        if (side_prev != HNONE) {
-            E[from_eye].twin = side_prev;
-            E[side_prev].twin = from_eye;
-        } else {
-            side_first = to_eye;
-        }
-        if (vbuf_push(q->a, &q->fresh, 4U, &fi) != NPH_RESULT_OK) {
-            q->oom = 1;
-            return;
-        }
-        side_prev = to_eye;
-    }
-    if (side_first != HNONE) {
-        QEdge *E = QE(q);
-        const u32 from_eye = E[side_first].next;
-        E[from_eye].twin = side_prev;
-        E[side_prev].twin = from_eye;
-    }
-}
-
-// Each orphan goes to the live new face it is highest above, beyond the
-// tolerance; the first such face on ties (hull.c:806-829).
-static void adopt_orphans(Qh *q)
-{
-    for (u32 i = 0U; i < q->orphans.len; ++i) {
-        const u32 p = VB(q->orphans, u32)[i];
-        f64 best = q->tol;
-        u32 best_face = HNONE;
-        for (u32 k = 0U; k < q->fresh.len; ++k) {
-            const u32 fi = VB(q->fresh, u32)[k];
-            if (QF(q)[fi].state != QF_LIVE) {
-                continue;
-            }
-            const f64 d = height(&QF(q)[fi], q->pts[p]);
-            if (d > best) {
-                best = d;
-                best_face = fi;
-            }
-        }
-        if (best_face != HNONE) {
-            outside_push(q, best_face, p, best);
-        }
-    }
-    q->orphans.len = 0U;
-}
-
-static void insert_point(Qh *q, u32 eye, u32 eye_face)
-{
-    outside_remove(q, eye_face, eye);
-    q->orphans.len = 0U;
-    carve(q, q->pts[eye], eye_face);
-    if (q->oom) {
+    fan(b, eye);
+    if (b->broke) {
         return;
     }
-    cone(q, eye);
-    if (q->oom) {
-        return;
-    }
-    for (u32 k = 0U; k < q->fresh.len; ++k) {
-        const u32 fi = VB(q->fresh, u32)[k];
-        if (QF(q)[fi].state == QF_LIVE) {
-            while (merge_once(q, fi, MERGE_WRT_LARGER)) {
+    for (u32 c = 0U; c < b->cone.len; ++c) {
+        const u32 f = U32S(b->cone)[c];
+        if (POLY(b)[f].fate == LIVE) {
+            while (merge_step(b, f, false)) {
             }
         }
     }
-    for (u32 k = 0U; k < q->fresh.len; ++k) {
-        const u32 fi = VB(q->fresh, u32)[k];
-        if (QF(q)[fi].state == QF_NONCONVEX) {
-            QF(q)[fi].state = QF_LIVE;
-            while (merge_once(q, fi, MERGE_WRT_EITHER)) {
+    for (u32 c = 0U; c < b->cone.len; ++c) {
+        const u32 f = U32S(b->cone)[c];
+        if (POLY(b)[f].fate == CONCAVE) {
+            POLY(b)[f].fate = LIVE;
+            while (merge_step(b, f, true)) {
             }
         }
     }
-    adopt_orphans(q);
+    rehome(b);
 }
 
-// The first tetrahedron (hull.c:856-962): the two extreme points along the
-// axis of largest spread, the point farthest from their line, the point
-// farthest from the plane of the three; then every other point to the face
-// it is highest above. False for a cloud without volume.
-static bool seed_tetrahedron(Qh *q)
+// The first index of the most extreme point along `axis`, toward +1 or -1.
+static u32 extreme(const Build *b, u32 axis, f64 sign)
 {
-    const NphDVec3 *P = q->pts;
-    u32 lo[3] = {0U, 0U, 0U};
-    u32 hi[3] = {0U, 0U, 0U};
-    for (u32 i = 1U; i < q->n; ++i) {
-        for (u32 ax = 0U; ax < 3U; ++ax) {
-            const f64 c = d3_comp(P[i], ax);
-            if (c < d3_comp(P[lo[ax]], ax)) {
-                lo[ax] = i;
-            }
-            if (c > d3_comp(P[hi[ax]], ax)) {
-                hi[ax] = i;
-            }
+    u32 best = 0U;
+    for (u32 i = 1U; i < b->npt; ++i) {
+        if (sign * dv_axis(b->pt[i], axis) > sign * dv_axis(b->pt[best], axis)) {
+            best = i;
         }
     }
-    f64 span = 0.0;
+    return best;
+}
+
+// The seed tetrahedron (QuickHull3D's createInitialSimplex): the extreme
+// pair of the widest axis, the point farthest from their line, the point
+// farthest from the plane of those three; every other point to the seed
+// face it is highest above. 0 on success, else which test failed: 1 the
+// points coincide, 2 they lie on a line, 3 in a plane.
+static int seed(Build *b)
+{
+    const NphDVec3 *pt = b->pt;
+    f64 spread = 0.0;
     u32 axis = 0U;
-    for (u32 ax = 0U; ax < 3U; ++ax) {
-        const f64 s = d3_comp(P[hi[ax]], ax) - d3_comp(P[lo[ax]], ax);
-        if (s > span) {
-            span = s;
-            axis = ax;
+    u32 hi[3], lo[3];
+    for (u32 k = 0U; k < 3U; ++k) {
+        hi[k] = extreme(b, k, 1.0);
+        lo[k] = extreme(b, k, -1.0);
+        const f64 s = dv_axis(pt[hi[k]], k) - dv_axis(pt[lo[k]], k);
+        if (s > spread) {
+            spread = s;
+            axis = k;
         }
     }
-    if (span <= q->tol) {
-        return false;
+    if (spread <= b->eps) {
+        return 1;
     }
-    u32 v[4] = {hi[axis], lo[axis], HNONE, HNONE};
-    const NphDVec3 dir = d3_unit_or_zero(d3_sub(P[v[1]], P[v[0]]));
-    f64 best = 0.0;
-    NphDVec3 nrm = d3(0.0, 0.0, 0.0);
-    for (u32 i = 0U; i < q->n; ++i) {
-        const NphDVec3 x = d3_cross(dir, d3_sub(P[i], P[v[0]]));
-        const f64 len = d3_dot(x, x);
-        if (len > best && i != v[0] && i != v[1]) {
-            best = len;
-            v[2] = i;
-            nrm = x;
-        }
-    }
-    if (v[2] == HNONE || sqrt(best) <= 100.0 * q->tol) {
-        return false;
-    }
-    nrm = d3_unit_or_zero(nrm);
-    const f64 d0 = d3_dot(P[v[2]], nrm);
-    best = 0.0;
-    for (u32 i = 0U; i < q->n; ++i) {
-        const f64 dist = fabs(d3_dot(P[i], nrm) - d0);
-        if (dist > best && i != v[0] && i != v[1] && i != v[2]) {
-            best = dist;
-            v[3] = i;
-        }
-    }
-    if (v[3] == HNONE || best <= 100.0 * q->tol) {
-        return false;
-    }
-    // Loops counter-clockwise from outside: the base faces away from v3.
-    u32 tri[4];
-    if (d3_dot(P[v[3]], nrm) - d0 < 0.0) {
-        tri[0] = add_tri(q, v[0], v[1], v[2]);
-        tri[1] = add_tri(q, v[3], v[1], v[0]);
-        tri[2] = add_tri(q, v[3], v[2], v[1]);
-        tri[3] = add_tri(q, v[3], v[0], v[2]);
-    } else {
-        tri[0] = add_tri(q, v[0], v[2], v[1]);
-        tri[1] = add_tri(q, v[3], v[0], v[1]);
-        tri[2] = add_tri(q, v[3], v[1], v[2]);
-        tri[3] = add_tri(q, v[3], v[2], v[0]);
-    }
-    if (q->oom) {
-        return false;
-    }
-    // Twins of the twelve half-edges: the one running the other way.
-    QEdge *E = QE(q);
-    const u32 base = QF(q)[tri[0]].edge;
-    for (u32 i = base; i < base + 12U; ++i) {
-        for (u32 j = base; j < base + 12U; ++j) {
-            if (E[j].head == tail_of(q, i) && tail_of(q, j) == E[i].head) {
-                E[i].twin = j;
-            }
-        }
-    }
-    for (u32 i = 0U; i < q->n; ++i) {
-        if (i == v[0] || i == v[1] || i == v[2] || i == v[3]) {
+    const u32 a0 = hi[axis];
+    const u32 a1 = lo[axis];
+    const NphDVec3 line = dv_unit(dv_sub(pt[a1], pt[a0]));
+    u32 a2 = NIL;
+    f64 reach = 0.0;
+    NphDVec3 n = dv(0.0, 0.0, 0.0);
+    for (u32 i = 0U; i < b->npt; ++i) {
+        if (i == a0 || i == a1) {
             continue;
         }
-        f64 top = q->tol;
-        u32 face = HNONE;
-        for (u32 k = 0U; k < 4U; ++k) {
-            const f64 d = height(&QF(q)[tri[k]], P[i]);
-            if (d > top) {
-                top = d;
-                face = tri[k];
-            }
-        }
-        if (face != HNONE) {
-            outside_push(q, face, i, top);
+        const NphDVec3 x = dv_cross(line, dv_sub(pt[i], pt[a0]));
+        const f64 sq = dv_dot(x, x);
+        if (sq > reach) {
+            reach = sq;
+            a2 = i;
+            n = x;
         }
     }
-    return true;
+    if (a2 == NIL || sqrt(reach) <= 100.0 * b->eps) {
+        return 2;
+    }
+    n = dv_unit(n);
+    const f64 base = dv_dot(pt[a2], n);
+    u32 a3 = NIL;
+    f64 depth = 0.0;
+    for (u32 i = 0U; i < b->npt; ++i) {
+        if (i == a0 || i == a1 || i == a2) {
+            continue;
+        }
+        const f64 d = fabs(dv_dot(pt[i], n) - base);
+        if (d > depth) {
+            depth = d;
+            a3 = i;
+        }
+    }
+    if (a3 == NIL || depth <= 100.0 * b->eps) {
+        return 3;
+    }
+    // The base faces away from the apex a3, then a side over each base edge,
+    // made in this order (their numbers are part of the result).
+    const bool below = dv_dot(pt[a3], n) - base < 0.0;
+    u32 tri[4];
+    tri[0] = below ? triangle(b, a0, a1, a2) : triangle(b, a0, a2, a1);
+    const u32 ring[4] = {a0, a1, a2, a0};
+    for (u32 k = 0U; k < 3U && !b->broke; ++k) {
+        const u32 x = ring[k + 1U];
+        const u32 y = ring[k];
+        tri[k + 1U] = below ? triangle(b, a3, x, y) : triangle(b, a3, y, x);
+    }
+    if (b->broke) {
+        return -1;
+    }
+    // Stitch the twelve half-edges: each to the one running the other way.
+    for (u32 x = 0U; x < 12U; ++x) {
+        for (u32 y = x + 1U; y < 12U; ++y) {
+            const u32 ex = POLY(b)[tri[x / 3U]].lead + x % 3U;
+            const u32 ey = POLY(b)[tri[y / 3U]].lead + y % 3U;
+            if (ARC(b)[ex].to == start_of(b, ey) && start_of(b, ex) == ARC(b)[ey].to) {
+                mate(b, ex, ey);
+            }
+        }
+    }
+    for (u32 i = 0U; i < b->npt; ++i) {
+        if (i == a0 || i == a1 || i == a2 || i == a3) {
+            continue;
+        }
+        f64 best = b->eps;
+        u32 to = NIL;
+        for (u32 k = 0U; k < 4U; ++k) {
+            const f64 h = height(&POLY(b)[tri[k]], pt[i]);
+            if (h > best) {
+                best = h;
+                to = tri[k];
+            }
+        }
+        if (to != NIL) {
+            enlist(b, to, i, best);
+        }
+    }
+    return 0;
 }
 
-static void qh_release(Qh *q)
+static void build_free(Build *b)
 {
-    hull_give(q->a, q->pts, (u64)q->n * sizeof(NphDVec3));
-    hull_give(q->a, q->link, (u64)q->n * 4U);
-    vbuf_free(q->a, &q->edges, (u32)sizeof(QEdge));
-    vbuf_free(q->a, &q->faces, (u32)sizeof(QFace));
-    vbuf_free(q->a, &q->horizon, 4U);
-    vbuf_free(q->a, &q->fresh, 4U);
-    vbuf_free(q->a, &q->orphans, 4U);
-    vbuf_free(q->a, &q->frames, (u32)sizeof(QFrame));
+    drop(b->a, b->pt, (u64)b->npt * sizeof(NphDVec3));
+    drop(b->a, b->after, (u64)b->npt * 4U);
+    vbuf_free(b->a, &b->arcs, (u32)sizeof(Arc));
+    vbuf_free(b->a, &b->polys, (u32)sizeof(Poly));
+    vbuf_free(b->a, &b->rim, 4U);
+    vbuf_free(b->a, &b->cone, 4U);
+    vbuf_free(b->a, &b->loose, 4U);
 }
 
-// The live faces, in creation order, as loops over the vertices numbered in
-// order of first use; packed (hull.c:977-1066).
-static NphResult emit(Qh *q, const NphVec3 *input, f32 tol, NphAllocator *a,
-                      NphPhysicsHull **out)
+// The living faces as loops over renumbered vertices, packed.
+static NphResult harvest(Build *b, const NphVec3 *input, f32 tol, NphAllocator *a,
+                         NphPhysicsHull **out)
 {
     u32 nf = 0U;
-    u32 nl = 0U;
-    for (u32 f = 0U; f < q->faces.len; ++f) {
-        if (QF(q)[f].state != QF_GONE) {
+    u32 corners = 0U;
+    for (u32 f = 0U; f < b->polys.len; ++f) {
+        if (POLY(b)[f].fate != DEAD) {
             nf += 1U;
-            nl += QF(q)[f].nverts;
+            corners += POLY(b)[f].degree;
         }
     }
-    const u64 b_remap = (u64)q->n * 4U;
-    const u64 b_loop = (u64)nl * 4U;
-    const u64 b_offs = (u64)(nf + 1U) * 4U;
-    const u64 b_verts = (u64)NPH_PHYSICS_HULL_MAX_VERTICES * sizeof(NphVec3);
-    u32 *remap = (u32 *)hull_take(a, b_remap);
-    u32 *loop = (u32 *)hull_take(a, b_loop);
-    u32 *offs = (u32 *)hull_take(a, b_offs);
-    NphVec3 *verts = (NphVec3 *)hull_take(a, b_verts);
+    const u32 cap = NPH_PHYSICS_HULL_MAX_VERTICES;
+    const u64 bytes = ((u64)b->npt + corners + nf + 1U) * 4U + (u64)cap * sizeof(NphVec3);
+    u32 *renum = (u32 *)grab(a, bytes);
+    if (renum == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    u32 *loop = renum + b->npt;
+    u32 *offsets = loop + corners;
+    NphVec3 *verts = (NphVec3 *)(offsets + nf + 1U);
+    for (u32 i = 0U; i < b->npt; ++i) {
+        renum[i] = NIL;
+    }
     NphResult r = NPH_RESULT_OK;
-    if (remap == NULL || loop == NULL || offs == NULL || verts == NULL) {
-        r = NPH_RESULT_OUT_OF_MEMORY;
-    } else {
-        for (u32 i = 0U; i < q->n; ++i) {
-            remap[i] = HNONE;
+    u32 nv = 0U;
+    u32 k = 0U;
+    u32 at = 0U;
+    for (u32 f = 0U; f < b->polys.len && r == NPH_RESULT_OK; ++f) {
+        const Poly *P = &POLY(b)[f];
+        if (P->fate == DEAD) {
+            continue;
         }
-        u32 nv = 0U;
-        u32 face = 0U;
-        u32 cursor = 0U;
-        const QEdge *E = QE(q);
-        for (u32 f = 0U; f < q->faces.len && r == NPH_RESULT_OK; ++f) {
-            const QFace *qf = &QF(q)[f];
-            if (qf->state == QF_GONE) {
-                continue;
-            }
-            offs[face++] = cursor;
-            u32 e = qf->edge;
-            do {
-                const u32 p = E[e].head;
-                if (remap[p] == HNONE) {
-                    if (nv == NPH_PHYSICS_HULL_MAX_VERTICES) {
-                        r = NPH_RESULT_INVALID_ARGUMENT;
-                        break;
-                    }
-                    remap[p] = nv;
-                    verts[nv++] = input[p];
+        offsets[k++] = at;
+        u32 e = P->lead;
+        do {
+            const u32 p = ARC(b)[e].to;
+            if (renum[p] == NIL) {
+                if (nv == cap) {
+                    r = NPH_RESULT_INVALID_ARGUMENT;
+                    break;
                 }
-                loop[cursor++] = remap[p];
-                e = E[e].next;
-            } while (e != qf->edge);
-        }
-        offs[face] = cursor;
-        if (r == NPH_RESULT_OK) {
-            r = hull_pack(verts, nv, loop, offs, nf, tol, a, out);
-        }
+                renum[p] = nv;
+                verts[nv++] = input[p];
+            }
+            loop[at++] = renum[p];
+            e = ARC(b)[e].fwd;
+        } while (e != P->lead);
     }
-    hull_give(a, remap, b_remap);
-    hull_give(a, loop, b_loop);
-    hull_give(a, offs, b_offs);
-    hull_give(a, verts, b_verts);
+    offsets[k] = at;
+    if (r == NPH_RESULT_OK) {
+        r = pack(verts, nv, loop, offsets, nf, tol, a, out);
+    }
+    drop(a, renum, bytes);
     return r;
 }
 
-static bool hull_check(const void *block, u32 block_size, f32 tolerance, bool loud);
+static bool check_block(const void *block, u32 block_size, f32 tolerance, bool report);
 
-// One build at `tol` (hull.c:1076-1139).
-static NphResult quickhull(const NphPhysicsHullDesc *desc, u32 budget, f64 tol, NphAllocator *a,
+// One build at tolerance `eps`.
+static NphResult quickhull(const NphPhysicsHullDesc *desc, u32 budget, f64 eps, NphAllocator *a,
                            NphPhysicsHull **out)
 {
-    Qh q;
-    memset(&q, 0, sizeof q);
-    q.a = a;
-    q.n = desc->point_count;
-    q.tol = tol;
-    q.pts = (NphDVec3 *)hull_take(a, (u64)q.n * sizeof(NphDVec3));
-    q.link = (u32 *)hull_take(a, (u64)q.n * 4U);
-    if (q.pts == NULL || q.link == NULL) {
-        qh_release(&q);
+    Build b;
+    memset(&b, 0, sizeof b);
+    b.a = a;
+    b.npt = desc->point_count;
+    b.eps = eps;
+    b.pt = (NphDVec3 *)grab(a, (u64)b.npt * sizeof(NphDVec3));
+    b.after = (u32 *)grab(a, (u64)b.npt * 4U);
+    if (b.pt == NULL || b.after == NULL) {
+        build_free(&b);
         return NPH_RESULT_OUT_OF_MEMORY;
     }
-    for (u32 i = 0U; i < q.n; ++i) {
-        q.pts[i] = d3_of(desc->points[i]);
-        q.link[i] = HNONE;
+    for (u32 i = 0U; i < b.npt; ++i) {
+        b.pt[i] = dv_wide(desc->points[i]);
+        b.after[i] = NIL;
     }
-    if (!seed_tetrahedron(&q)) {
-        const int oom = q.oom;
-        qh_release(&q);
-        return oom ? NPH_RESULT_OUT_OF_MEMORY : NPH_RESULT_INVALID_ARGUMENT;
+    const int why = seed(&b);
+    if (why != 0) {
+        build_free(&b);
+        if (why < 0) {
+            return NPH_RESULT_OUT_OF_MEMORY;
+        }
+        static const char *const shape[4] = {"", "coincide", "are collinear", "are coplanar"};
+        fprintf(stderr, "nepheline_amd: hull build: %u points %s (tolerance %g); no hull\n",
+                desc->point_count, shape[why], eps);
+        return NPH_RESULT_INVALID_ARGUMENT;
     }
-    u32 used = 4U;
-    while (!q.oom && used < budget) {
-        // The highest outside point over all faces; the lowest face index
-        // on ties.
-        u32 eye_face = HNONE;
-        f64 top = q.tol;
-        for (u32 f = 0U; f < q.faces.len; ++f) {
-            const QFace *face = &QF(&q)[f];
-            if (face->state != QF_GONE && face->far_point != HNONE && face->far_dist > top) {
-                top = face->far_dist;
-                eye_face = f;
+    // The highest outside point over every face next: the most volume
+    // first, so a vertex budget keeps the points that matter most.
+    for (u32 count = 4U; count < budget && !b.broke; ++count) {
+        u32 face = NIL;
+        f64 top = b.eps;
+        for (u32 f = 0U; f < b.polys.len; ++f) {
+            const Poly *P = &POLY(&b)[f];
+            if (P->fate != DEAD && P->apex != NIL && P->peak > top) {
+                top = P->peak;
+                face = f;
             }
         }
-        if (eye_face == HNONE) {
+        if (face == NIL) {
             break;
         }
-        insert_point(&q, QF(&q)[eye_face].far_point, eye_face);
-        used += 1U;
+        expand(&b, POLY(&b)[face].apex, face);
     }
-    NphResult r = NPH_RESULT_OUT_OF_MEMORY;
-    if (!q.oom) {
-        r = emit(&q, desc->points, (f32)tol, a, out);
-    }
-    qh_release(&q);
+    const NphResult r = b.broke ? NPH_RESULT_OUT_OF_MEMORY
+                                : harvest(&b, desc->points, (f32)eps, a, out);
+    build_free(&b);
     return r;
 }
 
@@ -1163,57 +1212,53 @@ NphResult nph_physics_hull_build(const NphPhysicsHullDesc *desc, NphAllocator *a
         !isfinite(desc->tolerance)) {
         return NPH_RESULT_INVALID_ARGUMENT;
     }
-    f64 mx = 0.0;
-    f64 my = 0.0;
-    f64 mz = 0.0;
+    f64 reach[3] = {0.0, 0.0, 0.0};
     for (u32 i = 0U; i < desc->point_count; ++i) {
         const NphVec3 p = desc->points[i];
-        if (!f3_finite(p) || fabsf(p.x) > NPH_PHYSICS_MAX_COORDINATE ||
-            fabsf(p.y) > NPH_PHYSICS_MAX_COORDINATE || fabsf(p.z) > NPH_PHYSICS_MAX_COORDINATE) {
+        if (!fv_finite(p) || !fv_bounded(p)) {
             return NPH_RESULT_INVALID_ARGUMENT;
         }
-        const f64 ax = fabs((f64)p.x), ay = fabs((f64)p.y), az = fabs((f64)p.z);
-        mx = mx > ax ? mx : ax;
-        my = my > ay ? my : ay;
-        mz = mz > az ? mz : az;
+        const f64 c[3] = {fabs((f64)p.x), fabs((f64)p.y), fabs((f64)p.z)};
+        for (u32 k = 0U; k < 3U; ++k) {
+            reach[k] = reach[k] > c[k] ? reach[k] : c[k];
+        }
     }
-    // The round-off bound of a plane distance over the cloud's extent
-    // (Barber et al. 1996, section 4).
-    f64 tol = desc->tolerance > 0.0F ? (f64)desc->tolerance
-                                     : 3.0 * (f64)FLT_EPSILON * (mx + my + mz);
-    // A cloud with features near the tolerance's scale can leave a merged
-    // face reflex beyond it; such a hull is built again at 4x, then 16x the
-    // tolerance, and refused after that.
-    for (u32 attempt = 0U; attempt < 3U; ++attempt) {
-        const NphResult r = quickhull(desc, budget, tol, alloc, out);
+    // By default, the round-off bound of a plane distance over the cloud's
+    // extent (Barber et al. 1996, section 4).
+    f64 eps = desc->tolerance > 0.0F ? (f64)desc->tolerance
+                                     : 3.0 * (f64)FLT_EPSILON * (reach[0] + reach[1] + reach[2]);
+    // Merging keeps faces convex within the tolerance, but features near its
+    // scale can still leave a face reflex beyond it: such a hull is built
+    // again at 4x, then 16x the tolerance, and refused after that.
+    for (u32 round = 0U; round < 3U; ++round, eps *= 4.0) {
+        const NphResult r = quickhull(desc, budget, eps, alloc, out);
         if (r != NPH_RESULT_OK) {
             return r;
         }
-        if (hull_check(*out, (*out)->byte_size, 0.0F, false)) {
+        if (check_block(*out, (*out)->byte_size, 0.0F, false)) {
             return NPH_RESULT_OK;
         }
         nph_physics_hull_free(*out, alloc);
         *out = NULL;
-        tol *= 4.0;
     }
-    fprintf(stderr, "nepheline_amd: hull build: the %u points give no hull valid within %g\n",
-            desc->point_count, tol / 4.0);
+    fprintf(stderr, "nepheline_amd: hull build: no valid hull of the %u points up to tolerance %g\n",
+            desc->point_count, eps / 4.0);
     return NPH_RESULT_INVALID_ARGUMENT;
 }
 
 // ---------------------------------------------------------------------------
-// Queries (hull.c:1203-1230)
+// Block queries (hull.c:1203-1230)
 // ---------------------------------------------------------------------------
 u32 nph_physics_hull_support(const NphPhysicsHull *hull, NphVec3 direction)
 {
     if (hull == NULL || hull->vertex_count == 0U) {
         return 0U;
     }
-    const NphVec3 *v = (const NphVec3 *)((const u8 *)hull + hull->vertices_offset);
+    const NphVec3 *v = AT_C(hull, hull->vertices_offset, NphVec3);
     u32 best = 0U;
-    f32 top = f3_dot(v[0], direction);
+    f32 top = fv_dot(v[0], direction);
     for (u32 i = 1U; i < hull->vertex_count; ++i) {
-        const f32 d = f3_dot(v[i], direction);
+        const f32 d = fv_dot(v[i], direction);
         if (d > top) {
             top = d;
             best = i;
@@ -1227,158 +1272,159 @@ u32 nph_physics_hull_face_vertex_count(const NphPhysicsHull *hull, u32 face)
     if (hull == NULL || face >= hull->face_count) {
         return 0U;
     }
-    const NphPhysicsHullEdge *E =
-        (const NphPhysicsHullEdge *)((const u8 *)hull + hull->edges_offset);
-    const u32 first = ((const u16 *)((const u8 *)hull + hull->faces_offset))[face];
-    u32 n = 0U;
-    u32 e = first;
-    do {
+    const NphPhysicsHullEdge *E = AT_C(hull, hull->edges_offset, NphPhysicsHullEdge);
+    const u32 lead = AT_C(hull, hull->faces_offset, u16)[face];
+    // Counts at most edge_count + 1 steps, so a loop that never returns ends.
+    u32 n = 1U;
+    for (u32 e = E[lead].next; e != lead && n <= hull->edge_count; e = E[e].next) {
         n += 1U;
-        e = E[e].next;
-    } while (e != first && n <= hull->edge_count);
+    }
     return n;
 }
 
 // ---------------------------------------------------------------------------
-// Validation (hull.c:1254-1377): the same checks in the same order, so a
-// build is accepted or retried exactly when the reference's is.
+// Validation: a block is accepted exactly when the reference's validator
+// accepts it (hull.c:1254-1377), which also decides when a build is made
+// again at a larger tolerance. The conditions, grouped so that each group
+// reads only what the groups before it have shown to be in the block:
+//   1. header and arrays inside the block, aligned;
+//   2. counts: 4..256 vertices, faces <= 2V - 4, edges <= 3V - 6 (convex
+//      polyhedra with three edges a face at least), V - E + F = 2;
+//   3. header numbers finite and in range;
+//   4. vertices finite and inside the bounds;
+//   5. half-edge links: mates at 2k and 2k + 1, next on the same face,
+//      mate starts where next starts, mate on another face;
+//   6. each face's lead on it, its normal unit, its loop closing, the loops
+//      covering every half-edge;
+//   7. each loop vertex on its face's plane and each corner convex, within
+//      the tolerance; no vertex in front of any face.
 // ---------------------------------------------------------------------------
-#define REJECT(...)                                                                                \
+#define FAIL_IF(cond, ...)                                                                         \
     do {                                                                                           \
-        if (loud) {                                                                                \
-            fprintf(stderr, "nepheline_amd: hull validate: " __VA_ARGS__);                        \
-            fputc('\n', stderr);                                                                   \
+        if (cond) {                                                                                \
+            if (report) {                                                                          \
+                fprintf(stderr, "nepheline_amd: hull block rejected: " __VA_ARGS__);              \
+                fputc('\n', stderr);                                                               \
+            }                                                                                      \
+            return false;                                                                          \
         }                                                                                          \
-        return false;                                                                              \
     } while (0)
 
-static bool in_range(f32 v)
+// Array [off, off + count * elem) inside `size` bytes, after the header, and
+// aligned to `align`.
+static bool array_fits(u32 off, u32 count, u32 elem, u64 size, u32 align)
 {
-    return fabsf(v) <= NPH_PHYSICS_MAX_COORDINATE; // false for NaN
+    return off >= (u32)sizeof(NphPhysicsHull) && (u64)off + (u64)count * elem <= size &&
+           (off & (align - 1U)) == 0U;
 }
 
-static bool v_in_range(NphVec3 v)
+static bool header_in_range(const NphPhysicsHull *h)
 {
-    return in_range(v.x) && in_range(v.y) && in_range(v.z);
+    const NphAABB bx = h->bounds;
+    if (!fv_bounded(bx.min) || !fv_bounded(bx.max) || !(bx.min.x <= bx.max.x) ||
+        !(bx.min.y <= bx.max.y) || !(bx.min.z <= bx.max.z) || !fv_bounded(h->centroid)) {
+        return false;
+    }
+    if (!(h->volume > 0.0F) || !isfinite(h->volume) || !(h->tolerance > 0.0F) ||
+        !(h->tolerance <= NPH_PHYSICS_MAX_COORDINATE)) {
+        return false;
+    }
+    if (!(h->inner_radius >= 0.0F) || !(h->inner_radius <= h->outer_radius) ||
+        !(h->outer_radius <= 4.0F * NPH_PHYSICS_MAX_COORDINATE)) {
+        return false;
+    }
+    return fv_finite(h->inertia.c[0]) && fv_finite(h->inertia.c[1]) && fv_finite(h->inertia.c[2]);
 }
 
-static bool hull_check(const void *block, u32 block_size, f32 tolerance, bool loud)
+static bool check_block(const void *block, u32 block_size, f32 tolerance, bool report)
 {
-    const u32 head = (u32)sizeof(NphPhysicsHull);
-    if (block == NULL || block_size < head || ((uintptr_t)block & 15U) != 0U) {
-        REJECT("no 16-byte aligned header in %u bytes", block_size);
-    }
-    const NphPhysicsHull *hull = (const NphPhysicsHull *)block;
-    if (hull->byte_size < head || hull->byte_size > block_size) {
-        REJECT("byte_size %u outside the %u-byte block", hull->byte_size, block_size);
-    }
-    const u64 size = hull->byte_size;
-    const u64 v_end = (u64)hull->vertices_offset + (u64)hull->vertex_count * sizeof(NphVec3);
-    const u64 p_end = (u64)hull->planes_offset + (u64)hull->face_count * sizeof(NphPlane);
-    const u64 f_end = (u64)hull->faces_offset + (u64)hull->face_count * 2U;
-    const u64 e_end = (u64)hull->edges_offset + (u64)hull->edge_count * sizeof(NphPhysicsHullEdge);
-    if (hull->vertices_offset < head || hull->planes_offset < head || hull->faces_offset < head ||
-        hull->edges_offset < head || v_end > size || p_end > size || f_end > size ||
-        e_end > size || (hull->vertices_offset & 3U) != 0U || (hull->planes_offset & 3U) != 0U ||
-        (hull->faces_offset & 1U) != 0U || (hull->edges_offset & 1U) != 0U) {
-        REJECT("arrays outside the %u-byte block", hull->byte_size);
-    }
-    const u32 nv = hull->vertex_count;
-    const u32 ne = hull->edge_count;
-    const u32 nf = hull->face_count;
-    // At most 2V - 4 faces and 3V - 6 edges for V vertices (Euler).
-    if (nv < 4U || nv > NPH_PHYSICS_HULL_MAX_VERTICES || nf < 4U || nf > 2U * nv - 4U ||
-        ne < 12U || (ne & 1U) != 0U || ne / 2U > 3U * nv - 6U) {
-        REJECT("counts nv=%u ne=%u nf=%u", nv, ne, nf);
-    }
-    const int64_t chi = (int64_t)nv - (int64_t)(ne / 2U) + (int64_t)nf;
-    if (chi != 2) {
-        REJECT("Euler characteristic nv - ne + nf = %lld", (long long)chi);
-    }
-    const NphAABB box = hull->bounds;
-    bool ok = v_in_range(box.min) && v_in_range(box.max) && box.min.x <= box.max.x &&
-              box.min.y <= box.max.y && box.min.z <= box.max.z && v_in_range(hull->centroid) &&
-              hull->volume > 0.0F && isfinite(hull->volume) && hull->tolerance > 0.0F &&
-              hull->tolerance <= NPH_PHYSICS_MAX_COORDINATE && hull->inner_radius >= 0.0F &&
-              hull->inner_radius <= hull->outer_radius &&
-              hull->outer_radius <= 4.0F * NPH_PHYSICS_MAX_COORDINATE;
-    for (u32 i = 0U; i < 3U; ++i) {
-        ok = ok && f3_finite(hull->inertia.c[i]);
-    }
-    if (!ok) {
-        REJECT("header numbers not finite or out of range");
-    }
-    const f32 tol = tolerance > 0.0F ? tolerance : f_max(hull->tolerance, 1e-6F) * 8.0F;
-    const NphVec3 *V = (const NphVec3 *)((const u8 *)hull + hull->vertices_offset);
+    // 1.
+    FAIL_IF(block == NULL || block_size < (u32)sizeof(NphPhysicsHull) || ((uintptr_t)block & 15U) != 0U,
+            "%u bytes hold no 16-byte aligned header", block_size);
+    const NphPhysicsHull *h = (const NphPhysicsHull *)block;
+    FAIL_IF(h->byte_size < (u32)sizeof(NphPhysicsHull) || h->byte_size > block_size,
+            "byte_size %u, block %u", h->byte_size, block_size);
+    const u32 nv = h->vertex_count;
+    const u32 ne = h->edge_count;
+    const u32 nf = h->face_count;
+    const u64 size = h->byte_size;
+    FAIL_IF(!array_fits(h->vertices_offset, nv, (u32)sizeof(NphVec3), size, 4U) ||
+                !array_fits(h->planes_offset, nf, (u32)sizeof(NphPlane), size, 4U) ||
+                !array_fits(h->faces_offset, nf, 2U, size, 2U) ||
+                !array_fits(h->edges_offset, ne, (u32)sizeof(NphPhysicsHullEdge), size, 2U),
+            "an array runs outside the %u bytes or is misaligned", h->byte_size);
+    // 2.
+    FAIL_IF(nv < 4U || nv > NPH_PHYSICS_HULL_MAX_VERTICES || nf < 4U || nf > 2U * nv - 4U ||
+                ne < 12U || (ne & 1U) != 0U || ne / 2U > 3U * nv - 6U,
+            "%u vertices, %u half-edges, %u faces cannot bound a convex polyhedron", nv, ne, nf);
+    FAIL_IF((int64_t)nv + (int64_t)nf - (int64_t)(ne / 2U) != 2, "V - E + F = %lld, not 2",
+            (long long)((int64_t)nv + (int64_t)nf - (int64_t)(ne / 2U)));
+    // 3.
+    FAIL_IF(!header_in_range(h), "bounds, centroid, volume, inertia, radii or tolerance invalid");
+    const f32 tol = tolerance > 0.0F ? tolerance : greater(h->tolerance, 1e-6F) * 8.0F;
+    // 4.
+    const NphVec3 *V = AT_C(h, h->vertices_offset, NphVec3);
+    const NphAABB bx = h->bounds;
     for (u32 v = 0U; v < nv; ++v) {
         const NphVec3 p = V[v];
-        if (!f3_finite(p) || !(p.x >= box.min.x && p.x <= box.max.x && p.y >= box.min.y &&
-                               p.y <= box.max.y && p.z >= box.min.z && p.z <= box.max.z)) {
-            REJECT("vertex %u not finite or outside the bounds", v);
-        }
+        const bool inside = p.x >= bx.min.x && p.x <= bx.max.x && p.y >= bx.min.y &&
+                            p.y <= bx.max.y && p.z >= bx.min.z && p.z <= bx.max.z;
+        FAIL_IF(!fv_finite(p) || !inside, "vertex %u (%g %g %g) outside the bounds", v, (f64)p.x,
+                (f64)p.y, (f64)p.z);
     }
-    const NphPlane *PL = (const NphPlane *)((const u8 *)hull + hull->planes_offset);
-    const u16 *FA = (const u16 *)((const u8 *)hull + hull->faces_offset);
-    const NphPhysicsHullEdge *E =
-        (const NphPhysicsHullEdge *)((const u8 *)hull + hull->edges_offset);
+    // 5.
+    const NphPhysicsHullEdge *E = AT_C(h, h->edges_offset, NphPhysicsHullEdge);
     for (u32 i = 0U; i < ne; ++i) {
-        const NphPhysicsHullEdge *e = &E[i];
-        if (e->twin != (i ^ 1U) || e->next >= ne || e->origin >= nv || e->face >= nf) {
-            REJECT("half-edge %u has bad links", i);
-        }
-        if (E[e->next].face != e->face) {
-            REJECT("half-edge %u: next leaves the face", i);
-        }
-        if (E[e->twin].origin != E[e->next].origin) {
-            REJECT("half-edge %u: twin does not start at its head", i);
-        }
-        if (E[e->twin].face == e->face) {
-            REJECT("half-edge %u: both sides on face %u", i, (u32)e->face);
-        }
+        const NphPhysicsHullEdge e = E[i];
+        FAIL_IF(e.twin != (i ^ 1U) || e.next >= ne || e.origin >= nv || e.face >= nf,
+                "half-edge %u: twin %u next %u origin %u face %u", i, (u32)e.twin, (u32)e.next,
+                (u32)e.origin, (u32)e.face);
+        FAIL_IF(E[e.next].face != e.face || E[e.twin].face == e.face ||
+                    E[e.twin].origin != E[e.next].origin,
+                "half-edge %u does not join two faces consistently", i);
     }
-    u32 seen = 0U;
+    // 6.
+    const NphPlane *PL = AT_C(h, h->planes_offset, NphPlane);
+    const u16 *lead = AT_C(h, h->faces_offset, u16);
+    u32 covered = 0U;
     for (u32 f = 0U; f < nf; ++f) {
-        if (FA[f] >= ne || E[FA[f]].face != f) {
-            REJECT("face %u: bad first edge", f);
-        }
+        FAIL_IF(lead[f] >= ne || E[lead[f]].face != f, "face %u: lead %u not on it", f,
+                (u32)lead[f]);
         const NphPlane pl = PL[f];
-        if (!f3_finite(pl.normal) || fabsf(f3_len(pl.normal) - 1.0F) > 1e-4F || !isfinite(pl.d)) {
-            REJECT("face %u: plane normal not unit", f);
+        FAIL_IF(!fv_finite(pl.normal) || !isfinite(pl.d) ||
+                    fabsf(fv_len(pl.normal) - 1.0F) > 1e-4F,
+                "face %u: normal of length %g", f, (f64)fv_len(pl.normal));
+        u32 steps = 1U;
+        u32 e = E[lead[f]].next;
+        while (e != lead[f] && steps <= ne) {
+            e = E[e].next;
+            steps += 1U;
         }
-        u32 n = 0U;
-        u32 e = FA[f];
-        do {
-            const u32 nx = E[e].next;
-            const NphVec3 a = V[E[e].origin];
-            const NphVec3 b = V[E[nx].origin];
-            const NphVec3 c = V[E[E[nx].next].origin];
-            if (fabsf(plane_height(pl, a)) > tol) {
-                REJECT("face %u: vertex %u is %g off its plane", f, (u32)E[e].origin,
-                       (f64)plane_height(pl, a));
-            }
-            // A convex corner: c left of a -> b in the face's plane.
-            const NphVec3 ab = f3_sub(b, a);
-            const NphVec3 bc = f3_sub(c, b);
-            if (f3_dot(f3_cross(ab, bc), pl.normal) < -tol * f3_len(ab)) {
-                REJECT("face %u: reflex corner at vertex %u", f, (u32)E[nx].origin);
-            }
-            n += 1U;
-            e = nx;
-        } while (e != FA[f] && n <= ne);
-        if (e != FA[f]) {
-            REJECT("face %u: loop does not close", f);
-        }
-        seen += n;
+        FAIL_IF(e != lead[f], "face %u: loop never returns to its lead", f);
+        covered += steps;
     }
-    if (seen != ne) {
-        REJECT("face loops cover %u of %u half-edges", seen, ne);
+    FAIL_IF(covered != ne, "loops cover %u half-edges, the block has %u", covered, ne);
+    // 7. Every half-edge is on exactly one loop now: check each corner once.
+    for (u32 i = 0U; i < ne; ++i) {
+        const NphPlane pl = PL[E[i].face];
+        const NphVec3 a = V[E[i].origin];
+        const u32 nx = E[i].next;
+        const NphVec3 b = V[E[nx].origin];
+        const NphVec3 c = V[E[E[nx].next].origin];
+        const f32 off = above(pl, a);
+        FAIL_IF(fabsf(off) > tol, "vertex %u is %g from face %u's plane", (u32)E[i].origin,
+                (f64)off, (u32)E[i].face);
+        // c's side of the line a -> b within the face: the turn's cross
+        // product along the normal, over |b - a|, is c's distance from it.
+        const NphVec3 ab = fv_sub(b, a);
+        const NphVec3 bc = fv_sub(c, b);
+        FAIL_IF(fv_dot(fv_cross(ab, bc), pl.normal) < -tol * fv_len(ab),
+                "face %u turns the wrong way at vertex %u", (u32)E[i].face, (u32)E[nx].origin);
     }
     for (u32 f = 0U; f < nf; ++f) {
         for (u32 v = 0U; v < nv; ++v) {
-            if (plane_height(PL[f], V[v]) > tol) {
-                REJECT("vertex %u is %g in front of face %u", v, (f64)plane_height(PL[f], V[v]),
-                       f);
-            }
+            const f32 d = above(PL[f], V[v]);
+            FAIL_IF(d > tol, "vertex %u is %g in front of face %u", v, (f64)d, f);
         }
     }
     return true;
@@ -1386,5 +1432,5 @@ static bool hull_check(const void *block, u32 block_size, f32 tolerance, bool lo
 
 bool nph_physics_hull_validate(const void *block, u32 size, f32 tolerance)
 {
-    return hull_check(block, size, tolerance, true);
+    return check_block(block, size, tolerance, true);
 }
