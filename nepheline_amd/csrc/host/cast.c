@@ -624,43 +624,17 @@ typedef int (*MeshTriangleFn)(void *user, const NphVec3 v[3]);
 static void mesh_walk(const NphPhysicsMesh *mesh, NphAABB box, NphDVec3 origin, MeshTriangleFn visit,
                       void *user)
 {
-    const NphPhysicsMeshNode *nodes = mesh_nodes(mesh);
     const u32 *tri = mesh_triangles(mesh);
     const NphVec3 *v = mesh_vertices(mesh);
-    u32 stack[NPH_MESH_STACK_DEPTH];
-    u32 top = 0U;
-    stack[top++] = 0U;
-    while (top > 0U) {
-        const u32 code = stack[--top];
-        if (!mesh_child_is_leaf(code)) {
-            const NphPhysicsMeshNode *n = &nodes[code];
-            for (u32 k = 4U; k > 0U; --k) {
-                const u32 c = k - 1U;
-                if (n->child[c] != NPH_PHYSICS_MESH_CHILD_EMPTY && top < NPH_MESH_STACK_DEPTH &&
-                    n->min_x[c] <= box.max.x && n->max_x[c] >= box.min.x &&
-                    n->min_y[c] <= box.max.y && n->max_y[c] >= box.min.y &&
-                    n->min_z[c] <= box.max.z && n->max_z[c] >= box.min.z) {
-                    stack[top++] = n->child[c];
-                }
-            }
-            continue;
+    NphBoxWalk w;
+    box_walk_begin(&w, mesh, box);
+    for (u32 t = box_walk_next(&w); t != NPH_NULL32; t = box_walk_next(&w)) {
+        NphVec3 rv[3];
+        for (u32 i = 0U; i < 3U; ++i) {
+            rv[i] = relative_vertex(v[tri[3U * t + i]], origin);
         }
-        const u32 first = (code & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
-        const u32 last = first + (code & 3U) + 1U;
-        for (u32 t = first; t < last; ++t) {
-            const NphVec3 a = v[tri[3U * t]];
-            const NphVec3 b = v[tri[3U * t + 1U]];
-            const NphVec3 c = v[tri[3U * t + 2U]];
-            if (!aabb_overlaps(aabb_make(v3_min(v3_min(a, b), c), v3_max(v3_max(a, b), c)), box)) {
-                continue;
-            }
-            NphVec3 rv[3];
-            for (u32 i = 0U; i < 3U; ++i) {
-                rv[i] = relative_vertex(v[tri[3U * t + i]], origin);
-            }
-            if (!visit(user, rv)) {
-                return;
-            }
+        if (!visit(user, rv)) {
+            return;
         }
     }
 }

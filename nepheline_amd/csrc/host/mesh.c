@@ -1,19 +1,29 @@
 // SPDX-License-Identifier: Apache-2.0
 // Mesh blocks on the host (engine/physics/include/nepheline/physics/mesh.h):
-// the builder and the block validator.
+// the builder, the validator and the box and ray queries.
 //
-// The world step's device narrow phase walks a mesh block's four-wide tree
-// and reads its triangles in leaf order; triangle indices and active-edge
-// flags go into contact ids and normals. So the builder writes the
-// reference's bytes for the same input (mesh.c:190-798): vertices welded
-// into the lowest-numbered earlier representative within the tolerance,
-// triangles without three distinct vertices or without area dropped, edges
-// marked active as the reference marks them (open, shared by more than two,
-// back to back, or bent convex beyond the angle), a binned SAH tree (Wald
-// 2007; 16 bins on the centres' longest axis, median splits past depth 64,
-// leaves of at most four) built depth first with the left half first, and
-// its collapse into four-wide nodes by opening the largest inner child
-// (Dammertz et al. 2008). The code is this library's own.
+// The narrow phase reads a block's triangles in leaf order, and triangle
+// numbers and active-edge flags go into contact ids and normals, so the
+// builder writes the reference's bytes for the same input (mesh.c:190-798).
+// The decisions that fix those bytes:
+//   - welding: each input vertex joins the lowest-numbered earlier
+//     representative within the tolerance among those in the grid cells
+//     (2 tol wide) its tolerance box meets; exact copies only for a
+//     negative tolerance;
+//   - triangles without three distinct representatives, or whose squared
+//     sine at the first corner is at most 1e-12, are dropped;
+//   - an edge is active unless exactly two triangles share it, wound
+//     against each other, neither back to back nor bent by more than the
+//     angle (convex only, for a one-sided mesh);
+//   - the binary tree: binned SAH (Wald 2007) with 16 bins on the centres'
+//     longest axis, medians past depth 64, leaves of at most four, the
+//     left half first; the in-place partition's order becomes leaf order;
+//   - four-wide nodes: each opens its largest inner child until it holds
+//     four (Dammertz, Hanika and Keller 2008), children numbered when their
+//     parent is written, parents in depth-first order;
+//   - output vertices numbered by first use in leaf order.
+// The walks of the queries are nph_bvh4.h's, which the device narrow phase
+// also runs.
 
 #include "world_impl.h"
 
@@ -21,19 +31,18 @@
 #include <math.h>
 #include <stdio.h>
 
-#define MNONE 0xFFFFFFFFu
+#define NIL 0xFFFFFFFFu
 #define SAH_BINS 16U
-#define SAH_DEPTH 64U      // binned SAH above this binary depth, medians below
-#define CHECK_STACK 160U   // the validator's (and the queries') stack bound
-#define SLIVER_SIN_SQ 1e-12 // a triangle's squared sine below this has no area
+#define SAH_DEPTH 64U       // binned SAH above this binary depth, medians below
+#define SLIVER_SIN_SQ 1e-12 // a triangle whose squared sine is not above this has no area
 
-static void *mesh_take(NphAllocator *a, u64 size)
+static void *take(NphAllocator *a, u64 size)
 {
     NphCallsite cs = {__FILE__, (u32)__LINE__};
     return a->vtable->alloc(a, size > 0U ? size : 16U, 16U, cs);
 }
 
-static void mesh_give(NphAllocator *a, void *p, u64 size)
+static void give(NphAllocator *a, void *p, u64 size)
 {
     NphCallsite cs = {__FILE__, (u32)__LINE__};
     if (p != NULL) {
@@ -41,117 +50,126 @@ static void mesh_give(NphAllocator *a, void *p, u64 size)
     }
 }
 
-// f32 helpers in the reference's operation order (vec.h, geometry.h).
-static f32 m_min(f32 a, f32 b)
-{
-    return a < b ? a : b;
-}
-static f32 m_max(f32 a, f32 b)
-{
-    return a > b ? a : b;
-}
-static NphVec3 m_v3(f32 x, f32 y, f32 z)
-{
-    NphVec3 r;
-    r.x = x;
-    r.y = y;
-    r.z = z;
-    return r;
-}
-static NphVec3 m_sub(NphVec3 a, NphVec3 b)
-{
-    return m_v3(a.x - b.x, a.y - b.y, a.z - b.z);
-}
-static NphVec3 m_vmin(NphVec3 a, NphVec3 b)
-{
-    return m_v3(m_min(a.x, b.x), m_min(a.y, b.y), m_min(a.z, b.z));
-}
-static NphVec3 m_vmax(NphVec3 a, NphVec3 b)
-{
-    return m_v3(m_max(a.x, b.x), m_max(a.y, b.y), m_max(a.z, b.z));
-}
-static f32 m_dot(NphVec3 a, NphVec3 b)
-{
-    return a.x * b.x + a.y * b.y + a.z * b.z;
-}
-static f32 m_at(NphVec3 v, u32 i)
-{
-    return i == 0U ? v.x : (i == 1U ? v.y : v.z);
-}
-static bool m_finite(NphVec3 v)
+static bool finite3(NphVec3 v)
 {
     return isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
 }
-// The largest |coordinate| within the limit (false for NaN).
-static bool m_in_range(NphVec3 v)
+// Every |coordinate| within the limit; false for NaN.
+static bool bounded3(NphVec3 v)
 {
-    return m_max(fabsf(v.x), m_max(fabsf(v.y), fabsf(v.z))) <= NPH_PHYSICS_MAX_COORDINATE;
+    const f32 lim = NPH_PHYSICS_MAX_COORDINATE;
+    return fabsf(v.x) <= lim && fabsf(v.y) <= lim && fabsf(v.z) <= lim;
 }
-// The unit normal of triangle (a, b, c), zero when it has none: the
-// normalisation of vec.h:325-355 (1 / sqrt, a rescale when the square
-// overflows).
-static NphVec3 m_unit_normal(NphVec3 a, NphVec3 b, NphVec3 c)
+static f32 half_area(NphAABB b)
 {
-    const NphVec3 u = m_sub(b, a);
-    const NphVec3 v = m_sub(c, a);
-    const NphVec3 n = m_v3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
-    const f32 sq = m_dot(n, n);
-    if (sq > 0.0F && sq <= FLT_MAX) {
-        const f32 s = 1.0F / sqrtf(sq);
-        return m_v3(n.x * s, n.y * s, n.z * s);
-    }
-    if (!(sq > FLT_MAX) || !m_finite(n)) {
-        return m_v3(0.0F, 0.0F, 0.0F);
-    }
-    const f32 m = m_max(fabsf(n.x), m_max(fabsf(n.y), fabsf(n.z)));
-    const f32 inv = 1.0F / m;
-    const NphVec3 w = m_v3(n.x * inv, n.y * inv, n.z * inv);
-    const f32 s = 1.0F / sqrtf(m_dot(w, w));
-    return m_v3(w.x * s, w.y * s, w.z * s);
-}
-static f32 box_area(NphAABB b)
-{
-    const NphVec3 d = m_sub(b.max, b.min);
+    const NphVec3 d = v3_sub(b.max, b.min);
     return d.x * d.y + d.y * d.z + d.z * d.x;
 }
-static NphAABB box_join(NphAABB a, NphAABB b)
+static NphAABB no_box(void)
 {
-    NphAABB r;
-    r.min = m_vmin(a.min, b.min);
-    r.max = m_vmax(a.max, b.max);
-    return r;
-}
-static NphAABB box_none(void)
-{
-    NphAABB r;
-    r.min = m_v3(INFINITY, INFINITY, INFINITY);
-    r.max = m_v3(-INFINITY, -INFINITY, -INFINITY);
-    return r;
-}
-
-static u32 leaf_code(u32 first, u32 count)
-{
-    return NPH_PHYSICS_MESH_CHILD_LEAF | (first << 2U) | (count - 1U);
-}
-static bool is_leaf(u32 child)
-{
-    return child != NPH_PHYSICS_MESH_CHILD_EMPTY && (child & NPH_PHYSICS_MESH_CHILD_LEAF) != 0U;
+    return aabb_make(v3_splat(INFINITY), v3_splat(-INFINITY));
 }
 
 // ---------------------------------------------------------------------------
-// Welding (mesh.c:165-249): a grid of cells 2 tol wide, each holding a chain
-// of the representatives in it; a vertex looks in the cells its tol-ball
-// meets and joins the lowest-numbered representative within tol.
+// Scratch
 // ---------------------------------------------------------------------------
-typedef struct WeldCell {
+typedef struct Item {
+    NphAABB box;
+    NphVec3 center;
+    u32 tri; // kept triangle
+} Item;
+
+// A binary node: inner (count 0; left and right) or a leaf of `count`
+// items from `first`.
+typedef struct BNode {
+    NphAABB box;
+    u32 left, right, first, count;
+} BNode;
+
+typedef struct Work {
+    NphAllocator *a;
+    u32 nv_in, nt_in;
+    u32 *remap;    // [nv_in] input vertex -> representative
+    u32 *chain;    // [nv_in] the next representative in the same cell
+    NphVec3 *pos;  // [nv_in] representatives
+    u32 *tris;     // [3 nt_in] kept triangles over representatives
+    u32 *orig;     // [nt_in] their input indices
+    u8 *active;    // [nt_in] active-edge bits
+    u32 *star_at;  // [nv_in + 1] each representative's file of edges
+    u64 *star;     // [3 nt_in] (upper vertex, occurrence) in lower-vertex files
+    Item *items;   // [nt_in]
+    BNode *bins;   // [2 nt_in]
+    u32 nbins;
+    u32 *out_of;   // [nv_in] representative -> output vertex
+    NphPhysicsMeshNode *nodes; // [node_cap(nt_in)]
+    u32 nnodes;
+} Work;
+
+// A four-wide tree over t triangles has at most t - 1 nodes (one per inner
+// node of the binary tree at most), and one for a single leaf.
+static u32 node_cap(u32 t)
+{
+    return (t > 2U ? t : 2U) - 1U;
+}
+
+static void work_free(Work *s)
+{
+    NphAllocator *a = s->a;
+    const u64 n = s->nv_in;
+    const u64 t = s->nt_in;
+    give(a, s->remap, n * 4U);
+    give(a, s->chain, n * 4U);
+    give(a, s->pos, n * sizeof(NphVec3));
+    give(a, s->tris, 12U * t);
+    give(a, s->orig, 4U * t);
+    give(a, s->active, t);
+    give(a, s->star_at, (n + 1U) * 4U);
+    give(a, s->star, 24U * t);
+    give(a, s->items, t * sizeof(Item));
+    give(a, s->bins, 2U * t * sizeof(BNode));
+    give(a, s->out_of, n * 4U);
+    give(a, s->nodes, (u64)node_cap(s->nt_in) * sizeof(NphPhysicsMeshNode));
+}
+
+static NphResult work_alloc(Work *s)
+{
+    NphAllocator *a = s->a;
+    const u64 n = s->nv_in;
+    const u64 t = s->nt_in;
+    s->remap = (u32 *)take(a, n * 4U);
+    s->chain = (u32 *)take(a, n * 4U);
+    s->pos = (NphVec3 *)take(a, n * sizeof(NphVec3));
+    s->tris = (u32 *)take(a, 12U * t);
+    s->orig = (u32 *)take(a, 4U * t);
+    s->active = (u8 *)take(a, t);
+    s->star_at = (u32 *)take(a, (n + 1U) * 4U);
+    s->star = (u64 *)take(a, 24U * t);
+    s->items = (Item *)take(a, t * sizeof(Item));
+    s->bins = (BNode *)take(a, 2U * t * sizeof(BNode));
+    s->out_of = (u32 *)take(a, n * 4U);
+    s->nodes = (NphPhysicsMeshNode *)take(a, (u64)node_cap(s->nt_in) * sizeof(NphPhysicsMeshNode));
+    if (s->remap == NULL || s->chain == NULL || s->pos == NULL || s->tris == NULL ||
+        s->orig == NULL || s->active == NULL || s->star_at == NULL || s->star == NULL ||
+        s->items == NULL || s->bins == NULL || s->out_of == NULL || s->nodes == NULL) {
+        return NPH_RESULT_OUT_OF_MEMORY;
+    }
+    memset(s->active, 0, (size_t)t);
+    return NPH_RESULT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Welding: a hashed grid of cells, each the head of a chain of the
+// representatives in it.
+// ---------------------------------------------------------------------------
+typedef struct Cell {
     i32 key[3];
-    u32 head; // MNONE: a free slot
-} WeldCell;
+    u32 head; // NIL: a free slot
+} Cell;
 
-typedef struct Weld {
-    WeldCell *cells;
+typedef struct Grid {
+    Cell *cells;
     u32 mask;
-} Weld;
+} Grid;
 
 static u32 cell_hash(const i32 k[3])
 {
@@ -164,11 +182,11 @@ static u32 cell_hash(const i32 k[3])
     return (u32)h;
 }
 
-static WeldCell *cell_find(Weld *w, const i32 k[3], bool insert)
+static Cell *grid_slot(Grid *g, const i32 k[3], bool insert)
 {
-    for (u32 i = cell_hash(k) & w->mask;; i = (i + 1U) & w->mask) {
-        WeldCell *c = &w->cells[i];
-        if (c->head == MNONE) {
+    for (u32 i = cell_hash(k) & g->mask;; i = (i + 1U) & g->mask) {
+        Cell *c = &g->cells[i];
+        if (c->head == NIL) {
             if (!insert) {
                 return NULL;
             }
@@ -181,7 +199,8 @@ static WeldCell *cell_find(Weld *w, const i32 k[3], bool insert)
     }
 }
 
-static void cell_at(NphVec3 p, f32 inv, i32 k[3])
+// The cell of p, cells 1 / inv wide.
+static void cell_of(NphVec3 p, f32 inv, i32 k[3])
 {
     k[0] = (i32)floorf(p.x * inv);
     k[1] = (i32)floorf(p.y * inv);
@@ -189,198 +208,154 @@ static void cell_at(NphVec3 p, f32 inv, i32 k[3])
 }
 
 // An exact copy's key: the coordinates' bits, -0 folded into +0.
-static void bits_at(NphVec3 p, i32 k[3])
+static void bits_of(NphVec3 p, i32 k[3])
 {
     const f32 c[3] = {p.x + 0.0F, p.y + 0.0F, p.z + 0.0F};
     memcpy(k, c, sizeof c);
 }
 
-typedef struct MeshWork {
-    NphAllocator *a;
-    u32 nv_in;
-    u32 nt_in;
-    u32 *remap;   // [nv_in] input vertex -> welded
-    u32 *chain;   // [nv_in] next representative in the same cell
-    NphVec3 *pos; // [nv_in] representatives
-    u32 *tris;    // [3 nt_in] kept triangles over representatives
-    u32 *orig;    // [nt_in]
-    u8 *flags;    // [nt_in] active edges
-    u64 *ekeys;   // [3 nt_in] edge sort keys
-    struct MItem *items;
-    struct MBin *bins;
-    u32 *stack; // [8 nt_in]
-    u32 *out_of; // [nv_in] representative -> output vertex
-    NphPhysicsMeshNode *nodes;
-} MeshWork;
-
-typedef struct MItem {
-    NphAABB box;
-    NphVec3 center;
-    u32 tri;
-} MItem;
-
-// A binary node: inner (count 0) or a leaf of `count` items from `first`.
-typedef struct MBin {
-    NphAABB box;
-    u32 left;
-    u32 right;
-    u32 first;
-    u32 count;
-} MBin;
-
-static u32 node_cap(u32 t)
+// The lowest-numbered representative within tol of p in the cells its
+// tolerance box meets; NIL when none.
+static u32 nearest_rep(const Work *s, Grid *g, NphVec3 p, f32 tol, f32 inv)
 {
-    return (t > 2U ? t : 2U) - 1U;
+    i32 lo[3], hi[3];
+    cell_of(v3_sub(p, v3_splat(tol)), inv, lo);
+    cell_of(v3_add(p, v3_splat(tol)), inv, hi);
+    const f32 tol_sq = tol * tol;
+    u32 best = NIL;
+    for (i32 x = lo[0]; x <= hi[0]; ++x) {
+        for (i32 y = lo[1]; y <= hi[1]; ++y) {
+            for (i32 z = lo[2]; z <= hi[2]; ++z) {
+                const i32 q[3] = {x, y, z};
+                const Cell *c = grid_slot(g, q, false);
+                for (u32 r = c != NULL ? c->head : NIL; r != NIL; r = s->chain[r]) {
+                    if (r < best && v3_dist_sq(s->pos[r], p) <= tol_sq) {
+                        best = r;
+                    }
+                }
+            }
+        }
+    }
+    return best;
 }
 
-static NphResult weld_vertices(MeshWork *s, const NphVec3 *v, f32 tol, u32 *out_count)
+static NphResult weld(Work *s, const NphVec3 *v, f32 tol, u32 *reps)
 {
     u32 cap = 16U;
     while ((u64)cap < 2ULL * s->nv_in && cap < 0x80000000U) {
         cap *= 2U;
     }
-    Weld w;
-    w.cells = (WeldCell *)mesh_take(s->a, (u64)cap * sizeof(WeldCell));
-    w.mask = cap - 1U;
-    if (w.cells == NULL) {
+    Grid g;
+    g.cells = (Cell *)take(s->a, (u64)cap * sizeof(Cell));
+    g.mask = cap - 1U;
+    if (g.cells == NULL) {
         return NPH_RESULT_OUT_OF_MEMORY;
     }
     for (u32 i = 0U; i < cap; ++i) {
-        w.cells[i].head = MNONE;
+        g.cells[i].head = NIL;
     }
     const bool exact = !(tol > 0.0F);
     const f32 inv = exact ? 0.0F : 0.5F / tol;
-    const f32 tol_sq = tol * tol;
     u32 count = 0U;
     for (u32 i = 0U; i < s->nv_in; ++i) {
         const NphVec3 p = v[i];
-        u32 best = MNONE;
         i32 k[3];
+        u32 found = NIL;
         if (exact) {
-            bits_at(p, k);
-            const WeldCell *c = cell_find(&w, k, false);
-            if (c != NULL) {
-                best = c->head;
-            }
+            bits_of(p, k);
+            const Cell *c = grid_slot(&g, k, false);
+            found = c != NULL ? c->head : NIL;
         } else {
-            i32 lo[3];
-            i32 hi[3];
-            cell_at(m_v3(p.x - tol, p.y - tol, p.z - tol), inv, lo);
-            cell_at(m_v3(p.x + tol, p.y + tol, p.z + tol), inv, hi);
-            for (i32 x = lo[0]; x <= hi[0]; ++x) {
-                for (i32 y = lo[1]; y <= hi[1]; ++y) {
-                    for (i32 z = lo[2]; z <= hi[2]; ++z) {
-                        const i32 q[3] = {x, y, z};
-                        const WeldCell *c = cell_find(&w, q, false);
-                        for (u32 r = c != NULL ? c->head : MNONE; r != MNONE; r = s->chain[r]) {
-                            const NphVec3 d = m_sub(s->pos[r], p);
-                            if (r < best && m_dot(d, d) <= tol_sq) {
-                                best = r;
-                            }
-                        }
-                    }
-                }
-            }
+            found = nearest_rep(s, &g, p, tol, inv);
+            cell_of(p, inv, k);
         }
-        if (best != MNONE) {
-            s->remap[i] = best;
+        if (found != NIL) {
+            s->remap[i] = found;
             continue;
         }
-        const u32 rep = count++;
-        s->pos[rep] = p;
-        s->remap[i] = rep;
-        if (exact) {
-            bits_at(p, k);
-        } else {
-            cell_at(p, inv, k);
-        }
-        WeldCell *c = cell_find(&w, k, true);
-        s->chain[rep] = c->head; // MNONE for a new cell
-        c->head = rep;
+        s->pos[count] = p;
+        s->remap[i] = count;
+        Cell *c = grid_slot(&g, k, true);
+        s->chain[count] = c->head; // NIL for a new cell
+        c->head = count;
+        count += 1U;
     }
-    mesh_give(s->a, w.cells, (u64)cap * sizeof(WeldCell));
-    *out_count = count;
+    give(s->a, g.cells, (u64)cap * sizeof(Cell));
+    *reps = count;
     return NPH_RESULT_OK;
 }
 
-// Keeps the triangles with three distinct representatives and an area
-// (mesh.c:261-294; the sine test in f64, where squared lengths cannot
-// overflow).
-static u32 keep_triangles(MeshWork *s, const u32 *idx)
+// Drops triangles without three distinct representatives or without area
+// (the squared sine of the angle at the first corner, in f64 where the
+// products of squared lengths cannot overflow). Returns the count kept.
+static u32 keep_triangles(Work *s, const u32 *idx)
 {
     u32 kept = 0U;
     for (u32 t = 0U; t < s->nt_in; ++t) {
-        const u32 a = s->remap[idx[3ULL * t]];
-        const u32 b = s->remap[idx[3ULL * t + 1U]];
-        const u32 c = s->remap[idx[3ULL * t + 2U]];
-        if (a == b || b == c || a == c) {
+        u32 c[3];
+        for (u32 k = 0U; k < 3U; ++k) {
+            c[k] = s->remap[idx[3ULL * t + k]];
+        }
+        if (c[0] == c[1] || c[1] == c[2] || c[0] == c[2]) {
             continue;
         }
-        const NphVec3 pa = s->pos[a];
-        const NphVec3 pb = s->pos[b];
-        const NphVec3 pc = s->pos[c];
-        const f64 ux = (f64)pb.x - (f64)pa.x, uy = (f64)pb.y - (f64)pa.y, uz = (f64)pb.z - (f64)pa.z;
-        const f64 vx = (f64)pc.x - (f64)pa.x, vy = (f64)pc.y - (f64)pa.y, vz = (f64)pc.z - (f64)pa.z;
-        const f64 cx = uy * vz - uz * vy;
-        const f64 cy = uz * vx - ux * vz;
-        const f64 cz = ux * vy - uy * vx;
-        const f64 cross_sq = cx * cx + cy * cy + cz * cz;
-        const f64 lu = ux * ux + uy * uy + uz * uz;
-        const f64 lv = vx * vx + vy * vy + vz * vz;
-        if (!(cross_sq > SLIVER_SIN_SQ * lu * lv)) {
+        const NphVec3 p0 = s->pos[c[0]];
+        f64 e[2][3];
+        for (u32 j = 0U; j < 2U; ++j) {
+            const NphVec3 p = s->pos[c[j + 1U]];
+            e[j][0] = (f64)p.x - (f64)p0.x;
+            e[j][1] = (f64)p.y - (f64)p0.y;
+            e[j][2] = (f64)p.z - (f64)p0.z;
+        }
+        const f64 nx = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+        const f64 ny = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+        const f64 nz = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+        const f64 n_sq = nx * nx + ny * ny + nz * nz;
+        const f64 l0 = e[0][0] * e[0][0] + e[0][1] * e[0][1] + e[0][2] * e[0][2];
+        const f64 l1 = e[1][0] * e[1][0] + e[1][1] * e[1][1] + e[1][2] * e[1][2];
+        if (!(n_sq > SLIVER_SIN_SQ * l0 * l1)) {
             continue;
         }
-        s->tris[3ULL * kept] = a;
-        s->tris[3ULL * kept + 1U] = b;
-        s->tris[3ULL * kept + 2U] = c;
+        memcpy(&s->tris[3ULL * kept], c, sizeof c);
         s->orig[kept] = t;
         kept += 1U;
     }
     return kept;
 }
 
-// Edge sort records: the undirected edge (lower, higher representative)
-// then the triangle and the edge's slot in it; equal edges are adjacent,
-// by triangle.
-typedef struct MEdge {
-    u64 key;
-    u32 tri;
-    u32 slot;
-} MEdge;
-
-static int edge_order(const void *pa, const void *pb)
+// ---------------------------------------------------------------------------
+// Active edges. Occurrence o = 3 t + e is edge e of kept triangle t, from
+// corner e to corner e + 1. Each occurrence is filed under its lower vertex
+// with its upper vertex as the key; a file sorted by (upper, o) holds each
+// edge's occurrences together, in triangle order.
+// ---------------------------------------------------------------------------
+static u32 corner(const Work *s, u32 o, u32 step)
 {
-    const MEdge *a = (const MEdge *)pa;
-    const MEdge *b = (const MEdge *)pb;
-    if (a->key != b->key) {
-        return a->key < b->key ? -1 : 1;
-    }
-    if (a->tri != b->tri) {
-        return a->tri < b->tri ? -1 : 1;
-    }
-    return a->slot < b->slot ? -1 : (a->slot > b->slot ? 1 : 0);
+    const u32 t = o / 3U;
+    return s->tris[3ULL * t + (o % 3U + step) % 3U];
 }
 
-static NphVec3 tri_normal(const MeshWork *s, u32 t)
+static NphVec3 unit_normal(const Work *s, u32 t)
 {
-    const u32 *v = s->tris + 3ULL * t;
-    return m_unit_normal(s->pos[v[0]], s->pos[v[1]], s->pos[v[2]]);
+    const NphVec3 a = s->pos[s->tris[3ULL * t]];
+    const NphVec3 b = s->pos[s->tris[3ULL * t + 1U]];
+    const NphVec3 c = s->pos[s->tris[3ULL * t + 2U]];
+    return v3_normalize_or(v3_cross(v3_sub(b, a), v3_sub(c, a)), v3_zero());
 }
 
-// An edge two triangles share is active when they wind it the same way, meet
-// back to back, or bend beyond the angle (either way for a two-sided mesh,
-// convex only otherwise); flat and concave edges are not (mesh.c:313-338).
-static bool shared_edge_active(const MeshWork *s, const MEdge *e1, const MEdge *e2, f32 cos_limit,
-                               bool two_sided)
+// An edge exactly two triangles share (occurrences o1 < o2) is active when
+// they run it the same way, meet back to back, or bend beyond the angle:
+// either way on a two-sided mesh, convex only on a one-sided one (the far
+// corner of the second triangle behind the first's plane). Jolt's
+// ActiveEdges::IsEdgeActive is the rule (mesh.c:313-338).
+static bool shared_edge_active(const Work *s, u32 o1, u32 o2, f32 cos_limit, bool two_sided)
 {
-    const u32 *t1 = s->tris + 3ULL * e1->tri;
-    const u32 *t2 = s->tris + 3ULL * e2->tri;
-    if (t1[e1->slot] == t2[e2->slot]) {
+    if (corner(s, o1, 0U) == corner(s, o2, 0U)) {
         return true;
     }
-    const NphVec3 n1 = tri_normal(s, e1->tri);
-    const NphVec3 n2 = tri_normal(s, e2->tri);
-    const f32 cos_angle = m_dot(n1, n2);
+    const NphVec3 n1 = unit_normal(s, o1 / 3U);
+    const NphVec3 n2 = unit_normal(s, o2 / 3U);
+    const f32 cos_angle = v3_dot(n1, n2);
     if (cos_angle < -0.99985F) {
         return true;
     }
@@ -390,216 +365,243 @@ static bool shared_edge_active(const MeshWork *s, const MEdge *e1, const MEdge *
     if (two_sided) {
         return true;
     }
-    const NphVec3 start = s->pos[t1[e1->slot]];
-    const NphVec3 apex = s->pos[t2[(e2->slot + 2U) % 3U]];
-    return m_dot(n1, m_sub(apex, start)) < 0.0F;
+    const NphVec3 start = s->pos[corner(s, o1, 0U)];
+    const NphVec3 far = s->pos[corner(s, o2, 2U)];
+    return v3_dot(n1, v3_sub(far, start)) < 0.0F;
 }
 
-static void mark_edges(MeshWork *s, u32 kept, f32 cos_limit, bool two_sided)
+static int key_order(const void *pa, const void *pb)
 {
-    MEdge *ek = (MEdge *)s->ekeys;
-    const u64 n = 3ULL * kept;
-    for (u32 t = 0U; t < kept; ++t) {
-        for (u32 e = 0U; e < 3U; ++e) {
-            const u32 a = s->tris[3ULL * t + e];
-            const u32 b = s->tris[3ULL * t + (e + 1U) % 3U];
-            MEdge *k = &ek[3ULL * t + e];
-            k->key = a < b ? ((u64)a << 32U) | b : ((u64)b << 32U) | a;
-            k->tri = t;
-            k->slot = e;
-        }
+    const u64 a = *(const u64 *)pa;
+    const u64 b = *(const u64 *)pb;
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+static void sort_file(u64 *f, u32 n)
+{
+    if (n > 24U) {
+        qsort(f, n, sizeof(u64), key_order);
+        return;
     }
-    qsort(ek, (size_t)n, sizeof(MEdge), edge_order);
-    for (u64 i = 0U; i < n;) {
-        u64 j = i + 1U;
-        while (j < n && ek[j].key == ek[i].key) {
-            j += 1U;
+    for (u32 i = 1U; i < n; ++i) {
+        const u64 x = f[i];
+        u32 j = i;
+        for (; j > 0U && f[j - 1U] > x; --j) {
+            f[j] = f[j - 1U];
         }
-        if (j - i != 2U || shared_edge_active(s, &ek[i], &ek[i + 1U], cos_limit, two_sided)) {
-            for (u64 k = i; k < j; ++k) {
-                s->flags[ek[k].tri] |= (u8)(1U << ek[k].slot);
+        f[j] = x;
+    }
+}
+
+static void mark_active_edges(Work *s, u32 kept, u32 reps, f32 cos_limit, bool two_sided)
+{
+    const u32 n = 3U * kept;
+    u32 *at = s->star_at;
+    memset(at, 0, (size_t)(reps + 1U) * 4U);
+    for (u32 o = 0U; o < n; ++o) {
+        const u32 a = corner(s, o, 0U);
+        const u32 b = corner(s, o, 1U);
+        at[(a < b ? a : b) + 1U] += 1U;
+    }
+    for (u32 v = 0U; v < reps; ++v) {
+        at[v + 1U] += at[v];
+    }
+    for (u32 o = 0U; o < n; ++o) {
+        const u32 a = corner(s, o, 0U);
+        const u32 b = corner(s, o, 1U);
+        const u32 lo = a < b ? a : b;
+        const u32 hi = a < b ? b : a;
+        s->star[at[lo]++] = ((u64)hi << 32U) | o;
+    }
+    // at[v] now ends v's file, and at[v - 1] starts it.
+    for (u32 v = 0U; v < reps; ++v) {
+        const u32 begin = v == 0U ? 0U : at[v - 1U];
+        u64 *f = s->star + begin;
+        const u32 len = at[v] - begin;
+        sort_file(f, len);
+        for (u32 i = 0U; i < len;) {
+            u32 j = i + 1U;
+            while (j < len && (f[j] >> 32U) == (f[i] >> 32U)) {
+                j += 1U;
             }
+            // Open, or shared by more than two: always active.
+            const bool on = j - i != 2U || shared_edge_active(s, (u32)f[i], (u32)f[i + 1U],
+                                                              cos_limit, two_sided);
+            for (u32 k = i; on && k < j; ++k) {
+                const u32 o = (u32)f[k];
+                s->active[o / 3U] |= (u8)(1U << (o % 3U));
+            }
+            i = j;
         }
-        i = j;
     }
 }
 
 // ---------------------------------------------------------------------------
-// The tree (mesh.c:376-649)
+// The binary tree
 // ---------------------------------------------------------------------------
-static u32 bin_index(const MItem *it, u32 axis, f32 lo, f32 scale)
+typedef struct Binning {
+    u32 axis;
+    f32 lo;
+    f32 scale;
+    u32 count[SAH_BINS];
+    NphAABB box[SAH_BINS];
+} Binning;
+
+static u32 bin_of(const Binning *b, const Item *it)
 {
-    const f32 x = (m_at(it->center, axis) - lo) * scale;
+    const f32 x = (v3_at(it->center, b->axis) - b->lo) * b->scale;
     if (!(x > 0.0F)) {
         return 0U;
     }
-    const u32 b = (u32)x;
-    return b < SAH_BINS - 1U ? b : SAH_BINS - 1U;
+    const u32 k = (u32)x;
+    return k < SAH_BINS - 1U ? k : SAH_BINS - 1U;
 }
 
-static int center_order_axis(const MItem *a, const MItem *b, u32 axis)
+static void fill_bins(Binning *b, const Item *items, u32 lo, u32 hi)
 {
-    const f32 x = m_at(a->center, axis);
-    const f32 y = m_at(b->center, axis);
-    if (x != y) {
-        return x < y ? -1 : 1;
+    for (u32 k = 0U; k < SAH_BINS; ++k) {
+        b->count[k] = 0U;
+        b->box[k] = no_box();
     }
-    return a->tri < b->tri ? -1 : (a->tri > b->tri ? 1 : 0);
-}
-static int center_order_x(const void *a, const void *b)
-{
-    return center_order_axis((const MItem *)a, (const MItem *)b, 0U);
-}
-static int center_order_y(const void *a, const void *b)
-{
-    return center_order_axis((const MItem *)a, (const MItem *)b, 1U);
-}
-static int center_order_z(const void *a, const void *b)
-{
-    return center_order_axis((const MItem *)a, (const MItem *)b, 2U);
-}
-
-// Splits items [lo, hi) (more than a leaf) and returns the split point, both
-// sides non-empty: the cheapest binned-SAH plane along the centres' longest
-// axis, or the median past SAH_DEPTH (sorted by centre, then triangle).
-static u32 split_items(MItem *items, u32 lo, u32 hi, u32 depth)
-{
-    const u32 count = hi - lo;
-    NphAABB cb;
-    cb.min = items[lo].center;
-    cb.max = items[lo].center;
-    for (u32 i = lo + 1U; i < hi; ++i) {
-        cb.min = m_vmin(cb.min, items[i].center);
-        cb.max = m_vmax(cb.max, items[i].center);
-    }
-    const NphVec3 ext = m_sub(cb.max, cb.min);
-    const u32 axis = ext.x >= ext.y ? (ext.x >= ext.z ? 0U : 2U) : (ext.y >= ext.z ? 1U : 2U);
-    const f32 axis_lo = m_at(cb.min, axis);
-    const f32 axis_len = m_at(ext, axis);
-    const u32 mid = lo + count / 2U;
-    if (!(axis_len > 0.0F)) {
-        return mid;
-    }
-    if (depth >= SAH_DEPTH) {
-        qsort(items + lo, count, sizeof(MItem),
-              axis == 0U ? center_order_x : (axis == 1U ? center_order_y : center_order_z));
-        return mid;
-    }
-    u32 n_in[SAH_BINS];
-    NphAABB b_box[SAH_BINS];
-    for (u32 b = 0U; b < SAH_BINS; ++b) {
-        n_in[b] = 0U;
-        b_box[b] = box_none();
-    }
-    const f32 scale = (f32)SAH_BINS / axis_len;
     for (u32 i = lo; i < hi; ++i) {
-        const u32 b = bin_index(&items[i], axis, axis_lo, scale);
-        n_in[b] += 1U;
-        b_box[b] = box_join(b_box[b], items[i].box);
+        const u32 k = bin_of(b, &items[i]);
+        b->count[k] += 1U;
+        b->box[k] = aabb_union(b->box[k], items[i].box);
     }
-    // Sweep from the right: the area and count of bins b.. for each b.
-    f32 r_area[SAH_BINS];
-    u32 r_count[SAH_BINS];
-    NphAABB acc = box_none();
+}
+
+// The plane between bins that minimises area x count summed over both
+// sides, both sides non-empty: its index (the first bin on the right), or 0
+// when there is none.
+static u32 cheapest_plane(const Binning *b)
+{
+    f32 right_area[SAH_BINS];
+    u32 right_count[SAH_BINS];
+    NphAABB acc = no_box();
     u32 n = 0U;
-    for (u32 b = SAH_BINS - 1U; b > 0U; --b) {
-        acc = box_join(acc, b_box[b]);
-        n += n_in[b];
-        r_area[b] = n > 0U ? box_area(acc) : 0.0F;
-        r_count[b] = n;
+    for (u32 k = SAH_BINS - 1U; k > 0U; --k) {
+        acc = aabb_union(acc, b->box[k]);
+        n += b->count[k];
+        right_area[k] = n > 0U ? half_area(acc) : 0.0F;
+        right_count[k] = n;
     }
-    acc = box_none();
+    acc = no_box();
     n = 0U;
     f32 best = FLT_MAX;
     u32 plane = 0U;
-    for (u32 b = 0U; b + 1U < SAH_BINS; ++b) {
-        acc = box_join(acc, b_box[b]);
-        n += n_in[b];
-        if (n == 0U || r_count[b + 1U] == 0U) {
+    for (u32 k = 1U; k < SAH_BINS; ++k) {
+        acc = aabb_union(acc, b->box[k - 1U]);
+        n += b->count[k - 1U];
+        if (n == 0U || right_count[k] == 0U) {
             continue;
         }
-        const f32 cost = box_area(acc) * (f32)n + r_area[b + 1U] * (f32)r_count[b + 1U];
+        const f32 cost = half_area(acc) * (f32)n + right_area[k] * (f32)right_count[k];
         if (cost < best) {
             best = cost;
-            plane = b + 1U;
+            plane = k;
         }
     }
-    if (!(best < FLT_MAX)) {
-        return mid;
-    }
-    // Partition in place: left of the plane to the front, swapping each
-    // other item with the last unexamined one.
-    u32 i = lo;
-    u32 j = hi;
-    while (i < j) {
-        if (bin_index(&items[i], axis, axis_lo, scale) < plane) {
-            i += 1U;
-        } else {
-            j -= 1U;
-            const MItem t = items[i];
-            items[i] = items[j];
-            items[j] = t;
-        }
-    }
-    return i > lo && i < hi ? i : mid;
+    return plane;
 }
 
-// The binary tree over items [0, count), depth first with the left half
-// built next (so its nodes and leaves come first); pending ranges wait on
-// an explicit stack of (lo, hi, parent link, depth).
-static void build_bins(MeshWork *s, u32 count)
+// Items left of the plane to the front, in place: scanning from the front,
+// an item that belongs right trades places with the last unscanned one.
+// Returns the first item on the right.
+static u32 partition_items(Item *items, u32 lo, u32 hi, const Binning *b, u32 plane)
 {
-    u32 made = 0U;
-    u32 *st = s->stack;
-    u32 top = 1U;
-    st[0] = 0U;
-    st[1] = count;
-    st[2] = MNONE;
-    st[3] = 0U;
-    while (top > 0U) {
-        top -= 1U;
-        const u32 lo = st[4U * top];
-        const u32 hi = st[4U * top + 1U];
-        const u32 link = st[4U * top + 2U];
-        const u32 depth = st[4U * top + 3U];
-        const u32 me = made++;
-        MBin *node = &s->bins[me];
-        node->box = s->items[lo].box;
-        for (u32 i = lo + 1U; i < hi; ++i) {
-            node->box = box_join(node->box, s->items[i].box);
-        }
-        if (link != MNONE) {
-            MBin *parent = &s->bins[link >> 1U];
-            if ((link & 1U) == 0U) {
-                parent->left = me;
-            } else {
-                parent->right = me;
-            }
-        }
-        if (hi - lo <= NPH_PHYSICS_MESH_LEAF_TRIANGLES) {
-            node->first = lo;
-            node->count = hi - lo;
-            node->left = MNONE;
-            node->right = MNONE;
+    u32 front = lo;
+    u32 back = hi;
+    while (front < back) {
+        if (bin_of(b, &items[front]) < plane) {
+            front += 1U;
             continue;
         }
-        const u32 mid = split_items(s->items, lo, hi, depth);
-        node->first = 0U;
-        node->count = 0U;
-        u32 *r = &st[4U * top];
-        r[0] = mid;
-        r[1] = hi;
-        r[2] = (me << 1U) | 1U;
-        r[3] = depth + 1U;
-        r[4] = lo;
-        r[5] = mid;
-        r[6] = me << 1U;
-        r[7] = depth + 1U;
-        top += 2U;
+        back -= 1U;
+        const Item t = items[front];
+        items[front] = items[back];
+        items[back] = t;
     }
+    return front;
 }
 
-static void put_child(NphPhysicsMeshNode *n, u32 k, NphAABB b, u32 child)
+#define CENTER_ORDER(NAME, C)                                                                      \
+    static int NAME(const void *pa, const void *pb)                                                \
+    {                                                                                              \
+        const Item *a = (const Item *)pa;                                                          \
+        const Item *b = (const Item *)pb;                                                          \
+        if (a->center.C != b->center.C) {                                                          \
+            return a->center.C < b->center.C ? -1 : 1;                                             \
+        }                                                                                          \
+        return a->tri < b->tri ? -1 : (a->tri > b->tri ? 1 : 0);                                   \
+    }
+CENTER_ORDER(by_center_x, x)
+CENTER_ORDER(by_center_y, y)
+CENTER_ORDER(by_center_z, z)
+
+// Where to split items [lo, hi) (more than a leaf): both sides non-empty.
+static u32 choose_cut(Item *items, u32 lo, u32 hi, u32 depth)
+{
+    NphAABB cb = aabb_make(items[lo].center, items[lo].center);
+    for (u32 i = lo + 1U; i < hi; ++i) {
+        cb.min = v3_min(cb.min, items[i].center);
+        cb.max = v3_max(cb.max, items[i].center);
+    }
+    const NphVec3 ext = v3_sub(cb.max, cb.min);
+    Binning b;
+    b.axis = v3_max_index(ext);
+    const f32 span = v3_at(ext, b.axis);
+    const u32 half = lo + (hi - lo) / 2U;
+    if (!(span > 0.0F)) {
+        return half; // every centre in one point
+    }
+    if (depth >= SAH_DEPTH) {
+        // The median by centre along the axis (then triangle): a depth bound.
+        int (*order)(const void *, const void *) =
+            b.axis == 0U ? by_center_x : (b.axis == 1U ? by_center_y : by_center_z);
+        qsort(items + lo, hi - lo, sizeof(Item), order);
+        return half;
+    }
+    b.lo = v3_at(cb.min, b.axis);
+    b.scale = (f32)SAH_BINS / span;
+    fill_bins(&b, items, lo, hi);
+    const u32 plane = cheapest_plane(&b);
+    if (plane == 0U) {
+        return half;
+    }
+    const u32 cut = partition_items(items, lo, hi, &b, plane);
+    return cut > lo && cut < hi ? cut : half;
+}
+
+// The subtree over items [lo, hi), nodes numbered in preorder, left first.
+static u32 build_binary(Work *s, u32 lo, u32 hi, u32 depth)
+{
+    const u32 me = s->nbins++;
+    NphAABB box = s->items[lo].box;
+    for (u32 i = lo + 1U; i < hi; ++i) {
+        box = aabb_union(box, s->items[i].box);
+    }
+    s->bins[me].box = box;
+    if (hi - lo <= NPH_PHYSICS_MESH_LEAF_TRIANGLES) {
+        s->bins[me].first = lo;
+        s->bins[me].count = hi - lo;
+        s->bins[me].left = NIL;
+        s->bins[me].right = NIL;
+        return me;
+    }
+    const u32 cut = choose_cut(s->items, lo, hi, depth);
+    s->bins[me].first = 0U;
+    s->bins[me].count = 0U;
+    const u32 left = build_binary(s, lo, cut, depth + 1U);
+    const u32 right = build_binary(s, cut, hi, depth + 1U);
+    s->bins[me].left = left;
+    s->bins[me].right = right;
+    return me;
+}
+
+// ---------------------------------------------------------------------------
+// Four-wide nodes
+// ---------------------------------------------------------------------------
+static void write_slot(NphPhysicsMeshNode *n, u32 k, NphAABB b, u32 child)
 {
     n->min_x[k] = b.min.x;
     n->min_y[k] = b.min.y;
@@ -610,85 +612,83 @@ static void put_child(NphPhysicsMeshNode *n, u32 k, NphAABB b, u32 child)
     n->child[k] = child;
 }
 
-// An unused slot: the inverted box no query box overlaps.
-static void put_empty(NphPhysicsMeshNode *n, u32 k)
+// An unused slot: the inverted box that overlaps nothing.
+static void write_empty(NphPhysicsMeshNode *n, u32 k)
 {
-    NphAABB b;
-    b.min = m_v3(FLT_MAX, FLT_MAX, FLT_MAX);
-    b.max = m_v3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
-    put_child(n, k, b, NPH_PHYSICS_MESH_CHILD_EMPTY);
+    write_slot(n, k, aabb_make(v3_splat(FLT_MAX), v3_splat(-FLT_MAX)), NPH_PHYSICS_MESH_CHILD_EMPTY);
 }
 
-// Four-wide nodes from the binary tree: a node takes its two children and
-// opens the one of largest area among its inner children (first on ties)
-// until it holds four, keeping their order; nodes are numbered depth first
-// in child order. Returns the count.
-static u32 collapse_bins(const MeshWork *s, NphPhysicsMeshNode *out)
+// Binary node b's children, with the inner child of largest area (the first
+// of equals) opened in place until there are four or only leaves.
+static u32 open_children(const BNode *bins, u32 b, u32 kids[4])
 {
-    const MBin *bins = s->bins;
-    if (bins[0].count > 0U) {
-        put_child(&out[0], 0U, bins[0].box, leaf_code(bins[0].first, bins[0].count));
-        for (u32 k = 1U; k < 4U; ++k) {
-            put_empty(&out[0], k);
-        }
-        return 1U;
-    }
-    u32 made = 1U;
-    u32 *st = s->stack; // (binary node, output node)
-    u32 top = 1U;
-    st[0] = 0U;
-    st[1] = 0U;
-    while (top > 0U) {
-        top -= 1U;
-        const u32 b = st[2U * top];
-        const u32 o = st[2U * top + 1U];
-        u32 kids[4] = {bins[b].left, bins[b].right, MNONE, MNONE};
-        u32 n = 2U;
-        while (n < 4U) {
-            u32 pick = MNONE;
-            f32 widest = -1.0F;
-            for (u32 k = 0U; k < n; ++k) {
-                const MBin *c = &bins[kids[k]];
-                if (c->count == 0U && box_area(c->box) > widest) {
-                    widest = box_area(c->box);
-                    pick = k;
-                }
-            }
-            if (pick == MNONE) {
-                break;
-            }
-            const MBin *c = &bins[kids[pick]];
-            for (u32 k = n; k > pick + 1U; --k) {
-                kids[k] = kids[k - 1U];
-            }
-            kids[pick] = c->left;
-            kids[pick + 1U] = c->right;
-            n += 1U;
-        }
-        u32 code[4];
+    kids[0] = bins[b].left;
+    kids[1] = bins[b].right;
+    u32 n = 2U;
+    while (n < 4U) {
+        u32 widest = NIL;
+        f32 area = -1.0F;
         for (u32 k = 0U; k < n; ++k) {
-            const MBin *c = &bins[kids[k]];
-            code[k] = c->count > 0U ? leaf_code(c->first, c->count) : made++;
-            put_child(&out[o], k, c->box, code[k]);
-        }
-        for (u32 k = n; k < 4U; ++k) {
-            put_empty(&out[o], k);
-        }
-        for (u32 k = n; k > 0U; --k) {
-            if (!is_leaf(code[k - 1U])) {
-                st[2U * top] = kids[k - 1U];
-                st[2U * top + 1U] = code[k - 1U];
-                top += 1U;
+            const BNode *c = &bins[kids[k]];
+            if (c->count == 0U && half_area(c->box) > area) {
+                area = half_area(c->box);
+                widest = k;
             }
         }
+        if (widest == NIL) {
+            break;
+        }
+        const BNode *c = &bins[kids[widest]];
+        for (u32 k = n; k > widest + 1U; --k) {
+            kids[k] = kids[k - 1U];
+        }
+        kids[widest] = c->left;
+        kids[widest + 1U] = c->right;
+        n += 1U;
     }
-    return made;
+    return n;
+}
+
+// Output node o from binary node b: its inner children take the next
+// numbers, then each is written in turn.
+static void fold(Work *s, u32 b, u32 o)
+{
+    u32 kids[4];
+    const u32 n = open_children(s->bins, b, kids);
+    u32 code[4];
+    for (u32 k = 0U; k < n; ++k) {
+        const BNode *c = &s->bins[kids[k]];
+        code[k] = c->count > 0U ? mesh_leaf_code(c->first, c->count) : s->nnodes++;
+        write_slot(&s->nodes[o], k, c->box, code[k]);
+    }
+    for (u32 k = n; k < 4U; ++k) {
+        write_empty(&s->nodes[o], k);
+    }
+    for (u32 k = 0U; k < n; ++k) {
+        if (!mesh_child_is_leaf(code[k])) {
+            fold(s, kids[k], code[k]);
+        }
+    }
+}
+
+static void fold_tree(Work *s)
+{
+    s->nnodes = 1U;
+    const BNode *root = &s->bins[0];
+    if (root->count > 0U) {
+        write_slot(&s->nodes[0], 0U, root->box, mesh_leaf_code(root->first, root->count));
+        for (u32 k = 1U; k < 4U; ++k) {
+            write_empty(&s->nodes[0], k);
+        }
+        return;
+    }
+    fold(s, 0U, 0U);
 }
 
 // ---------------------------------------------------------------------------
-// Building (mesh.c:651-798)
+// Building
 // ---------------------------------------------------------------------------
-static bool mesh_desc_ok(const NphPhysicsMeshDesc *d)
+static bool desc_ok(const NphPhysicsMeshDesc *d)
 {
     if (d == NULL || d->vertices == NULL || d->indices == NULL || d->vertex_count < 3U ||
         d->triangle_count == 0U || d->triangle_count > NPH_PHYSICS_MESH_MAX_TRIANGLES ||
@@ -697,7 +697,7 @@ static bool mesh_desc_ok(const NphPhysicsMeshDesc *d)
         return false;
     }
     for (u32 i = 0U; i < d->vertex_count; ++i) {
-        if (!m_finite(d->vertices[i]) || !m_in_range(d->vertices[i])) {
+        if (!finite3(d->vertices[i]) || !bounded3(d->vertices[i])) {
             return false;
         }
     }
@@ -709,50 +709,74 @@ static bool mesh_desc_ok(const NphPhysicsMeshDesc *d)
     return true;
 }
 
-static void work_release(MeshWork *s)
+// The block's layout after its header: nodes, vertices, triangles, edge
+// bits, then the originals from the next 4-byte boundary.
+typedef struct MeshLayout {
+    u64 nodes, verts, tris, edges, originals, size;
+} MeshLayout;
+
+static MeshLayout mesh_layout(u32 nn, u32 nv, u32 nt)
 {
-    NphAllocator *a = s->a;
-    const u64 n = s->nv_in;
-    const u64 t = s->nt_in;
-    mesh_give(a, s->remap, n * 4U);
-    mesh_give(a, s->chain, n * 4U);
-    mesh_give(a, s->pos, n * sizeof(NphVec3));
-    mesh_give(a, s->tris, 3U * t * 4U);
-    mesh_give(a, s->orig, t * 4U);
-    mesh_give(a, s->flags, t);
-    mesh_give(a, s->ekeys, 3U * t * sizeof(MEdge));
-    mesh_give(a, s->items, t * sizeof(MItem));
-    mesh_give(a, s->bins, 2U * t * sizeof(MBin));
-    mesh_give(a, s->stack, 8U * t * 4U);
-    mesh_give(a, s->out_of, n * 4U);
-    mesh_give(a, s->nodes, (u64)node_cap(s->nt_in) * sizeof(NphPhysicsMeshNode));
+    MeshLayout l;
+    l.nodes = sizeof(NphPhysicsMesh);
+    l.verts = l.nodes + (u64)nn * sizeof(NphPhysicsMeshNode);
+    l.tris = l.verts + (u64)nv * sizeof(NphVec3);
+    l.edges = l.tris + 12ULL * nt;
+    l.originals = (l.edges + nt + 3U) & ~3ULL;
+    l.size = l.originals + 4ULL * nt;
+    return l;
 }
 
-static NphResult work_reserve(MeshWork *s)
+static NphPhysicsMesh *write_block(Work *s, u32 kept, u32 reps, u32 nv, bool two_sided,
+                                   NphResult *r)
 {
-    NphAllocator *a = s->a;
-    const u64 n = s->nv_in;
-    const u64 t = s->nt_in;
-    s->remap = (u32 *)mesh_take(a, n * 4U);
-    s->chain = (u32 *)mesh_take(a, n * 4U);
-    s->pos = (NphVec3 *)mesh_take(a, n * sizeof(NphVec3));
-    s->tris = (u32 *)mesh_take(a, 3U * t * 4U);
-    s->orig = (u32 *)mesh_take(a, t * 4U);
-    s->flags = (u8 *)mesh_take(a, t);
-    s->ekeys = (u64 *)mesh_take(a, 3U * t * sizeof(MEdge));
-    s->items = (MItem *)mesh_take(a, t * sizeof(MItem));
-    s->bins = (MBin *)mesh_take(a, 2U * t * sizeof(MBin));
-    s->stack = (u32 *)mesh_take(a, 8U * t * 4U);
-    s->out_of = (u32 *)mesh_take(a, n * 4U);
-    s->nodes = (NphPhysicsMeshNode *)mesh_take(a, (u64)node_cap(s->nt_in) *
-                                                       sizeof(NphPhysicsMeshNode));
-    if (s->remap == NULL || s->chain == NULL || s->pos == NULL || s->tris == NULL ||
-        s->orig == NULL || s->flags == NULL || s->ekeys == NULL || s->items == NULL ||
-        s->bins == NULL || s->stack == NULL || s->out_of == NULL || s->nodes == NULL) {
-        return NPH_RESULT_OUT_OF_MEMORY;
+    const MeshLayout l = mesh_layout(s->nnodes, nv, kept);
+    if (l.size > 0xFFFFFFFFULL) {
+        *r = NPH_RESULT_INVALID_ARGUMENT; // too large for one block
+        return NULL;
     }
-    memset(s->flags, 0, (size_t)t);
-    return NPH_RESULT_OK;
+    NphPhysicsMesh *m = (NphPhysicsMesh *)take(s->a, l.size);
+    if (m == NULL) {
+        *r = NPH_RESULT_OUT_OF_MEMORY;
+        return NULL;
+    }
+    memset(m, 0, (size_t)l.size);
+    memcpy((u8 *)m + l.nodes, s->nodes, (size_t)s->nnodes * sizeof(NphPhysicsMeshNode));
+    NphVec3 *verts = (NphVec3 *)((u8 *)m + l.verts);
+    u32 *tris = (u32 *)((u8 *)m + l.tris);
+    u8 *edges = (u8 *)m + l.edges;
+    u32 *orig = (u32 *)((u8 *)m + l.originals);
+    for (u32 i = 0U; i < reps; ++i) {
+        if (s->out_of[i] != NIL) {
+            verts[s->out_of[i]] = s->pos[i];
+        }
+    }
+    NphAABB bounds = aabb_make(verts[0], verts[0]);
+    for (u32 i = 1U; i < nv; ++i) {
+        bounds.min = v3_min(bounds.min, verts[i]);
+        bounds.max = v3_max(bounds.max, verts[i]);
+    }
+    for (u32 i = 0U; i < kept; ++i) {
+        const u32 t = s->items[i].tri;
+        for (u32 k = 0U; k < 3U; ++k) {
+            tris[3ULL * i + k] = s->out_of[s->tris[3ULL * t + k]];
+        }
+        edges[i] = s->active[t];
+        orig[i] = s->orig[t];
+    }
+    m->byte_size = (u32)l.size;
+    m->vertex_count = nv;
+    m->triangle_count = kept;
+    m->node_count = s->nnodes;
+    m->nodes_offset = (u32)l.nodes;
+    m->vertices_offset = (u32)l.verts;
+    m->triangles_offset = (u32)l.tris;
+    m->edges_offset = (u32)l.edges;
+    m->originals_offset = (u32)l.originals;
+    m->flags = two_sided ? NPH_PHYSICS_MESH_TWO_SIDED : 0U;
+    m->bounds = bounds;
+    *r = NPH_RESULT_OK;
+    return m;
 }
 
 NphResult nph_physics_mesh_build(const NphPhysicsMeshDesc *desc, NphAllocator *alloc,
@@ -762,554 +786,329 @@ NphResult nph_physics_mesh_build(const NphPhysicsMeshDesc *desc, NphAllocator *a
         return NPH_RESULT_INVALID_ARGUMENT;
     }
     *out = NULL;
-    if (!mesh_desc_ok(desc)) {
+    if (!desc_ok(desc)) {
         return NPH_RESULT_INVALID_ARGUMENT;
     }
     f32 extent = 0.0F;
     for (u32 i = 0U; i < desc->vertex_count; ++i) {
-        const NphVec3 p = desc->vertices[i];
-        extent = m_max(extent, m_max(fabsf(p.x), m_max(fabsf(p.y), fabsf(p.z))));
+        const NphVec3 a = v3_abs(desc->vertices[i]);
+        extent = nph_maxf(extent, nph_maxf(a.x, nph_maxf(a.y, a.z)));
     }
-    // At least 2^-29 of the extent, so cell indices fit 32 bits.
+    // Default 1e-6 of the extent; never below 2^-29 of it, so cell indices
+    // fit 32 bits.
     f32 tol = desc->weld_tolerance;
     if (tol == 0.0F) {
         tol = 1e-6F * extent;
     }
     if (tol > 0.0F) {
-        tol = m_max(tol, extent * 1.8626451e-9F);
+        tol = nph_maxf(tol, extent * 1.8626451e-9F);
     }
     const f32 angle = desc->active_edge_angle > 0.0F ? desc->active_edge_angle
                                                      : NPH_PHYSICS_MESH_DEFAULT_ACTIVE_ANGLE;
     const bool two_sided = desc->two_sided != 0;
 
-    MeshWork s;
+    Work s;
     memset(&s, 0, sizeof s);
     s.a = alloc;
     s.nv_in = desc->vertex_count;
     s.nt_in = desc->triangle_count;
-    NphResult r = work_reserve(&s);
+    NphResult r = work_alloc(&s);
     u32 reps = 0U;
     if (r == NPH_RESULT_OK) {
-        r = weld_vertices(&s, desc->vertices, tol, &reps);
+        r = weld(&s, desc->vertices, tol, &reps);
     }
     const u32 kept = r == NPH_RESULT_OK ? keep_triangles(&s, desc->indices) : 0U;
     if (r == NPH_RESULT_OK && kept == 0U) {
         r = NPH_RESULT_INVALID_ARGUMENT;
     }
     if (r != NPH_RESULT_OK) {
-        work_release(&s);
+        work_free(&s);
         return r;
     }
-    mark_edges(&s, kept, cosf(angle), two_sided);
+    mark_active_edges(&s, kept, reps, cosf(angle), two_sided);
     for (u32 t = 0U; t < kept; ++t) {
-        const u32 *v = s.tris + 3ULL * t;
-        MItem *it = &s.items[t];
-        it->box.min = m_vmin(m_vmin(s.pos[v[0]], s.pos[v[1]]), s.pos[v[2]]);
-        it->box.max = m_vmax(m_vmax(s.pos[v[0]], s.pos[v[1]]), s.pos[v[2]]);
-        it->center = m_v3((it->box.min.x + it->box.max.x) * 0.5F,
-                          (it->box.min.y + it->box.max.y) * 0.5F,
-                          (it->box.min.z + it->box.max.z) * 0.5F);
-        it->tri = t;
+        NphVec3 p[3];
+        for (u32 k = 0U; k < 3U; ++k) {
+            p[k] = s.pos[s.tris[3ULL * t + k]];
+        }
+        s.items[t].box = corners_box(p);
+        s.items[t].center = aabb_center(s.items[t].box);
+        s.items[t].tri = t;
     }
-    build_bins(&s, kept);
-    // Output vertices: the used representatives, numbered by first use in
-    // leaf order.
+    s.nbins = 0U;
+    build_binary(&s, 0U, kept, 0U);
     for (u32 i = 0U; i < reps; ++i) {
-        s.out_of[i] = MNONE;
+        s.out_of[i] = NIL;
     }
     u32 nv = 0U;
     for (u32 i = 0U; i < kept; ++i) {
-        const u32 *v = s.tris + 3ULL * s.items[i].tri;
+        const u32 *c = s.tris + 3ULL * s.items[i].tri;
         for (u32 k = 0U; k < 3U; ++k) {
-            if (s.out_of[v[k]] == MNONE) {
-                s.out_of[v[k]] = nv++;
+            if (s.out_of[c[k]] == NIL) {
+                s.out_of[c[k]] = nv++;
             }
         }
     }
-    const u32 nn = collapse_bins(&s, s.nodes);
-    const u64 o_nodes = sizeof(NphPhysicsMesh);
-    const u64 o_verts = o_nodes + (u64)nn * sizeof(NphPhysicsMeshNode);
-    const u64 o_tris = o_verts + (u64)nv * sizeof(NphVec3);
-    const u64 o_edges = o_tris + 12ULL * kept;
-    const u64 o_orig = (o_edges + kept + 3U) & ~3ULL;
-    const u64 size = o_orig + 4ULL * kept;
-    if (size > 0xFFFFFFFFULL) {
-        work_release(&s);
-        return NPH_RESULT_INVALID_ARGUMENT;
-    }
-    NphPhysicsMesh *m = (NphPhysicsMesh *)mesh_take(alloc, size);
-    if (m == NULL) {
-        work_release(&s);
-        return NPH_RESULT_OUT_OF_MEMORY;
-    }
-    memset(m, 0, (size_t)size);
-    memcpy((u8 *)m + o_nodes, s.nodes, (size_t)nn * sizeof(NphPhysicsMeshNode));
-    NphVec3 *verts = (NphVec3 *)((u8 *)m + o_verts);
-    u32 *tris = (u32 *)((u8 *)m + o_tris);
-    u8 *edges = (u8 *)m + o_edges;
-    u32 *orig = (u32 *)((u8 *)m + o_orig);
-    for (u32 i = 0U; i < reps; ++i) {
-        if (s.out_of[i] != MNONE) {
-            verts[s.out_of[i]] = s.pos[i];
-        }
-    }
-    NphAABB bounds;
-    bounds.min = verts[0];
-    bounds.max = verts[0];
-    for (u32 i = 1U; i < nv; ++i) {
-        bounds.min = m_vmin(bounds.min, verts[i]);
-        bounds.max = m_vmax(bounds.max, verts[i]);
-    }
-    for (u32 i = 0U; i < kept; ++i) {
-        const u32 t = s.items[i].tri;
-        for (u32 k = 0U; k < 3U; ++k) {
-            tris[3ULL * i + k] = s.out_of[s.tris[3ULL * t + k]];
-        }
-        edges[i] = s.flags[t];
-        orig[i] = s.orig[t];
-    }
-    m->byte_size = (u32)size;
-    m->vertex_count = nv;
-    m->triangle_count = kept;
-    m->node_count = nn;
-    m->nodes_offset = (u32)o_nodes;
-    m->vertices_offset = (u32)o_verts;
-    m->triangles_offset = (u32)o_tris;
-    m->edges_offset = (u32)o_edges;
-    m->originals_offset = (u32)o_orig;
-    m->flags = two_sided ? NPH_PHYSICS_MESH_TWO_SIDED : 0U;
-    m->bounds = bounds;
-    work_release(&s);
+    fold_tree(&s);
+    NphPhysicsMesh *m = write_block(&s, kept, reps, nv, two_sided, &r);
+    work_free(&s);
     *out = m;
-    return NPH_RESULT_OK;
+    return r;
 }
 
 void nph_physics_mesh_free(NphPhysicsMesh *mesh, NphAllocator *alloc)
 {
     if (mesh != NULL && alloc != NULL) {
-        mesh_give(alloc, mesh, mesh->byte_size);
+        give(alloc, mesh, mesh->byte_size);
     }
 }
 
 void nph_physics_mesh_triangle(const NphPhysicsMesh *m, u32 t, NphVec3 out[3])
 {
-    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
-    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset) + 3ULL * t;
-    out[0] = v[tri[0]];
-    out[1] = v[tri[1]];
-    out[2] = v[tri[2]];
+    mesh_corners(m, t, out);
 }
 
 // ---------------------------------------------------------------------------
-// Validation (mesh.c:829-996): the same checks, so a block is accepted
-// exactly when the reference accepts it.
+// Validation: accepts exactly the blocks the reference's validator accepts
+// (mesh.c:829-996). Layout, counts and flags; vertices finite inside the
+// bounds; indices and edge bits in range; then the tree, depth first in slot
+// order: every node once and after its parent, inside its parent's slot
+// box, its slots filled from the front, the empty ones holding the empty
+// box; the leaves covering the triangles in order inside their slot boxes;
+// and no node so deep, or with so many siblings waiting above it, that a
+// query's stack of NPH_BVH4_STACK codes could overflow.
 // ---------------------------------------------------------------------------
-#define MREJECT(...)                                                                               \
-    do {                                                                                           \
-        fprintf(stderr, "nepheline_amd: mesh validate: " __VA_ARGS__);                             \
-        fputc('\n', stderr);                                                                       \
-        return false;                                                                              \
-    } while (0)
+static void complain(const char *what, u32 a, u32 b)
+{
+    fprintf(stderr, "nepheline_amd: mesh block rejected: %s (%u, %u)\n", what, a, b);
+}
 
-typedef struct MPending {
-    u32 child;
-    u32 parent;
-    u32 slot;
-    u32 depth;
-} MPending;
-
-static bool span_ok(u64 offset, u64 count, u64 elem, u64 align, u64 size)
+static bool fits(u64 offset, u64 count, u64 elem, u64 align, u64 size)
 {
     return offset >= sizeof(NphPhysicsMesh) && offset % align == 0U && count <= size &&
            offset + count * elem <= size;
 }
 
-static bool slot_holds_point(const NphPhysicsMeshNode *n, u32 k, NphVec3 p)
+static bool slot_empty_box(const NphPhysicsMeshNode *n, u32 k)
 {
-    return p.x >= n->min_x[k] && p.x <= n->max_x[k] && p.y >= n->min_y[k] && p.y <= n->max_y[k] &&
-           p.z >= n->min_z[k] && p.z <= n->max_z[k];
+    return n->min_x[k] == FLT_MAX && n->min_y[k] == FLT_MAX && n->min_z[k] == FLT_MAX &&
+           n->max_x[k] == -FLT_MAX && n->max_y[k] == -FLT_MAX && n->max_z[k] == -FLT_MAX;
 }
 
-static bool slot_holds_node(const NphPhysicsMeshNode *n, u32 k, const NphPhysicsMeshNode *c)
+static bool slot_holds(const NphPhysicsMeshNode *n, u32 k, NphVec3 p)
 {
+    const NphAABB b = bvh4_slot_box(n, k);
+    return p.x >= b.min.x && p.x <= b.max.x && p.y >= b.min.y && p.y <= b.max.y &&
+           p.z >= b.min.z && p.z <= b.max.z;
+}
+
+// Every filled slot of `child` inside slot k of `n`.
+static bool slot_holds_node(const NphPhysicsMeshNode *n, u32 k, const NphPhysicsMeshNode *child)
+{
+    const NphAABB outer = bvh4_slot_box(n, k);
     for (u32 j = 0U; j < 4U; ++j) {
-        if (c->child[j] == NPH_PHYSICS_MESH_CHILD_EMPTY) {
-            continue;
-        }
-        if (c->min_x[j] < n->min_x[k] || c->max_x[j] > n->max_x[k] || c->min_y[j] < n->min_y[k] ||
-            c->max_y[j] > n->max_y[k] || c->min_z[j] < n->min_z[k] || c->max_z[j] > n->max_z[k]) {
+        if (child->child[j] != NPH_PHYSICS_MESH_CHILD_EMPTY &&
+            !aabb_contains(outer, bvh4_slot_box(child, j))) {
             return false;
         }
     }
     return true;
 }
 
-static bool slot_is_empty_box(const NphPhysicsMeshNode *n, u32 k)
+typedef struct TreeCheck {
+    const NphPhysicsMeshNode *nodes;
+    const NphVec3 *v;
+    const u32 *tri;
+    u32 nn, nt;
+    u32 seen;     // nodes reached
+    u32 next_tri; // the triangle the next leaf must start at
+} TreeCheck;
+
+// Child `code` in slot `slot` of node `parent` (NIL for the root) at
+// `depth`, with `waiting` codes of later siblings pending above it.
+static bool check_subtree(TreeCheck *c, u32 code, u32 parent, u32 slot, u32 depth, u32 waiting)
 {
-    return n->min_x[k] == FLT_MAX && n->min_y[k] == FLT_MAX && n->min_z[k] == FLT_MAX &&
-           n->max_x[k] == -FLT_MAX && n->max_y[k] == -FLT_MAX && n->max_z[k] == -FLT_MAX;
+    if (mesh_child_is_leaf(code)) {
+        const u32 first = mesh_leaf_first(code);
+        const u32 count = mesh_leaf_count(code);
+        if (first != c->next_tri || (u64)first + count > c->nt) {
+            complain("leaf out of order", first, count);
+            return false;
+        }
+        for (u32 t = first; t < first + count; ++t) {
+            for (u32 j = 0U; j < 3U; ++j) {
+                if (!slot_holds(&c->nodes[parent], slot, c->v[c->tri[3ULL * t + j]])) {
+                    complain("triangle outside its leaf's box", t, j);
+                    return false;
+                }
+            }
+        }
+        c->next_tri = first + count;
+        return true;
+    }
+    if (code >= c->nn ||
+        (parent != NIL &&
+         (code <= parent || !slot_holds_node(&c->nodes[parent], slot, &c->nodes[code])))) {
+        complain("node out of order or outside its slot", code, parent);
+        return false;
+    }
+    c->seen += 1U;
+    if (c->seen > c->nn) {
+        complain("node reached twice", code, c->nn);
+        return false;
+    }
+    const NphPhysicsMeshNode *n = &c->nodes[code];
+    u32 used = 0U;
+    for (u32 k = 0U; k < 4U; ++k) {
+        if (n->child[k] == NPH_PHYSICS_MESH_CHILD_EMPTY) {
+            if (!slot_empty_box(n, k)) {
+                complain("empty slot without the empty box", code, k);
+                return false;
+            }
+            continue;
+        }
+        const NphAABB b = bvh4_slot_box(n, k);
+        if (used != k || !bounded3(b.min) || !bounded3(b.max) || !(b.min.x <= b.max.x) ||
+            !(b.min.y <= b.max.y) || !(b.min.z <= b.max.z)) {
+            complain("slot after an empty one, or its box inverted or out of range", code, k);
+            return false;
+        }
+        used += 1U;
+    }
+    if (used == 0U) {
+        complain("node without children", code, 0U);
+        return false;
+    }
+    if (3U * depth + 4U > NPH_BVH4_STACK || waiting + used > NPH_BVH4_STACK) {
+        complain("tree deeper than a query's stack", depth, waiting + used);
+        return false;
+    }
+    for (u32 k = 0U; k < used; ++k) {
+        if (!check_subtree(c, n->child[k], code, k, depth + 1U, waiting + (used - 1U - k))) {
+            return false;
+        }
+    }
+    return true;
 }
 
 bool nph_physics_mesh_validate(const void *block, u32 size)
 {
     const u64 head = sizeof(NphPhysicsMesh);
     if (block == NULL || size < head || ((uintptr_t)block & 15U) != 0U) {
-        MREJECT("no 16-byte aligned header in %u bytes", size);
+        complain("no 16-byte aligned header", size, 0U);
+        return false;
     }
     const NphPhysicsMesh *m = (const NphPhysicsMesh *)block;
     const u64 bytes = m->byte_size;
     if (bytes < head || bytes > size) {
-        MREJECT("byte_size %u outside the %u-byte block", m->byte_size, size);
+        complain("byte_size against the block", m->byte_size, size);
+        return false;
     }
     const u32 nt = m->triangle_count;
     const u32 nv = m->vertex_count;
     const u32 nn = m->node_count;
     if (nt == 0U || nt > NPH_PHYSICS_MESH_MAX_TRIANGLES || nv < 3U || nn == 0U ||
         nn > node_cap(nt) || (m->flags & ~(u32)NPH_PHYSICS_MESH_TWO_SIDED) != 0U) {
-        MREJECT("counts triangles=%u vertices=%u nodes=%u flags=%u", nt, nv, nn, m->flags);
+        complain("triangle, vertex or node count, or flags", nt, nn);
+        return false;
     }
-    if (!span_ok(m->nodes_offset, nn, sizeof(NphPhysicsMeshNode), 16U, bytes) ||
-        !span_ok(m->vertices_offset, nv, sizeof(NphVec3), 4U, bytes) ||
-        !span_ok(m->triangles_offset, 3ULL * nt, 4U, 4U, bytes) ||
-        !span_ok(m->edges_offset, nt, 1U, 1U, bytes) ||
-        !span_ok(m->originals_offset, nt, 4U, 4U, bytes)) {
-        MREJECT("arrays outside the %u-byte block", m->byte_size);
+    if (!fits(m->nodes_offset, nn, sizeof(NphPhysicsMeshNode), 16U, bytes) ||
+        !fits(m->vertices_offset, nv, sizeof(NphVec3), 4U, bytes) ||
+        !fits(m->triangles_offset, 3ULL * nt, 4U, 4U, bytes) ||
+        !fits(m->edges_offset, nt, 1U, 1U, bytes) || !fits(m->originals_offset, nt, 4U, 4U, bytes)) {
+        complain("an array outside the block or misaligned", m->byte_size, 0U);
+        return false;
     }
-    if (!m_in_range(m->bounds.min) || !m_in_range(m->bounds.max)) {
-        MREJECT("bounds not finite or beyond the coordinate limit");
-    }
-    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
     const NphAABB bx = m->bounds;
+    if (!bounded3(bx.min) || !bounded3(bx.max)) {
+        complain("bounds out of range", 0U, 0U);
+        return false;
+    }
+    const NphVec3 *v = mesh_vertices(m);
     for (u32 i = 0U; i < nv; ++i) {
         const NphVec3 p = v[i];
-        if (!m_finite(p) || !(p.x >= bx.min.x && p.x <= bx.max.x && p.y >= bx.min.y &&
-                              p.y <= bx.max.y && p.z >= bx.min.z && p.z <= bx.max.z)) {
-            MREJECT("vertex %u not finite or outside the bounds", i);
+        if (!finite3(p) || !(p.x >= bx.min.x && p.x <= bx.max.x && p.y >= bx.min.y &&
+                             p.y <= bx.max.y && p.z >= bx.min.z && p.z <= bx.max.z)) {
+            complain("vertex outside the bounds", i, 0U);
+            return false;
         }
     }
-    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset);
-    const u8 *edges = (const u8 *)m + m->edges_offset;
+    const u32 *tri = mesh_triangles(m);
     for (u64 i = 0U; i < 3ULL * nt; ++i) {
         if (tri[i] >= nv) {
-            MREJECT("triangle %u's vertex %u out of range", (u32)(i / 3U), tri[i]);
+            complain("triangle corner out of range", (u32)(i / 3U), tri[i]);
+            return false;
         }
     }
+    const u8 *edges = mesh_edges(m);
     for (u32 t = 0U; t < nt; ++t) {
         if ((edges[t] & ~(u32)NPH_PHYSICS_MESH_EDGES) != 0U) {
-            MREJECT("triangle %u's edge flags %u", t, (u32)edges[t]);
+            complain("edge bits", t, edges[t]);
+            return false;
         }
     }
-    // Depth first in child order: each node once, after its parent; leaves
-    // covering the triangles in order; every box holding what it contains;
-    // no deeper than a query's stack (three siblings a level plus four).
-    const NphPhysicsMeshNode *nodes =
-        (const NphPhysicsMeshNode *)((const u8 *)m + m->nodes_offset);
-    MPending st[CHECK_STACK];
-    u32 top = 0U;
-    st[top].child = 0U;
-    st[top].parent = MNONE;
-    st[top].slot = 0U;
-    st[top].depth = 0U;
-    top += 1U;
-    u32 seen = 0U;
-    u32 next_tri = 0U;
-    while (top > 0U) {
-        const MPending e = st[--top];
-        if (is_leaf(e.child)) {
-            const u32 first = (e.child & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
-            const u32 count = (e.child & 3U) + 1U;
-            if (first != next_tri || (u64)first + count > nt) {
-                MREJECT("leaf %u+%u out of order", first, count);
-            }
-            for (u32 t = first; t < first + count; ++t) {
-                for (u32 j = 0U; j < 3U; ++j) {
-                    if (!slot_holds_point(&nodes[e.parent], e.slot, v[tri[3ULL * t + j]])) {
-                        MREJECT("triangle %u outside its leaf's box", t);
-                    }
-                }
-            }
-            next_tri = first + count;
-            continue;
-        }
-        const u32 idx = e.child;
-        if (idx >= nn || (e.parent != MNONE && (idx <= e.parent || !slot_holds_node(
-                                                                      &nodes[e.parent], e.slot,
-                                                                      &nodes[idx])))) {
-            MREJECT("node %u out of order or outside its parent's box", idx);
-        }
-        seen += 1U;
-        if (seen > nn) {
-            MREJECT("a node reached twice");
-        }
-        const NphPhysicsMeshNode *n = &nodes[idx];
-        u32 used = 0U;
-        for (u32 k = 0U; k < 4U; ++k) {
-            if (n->child[k] == NPH_PHYSICS_MESH_CHILD_EMPTY) {
-                if (!slot_is_empty_box(n, k)) {
-                    MREJECT("node %u: empty slot %u's box is not the empty box", idx, k);
-                }
-                continue;
-            }
-            if (used != k) {
-                MREJECT("node %u: a child after an empty slot", idx);
-            }
-            const NphVec3 lo = m_v3(n->min_x[k], n->min_y[k], n->min_z[k]);
-            const NphVec3 hi = m_v3(n->max_x[k], n->max_y[k], n->max_z[k]);
-            if (!m_in_range(lo) || !m_in_range(hi) || !(lo.x <= hi.x) || !(lo.y <= hi.y) ||
-                !(lo.z <= hi.z)) {
-                MREJECT("node %u: child %u's box inverted or not finite", idx, k);
-            }
-            used += 1U;
-        }
-        if (used == 0U) {
-            MREJECT("node %u is empty", idx);
-        }
-        if (3U * e.depth + 4U > CHECK_STACK || top + used > CHECK_STACK) {
-            MREJECT("deeper than the query stack");
-        }
-        for (u32 k = used; k > 0U; --k) {
-            st[top].child = n->child[k - 1U];
-            st[top].parent = idx;
-            st[top].slot = k - 1U;
-            st[top].depth = e.depth + 1U;
-            top += 1U;
-        }
+    TreeCheck c;
+    c.nodes = mesh_nodes(m);
+    c.v = v;
+    c.tri = tri;
+    c.nn = nn;
+    c.nt = nt;
+    c.seen = 0U;
+    c.next_tri = 0U;
+    if (!check_subtree(&c, 0U, NIL, 0U, 0U, 0U)) {
+        return false;
     }
-    if (seen != nn || next_tri != nt) {
-        MREJECT("%u of %u nodes reached, %u of %u triangles in leaves", seen, nn, next_tri, nt);
+    if (c.seen != nn || c.next_tri != nt) {
+        complain("nodes reached, triangles in leaves", c.seen, c.next_tri);
+        return false;
     }
     return true;
 }
 
 // ---------------------------------------------------------------------------
-// Queries (mesh.c:998-1199): the box overlap in tree order, and the closest
-// ray hit by the watertight test of Woop, Benthin and Wald ("Watertight
-// Ray/Triangle Intersection", JCGT 2013), nearest child first.
+// Queries (mesh.c:998-1192) over the walks of nph_bvh4.h. A block deeper
+// than the walks' stack, which the builder never writes and the validator
+// refuses, ends the walk with a message instead of a partial answer given
+// as a whole one.
 // ---------------------------------------------------------------------------
 void nph_physics_mesh_overlap(const NphPhysicsMesh *m, NphAABB box, NphPhysicsMeshVisit visit,
                               void *user)
 {
-    const NphPhysicsMeshNode *nodes =
-        (const NphPhysicsMeshNode *)((const u8 *)m + m->nodes_offset);
-    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset);
-    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
-    u32 st[CHECK_STACK];
-    u32 top = 0U;
-    st[top++] = 0U;
-    while (top > 0U) {
-        const u32 code = st[--top];
-        if (!is_leaf(code)) {
-            const NphPhysicsMeshNode *n = &nodes[code];
-            // Pushed last to first, so the children come off in order; an
-            // empty slot's inverted box overlaps nothing.
-            for (u32 k = 4U; k > 0U; --k) {
-                const u32 j = k - 1U;
-                if (n->min_x[j] <= box.max.x && n->max_x[j] >= box.min.x &&
-                    n->min_y[j] <= box.max.y && n->max_y[j] >= box.min.y &&
-                    n->min_z[j] <= box.max.z && n->max_z[j] >= box.min.z &&
-                    n->child[j] != NPH_PHYSICS_MESH_CHILD_EMPTY && top < CHECK_STACK) {
-                    st[top++] = n->child[j];
-                }
-            }
-            continue;
-        }
-        const u32 first = (code & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
-        const u32 last = first + (code & 3U) + 1U;
-        for (u32 t = first; t < last; ++t) {
-            const NphVec3 a = v[tri[3ULL * t]];
-            const NphVec3 b = v[tri[3ULL * t + 1U]];
-            const NphVec3 c = v[tri[3ULL * t + 2U]];
-            const NphVec3 lo = m_vmin(m_vmin(a, b), c);
-            const NphVec3 hi = m_vmax(m_vmax(a, b), c);
-            if (lo.x <= box.max.x && hi.x >= box.min.x && lo.y <= box.max.y &&
-                hi.y >= box.min.y && lo.z <= box.max.z && hi.z >= box.min.z && !visit(user, t)) {
-                return;
-            }
+    NphBoxWalk w;
+    box_walk_begin(&w, m, box);
+    for (u32 t = box_walk_next(&w); t != NIL; t = box_walk_next(&w)) {
+        if (!visit(user, t)) {
+            return;
         }
     }
-}
-
-// The ray in Woop et al.'s frame: kz the axis of the direction's largest
-// magnitude, kx and ky the next two (swapped for a negative dz, keeping the
-// winding), and the shear that makes the ray run along +z.
-typedef struct WRay {
-    NphVec3 origin;
-    u32 kx, ky, kz;
-    f32 sx, sy, sz;
-} WRay;
-
-static WRay wray_make(NphVec3 origin, NphVec3 delta)
-{
-    WRay r;
-    r.origin = origin;
-    const f32 ax = fabsf(delta.x), ay = fabsf(delta.y), az = fabsf(delta.z);
-    r.kz = ax >= ay ? (ax >= az ? 0U : 2U) : (ay >= az ? 1U : 2U);
-    r.kx = (r.kz + 1U) % 3U;
-    r.ky = (r.kx + 1U) % 3U;
-    const f32 dz = m_at(delta, r.kz);
-    if (dz < 0.0F) {
-        const u32 t = r.kx;
-        r.kx = r.ky;
-        r.ky = t;
-    }
-    r.sx = m_at(delta, r.kx) / dz;
-    r.sy = m_at(delta, r.ky) / dz;
-    r.sz = 1.0F / dz;
-    return r;
-}
-
-// The fraction where the ray crosses triangle (pa, pb, pc), negative for a
-// miss; *back when it crosses from behind. An edge through the ray (a zero
-// barycentric) is decided in f64, so two triangles sharing it never both
-// miss (Woop et al., section 3).
-static f32 wray_hit(const WRay *r, NphVec3 pa, NphVec3 pb, NphVec3 pc, bool *back)
-{
-    const NphVec3 a = m_sub(pa, r->origin);
-    const NphVec3 b = m_sub(pb, r->origin);
-    const NphVec3 c = m_sub(pc, r->origin);
-    const f32 az = m_at(a, r->kz), bz = m_at(b, r->kz), cz = m_at(c, r->kz);
-    const f32 ax = m_at(a, r->kx) - r->sx * az;
-    const f32 ay = m_at(a, r->ky) - r->sy * az;
-    const f32 bx = m_at(b, r->kx) - r->sx * bz;
-    const f32 by = m_at(b, r->ky) - r->sy * bz;
-    const f32 cx = m_at(c, r->kx) - r->sx * cz;
-    const f32 cy = m_at(c, r->ky) - r->sy * cz;
-    f32 u = cx * by - cy * bx;
-    f32 v = ax * cy - ay * cx;
-    f32 w = bx * ay - by * ax;
-    if (u == 0.0F || v == 0.0F || w == 0.0F) {
-        u = (f32)((f64)cx * (f64)by - (f64)cy * (f64)bx);
-        v = (f32)((f64)ax * (f64)cy - (f64)ay * (f64)cx);
-        w = (f32)((f64)bx * (f64)ay - (f64)by * (f64)ax);
-    }
-    if ((u < 0.0F || v < 0.0F || w < 0.0F) && (u > 0.0F || v > 0.0F || w > 0.0F)) {
-        return -1.0F;
-    }
-    const f32 det = u + v + w;
-    if (det == 0.0F) {
-        return -1.0F;
-    }
-    const f32 t = (u * r->sz * az + v * r->sz * bz + w * r->sz * cz) / det;
-    if (!(t >= 0.0F && t <= 1.0F)) {
-        return -1.0F;
-    }
-    *back = det < 0.0F;
-    return t;
-}
-
-// A finite reciprocal: an axis-parallel ray gets +-1e30, whose products
-// stay ordered.
-static f32 recip_finite(f32 d)
-{
-    if (fabsf(d) < 1e-30F) {
-        return d < 0.0F ? -1e30F : 1e30F;
-    }
-    return 1.0F / d;
-}
-
-// The slab test of the ray against a node's four child boxes: each child's
-// entry fraction, FLT_MAX for a miss (enter after leave, or beyond `limit`).
-static void slab_enter(const NphPhysicsMeshNode *n, NphVec3 o, NphVec3 inv, f32 limit,
-                       f32 out[4])
-{
-    for (u32 k = 0U; k < 4U; ++k) {
-        const f32 t1x = (n->min_x[k] - o.x) * inv.x;
-        const f32 t2x = (n->max_x[k] - o.x) * inv.x;
-        const f32 t1y = (n->min_y[k] - o.y) * inv.y;
-        const f32 t2y = (n->max_y[k] - o.y) * inv.y;
-        const f32 t1z = (n->min_z[k] - o.z) * inv.z;
-        const f32 t2z = (n->max_z[k] - o.z) * inv.z;
-        const f32 enter = m_max(m_max(m_min(t1x, t2x), m_min(t1y, t2y)),
-                                m_max(m_min(t1z, t2z), 0.0F));
-        const f32 leave = m_min(m_min(m_max(t1x, t2x), m_max(t1y, t2y)),
-                                m_min(m_max(t1z, t2z), limit));
-        out[k] = enter <= leave ? enter : FLT_MAX;
+    if (w.overflow != 0U) {
+        fprintf(stderr, "nepheline_amd: mesh overlap: the tree is deeper than %u pending "
+                        "children (an unvalidated block); the walk stopped early\n",
+                NPH_BVH4_STACK);
     }
 }
 
 bool nph_physics_mesh_raycast(const NphPhysicsMesh *m, NphVec3 origin, NphVec3 delta,
                               NphPhysicsMeshHit *hit)
 {
-    if (!(m_dot(delta, delta) > 0.0F)) {
+    u32 overflow = 0U;
+    const NphBvh4Hit h = bvh4_raycast(m, origin, delta, &overflow);
+    if (overflow != 0U) {
+        fprintf(stderr, "nepheline_amd: mesh raycast: the tree is deeper than %u pending "
+                        "children (an unvalidated block); no hit reported\n",
+                NPH_BVH4_STACK);
         return false;
     }
-    const NphPhysicsMeshNode *nodes =
-        (const NphPhysicsMeshNode *)((const u8 *)m + m->nodes_offset);
-    const u32 *tri = (const u32 *)((const u8 *)m + m->triangles_offset);
-    const NphVec3 *v = (const NphVec3 *)((const u8 *)m + m->vertices_offset);
-    const bool two_sided = (m->flags & NPH_PHYSICS_MESH_TWO_SIDED) != 0U;
-    const WRay ray = wray_make(origin, delta);
-    const NphVec3 inv = m_v3(recip_finite(delta.x), recip_finite(delta.y), recip_finite(delta.z));
-    f32 best = FLT_MAX;
-    u32 best_tri = MNONE;
-    bool best_back = false;
-    u32 st[CHECK_STACK];
-    f32 st_enter[CHECK_STACK];
-    u32 top = 0U;
-    st[top] = 0U;
-    st_enter[top] = 0.0F;
-    top += 1U;
-    while (top > 0U) {
-        top -= 1U;
-        const u32 code = st[top];
-        if (st_enter[top] > best) {
-            continue;
-        }
-        if (is_leaf(code)) {
-            const u32 first = (code & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
-            const u32 last = first + (code & 3U) + 1U;
-            for (u32 t = first; t < last; ++t) {
-                bool back = false;
-                const f32 f = wray_hit(&ray, v[tri[3ULL * t]], v[tri[3ULL * t + 1U]],
-                                       v[tri[3ULL * t + 2U]], &back);
-                if (f >= 0.0F && f < best && (two_sided || !back)) {
-                    best = f; // equal fractions keep the first found
-                    best_tri = t;
-                    best_back = back;
-                }
-            }
-            continue;
-        }
-        const NphPhysicsMeshNode *n = &nodes[code];
-        f32 enter[4];
-        slab_enter(n, origin, inv, m_min(best, 1.0F), enter);
-        for (u32 k = 0U; k < 4U; ++k) {
-            if (n->child[k] == NPH_PHYSICS_MESH_CHILD_EMPTY) {
-                enter[k] = FLT_MAX;
-            }
-        }
-        // Farthest first onto the stack (an insertion sort, later children
-        // after earlier ones on ties), so the nearest comes off next.
-        u32 order[4] = {0U, 1U, 2U, 3U};
-        for (u32 i = 1U; i < 4U; ++i) {
-            for (u32 j = i; j > 0U && enter[order[j]] > enter[order[j - 1U]]; --j) {
-                const u32 t = order[j];
-                order[j] = order[j - 1U];
-                order[j - 1U] = t;
-            }
-        }
-        for (u32 i = 0U; i < 4U; ++i) {
-            const u32 k = order[i];
-            if (enter[k] < FLT_MAX && enter[k] <= best && top < CHECK_STACK) {
-                st[top] = n->child[k];
-                st_enter[top] = enter[k];
-                top += 1U;
-            }
-        }
-    }
-    if (best_tri == MNONE) {
+    if (h.triangle == NIL) {
         return false;
     }
     NphVec3 p[3];
-    nph_physics_mesh_triangle(m, best_tri, p);
-    NphVec3 nrm = m_unit_normal(p[0], p[1], p[2]);
-    if (best_back) {
-        nrm = m_v3(-nrm.x, -nrm.y, -nrm.z);
-    }
-    hit->fraction = best;
-    hit->normal = nrm;
-    hit->triangle = best_tri;
-    hit->original = ((const u32 *)((const u8 *)m + m->originals_offset))[best_tri];
-    hit->back = best_back ? 1 : 0;
+    mesh_corners(m, h.triangle, p);
+    const NphVec3 n = v3_normalize_or(v3_cross(v3_sub(p[1], p[0]), v3_sub(p[2], p[0])), v3_zero());
+    hit->fraction = h.fraction;
+    hit->normal = h.back ? v3_neg(n) : n;
+    hit->triangle = h.triangle;
+    hit->original = mesh_originals(m)[h.triangle];
+    hit->back = h.back ? 1 : 0;
     return true;
 }

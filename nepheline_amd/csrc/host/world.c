@@ -886,6 +886,13 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
             }
         }
     }
+    // The device walks a mesh's tree with a fixed stack: only a block the
+    // validator accepts (every one the builder writes) fits it.
+    if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH &&
+        (desc->geometry.mesh == NULL ||
+         !nph_physics_mesh_validate(desc->geometry.mesh, desc->geometry.mesh->byte_size))) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
     // Continuous sweeps against mesh triangles (continuous.c:211-300) are
     // not built: a world that sweeps takes no meshes.
     if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH && w->desc.enable_continuous != 0) {

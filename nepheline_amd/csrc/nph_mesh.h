@@ -14,6 +14,7 @@
 #ifndef NPH_MESH_H
 #define NPH_MESH_H
 
+#include "nph_bvh4.h"
 #include "nph_polytope.h"
 
 #define NPH_MESH_MERGE_COS 0.99619470F
@@ -26,36 +27,11 @@
 #define NPH_MESH_AGAINST_FACE (-1e-4F)
 #define NPH_MESH_FACE_COS 0.99999F
 #define NPH_MESH_PARALLEL_SIN_SQ 1e-4F
-#define NPH_MESH_STACK_DEPTH 160U
 #define NPH_MESH_COMPACT_NEAREST 8U
 #define NPH_TOUCH_SEPARATION (0.1F * NPH_PHYSICS_LINEAR_SLOP)
 
 enum { NPH_FEATURE_FACE = 0U, NPH_FEATURE_EDGE = 1U, NPH_FEATURE_VERTEX = 4U };
 enum { NPH_RULE_DROP = 0, NPH_RULE_KEEP = 1, NPH_RULE_FACE = 2 };
-
-// ---------------------------------------------------------------------------
-// The block (mesh.h:99-150)
-// ---------------------------------------------------------------------------
-NPH_HD const NphPhysicsMeshNode *mesh_nodes(const NphPhysicsMesh *m)
-{
-    return (const NphPhysicsMeshNode *)((const u8 *)m + m->nodes_offset);
-}
-NPH_HD const NphVec3 *mesh_vertices(const NphPhysicsMesh *m)
-{
-    return (const NphVec3 *)((const u8 *)m + m->vertices_offset);
-}
-NPH_HD const u32 *mesh_triangles(const NphPhysicsMesh *m)
-{
-    return (const u32 *)((const u8 *)m + m->triangles_offset);
-}
-NPH_HD const u8 *mesh_edges(const NphPhysicsMesh *m)
-{
-    return (const u8 *)m + m->edges_offset;
-}
-NPH_HD int mesh_child_is_leaf(u32 child)
-{
-    return child != NPH_PHYSICS_MESH_CHILD_EMPTY && (child & NPH_PHYSICS_MESH_CHILD_LEAF) != 0U;
-}
 
 // collide_mesh.c:95-102: the inverse rotation of a 64-bit vector, in 64 bits.
 NPH_HD NphDVec3 drotate_inverse(NphQuat q, NphDVec3 v)
@@ -757,59 +733,20 @@ NPH_HD void surface_triangle(NphSurfaceQuery *q, u32 t, const NphVec3 v[3], u32 
 // The mesh's triangles under a box (mesh.c:998-1050), each moved next to the
 // shape in 64 bits (collide_mesh.c:886-916)
 // ---------------------------------------------------------------------------
-NPH_HD u32 node_overlap_mask(const NphPhysicsMeshNode *n, NphAABB box)
-{
-    u32 mask = 0U;
-    for (u32 k = 0U; k < 4U; ++k) {
-        const int x = n->min_x[k] <= box.max.x && n->max_x[k] >= box.min.x;
-        const int y = n->min_y[k] <= box.max.y && n->max_y[k] >= box.min.y;
-        const int z = n->min_z[k] <= box.max.z && n->max_z[k] >= box.min.z;
-        if (x && y && z) {
-            mask |= 1U << k;
-        }
-    }
-    return mask;
-}
-
 NPH_HD void mesh_visit_box(const NphPhysicsMesh *mesh, NphAABB box, NphDVec3 origin,
                            NphSurfaceQuery *q)
 {
-    const NphPhysicsMeshNode *nodes = mesh_nodes(mesh);
     const u32 *tri = mesh_triangles(mesh);
     const NphVec3 *v = mesh_vertices(mesh);
     const u8 *edges = mesh_edges(mesh);
-    u32 stack[NPH_MESH_STACK_DEPTH];
-    u32 top = 0U;
-    stack[top++] = 0U;
-    while (top > 0U) {
-        const u32 code = stack[--top];
-        if (!mesh_child_is_leaf(code)) {
-            const NphPhysicsMeshNode *n = &nodes[code];
-            const u32 mask = node_overlap_mask(n, box);
-            for (u32 k = 4U; k > 0U; --k) {
-                if ((mask & (1U << (k - 1U))) != 0U &&
-                    n->child[k - 1U] != NPH_PHYSICS_MESH_CHILD_EMPTY &&
-                    top < NPH_MESH_STACK_DEPTH) {
-                    stack[top++] = n->child[k - 1U];
-                }
-            }
-            continue;
+    NphBoxWalk w;
+    box_walk_begin(&w, mesh, box);
+    for (u32 t = box_walk_next(&w); t != NPH_NULL32; t = box_walk_next(&w)) {
+        NphVec3 rv[3];
+        for (u32 i = 0U; i < 3U; ++i) {
+            rv[i] = relative_vertex(v[tri[3U * t + i]], origin);
         }
-        const u32 first = (code & ~NPH_PHYSICS_MESH_CHILD_LEAF) >> 2U;
-        const u32 last = first + (code & 3U) + 1U;
-        for (u32 t = first; t < last; ++t) {
-            const NphVec3 a = v[tri[3U * t]];
-            const NphVec3 b = v[tri[3U * t + 1U]];
-            const NphVec3 d = v[tri[3U * t + 2U]];
-            const NphAABB tb = aabb_make(v3_min(v3_min(a, b), d), v3_max(v3_max(a, b), d));
-            if (aabb_overlaps(tb, box)) {
-                NphVec3 rv[3];
-                for (u32 i = 0U; i < 3U; ++i) {
-                    rv[i] = relative_vertex(v[tri[3U * t + i]], origin);
-                }
-                surface_triangle(q, t, rv, edges[t]);
-            }
-        }
+        surface_triangle(q, t, rv, edges[t]);
     }
 }
 
