@@ -198,7 +198,8 @@ struct NphDeviceBatch {
     // their counts.
     NphSortedEntry *team_small;
     NphSortedEntry *team_big;
-    u32 cap_team_small, cap_team_big;
+    NphSortedEntry *team_mesh;
+    u32 cap_team_small, cap_team_big, cap_team_mesh;
     u32 *team_counts;
     u32 *keys; // counts [0, 64), cursors [64, 128)
     Stage solve;
@@ -387,11 +388,10 @@ __global__ void __launch_bounds__(256) k_b_key_scatter(Tab tab, u32 nw,
     }
 }
 
-// The sorted list, a contact a thread: convex pairs, or (mesh) the mesh
-// pairs, whose frames hold the triangles' clusters.
-// The convex narrow phase over the sorted list (device_narrow.h): lanes,
-// then teams; a wave of the lane pass covers pairs of one type, and so do
-// most teams of a wave of the team pass, whose list keeps the waves' order.
+// The narrow phase over the sorted list (device_narrow.h): lanes, then
+// teams (convex pairs, large hulls, mesh pairs); a wave of the lane pass
+// covers pairs of one type, and so do most teams of a wave of the team
+// pass, whose list keeps the waves' order.
 __global__ void __launch_bounds__(64) k_b_narrow_clear(u32 *counts) { narrow_counts_clear(counts); }
 
 __global__ void __launch_bounds__(NPH_BLOCK) k_b_narrow_lanes(const NphSortedEntry *__restrict__ sorted, u32 n,
@@ -425,21 +425,12 @@ __global__ void __launch_bounds__(NPH_BIG_TEAM_BLOCK) k_b_narrow_big_teams(const
         list, count, w, codes, step_dt);
 }
 
-__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_b_collide_mesh(const NphSortedEntry *__restrict__ sorted, u32 n,
-                                                       const NphCollideWorld *__restrict__ cw,
-                                                       u32 *codes, f32 step_dt)
+__global__ void __launch_bounds__(NPH_MESH_TEAM_BLOCK) k_b_narrow_mesh_teams(const NphSortedEntry *list,
+                                                                             const u32 *count,
+                                                                             NphManyWorlds w, u32 *codes,
+                                                                             f32 step_dt)
 {
-    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) {
-        return;
-    }
-    const NphSortedEntry e = sorted[j];
-    const NphCollideWorld c = cw[e.world];
-    if (c.contacts[e.record].child == NPH_NULL32) {
-        return;
-    }
-    contact_update_mesh(c.contacts, e.record, c.shapes, c.bodies, step_dt);
-    codes[e.index] = c.contacts[e.record].change;
+    narrow_mesh_teams(list, count, w, codes, step_dt);
 }
 
 __global__ void __launch_bounds__(256) k_b_map_fill(Tab tab, u32 nw, Worlds ws)
@@ -1003,7 +994,7 @@ void nph_devb_destroy(NphDeviceBatch *b)
     }
     void *lbufs[] = {b->lx,     b->ly,         b->lres,      b->acc,
                      b->sorted, b->keys,       b->team_small, b->team_big,
-                     b->team_counts};
+                     b->team_mesh, b->team_counts};
     for (u32 i = 0U; i < sizeof lbufs / sizeof lbufs[0]; ++i) {
         if (lbufs[i] != NULL) {
             (void)hipFree(lbufs[i]);
@@ -1227,6 +1218,9 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
         if (r == NPH_RESULT_OK) {
             r = dev_reserve(&b->team_big, &b->cap_team_big, m, s);
         }
+        if (r == NPH_RESULT_OK) {
+            r = dev_reserve(&b->team_mesh, &b->cap_team_mesh, m, s);
+        }
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -1235,6 +1229,7 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
         NphTeamLists lists;
         lists.small = b->team_small;
         lists.big = b->team_big;
+        lists.mesh = b->team_mesh;
         lists.counts = b->team_counts;
         hipLaunchKernelGGL(k_b_narrow_clear, dim3(1), dim3(64), 0, s, b->team_counts);
         const dim3 lane_grid(devi_blocks(m, NPH_BLOCK));
@@ -1261,9 +1256,9 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
                                h->codes, step_dt);
         }
         if (mesh) {
-            hipLaunchKernelGGL(k_b_collide_mesh, dim3(devi_blocks(m, NPH_COLLIDE_BLOCK)),
-                               dim3(NPH_COLLIDE_BLOCK), 0, s, b->sorted, m, dcw, h->codes,
-                               step_dt);
+            hipLaunchKernelGGL(k_b_narrow_mesh_teams, dim3(devi_team_grid(m, NPH_MESH_TEAM_BLOCK)),
+                               dim3(NPH_MESH_TEAM_BLOCK), 0, s, b->team_mesh, b->team_counts + 2,
+                               w, h->codes, step_dt);
         }
     }
     HIP_TRY(hipGetLastError());

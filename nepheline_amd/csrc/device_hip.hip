@@ -98,7 +98,7 @@ void nph_dev_destroy(NphDevice *d)
                     d->points, d->grown, d->grown_blocks, d->joint_conf, d->joint_state, d->step_joints,
                     d->fast_list, d->fast_pack, d->gate, d->mlp_params, d->lx, d->lraw, d->ly,
                     d->lres, d->wstats, d->log_x, d->log_t, d->log_count, d->team_small,
-                    d->team_big, d->team_counts, d->pf_nodes[0], d->pf_nodes[1],
+                    d->team_big, d->team_mesh, d->team_counts, d->pf_nodes[0], d->pf_nodes[1],
                     d->pf_nodes[2], d->pf_shapes, d->pf_stamp, d->pf_set};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
@@ -555,17 +555,12 @@ __global__ void __launch_bounds__(NPH_BIG_TEAM_BLOCK) k_narrow_big_teams(NphOneW
         list, count, w, codes, step_dt);
 }
 
-// The mesh pairs of the same list, one a thread (their frames hold the
-// triangles' clusters, so they keep out of the convex pairs' kernel).
-__global__ void __launch_bounds__(NPH_COLLIDE_BLOCK) k_collide_mesh(NphContactRec *contacts, const NphShapeRec *shapes,
-                          const NphBodyRec *bodies, const u32 *list, u32 *codes, u32 n, f32 step_dt)
+// The mesh pairs the lane pass listed, a team a pair.
+__global__ void __launch_bounds__(NPH_MESH_TEAM_BLOCK) k_narrow_mesh_teams(NphOneWorld w, const NphSortedEntry *list,
+                                                                           const u32 *count, u32 *codes,
+                                                                           f32 step_dt)
 {
-    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || contacts[list[i]].child == NPH_NULL32) {
-        return;
-    }
-    contact_update_mesh(contacts, list[i], shapes, bodies, step_dt);
-    codes[i] = contacts[list[i]].change;
+    narrow_mesh_teams(list, count, w, codes, step_dt);
 }
 
 // Exclusive scan of one value per thread over a block of NPH_SCAN_BLOCK.
@@ -1048,6 +1043,8 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
         if ((r = devi_grow_buffer((void **)&d->team_small, 0, (u64)c * sizeof(NphSortedEntry),
                                   d->stream, 0)) != NPH_RESULT_OK ||
             (r = devi_grow_buffer((void **)&d->team_big, 0, (u64)c * sizeof(NphSortedEntry),
+                                  d->stream, 0)) != NPH_RESULT_OK ||
+            (r = devi_grow_buffer((void **)&d->team_mesh, 0, (u64)c * sizeof(NphSortedEntry),
                                   d->stream, 0)) != NPH_RESULT_OK) {
             return r;
         }
@@ -1062,6 +1059,7 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
     NphTeamLists lists;
     lists.small = d->team_small;
     lists.big = d->team_big;
+    lists.mesh = d->team_mesh;
     lists.counts = d->team_counts;
     hipLaunchKernelGGL(k_narrow_clear, dim3(1), dim3(64), 0, d->stream, d->team_counts);
     hipLaunchKernelGGL(k_narrow_lanes, dim3(devi_blocks(n, NPH_BLOCK)), dim3(NPH_BLOCK), 0,
@@ -1075,9 +1073,9 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
                            d->team_counts + 1, d->codes + first, step_dt);
     }
     if (d->mesh_shapes > 0U) {
-        hipLaunchKernelGGL(k_collide_mesh, dim3((n + NPH_COLLIDE_BLOCK - 1U) / NPH_COLLIDE_BLOCK),
-                           dim3(NPH_COLLIDE_BLOCK), 0, d->stream, d->contacts, d->shapes,
-                           d->bodies, d->list + first, d->codes + first, n, step_dt);
+        hipLaunchKernelGGL(k_narrow_mesh_teams, dim3(devi_team_grid(n, NPH_MESH_TEAM_BLOCK)),
+                           dim3(NPH_MESH_TEAM_BLOCK), 0, d->stream, w, d->team_mesh,
+                           d->team_counts + 2, d->codes + first, step_dt);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, d->stream));

@@ -75,6 +75,7 @@ struct NphDevice {
     // their counts.
     struct NphSortedEntry *team_small;
     struct NphSortedEntry *team_big;
+    struct NphSortedEntry *team_mesh;
     u32 cap_team;
     u32 *team_counts;
     NphContactInit *inits;

@@ -29,11 +29,15 @@
 #define NPH_BIG_HULL_VERTICES 256U
 #define NPH_BIG_CLIP_CAPACITY 132U
 
+// Mesh pairs: a team of NPH_SURFACE_TEAM lanes a pair, four teams a block.
+#define NPH_MESH_TEAM_BLOCK 64U
+
 // The team lists of one narrow-phase launch: entries for 32-vertex teams,
-// entries for large-hull teams, and their two counts.
+// entries for large-hull teams, mesh pairs, and their three counts.
 typedef struct NphTeamLists {
     NphSortedEntry *small;
     NphSortedEntry *big;
+    NphSortedEntry *mesh;
     u32 *counts;
 } NphTeamLists;
 
@@ -59,7 +63,7 @@ struct NphManyWorlds {
 
 __device__ inline void narrow_counts_clear(u32 *counts)
 {
-    if (threadIdx.x < 2U) {
+    if (threadIdx.x < 3U) {
         counts[threadIdx.x] = 0U;
     }
 }
@@ -90,11 +94,21 @@ template <class WorldOf>
 __device__ inline void narrow_lane(bool valid, const NphSortedEntry &e, const WorldOf &world,
                                    u32 *codes, f32 step_dt, const NphTeamLists &lists)
 {
-    int route = 0; // 1: a 32-vertex team, 2: a large-hull team
+    int route = 0; // 1: a 32-vertex team, 2: a large-hull team, 3: a mesh team
     if (valid) {
         const NphCollideWorld c = world(e.world);
         NphContactRec *rec = &c.contacts[e.record];
-        if (rec->child == NPH_NULL32) { // mesh pairs: their own kernel
+        if (rec->child != NPH_NULL32) {
+            // A mesh pair (a chain of records): its fat-box test here
+            // (contact.c:523-526), the rest on a mesh team.
+            rec->change = 0U;
+            if (!aabb_overlaps(c.shapes[rec->shape_a].fat_aabb, c.shapes[rec->shape_b].fat_aabb)) {
+                rec->change = NPH_CHANGE_DISJOINT;
+                codes[e.index] = NPH_CHANGE_DISJOINT;
+            } else {
+                route = 3;
+            }
+        } else {
             NphPairJob job;
             if (!contact_job(rec, c.shapes, c.bodies, step_dt, &job)) {
                 rec->change = NPH_CHANGE_DISJOINT;
@@ -131,6 +145,7 @@ __device__ inline void narrow_lane(bool valid, const NphSortedEntry &e, const Wo
     }
     wave_append(route == 1, e, lists.small, &lists.counts[0]);
     wave_append(route == 2, e, lists.big, &lists.counts[1]);
+    wave_append(route == 3, e, lists.mesh, &lists.counts[2]);
 }
 
 // The team pass over a team list of `*count` entries (BLOCK threads a
@@ -156,6 +171,32 @@ __device__ inline void narrow_teams(const NphSortedEntry *list, const u32 *count
         if (tm.lane == 0U) {
             contact_store(rec, c.shapes, &job, &m);
             codes[e.index] = rec->change;
+        }
+        tm.sync();
+    }
+}
+
+// The mesh pairs of a mesh list of `*count` entries, a team a pair
+// (nph_surface.h), persistent like the convex teams; lane 0 carries the
+// manifolds into the pair's chain of records.
+template <class WorldOf>
+__device__ inline void narrow_mesh_teams(const NphSortedEntry *list, const u32 *count,
+                                         const WorldOf &world, u32 *codes, f32 step_dt)
+{
+    constexpr u32 TEAMS = NPH_MESH_TEAM_BLOCK / NPH_SURFACE_TEAM;
+    __shared__ NphSurfaceLds<NPH_SURFACE_TEAM> lds[TEAMS];
+    const NphTeam<NPH_SURFACE_TEAM> tm;
+    const u32 slot = threadIdx.x / NPH_SURFACE_TEAM;
+    const u32 n = *count;
+    for (u32 t = blockIdx.x * TEAMS + slot; t < n; t += gridDim.x * TEAMS) {
+        const NphSortedEntry e = list[t];
+        const NphCollideWorld c = world(e.world);
+        NphMeshFrame f;
+        mesh_frame(&c.contacts[e.record], c.shapes, c.bodies, step_dt, &f);
+        surface_collide(tm, &lds[slot], f.other, f.rotation, f.position, f.mesh, f.speculative);
+        if (tm.lane == 0U) {
+            mesh_store(c.contacts, e.record, c.shapes, &f, &lds[slot].out);
+            codes[e.index] = c.contacts[e.record].change;
         }
         tm.sync();
     }

@@ -17,7 +17,7 @@
 #ifndef NPH_STEP_H
 #define NPH_STEP_H
 
-#include "nph_mesh.h"
+#include "nph_surface.h"
 
 #define NPH_LANES 8U
 #define NPH_IMPACT_SPEED 1.0F
@@ -595,33 +595,52 @@ NPH_HD void pair_surface(const NphShapeRec *a, const NphShapeRec *b, f32 *fricti
 }
 
 #if defined(__cplusplus) // the narrow phase of a pair (nph_polytope.h is C++)
-// contact.c:378-483: a pair with a mesh, up to four manifolds, the deepest
-// in the pair's record and the rest in its children; impulses carried over
-// the whole chain, by id then by place (with the manifolds' normals within
-// the merge angle).  The fat-box test has been made on the parent.
-NPH_HD void mesh_update(NphContactRec *contacts, u32 contact, const NphShapeRec *shapes,
-                        const NphBodyRec *bodies, f32 step_dt)
+// A mesh pair (contact.c:378-483): the other shape in the mesh shape's
+// frame, and what carrying its manifolds back to world anchors needs.
+struct NphMeshFrame {
+    int mesh_a;           // the mesh is the pair's shape A
+    const NphGeom *other; // the convex shape
+    const NphPhysicsMesh *mesh;
+    NphQuat qm;           // the mesh shape's world rotation
+    NphQuat rotation;     // the other shape's, in the mesh's frame
+    NphDVec3 position;    // the other shape's place in the mesh's frame
+    NphVec3 shape_from_x; // the other shape from its body's centre
+    NphVec3 x_from_m;     // that centre from the mesh body's centre
+    f32 speculative;
+};
+
+NPH_HD void mesh_frame(const NphContactRec *c, const NphShapeRec *shapes, const NphBodyRec *bodies,
+                       f32 step_dt, NphMeshFrame *f)
 {
-    NphContactRec *c = &contacts[contact];
     const NphShapeRec *sa = &shapes[c->shape_a];
     const NphShapeRec *sb = &shapes[c->shape_b];
-    const int mesh_a = sa->geom.type == NPH_PHYSICS_SHAPE_MESH;
-    const NphShapeRec *sm = mesh_a ? sa : sb;
-    const NphShapeRec *sx = mesh_a ? sb : sa;
+    f->mesh_a = sa->geom.type == NPH_PHYSICS_SHAPE_MESH;
+    const NphShapeRec *sm = f->mesh_a ? sa : sb;
+    const NphShapeRec *sx = f->mesh_a ? sb : sa;
     const NphBodyRec *bm = &bodies[sm->body];
     const NphBodyRec *bx = &bodies[sx->body];
-    const NphQuat qm = q_mul(bm->rotation, sm->pose.rotation);
+    f->other = &sx->geom;
+    f->mesh = sm->geom.mesh;
+    f->qm = q_mul(bm->rotation, sm->pose.rotation);
     const NphQuat qx = q_mul(bx->rotation, sx->pose.rotation);
     const NphDVec3 pm = dv3_add_v3(bm->origin, q_rotate(bm->rotation, sm->pose.position));
     const NphDVec3 px = dv3_add_v3(bx->origin, q_rotate(bx->rotation, sx->pose.position));
-    const NphQuat rotation = q_mul(q_conj(qm), qx);
-    const NphDVec3 position = drotate_inverse(qm, dv3(px.x - pm.x, px.y - pm.y, px.z - pm.z));
-    NphSurfaceQuery q;
-    NphMeshManifolds mm;
-    collide_mesh_local(&sx->geom, rotation, position, sm->geom.mesh,
-                       pair_speculative(bm, bx, step_dt), &q, &mm);
-    const NphVec3 shape_from_x = dv3_relative(px, bx->center);
-    const NphVec3 x_from_m = dv3_relative(bx->center, bm->center);
+    f->rotation = q_mul(q_conj(f->qm), qx);
+    f->position = drotate_inverse(f->qm, dv3(px.x - pm.x, px.y - pm.y, px.z - pm.z));
+    f->shape_from_x = dv3_relative(px, bx->center);
+    f->x_from_m = dv3_relative(bx->center, bm->center);
+    f->speculative = pair_speculative(bm, bx, step_dt);
+}
+
+// The pair's manifolds (mesh frame, deepest first) into its chain of
+// records: the deepest in the pair's record and the rest in its children;
+// impulses carried over the whole chain, by id then by place (with the
+// manifolds' normals within the merge angle). Change codes: the record's
+// own, and each child's that started or stopped.
+NPH_HD void mesh_store(NphContactRec *contacts, u32 contact, const NphShapeRec *shapes,
+                       const NphMeshFrame *f, const NphMeshManifolds *mm)
+{
+    NphContactRec *c = &contacts[contact];
     // The chain's points of the last step.
     NphPointRec old[NPH_PHYSICS_MESH_MAX_MANIFOLDS * NPH_PHYSICS_MANIFOLD_MAX_POINTS];
     NphVec3 old_normals[NPH_PHYSICS_MESH_MAX_MANIFOLDS * NPH_PHYSICS_MANIFOLD_MAX_POINTS];
@@ -637,24 +656,24 @@ NPH_HD void mesh_update(NphContactRec *contacts, u32 contact, const NphShapeRec 
     u32 unmatched = 0U; // bit 4i + j: point j of manifold i
     f32 friction = 0.0F;
     f32 restitution = 0.0F;
-    pair_surface(sa, sb, &friction, &restitution);
+    pair_surface(&shapes[c->shape_a], &shapes[c->shape_b], &friction, &restitution);
     u32 children = 0U;
     u32 rec = contact;
     for (u32 i = 0U; rec != NPH_NULL32; ++i) {
         NphContactRec *r = &contacts[rec];
         const int was = r->point_count > 0U;
         r->point_count = 0U;
-        if (i < mm.count) {
-            const NphManifold *m = &mm.manifolds[i];
-            const NphVec3 n = q_rotate(qm, m->normal);
-            r->normal = mesh_a ? v3_neg(n) : n;
+        if (i < mm->count) {
+            const NphManifold *m = &mm->manifolds[i];
+            const NphVec3 n = q_rotate(f->qm, m->normal);
+            r->normal = f->mesh_a ? v3_neg(n) : n;
             for (u32 j = 0U; j < m->point_count; ++j) {
                 NphPointRec *p = &r->points[j];
                 memset(p, 0, sizeof *p);
-                const NphVec3 ax = v3_add(shape_from_x, q_rotate(qm, m->points[j].point));
-                const NphVec3 am = v3_add(ax, x_from_m);
-                p->anchor_a = mesh_a ? am : ax;
-                p->anchor_b = mesh_a ? ax : am;
+                const NphVec3 ax = v3_add(f->shape_from_x, q_rotate(f->qm, m->points[j].point));
+                const NphVec3 am = v3_add(ax, f->x_from_m);
+                p->anchor_a = f->mesh_a ? am : ax;
+                p->anchor_b = f->mesh_a ? ax : am;
                 p->separation = m->points[j].separation;
                 p->id = m->points[j].id;
                 int found = 0;
@@ -689,7 +708,7 @@ NPH_HD void mesh_update(NphContactRec *contacts, u32 contact, const NphShapeRec 
     }
     if (unmatched != 0U && claimed != (1U << old_count) - 1U) {
         rec = contact;
-        for (u32 i = 0U; rec != NPH_NULL32 && i < mm.count; ++i) {
+        for (u32 i = 0U; rec != NPH_NULL32 && i < mm->count; ++i) {
             NphContactRec *r = &contacts[rec];
             for (u32 j = 0U; j < r->point_count; ++j) {
                 if ((unmatched & (1U << (i * NPH_PHYSICS_MANIFOLD_MAX_POINTS + j))) == 0U) {
@@ -700,7 +719,7 @@ NPH_HD void mesh_update(NphContactRec *contacts, u32 contact, const NphShapeRec 
                 f32 best_d = NPH_WARM_DISTANCE * NPH_WARM_DISTANCE;
                 for (u32 o = 0U; o < old_count; ++o) { // warm_by_place
                     if ((claimed & (1U << o)) != 0U ||
-                        v3_dot(old_normals[o], r->normal) < NPH_MESH_MERGE_COS) {
+                        v3_dot(old_normals[o], r->normal) < NPH_SURFACE_MERGE_COS) {
                         continue;
                     }
                     const f32 d = nph_maxf(v3_dist_sq(old[o].anchor_a, p->anchor_a),
@@ -722,19 +741,6 @@ NPH_HD void mesh_update(NphContactRec *contacts, u32 contact, const NphShapeRec 
     if (children != 0U) {
         c->change |= NPH_CHANGE_CHILDREN | children;
     }
-}
-
-// contact.c:516-530 for a mesh pair: the disjoint test, then its chain.
-NPH_HD void contact_update_mesh(NphContactRec *contacts, u32 index, const NphShapeRec *shapes,
-                                const NphBodyRec *bodies, f32 step_dt)
-{
-    NphContactRec *c = &contacts[index];
-    c->change = 0U;
-    if (!aabb_overlaps(shapes[c->shape_a].fat_aabb, shapes[c->shape_b].fat_aabb)) {
-        c->change = NPH_CHANGE_DISJOINT;
-        return;
-    }
-    mesh_update(contacts, index, shapes, bodies, step_dt);
 }
 
 // Whether a contact's pair has a hull beyond the step kernels' capacity
