@@ -1,20 +1,35 @@
 // SPDX-License-Identifier: Apache-2.0
 // Continuous collision (engine/physics/src/continuous.c:1-409): a body that
-// moved far in a step is swept over it, its centre along a line and its
-// rotation by slerp, against the static shapes (bullets also against
-// kinematic and dynamic bodies at their final poses), and set back to the
-// first time any comes within a slop of it; conservative advancement on GJK
-// distances with a bound on rotation.
+// moved far in a step is swept back over the step against what it could
+// have passed through, and set back to the first moment it comes within a
+// slop of any of it.
 //
-// The device marks the fast bodies in finalize and leaves their boxes
-// alone; the sweeps run here, on the host, between finalize and the grown
-// proxies, because the sweep's slerp takes the C library's acosf and sinf,
-// whose bits the device cannot reproduce, and because the trees they query
-// live on the host.  Fast bodies are few (the bodies that moved farther in a
-// step than half their thinnest shape); each sweep reads only static shapes
-// (or, for bullets, every body at its final pose) and writes only its own
-// body, so the sweeps run in parallel and the results do not depend on
-// their order.
+// Where it runs. The device marks the fast bodies in finalize and leaves
+// their boxes alone; the sweeps run here, between finalize and the grown
+// proxies, for two reasons: the motion's rotation is a slerp through the C
+// library's acosf and sinf, whose bits the device does not reproduce, and
+// the trees the sweeps query live on the host. Fast bodies are few, each
+// sweep reads only static shapes (bullets: every body at its final pose)
+// and writes only its own body, so the results do not depend on the order.
+//
+// How a sweep is organised here:
+//  * a Motion is the body's step as a function of the fraction t: its
+//    centre moves along a line, its rotation turns by slerp, and `arc`
+//    bounds how fast a point of its shapes moves because of the turn;
+//  * a Probe is one of the body's shapes in three erosion stages: the shape
+//    itself, its points with a radius smaller by half the inner radius, and
+//    a ball of a quarter of the inner radius at its centroid. A stage that
+//    starts the step already within reach of the target hands over to the
+//    next, thinner one (the contacts push out what touches at the start);
+//  * each stage advances conservatively: the GJK gap, less the target gap,
+//    over the fastest the gap can close, until the gap is within tolerance
+//    (Mirtich's conservative advancement);
+//  * an impact of the full shape that the step's contacts already caught
+//    (the gap along the impact axis stays within a slop at both ends of the
+//    step, less the bow of the arc) is no impact;
+//  * static targets are convex shapes or the triangles of a mesh, taken in
+//    the mesh's frame (a triangle the body does not approach from its front
+//    face is skipped).
 #define _POSIX_C_SOURCE 200809L
 #include "world_impl.h"
 
@@ -25,197 +40,303 @@
 #define TOI_STEPS 32U
 #define INSET_FRACTION 0.5F
 #define CORE_FRACTION 0.25F
+#define TRIANGLE_APPROACH NPH_PHYSICS_LINEAR_SLOP
 
 // ---------------------------------------------------------------------------
-// Quaternion interpolation (quat.h:214-218, quat.c:74-93) with the C
-// library's acosf and sinf, as the reference calls them.
+// The motion (quat.h:214-218 and quat.c:74-93 for the slerp, with the C
+// library's acosf and sinf, as the reference calls them)
 // ---------------------------------------------------------------------------
-static NphQuat qneg(NphQuat q) { return q4(-q.x, -q.y, -q.z, -q.w); }
-static f32 qdot(NphQuat a, NphQuat b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+static f32 q4_dot(NphQuat a, NphQuat b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 
-static NphQuat nlerp(NphQuat a, NphQuat b, f32 t)
+static NphQuat q4_blend(NphQuat a, f32 wa, NphQuat b, f32 wb)
 {
-    const NphQuat bb = qdot(a, b) < 0.0F ? qneg(b) : b;
-    return q_normalize(q_add(q_scale(a, 1.0F - t), q_scale(bb, t)));
+    return q_add(q_scale(a, wa), q_scale(b, wb));
 }
 
-static NphQuat slerp(NphQuat a, NphQuat b, f32 t)
+static NphQuat turn_between(NphQuat a, NphQuat b, f32 t)
 {
-    f32 cos_theta = qdot(a, b);
-    NphQuat bb = b;
-    if (cos_theta < 0.0F) {
-        bb = qneg(b);
-        cos_theta = -cos_theta;
+    f32 c = q4_dot(a, b);
+    if (c < 0.0F) { // q and -q: the shorter arc
+        b = q4(-b.x, -b.y, -b.z, -b.w);
+        c = -c;
     }
-    if (cos_theta > 0.9995F) {
-        return nlerp(a, bb, t);
+    if (c > 0.9995F) { // nearly parallel: the normalized blend
+        const NphQuat bb = q4_dot(a, b) < 0.0F ? q4(-b.x, -b.y, -b.z, -b.w) : b;
+        return q_normalize(q4_blend(a, 1.0F - t, bb, t));
     }
-    const f32 theta = acosf(cos_theta < -1.0F ? -1.0F : (cos_theta > 1.0F ? 1.0F : cos_theta));
-    const f32 inv_sin = 1.0F / sinf(theta);
-    const f32 wa = sinf((1.0F - t) * theta) * inv_sin;
-    const f32 wb = sinf(t * theta) * inv_sin;
-    return q_add(q_scale(a, wa), q_scale(bb, wb));
+    const f32 theta = acosf(c < -1.0F ? -1.0F : (c > 1.0F ? 1.0F : c));
+    const f32 over_sin = 1.0F / sinf(theta);
+    return q4_blend(a, sinf((1.0F - t) * theta) * over_sin, b, sinf(t * theta) * over_sin);
 }
 
-static f32 quat_angle(NphQuat q) { return 2.0F * atan2f(v3_len(q_xyz(q)), nph_absf(q.w)); }
+typedef struct Motion {
+    NphVec3 shift;      // the centre's displacement over the step
+    NphQuat from, to;   // the rotation at the step's start and end
+    f32 arc;            // the turn's angle times the reach
+    f32 reach;          // how far the body's shapes reach from its centre
+} Motion;
 
-// ---------------------------------------------------------------------------
-// The sweep (continuous.c:64-219)
-// ---------------------------------------------------------------------------
-typedef struct Sweep {
-    NphVec3 dc;
-    NphQuat q0;
-    NphQuat q1;
-    f32 spin;
-    f32 reach;
-} Sweep;
-
-static void sweep_at(const Sweep *s, f32 t, NphVec3 *centre, NphQuat *rotation)
+static NphQuat motion_turn(const Motion *m, f32 t)
 {
-    *centre = v3_scale(s->dc, t);
-    *rotation = t <= 0.0F ? s->q0 : (t >= 1.0F ? s->q1 : slerp(s->q0, s->q1, t));
+    if (t <= 0.0F) {
+        return m->from;
+    }
+    return t >= 1.0F ? m->to : turn_between(m->from, m->to, t);
 }
 
-typedef struct Mover {
-    NphProxy proxy;
+// The frame at fraction t of something fixed on the body at `local` (about
+// the centre of mass) turned by `local_q`.
+static NphPhysicsPose motion_frame(const Motion *m, f32 t, NphVec3 local, NphQuat local_q)
+{
+    const NphQuat q = motion_turn(m, t);
+    return pose_make(v3_add(v3_scale(m->shift, t), q_rotate(q, local)), q_mul(q, local_q));
+}
+
+// The motion seen in a frame turned by `inv` (a surface's).
+static Motion motion_in(const Motion *m, NphQuat inv)
+{
+    Motion r = *m;
+    r.shift = q_rotate(inv, m->shift);
+    r.from = q_mul(inv, m->from);
+    r.to = q_mul(inv, m->to);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// A shape of the swept body, in its three erosion stages
+// ---------------------------------------------------------------------------
+enum { STAGE_SHAPE = 0, STAGE_INSET, STAGE_CORE, STAGES };
+
+typedef struct Probe {
+    NphProxy stage[STAGES];
     NphVec3 storage[8];
-    NphProxy inset;
-    NphProxy core;
-    NphVec3 core_point;
-    NphVec3 local;
+    NphVec3 centroid; // the core ball's centre (the hull's centroid, or the origin)
+    NphVec3 local;    // the shape's place about the centre of mass
     NphQuat local_q;
-} Mover;
+} Probe;
 
-static void mover_init(Mover *mv, const NphBodyRec *b, const HShape *s)
+static void probe_make(Probe *p, const NphBodyRec *b, const HShape *s)
 {
-    mv->proxy = proxy_of(&s->geometry, mv->storage);
     const NphGeom g = world_host_geom(s);
     const f32 inner = geom_inner_radius(&g);
-    mv->inset = mv->proxy;
-    mv->inset.radius = mv->proxy.radius - INSET_FRACTION * inner;
-    mv->core_point =
+    p->stage[STAGE_SHAPE] = proxy_of(&s->geometry, p->storage);
+    p->stage[STAGE_INSET] = p->stage[STAGE_SHAPE];
+    p->stage[STAGE_INSET].radius = p->stage[STAGE_SHAPE].radius - INSET_FRACTION * inner;
+    p->centroid =
         s->geometry.type == NPH_PHYSICS_SHAPE_HULL ? s->geometry.hull->centroid : v3_zero();
-    mv->core.points = &mv->core_point;
-    mv->core.count = 1U;
-    mv->core.radius = CORE_FRACTION * inner;
-    mv->local = v3_sub(s->pose.position, b->local_center);
-    mv->local_q = s->pose.rotation;
+    p->stage[STAGE_CORE].points = &p->centroid;
+    p->stage[STAGE_CORE].count = 1U;
+    p->stage[STAGE_CORE].radius = CORE_FRACTION * inner;
+    p->local = v3_sub(s->pose.position, b->local_center);
+    p->local_q = s->pose.rotation;
 }
 
-static NphPhysicsPose mover_at(const Mover *mv, const Sweep *s, f32 t)
+// A still target: its proxy and its pose in the motion's frame.
+typedef struct Target {
+    const NphProxy *proxy;
+    NphPhysicsPose pose;
+} Target;
+
+// The GJK gap between stage `k` of the probe at fraction t and the target,
+// measured in a frame at the probe's place (as the casts measure it).
+static NphDistance gap_probe(const Probe *p, u32 k, const Motion *m, const Target *tg, f32 t,
+                             NphSimplexCache *cache, NphPhysicsPose *at_p, NphPhysicsPose *at_t)
 {
-    NphVec3 c;
-    NphQuat q;
-    sweep_at(s, t, &c, &q);
-    return pose_make(v3_add(c, q_rotate(q, mv->local)), q_mul(q, mv->local_q));
+    const NphPhysicsPose f = motion_frame(m, t, p->local, p->local_q);
+    *at_p = pose_make(v3_zero(), f.rotation);
+    *at_t = pose_make(v3_sub(tg->pose.position, f.position), tg->pose.rotation);
+    NphProxyDistance pd;
+    proxy_distance(tg->proxy, *at_t, &p->stage[k], *at_p, 0, cache, &pd);
+    return pd.d;
 }
 
-static f32 advance(const Mover *mv, const NphProxy *m, const Sweep *s, const NphProxy *t,
-                   NphPhysicsPose pose_t, f32 limit, NphVec3 *axis)
+// Conservative advancement of stage `k` towards the target, short of
+// `limit`: the fraction where the gap first comes within the target
+// tolerance (0 when it starts there), or -1 when it does not before
+// `limit` (or the gap never closes); `axis` gets the axis between them
+// there, from the target towards the probe.
+static f32 advance_stage(const Probe *p, u32 k, const Motion *m, const Target *tg, f32 limit,
+                         NphVec3 *axis)
 {
-    const f32 radius = m->radius + t->radius;
+    const f32 reach = p->stage[k].radius + tg->proxy->radius;
     NphSimplexCache cache;
     memset(&cache, 0, sizeof cache);
-    f32 alpha = 0.0F;
-    for (u32 step = 0U; step < TOI_STEPS; ++step) {
-        const NphPhysicsPose pm = mover_at(mv, s, alpha);
-        const NphPhysicsPose at_m = pose_make(v3_zero(), pm.rotation);
-        const NphPhysicsPose at_t = pose_make(v3_sub(pose_t.position, pm.position), pose_t.rotation);
-        NphProxyDistance pd;
-        proxy_distance(t, at_t, m, at_m, 0, &cache, &pd);
-        const NphDistance d = pd.d;
-        const f32 separation = d.distance - radius;
+    f32 t = 0.0F;
+    for (u32 i = 0U; i < TOI_STEPS; ++i) {
+        NphPhysicsPose at_p, at_t;
+        const NphDistance d = gap_probe(p, k, m, tg, t, &cache, &at_p, &at_t);
         if (!(d.distance > 0.0F)) {
-            return alpha;
+            return t; // the cores touch: no better place to stop
         }
+        const f32 gap = d.distance - reach;
         *axis = d.normal;
-        if (separation <= TOI_TARGET + TOI_TOLERANCE) {
-            *axis = proxy_contact_axis(m, at_m, t, at_t, &cache, d.normal);
-            return alpha;
+        if (gap <= TOI_TARGET + TOI_TOLERANCE) {
+            *axis = proxy_contact_axis(&p->stage[k], at_p, tg->proxy, at_t, &cache, d.normal);
+            return t;
         }
-        const f32 closing = -v3_dot(s->dc, d.normal) + s->spin;
+        // The gap closes no faster than the centre comes along the normal
+        // plus the fastest a point turns.
+        const f32 closing = -v3_dot(m->shift, d.normal) + m->arc;
         if (!(closing > 0.0F)) {
             return -1.0F;
         }
-        alpha += (separation - TOI_TARGET) / closing;
-        if (!(alpha < limit)) {
+        t += (gap - TOI_TARGET) / closing;
+        if (!(t < limit)) {
             return -1.0F;
         }
     }
-    return alpha;
+    return t; // still approaching, grazing: the last safe fraction
 }
 
-static f32 gap_at(const Mover *mv, const NphProxy *m, const Sweep *s, const NphProxy *t,
-                  NphPhysicsPose pose_t, NphVec3 axis, f32 alpha)
+// The shape's gap to the target along `axis` at fraction t.
+static f32 gap_along(const Probe *p, const Motion *m, const Target *tg, NphVec3 axis, f32 t)
 {
-    const NphPhysicsPose pm = mover_at(mv, s, alpha);
-    return proxy_gap(m, pose_make(v3_zero(), pm.rotation), t,
-                     pose_make(v3_sub(pose_t.position, pm.position), pose_t.rotation), axis);
+    const NphPhysicsPose f = motion_frame(m, t, p->local, p->local_q);
+    return proxy_gap(&p->stage[STAGE_SHAPE], pose_make(v3_zero(), f.rotation), tg->proxy,
+                     pose_make(v3_sub(tg->pose.position, f.position), tg->pose.rotation), axis);
 }
 
-static f32 shape_toi(const Mover *mv, const Sweep *s, const NphProxy *t, NphPhysicsPose pose_t,
-                     f32 limit)
+// The step's contacts already hold an impact whose gap along its axis is
+// within a slop at both ends of the step once the arc's bow (reach angle^2
+// / 8) is taken off: the shape never passed through the target.
+static bool impact_caught(const Probe *p, const Motion *m, const Target *tg, NphVec3 axis)
+{
+    const f32 angle = m->reach > 0.0F ? m->arc / m->reach : 0.0F;
+    const f32 bow = 0.125F * m->arc * angle;
+    const f32 low = nph_minf(gap_along(p, m, tg, axis, 0.0F), gap_along(p, m, tg, axis, 1.0F));
+    return low - bow >= -NPH_PHYSICS_LINEAR_SLOP;
+}
+
+// The probe's time of impact with the target before `limit`, or -1.
+static f32 probe_toi(const Probe *p, const Motion *m, const Target *tg, f32 limit)
 {
     NphVec3 axis = v3(0.0F, 1.0F, 0.0F);
-    const f32 f = advance(mv, &mv->proxy, s, t, pose_t, limit, &axis);
-    if (f > 0.0F) {
-        const f32 angle = s->reach > 0.0F ? s->spin / s->reach : 0.0F;
-        const f32 bow = 0.125F * s->spin * angle;
-        const f32 low = nph_minf(gap_at(mv, &mv->proxy, s, t, pose_t, axis, 0.0F),
-                                 gap_at(mv, &mv->proxy, s, t, pose_t, axis, 1.0F));
-        return low - bow >= -NPH_PHYSICS_LINEAR_SLOP ? -1.0F : f;
+    for (u32 k = STAGE_SHAPE; k < STAGES; ++k) {
+        const f32 t = advance_stage(p, k, m, tg, limit, &axis);
+        if (t == 0.0F && k + 1U < STAGES) {
+            continue; // within reach at the start: the thinner stage decides
+        }
+        if (k == STAGE_SHAPE && t > 0.0F && impact_caught(p, m, tg, axis)) {
+            return -1.0F;
+        }
+        return t > 0.0F ? t : -1.0F;
     }
-    if (f < 0.0F) {
-        return f;
-    }
-    const f32 g = advance(mv, &mv->inset, s, t, pose_t, limit, &axis);
-    if (g != 0.0F) {
-        return g > 0.0F ? g : -1.0F;
-    }
-    const f32 h = advance(mv, &mv->core, s, t, pose_t, limit, &axis);
-    return h > 0.0F ? h : -1.0F;
+    return -1.0F;
 }
 
 // ---------------------------------------------------------------------------
-// A fast body (continuous.c:299-409)
+// Mesh targets (continuous.c:211-300): the motion in the mesh's frame, the
+// triangles under the box the body's reach sweeps
 // ---------------------------------------------------------------------------
-typedef struct BodySweep {
-    NphPhysicsWorld *w;
-    const NphBodyRec *b; // at the step's end
-    u32 slot;
-    const NphBodyRec *records; // every body's record (targets)
-    const u8 *bullet;          // per body slot: a bullet, as the step began
-    NphDVec3 start;
-    Sweep sweep;
-    f32 best;
-} BodySweep;
-
-static void sweep_target(BodySweep *bs, u32 target_body, const HShape *ts)
+static f32 mesh_toi(const Probe *p, const Motion *world, NphDVec3 start, const NphBodyRec *mb,
+                    const HShape *ms, f32 limit)
 {
-    NphPhysicsWorld *w = bs->w;
-    const HBody *tb = &VB(w->bodies, HBody)[target_body];
-    if (target_body == bs->slot ||
-        (tb->type != (u32)NPH_PHYSICS_BODY_STATIC && bs->bullet[target_body] != 0U) ||
-        !joints_allow_contact(w, bs->slot, target_body)) {
-        return;
-    }
-    NphVec3 storage[8];
-    const NphProxy tp = proxy_of(&ts->geometry, storage);
-    NphShapeRec trec;
-    memset(&trec, 0, sizeof trec);
-    trec.pose = ts->pose;
-    const NphPhysicsPose pose_t = shape_pose_relative(&bs->records[target_body], &trec, bs->start);
-    const HBody *hb = &VB(w->bodies, HBody)[bs->slot];
-    for (u32 s = hb->shape_head; s != NPH_NULL32; s = VB(w->shapes, HShape)[s].next) {
-        const HShape *ms = &VB(w->shapes, HShape)[s];
-        if (!nph_physics_filter_should_collide(ms->filter, ts->filter)) {
+    const NphPhysicsMesh *mesh = ms->geometry.mesh;
+    const NphQuat qs = q_mul(mb->rotation, ms->pose.rotation);
+    const NphDVec3 ps = dv3_add_v3(mb->origin, q_rotate(mb->rotation, ms->pose.position));
+    const NphDVec3 origin = drotate_inverse(qs, dv3(start.x - ps.x, start.y - ps.y, start.z - ps.z));
+    const Motion m = motion_in(world, q_conj(qs));
+    const NphVec3 at = v3((f32)origin.x, (f32)origin.y, (f32)origin.z);
+    const NphVec3 aa = v3_abs(at);
+    const f32 grow = world->reach + TOI_TARGET + TOI_TOLERANCE +
+                     2.4e-7F * nph_maxf(aa.x, nph_maxf(aa.y, aa.z));
+    const NphVec3 end = v3_add(at, m.shift);
+    const NphAABB box = aabb_make(v3_sub(v3_min(at, end), v3_splat(grow)),
+                                  v3_add(v3_max(at, end), v3_splat(grow)));
+    const int two_sided = (mesh->flags & NPH_PHYSICS_MESH_TWO_SIDED) != 0U;
+    // The centroid's place at both ends of the step (constant per probe).
+    const NphPhysicsPose f0 = motion_frame(&m, 0.0F, p->local, p->local_q);
+    const NphPhysicsPose f1 = motion_frame(&m, 1.0F, p->local, p->local_q);
+    const NphVec3 c0 = v3_add(q_rotate(f0.rotation, p->centroid), f0.position);
+    const NphVec3 c1 = v3_add(q_rotate(f1.rotation, p->centroid), f1.position);
+    f32 best = limit;
+    NphBoxWalk walk;
+    box_walk_begin(&walk, mesh, box);
+    for (u32 t = box_walk_next(&walk); t != NPH_NULL32; t = box_walk_next(&walk)) {
+        NphVec3 v[3];
+        mesh_corners(mesh, t, v);
+        for (u32 k = 0U; k < 3U; ++k) {
+            v[k] = relative_vertex(v[k], origin);
+        }
+        NphVec3 n = v3_cross(v3_sub(v[1], v[0]), v3_sub(v[2], v[0]));
+        const f32 len = v3_len(n);
+        if (!(len > 0.0F)) {
             continue;
         }
-        Mover mv;
-        mover_init(&mv, bs->b, ms);
-        const f32 f = shape_toi(&mv, &bs->sweep, &tp, pose_t, bs->best);
-        if (f >= 0.0F && f < bs->best) {
-            bs->best = f;
+        n = v3_scale(n, 1.0F / len);
+        // Heights of the centroid over the plane at the ends: a triangle
+        // the body is behind (one-sided) or does not come towards is not
+        // one it could tunnel through.
+        f32 h0 = v3_dot(n, v3_sub(c0, v[0]));
+        f32 h1 = v3_dot(n, v3_sub(c1, v[0]));
+        if (two_sided && h0 < 0.0F) {
+            h0 = -h0;
+            h1 = -h1;
+        }
+        if (h0 < -NPH_PHYSICS_LINEAR_SLOP || !(h0 - h1 > TRIANGLE_APPROACH)) {
+            continue;
+        }
+        const NphProxy tri = {v, 3U, 0.0F};
+        const Target tg = {&tri, pose_identity()};
+        const f32 toi = probe_toi(p, &m, &tg, best);
+        if (toi >= 0.0F && toi < best) {
+            best = toi;
+        }
+    }
+    return best < limit ? best : -1.0F;
+}
+
+// ---------------------------------------------------------------------------
+// One fast body (continuous.c:302-409)
+// ---------------------------------------------------------------------------
+typedef struct Sweeper {
+    NphPhysicsWorld *w;
+    const NphBodyRec *b;       // the body at the step's end
+    u32 slot;
+    const NphBodyRec *records; // every body's record (the targets)
+    const u8 *bullet;          // per body slot: a bullet as the step began
+    NphDVec3 start;            // the centre of mass at the step's start
+    Motion motion;             // world axes, about `start`
+    f32 best;                  // the first impact so far (1: none)
+} Sweeper;
+
+static void sweep_against(Sweeper *sw, const HShape *ts)
+{
+    NphPhysicsWorld *w = sw->w;
+    const u32 target = ts->body;
+    const HBody *tb = &VB(w->bodies, HBody)[target];
+    // Not its own shapes, nor a bullet's (bullets do not sweep against each
+    // other), nor a body a joint keeps it from touching.
+    if (target == sw->slot ||
+        (tb->type != (u32)NPH_PHYSICS_BODY_STATIC && sw->bullet[target] != 0U) ||
+        !joints_allow_contact(w, sw->slot, target)) {
+        return;
+    }
+    const int mesh = ts->geometry.type == NPH_PHYSICS_SHAPE_MESH;
+    NphVec3 storage[8];
+    NphProxy tp = {storage, 1U, 0.0F};
+    Target tg = {&tp, pose_identity()};
+    if (!mesh) {
+        tp = proxy_of(&ts->geometry, storage);
+        NphShapeRec rec;
+        memset(&rec, 0, sizeof rec);
+        rec.pose = ts->pose;
+        tg.pose = shape_pose_relative(&sw->records[target], &rec, sw->start);
+    }
+    const HBody *hb = &VB(w->bodies, HBody)[sw->slot];
+    for (u32 s = hb->shape_head; s != NPH_NULL32; s = VB(w->shapes, HShape)[s].next) {
+        const HShape *ms = &VB(w->shapes, HShape)[s];
+        if (((ms->flags | ts->flags) & (HS_SENSOR | HS_GHOST)) != 0U ||
+            !nph_physics_filter_should_collide(ms->filter, ts->filter)) {
+            continue;
+        }
+        Probe p;
+        probe_make(&p, sw->b, ms);
+        const f32 toi = mesh ? mesh_toi(&p, &sw->motion, sw->start, &sw->records[target], ts,
+                                        sw->best)
+                             : probe_toi(&p, &sw->motion, &tg, sw->best);
+        if (toi >= 0.0F && toi < sw->best) {
+            sw->best = toi;
         }
     }
 }
@@ -223,32 +344,33 @@ static void sweep_target(BodySweep *bs, u32 target_body, const HShape *ts)
 static int sweep_leaf(u32 proxy, u64 leaf, void *context)
 {
     (void)proxy;
-    BodySweep *bs = (BodySweep *)context;
-    const HShape *ts = &VB(bs->w->shapes, HShape)[(u32)leaf];
-    sweep_target(bs, ts->body, ts);
+    Sweeper *sw = (Sweeper *)context;
+    sweep_against(sw, &VB(sw->w->shapes, HShape)[(u32)leaf]);
     return 1;
 }
 
-// The body's sweep; on an impact its record is set back (the boxes are the
-// caller's).  Returns 1 on an impact.
+// The body's sweep; on an impact its record is set back to it (the boxes
+// are the caller's). Returns 1 on an impact.
 static int sweep_body(NphPhysicsWorld *w, const NphFastBody *f, NphBodyRec *b,
                       const NphBodyRec *records, const u8 *bullet, int bullets)
 {
-    BodySweep bs;
-    bs.w = w;
-    bs.b = b;
-    bs.slot = f->slot;
-    bs.records = records;
-    bs.bullet = bullet;
     const NphVec3 dp = sb_dp(&f->sb);
+    Sweeper sw;
+    sw.w = w;
+    sw.b = b;
+    sw.slot = f->slot;
+    sw.records = records;
+    sw.bullet = bullet;
+    sw.start = dv3_add_v3(b->center, v3_neg(dp));
+    sw.motion.shift = dp;
+    sw.motion.from = f->params.q0;
+    sw.motion.to = b->rotation;
+    sw.motion.reach = b->max_extent;
     const NphQuat dq = sb_dq(&f->sb);
-    bs.start = dv3_add_v3(b->center, v3_neg(dp));
-    bs.sweep.dc = dp;
-    bs.sweep.q0 = f->params.q0;
-    bs.sweep.q1 = b->rotation;
-    bs.sweep.reach = b->max_extent;
-    bs.sweep.spin = quat_angle(dq) * b->max_extent;
-    bs.best = 1.0F;
+    sw.motion.arc = 2.0F * atan2f(v3_len(q_xyz(dq)), nph_absf(dq.w)) * b->max_extent;
+    sw.best = 1.0F;
+    // Targets in a category some shape of the body accepts (each pair then
+    // checks both filters), under the box the body's reach sweeps.
     u64 mask = 0U;
     const HBody *hb = &VB(w->bodies, HBody)[f->slot];
     for (u32 s = hb->shape_head; s != NPH_NULL32; s = VB(w->shapes, HShape)[s].next) {
@@ -258,19 +380,18 @@ static int sweep_body(NphPhysicsWorld *w, const NphFastBody *f, NphBodyRec *b,
     NphAABB rel;
     rel.min = v3_sub(v3_min(v3_zero(), dp), v3_splat(grow));
     rel.max = v3_add(v3_max(v3_zero(), dp), v3_splat(grow));
-    const NphAABB box = nph_world_box(bs.start, rel);
-    tree_query(&w->trees[NPH_TREE_STATIC], box, mask, sweep_leaf, &bs);
+    const NphAABB box = nph_world_box(sw.start, rel);
+    tree_query(&w->trees[NPH_TREE_STATIC], box, mask, sweep_leaf, &sw);
     if (bullets) {
-        tree_query(&w->trees[NPH_TREE_KINEMATIC], box, mask, sweep_leaf, &bs);
-        tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, sweep_leaf, &bs);
+        tree_query(&w->trees[NPH_TREE_KINEMATIC], box, mask, sweep_leaf, &sw);
+        tree_query(&w->trees[NPH_TREE_DYNAMIC], box, mask, sweep_leaf, &sw);
     }
-    if (!(bs.best < 1.0F)) {
+    if (!(sw.best < 1.0F)) {
         return 0;
     }
-    NphVec3 c;
-    NphQuat q;
-    sweep_at(&bs.sweep, bs.best, &c, &q);
-    b->center = dv3_add_v3(bs.start, c);
+    // Back to the first impact: the next step's contacts take it from there.
+    const NphQuat q = motion_turn(&sw.motion, sw.best);
+    b->center = dv3_add_v3(sw.start, v3_scale(sw.motion.shift, sw.best));
     b->rotation = q_normalize(q);
     b->origin = dv3_add_v3(b->center, v3_neg(q_rotate(b->rotation, b->local_center)));
     return 1;

@@ -875,28 +875,12 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
     if (desc->sensor != 0 || desc->ghost != 0) {
         return NPH_RESULT_UNSUPPORTED; // sensors and ghosts: not on this path
     }
-    if (desc->geometry.type == NPH_PHYSICS_SHAPE_HULL) {
-        // A clipped face against a mesh triangle must fit the narrow
-        // phase's 68 points, as the reference's 264 hold it (faces of at
-        // most 64 vertices; any number of vertices up to 256).
-        const NphPhysicsHull *h = desc->geometry.hull;
-        for (u32 f = 0U; f < h->face_count; ++f) {
-            if (nph_physics_hull_face_vertex_count(h, f) > NPH_HULL_MAX_FACE_VERTICES) {
-                return NPH_RESULT_UNSUPPORTED;
-            }
-        }
-    }
     // The device walks a mesh's tree with a fixed stack: only a block the
     // validator accepts (every one the builder writes) fits it.
     if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH &&
         (desc->geometry.mesh == NULL ||
          !nph_physics_mesh_validate(desc->geometry.mesh, desc->geometry.mesh->byte_size))) {
         return NPH_RESULT_INVALID_ARGUMENT;
-    }
-    // Continuous sweeps against mesh triangles (continuous.c:211-300) are
-    // not built: a world that sweeps takes no meshes.
-    if (desc->geometry.type == NPH_PHYSICS_SHAPE_MESH && w->desc.enable_continuous != 0) {
-        return NPH_RESULT_UNSUPPORTED;
     }
     NphResult r = world_sync_bodies(w);
     const void *device_block = NULL;
