@@ -430,7 +430,16 @@ __global__ void __launch_bounds__(NPH_MESH_TEAM_BLOCK) k_b_narrow_mesh_teams(con
                                                                              NphManyWorlds w, u32 *codes,
                                                                              f32 step_dt)
 {
-    narrow_mesh_teams(list, count, w, codes, step_dt);
+    narrow_mesh_teams<NPH_CLIP_CAPACITY, NPH_MESH_TEAM_BLOCK>(list, count, w, codes, step_dt);
+}
+
+__global__ void __launch_bounds__(NPH_BIG_MESH_TEAM_BLOCK) k_b_narrow_big_mesh_teams(const NphSortedEntry *list,
+                                                                                     const u32 *count,
+                                                                                     NphManyWorlds w, u32 *codes,
+                                                                                     f32 step_dt)
+{
+    narrow_mesh_teams<NPH_BIG_CLIP_CAPACITY, NPH_BIG_MESH_TEAM_BLOCK>(list, count, w, codes,
+                                                                      step_dt);
 }
 
 __global__ void __launch_bounds__(256) k_b_map_fill(Tab tab, u32 nw, Worlds ws)
@@ -1255,7 +1264,12 @@ NphResult nph_devb_collide(NphDeviceBatch *b, const NphCollidePart *parts, u32 n
                                dim3(NPH_BIG_TEAM_BLOCK), 0, s, b->team_big, b->team_counts + 1, w,
                                h->codes, step_dt);
         }
-        if (mesh) {
+        if (mesh && big) {
+            hipLaunchKernelGGL(k_b_narrow_big_mesh_teams,
+                               dim3(devi_team_grid(m, NPH_BIG_MESH_TEAM_BLOCK)),
+                               dim3(NPH_BIG_MESH_TEAM_BLOCK), 0, s, b->team_mesh,
+                               b->team_counts + 2, w, h->codes, step_dt);
+        } else if (mesh) {
             hipLaunchKernelGGL(k_b_narrow_mesh_teams, dim3(devi_team_grid(m, NPH_MESH_TEAM_BLOCK)),
                                dim3(NPH_MESH_TEAM_BLOCK), 0, s, b->team_mesh, b->team_counts + 2,
                                w, h->codes, step_dt);

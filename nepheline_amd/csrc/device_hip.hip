@@ -560,7 +560,15 @@ __global__ void __launch_bounds__(NPH_MESH_TEAM_BLOCK) k_narrow_mesh_teams(NphOn
                                                                            const u32 *count, u32 *codes,
                                                                            f32 step_dt)
 {
-    narrow_mesh_teams(list, count, w, codes, step_dt);
+    narrow_mesh_teams<NPH_CLIP_CAPACITY, NPH_MESH_TEAM_BLOCK>(list, count, w, codes, step_dt);
+}
+
+__global__ void __launch_bounds__(NPH_BIG_MESH_TEAM_BLOCK) k_narrow_big_mesh_teams(NphOneWorld w, const NphSortedEntry *list,
+                                                                                   const u32 *count, u32 *codes,
+                                                                                   f32 step_dt)
+{
+    narrow_mesh_teams<NPH_BIG_CLIP_CAPACITY, NPH_BIG_MESH_TEAM_BLOCK>(list, count, w, codes,
+                                                                      step_dt);
 }
 
 // Exclusive scan of one value per thread over a block of NPH_SCAN_BLOCK.
@@ -1072,7 +1080,11 @@ static NphResult collide_range(NphDevice *d, const u32 *awake, u32 first, u32 co
                            dim3(NPH_BIG_TEAM_BLOCK), 0, d->stream, w, d->team_big,
                            d->team_counts + 1, d->codes + first, step_dt);
     }
-    if (d->mesh_shapes > 0U) {
+    if (d->mesh_shapes > 0U && d->big_hull_shapes > 0U) {
+        hipLaunchKernelGGL(k_narrow_big_mesh_teams, dim3(devi_team_grid(n, NPH_BIG_MESH_TEAM_BLOCK)),
+                           dim3(NPH_BIG_MESH_TEAM_BLOCK), 0, d->stream, w, d->team_mesh,
+                           d->team_counts + 2, d->codes + first, step_dt);
+    } else if (d->mesh_shapes > 0U) {
         hipLaunchKernelGGL(k_narrow_mesh_teams, dim3(devi_team_grid(n, NPH_MESH_TEAM_BLOCK)),
                            dim3(NPH_MESH_TEAM_BLOCK), 0, d->stream, w, d->team_mesh,
                            d->team_counts + 2, d->codes + first, step_dt);

@@ -27,10 +27,12 @@
 #define NPH_BIG_TEAM_BLOCK 64U // four teams a block (hulls of up to 256 vertices)
 #define NPH_TEAM_GRID_MAX 2048U
 #define NPH_BIG_HULL_VERTICES 256U
-#define NPH_BIG_CLIP_CAPACITY 132U
+#define NPH_BIG_CLIP_CAPACITY (NPH_BIG_HULL_VERTICES + 8U) // the reference's
 
-// Mesh pairs: a team of NPH_SURFACE_TEAM lanes a pair, four teams a block.
+// Mesh pairs: a team of NPH_SURFACE_TEAM lanes a pair, four teams a block;
+// two with the large hulls' clip capacity (worlds that hold such hulls).
 #define NPH_MESH_TEAM_BLOCK 64U
+#define NPH_BIG_MESH_TEAM_BLOCK 32U
 
 // The team lists of one narrow-phase launch: entries for 32-vertex teams,
 // entries for large-hull teams, mesh pairs, and their three counts.
@@ -179,12 +181,12 @@ __device__ inline void narrow_teams(const NphSortedEntry *list, const u32 *count
 // The mesh pairs of a mesh list of `*count` entries, a team a pair
 // (nph_surface.h), persistent like the convex teams; lane 0 carries the
 // manifolds into the pair's chain of records.
-template <class WorldOf>
+template <u32 CAP, u32 BLOCK, class WorldOf>
 __device__ inline void narrow_mesh_teams(const NphSortedEntry *list, const u32 *count,
                                          const WorldOf &world, u32 *codes, f32 step_dt)
 {
-    constexpr u32 TEAMS = NPH_MESH_TEAM_BLOCK / NPH_SURFACE_TEAM;
-    __shared__ NphSurfaceLds<NPH_SURFACE_TEAM> lds[TEAMS];
+    constexpr u32 TEAMS = BLOCK / NPH_SURFACE_TEAM;
+    __shared__ NphSurfaceLds<NPH_SURFACE_TEAM, CAP> lds[TEAMS];
     const NphTeam<NPH_SURFACE_TEAM> tm;
     const u32 slot = threadIdx.x / NPH_SURFACE_TEAM;
     const u32 n = *count;

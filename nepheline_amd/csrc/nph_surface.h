@@ -531,7 +531,7 @@ NPH_TD NphView facet_view(const NphFacetView *s)
     return h;
 }
 
-template <u32 T> struct NphSurfaceLds {
+template <u32 T, u32 CAP> struct NphSurfaceLds {
     NphClusterHead head[NPH_SURFACE_CLUSTERS];
     NphMPoint pts[NPH_SURFACE_CLUSTERS][NPH_SURFACE_CLUSTER_POINTS];
     u32 stack[NPH_BVH4_STACK];
@@ -539,7 +539,7 @@ template <u32 T> struct NphSurfaceLds {
     NphFacetHit hits[T];
     NphFacetView facet;
     NphMeshManifolds out;
-    NphPolyScratch<4U, NPH_CLIP_CAPACITY> poly;
+    NphPolyScratch<4U, CAP> poly; // the facet's corners, clip polygons
 };
 
 // The convex shape in the mesh's frame: at the origin, turned by `rotation`.
@@ -567,8 +567,8 @@ NPH_TD void thin_direction(u32 k, f32 *x, f32 *y)
 }
 
 // Thins cluster c (full) in place, keeping its order.
-template <u32 T>
-NPH_TD void cluster_thin(const NphTeam<T> &tm, NphSurfaceLds<T> *S, u32 c)
+template <u32 T, u32 CAP>
+NPH_TD void cluster_thin(const NphTeam<T> &tm, NphSurfaceLds<T, CAP> *S, u32 c)
 {
     constexpr u32 R = NPH_SURFACE_CLUSTER_POINTS / T; // points a lane
     NphMPoint *pts = S->pts[c];
@@ -637,8 +637,8 @@ NPH_TD void cluster_thin(const NphTeam<T> &tm, NphSurfaceLds<T> *S, u32 c)
 
 // One facet's contact (normal, `count` points at `pts`, the same on every
 // lane) merged into the clusters; `nc` is the cluster count (uniform).
-template <u32 T>
-NPH_TD void cluster_merge(const NphTeam<T> &tm, NphSurfaceLds<T> *S, u32 *nc, NphVec3 normal,
+template <u32 T, u32 CAP>
+NPH_TD void cluster_merge(const NphTeam<T> &tm, NphSurfaceLds<T, CAP> *S, u32 *nc, NphVec3 normal,
                           u32 count, const NphMPoint *pts, u32 facet, f32 speculative, bool single)
 {
     f32 depth = NPH_F32_MAX;
@@ -790,9 +790,60 @@ NPH_TD u32 capsule_keep(NphMPoint *pts, u32 count, NphVec3 axis)
 // The pair
 // ---------------------------------------------------------------------------
 
+// A polytope shape against one facet on the whole team: the separating
+// axes and face clipping of nph_polytope.h (the facet a flat polytope whose
+// face wins ties), then the facet's rules at the deepest point: a contact
+// from behind is dropped; one at an edge or corner that may not keep its
+// normal is clipped against the face itself instead, unless that reaches
+// much deeper than the pair's own contact (collide_mesh.c:590-629).
+template <u32 T, u32 CAP>
+NPH_TD void facet_polytope(const NphTeam<T> &tm, NphSurfaceLds<T, CAP> *S,
+                           const NphSurfaceShape &sh, const NphFacet &f, f32 speculative,
+                           NphTriContact *hit)
+{
+    tm.sync();
+    if (tm.lane == 0U) {
+        for (u32 k = 0U; k < 3U; ++k) {
+            S->facet.v[k] = f.v[k];
+        }
+        S->facet.planes[0] = plane_make(f.n, -v3_dot(f.n, f.v[0]));
+        S->facet.planes[1] = plane_make(v3_neg(f.n), v3_dot(f.n, f.v[0]));
+    }
+    tm.sync();
+    const NphView fv = facet_view(&S->facet);
+    tri_clear(hit);
+    team_polytopes(tm, &S->poly, &sh.view, sh.pose, &fv, pose_identity(), speculative, 0, 1,
+                   NPH_TRI_POINTS, hit);
+    if (hit->point_count == 0U || v3_dot(hit->normal, f.n) <= -NPH_SURFACE_FACE_COS) {
+        return;
+    }
+    u32 deepest = 0U;
+    for (u32 i = 1U; i < hit->point_count; ++i) {
+        deepest = hit->points[i].separation < hit->points[deepest].separation ? i : deepest;
+    }
+    const f32 depth = hit->points[deepest].separation;
+    const NphVec3 on = v3_mul_add(hit->points[deepest].point, hit->normal, 0.5F * depth);
+    const int verdict = facet_verdict(&f, facet_locate(&f, on), v3_neg(hit->normal));
+    if (verdict == NPH_VERDICT_KEEP) {
+        return;
+    }
+    if (verdict == NPH_VERDICT_FACE) {
+        team_hull_face(tm, &S->poly, &sh.view, sh.pose, &fv, pose_identity(), 0U, speculative,
+                       hit);
+        f32 s = NPH_F32_MAX;
+        for (u32 i = 0U; i < hit->point_count; ++i) {
+            s = nph_minf(s, hit->points[i].separation);
+        }
+        if (hit->point_count == 0U || s >= depth - NPH_PHYSICS_LINEAR_SLOP) {
+            return;
+        }
+    }
+    hit->point_count = 0U;
+}
+
 // The queued facets q[0..n) collided and merged in order.
-template <u32 T>
-NPH_TD void surface_chunk(const NphTeam<T> &tm, NphSurfaceLds<T> *S, const NphSurfaceShape &sh,
+template <u32 T, u32 CAP>
+NPH_TD void surface_chunk(const NphTeam<T> &tm, NphSurfaceLds<T, CAP> *S, const NphSurfaceShape &sh,
                           const NphPhysicsMesh *mesh, NphDVec3 origin, u32 n, f32 speculative,
                           int two_sided, u32 *nc)
 {
@@ -847,44 +898,8 @@ NPH_TD void surface_chunk(const NphTeam<T> &tm, NphSurfaceLds<T> *S, const NphSu
         if (!facet_make(&f, c, edges[t], two_sided, sh.center)) {
             continue;
         }
-        tm.sync();
-        if (tm.lane == 0U) {
-            for (u32 k = 0U; k < 3U; ++k) {
-                S->facet.v[k] = f.v[k];
-            }
-            S->facet.planes[0] = plane_make(f.n, -v3_dot(f.n, f.v[0]));
-            S->facet.planes[1] = plane_make(v3_neg(f.n), v3_dot(f.n, f.v[0]));
-        }
-        tm.sync();
-        const NphView fv = facet_view(&S->facet);
         NphTriContact hit;
-        tri_clear(&hit);
-        team_polytopes(tm, &S->poly, &sh.view, sh.pose, &fv, pose_identity(), speculative, 0, 1,
-                       NPH_TRI_POINTS, &hit);
-        if (hit.point_count > 0U && !(v3_dot(hit.normal, f.n) <= -NPH_SURFACE_FACE_COS)) {
-            u32 deepest = 0U;
-            for (u32 i = 1U; i < hit.point_count; ++i) {
-                deepest = hit.points[i].separation < hit.points[deepest].separation ? i : deepest;
-            }
-            const f32 depth = hit.points[deepest].separation;
-            const NphVec3 on = v3_mul_add(hit.points[deepest].point, hit.normal, 0.5F * depth);
-            const int verdict = facet_verdict(&f, facet_locate(&f, on), v3_neg(hit.normal));
-            if (verdict == NPH_VERDICT_FACE) {
-                // Clipped against the face itself instead, unless that
-                // reaches much deeper than the pair's own contact.
-                team_hull_face(tm, &S->poly, &sh.view, sh.pose, &fv, pose_identity(), 0U,
-                               speculative, &hit);
-                f32 s = NPH_F32_MAX;
-                for (u32 i = 0U; i < hit.point_count; ++i) {
-                    s = nph_minf(s, hit.points[i].separation);
-                }
-                if (hit.point_count > 0U && !(s >= depth - NPH_PHYSICS_LINEAR_SLOP)) {
-                    hit.point_count = 0U;
-                }
-            } else if (verdict == NPH_VERDICT_DROP) {
-                hit.point_count = 0U;
-            }
-        }
+        facet_polytope(tm, S, sh, f, speculative, &hit);
         if (hit.point_count > 0U) {
             cluster_merge(tm, S, nc, hit.normal, hit.point_count, hit.points, t, speculative,
                           false);
@@ -893,16 +908,12 @@ NPH_TD void surface_chunk(const NphTeam<T> &tm, NphSurfaceLds<T> *S, const NphSu
     tm.sync();
 }
 
-// Shape `g`, turned by `rotation` and placed at `position` in the mesh's
-// frame, against the mesh: the pair's manifolds (at most four, deepest
-// first) in S->out, in the mesh's frame relative to `position`.
-template <u32 T>
-NPH_TD void surface_collide(const NphTeam<T> &tm, NphSurfaceLds<T> *S, const NphGeom *g,
-                            NphQuat rotation, NphDVec3 position, const NphPhysicsMesh *mesh,
-                            f32 speculative_distance)
+// The convex shape of geometry `g` at the origin, turned by `rotation`; a
+// box's corners go to the team's LDS.
+template <u32 T, u32 CAP>
+NPH_TD NphSurfaceShape surface_shape(const NphTeam<T> &tm, NphSurfaceLds<T, CAP> *S,
+                                     const NphGeom *g, NphQuat rotation)
 {
-    const f32 speculative = nph_maxf(speculative_distance, 0.0F);
-    const int two_sided = (mesh->flags & NPH_PHYSICS_MESH_TWO_SIDED) != 0U;
     NphSurfaceShape sh;
     sh.radius = 0.0F;
     sh.core0 = sh.core1 = v3_zero();
@@ -924,6 +935,46 @@ NPH_TD void surface_collide(const NphTeam<T> &tm, NphSurfaceLds<T> *S, const Nph
             sh.center = q_rotate(rotation, g->hull->centroid);
         }
     }
+    return sh;
+}
+
+// One facet's contact with the shape on the whole team (every lane ends
+// with it): a ball's or a capsule's closed form, or a polytope's.
+template <u32 T, u32 CAP>
+NPH_TD void facet_contact(const NphTeam<T> &tm, NphSurfaceLds<T, CAP> *S,
+                          const NphSurfaceShape &sh, const NphFacet &f, f32 speculative,
+                          NphTriContact *out)
+{
+    if (sh.kind == 2U) {
+        facet_polytope(tm, S, sh, f, speculative, out);
+        return;
+    }
+    NphFacetHit h;
+    h.normal = v3(0.0F, 1.0F, 0.0F);
+    h.count = 0U;
+    if (sh.kind == 0U) {
+        ball_facet(&f, sh.radius, speculative, &h);
+    } else {
+        capsule_facet(&f, sh.core0, sh.core1, sh.radius, speculative, &h);
+    }
+    tri_clear(out);
+    out->normal = h.normal;
+    for (u32 i = 0U; i < h.count; ++i) {
+        tri_push(out, h.p[i].point, h.p[i].separation, h.p[i].id);
+    }
+}
+
+// Shape `g`, turned by `rotation` and placed at `position` in the mesh's
+// frame, against the mesh: the pair's manifolds (at most four, deepest
+// first) in S->out, in the mesh's frame relative to `position`.
+template <u32 T, u32 CAP>
+NPH_TD void surface_collide(const NphTeam<T> &tm, NphSurfaceLds<T, CAP> *S, const NphGeom *g,
+                            NphQuat rotation, NphDVec3 position, const NphPhysicsMesh *mesh,
+                            f32 speculative_distance)
+{
+    const f32 speculative = nph_maxf(speculative_distance, 0.0F);
+    const int two_sided = (mesh->flags & NPH_PHYSICS_MESH_TWO_SIDED) != 0U;
+    const NphSurfaceShape sh = surface_shape(tm, S, g, rotation);
     // The query box: the shape's box at its rounded place, grown by the
     // speculative distance and the rounding of that place.
     const NphVec3 at = v3((f32)position.x, (f32)position.y, (f32)position.z);

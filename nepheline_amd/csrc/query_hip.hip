@@ -13,24 +13,33 @@
 // nph_amd_collide_many_static is the same query for callers that cannot
 // allocate device memory: it stages the pairs, results and hull blocks of a
 // call in one fixed device array (`scratch`, below) and calls no allocator.
-//
-// Hulls with a face of more than NPH_HULL_MAX_FACE_VERTICES vertices are
-// refused here as the world refuses them: their clipped faces would not fit
-// the large-hull kernel's clip capacity (the reference's holds them).
 
 #include "device_impl.h"
 #include "device_narrow.h"
 
 #include <mutex>
 
+// One surface query on one team: a convex shape against one triangle
+// (mode 0) or against a mesh (mode 1), the world step's mesh narrow phase
+// (nph_surface.h). The shape's place and turn are in the surface's frame.
+struct NphSurfaceQuery {
+    NphGeom a;
+    NphQuat rotation;
+    NphDVec3 position;          // mesh: the shape in the mesh's frame (64 bits)
+    NphVec3 tri[3];             // triangle: its corners relative to the shape
+    u32 active;
+    u32 two_sided;
+    const NphPhysicsMesh *mesh; // device copy
+    f32 speculative;
+    u32 mode;
+};
+
 namespace {
 
-// A hull block the queries have seen: its device copy, and whether a face
-// of it is too wide for the clip buffers.
+// A hull block the queries have seen, and its device copy.
 struct QueryBlock {
     const void *host;
     const void *device;
-    int wide_face;
 };
 
 struct QueryContext {
@@ -47,6 +56,8 @@ struct QueryContext {
     u32 *count = NULL;
     QueryBlock *blocks = NULL;
     u32 block_count = 0U, block_cap = 0U;
+    NphSurfaceQuery *surface_in = NULL; // one surface query's input and output
+    NphMeshManifolds *surface_out = NULL;
 };
 
 QueryContext g_query;
@@ -72,28 +83,16 @@ NphResult query_ready(QueryContext *q)
     return NPH_RESULT_OK;
 }
 
-// True when a face of the hull has more vertices than the clip buffers of
-// the narrow phase take (the world refuses such a shape at creation).
-int hull_has_wide_face(const NphPhysicsHull *h)
-{
-    for (u32 f = 0U; f < h->face_count; ++f) {
-        if (nph_physics_hull_face_vertex_count(h, f) > NPH_HULL_MAX_FACE_VERTICES) {
-            return 1;
-        }
-    }
-    return 0;
-}
-
-// The device copy of a hull block (made once per block).
-NphResult query_block(QueryContext *q, const NphPhysicsHull *block, const void **out)
+// The device copy of a hull or mesh block of `bytes` bytes (made once per
+// block).
+NphResult query_block(QueryContext *q, const void *block, u32 bytes, const void **out)
 {
     for (u32 i = 0U; i < q->block_count; ++i) {
         if (q->blocks[i].host == block) {
             *out = q->blocks[i].device;
-            return q->blocks[i].wide_face ? NPH_RESULT_UNSUPPORTED : NPH_RESULT_OK;
+            return NPH_RESULT_OK;
         }
     }
-    const u32 bytes = block->byte_size;
     if (q->block_count == q->block_cap) {
         const u32 c = q->block_cap == 0U ? 16U : 2U * q->block_cap;
         QueryBlock *n = (QueryBlock *)realloc(q->blocks, (size_t)c * sizeof(QueryBlock));
@@ -103,25 +102,21 @@ NphResult query_block(QueryContext *q, const NphPhysicsHull *block, const void *
         q->blocks = n;
         q->block_cap = c;
     }
-    const int wide = hull_has_wide_face(block);
     void *p = NULL;
-    if (!wide) {
-        HIP_TRY(hipMalloc(&p, bytes));
-        HIP_TRY(hipMemcpy(p, block, bytes, hipMemcpyHostToDevice));
-    }
+    HIP_TRY(hipMalloc(&p, bytes));
+    HIP_TRY(hipMemcpy(p, block, bytes, hipMemcpyHostToDevice));
     q->blocks[q->block_count].host = block;
     q->blocks[q->block_count].device = p;
-    q->blocks[q->block_count].wide_face = wide;
     q->block_count += 1U;
     *out = p;
-    return wide ? NPH_RESULT_UNSUPPORTED : NPH_RESULT_OK;
+    return NPH_RESULT_OK;
 }
 
 NphResult query_geom(QueryContext *q, const NphPhysicsGeometry *g, NphGeom *out)
 {
     const void *block = NULL;
     if (g->type == NPH_PHYSICS_SHAPE_HULL) {
-        const NphResult r = query_block(q, g->hull, &block);
+        const NphResult r = query_block(q, g->hull, g->hull->byte_size, &block);
         if (r != NPH_RESULT_OK) {
             return r;
         }
@@ -212,9 +207,6 @@ NphResult static_blocks(const NphAmdCollideQuery *queries, u32 n, u64 base, Stat
             if (j < k) {
                 continue;
             }
-            if (hull_has_wide_face(h)) {
-                return NPH_RESULT_UNSUPPORTED;
-            }
             if (k == NPH_AMD_STATIC_MAX_HULLS) {
                 fprintf(stderr, "nepheline_amd: nph_amd_collide_many_static takes at most %u "
                                 "distinct hull blocks a call\n",
@@ -259,6 +251,41 @@ static void query_launch(hipStream_t s, const NphQueryPair *pairs, u32 n, bool a
     if (any_big) {
         hipLaunchKernelGGL(k_query_big_teams, dim3(devi_team_grid(n, NPH_BIG_TEAM_BLOCK)),
                            dim3(NPH_BIG_TEAM_BLOCK), 0, s, pairs, n, out, caches);
+    }
+}
+
+template <u32 CAP>
+__global__ void __launch_bounds__(NPH_SURFACE_TEAM) k_query_surface(const NphSurfaceQuery *in,
+                                                                    NphMeshManifolds *out)
+{
+    __shared__ NphSurfaceLds<NPH_SURFACE_TEAM, CAP> S;
+    const NphTeam<NPH_SURFACE_TEAM> tm;
+    const NphSurfaceQuery x = *in;
+    if (x.mode == 1U) {
+        surface_collide(tm, &S, &x.a, x.rotation, x.position, x.mesh, x.speculative);
+        if (tm.lane == 0U) {
+            *out = S.out;
+        }
+        return;
+    }
+    const NphSurfaceShape sh = surface_shape(tm, &S, &x.a, x.rotation);
+    NphFacet f;
+    NphTriContact c;
+    tri_clear(&c);
+    if (facet_make(&f, x.tri, x.active, (int)x.two_sided, sh.center)) {
+        facet_contact(tm, &S, sh, f, nph_maxf(x.speculative, 0.0F), &c);
+    }
+    if (tm.lane == 0U) {
+        c.point_count = manifold_reduce(c.points, c.point_count, c.normal);
+        NphManifold *m = &out->manifolds[0];
+        manifold_clear(m);
+        if (c.point_count > 0U) {
+            m->normal = c.normal;
+        }
+        for (u32 i = 0U; i < c.point_count; ++i) {
+            manifold_push(m, c.points[i].point, c.points[i].separation, c.points[i].id);
+        }
+        out->count = 1U;
     }
 }
 
@@ -381,6 +408,121 @@ void nph_physics_collide(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
     if (nph_amd_collide_many(&qy, 1U, out, c) != NPH_RESULT_OK) {
         out->normal = v3(0.0F, 1.0F, 0.0F);
         out->point_count = 0U;
+    }
+}
+
+// Runs one surface query and copies its manifolds back; false without a
+// device or on a refused input (the caller then reports no contact).
+static bool surface_query(NphSurfaceQuery *x, const NphPhysicsGeometry *a, NphMeshManifolds *out)
+{
+    QueryContext *q = &g_query;
+    std::lock_guard<std::mutex> guard(q->lock);
+    if (query_ready(q) != NPH_RESULT_OK || !convex(a->type) || query_geom(q, a, &x->a) !=
+                                                                     NPH_RESULT_OK) {
+        return false;
+    }
+    if (x->mode == 1U) {
+        const void *mesh = NULL;
+        if (query_block(q, x->mesh, x->mesh->byte_size, &mesh) != NPH_RESULT_OK) {
+            return false;
+        }
+        x->mesh = (const NphPhysicsMesh *)mesh;
+    }
+    if (q->surface_in == NULL &&
+        (hipMalloc((void **)&q->surface_in, sizeof(NphSurfaceQuery)) != hipSuccess ||
+         hipMalloc((void **)&q->surface_out, sizeof(NphMeshManifolds)) != hipSuccess)) {
+        q->surface_in = NULL;
+        return false;
+    }
+    hipError_t e = hipMemcpyAsync(q->surface_in, x, sizeof *x, hipMemcpyHostToDevice, q->stream);
+    if (e == hipSuccess) {
+        if (big_hull(a)) {
+            hipLaunchKernelGGL(k_query_surface<NPH_BIG_CLIP_CAPACITY>, dim3(1),
+                               dim3(NPH_SURFACE_TEAM), 0, q->stream, q->surface_in,
+                               q->surface_out);
+        } else {
+            hipLaunchKernelGGL(k_query_surface<NPH_CLIP_CAPACITY>, dim3(1), dim3(NPH_SURFACE_TEAM),
+                               0, q->stream, q->surface_in, q->surface_out);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(out, q->surface_out, sizeof *out, hipMemcpyDeviceToHost, q->stream);
+    }
+    if (e == hipSuccess) {
+        e = hipStreamSynchronize(q->stream);
+    }
+    if (e != hipSuccess) {
+        fprintf(stderr, "nepheline_amd: surface query failed: %s\n", hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+void nph_physics_collide_triangle(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                                  const NphPhysicsTriangle *t, f32 speculative_distance,
+                                  NphPhysicsManifold *out)
+{
+    // collide_mesh.c:631-657: the triangle moved next to the shape, which
+    // stays turned; points moved back by the shape's place.
+    NphSurfaceQuery x;
+    memset(&x, 0, sizeof x);
+    x.rotation = pose_a.rotation;
+    for (u32 i = 0U; i < 3U; ++i) {
+        x.tri[i] = v3_sub(t->vertices[i], pose_a.position);
+    }
+    x.active = t->active_edges & NPH_PHYSICS_MESH_EDGES;
+    x.two_sided = t->two_sided != 0 ? 1U : 0U;
+    x.speculative = speculative_distance;
+    x.mode = 0U;
+    NphMeshManifolds mm;
+    manifold_clear((NphManifold *)out);
+    if (!surface_query(&x, a, &mm)) {
+        return;
+    }
+    const NphManifold *m = &mm.manifolds[0];
+    out->normal = m->normal;
+    out->point_count = m->point_count;
+    for (u32 i = 0U; i < m->point_count; ++i) {
+        out->points[i].point = v3_add(m->points[i].point, pose_a.position);
+        out->points[i].separation = m->points[i].separation;
+        out->points[i].id = m->points[i].id;
+    }
+}
+
+void nph_physics_collide_mesh(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                              const NphPhysicsMesh *mesh, NphPhysicsPose pose_mesh,
+                              f32 speculative_distance, NphPhysicsMeshManifolds *out)
+{
+    // collide_mesh.c:1033-1079: the shape placed in the mesh's frame in 64
+    // bits, the manifolds turned back and moved by the shape's place.
+    NphSurfaceQuery x;
+    memset(&x, 0, sizeof x);
+    x.rotation = q_mul(q_conj(pose_mesh.rotation), pose_a.rotation);
+    x.position = drotate_inverse(pose_mesh.rotation,
+                                 dv3((f64)pose_a.position.x - (f64)pose_mesh.position.x,
+                                     (f64)pose_a.position.y - (f64)pose_mesh.position.y,
+                                     (f64)pose_a.position.z - (f64)pose_mesh.position.z));
+    x.mesh = mesh;
+    x.speculative = speculative_distance;
+    x.mode = 1U;
+    NphMeshManifolds mm;
+    out->count = 0U;
+    if (mesh == NULL || !surface_query(&x, a, &mm)) {
+        return;
+    }
+    out->count = mm.count;
+    for (u32 i = 0U; i < mm.count; ++i) {
+        const NphManifold *m = &mm.manifolds[i];
+        NphPhysicsManifold *o = &out->manifolds[i];
+        o->normal = q_rotate(pose_mesh.rotation, m->normal);
+        o->point_count = m->point_count;
+        for (u32 j = 0U; j < m->point_count; ++j) {
+            o->points[j].point = v3_add(pose_a.position, q_rotate(pose_mesh.rotation,
+                                                                  m->points[j].point));
+            o->points[j].separation = m->points[j].separation;
+            o->points[j].id = m->points[j].id;
+        }
     }
 }
 

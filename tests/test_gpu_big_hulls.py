@@ -146,40 +146,89 @@ def test_a_batch_with_large_hulls_matches_the_reference(gpu_api, ref_api, small_
     batch.close()
 
 
-def test_hull_faces_beyond_64_vertices_are_refused(gpu_api):
-    """A 100-segment cylinder's caps are 100-gons: refused
-    (NPH_RESULT_UNSUPPORTED); its 60-segment sibling (120 vertices, 60-gon
-    caps) is taken."""
+def _cylinders(api):
+    """Faceted cylinders with 100-gon and 128-gon caps (200 and 256
+    vertices): their clipped cap-on-cap faces hold up to 256 points, past the
+    reference's clip capacity of 264 where two caps are rotated."""
     import numpy as np
-    shapes = {}
-    for seg in (100, 60):
+    blocks = []
+    for seg in (100, 128):
         a = 2.0 * np.pi * np.arange(seg) / seg
         pts = np.zeros((2 * seg, 3), np.float32)
         pts[0::2] = np.stack([np.cos(a), np.full(seg, -0.5), np.sin(a)], 1)
         pts[1::2] = np.stack([np.cos(a), np.full(seg, 0.5), np.sin(a)], 1)
-        rc, block = P.build_hull(gpu_api, pts, 256)
+        rc, block = P.build_hull(api, pts, 256)
         assert rc == 0
-        shapes[seg] = S.HullBank(block, [0])
-    w = P.World(gpu_api, S.DESC)
-    b = w.add_body(P.BODY_DYNAMIC, (0.0, 2.0, 0.0))
-    with pytest.raises(RuntimeError, match="failed: 8"):
-        w.add_shape(b, P.geometry_hull(shapes[100].ptrs[0]))
-    w.add_shape(b, P.geometry_hull(shapes[60].ptrs[0]))
-    w.step(S.DT)
-    w.close()
-    # The pair queries refuse the 100-gon caps too (an empty manifold, and
-    # NPH_RESULT_UNSUPPORTED from the batched queries), and take the 60-gon's.
+        blocks.append(block + b"\0" * ((16 - len(block) % 16) % 16))
+    return S.HullBank(b"".join(blocks), [0, len(blocks[0])])
+
+
+def test_hull_faces_of_any_width_match_the_reference(gpu_api, ref_api):
+    """Caps of 100 and 128 vertices against each other (turned about the
+    axis, so every clip plane cuts), tilted and shifted, twice through the
+    cache, in single and batched queries; then cylinders dropped in a stack
+    on a box floor and on a mesh terrain, 60 steps: bit for bit against the
+    reference."""
+    bank = _cylinders(gpu_api)
     g = P.bind_collide(gpu_api)
-    low = P.Pose(P.Vec3(0.0, 0.0, 0.0), P.Quat(0.0, 0.0, 0.0, 1.0))
-    high = P.Pose(P.Vec3(0.0, 0.95, 0.0), P.Quat(0.0, 0.0, 0.0, 1.0))
-    for seg, touching in ((100, False), (60, True)):
-        geom = P.geometry_hull(shapes[seg].ptrs[0])
-        m = P.Manifold()
-        g.collide(C.byref(geom), low, C.byref(geom), high, 0.02, None, C.byref(m))
-        assert (m.point_count > 0) == touching, seg
-        q = (P.CollideQuery * 1)()
-        q[0].a, q[0].pose_a, q[0].b, q[0].pose_b = geom, low, geom, high
-        q[0].speculative_distance = 0.02
-        out = (P.Manifold * 1)()
-        rc = g.collide_many(q, 1, out, None)
-        assert rc == (P.OK if touching else P.UNSUPPORTED), seg
+    f = P.bind_collide(ref_api)
+    r = S.Pcg32(77)
+    queries = []
+    touching = 0
+    for i in range(120):
+        ga = P.geometry_hull(bank.ptrs[i % 2])
+        gb = P.geometry_hull(bank.ptrs[(i // 2) % 2])
+        tilt = float(r.next_range(0.0, 0.05))
+        turn = float(r.next_range(0.0, 6.283))
+        qa = P.Quat(0.0, 0.0, 0.0, 1.0)
+        qb = P.Quat(math.sin(tilt / 2) * math.cos(turn), math.sin(turn / 2) * math.cos(tilt / 2),
+                    math.sin(tilt / 2) * math.sin(turn), math.cos(turn / 2) * math.cos(tilt / 2))
+        n = math.sqrt(qb.x ** 2 + qb.y ** 2 + qb.z ** 2 + qb.w ** 2)
+        qb = P.Quat(qb.x / n, qb.y / n, qb.z / n, qb.w / n)
+        pa = P.Pose(P.Vec3(0.0, 0.0, 0.0), qa)
+        pb = P.Pose(P.Vec3(float(r.next_range(-0.3, 0.3)), float(r.next_range(0.97, 1.01)),
+                           float(r.next_range(-0.3, 0.3))), qb)
+        spec = float(r.next_range(0.0, 0.05))
+        cg, cr = P.CollideCache(), P.CollideCache()
+        for _ in range(2):
+            mg, mr = P.Manifold(), P.Manifold()
+            g.collide(C.byref(ga), pa, C.byref(gb), pb, spec, C.byref(cg), C.byref(mg))
+            f.collide(C.byref(ga), pa, C.byref(gb), pb, spec, C.byref(cr), C.byref(mr))
+            assert _manifold_bytes(mg) == _manifold_bytes(mr), i
+            assert bytes(cg) == bytes(cr), i
+        touching += mr.point_count > 0
+        queries.append((ga, pa, gb, pb, spec, mr))
+    assert touching > 60
+    qs = (P.CollideQuery * len(queries))()
+    for q, (ga, pa, gb, pb, spec, _) in zip(qs, queries):
+        q.a, q.pose_a, q.b, q.pose_b, q.speculative_distance = ga, pa, gb, pb, spec
+    out = (P.Manifold * len(queries))()
+    assert g.collide_many(qs, len(queries), out, None) == P.OK
+    for k, (*_, mr) in enumerate(queries):
+        assert _manifold_bytes(out[k]) == _manifold_bytes(mr), k
+    meshes = S.load_mesh_bank(os.path.join(GOLDEN, "meshes.bin"))
+
+    def stack(api, floor):
+        w = P.World(api, S.DESC)
+        if floor == "box":
+            ground = w.add_body(P.BODY_STATIC, (0.0, -1.0, 0.0))
+            w.add_shape(ground, P.geometry_box(20.0, 1.0, 20.0))
+        else:
+            ground = w.add_body(P.BODY_STATIC, (0.0, 0.0, 0.0))
+            w.add_shape(ground, P.geometry_mesh(meshes.ptrs[0]))
+        for k in range(6):
+            b = w.add_body(P.BODY_DYNAMIC, (0.3 * (k % 2), 4.0 + 1.05 * k, 0.0),
+                           rotation=(0.0, math.sin(0.2 * k), 0.0, math.cos(0.2 * k)))
+            w.add_shape(b, P.geometry_hull(bank.ptrs[k % 2]))
+        return w
+
+    for floor in ("box", "mesh"):
+        print(f"wide faces: a stack of cylinders on a {floor}, 60 steps", flush=True)
+        w, ref = stack(gpu_api, floor), stack(ref_api, floor)
+        for step in range(60):
+            w.step(S.DT)
+            ref.step(S.DT)
+            assert w.hash() == ref.hash(), f"{floor}: hash differs at step {step + 1}"
+        assert w.stats().touching_contacts >= 5
+        w.close()
+        ref.close()

@@ -101,7 +101,8 @@ def test_static_queries_refuse_what_does_not_fit(gpu_api, hull_bank):
     # A mesh is not a convex type.
     q[0].b = P.geometry_mesh(None)
     assert g.collide_many_static(q, 1, out, None) == P.INVALID_ARGUMENT
-    # A hull with a face of more than 64 vertices is refused, as everywhere.
+    # A hull with 100-gon faces is taken (the wide-clip build), as by the
+    # allocating query.
     import numpy as np
     seg = 100
     ang = 2.0 * np.pi * np.arange(seg) / seg
@@ -113,7 +114,13 @@ def test_static_queries_refuse_what_does_not_fit(gpu_api, hull_bank):
     wide = S.HullBank(block, [0])
     q[0].a = P.geometry_hull(wide.ptrs[0])
     q[0].b = P.geometry_hull(wide.ptrs[0])
-    assert g.collide_many_static(q, 1, out, None) == P.UNSUPPORTED
+    q[0].pose_b = P.Pose(P.Vec3(0.05, 0.98, 0.0), P.Quat(0.0, 0.0871557, 0.0, 0.9961947))
+    q[0].speculative_distance = 0.02
+    assert g.collide_many_static(q, 1, out, None) == P.OK
+    alloc = (P.Manifold * 1)()
+    assert g.collide_many(q, 1, alloc, None) == P.OK
+    assert out[0].point_count == 4
+    assert bytes(out[0]) == bytes(alloc[0])
     # More than 256 distinct hull blocks in one call.
     many = S.built_hull_bank(gpu_api, [8] * 260, seed=23)
     for i in range(260):
