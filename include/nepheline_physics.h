@@ -323,6 +323,34 @@ NphPhysicsMassData nph_physics_geometry_mass(const NphPhysicsGeometry *g, f32 de
 NphAABB nph_physics_geometry_aabb(const NphPhysicsGeometry *g, NphPhysicsPose pose);
 bool nph_physics_geometry_is_valid(const NphPhysicsGeometry *g);
 
+// types.h:132-142: mass properties carried to a parent frame, added, and
+// the parallel-axis tensor of a point mass.
+NphPhysicsMassData nph_physics_mass_transform(NphPhysicsMassData m, NphPhysicsPose pose);
+NphPhysicsMassData nph_physics_mass_add(NphPhysicsMassData a, NphPhysicsMassData b);
+NphMat3 nph_physics_inertia_offset(f32 mass, NphVec3 r);
+
+// geometry.h:125-151: the farthest point along a direction (rounding
+// included), the inner and outer radii about the centre of mass, and the
+// outward normal of the geometry's own surface at a point.
+NphVec3 nph_physics_geometry_support(const NphPhysicsGeometry *g, NphVec3 direction);
+f32 nph_physics_geometry_inner_radius(const NphPhysicsGeometry *g);
+f32 nph_physics_geometry_outer_radius(const NphPhysicsGeometry *g);
+NphVec3 nph_physics_geometry_surface_normal(const NphPhysicsGeometry *g, NphVec3 point,
+                                            u32 triangle, NphVec3 toward);
+
+// geometry.h:163-181: a convex shape as the hull of a few points rounded by
+// a radius (spheres one point, capsules two, boxes eight corners written to
+// `storage`, hulls their vertices).
+typedef struct NphPhysicsProxy {
+    const NphVec3 *points;
+    u32 count;
+    f32 radius;
+} NphPhysicsProxy;
+#define NPH_PHYSICS_PROXY_STORAGE 8U
+NphPhysicsProxy nph_physics_geometry_proxy(const NphPhysicsGeometry *g,
+                                           NphVec3 storage[NPH_PHYSICS_PROXY_STORAGE]);
+u32 nph_physics_proxy_support(const NphPhysicsProxy *p, NphVec3 direction);
+
 // ---------------------------------------------------------------------------
 // Pair queries (physics/collide.h:62-121): the narrow phase of one pair,
 // run on the device by the same code as the world step's.
@@ -359,6 +387,53 @@ void nph_physics_collide(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
                          NphPhysicsManifold *out);
 // collide.h:121: `count` candidates reduced to at most four, in place.
 u32 nph_physics_manifold_reduce(NphPhysicsManifoldPoint *points, u32 count, NphVec3 normal);
+
+// collide.h:129-168: a convex shape against one triangle and against a
+// mesh (up to four manifolds, the deepest first), on the device by the
+// world step's mesh narrow phase.
+typedef struct NphPhysicsTriangle {
+    NphVec3 vertices[3];
+    u32 active_edges; // NPH_PHYSICS_MESH_EDGE_* bits
+    b32 two_sided;
+} NphPhysicsTriangle;
+typedef struct NphPhysicsMeshManifolds {
+    u32 count;
+    NphPhysicsManifold manifolds[NPH_PHYSICS_MESH_MAX_MANIFOLDS];
+} NphPhysicsMeshManifolds;
+void nph_physics_collide_triangle(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                                  const NphPhysicsTriangle *t, f32 speculative_distance,
+                                  NphPhysicsManifold *out);
+void nph_physics_collide_mesh(const NphPhysicsGeometry *a, NphPhysicsPose pose_a,
+                              const NphPhysicsMesh *mesh, NphPhysicsPose pose_mesh,
+                              f32 speculative_distance, NphPhysicsMeshManifolds *out);
+
+// collide.h:181-235: closest points of two segments and of two proxies
+// (GJK, with an optional simplex cache).
+typedef struct NphPhysicsSegmentDistance {
+    NphVec3 closest1;
+    NphVec3 closest2;
+    f32 fraction1;
+    f32 fraction2;
+    f32 distance_squared;
+} NphPhysicsSegmentDistance;
+NphPhysicsSegmentDistance nph_physics_segment_distance(NphVec3 p1, NphVec3 q1, NphVec3 p2,
+                                                       NphVec3 q2);
+typedef struct NphPhysicsSimplexCache {
+    u32 count;
+    u8 index_a[4];
+    u8 index_b[4];
+} NphPhysicsSimplexCache;
+typedef struct NphPhysicsDistance {
+    NphVec3 point_a;
+    NphVec3 point_b;
+    NphVec3 normal; // from A towards B; +Y when the shapes touch or overlap
+    f32 distance;
+    u32 iterations;
+    u32 simplex_count;
+} NphPhysicsDistance;
+void nph_physics_distance(const NphPhysicsProxy *a, NphPhysicsPose pose_a, const NphPhysicsProxy *b,
+                          NphPhysicsPose pose_b, bool use_radii, NphPhysicsSimplexCache *cache,
+                          NphPhysicsDistance *out);
 
 // ---------------------------------------------------------------------------
 // Casts and overlaps against one geometry (collide.h:240-318, cast.c), host
@@ -791,6 +866,10 @@ void nph_physics_world_collide_shape(const NphPhysicsWorld *world,
                                      NphQuat rotation, f32 speculative_distance,
                                      const NphPhysicsQueryFilter *filter,
                                      NphPhysicsQueryContactFn fn, void *user);  // query.c:548-566
+// query.h:180-181: the outward normal of a shape's own surface at a hit's
+// world point (nph_physics_geometry_surface_normal in the shape's frame).
+NphVec3 nph_physics_world_surface_normal(const NphPhysicsWorld *world, NphPhysicsShape shape,
+                                         NphDVec3 point, u32 triangle, NphVec3 toward);
 
 // ---------------------------------------------------------------------------
 // Joints (joint.h:51-272).  Ball, hinge and fixed joints are solved; the

@@ -102,6 +102,7 @@ void proxy_distance(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, Nph
     }
     gjk_finish(&g, &out->d);
     out->simplex_count = g.count;
+    out->iterations = g.iterations;
     if (rounded) {
         NphDistance *d = &out->d;
         const f32 ra = a->radius;

@@ -610,3 +610,25 @@ void nph_physics_world_collide_shape(const NphPhysicsWorld *world,
     vbuf_free(sq.q.w->alloc, &sq.pairs, sizeof(NphAmdCollideQuery));
     vbuf_free(sq.q.w->alloc, &sq.targets, 4U);
 }
+
+// query.c:585-605: the shape's own surface normal at a world point, the
+// point and `toward` brought into the shape's frame (64 bits to its origin).
+NphVec3 nph_physics_world_surface_normal(const NphPhysicsWorld *world, NphPhysicsShape shape,
+                                         NphDVec3 point, u32 triangle, NphVec3 toward)
+{
+    const HShape *s = world != NULL ? world_shape(world, shape) : NULL;
+    const int finite = __builtin_isfinite(point.x) && __builtin_isfinite(point.y) &&
+                       __builtin_isfinite(point.z) &&
+                       nph_isfinitef(toward.x) && nph_isfinitef(toward.y) && nph_isfinitef(toward.z);
+    Query q;
+    if (s == NULL || !finite || !query_begin(&q, world, NULL, point)) {
+        return toward;
+    }
+    NphQuat rotation;
+    NphDVec3 offset;
+    shape_frame(&q, s, &rotation, &offset);
+    const NphVec3 n = nph_physics_geometry_surface_normal(
+        &s->geometry, narrow(origin_in_shape(rotation, offset)), triangle,
+        q_rotate_inv(rotation, toward));
+    return q_rotate(rotation, n);
+}

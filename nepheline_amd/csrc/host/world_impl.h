@@ -324,6 +324,7 @@ typedef struct NphSimplexCache {
 typedef struct NphProxyDistance {
     NphDistance d;
     u32 simplex_count; // 4: the cores overlap
+    u32 iterations;    // of the GJK loop
 } NphProxyDistance;
 NphProxy proxy_of(const NphPhysicsGeometry *g, NphVec3 storage[8]);
 void proxy_distance(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, NphPhysicsPose pb,

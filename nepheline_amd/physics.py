@@ -109,6 +109,69 @@ def bind_collide(a):
     return a
 
 
+class Proxy(C.Structure):
+    """geometry.h:163-167"""
+    _fields_ = [("points", C.c_void_p), ("count", C.c_uint32), ("radius", C.c_float)]
+
+
+class SegmentDistance(C.Structure):
+    _fields_ = [("closest1", Vec3), ("closest2", Vec3), ("fraction1", C.c_float),
+                ("fraction2", C.c_float), ("distance_squared", C.c_float)]
+
+
+class SimplexCache(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("index_a", C.c_uint8 * 4), ("index_b", C.c_uint8 * 4)]
+
+
+class Distance(C.Structure):
+    _fields_ = [("point_a", Vec3), ("point_b", Vec3), ("normal", Vec3), ("distance", C.c_float),
+                ("iterations", C.c_uint32), ("simplex_count", C.c_uint32)]
+
+
+class Triangle(C.Structure):
+    _fields_ = [("vertices", Vec3 * 3), ("active_edges", C.c_uint32), ("two_sided", C.c_int32)]
+
+
+class MeshManifolds(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("manifolds", Manifold * 4)]
+
+
+def bind_geometry(a):
+    """The geometry, mass and closest-point functions (geometry.h:120-181,
+    types.h:130-142, collide.h:129-235)."""
+    L = a.lib
+    G = C.POINTER(Geometry)
+    a.geometry_support = _bind(L, "nph_physics_geometry_support", Vec3, [G, Vec3])
+    a.geometry_inner_radius = _bind(L, "nph_physics_geometry_inner_radius", C.c_float, [G])
+    a.geometry_outer_radius = _bind(L, "nph_physics_geometry_outer_radius", C.c_float, [G])
+    a.geometry_surface_normal = _bind(L, "nph_physics_geometry_surface_normal", Vec3,
+                                      [G, Vec3, C.c_uint32, Vec3])
+    a.geometry_proxy = _bind(L, "nph_physics_geometry_proxy", Proxy, [G, C.POINTER(Vec3)])
+    a.proxy_support = _bind(L, "nph_physics_proxy_support", C.c_uint32,
+                            [C.POINTER(Proxy), Vec3])
+    a.mass_transform = _bind(L, "nph_physics_mass_transform", MassData, [MassData, Pose])
+    a.mass_add = _bind(L, "nph_physics_mass_add", MassData, [MassData, MassData])
+    a.inertia_offset = _bind(L, "nph_physics_inertia_offset", Mat3, [C.c_float, Vec3])
+    a.segment_distance = _bind(L, "nph_physics_segment_distance", SegmentDistance,
+                               [Vec3, Vec3, Vec3, Vec3])
+    a.distance = _bind(L, "nph_physics_distance", None,
+                       [C.POINTER(Proxy), Pose, C.POINTER(Proxy), Pose, C.c_bool,
+                        C.POINTER(SimplexCache), C.POINTER(Distance)])
+    return a
+
+
+def bind_surface_queries(a):
+    """A convex shape against one triangle and against a mesh
+    (collide.h:129-168)."""
+    L = a.lib
+    G = C.POINTER(Geometry)
+    a.collide_triangle = _bind(L, "nph_physics_collide_triangle", None,
+                               [G, Pose, C.POINTER(Triangle), C.c_float, C.POINTER(Manifold)])
+    a.collide_mesh = _bind(L, "nph_physics_collide_mesh", None,
+                           [G, Pose, C.c_void_p, Pose, C.c_float, C.POINTER(MeshManifolds)])
+    return a
+
+
 class HullDesc(C.Structure):
     """hull.h:111-132."""
     _fields_ = [("points", C.c_void_p), ("point_count", C.c_uint32),

@@ -1,0 +1,152 @@
+# SPDX-License-Identifier: Apache-2.0
+"""The geometry, mass-property and closest-point functions of the public API
+(geometry.h:120-181, types.h:130-142, collide.h:181-235; host code), each
+against the reference's own (oracle/_ref) on seeded inputs, bit for bit:
+support points, inner and outer radii and surface normals of every convex
+type and of meshes, proxies and their support indices, segment distances
+(degenerate segments included), GJK distances with and without radii and
+through a simplex cache, and mass properties carried and added."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from nepheline_amd import physics as P
+from nepheline_amd import scenes as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def libs(ref_api, hull_bank):
+    meshes = S.load_mesh_bank(os.path.join(GOLDEN, "meshes.bin"))
+    return P.bind_geometry(P.product_api()), P.bind_geometry(ref_api), meshes
+
+
+def _vec(r, lo=-1.0, hi=1.0):
+    return P.Vec3(*(float(r.next_range(lo, hi)) for _ in range(3)))
+
+
+def _geometries(r, hull_bank, meshes, n):
+    out = []
+    for i in range(n):
+        k = i % 5
+        if k == 0:
+            out.append(P.geometry_sphere(float(r.next_range(0.1, 1.0))))
+        elif k == 1:
+            out.append(P.geometry_capsule(float(r.next_range(0.0, 0.8)),
+                                          float(r.next_range(0.1, 0.5))))
+        elif k == 2:
+            out.append(P.geometry_box(*(float(r.next_range(0.1, 0.8)) for _ in range(3))))
+        elif k == 3:
+            out.append(P.geometry_hull(hull_bank.ptrs[r.next_bounded(len(hull_bank.ptrs))]))
+        else:
+            out.append(P.geometry_mesh(meshes.ptrs[r.next_bounded(len(meshes.ptrs))]))
+    return out
+
+
+def _b(x):
+    return bytes(x)
+
+
+def test_support_radii_and_surface_normals(libs, hull_bank):
+    g, f, meshes = libs
+    r = S.Pcg32(301)
+    for geom in _geometries(r, hull_bank, meshes, 400):
+        gp = C.byref(geom)
+        assert g.geometry_inner_radius(gp) == f.geometry_inner_radius(gp)
+        assert g.geometry_outer_radius(gp) == f.geometry_outer_radius(gp)
+        for j in range(6):
+            d = _vec(r) if j else P.Vec3(0.0, 0.0, 0.0)
+            s1, s2 = g.geometry_support(gp, d), f.geometry_support(gp, d)
+            assert _b(s1) == _b(s2)
+            # Normals at the support point (on the surface), nudged, and at
+            # a random point, for random `toward`; meshes at random triangles.
+            toward = _vec(r)
+            tri = r.next_bounded(80)
+            for p in (s1, P.Vec3(s1.x * 0.999, s1.y, s1.z), _vec(r, -0.5, 0.5)):
+                n1 = g.geometry_surface_normal(gp, p, tri, toward)
+                n2 = f.geometry_surface_normal(gp, p, tri, toward)
+                assert _b(n1) == _b(n2)
+
+
+def test_proxies_and_their_support(libs, hull_bank):
+    g, f, meshes = libs
+    r = S.Pcg32(302)
+    for geom in _geometries(r, hull_bank, meshes, 200):
+        if geom.type == 4:  # meshes have no proxy
+            continue
+        s1, s2 = (P.Vec3 * 8)(), (P.Vec3 * 8)()
+        p1 = g.geometry_proxy(C.byref(geom), s1)
+        p2 = f.geometry_proxy(C.byref(geom), s2)
+        assert (p1.count, p1.radius) == (p2.count, p2.radius)
+        pts1 = C.string_at(p1.points, 12 * p1.count)
+        assert pts1 == C.string_at(p2.points, 12 * p2.count)
+        for _ in range(4):
+            d = _vec(r)
+            assert g.proxy_support(C.byref(p1), d) == f.proxy_support(C.byref(p2), d)
+
+
+def test_segment_distances(libs):
+    g, f, _ = libs
+    r = S.Pcg32(303)
+    for i in range(2000):
+        p1, q1, p2, q2 = (_vec(r, -2.0, 2.0) for _ in range(4))
+        if i % 7 == 0:
+            q1 = p1  # a point
+        if i % 11 == 0:
+            q2 = p2
+        if i % 13 == 0:  # parallel
+            q2 = P.Vec3(p2.x + (q1.x - p1.x), p2.y + (q1.y - p1.y), p2.z + (q1.z - p1.z))
+        assert _b(g.segment_distance(p1, q1, p2, q2)) == _b(f.segment_distance(p1, q1, p2, q2))
+
+
+def test_gjk_distances(libs, hull_bank):
+    g, f, meshes = libs
+    r = S.Pcg32(304)
+    geoms = [x for x in _geometries(r, hull_bank, meshes, 300) if x.type != 4]
+    close = 0
+    for i in range(0, len(geoms) - 1):
+        ga, gb = geoms[i], geoms[i + 1]
+        sa, sb = (P.Vec3 * 8)(), (P.Vec3 * 8)()
+        pa = g.geometry_proxy(C.byref(ga), sa)
+        pb = g.geometry_proxy(C.byref(gb), sb)
+        posa = P.Pose(_vec(r, -0.5, 0.5), P.Quat(*S.rotation(r)))
+        posb = P.Pose(_vec(r, -1.5, 1.5), P.Quat(*S.rotation(r)))
+        for radii in (False, True):
+            c1, c2 = P.SimplexCache(), P.SimplexCache()
+            for _ in range(2):  # the second call starts from the cache
+                d1, d2 = P.Distance(), P.Distance()
+                g.distance(C.byref(pa), posa, C.byref(pb), posb, radii, C.byref(c1), C.byref(d1))
+                f.distance(C.byref(pa), posa, C.byref(pb), posb, radii, C.byref(c2), C.byref(d2))
+                assert _b(d1) == _b(d2), i
+                assert _b(c1) == _b(c2), i
+            d1, d2 = P.Distance(), P.Distance()
+            g.distance(C.byref(pa), posa, C.byref(pb), posb, radii, None, C.byref(d1))
+            f.distance(C.byref(pa), posa, C.byref(pb), posb, radii, None, C.byref(d2))
+            assert _b(d1) == _b(d2), i
+            close += d1.distance < 0.05
+    assert close > 10
+
+
+def test_mass_properties(libs):
+    g, f, _ = libs
+    r = S.Pcg32(305)
+
+    def mass():
+        m = P.MassData()
+        m.mass = float(r.next_range(0.0, 5.0)) if r.next_bounded(8) else 0.0
+        m.center = _vec(r)
+        for k in range(3):
+            m.inertia.c[k] = _vec(r, 0.0, 2.0)
+        return m
+
+    for _ in range(500):
+        a, b = mass(), mass()
+        pose = P.Pose(_vec(r, -3.0, 3.0), P.Quat(*S.rotation(r)))
+        assert _b(g.mass_transform(a, pose)) == _b(f.mass_transform(a, pose))
+        assert _b(g.mass_add(a, b)) == _b(f.mass_add(a, b))
+        off = _vec(r, -2.0, 2.0)
+        assert _b(g.inertia_offset(a.mass, off)) == _b(f.inertia_offset(a.mass, off))
