@@ -193,19 +193,115 @@ def train_network(args, bank):
 # ---------------------------------------------------------------------------
 # The batch (headline)
 # ---------------------------------------------------------------------------
-def measured_traffic(kernel, key):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), when
-    it was taken on this workload; else None."""
-    path = os.path.join(ROOT, "profiles", "%s_batch1024_traffic.json" % kernel)
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_b_color_pass_batch1024_traffic.json")
+# HBM read requests by size (the L2's misses to the memory fabric) and the
+# written kilobytes, per dispatch: two rocprofv3 --pmc passes.
+READ_COUNTERS = {"TCC_EA0_RDREQ_32B_sum": 32, "TCC_EA0_RDREQ_64B_sum": 64,
+                 "TCC_EA0_RDREQ_128B_sum": 128}
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the batched colour pass from the committed
+    counter summary, when it was taken on this workload; else None."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_JSON) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    if d.get("kernel") != kernel or d.get("workload_key") != key:
+    if d.get("kernel") != "k_b_color_pass" or d.get("workload_key") != key:
         return None
     return d
+
+
+def _profile_rows(directory, suffix):
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True)):
+        with open(path, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    if not rows:
+        raise SystemExit("no %s under %s" % (suffix, directory))
+    return rows
+
+
+def _timed_window(rows, groups=2, warmup=5):
+    """Dispatch ids of the plain batch's timed colour passes: after its
+    warm-up steps (each step ends with one k_b_finalize a world group) and
+    before the learned batch's first k_b_learned_features."""
+    ids = sorted({int(r["Dispatch_Id"]): r["Kernel_Name"] for r in rows}.items())
+    finals, window = 0, []
+    for did, name in ids:
+        if name.startswith("k_b_learned_features"):
+            break
+        if name.startswith("k_b_finalize"):
+            finals += 1
+        elif name.startswith("k_b_color_pass") and finals >= groups * warmup:
+            window.append(did)
+    return set(window)
+
+
+def summarize_profiles(trace_dir, read_dir, write_dir, copy_read_dir, copy_write_dir, key):
+    """The batched colour pass's HBM traffic and duration over the plain
+    batch's timed steps, from rocprofv3 runs of the bench's own command: a
+    kernel trace, and two counter passes (read requests by size; WRITE_SIZE)
+    restricted to k_b_color_pass, k_b_finalize and k_b_learned_features.
+    The same counters over a 256 MiB device copy (--copy-calibration) show
+    what they read for a known byte count."""
+    trace = [r for r in _profile_rows(trace_dir, "kernel_trace.csv")
+             if r["Kernel_Name"].startswith(("k_b_color_pass", "k_b_finalize",
+                                             "k_b_learned_features"))]
+    win = _timed_window(trace, warmup=key["warmup"])
+    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace
+            if int(r["Dispatch_Id"]) in win]
+
+    def per_dispatch(directory, names):
+        rows = _profile_rows(directory, "counter_collection.csv")
+        w = _timed_window(rows, warmup=key["warmup"])
+        acc = {}
+        for r in rows:
+            if int(r["Dispatch_Id"]) in w and r["Counter_Name"] in names:
+                acc.setdefault(int(r["Dispatch_Id"]), {})[r["Counter_Name"]] = float(
+                    r["Counter_Value"])
+        return acc
+
+    reads = per_dispatch(read_dir, READ_COUNTERS)
+    writes = per_dispatch(write_dir, {"WRITE_SIZE"})
+    read_b = [sum(v[c] * READ_COUNTERS[c] for c in READ_COUNTERS) for v in reads.values()]
+    write_b = [v["WRITE_SIZE"] * 1024.0 for v in writes.values()]
+
+    def copy_bytes(directory, fn):
+        rows = _profile_rows(directory, "counter_collection.csv")
+        acc = {}
+        for r in rows:
+            if "copy" in r["Kernel_Name"].lower():
+                acc.setdefault(int(r["Dispatch_Id"]), {})[r["Counter_Name"]] = float(
+                    r["Counter_Value"])
+        return [fn(v) for v in acc.values()]
+
+    copy_read = copy_bytes(copy_read_dir,
+                           lambda v: sum(v.get(c, 0.0) * READ_COUNTERS[c] for c in READ_COUNTERS))
+    copy_write = copy_bytes(copy_write_dir, lambda v: v.get("WRITE_SIZE", 0.0) * 1024.0)
+    out = {
+        "kernel": "k_b_color_pass",
+        "workload": "the bench's 1 024-world batch (seed 4), plain start: its timed steps",
+        "workload_key": key,
+        "window": "k_b_color_pass dispatches after the plain batch's warm-up steps (one "
+                  "k_b_finalize a world group a step) and before the learned batch's first "
+                  "k_b_learned_features",
+        "dispatches": {"trace": len(durs), "read_counters": len(read_b),
+                       "write_counters": len(write_b)},
+        "avg_duration_us": float(np.mean(durs)) / 1e3 if durs else None,
+        "read_bytes_per_dispatch": float(np.mean(read_b)) if read_b else None,
+        "write_bytes_per_dispatch": float(np.mean(write_b)) if write_b else None,
+        "read_counters": "32 TCC_EA0_RDREQ_32B + 64 TCC_EA0_RDREQ_64B + 128 TCC_EA0_RDREQ_128B",
+        "write_counter": "WRITE_SIZE (KiB)",
+        "copy_check": {"copy_bytes": 256 << 20, "read_bytes": copy_read,
+                       "write_bytes": copy_write},
+    }
+    out["hbm_bytes_per_launch"] = (out["read_bytes_per_dispatch"] or 0.0) + (
+        out["write_bytes_per_dispatch"] or 0.0)
+    return out
 
 
 def batch_rooflines(a, batch, plan, solved, window, key=None):
@@ -231,11 +327,11 @@ def batch_rooflines(a, batch, plan, solved, window, key=None):
                      window=window, launches=int(kt[1]), avg_launch_us=avg_s * 1e6, device_ms=kt[0],
                      bytes_per_launch=bytes_per_launch, bytes_per_contact_substep=mean_bytes,
                      contact_substeps=solved * SUBSTEPS)
-    counters = measured_traffic("k_b_color_pass", key)
+    counters = measured_traffic(key)
     if counters is not None:
         color["traffic"] = counters["hbm_bytes_per_launch"]
-        color["traffic_source"] = ("profiles/k_b_color_pass_batch1024_traffic.json (%s)"
-                                   % counters["window"])
+        color["traffic_source"] = ("profiles/k_b_color_pass_batch1024_traffic.json: "
+                                   "rocprofv3 counters of this command, %s" % counters["window"])
         color["traffic_over_algorithmic"] = counters["hbm_bytes_per_launch"] / bytes_per_launch
     claunches = max(int(ct[1]), 1)
     cavg_s = ct[0] * 1e-3 / claunches
@@ -553,11 +649,23 @@ def main():
     ap.add_argument("--train-capacity", type=int, default=40_000_000,
                     help="rows the trainer holds")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy 256 MiB device to device three times (the reference "
-                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
+                    help="only copy 256 MiB device to device three times (a known byte count "
+                         "for the memory counters) and exit")
+    ap.add_argument("--summarize-profiles", nargs=5, metavar=("TRACE", "READS", "WRITES",
+                                                              "COPY_READS", "COPY_WRITES"),
+                    help="write the colour pass's traffic summary from rocprofv3 output "
+                         "directories of this command and of --copy-calibration, and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.summarize_profiles:
+        key = {"worlds": args.batch_worlds, "seed": args.batch_seed, "steps": args.steps,
+               "warmup": args.warmup}
+        out = summarize_profiles(*args.summarize_profiles, key)
+        with open(TRAFFIC_JSON, "w") as f:
+            json.dump(out, f, indent=1)
+        print(json.dumps(out, indent=1))
         return
     if args.gpus != 1:
         print("bench.py: measures one GPU only; the world step has no multi-GPU path "
