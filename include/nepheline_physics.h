@@ -435,6 +435,18 @@ void nph_physics_distance(const NphPhysicsProxy *a, NphPhysicsPose pose_a, const
                           NphPhysicsPose pose_b, bool use_radii, NphPhysicsSimplexCache *cache,
                           NphPhysicsDistance *out);
 
+// collide.h:223-236: how deep two rounded proxies overlap (GJK, then an
+// expanding polytope when the cores overlap); false when they do not.
+typedef struct NphPhysicsPenetration {
+    NphVec3 normal;  // unit, from A towards B: B moved by depth * normal separates them
+    f32 depth;       // > 0
+    NphVec3 point_a; // the deepest point of A in B, on A's rounded surface
+    NphVec3 point_b; // the deepest point of B in A
+} NphPhysicsPenetration;
+bool nph_physics_penetration(const NphPhysicsProxy *a, NphPhysicsPose pose_a,
+                             const NphPhysicsProxy *b, NphPhysicsPose pose_b,
+                             NphPhysicsPenetration *out);
+
 // ---------------------------------------------------------------------------
 // Casts and overlaps against one geometry (collide.h:240-318, cast.c), host
 // code.  Meshes: rays and overlaps; a shape cast against a mesh is not

@@ -63,43 +63,50 @@ NphProxy proxy_of(const NphPhysicsGeometry *g, NphVec3 storage[8])
 
 static u32 support(const NphProxy *p, NphVec3 d) { return points_support(p->points, p->count, d); }
 
-// gjk.c:306-416 on the GJK stepper: supports searched in each proxy's own
+// gjk.c:306-374 on the GJK stepper: supports searched in each proxy's own
 // frame, vertices posed into the common one; the simplex starts from the
 // cache when it names valid points, and the cache keeps the final one.
-// With `rounded`, the radii round the cores (the points move onto the
-// rounded surfaces, or both to the middle when those touch).
-void proxy_distance(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, NphPhysicsPose pb,
-                    int rounded, NphSimplexCache *cache, NphProxyDistance *out)
+void proxy_gjk(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, NphPhysicsPose pb,
+               NphSimplexCache *cache, NphGjk *g)
 {
-    NphGjk g;
-    gjk_reset(&g);
+    gjk_reset(g);
     if (cache != NULL && cache->count >= 1U && cache->count <= 4U) {
         for (u32 i = 0U; i < cache->count; ++i) {
             const u32 ia = cache->index_a[i];
             const u32 ib = cache->index_b[i];
             if (ia >= a->count || ib >= b->count) {
-                gjk_reset(&g);
+                gjk_reset(g);
                 break;
             }
-            gjk_seed(&g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
+            gjk_seed(g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
         }
     }
-    if (g.count == 0U) {
-        gjk_seed(&g, 0U, pose_point(pa, a->points[0]), 0U, pose_point(pb, b->points[0]));
+    if (g->count == 0U) {
+        gjk_seed(g, 0U, pose_point(pa, a->points[0]), 0U, pose_point(pb, b->points[0]));
     }
-    gjk_begin(&g);
-    while (!g.done) {
-        const u32 ia = support(a, q_rotate_inv(pa.rotation, v3_neg(g.dir)));
-        const u32 ib = support(b, q_rotate_inv(pb.rotation, g.dir));
-        gjk_feed(&g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
+    gjk_begin(g);
+    while (!g->done) {
+        const u32 ia = support(a, q_rotate_inv(pa.rotation, v3_neg(g->dir)));
+        const u32 ib = support(b, q_rotate_inv(pb.rotation, g->dir));
+        gjk_feed(g, ia, pose_point(pa, a->points[ia]), ib, pose_point(pb, b->points[ib]));
     }
     if (cache != NULL) {
-        cache->count = g.count;
+        cache->count = g->count;
         for (u32 i = 0U; i < 4U; ++i) {
-            cache->index_a[i] = i < g.count ? (u8)g.v[i].ia : 0U;
-            cache->index_b[i] = i < g.count ? (u8)g.v[i].ib : 0U;
+            cache->index_a[i] = i < g->count ? (u8)g->v[i].ia : 0U;
+            cache->index_b[i] = i < g->count ? (u8)g->v[i].ib : 0U;
         }
     }
+}
+
+// gjk.c:376-416: the distance of the final simplex; with `rounded`, the
+// radii round the cores (the points move onto the rounded surfaces, or
+// both to the middle when those touch).
+void proxy_distance(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, NphPhysicsPose pb,
+                    int rounded, NphSimplexCache *cache, NphProxyDistance *out)
+{
+    NphGjk g;
+    proxy_gjk(a, pa, b, pb, cache, &g);
     gjk_finish(&g, &out->d);
     out->simplex_count = g.count;
     out->iterations = g.iterations;

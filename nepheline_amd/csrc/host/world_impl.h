@@ -327,6 +327,8 @@ typedef struct NphProxyDistance {
     u32 iterations;    // of the GJK loop
 } NphProxyDistance;
 NphProxy proxy_of(const NphPhysicsGeometry *g, NphVec3 storage[8]);
+void proxy_gjk(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, NphPhysicsPose pb,
+               NphSimplexCache *cache, NphGjk *g);
 void proxy_distance(const NphProxy *a, NphPhysicsPose pa, const NphProxy *b, NphPhysicsPose pb,
                     int rounded, NphSimplexCache *cache, NphProxyDistance *out);
 f32 proxy_gap(const NphProxy *m, NphPhysicsPose pose_m, const NphProxy *t, NphPhysicsPose pose_t,

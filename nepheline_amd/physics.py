@@ -128,6 +128,10 @@ class Distance(C.Structure):
                 ("iterations", C.c_uint32), ("simplex_count", C.c_uint32)]
 
 
+class Penetration(C.Structure):
+    _fields_ = [("normal", Vec3), ("depth", C.c_float), ("point_a", Vec3), ("point_b", Vec3)]
+
+
 class Triangle(C.Structure):
     _fields_ = [("vertices", Vec3 * 3), ("active_edges", C.c_uint32), ("two_sided", C.c_int32)]
 
@@ -157,6 +161,9 @@ def bind_geometry(a):
     a.distance = _bind(L, "nph_physics_distance", None,
                        [C.POINTER(Proxy), Pose, C.POINTER(Proxy), Pose, C.c_bool,
                         C.POINTER(SimplexCache), C.POINTER(Distance)])
+    a.penetration = _bind(L, "nph_physics_penetration", C.c_bool,
+                          [C.POINTER(Proxy), Pose, C.POINTER(Proxy), Pose,
+                           C.POINTER(Penetration)])
     return a
 
 
