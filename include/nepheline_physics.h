@@ -1117,9 +1117,15 @@ void nph_amd_kernel_times(const NphPhysicsWorld *world, f64 out[2]);
 #define NPH_AMD_MLP_OUTPUTS 3U
 #define NPH_AMD_MLP_PARAMS 6288U
 
-// y[r][0..3] = the network at x[r][0..15] (BF16), for r < rows.
+// y[r][0..3] = the network at x[r][0..15] (BF16), for r < rows. x must be
+// 8-byte and y 16-byte aligned (INVALID_ARGUMENT otherwise).
 NphResult nph_amd_mlp_forward(const void *x_bf16, u32 rows, const f32 *params, f32 *y,
                               void *stream);
+// The network's GELU at x[0..n): out[4 i + 0..3] = GELU and its derivative
+// by the approximation the kernels use, then both through the device's
+// erff and expf (a check of the approximation; no counterpart in the
+// reference).
+NphResult nph_amd_mlp_gelu(const f32 *x, u32 n, f32 *out, void *stream);
 // Adds the gradient of scale * sum (y - target)^2 over rows and outputs
 // 0..2 (targets as y's layout, 4 floats a row) into grad[NPH_AMD_MLP_PARAMS],
 // and scale * sum (y - target)^2 into *loss.

@@ -10,8 +10,11 @@
 // A wave works on 16 rows (contacts) at a time.  For the 16x16x16 MFMA,
 // lane l holds A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j][l % 16] and
 // D[4 (l / 16) + j][l % 16], j = 0..3.  An accumulator in D layout is
-// therefore already a B operand; the A operands go through the wave's LDS
-// tiles.  The weights are rounded to BF16 into LDS once per block.
+// therefore already a B operand.  Inference computes each layer
+// transposed so that activations stay in registers as B operands and the
+// weights, A operands, stay in VGPRs (k_mlp_forward).  The gradient kernel
+// passes A operands through the wave's LDS tiles, with the weights rounded
+// to BF16 into LDS once per block.
 //
 // Parameters are one FP32 array (the master copy Adam updates):
 //   W1 [16][64], b1 [64], W2 [64][64], b2 [64], W3 [64][16], b3 [16]
@@ -29,6 +32,7 @@
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
 
 #define IN 16
 #define HID 64
@@ -54,13 +58,69 @@ __device__ static inline unsigned short bf16_of(float f)
     return (unsigned short)(u >> 16U);
 }
 
-__device__ static inline float gelu(float x) { return 0.5F * x * (1.0F + erff(x * 0.70710678118F)); }
+// The standard normal distribution at x: its CDF by Hastings' rational
+// approximation (Abramowitz and Stegun 26.2.17: for a >= 0,
+// Phi(-a) = phi(a) (b1 t + b2 t^2 + ... + b5 t^5), t = 1 / (1 + p a),
+// |error| < 7.5e-8) and its density phi. The exponential and the
+// reciprocal run on the transcendental unit (v_exp_f32, v_rcp_f32), with
+// no branch; against the exact CDF the result stays within 5e-7 (float32
+// rounding included).
+struct Gauss {
+    float cdf;
+    float pdf;
+};
+
+#define HASTINGS_P 0.2316419F
+#define HASTINGS_B1 0.319381530F
+#define HASTINGS_B2 -0.356563782F
+#define HASTINGS_B3 1.781477937F
+#define HASTINGS_B4 -1.821255978F
+#define HASTINGS_B5 1.330274429F
+#define INV_SQRT_2PI 0.39894228040F
+#define HALF_LOG2E 0.72134752044F // exp(-a^2 / 2) = 2^(-a^2 log2(e) / 2)
+
+__device__ static inline Gauss gauss(float x)
+{
+    const float a = fabsf(x);
+    const float pdf = INV_SQRT_2PI * __builtin_amdgcn_exp2f(-HALF_LOG2E * a * a);
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(HASTINGS_P, a, 1.0F));
+    float poly = __builtin_fmaf(t, HASTINGS_B5, HASTINGS_B4);
+    poly = __builtin_fmaf(t, poly, HASTINGS_B3);
+    poly = __builtin_fmaf(t, poly, HASTINGS_B2);
+    poly = __builtin_fmaf(t, poly, HASTINGS_B1);
+    const float below = pdf * (t * poly); // Phi(-a)
+    return Gauss{x >= 0.0F ? 1.0F - below : below, pdf};
+}
+
+// GELU in its erf form, x Phi(x), and its derivative Phi(x) + x phi(x).
+// With a = |x|: x Phi(x) = max(x, 0) - a Phi(-a), and the density's
+// constant is folded into the polynomial (the inference kernel's form).
+__device__ static inline float gelu(float x)
+{
+    const float a = fabsf(x);
+    const float e = __builtin_amdgcn_exp2f(-HALF_LOG2E * a * a);
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(HASTINGS_P, a, 1.0F));
+    float poly = __builtin_fmaf(t, INV_SQRT_2PI * HASTINGS_B5, INV_SQRT_2PI * HASTINGS_B4);
+    poly = __builtin_fmaf(t, poly, INV_SQRT_2PI * HASTINGS_B3);
+    poly = __builtin_fmaf(t, poly, INV_SQRT_2PI * HASTINGS_B2);
+    poly = __builtin_fmaf(t, poly, INV_SQRT_2PI * HASTINGS_B1);
+    return __builtin_fmaf(-a, e * (t * poly), x > 0.0F ? x : 0.0F);
+}
 
 __device__ static inline float gelu_grad(float x)
 {
+    const Gauss g = gauss(x);
+    return g.cdf + x * g.pdf;
+}
+
+// The same two through the device library's erff and expf, for checking
+// the approximation.
+__device__ static inline float gelu_erf(float x) { return 0.5F * x * (1.0F + erff(x * 0.70710678118F)); }
+
+__device__ static inline float gelu_grad_erf(float x)
+{
     const float cdf = 0.5F * (1.0F + erff(x * 0.70710678118F));
-    const float pdf = 0.39894228040F * expf(-0.5F * x * x);
-    return cdf + x * pdf;
+    return cdf + x * (0.39894228040F * expf(-0.5F * x * x));
 }
 
 struct MlpShared {
@@ -199,30 +259,99 @@ __device__ static void forward_tile(const MlpShared *s, unsigned short *ta, unsi
     *y = acc;
 }
 
-// Inference: y[r][0..3] for r < n (x: n rows of 16 BF16).
+// GELU of four accumulators, rounded to BF16 (nearest even) two at a
+// time by v_cvt_pk_bf16_f32.
+__device__ static inline v4s activate(v4f z)
+{
+    const v4f g = {gelu(z[0]), gelu(z[1]), gelu(z[2]), gelu(z[3])};
+    return __builtin_bit_cast(v4s, __builtin_convertvector(g, v4bf));
+}
+
+// A operand of W^T from a row-major [K][ldn] FP32 matrix W, rounded to
+// BF16: A[m][k] = W[kb*16 + k][mb*16 + m], i.e. W[kb*16 + q + j][mb*16 + c].
+__device__ static inline v4s frag_wt(const float *w, unsigned ldn, unsigned kb, unsigned mb,
+                                     unsigned l)
+{
+    const float *p = w + (kb * 16U + 4U * (l >> 4U)) * ldn + mb * 16U + (l & 15U);
+    return v4s{(short)bf16_of(p[0]), (short)bf16_of(p[ldn]), (short)bf16_of(p[2U * ldn]),
+               (short)bf16_of(p[3U * ldn])};
+}
+
+// Inference: y[r][0..3] for r < n (x: n rows of 16 BF16), output 3 zero.
+//
+// Each layer is computed transposed, Z^T = W^T H^T, on 16 rows at a time:
+// the weights are the A operands, held in VGPRs for the wave's whole run
+// (24 fragments, 48 registers); the rows are the B operands. A D-layout
+// accumulator (lane l: Z^T[q + j][c], q = 4 (l / 16), c = l % 16) biased,
+// activated and rounded to BF16 is the next layer's B operand for the same
+// 16 features as it stands, so the layers chain in registers with no LDS
+// and no synchronisation. The input tile is the B operand of layer one as
+// loaded (lane l: row c, features q..q+3: 8 contiguous bytes), and lanes
+// 0..15 end holding outputs 0..3 of row c (one 16-byte store each).
 __global__ void __launch_bounds__(BLOCK) k_mlp_forward(const unsigned short *x, unsigned n,
                                                        const float *params, float *y)
 {
-    __shared__ MlpShared s;
-    load_params(&s, params);
     const unsigned l = threadIdx.x & 63U;
-    const unsigned w = threadIdx.x >> 6U;
+    const unsigned c = l & 15U;
+    const unsigned q = 4U * (l >> 4U);
+    v4s w1[4], w2[4][4], w3[4];
+    v4f b1[4], b2[4], b3; // biases in D layout: the accumulators' start
+    for (unsigned mb = 0U; mb < 4U; ++mb) {
+        w1[mb] = frag_wt(params + P_W1, HID, 0U, mb, l);
+        w3[mb] = frag_wt(params + P_W3, OUT, mb, 0U, l);
+        for (unsigned kb = 0U; kb < 4U; ++kb) {
+            w2[mb][kb] = frag_wt(params + P_W2, HID, kb, mb, l);
+        }
+        for (unsigned j = 0U; j < 4U; ++j) {
+            b1[mb][j] = params[P_B1 + mb * 16U + q + j];
+            b2[mb][j] = params[P_B2 + mb * 16U + q + j];
+        }
+        b3[mb] = params[P_B3 + q + mb];
+    }
     const unsigned waves = gridDim.x * WAVES;
     const unsigned tiles = (n + 15U) / 16U;
-    for (unsigned t = blockIdx.x * WAVES + w; t < tiles; t += waves) {
-        v4s xa;
-        v4f z1[4], z2[4], out;
-        forward_tile(&s, s.ta[w], s.tb[w], x, t * 16U, n, l, &xa, z1, z2, &out);
-        const unsigned col = l & 15U;
-        if (col < 4U) {
-            for (unsigned j = 0U; j < 4U; ++j) {
-                const unsigned row = t * 16U + 4U * (l >> 4U) + j;
-                if (row < n) {
-                    y[(unsigned long)row * 4U + col] = col < NOUT ? out[j] : 0.0F;
-                }
+    for (unsigned t = blockIdx.x * WAVES + (threadIdx.x >> 6U); t < tiles; t += waves) {
+        const unsigned row = t * 16U + c;
+        v4s xb = v4s{0, 0, 0, 0};
+        if (row < n) {
+            xb = *(const v4s *)(x + (unsigned long)row * IN + q);
+        }
+        v4s h1[4];
+        for (unsigned mb = 0U; mb < 4U; ++mb) {
+            const v4f z = mfma(w1[mb], xb, b1[mb]);
+            h1[mb] = activate(z);
+        }
+        v4f z2[4] = {b2[0], b2[1], b2[2], b2[3]};
+        for (unsigned kb = 0U; kb < 4U; ++kb) {
+            for (unsigned mb = 0U; mb < 4U; ++mb) {
+                z2[mb] = mfma(w2[mb][kb], h1[kb], z2[mb]);
             }
         }
-        wave_sync();
+        v4s h2[4];
+        for (unsigned mb = 0U; mb < 4U; ++mb) {
+            h2[mb] = activate(z2[mb]);
+        }
+        v4f o = b3;
+        for (unsigned kb = 0U; kb < 4U; ++kb) {
+            o = mfma(w3[kb], h2[kb], o);
+        }
+        if (q == 0U && row < n) {
+            *(float4 *)(y + (unsigned long)row * 4U) = make_float4(o[0], o[1], o[2], 0.0F);
+        }
+    }
+}
+
+// The GELU and its derivative at n points, by the approximation the
+// network uses and through erff, for checking one against the other.
+__global__ void __launch_bounds__(256) k_mlp_gelu(const float *x, unsigned n, float *out)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float v = x[i];
+        out[4UL * i] = gelu(v);
+        out[4UL * i + 1U] = gelu_grad(v);
+        out[4UL * i + 2U] = gelu_erf(v);
+        out[4UL * i + 3U] = gelu_grad_erf(v);
     }
 }
 
@@ -554,11 +683,26 @@ NphResult nph_amd_mlp_forward(const void *x_bf16, u32 rows, const f32 *params, f
     if (rows == 0U) {
         return NPH_RESULT_OK;
     }
-    if (x_bf16 == NULL || params == NULL || y == NULL) {
+    if (x_bf16 == NULL || params == NULL || y == NULL || ((uintptr_t)x_bf16 & 7U) != 0U ||
+        ((uintptr_t)y & 15U) != 0U) {
         return NPH_RESULT_INVALID_ARGUMENT;
     }
     hipLaunchKernelGGL(k_mlp_forward, dim3(blocks_for_rows(rows)), dim3(BLOCK), 0,
                        (hipStream_t)stream, (const unsigned short *)x_bf16, rows, params, y);
+    MLP_TRY(hipGetLastError());
+    return NPH_RESULT_OK;
+}
+
+NphResult nph_amd_mlp_gelu(const f32 *x, u32 n, f32 *out, void *stream)
+{
+    if (n == 0U) {
+        return NPH_RESULT_OK;
+    }
+    if (x == NULL || out == NULL) {
+        return NPH_RESULT_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(k_mlp_gelu, dim3((n + 255U) / 256U), dim3(256), 0, (hipStream_t)stream, x,
+                       n, out);
     MLP_TRY(hipGetLastError());
     return NPH_RESULT_OK;
 }

@@ -71,6 +71,7 @@ def _bind(a: P.Api):
     b("nph_amd_mlp_forward", C.c_int, [_VP, _U32, _VP, _VP, _VP])
     b("nph_amd_mlp_grad", C.c_int, [_VP, _VP, _U32, _VP, _F32, _VP, _VP, _VP])
     b("nph_amd_mlp_adam", C.c_int, [_VP, _VP, _VP, _VP, _U32, _F32, _VP])
+    b("nph_amd_mlp_gelu", C.c_int, [_VP, _U32, _VP, _VP])
     b("nph_amd_device_alloc", _VP, [C.c_uint64])
     b("nph_amd_device_free", None, [_VP])
     b("nph_amd_device_copy", C.c_int, [_VP, _VP, C.c_uint64])

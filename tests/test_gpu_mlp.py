@@ -3,8 +3,9 @@
 (oracle/mlp.py): the forward pass and the loss gradient, through the C ABI,
 on seeded inputs; and the network learns (Adam on its own gradients).
 Tolerances: the kernels and the oracle round the same values to BF16; they
-differ by FP32 versus float64 accumulation and by the device's erff and
-expf, so a GELU output can land one BF16 step apart.  Forward outputs agree
+differ by FP32 versus float64 accumulation and by the kernels' GELU
+approximation (within 5e-7 of the erf form, tested below), so a GELU
+output can land one BF16 step apart.  Forward outputs agree
 within 1e-2 of the output scale, gradients within 2e-2 relative (L2) — a
 wrong lane layout or a dropped term is off by order one."""
 import os
@@ -35,7 +36,7 @@ def _case(rng, n):
     return bits, mlp.from_bits(bits), t, p
 
 
-@pytest.mark.parametrize("n", [1, 17, 1000, 4099])
+@pytest.mark.parametrize("n", [1, 17, 1000, 4099, 300001])
 def test_forward_matches_oracle(gpu_api, n):
     print(f"mlp forward n={n}", flush=True)
     rng = np.random.default_rng(n)
@@ -98,3 +99,31 @@ def test_the_network_learns_a_teacher(gpu_api):
             net.adam(3e-3)
         hist.append(mean_loss())
     assert hist[-1] < 0.1 * before, (before, hist)
+
+
+def test_gelu_approximation_bound(gpu_api):
+    """The kernels' GELU (x Phi(x), Phi by Hastings' rational approximation,
+    Abramowitz and Stegun 26.2.17) and its derivative, on the device over
+    [-12, 12]: within 5e-7 (gelu) and 6e-7 (derivative) times max(1, |x|)
+    of the exact erf form (float64), and within 1e-6 times max(1, |x|) of
+    the same through the device's erff and expf. A wrong coefficient or a
+    dropped term misses by 1e-3 or more."""
+    from scipy.special import erf
+    print("mlp gelu bound", flush=True)
+    x = np.linspace(-12.0, 12.0, 1_000_001).astype(np.float32)
+    x = np.concatenate([x, np.float32([0.0, -0.0, 1e-30, -1e-30, 40.0, -40.0])])
+    f = L._bind(gpu_api)
+    dx = L.DeviceArray(gpu_api, x)
+    out = L.DeviceArray(gpu_api, shape=(len(x), 4), dtype=np.float32)
+    assert f.mlp_gelu(dx.ptr, len(x), out.ptr, None) == 0
+    got = out.numpy().astype(np.float64)
+    xd = x.astype(np.float64)
+    cdf = 0.5 * (1.0 + erf(xd / np.sqrt(2.0)))
+    gelu = xd * cdf
+    grad = cdf + xd * np.exp(-0.5 * xd * xd) / np.sqrt(2.0 * np.pi)
+    scale = np.maximum(1.0, np.abs(xd))
+    assert np.all(np.isfinite(got))
+    assert (np.abs(got[:, 0] - gelu) / scale).max() <= 5e-7
+    assert (np.abs(got[:, 1] - grad) / scale).max() <= 6e-7
+    assert (np.abs(got[:, 0] - got[:, 2]) / scale).max() <= 1e-6
+    assert (np.abs(got[:, 1] - got[:, 3]) / scale).max() <= 1e-6
