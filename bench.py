@@ -55,6 +55,10 @@ HOST_THREADS_PER_GPU = 16
 # V = 24 (v, w) read and written per pass, X = 28 (dp, dq) read by solve and
 # relax; P points, E dynamic endpoints of the contact.
 C_BYTES, K_BYTES, M_BYTES, V_BYTES, X_BYTES = 68, 116, 16, 24, 28
+# A joint in a colour, per pass: its configuration (112) and step state
+# (228) read, its eight accumulated impulses (56) written; its dynamic
+# bodies as a contact's (V, X above).
+J_BYTES = 112 + 228 + 56
 # Algorithmic bytes per pair of the narrow phase: the contact record read
 # and written (280 + 280), both shapes' geometry and relative pose (2 x 48),
 # both bodies' f64 origin and rotation (2 x 40), the sorted entry (16) and
@@ -94,7 +98,8 @@ def graph_bytes(world, static_slots):
     buf = (P.ContactInfo * n)()
     world.api.world_contacts(world.ptr, buf, n)
     arr = np.frombuffer(buf, dtype=np.dtype(P.ContactInfo), count=n)
-    sel = (arr["color"] != P.NULL32) & (arr["point_count"] > 0)
+    # Colours only: the overflow (colour 24) runs in its own kernel.
+    sel = (arr["color"] < 24) & (arr["point_count"] > 0)
     p = arr["point_count"][sel].astype(np.int64)
     e = (arr["body_a"][sel] >= static_slots).astype(np.int64) + (
         arr["body_b"][sel] >= static_slots).astype(np.int64)
@@ -309,10 +314,11 @@ def summarize_profiles(trace_dir, read_dir, write_dir, copy_read_dir, copy_write
     return out
 
 
-def batch_rooflines(a, batch, plan, solved, window, key=None):
+def batch_rooflines(a, batch, plan, solved, window, key=None, joints=(0, 0)):
     """The colour pass's and the convex narrow phase's rooflines from the
     dispatch-stamped HIP events around every launch of `window` (kernel
-    timing was on for its steps, which solved `solved` graph contacts)."""
+    timing was on for its steps, which solved `solved` graph contacts and
+    `joints` = (joints in colours, their dynamic ends), summed over steps)."""
     kt = (C.c_double * 2)()
     a.batch_kernel_times(batch.ptr, kt)
     ct = (C.c_double * 3)()
@@ -325,13 +331,15 @@ def batch_rooflines(a, batch, plan, solved, window, key=None):
         tot_n += n
     mean_bytes = tot_b / tot_n if tot_n else 0.0
     launches = max(int(kt[1]), 1)
-    bytes_per_launch = solved * SUBSTEPS * mean_bytes / launches
+    joint_bytes = SUBSTEPS * (joints[0] * 3 * J_BYTES + joints[1] * (3 * 2 * V_BYTES + 2 * X_BYTES))
+    bytes_per_launch = (solved * SUBSTEPS * mean_bytes + joint_bytes) / launches
     avg_s = kt[0] * 1e-3 / launches
     color = roofline("hbm", bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0, HBM_PEAK_GBS,
                      "GB/s", kernel="k_b_color_pass",
                      window=window, launches=int(kt[1]), avg_launch_us=avg_s * 1e6, device_ms=kt[0],
                      bytes_per_launch=bytes_per_launch, bytes_per_contact_substep=mean_bytes,
-                     contact_substeps=solved * SUBSTEPS)
+                     contact_substeps=solved * SUBSTEPS, joint_substeps=joints[0] * SUBSTEPS,
+                     joint_bytes=joint_bytes)
     counters = measured_traffic(key)
     if counters is not None:
         color["traffic"] = counters["hbm_bytes_per_launch"]
@@ -388,12 +396,15 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
         a.batch_time_kernels(batch.ptr, True)
     e0 = energy(batch.worlds)
     solved = 0
+    joints = [0, 0]
     phases = np.zeros(16)
     t0 = time.perf_counter()
     for i in range(args.steps):
         batch.step(DT)
         st = batch.stats()
         solved += st.solved_contacts
+        joints[0] += st.colored_joints
+        joints[1] += st.colored_joint_ends
         phases += (st.step_ms, st.narrow_ms, st.pairs_ms, st.changes_ms, st.solve_ms, st.post_ms,
                    st.device_solve_ms, st.device_passes_ms, st.device_collide_ms,
                    st.wait_narrow_ms, st.wait_solve_ms, st.device_pairs_ms, st.pairs_trees_ms,
@@ -422,18 +433,23 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
     if rooflines:
         window = "the %d timed steps" % args.steps
         solved_window = solved
+        joints_window = list(joints)
         if not timed_kernels:
             batch.set_tuning(tune.pass_block, tune.body_block, 0, tune.groups)
             a.batch_time_kernels(batch.ptr, True)
             solved_window = 0
+            joints_window = [0, 0]
             for _ in range(args.roofline_steps):
                 batch.step(DT)
-                solved_window += batch.stats().solved_contacts
+                st = batch.stats()
+                solved_window += st.solved_contacts
+                joints_window[0] += st.colored_joints
+                joints_window[1] += st.colored_joint_ends
             window = "%d steps after the timed region (stream launches)" % args.roofline_steps
         key = {"worlds": len(plan), "seed": args.batch_seed, "steps": args.steps,
                "warmup": args.warmup}
         out["rooflines"] = batch_rooflines(a, batch, plan, solved_window, window,
-                                           key if timed_kernels else None)
+                                           key if timed_kernels else None, joints_window)
         a.batch_time_kernels(batch.ptr, False)
     batch.close()
     return out

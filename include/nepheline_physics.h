@@ -1271,6 +1271,8 @@ typedef struct NphAmdBatchStats {
     f64 pairs_search_ms; // the searches: upload, launches, pairs back (wall)
     f64 pairs_create_ms; // the new contacts (wall)
     f64 rebuild_cpu_ms;  // the trees' rebuilds, summed over the worlds' threads
+    u64 colored_joints;  // the last step's joints in colours, all worlds
+    u64 colored_joint_ends; // their dynamic bodies
 } NphAmdBatchStats;
 
 NphResult nph_amd_batch_create(const NphPhysicsWorldDesc *desc, u32 world_count,

@@ -396,6 +396,8 @@ static NphResult group_to_solve(NphAmdBatch *b, u32 g, f32 dt)
             b->active[s0 + active] = i;
             active += 1U;
             b->stats.solved_contacts += w->solved_contacts;
+            b->stats.colored_joints += w->solved_joints;
+            b->stats.colored_joint_ends += w->solved_joint_ends;
             b->stats.awake_bodies += w->awake_bodies.len;
         } else {
             w->solved_contacts = 0U;
@@ -448,6 +450,8 @@ NphResult nph_amd_batch_step(NphAmdBatch *b, f32 dt)
     NphAmdBatchStats *st = &b->stats;
     st->solved_contacts = 0U;
     st->awake_bodies = 0U;
+    st->colored_joints = 0U;
+    st->colored_joint_ends = 0U;
     st->narrow_ms = st->pairs_ms = st->changes_ms = st->solve_ms = st->post_ms = 0.0;
     st->device_pairs_ms = st->pairs_trees_ms = st->pairs_search_ms = st->pairs_create_ms = 0.0;
     st->rebuild_cpu_ms = 0.0;

@@ -1133,6 +1133,18 @@ NphResult world_solve_input(NphPhysicsWorld *w, f32 dt, NphSolveInput *out)
         at += in.joint_color_count[c];
     }
     in.graph_joints = VB(w->step_joints, u32);
+    // The colour passes' joints and their dynamic ends, for measurement.
+    w->solved_joints = 0U;
+    w->solved_joint_ends = 0U;
+    for (u32 c = 0U; c < NPH_GRAPH_COLORS; ++c) {
+        w->solved_joints += w->color_joints[c].len;
+        for (u32 i = 0U; i < w->color_joints[c].len; ++i) {
+            const HJoint *j = &VB(w->joints, HJoint)[VB(w->color_joints[c], u32)[i]];
+            w->solved_joint_ends +=
+                (VB(w->bodies, HBody)[j->body_a].type == (u32)NPH_PHYSICS_BODY_DYNAMIC ? 1U : 0U) +
+                (VB(w->bodies, HBody)[j->body_b].type == (u32)NPH_PHYSICS_BODY_DYNAMIC ? 1U : 0U);
+        }
+    }
     NphStepParams *sp = &in.params;
     sp->gravity = w->desc.gravity;
     sp->dt = dt;

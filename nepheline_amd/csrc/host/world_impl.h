@@ -278,6 +278,8 @@ struct NphPhysicsWorld {
     f32 step_dt;
     f32 last_substep; // the last step's substep length (the joint reaction's)
     u32 solved_contacts; // the last step's graph contacts
+    u32 solved_joints;     // the last step's joints in colours (not the overflow)
+    u32 solved_joint_ends; // their dynamic bodies
 };
 
 u32 phys_handle(u32 slot, u32 generation);
