@@ -884,6 +884,34 @@ NphVec3 nph_physics_world_surface_normal(const NphPhysicsWorld *world, NphPhysic
                                          NphDVec3 point, u32 triangle, NphVec3 toward);
 
 // ---------------------------------------------------------------------------
+// Debug drawing (world.h:291-325): lines through a callback, in 32-bit
+// coordinates relative to `origin`; colours 0xAABBGGRR.
+// ---------------------------------------------------------------------------
+enum {
+    NPH_PHYSICS_DRAW_SHAPES = 1U << 0U,
+    NPH_PHYSICS_DRAW_AABBS = 1U << 1U,    // the broadphase's fattened boxes
+    NPH_PHYSICS_DRAW_CONTACTS = 1U << 2U, // touching points and normals
+    NPH_PHYSICS_DRAW_MASS = 1U << 3U,     // centres of mass
+    NPH_PHYSICS_DRAW_SLEEP = 1U << 4U,    // sleeping bodies' shapes in their own colour
+    NPH_PHYSICS_DRAW_JOINTS = 1U << 5U,   // anchors to the centres, and the frames' x axes
+};
+#define NPH_PHYSICS_DRAW_SENSOR_RGBA 0xFF40D0E0U
+#define NPH_PHYSICS_DRAW_GHOST_RGBA 0xFFE0A0C0U
+typedef struct NphPhysicsDebugDraw {
+    void (*line)(void *user, NphVec3 a, NphVec3 b, u32 rgba);
+    void *user;
+    NphDVec3 origin;
+    u32 flags; // NPH_PHYSICS_DRAW_*
+} NphPhysicsDebugDraw;
+// The world through `draw`, between steps (reads the device's poses and
+// contact records).
+void nph_physics_world_draw(const NphPhysicsWorld *world, const NphPhysicsDebugDraw *draw);
+// One shape's outline, as the world draws its shapes: `geometry` at `pose`
+// relative to `base` (draw's flags are not read).
+void nph_physics_geometry_draw(const NphPhysicsDebugDraw *draw, const NphPhysicsGeometry *geometry,
+                               NphDVec3 base, NphPhysicsPose pose, u32 rgba);
+
+// ---------------------------------------------------------------------------
 // Joints (joint.h:51-272).  Ball, hinge and fixed joints are solved; the
 // slider, distance and 6-DOF types are declared for binding compatibility and
 // rejected by nph_physics_joint_create with NPH_RESULT_UNSUPPORTED.

@@ -1016,6 +1016,8 @@ static f64 now_ms(void)
 
 static NphResult read_contacts(NphPhysicsWorld *w);
 
+NphResult world_read_contacts(NphPhysicsWorld *w) { return read_contacts(w); }
+
 // solver.c:919-958: after the step, one hit a pair for the graph's
 // contacts that asked for them, in colour order: the pair's fastest
 // approaching point (over a mesh pair's chain) that carried load, when it

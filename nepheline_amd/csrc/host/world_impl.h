@@ -290,6 +290,8 @@ void slots_release(NphAllocator *a, NphSlots *s, u32 slot);
 HBody *world_body(const NphPhysicsWorld *w, NphPhysicsBody h);
 HShape *world_shape(const NphPhysicsWorld *w, NphPhysicsShape h);
 NphResult world_sync_bodies(NphPhysicsWorld *w); // device -> host copy when stale
+// The device's contact records into contact_view (NphContactRec by record).
+NphResult world_read_contacts(NphPhysicsWorld *w);
 NphResult world_upload_body(NphPhysicsWorld *w, u32 slot);
 NphResult world_wake_body(NphPhysicsWorld *w, u32 slot);
 

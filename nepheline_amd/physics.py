@@ -167,6 +167,39 @@ def bind_geometry(a):
     return a
 
 
+DRAW_SHAPES, DRAW_AABBS, DRAW_CONTACTS, DRAW_MASS, DRAW_SLEEP, DRAW_JOINTS = (
+    1, 2, 4, 8, 16, 32)
+DrawLineFn = C.CFUNCTYPE(None, C.c_void_p, Vec3, Vec3, C.c_uint32)
+
+
+class DebugDraw(C.Structure):
+    """world.h:310-315"""
+    _fields_ = [("line", DrawLineFn), ("user", C.c_void_p), ("origin", DVec3),
+                ("flags", C.c_uint32)]
+
+
+class LineRecorder:
+    """A debug draw whose lines are kept as (a, b, rgba) byte strings."""
+
+    def __init__(self, origin=(0.0, 0.0, 0.0), flags=0):
+        self.lines = []
+
+        def line(_user, a, b, rgba):
+            self.lines.append(bytes(a) + bytes(b) + int(rgba).to_bytes(4, "little"))
+
+        self._fn = DrawLineFn(line)
+        self.draw = DebugDraw(self._fn, None, DVec3(*origin), flags)
+
+
+def bind_draw(a):
+    """Debug drawing (world.h:291-325)."""
+    L = a.lib
+    a.world_draw = _bind(L, "nph_physics_world_draw", None, [C.c_void_p, C.POINTER(DebugDraw)])
+    a.geometry_draw = _bind(L, "nph_physics_geometry_draw", None,
+                            [C.POINTER(DebugDraw), C.POINTER(Geometry), DVec3, Pose, C.c_uint32])
+    return a
+
+
 def bind_surface_queries(a):
     """A convex shape against one triangle and against a mesh
     (collide.h:129-168)."""
