@@ -10,9 +10,8 @@
 // The step runs the reference's phases in its order: new pairs (host tree),
 // the narrow phase (device), the changes it found (host), the solver and
 // finalize (device), continuous sweeps of the fast bodies (host), the
-// proxies that grew (host).  The world keeps sleep off (the reference's
-// throughput setting); asking for it, or for the TGS or XPBD solver, is
-// NPH_RESULT_UNSUPPORTED.
+// proxies that grew (host), then islands and sleep (host).  The TGS and
+// XPBD solvers, and sleep in a batch's worlds, are NPH_RESULT_UNSUPPORTED.
 #define _POSIX_C_SOURCE 200809L
 #include "world_impl.h"
 
