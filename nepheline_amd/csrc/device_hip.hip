@@ -99,7 +99,7 @@ void nph_dev_destroy(NphDevice *d)
                     d->fast_list, d->fast_pack, d->gate, d->mlp_params, d->lx, d->lraw, d->ly,
                     d->lres, d->wstats, d->log_x, d->log_t, d->log_count, d->team_small,
                     d->team_big, d->team_mesh, d->team_counts, d->pf_nodes[0], d->pf_nodes[1],
-                    d->pf_nodes[2], d->pf_shapes, d->pf_stamp, d->pf_set};
+                    d->pf_nodes[2], d->pf_shapes, d->pf_stamp, d->pf_set, d->contacts_saved};
     for (u32 i = 0U; i < sizeof bufs / sizeof bufs[0]; ++i) {
         if (bufs[i] != NULL) {
             (void)hipFree(bufs[i]);
@@ -507,6 +507,46 @@ NphResult nph_dev_read_contacts(NphDevice *d, NphContactRec *out, u32 count)
                                d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
     }
+    return NPH_RESULT_OK;
+}
+
+// The first `count` contact records copied aside (stream-ordered), for
+// nph_dev_contacts_restore.
+NphResult nph_dev_contacts_checkpoint(NphDevice *d, u32 count)
+{
+    const NphResult r = devi_flush_staged(d);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
+    d->contacts_saved_count = 0U;
+    if (count == 0U) {
+        return NPH_RESULT_OK;
+    }
+    if (count > d->cap_contacts_saved) {
+        const u32 cap = devi_grow_cap(d->cap_contacts_saved, count);
+        const NphResult g = devi_grow_buffer((void **)&d->contacts_saved, 0U,
+                                             (u64)cap * sizeof(NphContactRec), d->stream, 0);
+        if (g != NPH_RESULT_OK) {
+            return g;
+        }
+        d->cap_contacts_saved = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(d->contacts_saved, d->contacts, (u64)count * sizeof(NphContactRec),
+                           hipMemcpyDeviceToDevice, d->stream));
+    d->contacts_saved_count = count;
+    return NPH_RESULT_OK;
+}
+
+// The records the last checkpoint saved put back, after the device's work
+// so far (the narrow phase queued since) has run.
+NphResult nph_dev_contacts_restore(NphDevice *d)
+{
+    if (d->contacts_saved_count > 0U) {
+        HIP_TRY(hipMemcpyAsync(d->contacts, d->contacts_saved,
+                               (u64)d->contacts_saved_count * sizeof(NphContactRec),
+                               hipMemcpyDeviceToDevice, d->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(d->stream));
     return NPH_RESULT_OK;
 }
 

@@ -63,6 +63,10 @@ struct NphDevice {
     NphShapeRec *shapes;
     NphContactRec *contacts;
     u32 cap_bodies, cap_shapes, cap_contacts;
+    // The contact records as the step's narrow phase found them, put back
+    // when the step is abandoned before its changes are applied.
+    NphContactRec *contacts_saved;
+    u32 cap_contacts_saved, contacts_saved_count;
     // Per step.
     u32 *solver_of_body; // cap_bodies
     u32 *awake_bodies;   // cap_bodies

@@ -78,11 +78,12 @@ static u32 map_find(const NphPairMap *m, u64 key)
 
 static NphResult map_insert(NphAllocator *a, NphPairMap *m, u64 key, u32 value);
 
-static NphResult map_grow(NphAllocator *a, NphPairMap *m)
+// The map rebuilt at capacity `cap` (removed keys dropped).
+static NphResult map_rebuild(NphAllocator *a, NphPairMap *m, u32 cap)
 {
     NphPairMap n;
     memset(&n, 0, sizeof n);
-    n.cap = m->count * 4U > m->cap ? m->cap * 2U : m->cap;
+    n.cap = cap;
     n.keys = (u64 *)halloc(a, (u64)n.cap * 8U);
     n.values = (u32 *)halloc(a, (u64)n.cap * 4U);
     if (n.keys == NULL || n.values == NULL) {
@@ -99,6 +100,24 @@ static NphResult map_grow(NphAllocator *a, NphPairMap *m)
     pairmap_free(a, m);
     *m = n;
     return NPH_RESULT_OK;
+}
+
+static NphResult map_grow(NphAllocator *a, NphPairMap *m)
+{
+    return map_rebuild(a, m, m->count * 4U > m->cap ? m->cap * 2U : m->cap);
+}
+
+// Room for `extra` more keys without growing.
+static NphResult map_reserve(NphAllocator *a, NphPairMap *m, u32 extra)
+{
+    if ((u64)(m->used + extra) * 2U <= m->cap) {
+        return NPH_RESULT_OK;
+    }
+    u32 cap = m->cap;
+    while ((u64)(m->count + extra) * 2U > cap) {
+        cap *= 2U;
+    }
+    return map_rebuild(a, m, cap);
 }
 
 static NphResult map_insert(NphAllocator *a, NphPairMap *m, u64 key, u32 value)
@@ -570,8 +589,74 @@ NphResult contacts_collide_begin(NphPhysicsWorld *w)
         return r;
     }
     w->inits.len = 0U;
+    // The records as they stand, put back if the step is abandoned before
+    // the changes this narrow phase finds reach the host.
+    r = nph_dev_contacts_checkpoint(w->dev, w->contacts.len);
+    if (r != NPH_RESULT_OK) {
+        return r;
+    }
     return nph_dev_collide_begin(w->dev, VB(w->awake_contacts, u32), w->awake_contacts.len,
                                  w->step_dt);
+}
+
+// Room for the changes of every awake contact, taken before the narrow
+// phase's second half so that a lack of memory abandons the step before
+// any change is applied: the change lists, an event each, the awake list
+// and each colour's list grown by every contact at most, and the solver's
+// lists.
+NphResult contacts_reserve_changes(NphPhysicsWorld *w)
+{
+    const u32 n = w->awake_contacts.len > 0U ? w->awake_contacts.len : 1U;
+    NphResult r = vbuf_reserve(w->alloc, &w->changed, 4U, n);
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->codes, 4U, n);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->begin_events, sizeof(NphPhysicsContactBeginEvent),
+                         w->begin_events.len + n);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->end_events, sizeof(NphPhysicsContactEndEvent),
+                         w->end_events.len + n);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->awake_contacts, 4U, w->contacts.len);
+    }
+    for (u32 c = 0U; c <= NPH_GRAPH_COLORS && r == NPH_RESULT_OK; ++c) {
+        r = vbuf_reserve(w->alloc, &w->color_contacts[c], 4U, w->color_contacts[c].len + n);
+    }
+    // The solver's lists: every contact and joint at most.
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->graph, 4U, w->contacts.len > 0U ? w->contacts.len : 1U);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->step_joints, 4U, w->joints.len > 0U ? w->joints.len : 1U);
+    }
+    return r;
+}
+
+// Room for `pairs` new contacts.
+NphResult contacts_reserve_new(NphPhysicsWorld *w, u32 pairs)
+{
+    const u32 records = NPH_PHYSICS_MESH_MAX_MANIFOLDS * pairs;
+    const u32 most = w->contacts.len + records;
+    NphResult r = vbuf_reserve(w->alloc, &w->contacts, sizeof(HContact), most);
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->free_contacts, 4U, most);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = nph_dev_reserve(w->dev, 0U, 0U, most);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = map_reserve(w->alloc, &w->pairs, pairs);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->awake_contacts, 4U, most);
+    }
+    if (r == NPH_RESULT_OK) {
+        r = vbuf_reserve(w->alloc, &w->inits, sizeof(NphContactInit), w->inits.len + records);
+    }
+    return r;
 }
 
 NphResult contacts_collide_end(NphPhysicsWorld *w)
@@ -883,6 +968,13 @@ void pairs_sent(NphPhysicsWorld *w)
 // pair in order.
 NphResult pairs_finish(NphPhysicsWorld *w, const u32 *pairs, u32 count)
 {
+    // Room for every pair's contact (a mesh pair takes four records) before
+    // the move buffer is let go: a lack of memory then abandons the step
+    // with the search still to do.
+    NphResult room = contacts_reserve_new(w, count);
+    if (room != NPH_RESULT_OK) {
+        return room;
+    }
     for (u32 i = 0U; i < w->move_buffer.len; ++i) {
         shape_at(w, VB(w->pf_moved_shapes, u32)[i])->flags &= ~(u32)HS_IN_MOVE_BUFFER;
     }

@@ -57,7 +57,7 @@ static NphAllocator g_sys_alloc = {&g_sys_vtable, "nph_amd.system"};
 
 NphAllocator *nph_amd_allocator_system(void) { return &g_sys_alloc; }
 
-NphResult vbuf_reserve(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap)
+NphResult vbuf_reserve_at(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap, NphCallsite site)
 {
     if (cap <= v->cap) {
         return NPH_RESULT_OK;
@@ -66,10 +66,9 @@ NphResult vbuf_reserve(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap)
     while (c < cap) {
         c *= 2U;
     }
-    NphCallsite cs = {__FILE__, (u32)__LINE__};
-    void *p = v->data == NULL ? a->vtable->alloc(a, (u64)c * elem, 16U, cs)
+    void *p = v->data == NULL ? a->vtable->alloc(a, (u64)c * elem, 16U, site)
                               : a->vtable->realloc(a, v->data, (u64)v->cap * elem, (u64)c * elem,
-                                                   16U, cs);
+                                                   16U, site);
     if (p == NULL) {
         return NPH_RESULT_OUT_OF_MEMORY;
     }
@@ -78,9 +77,9 @@ NphResult vbuf_reserve(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap)
     return NPH_RESULT_OK;
 }
 
-NphResult vbuf_push(NphAllocator *a, NphVecBuf *v, u32 elem, const void *item)
+NphResult vbuf_push_at(NphAllocator *a, NphVecBuf *v, u32 elem, const void *item, NphCallsite site)
 {
-    NphResult r = vbuf_reserve(a, v, elem, v->len + 1U);
+    NphResult r = vbuf_reserve_at(a, v, elem, v->len + 1U, site);
     if (r != NPH_RESULT_OK) {
         return r;
     }
@@ -1328,10 +1327,21 @@ NphResult nph_physics_world_step(NphPhysicsWorld *w, f32 dt, NphJobSystem *jobs)
     f64 t = now_ms();
     r = contacts_collide_begin(w);
     f64 collide_ms = now_ms() - t;
+    const int queued = r == NPH_RESULT_OK;
     if (r == NPH_RESULT_OK) {
         t = now_ms();
         r = contacts_update_pairs(w);
         w->stats.pairs_ms = now_ms() - t;
+    }
+    if (r == NPH_RESULT_OK) {
+        r = contacts_reserve_changes(w);
+    }
+    if (r != NPH_RESULT_OK && queued) {
+        // Abandoned before the queued narrow phase's changes reached the
+        // host (world.h:161-166: consistent, not advanced): its records go
+        // back to how the step found them, so the next step finds the same
+        // changes. Contacts the pair search made keep their staged inits.
+        (void)nph_dev_contacts_restore(w->dev);
     }
     if (r == NPH_RESULT_OK) {
         t = now_ms();

@@ -20,8 +20,12 @@ typedef struct NphVecBuf {
     u32 cap;
 } NphVecBuf;
 
-NphResult vbuf_reserve(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap);
-NphResult vbuf_push(NphAllocator *a, NphVecBuf *v, u32 elem, const void *item);
+// The allocator sees the caller's file and line.
+NphResult vbuf_reserve_at(NphAllocator *a, NphVecBuf *v, u32 elem, u32 cap, NphCallsite site);
+NphResult vbuf_push_at(NphAllocator *a, NphVecBuf *v, u32 elem, const void *item, NphCallsite site);
+#define NPH_SITE ((NphCallsite){__FILE__, (u32)__LINE__})
+#define vbuf_reserve(a, v, elem, cap) vbuf_reserve_at((a), (v), (elem), (cap), NPH_SITE)
+#define vbuf_push(a, v, elem, item) vbuf_push_at((a), (v), (elem), (item), NPH_SITE)
 void vbuf_free(NphAllocator *a, NphVecBuf *v, u32 elem);
 #define VB(v, T) ((T *)(v).data)
 
@@ -375,6 +379,8 @@ NphResult world_solve_input(NphPhysicsWorld *w, f32 dt, NphSolveInput *in);
 // After the device's solve: grown proxies, pairs' bookkeeping, hit events.
 NphResult world_solve_post(NphPhysicsWorld *w, const NphGrown *grown, u32 grown_count);
 f64 world_now_ms(void);
+NphResult contacts_reserve_changes(NphPhysicsWorld *w);
+NphResult contacts_reserve_new(NphPhysicsWorld *w, u32 pairs);
 NphResult contacts_collide_begin(NphPhysicsWorld *w);
 NphResult contacts_collide_end(NphPhysicsWorld *w);
 // The narrow phase's change codes of the awake list, entries [0, na) from

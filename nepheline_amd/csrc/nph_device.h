@@ -166,6 +166,10 @@ NphResult nph_dev_init_contacts(NphDevice *d, const NphContactInit *list, u32 co
 // since, after init_contacts), waits for both, and points *codes at the change
 // code of every entry (0: none), valid until the next call.
 NphResult nph_dev_collide_begin(NphDevice *d, const u32 *awake, u32 count, f32 step_dt);
+// The first `count` contact records saved before a step changes them, and
+// put back if the step is abandoned before its changes reach the host.
+NphResult nph_dev_contacts_checkpoint(NphDevice *d, u32 count);
+NphResult nph_dev_contacts_restore(NphDevice *d);
 NphResult nph_dev_collide_end(NphDevice *d, const u32 *awake, u32 count, f32 step_dt,
                               const u32 **codes);
 // A fast body (continuous collision) after finalize: its record (pose at
