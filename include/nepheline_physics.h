@@ -318,6 +318,42 @@ typedef struct NphPhysicsGeometry {
     };
 } NphPhysicsGeometry;
 
+// The helpers the reference's headers define inline, exported with the same
+// results (types.h:80-130, geometry.h:49-120, hull.h:80-110,
+// mesh.h:60-150): pose arithmetic, geometry constructors, the blocks'
+// array accessors and mesh child codes.
+NphPhysicsPose nph_physics_pose(NphVec3 position, NphQuat rotation);
+NphPhysicsPose nph_physics_pose_identity(void);
+NphVec3 nph_physics_pose_transform_point(NphPhysicsPose t, NphVec3 p);
+NphVec3 nph_physics_pose_transform_vector(NphPhysicsPose t, NphVec3 v);
+NphVec3 nph_physics_pose_inverse_transform_point(NphPhysicsPose t, NphVec3 p);
+NphVec3 nph_physics_pose_inverse_transform_vector(NphPhysicsPose t, NphVec3 v);
+NphPhysicsPose nph_physics_pose_mul(NphPhysicsPose a, NphPhysicsPose b);
+NphPhysicsPose nph_physics_pose_inv_mul(NphPhysicsPose a, NphPhysicsPose b);
+NphPhysicsPose nph_physics_pose_inverse(NphPhysicsPose t);
+bool nph_physics_shape_type_is_convex(NphPhysicsShapeType type);
+NphPhysicsGeometry nph_physics_geometry_sphere(f32 radius);
+NphPhysicsGeometry nph_physics_geometry_capsule(f32 half_height, f32 radius);
+NphPhysicsGeometry nph_physics_geometry_box(NphVec3 half_extents);
+NphPhysicsGeometry nph_physics_geometry_hull(const NphPhysicsHull *hull);
+NphPhysicsGeometry nph_physics_geometry_mesh(const NphPhysicsMesh *mesh);
+NphPhysicsGeometry nph_physics_geometry_heightfield(const NphPhysicsHeightfield *h);
+const void *nph_physics_hull_bytes(const NphPhysicsHull *h, u32 offset);
+const NphVec3 *nph_physics_hull_vertices(const NphPhysicsHull *h);
+const NphPlane *nph_physics_hull_planes(const NphPhysicsHull *h);
+const u16 *nph_physics_hull_faces(const NphPhysicsHull *h);
+const NphPhysicsHullEdge *nph_physics_hull_edges(const NphPhysicsHull *h);
+u32 nph_physics_mesh_leaf(u32 first, u32 count);
+bool nph_physics_mesh_child_is_leaf(u32 child);
+u32 nph_physics_mesh_leaf_first(u32 child);
+u32 nph_physics_mesh_leaf_count(u32 child);
+const void *nph_physics_mesh_bytes(const NphPhysicsMesh *m, u32 offset);
+const NphPhysicsMeshNode *nph_physics_mesh_nodes(const NphPhysicsMesh *m);
+const NphVec3 *nph_physics_mesh_vertices(const NphPhysicsMesh *m);
+const u32 *nph_physics_mesh_triangles(const NphPhysicsMesh *m);
+const u8 *nph_physics_mesh_edges(const NphPhysicsMesh *m);
+const u32 *nph_physics_mesh_originals(const NphPhysicsMesh *m);
+
 // geometry.c:57-140 (mass closed forms), :178-229 (boxes).
 NphPhysicsMassData nph_physics_geometry_mass(const NphPhysicsGeometry *g, f32 density);
 NphAABB nph_physics_geometry_aabb(const NphPhysicsGeometry *g, NphPhysicsPose pose);

@@ -197,3 +197,26 @@ u32 ref_shim_sizes(u32 which)
         return (u32)sizeof(PhysPrepareLanes);
     }
 }
+
+// The reference's header-inline helpers (types.h:80-130, geometry.h:49-120,
+// hull.h:80-110, mesh.h:60-150) as callable functions, for comparing this
+// library's exported versions with them.
+void ref_shim_pose_ops(NphPhysicsPose a, NphPhysicsPose b, NphVec3 p, NphPhysicsPose out_pose[3],
+                       NphVec3 out_vec[4])
+{
+    out_pose[0] = nph_physics_pose_mul(a, b);
+    out_pose[1] = nph_physics_pose_inv_mul(a, b);
+    out_pose[2] = nph_physics_pose_inverse(a);
+    out_vec[0] = nph_physics_pose_transform_point(a, p);
+    out_vec[1] = nph_physics_pose_transform_vector(a, p);
+    out_vec[2] = nph_physics_pose_inverse_transform_point(a, p);
+    out_vec[3] = nph_physics_pose_inverse_transform_vector(a, p);
+}
+
+u32 ref_shim_mesh_leaf_ops(u32 first, u32 count, u32 child, u32 out[3])
+{
+    out[0] = nph_physics_mesh_child_is_leaf(child) ? 1U : 0U;
+    out[1] = nph_physics_mesh_leaf_first(child);
+    out[2] = nph_physics_mesh_leaf_count(child);
+    return nph_physics_mesh_leaf(first, count);
+}

@@ -245,3 +245,92 @@ def test_penetration_depths(libs, hull_bank):
             if hit and (out.normal.x, out.normal.y, out.normal.z) == (0.0, 1.0, 0.0):
                 flat_touches += 1
     assert flat_touches >= 5, flat_touches
+
+
+def test_header_helpers_match_the_reference(libs, hull_bank):
+    """The helpers the reference's headers define inline (types.h:80-130,
+    geometry.h:49-120, hull.h:80-110, mesh.h:60-150), exported here: pose
+    arithmetic byte for byte against the reference's inline code (called
+    through oracle/ref_shim.c), mesh child codes likewise, and the
+    constructors and block accessors against the blocks' own layout."""
+    g, f, meshes = libs
+    L = g.lib
+    R = f.lib
+    Pose3 = P.Pose * 3
+    Vec4 = P.Vec3 * 4
+    R.ref_shim_pose_ops.argtypes = [P.Pose, P.Pose, P.Vec3, Pose3, Vec4]
+    R.ref_shim_pose_ops.restype = None
+    for name in ("nph_physics_pose_mul", "nph_physics_pose_inv_mul"):
+        getattr(L, name).restype = P.Pose
+        getattr(L, name).argtypes = [P.Pose, P.Pose]
+    L.nph_physics_pose_inverse.restype = P.Pose
+    L.nph_physics_pose_inverse.argtypes = [P.Pose]
+    for name in ("nph_physics_pose_transform_point", "nph_physics_pose_transform_vector",
+                 "nph_physics_pose_inverse_transform_point",
+                 "nph_physics_pose_inverse_transform_vector"):
+        getattr(L, name).restype = P.Vec3
+        getattr(L, name).argtypes = [P.Pose, P.Vec3]
+    r = S.Pcg32(307)
+    for i in range(2000):
+        far = 1e4 if i % 5 == 0 else 3.0
+        a = P.Pose(_vec(r, -far, far), P.Quat(*S.rotation(r)))
+        b = P.Pose(_vec(r, -far, far), P.Quat(*S.rotation(r)))
+        p = _vec(r, -far, far)
+        poses, vecs = Pose3(), Vec4()
+        R.ref_shim_pose_ops(a, b, p, poses, vecs)
+        assert _b(L.nph_physics_pose_mul(a, b)) == _b(poses[0])
+        assert _b(L.nph_physics_pose_inv_mul(a, b)) == _b(poses[1])
+        assert _b(L.nph_physics_pose_inverse(a)) == _b(poses[2])
+        assert _b(L.nph_physics_pose_transform_point(a, p)) == _b(vecs[0])
+        assert _b(L.nph_physics_pose_transform_vector(a, p)) == _b(vecs[1])
+        assert _b(L.nph_physics_pose_inverse_transform_point(a, p)) == _b(vecs[2])
+        assert _b(L.nph_physics_pose_inverse_transform_vector(a, p)) == _b(vecs[3])
+    R.ref_shim_mesh_leaf_ops.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32 * 3]
+    R.ref_shim_mesh_leaf_ops.restype = C.c_uint32
+    for name in ("nph_physics_mesh_leaf",):
+        getattr(L, name).restype = C.c_uint32
+        getattr(L, name).argtypes = [C.c_uint32, C.c_uint32]
+    for name in ("nph_physics_mesh_leaf_first", "nph_physics_mesh_leaf_count"):
+        getattr(L, name).restype = C.c_uint32
+        getattr(L, name).argtypes = [C.c_uint32]
+    L.nph_physics_mesh_child_is_leaf.restype = C.c_bool
+    L.nph_physics_mesh_child_is_leaf.argtypes = [C.c_uint32]
+    for first, count, child in ((0, 1, 0), (5, 4, 0x80000000), (123456, 3, 0xFFFFFFFF),
+                                (7, 2, 0x80000011), (1 << 20, 4, 0x7FFFFFFF)):
+        out = (C.c_uint32 * 3)()
+        assert L.nph_physics_mesh_leaf(first, count) == R.ref_shim_mesh_leaf_ops(first, count,
+                                                                                 child, out)
+        assert (L.nph_physics_mesh_child_is_leaf(child), L.nph_physics_mesh_leaf_first(child),
+                L.nph_physics_mesh_leaf_count(child)) == (bool(out[0]), out[1], out[2])
+    # Constructors and accessors.
+    L.nph_physics_geometry_box.restype = P.Geometry
+    L.nph_physics_geometry_box.argtypes = [P.Vec3]
+    assert _b(L.nph_physics_geometry_box(P.Vec3(1.0, 2.0, 3.0))) == _b(P.geometry_box(1.0, 2.0, 3.0))
+    L.nph_physics_geometry_capsule.restype = P.Geometry
+    L.nph_physics_geometry_capsule.argtypes = [C.c_float, C.c_float]
+    assert _b(L.nph_physics_geometry_capsule(0.5, 0.25)) == _b(P.geometry_capsule(0.5, 0.25))
+    L.nph_physics_shape_type_is_convex.restype = C.c_bool
+    L.nph_physics_shape_type_is_convex.argtypes = [C.c_int]
+    assert [L.nph_physics_shape_type_is_convex(t) for t in range(6)] == [True] * 4 + [False] * 2
+    hull = hull_bank.ptrs[0]
+    words = C.cast(hull, C.POINTER(C.c_uint32))
+    base = C.cast(hull, C.c_void_p).value
+    for name, word in (("nph_physics_hull_vertices", 4), ("nph_physics_hull_planes", 5),
+                       ("nph_physics_hull_faces", 6), ("nph_physics_hull_edges", 7)):
+        fn = getattr(L, name)
+        fn.restype = C.c_void_p
+        fn.argtypes = [C.c_void_p]
+        assert fn(hull) == base + words[word], name
+    mesh = meshes.ptrs[0]
+    hdr = C.cast(mesh, C.POINTER(P.Mesh)).contents
+    mbase = C.cast(mesh, C.c_void_p).value
+    for name, off in (("nph_physics_mesh_nodes", hdr.nodes_offset),
+                      ("nph_physics_mesh_vertices", hdr.vertices_offset),
+                      ("nph_physics_mesh_triangles", hdr.triangles_offset),
+                      ("nph_physics_mesh_edges", hdr.edges_offset),
+                      ("nph_physics_mesh_originals", hdr.originals_offset)):
+        fn = getattr(L, name)
+        fn.restype = C.c_void_p
+        fn.argtypes = [C.c_void_p]
+        assert fn(mesh) == mbase + off, name
