@@ -516,10 +516,13 @@ def world_run(a, make, steps, warmup, network=None, measure=True, kernel_steps=0
     solved = 0
     res_sum, res_n, res_max, pen, learned, gated = 0.0, 0, 0.0, 0.0, 0, 0
     t_wall = 0.0
+    phases = np.zeros(4)
     for i in range(steps):
         t0 = time.perf_counter()
         w.step(DT)
         t_wall += time.perf_counter() - t0
+        st = w.stats()  # outside the timed region: it reads the contacts back
+        phases += (st.pairs_ms, st.collide_ms, st.solve_ms, st.sleep_ms)
         solved += a.lib.nph_amd_solved_contacts(w.ptr)
         if measure or network is not None:
             s = L.warm_stats(w)
@@ -533,7 +536,9 @@ def world_run(a, make, steps, warmup, network=None, measure=True, kernel_steps=0
     out = {"value": solved * SUBSTEPS / t_wall, "ms_per_step": t_wall * 1e3 / steps,
            "steps": steps, "warmup": warmup, "residual_mean": res_sum / max(res_n, 1),
            "residual_max": res_max, "penetration_max": pen, "contacts_from_network": learned,
-           "gated": gated, "energy": drift(e0, e1)}
+           "gated": gated, "energy": drift(e0, e1),
+           "phases_ms_per_step": {k: round(v / max(steps, 1), 3) for k, v in
+                                  zip(("pairs", "collide", "solve", "islands"), phases)}}
     if kernel_steps:
         a.lib.nph_amd_time_kernels(w.ptr, True)
         k_solved = 0

@@ -424,18 +424,26 @@ static NphResult finish_body(NphPhysicsWorld *w, const NphFastBody *f, NphBodyRe
                              const NphBodyRec *records, const u8 *bullet, int is_bullet,
                              u32 *impacts)
 {
-    NphShapeRec recs[64];
-    u32 n = 0U;
+    // The records on the stack for up to 64 shapes, else in the world's
+    // scratch list.
+    NphShapeRec few[64];
     const HBody *hb = &VB(w->bodies, HBody)[f->slot];
-    for (u32 s = hb->shape_head; s != NPH_NULL32 && n < 64U; s = VB(w->shapes, HShape)[s].next) {
+    NphShapeRec *recs = few;
+    if (hb->shape_count > 64U) {
+        const NphResult g = vbuf_reserve(w->alloc, &w->sweep_shapes, sizeof(NphShapeRec),
+                                         hb->shape_count);
+        if (g != NPH_RESULT_OK) {
+            return g;
+        }
+        recs = VB(w->sweep_shapes, NphShapeRec);
+    }
+    u32 n = 0U;
+    for (u32 s = hb->shape_head; s != NPH_NULL32; s = VB(w->shapes, HShape)[s].next) {
         NphResult r = world_shape_record(w, s, &recs[n]);
         if (r != NPH_RESULT_OK) {
             return r;
         }
         n += 1U;
-    }
-    if (n < hb->shape_count) {
-        return NPH_RESULT_UNSUPPORTED; // more shapes than a fast body's sweep holds
     }
     if (is_bullet) {
         refresh(b, recs, n); // as finalize did on the device

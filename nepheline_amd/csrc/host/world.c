@@ -310,6 +310,7 @@ void nph_physics_world_destroy(NphPhysicsWorld *w)
     vbuf_free(a, &w->awake_bodies, 4U);
     vbuf_free(a, &w->awake_contacts, 4U);
     vbuf_free(a, &w->inits, sizeof(NphContactInit));
+    vbuf_free(a, &w->sweep_shapes, sizeof(NphShapeRec));
     vbuf_free(a, &w->move_buffer, 4U);
     nph_pairs_destroy(w->pair_search);
     vbuf_free(a, &w->pf_dirty_shapes, 4U);

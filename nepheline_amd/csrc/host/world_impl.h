@@ -229,6 +229,7 @@ struct NphPhysicsWorld {
     NphVecBuf awake_contacts; // u32
     NphPairMap pairs;
     NphVecBuf inits; // NphContactInit of records made since the last collide
+    NphVecBuf sweep_shapes; // NphShapeRec: a fast body's shapes beyond 64
 
     NphTree trees[3];
     NphVecBuf move_buffer; // u32 proxy keys

@@ -111,3 +111,31 @@ def test_fast_bodies_onto_meshes_match_the_reference(gpu_api, ref_api, hull_bank
         swept += sg.swept_bodies
         impacts += sg.impact_bodies
     assert swept > 40 and impacts > 20
+
+
+def _compound_shots(api):
+    """Fast compound bodies of 70 and 100 small shapes (more than 64, the
+    sweep's stack list) fired at the thin wall, one a bullet."""
+    from nepheline_amd import physics as P
+    w = S.projectiles(api, 4, 13)
+    for k, (count, bullet) in enumerate(((70, 0), (100, 1))):
+        b = w.add_body(P.BODY_DYNAMIC, (-1.0 + 2.0 * k, 2.0, -1.0), velocity=(0.0, 0.0, 120.0),
+                       bullet=bullet)
+        for i in range(count):
+            offset = (0.02 * (i % 5) - 0.04, 0.02 * ((i // 5) % 5) - 0.04, 0.02 * (i // 25))
+            w.add_shape(b, P.geometry_sphere(0.02), pose=(offset, (0.0, 0.0, 0.0, 1.0)),
+                        density=100.0)
+    return w
+
+
+def test_compound_fast_bodies_match_the_reference(gpu_api, ref_api):
+    print("compound fast bodies of 70 and 100 shapes", flush=True)
+    wg = _compound_shots(gpu_api)
+    wr = _compound_shots(ref_api)
+    swept = 0
+    for i in range(40):
+        wg.step(1.0 / 60.0)
+        wr.step(1.0 / 60.0)
+        swept += wg.stats().swept_bodies
+        assert wg.hash() == wr.hash(), f"differs at step {i + 1}"
+    assert swept > 10
