@@ -235,18 +235,21 @@ def _profile_rows(directory, suffix):
     return rows
 
 
-def _timed_window(rows, groups=2, warmup=5):
-    """Dispatch ids of the plain batch's timed colour passes: after its
-    warm-up steps (each step ends with one k_b_finalize a world group) and
-    before the learned batch's first k_b_learned_features."""
+def _timed_window(rows, groups=2, warmup=5, steps=20, roofline_steps=10):
+    """Dispatch ids of the plain batch's colour passes in its roofline
+    window: the steps right after its warm-up and timed steps (each step
+    ends with one k_b_finalize a world group), up to the end of the window
+    and before the learned batch's first k_b_learned_features."""
     ids = sorted({int(r["Dispatch_Id"]): r["Kernel_Name"] for r in rows}.items())
+    first = groups * (warmup + steps)
+    last = first + groups * roofline_steps
     finals, window = 0, []
     for did, name in ids:
         if name.startswith("k_b_learned_features"):
             break
         if name.startswith("k_b_finalize"):
             finals += 1
-        elif name.startswith("k_b_color_pass") and finals >= groups * warmup:
+        elif name.startswith("k_b_color_pass") and first <= finals < last:
             window.append(did)
     return set(window)
 
@@ -261,13 +264,15 @@ def summarize_profiles(trace_dir, read_dir, write_dir, copy_read_dir, copy_write
     trace = [r for r in _profile_rows(trace_dir, "kernel_trace.csv")
              if r["Kernel_Name"].startswith(("k_b_color_pass", "k_b_finalize",
                                              "k_b_learned_features"))]
-    win = _timed_window(trace, warmup=key["warmup"])
+    win = _timed_window(trace, warmup=key["warmup"], steps=key["steps"],
+                        roofline_steps=key["roofline_steps"])
     durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace
             if int(r["Dispatch_Id"]) in win]
 
     def per_dispatch(directory, names):
         rows = _profile_rows(directory, "counter_collection.csv")
-        w = _timed_window(rows, warmup=key["warmup"])
+        w = _timed_window(rows, warmup=key["warmup"], steps=key["steps"],
+                          roofline_steps=key["roofline_steps"])
         acc = {}
         for r in rows:
             if int(r["Dispatch_Id"]) in w and r["Counter_Name"] in names:
@@ -294,11 +299,11 @@ def summarize_profiles(trace_dir, read_dir, write_dir, copy_read_dir, copy_write
     copy_write = copy_bytes(copy_write_dir, lambda v: v.get("WRITE_SIZE", 0.0) * 1024.0)
     out = {
         "kernel": "k_b_color_pass",
-        "workload": "the bench's 1 024-world batch (seed 4), plain start: its timed steps",
+        "workload": "the bench's 1 024-world batch (seed 4), plain start: its roofline window",
         "workload_key": key,
-        "window": "k_b_color_pass dispatches after the plain batch's warm-up steps (one "
-                  "k_b_finalize a world group a step) and before the learned batch's first "
-                  "k_b_learned_features",
+        "window": "k_b_color_pass dispatches of the plain batch's roofline window: the "
+                  "steps right after its warm-up and timed steps (one k_b_finalize a world "
+                  "group a step), before the learned batch's first k_b_learned_features",
         "dispatches": {"trace": len(durs), "read_counters": len(read_b),
                        "write_counters": len(write_b)},
         "avg_duration_us": float(np.mean(durs)) / 1e3 if durs else None,
@@ -388,16 +393,13 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
         how = "fixed (the warm-up is too short for the autotuner)"
     L.batch_measure_residual(batch, True)
     L.batch_warm_stats(batch, reset=True)
-    # The rooflines' kernel timing runs over the timed steps themselves
-    # (dispatch-stamped events on each launch's own stream), unless the
-    # substep loop is a captured graph: then over a window after them.
-    timed_kernels = rooflines and not tune.use_graph
-    if timed_kernels:
-        a.batch_time_kernels(batch.ptr, True)
+    # The timed steps run uninstrumented; the rooflines' kernel timing
+    # (dispatch-stamped events on each launch's own stream) runs over a
+    # window of steps right after them.
     e0 = energy(batch.worlds)
     solved = 0
     joints = [0, 0]
-    phases = np.zeros(16)
+    phases = np.zeros(19)
     t0 = time.perf_counter()
     for i in range(args.steps):
         batch.step(DT)
@@ -408,7 +410,8 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
         phases += (st.step_ms, st.narrow_ms, st.pairs_ms, st.changes_ms, st.solve_ms, st.post_ms,
                    st.device_solve_ms, st.device_passes_ms, st.device_collide_ms,
                    st.wait_narrow_ms, st.wait_solve_ms, st.device_pairs_ms, st.pairs_trees_ms,
-                   st.pairs_search_ms, st.pairs_create_ms, st.rebuild_cpu_ms)
+                   st.pairs_search_ms, st.pairs_create_ms, st.rebuild_cpu_ms,
+                   st.pairs_gather_ms, st.pairs_upload_ms, st.pairs_upload_bytes / 1e6)
         if (i + 1) % 10 == 0:
             log(f"batch step {i + 1}/{args.steps}")
     wall = time.perf_counter() - t0
@@ -418,7 +421,8 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
     n = st.contacts_plain + st.contacts_learned
     names = ("step", "narrow", "pairs", "host_changes", "solve", "host_post", "dev_solve",
              "dev_passes", "dev_collide", "wait_narrow", "wait_solve", "dev_pairs",
-             "pairs_trees", "pairs_search", "pairs_create", "rebuild_cpu")
+             "pairs_trees", "pairs_search", "pairs_create", "rebuild_cpu",
+             "pairs_gather", "pairs_upload", "pairs_upload_mb")
     out = {"value": solved * SUBSTEPS / wall, "ms_per_step": wall * 1e3 / args.steps,
            "contact_substeps": solved * SUBSTEPS, "setup_s": setup, "bodies": bodies,
            "tuning": {"how": how, "pass_block": tune.pass_block, "body_block": tune.body_block,
@@ -431,25 +435,22 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
            "learned_fraction": st.contacts_learned / n if n else 0.0, "gated": st.gated,
            "energy": drift(e0, e1)}
     if rooflines:
-        window = "the %d timed steps" % args.steps
-        solved_window = solved
-        joints_window = list(joints)
-        if not timed_kernels:
-            batch.set_tuning(tune.pass_block, tune.body_block, 0, tune.groups)
-            a.batch_time_kernels(batch.ptr, True)
-            solved_window = 0
-            joints_window = [0, 0]
-            for _ in range(args.roofline_steps):
-                batch.step(DT)
-                st = batch.stats()
-                solved_window += st.solved_contacts
-                joints_window[0] += st.colored_joints
-                joints_window[1] += st.colored_joint_ends
-            window = "%d steps after the timed region (stream launches)" % args.roofline_steps
+        batch.set_tuning(tune.pass_block, tune.body_block, 0, tune.groups)
+        a.batch_time_kernels(batch.ptr, True)
+        solved_window = 0
+        joints_window = [0, 0]
+        for _ in range(args.roofline_steps):
+            batch.step(DT)
+            st = batch.stats()
+            solved_window += st.solved_contacts
+            joints_window[0] += st.colored_joints
+            joints_window[1] += st.colored_joint_ends
+        window = "the %d steps right after the timed ones (stream launches)" % (
+            args.roofline_steps)
         key = {"worlds": len(plan), "seed": args.batch_seed, "steps": args.steps,
-               "warmup": args.warmup}
-        out["rooflines"] = batch_rooflines(a, batch, plan, solved_window, window,
-                                           key if timed_kernels else None, joints_window)
+               "warmup": args.warmup, "roofline_steps": args.roofline_steps}
+        out["rooflines"] = batch_rooflines(a, batch, plan, solved_window, window, key,
+                                           joints_window)
         a.batch_time_kernels(batch.ptr, False)
     batch.close()
     return out
@@ -651,8 +652,8 @@ def main():
                     "runs inside it when it has at least 72)")
     ap.add_argument("--batch-worlds", type=int, default=1024)
     ap.add_argument("--batch-seed", type=int, default=4)
-    ap.add_argument("--roofline-steps", type=int, default=5,
-                    help="steps after a timed region with per-launch kernel timing")
+    ap.add_argument("--roofline-steps", type=int, default=10,
+                    help="steps right after the timed ones with per-launch kernel timing")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-stride", type=int, default=16,
                     help="the CPU baseline steps every n-th world of the batch")
@@ -687,7 +688,7 @@ def main():
         return
     if args.summarize_profiles:
         key = {"worlds": args.batch_worlds, "seed": args.batch_seed, "steps": args.steps,
-               "warmup": args.warmup}
+               "warmup": args.warmup, "roofline_steps": args.roofline_steps}
         out = summarize_profiles(*args.summarize_profiles, key)
         with open(TRAFFIC_JSON, "w") as f:
             json.dump(out, f, indent=1)

@@ -1337,6 +1337,9 @@ typedef struct NphAmdBatchStats {
     f64 rebuild_cpu_ms;  // the trees' rebuilds, summed over the worlds' threads
     u64 colored_joints;  // the last step's joints in colours, all worlds
     u64 colored_joint_ends; // their dynamic bodies
+    f64 pairs_gather_ms;    // of pairs_search_ms: packing the trees' written nodes (host)
+    f64 pairs_upload_ms;    // the upload's copy (device events)
+    u64 pairs_upload_bytes; // its bytes
 } NphAmdBatchStats;
 
 NphResult nph_amd_batch_create(const NphPhysicsWorldDesc *desc, u32 world_count,

@@ -318,6 +318,11 @@ static NphResult group_to_solve(NphAmdBatch *b, u32 g, f32 dt)
         }
         if (r == NPH_RESULT_OK) {
             b->stats.device_pairs_ms += nph_pairs_last_ms(b->pairs);
+            f64 detail[3];
+            nph_pairs_last_detail(b->pairs, detail);
+            b->stats.pairs_gather_ms += detail[0];
+            b->stats.pairs_upload_ms += detail[1];
+            b->stats.pairs_upload_bytes += (u64)detail[2];
         }
     }
     const f64 t2 = world_now_ms();
@@ -452,6 +457,8 @@ NphResult nph_amd_batch_step(NphAmdBatch *b, f32 dt)
     st->awake_bodies = 0U;
     st->colored_joints = 0U;
     st->colored_joint_ends = 0U;
+    st->pairs_gather_ms = st->pairs_upload_ms = 0.0;
+    st->pairs_upload_bytes = 0U;
     st->narrow_ms = st->pairs_ms = st->changes_ms = st->solve_ms = st->post_ms = 0.0;
     st->device_pairs_ms = st->pairs_trees_ms = st->pairs_search_ms = st->pairs_create_ms = 0.0;
     st->rebuild_cpu_ms = 0.0;

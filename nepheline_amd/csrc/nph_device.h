@@ -125,6 +125,9 @@ typedef struct NphPairInput {
 
 typedef struct NphPairSearch NphPairSearch;
 NphResult nph_pairs_create(NphPairSearch **out);
+// The last search's host packing time (ms), its upload's device time (ms)
+// and the upload's bytes.
+void nph_pairs_last_detail(const NphPairSearch *p, f64 out[3]);
 void nph_pairs_destroy(NphPairSearch *p);
 // The new pairs of n worlds in one search: world k's are pairs
 // [(*starts)[k], (*starts)[k + 1]) of *pairs, each (shape a, shape b), in

@@ -23,6 +23,15 @@
 // it would pass that.
 #include "device_impl.h"
 
+#include <time.h>
+
+static f64 pf_now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (f64)ts.tv_sec * 1e3 + (f64)ts.tv_nsec * 1e-6;
+}
+
 #define PF_BLOCK 256U
 #define PF_QBLOCK 64U // a query's walk is long and branchy: small blocks spread it over the CUs
 #define PF_SMALL 64U  // moved proxies a one-block search takes
@@ -76,8 +85,11 @@ static_assert(sizeof(PfNode) == 64U && sizeof(PfShape) == 48U && sizeof(PfKey) =
 
 struct NphPairSearch {
     hipStream_t stream;
-    hipEvent_t ev[2];
+    hipEvent_t ev[3]; // the search's start and end, the upload's start
     f64 last_ms;
+    f64 last_gather_ms; // host: the records packed for the upload
+    f64 last_upload_ms; // device: the upload's copy
+    u64 last_upload_bytes;
     u8 *h_up; // pinned upload
     u64 cap_h_up;
     u8 *d_up;
@@ -453,7 +465,8 @@ NphResult nph_pairs_create(NphPairSearch **out)
         return NPH_RESULT_OUT_OF_MEMORY;
     }
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&p->ev[0]) != hipSuccess || hipEventCreate(&p->ev[1]) != hipSuccess) {
+        hipEventCreate(&p->ev[0]) != hipSuccess || hipEventCreate(&p->ev[1]) != hipSuccess ||
+        hipEventCreate(&p->ev[2]) != hipSuccess) {
         free(p);
         return NPH_RESULT_DEVICE_LOST;
     }
@@ -480,13 +493,21 @@ void nph_pairs_destroy(NphPairSearch *p)
             (void)hipHostFree(pinned[i]);
         }
     }
-    (void)hipEventDestroy(p->ev[0]);
-    (void)hipEventDestroy(p->ev[1]);
+    for (u32 i = 0U; i < 3U; ++i) {
+        (void)hipEventDestroy(p->ev[i]);
+    }
     (void)hipStreamDestroy(p->stream);
     free(p);
 }
 
 f64 nph_pairs_last_ms(const NphPairSearch *p) { return p->last_ms; }
+
+void nph_pairs_last_detail(const NphPairSearch *p, f64 out[3])
+{
+    out[0] = p->last_gather_ms;
+    out[1] = p->last_upload_ms;
+    out[2] = (f64)p->last_upload_bytes;
+}
 
 static NphResult pf_pinned(void **p, u64 *cap, u64 bytes)
 {
@@ -627,6 +648,7 @@ NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const 
     PfWorld *uw = (PfWorld *)(p->h_up + o_worlds);
     u32 *uf = (u32 *)(p->h_up + o_first);
     // Each world's records in parallel (the node gather is the bulk of it).
+    const f64 g0 = pf_now_ms();
 #pragma omp parallel for schedule(dynamic, 4)
     for (u32 k = 0U; k < n; ++k) {
         u64 at_n = p->at[4U * k], at_s = p->at[4U * k + 1U];
@@ -676,6 +698,8 @@ NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const 
         }
     }
     uf[n] = (u32)nm;
+    p->last_gather_ms = pf_now_ms() - g0;
+    p->last_upload_bytes = bytes;
     // Scratch for the counts and the pairs.
     const u32 moved = (u32)nm;
     const u32 chunk_count = (moved + PF_SCAN - 1U) / PF_SCAN;
@@ -718,6 +742,7 @@ NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const 
         }
         p->cap_pairs = 4096U;
     }
+    HIP_TRY(hipEventRecord(p->ev[2], s));
     HIP_TRY(hipMemcpyAsync(p->d_up, p->h_up, bytes, hipMemcpyHostToDevice, s));
     PfArgs a;
     a.worlds = (const PfWorld *)(p->d_up + o_worlds);
@@ -803,6 +828,9 @@ NphResult nph_pairs_find(NphPairSearch *p, const NphPairInput *in, u32 n, const 
     float ms = 0.0F;
     if (hipEventElapsedTime(&ms, p->ev[0], p->ev[1]) == hipSuccess) {
         p->last_ms = (f64)ms;
+    }
+    if (hipEventElapsedTime(&ms, p->ev[2], p->ev[0]) == hipSuccess) {
+        p->last_upload_ms = (f64)ms;
     }
     *pairs = p->h_pairs;
     *starts = p->h_starts;

@@ -1017,7 +1017,9 @@ class BatchStats(C.Structure):
                 ("wait_solve_ms", C.c_double), ("device_pairs_ms", C.c_double),
                 ("pairs_trees_ms", C.c_double), ("pairs_search_ms", C.c_double),
                 ("pairs_create_ms", C.c_double), ("rebuild_cpu_ms", C.c_double),
-                ("colored_joints", C.c_uint64), ("colored_joint_ends", C.c_uint64)]
+                ("colored_joints", C.c_uint64), ("colored_joint_ends", C.c_uint64),
+                ("pairs_gather_ms", C.c_double), ("pairs_upload_ms", C.c_double),
+                ("pairs_upload_bytes", C.c_uint64)]
 
 
 def _bind_batch(a: Api):
