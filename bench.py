@@ -388,7 +388,7 @@ def run_batch(args, bank, plan, network=None, rooflines=False):
         tune = batch.autotune(DT, rounds=tune_rounds)  # 32 candidates a round, a step each
         how = "autotuned (%d rounds of 32 candidates inside the warm-up)" % tune_rounds
     else:
-        batch.set_tuning(256, 64, 0, 2)
+        batch.set_tuning(256, 64, 0, args.batch_groups)
         tune = batch.tuning()
         how = "fixed (the warm-up is too short for the autotuner)"
     L.batch_measure_residual(batch, True)
@@ -652,6 +652,8 @@ def main():
                     "runs inside it when it has at least 72)")
     ap.add_argument("--batch-worlds", type=int, default=1024)
     ap.add_argument("--batch-seed", type=int, default=4)
+    ap.add_argument("--batch-groups", type=int, default=2,
+                    help="world groups the batch step is pipelined over when not autotuned")
     ap.add_argument("--roofline-steps", type=int, default=10,
                     help="steps right after the timed ones with per-launch kernel timing")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
