@@ -118,6 +118,7 @@ void nph_physics_body_destroy(NphPhysicsWorld *w, NphPhysicsBody body)
         b->shape_head = shape->next;
         destroy_proxy(w, s);
         shape->handle = 0U;
+        w->hit_event_shapes -= (shape->flags & HS_HIT_EVENTS) != 0U ? 1U : 0U;
         slots_release(w->alloc, &w->shape_slots, s);
     }
     b->shape_count = 0U;
@@ -430,6 +431,7 @@ void nph_physics_shape_destroy(NphPhysicsWorld *w, NphPhysicsShape shape)
     }
     b->shape_count -= 1U;
     s->handle = 0U;
+    w->hit_event_shapes -= (s->flags & HS_HIT_EVENTS) != 0U ? 1U : 0U;
     slots_release(w->alloc, &w->shape_slots, slot);
     world_update_mass(w, body);
     if (b->type != NPH_PHYSICS_BODY_STATIC) {

@@ -919,6 +919,7 @@ NphResult nph_physics_shape_create(NphPhysicsWorld *w, NphPhysicsBody body,
     s->filter = desc->filter;
     s->flags = (desc->contact_events != 0 ? HS_CONTACT_EVENTS : 0U) |
                (desc->hit_events != 0 ? HS_HIT_EVENTS : 0U);
+    w->hit_event_shapes += desc->hit_events != 0 ? 1U : 0U;
     s->user_data = desc->user_data;
     s->next = hb->shape_head;
     hb->shape_head = slot;
@@ -1024,6 +1025,10 @@ NphResult world_read_contacts(NphPhysicsWorld *w) { return read_contacts(w); }
 // contact in the graph wants hits.
 static NphResult hit_events(NphPhysicsWorld *w)
 {
+    // A contact's hit flag comes from its shapes, which outlive it.
+    if (w->hit_event_shapes == 0U) {
+        return NPH_RESULT_OK;
+    }
     int any = 0;
     for (u32 c = 0U; c <= NPH_GRAPH_COLORS && !any; ++c) {
         const u32 *list = VB(w->color_contacts[c], u32);
