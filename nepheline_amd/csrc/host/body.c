@@ -129,6 +129,7 @@ void nph_physics_body_destroy(NphPhysicsWorld *w, NphPhysicsBody body)
     islands_remove_body(w, slot);
     b = hbody(w, slot);
     b->handle = 0U;
+    w->kinematic_bodies -= b->type == NPH_PHYSICS_BODY_KINEMATIC ? 1U : 0U;
     slots_release(w->alloc, &w->body_slots, slot);
     (void)world_upload_body(w, slot);
 }

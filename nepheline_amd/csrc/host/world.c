@@ -564,6 +564,7 @@ NphResult nph_physics_body_create(NphPhysicsWorld *w, const NphPhysicsBodyDesc *
     memset(b, 0, sizeof *b);
     hb->type = (u32)desc->type;
     b->type = (u32)desc->type;
+    w->kinematic_bodies += desc->type == NPH_PHYSICS_BODY_KINEMATIC ? 1U : 0U;
     b->locks = desc->locks;
     b->origin = desc->position;
     b->rotation = q_normalize(desc->rotation);
@@ -603,6 +604,7 @@ NphResult nph_physics_body_create(NphPhysicsWorld *w, const NphPhysicsBodyDesc *
             islands_remove_body(w, slot);
         }
         hb->handle = 0U;
+        w->kinematic_bodies -= hb->type == NPH_PHYSICS_BODY_KINEMATIC ? 1U : 0U;
         slots_release(w->alloc, &w->body_slots, slot);
         return r;
     }
@@ -1208,7 +1210,7 @@ static NphResult post_finalize(NphPhysicsWorld *w)
         }
     }
     int kinematic = 0;
-    for (u32 i = 0U; i < w->awake_bodies.len && !kinematic; ++i) {
+    for (u32 i = 0U; w->kinematic_bodies != 0U && i < w->awake_bodies.len && !kinematic; ++i) {
         kinematic = VB(w->bodies, HBody)[VB(w->awake_bodies, u32)[i]].type ==
                     NPH_PHYSICS_BODY_KINEMATIC;
     }

@@ -257,6 +257,7 @@ struct NphPhysicsWorld {
     NphVecBuf begin_events; // NphPhysicsContactBeginEvent
     NphVecBuf hit_events;   // NphPhysicsContactHitEvent
     u32 hit_event_shapes;   // live shapes created with hit events; none means no contact reports hits
+    u32 kinematic_bodies;   // live kinematic bodies (a body keeps its type for life)
     NphVecBuf end_events;
     NphVecBuf move_events; // NphPhysicsBodyMoveEvent, built at the first sync after the step
     int move_events_ready;
